@@ -1,0 +1,222 @@
+"""ctypes access to the CPU checkers (test infrastructure only).
+
+* ``liboracle.so``     -- this project's CPU restatement of the VirtualFluids
+  D3Q27 hot path (``oracle.cpp``).
+* ``_ref/libvfref.so`` -- the VirtualFluids reference CPU closure compiled from
+  the reference sources with ``ref_driver.cpp`` (built only where the
+  reference tree exists; the built library may travel without it).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg import this package, and only to check or time the reference behaviour.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libvfref.so")
+REF_SRC = "/root/reference"
+
+_dp = C.POINTER(C.c_double)
+_u8p = C.POINTER(C.c_uint8)
+
+
+class OrcBC(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("n", C.c_int32), ("node", C.POINTER(C.c_int64)),
+                ("q", C.POINTER(C.c_float)), ("ux", _dp), ("uy", _dp), ("uz", _dp),
+                ("cpu_float_velocity", C.c_int32)]
+
+
+def build(ref: bool | None = None) -> None:
+    """Compile liboracle.so (and the reference closure when its sources exist)."""
+    targets = ["all"]
+    if ref is None:
+        ref = os.path.isdir(REF_SRC)
+    if ref:
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE, f"-j{min(8, os.cpu_count() or 1)}"] + targets, check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build(ref=False)
+        L = C.CDLL(LIB_PATH)
+        L.orc_k17_collide.argtypes = [_dp, C.c_double, _dp, _dp, _dp]
+        L.orc_bgk_collide.argtypes = [_dp, C.c_double, _dp]
+        L.orc_chimera.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_macroscopic.argtypes = [_dp, _dp]
+        L.orc_equilibrium.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.orc_init_field.argtypes = [_dp, C.c_int64, _dp, _dp, _dp, _dp]
+        L.orc_wall_term.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int]
+        L.orc_wall_term.restype = C.c_double
+        L.orc_run.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _u8p, C.c_int, C.c_double, _dp, _dp,
+                              C.c_int, C.POINTER(OrcBC), C.c_int, _dp]
+        L.orc_run.restype = C.c_int
+        L.orc_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        L = C.CDLL(REF_PATH)
+        L.ref_k17_collide.argtypes = [_dp, C.c_double, _dp, _dp, _dp]
+        L.ref_chimera.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.ref_macroscopic.argtypes = [_dp, _dp]
+        L.ref_run_periodic.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]
+        L.ref_run_cavity.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_int]
+        L.ref_time_blocks.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]
+        L.ref_time_blocks.restype = C.c_double
+        _ref = L
+    return _ref
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+LIMITER = np.array([0.01, 0.01, 0.01])
+
+
+# ------------------------------------------------------------ single node ---
+def k17_collide(f27, omega, lim=LIMITER, force=(0.0, 0.0, 0.0), impl="oracle"):
+    """Post-collision populations and (drho, vx, vy, vz) of one node."""
+    f = np.ascontiguousarray(f27, dtype=np.float64).copy()
+    lim = np.ascontiguousarray(lim, dtype=np.float64)
+    fo = np.ascontiguousarray(force, dtype=np.float64)
+    mv = np.zeros(4)
+    L = lib() if impl == "oracle" else ref()
+    fn = L.orc_k17_collide if impl == "oracle" else L.ref_k17_collide
+    fn(_p(f), float(omega), _p(lim), _p(fo), _p(mv))
+    return f, mv
+
+
+def bgk_collide(f27, omega):
+    f = np.ascontiguousarray(f27, dtype=np.float64).copy()
+    mv = np.zeros(4)
+    lib().orc_bgk_collide(_p(f), float(omega), _p(mv))
+    return f, mv
+
+
+def chimera(kind, abc, vv, v2, kinv=1.0, k=1.0, impl="oracle"):
+    """kind: 0 forward-with-K, 1 backward-with-K, 2 forward, 3 backward."""
+    a = np.ascontiguousarray(abc, dtype=np.float64).copy()
+    L = lib() if impl == "oracle" else ref()
+    fn = L.orc_chimera if impl == "oracle" else L.ref_chimera
+    fn(int(kind), _p(a), float(vv), float(v2), float(kinv), float(k))
+    return a
+
+
+def macroscopic(f27, impl="oracle"):
+    f = np.ascontiguousarray(f27, dtype=np.float64)
+    out = np.zeros(4)
+    L = lib() if impl == "oracle" else ref()
+    fn = L.orc_macroscopic if impl == "oracle" else L.ref_macroscopic
+    fn(_p(f), _p(out))
+    return out
+
+
+def equilibrium(drho, ux, uy, uz):
+    f = np.zeros(27)
+    lib().orc_equilibrium(_p(f), float(drho), float(ux), float(uy), float(uz))
+    return f
+
+
+def init_field(drho, ux, uy, uz):
+    """Node-major shifted equilibrium populations for flat field arrays."""
+    drho = np.ascontiguousarray(drho, dtype=np.float64).ravel()
+    ux = np.ascontiguousarray(ux, dtype=np.float64).ravel()
+    uy = np.ascontiguousarray(uy, dtype=np.float64).ravel()
+    uz = np.ascontiguousarray(uz, dtype=np.float64).ravel()
+    F = np.zeros((drho.size, 27))
+    lib().orc_init_field(_p(F), drho.size, _p(drho), _p(ux), _p(uy), _p(uz))
+    return F
+
+
+def wall_term(d, ux, uy, uz, cpu_float=False):
+    return lib().orc_wall_term(int(d), float(ux), float(uy), float(uz), int(cpu_float))
+
+
+# ------------------------------------------------------------- lattices ---
+class BCGroup:
+    """Velocity bounce-back group in the VirtualFluids QforBoundaryConditions
+    shape: node indices, q[27][n] (link d is a wall link iff 0<=q<=1), and
+    per-node wall velocity."""
+
+    def __init__(self, node, q, ux=None, uy=None, uz=None, cpu_float_velocity=False, kind=0):
+        self.node = np.ascontiguousarray(node, dtype=np.int64)
+        self.q = np.ascontiguousarray(q, dtype=np.float32)
+        n = self.node.size
+        assert self.q.shape == (27, n)
+        z = np.zeros(n)
+        self.ux = np.ascontiguousarray(z if ux is None else np.broadcast_to(ux, (n,)), dtype=np.float64)
+        self.uy = np.ascontiguousarray(z if uy is None else np.broadcast_to(uy, (n,)), dtype=np.float64)
+        self.uz = np.ascontiguousarray(z if uz is None else np.broadcast_to(uz, (n,)), dtype=np.float64)
+        self.cpu_float_velocity = bool(cpu_float_velocity)
+        self.kind = kind
+
+    def c(self):
+        return OrcBC(self.kind, self.node.size, self.node.ctypes.data_as(C.POINTER(C.c_int64)),
+                     self.q.ctypes.data_as(C.POINTER(C.c_float)), _p(self.ux), _p(self.uy), _p(self.uz),
+                     int(self.cpu_float_velocity))
+
+
+def run(F, dims, nsteps, omega, collision="k17", solid=None, bcs=(), lim=LIMITER, force=(0.0, 0.0, 0.0),
+        want_macro=False):
+    """Advance node-major populations F (N x 27, x fastest) by nsteps."""
+    nx, ny, nz = dims
+    F = np.ascontiguousarray(F, dtype=np.float64).copy()
+    assert F.shape == (nx * ny * nz, 27)
+    sol = None
+    if solid is not None:
+        sol = np.ascontiguousarray(solid, dtype=np.uint8).ravel()
+        assert sol.size == nx * ny * nz
+    bcl = list(bcs)
+    arr = (OrcBC * max(1, len(bcl)))(*[b.c() for b in bcl]) if bcl else None
+    macro = np.zeros((nx * ny * nz, 4)) if want_macro else None
+    lim = np.ascontiguousarray(lim, dtype=np.float64)
+    fo = np.ascontiguousarray(force, dtype=np.float64)
+    rc = lib().orc_run(_p(F), nx, ny, nz, sol.ctypes.data_as(_u8p) if sol is not None else None,
+                       0 if collision == "k17" else 1, float(omega), _p(lim), _p(fo), len(bcl), arr,
+                       int(nsteps), _p(macro) if macro is not None else None)
+    if rc != 0:
+        raise RuntimeError(f"orc_run failed ({rc}): a fluid->solid link has no boundary condition")
+    return (F, macro) if want_macro else F
+
+
+def ref_run_periodic(F, dims, nsteps, omega):
+    nx, ny, nz = dims
+    F = np.ascontiguousarray(F, dtype=np.float64).copy()
+    ref().ref_run_periodic(_p(F), nx, ny, nz, float(omega), int(nsteps))
+    return F
+
+
+def ref_run_cavity(F, dims, nsteps, omega, lid=(0.05, 0.0, 0.0)):
+    nx, ny, nz = dims
+    F = np.ascontiguousarray(F, dtype=np.float64).copy()
+    ref().ref_run_cavity(_p(F), nx, ny, nz, float(omega), float(lid[0]), float(lid[1]), float(lid[2]),
+                         int(nsteps))
+    return F
+
+
+def ref_time_blocks(nblocks, bs, omega, nsteps):
+    t = C.c_int(0)
+    secs = ref().ref_time_blocks(int(nblocks), int(bs), float(omega), int(nsteps), C.byref(t))
+    return secs, t.value

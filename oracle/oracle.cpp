@@ -1,0 +1,644 @@
+// oracle.cpp -- CPU restatement of the VirtualFluids D3Q27 hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product (virtualfluids_amd/, include/)
+// may include, link or call this file; only tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg use it, and only as the checker.
+//
+// What it restates (reference paths relative to the VirtualFluids source root):
+//   * K17 cumulant collision      src/lbm/collision/K17CompressibleNavierStokes.h:87-591
+//   * chimera transforms          src/lbm/ChimeraTransformation.h:58-121
+//   * macroscopic sums            src/lbm/MacroscopicQuantities.h:55-133
+//   * compressible BGK            src/gpu/core/Kernel/Compressible/NavierStokes/B92/
+//                                   B92CompressibleNavierStokes_Device.cu:189-234
+//   * shifted equilibrium init    src/gpu/core/PreProcessor/PreProcessorStrategy/
+//                                   InitNavierStokesCompressible/InitNavierStokesCompressible_Device.cu:170-210
+//   * step order                  src/cpu/core/Simulation/Simulation.cpp:249-364
+//                                 (pre-BC -> collide -> stream -> post-BC -> swap)
+//   * velocity bounce-back        src/gpu/core/BoundaryConditions/Velocity/VelocityBounceBack.cu:159-184
+//                                 src/cpu/core/BoundaryConditions/VelocityBounceBack.cpp:59-79
+//                                 (+ float-rounded wall term, BoundaryConditions.h:195-262)
+//
+// Storage in the oracle is deliberately NOT the esoteric twist: the lattice is a
+// plain node-major array F[node][27] of pre-collision shifted populations
+// (f_i - w_i) and a step is collide -> push-stream into a second array -> swap.
+// Streaming is a pure copy, so this gives bit-identical populations to any
+// in-place scheme (EsoSplit on the CPU, the GPU esoteric twist) while sharing no
+// index arithmetic with them.
+//
+// Arithmetic: every expression keeps the reference's evaluation order, and the
+// file is compiled with -ffp-contract=off, so the FP64 results are the
+// reference's bit for bit.
+//
+// Parity pins: see tests/test_oracle_*.py -- K17 and the full step are checked
+// bit-exact against the compiled reference CPU closure (oracle/_ref) and against
+// committed golden vectors; the chimera/macroscopic helpers against the
+// reference unit tests' known answers; BGK against an independent textbook
+// formulation (unshifted populations) plus conservation/fixed-point invariants.
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+// ---------------------------------------------------------------- D3Q27 ----
+// Direction numbering of src/lbm/constants/D3Q27.h:42-72.
+enum : int {
+    d000 = 0, dP00, dM00, d0P0, d0M0, d00P, d00M, dPP0, dMM0, dPM0, dMP0,
+    dP0P, dM0M, dP0M, dM0P, d0PP, d0MM, d0PM, d0MP,
+    dPPP, dMPP, dPMP, dMMP, dPPM, dMPM, dPMM, dMMM
+};
+constexpr int CX[27] = {0, 1,-1, 0, 0, 0, 0, 1,-1, 1,-1, 1,-1, 1,-1, 0, 0, 0, 0, 1,-1, 1,-1, 1,-1, 1,-1};
+constexpr int CY[27] = {0, 0, 0, 1,-1, 0, 0, 1,-1,-1, 1, 0, 0, 0, 0, 1,-1, 1,-1, 1, 1,-1,-1, 1, 1,-1,-1};
+constexpr int CZ[27] = {0, 0, 0, 0, 0, 1,-1, 0, 0, 0, 0, 1,-1,-1, 1, 1,-1,-1, 1, 1, 1, 1, 1,-1,-1,-1,-1};
+
+int inverse_of(int d)
+{
+    for (int e = 0; e < 27; ++e)
+        if (CX[e] == -CX[d] && CY[e] == -CY[d] && CZ[e] == -CZ[d]) return e;
+    return -1;
+}
+
+// direction with velocity components (i-1, j-1, k-1), i,j,k in {0,1,2}
+int dir_of(int i, int j, int k)
+{
+    for (int e = 0; e < 27; ++e)
+        if (CX[e] == i - 1 && CY[e] == j - 1 && CZ[e] == k - 1) return e;
+    return -1;
+}
+
+struct Tables {
+    int inv[27];
+    int cube[3][3][3];
+    Tables()
+    {
+        for (int d = 0; d < 27; ++d) inv[d] = inverse_of(d);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                for (int k = 0; k < 3; ++k) cube[i][j][k] = dir_of(i, j, k);
+    }
+};
+const Tables T;
+
+// rational constants as the reference spells them (NumericConstants.h)
+constexpr double c1o2 = 1. / 2., c3o2 = 3. / 2., c1o3 = 1. / 3., c2o3 = 2. / 3., c1o6 = 1. / 6.;
+constexpr double c1o9 = 1. / 9., c2o9 = 2. / 9., c4o9 = 4. / 9., c1o18 = 1. / 18., c1o27 = 1. / 27.;
+constexpr double c1o36 = 1. / 36., c8o27 = 8. / 27., c2o27 = 2. / 27., c1o54 = 1. / 54., c1o216 = 1. / 216.;
+constexpr double c9o4 = 9. / 4., c9o2 = 9. / 2.;
+
+double weight(int d)
+{
+    int n = std::abs(CX[d]) + std::abs(CY[d]) + std::abs(CZ[d]);
+    return n == 0 ? c8o27 : n == 1 ? c2o27 : n == 2 ? c1o54 : c1o216;
+}
+
+// -------------------------------------------------------- chimera helpers ----
+// ChimeraTransformation.h:58-70
+inline void fwd_k(double& a, double& b, double& c, double vv, double v2, double kinv, double k)
+{
+    const double s = a + c;
+    const double d = c - a;
+    double m0 = s + b;
+    a = m0;
+    m0 *= kinv;
+    m0 += 1.0;
+    b = (d * kinv - m0 * vv) * k;
+    c = ((s - 2.0 * d * vv) * kinv + v2 * m0) * k;
+}
+// ChimeraTransformation.h:78-87
+inline void bwd_k(double& a, double& b, double& c, double vv, double v2, double kinv, double k)
+{
+    const double r0 = (((c - b) * c1o2 + b * vv) * kinv + (a * kinv + 1.0) * (v2 - vv) * c1o2) * k;
+    const double r1 = (((a - c) - 2.0 * b * vv) * kinv + (a * kinv + 1.0) * (-v2)) * k;
+    c = (((c + b) * c1o2 + b * vv) * kinv + (a * kinv + 1.0) * (v2 + vv) * c1o2) * k;
+    a = r0;
+    b = r1;
+}
+// ChimeraTransformation.h:96-104
+inline void fwd(double& a, double& b, double& c, double vv, double v2)
+{
+    const double s = (a + c) + b;
+    const double d = c - a;
+    c = (c + a) + (v2 * s - 2.0 * vv * d);
+    b = d - vv * s;
+    a = s;
+}
+// ChimeraTransformation.h:113-121
+inline void bwd(double& a, double& b, double& c, double vv, double v2)
+{
+    const double ra = (c + a * (v2 - vv)) * c1o2 + b * (vv - c1o2);
+    const double rb = ((a - c) - a * v2) - 2.0 * b * vv;
+    c = (c + a * (v2 + vv)) * c1o2 + b * (vv + c1o2);
+    b = rb;
+    a = ra;
+}
+
+// ------------------------------------------------------ macroscopic sums ----
+// MacroscopicQuantities.h:55-64 (pyramid summation)
+double density(const double* f)
+{
+    const double corners = ((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]));
+    const double edges = (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P]))) +
+                         ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]));
+    const double faces = ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0])) + (f[d00M] + f[d00P]);
+    return ((corners + edges) + f[d000]) + faces;
+}
+// MacroscopicQuantities.h:69-85
+double mom_x(const double* f)
+{
+    return (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dPMM] - f[dMPP]) + (f[dPPM] - f[dMMP]))) +
+           (((f[dP0M] - f[dM0P]) + (f[dP0P] - f[dM0M])) + ((f[dPM0] - f[dMP0]) + (f[dPP0] - f[dMM0]))) +
+           (f[dP00] - f[dM00]);
+}
+double mom_y(const double* f)
+{
+    return (((f[dPPP] - f[dMMM]) + (f[dMPM] - f[dPMP])) + ((f[dMPP] - f[dPMM]) + (f[dPPM] - f[dMMP]))) +
+           (((f[d0PM] - f[d0MP]) + (f[d0PP] - f[d0MM])) + ((f[dMP0] - f[dPM0]) + (f[dPP0] - f[dMM0]))) +
+           (f[d0P0] - f[d0M0]);
+}
+double mom_z(const double* f)
+{
+    return (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dMPP] - f[dPMM]) + (f[dMMP] - f[dPPM]))) +
+           (((f[d0MP] - f[d0PM]) + (f[d0PP] - f[d0MM])) + ((f[dM0P] - f[dP0M]) + (f[dP0P] - f[dM0M]))) +
+           (f[d00P] - f[d00M]);
+}
+
+// ------------------------------------------------------------- K17 -------
+// Weight of a velocity-space slot along one axis before that axis is
+// transformed: M/P -> 1/6, 0 -> 2/3; after it is transformed the slot holds a
+// moment of order 0,1,2 whose weight factor is 1, (none), 1/3.  The chimera
+// constant K of a row is the product of the factors of the two other axes.
+// Reproduces the K arguments of K17CompressibleNavierStokes.h:190-222 and
+// :525-557, including the (K^-1 = 3, K = 1/9) pair the reference passes for
+// the (y,z) = (2,2) row of the forward x sweep (:222).
+void k17_collide(double* f, double omega, const double* lim, const double* force, double* macro)
+{
+    double m[3][3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) m[i][j][k] = f[T.cube[i][j][k]];
+
+    // :157 macroscopic values
+    const double drho = density(f);
+    double vx = mom_x(f), vy = mom_y(f), vz = mom_z(f);
+    const double oneOverRho = 1.0 / (drho + 1.0);
+    vx *= oneOverRho;
+    vy *= oneOverRho;
+    vz *= oneOverRho;
+    // :164-166 half force
+    vx += force[0];
+    vy += force[1];
+    vz += force[2];
+    const double vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+    const double limP = lim[0], limM = lim[1], limD = lim[2];
+
+    // forward z sweep (:190-198): rows (i,j), K = w(i)*w(j) with w(M)=w(P)=1/6, w(0)=2/3
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double kinv, k;
+            if (i == 1 && j == 1) { kinv = c9o4; k = c4o9; }
+            else if (i == 1 || j == 1) { kinv = 9.0; k = c1o9; }
+            else { kinv = 36.0; k = c1o36; }
+            fwd_k(m[i][j][0], m[i][j][1], m[i][j][2], vz, vz2, kinv, k);
+        }
+    // forward y sweep (:202-210): rows (i, cz); cz==1 has no K
+    for (int i = 0; i < 3; ++i)
+        for (int c = 0; c < 3; ++c) {
+            if (c == 1) { fwd(m[i][0][c], m[i][1][c], m[i][2][c], vy, vy2); continue; }
+            double kinv, k;
+            if (i == 1) { kinv = (c == 0) ? c3o2 : c9o2; k = (c == 0) ? c2o3 : c2o9; }
+            else        { kinv = (c == 0) ? 6.0 : 18.0; k = (c == 0) ? c1o6 : c1o18; }
+            fwd_k(m[i][0][c], m[i][1][c], m[i][2][c], vy, vy2, kinv, k);
+        }
+    // forward x sweep (:214-222): rows (cy, cz)
+    for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+            if (b == 1 || c == 1) { fwd(m[0][b][c], m[1][b][c], m[2][b][c], vx, vx2); continue; }
+            double kinv, k;
+            if (b == 0 && c == 0) { kinv = 1.0; k = 1.0; }
+            else if (b == 2 && c == 2) { kinv = 3.0; k = c1o9; }   // as passed at :222
+            else { kinv = 3.0; k = c1o3; }
+            fwd_k(m[0][b][c], m[1][b][c], m[2][b][c], vx, vx2, kinv, k);
+        }
+
+    // aliases: m[a][b][c] is the central moment of order (a,b,c)
+    double& m000 = m[0][0][0]; double& m100 = m[1][0][0]; double& m010 = m[0][1][0]; double& m001 = m[0][0][1];
+    double& m200 = m[2][0][0]; double& m020 = m[0][2][0]; double& m002 = m[0][0][2];
+    double& m110 = m[1][1][0]; double& m101 = m[1][0][1]; double& m011 = m[0][1][1];
+    double& m111 = m[1][1][1];
+    double& m210 = m[2][1][0]; double& m012 = m[0][1][2]; double& m201 = m[2][0][1];
+    double& m021 = m[0][2][1]; double& m120 = m[1][2][0]; double& m102 = m[1][0][2];
+    double& m211 = m[2][1][1]; double& m121 = m[1][2][1]; double& m112 = m[1][1][2];
+    double& m220 = m[2][2][0]; double& m202 = m[2][0][2]; double& m022 = m[0][2][2];
+    double& m122 = m[1][2][2]; double& m212 = m[2][1][2]; double& m221 = m[2][2][1];
+    double& m222 = m[2][2][2];
+
+    // :245-273 relaxation rates (TurbulenceModel::None)
+    const double OxxPyyPzz = 1.0;
+    const double OxyyPxzz = 8.0 * (-2.0 + omega) * (1.0 + 2.0 * omega) / (-8.0 - 14.0 * omega + 7.0 * omega * omega);
+    const double OxyyMxzz = 8.0 * (-2.0 + omega) * (-7.0 + 4.0 * omega) / (56.0 - 50.0 * omega + 9.0 * omega * omega);
+    const double Oxyz = 24.0 * (-2.0 + omega) * (-2.0 - 7.0 * omega + 3.0 * omega * omega) /
+                        (48.0 + 152.0 * omega - 130.0 * omega * omega + 29.0 * omega * omega * omega);
+    const double O4 = 1.0, O5 = 1.0, O6 = 1.0;
+    const double factorA = (4.0 + 2.0 * omega - 3.0 * omega * omega) / (2.0 - 7.0 * omega + 5.0 * omega * omega);
+    const double factorB = (4.0 + 28.0 * omega - 14.0 * omega * omega) / (6.0 - 21.0 * omega + 15.0 * omega * omega);
+
+    // :282-316 cumulants of order 4..6
+    double cm211 = m211 - ((m200 + c1o3) * m011 + 2.0 * m110 * m101) * oneOverRho;
+    double cm121 = m121 - ((m020 + c1o3) * m101 + 2.0 * m110 * m011) * oneOverRho;
+    double cm112 = m112 - ((m002 + c1o3) * m110 + 2.0 * m101 * m011) * oneOverRho;
+    double cm220 = m220 - (((m200 * m020 + 2.0 * m110 * m110) + c1o3 * (m200 + m020)) * oneOverRho - c1o9 * (drho * oneOverRho));
+    double cm202 = m202 - (((m200 * m002 + 2.0 * m101 * m101) + c1o3 * (m200 + m002)) * oneOverRho - c1o9 * (drho * oneOverRho));
+    double cm022 = m022 - (((m002 * m020 + 2.0 * m011 * m011) + c1o3 * (m002 + m020)) * oneOverRho - c1o9 * (drho * oneOverRho));
+    double cm122 = m122 - ((m002 * m120 + m020 * m102 + 4.0 * m011 * m111 + 2.0 * (m101 * m021 + m110 * m012)) +
+                           c1o3 * (m120 + m102)) * oneOverRho;
+    double cm212 = m212 - ((m002 * m210 + m200 * m012 + 4.0 * m101 * m111 + 2.0 * (m011 * m201 + m110 * m102)) +
+                           c1o3 * (m210 + m012)) * oneOverRho;
+    double cm221 = m221 - ((m200 * m021 + m020 * m201 + 4.0 * m110 * m111 + 2.0 * (m101 * m120 + m011 * m210)) +
+                           c1o3 * (m021 + m201)) * oneOverRho;
+    double cm222 = m222 + ((-4.0 * m111 * m111 - (m200 * m022 + m020 * m202 + m002 * m220) -
+                            4.0 * (m011 * m211 + m101 * m121 + m110 * m112) -
+                            2.0 * (m120 * m102 + m210 * m012 + m201 * m021)) * oneOverRho +
+                           (4.0 * (m101 * m101 * m020 + m011 * m011 * m200 + m110 * m110 * m002) +
+                            2.0 * (m200 * m020 * m002) + 16.0 * m110 * m101 * m011) * oneOverRho * oneOverRho -
+                           c1o3 * (m022 + m202 + m220) * oneOverRho - c1o9 * (m200 + m020 + m002) * oneOverRho +
+                           (2.0 * (m101 * m101 + m011 * m011 + m110 * m110) +
+                            (m002 * m020 + m002 * m200 + m020 * m200) + c1o3 * (m002 + m020 + m200)) *
+                               oneOverRho * oneOverRho * c2o3 +
+                           c1o27 * ((drho * drho - drho) * oneOverRho * oneOverRho));
+
+    // :323-335 linear combinations
+    double mxxPyyPzz = m200 + m020 + m002;
+    double mxxMyy = m200 - m020;
+    double mxxMzz = m200 - m002;
+    double mxxyPyzz = m210 + m012, mxxyMyzz = m210 - m012;
+    double mxxzPyyz = m201 + m021, mxxzMyyz = m201 - m021;
+    double mxyyPxzz = m120 + m102, mxyyMxzz = m120 - m102;
+
+    // :347-352 velocity gradients (times rho)
+    const double Dxy = -3.0 * omega * m110;
+    const double Dxz = -3.0 * omega * m101;
+    const double Dyz = -3.0 * omega * m011;
+    const double dxux = c1o2 * (-omega) * (mxxMyy + mxxMzz) + c1o2 * OxxPyyPzz * (m000 - mxxPyyPzz);
+    const double dyuy = dxux + omega * c3o2 * mxxMyy;
+    const double dzuz = dxux + omega * c3o2 * mxxMzz;
+
+    // :375-387 second order with correction
+    mxxPyyPzz += OxxPyyPzz * (m000 - mxxPyyPzz) - 3.0 * (1.0 - c1o2 * OxxPyyPzz) * (vx2 * dxux + vy2 * dyuy + vz2 * dzuz);
+    mxxMyy += omega * (-mxxMyy) - 3.0 * (1.0 + c1o2 * (-omega)) * (vx2 * dxux - vy2 * dyuy);
+    mxxMzz += omega * (-mxxMzz) - 3.0 * (1.0 + c1o2 * (-omega)) * (vx2 * dxux - vz2 * dzuz);
+    m011 += omega * (-m011);
+    m101 += omega * (-m101);
+    m110 += omega * (-m110);
+
+    // :397-410 limited third order
+    auto limited = [](double rate, double x, double l) { return rate + (1.0 - rate) * std::fabs(x) / (std::fabs(x) + l); };
+    double w = limited(Oxyz, m111, limD);
+    m111 += w * (-m111);
+    w = limited(OxyyPxzz, mxxyPyzz, limP); mxxyPyzz += w * (-mxxyPyzz);
+    w = limited(OxyyMxzz, mxxyMyzz, limM); mxxyMyzz += w * (-mxxyMyzz);
+    w = limited(OxyyPxzz, mxxzPyyz, limP); mxxzPyyz += w * (-mxxzPyyz);
+    w = limited(OxyyMxzz, mxxzMyyz, limM); mxxzMyyz += w * (-mxxzMyyz);
+    w = limited(OxyyPxzz, mxyyPxzz, limP); mxyyPxzz += w * (-mxyyPxzz);
+    w = limited(OxyyMxzz, mxyyMxzz, limM); mxyyMxzz += w * (-mxyyMxzz);
+
+    // :424-433 back from linear combinations
+    m200 = c1o3 * (mxxMyy + mxxMzz + mxxPyyPzz);
+    m020 = c1o3 * (-2.0 * mxxMyy + mxxMzz + mxxPyyPzz);
+    m002 = c1o3 * (mxxMyy - 2.0 * mxxMzz + mxxPyyPzz);
+    m210 = (mxxyMyzz + mxxyPyzz) * c1o2;
+    m012 = (-mxxyMyzz + mxxyPyzz) * c1o2;
+    m201 = (mxxzMyyz + mxxzPyyz) * c1o2;
+    m021 = (-mxxzMyyz + mxxzPyyz) * c1o2;
+    m120 = (mxyyMxzz + mxyyPxzz) * c1o2;
+    m102 = (-mxyyMxzz + mxyyPxzz) * c1o2;
+
+    // :443-459 fourth order to modified equilibrium, 5th/6th to zero
+    cm022 = -O4 * (1.0 / omega - c1o2) * (dyuy + dzuz) * c2o3 * factorA + (1.0 - O4) * (cm022);
+    cm202 = -O4 * (1.0 / omega - c1o2) * (dxux + dzuz) * c2o3 * factorA + (1.0 - O4) * (cm202);
+    cm220 = -O4 * (1.0 / omega - c1o2) * (dyuy + dxux) * c2o3 * factorA + (1.0 - O4) * (cm220);
+    cm112 = -O4 * (1.0 / omega - c1o2) * Dxy * c1o3 * factorB + (1.0 - O4) * (cm112);
+    cm121 = -O4 * (1.0 / omega - c1o2) * Dxz * c1o3 * factorB + (1.0 - O4) * (cm121);
+    cm211 = -O4 * (1.0 / omega - c1o2) * Dyz * c1o3 * factorB + (1.0 - O4) * (cm211);
+    cm122 += O5 * (-cm122);
+    cm212 += O5 * (-cm212);
+    cm221 += O5 * (-cm221);
+    cm222 += O6 * (-cm222);
+
+    // :469-505 central moments from post-collision cumulants
+    m211 = cm211 + c1o3 * ((3.0 * m200 + 1.0) * m011 + 6.0 * m110 * m101) * oneOverRho;
+    m121 = cm121 + c1o3 * ((3.0 * m020 + 1.0) * m101 + 6.0 * m110 * m011) * oneOverRho;
+    m112 = cm112 + c1o3 * ((3.0 * m002 + 1.0) * m110 + 6.0 * m101 * m011) * oneOverRho;
+    m220 = cm220 + (((m200 * m020 + 2.0 * m110 * m110) * 9.0 + 3.0 * (m200 + m020)) * oneOverRho - (drho * oneOverRho)) * c1o9;
+    m202 = cm202 + (((m200 * m002 + 2.0 * m101 * m101) * 9.0 + 3.0 * (m200 + m002)) * oneOverRho - (drho * oneOverRho)) * c1o9;
+    m022 = cm022 + (((m002 * m020 + 2.0 * m011 * m011) * 9.0 + 3.0 * (m002 + m020)) * oneOverRho - (drho * oneOverRho)) * c1o9;
+    m122 = cm122 + c1o3 * (3.0 * (m002 * m120 + m020 * m102 + 4.0 * m011 * m111 + 2.0 * (m101 * m021 + m110 * m012)) +
+                           (m120 + m102)) * oneOverRho;
+    m212 = cm212 + c1o3 * (3.0 * (m002 * m210 + m200 * m012 + 4.0 * m101 * m111 + 2.0 * (m011 * m201 + m110 * m102)) +
+                           (m210 + m012)) * oneOverRho;
+    m221 = cm221 + c1o3 * (3.0 * (m200 * m021 + m020 * m201 + 4.0 * m110 * m111 + 2.0 * (m101 * m120 + m011 * m210)) +
+                           (m021 + m201)) * oneOverRho;
+    m222 = cm222 - ((-4.0 * m111 * m111 - (m200 * m022 + m020 * m202 + m002 * m220) -
+                     4.0 * (m011 * m211 + m101 * m121 + m110 * m112) -
+                     2.0 * (m120 * m102 + m210 * m012 + m201 * m021)) * oneOverRho +
+                    (4.0 * (m101 * m101 * m020 + m011 * m011 * m200 + m110 * m110 * m002) +
+                     2.0 * (m200 * m020 * m002) + 16.0 * m110 * m101 * m011) * oneOverRho * oneOverRho -
+                    c1o3 * (m022 + m202 + m220) * oneOverRho - c1o9 * (m200 + m020 + m002) * oneOverRho +
+                    (2.0 * (m101 * m101 + m011 * m011 + m110 * m110) +
+                     (m002 * m020 + m002 * m200 + m020 * m200) + c1o3 * (m002 + m020 + m200)) *
+                        oneOverRho * oneOverRho * c2o3 +
+                    c1o27 * ((drho * drho - drho) * oneOverRho * oneOverRho));
+
+    // :512-514
+    m100 = -m100;
+    m010 = -m010;
+    m001 = -m001;
+
+    // backward x sweep (:525-533): rows (cy,cz)
+    for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+            if (b == 1 || c == 1) { bwd(m[0][b][c], m[1][b][c], m[2][b][c], vx, vx2); continue; }
+            double kinv, k;
+            if (b == 0 && c == 0) { kinv = 1.0; k = 1.0; }
+            else if (b == 2 && c == 2) { kinv = 9.0; k = c1o9; }
+            else { kinv = 3.0; k = c1o3; }
+            bwd_k(m[0][b][c], m[1][b][c], m[2][b][c], vx, vx2, kinv, k);
+        }
+    // backward y sweep (:537-545): rows (i, cz)
+    for (int i = 0; i < 3; ++i)
+        for (int c = 0; c < 3; ++c) {
+            if (c == 1) { bwd(m[i][0][c], m[i][1][c], m[i][2][c], vy, vy2); continue; }
+            double kinv, k;
+            if (i == 1) { kinv = (c == 0) ? c3o2 : c9o2; k = (c == 0) ? c2o3 : c2o9; }
+            else        { kinv = (c == 0) ? 6.0 : 18.0; k = (c == 0) ? c1o6 : c1o18; }
+            bwd_k(m[i][0][c], m[i][1][c], m[i][2][c], vy, vy2, kinv, k);
+        }
+    // backward z sweep (:549-557): rows (i,j)
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double kinv, k;
+            if (i == 1 && j == 1) { kinv = c9o4; k = c4o9; }
+            else if (i == 1 || j == 1) { kinv = 9.0; k = c1o9; }
+            else { kinv = 36.0; k = c1o36; }
+            bwd_k(m[i][j][0], m[i][j][1], m[i][j][2], vz, vz2, kinv, k);
+        }
+
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) f[T.cube[i][j][k]] = m[i][j][k];
+
+    if (macro) { macro[0] = drho; macro[1] = vx; macro[2] = vy; macro[3] = vz; }
+}
+
+// ----------------------------------------------------------- BGK (B92) ----
+// B92CompressibleNavierStokes_Device.cu:189-234; the naming there is
+// E=dP00 W=dM00 N=d0P0 S=d0M0 T=d00P B=d00M and compounds thereof.
+void bgk_collide(double* f, double omega, double* macro)
+{
+    const double drho = (f[dPPP] + f[dMMM]) + (f[dMMP] + f[dPPM]) + (f[dPMP] + f[dMPM]) + (f[dMPP] + f[dPMM]) +
+                        (f[dPP0] + f[dMM0]) + (f[dMP0] + f[dPM0]) + (f[dP0P] + f[dM0M]) + (f[dP0M] + f[dM0P]) +
+                        (f[d0PP] + f[d0MM]) + (f[d0PM] + f[d0MP]) + (f[dP00] + f[dM00]) + (f[d0P0] + f[d0M0]) +
+                        (f[d00P] + f[d00M]) + f[d000];
+    const double rho = drho + 1.0;
+    const double oor = 1.0 / rho;
+    const double vx1 = oor * ((f[dPPP] - f[dMMM]) + (f[dPPM] - f[dMMP]) + (f[dPMP] - f[dMPM]) + (f[dPMM] - f[dMPP]) +
+                              (f[dPP0] - f[dMM0]) + (f[dPM0] - f[dMP0]) + (f[dP0P] - f[dM0M]) + (f[dP0M] - f[dM0P]) +
+                              (f[dP00] - f[dM00]));
+    const double vx2 = oor * ((f[dPPP] - f[dMMM]) + (f[dPPM] - f[dMMP]) + (f[dMPM] - f[dPMP]) + (f[dMPP] - f[dPMM]) +
+                              (f[dPP0] - f[dMM0]) + (f[dMP0] - f[dPM0]) + (f[d0PP] - f[d0MM]) + (f[d0PM] - f[d0MP]) +
+                              (f[d0P0] - f[d0M0]));
+    const double vx3 = oor * ((f[dPPP] - f[dMMM]) + (f[dMMP] - f[dPPM]) + (f[dPMP] - f[dMPM]) + (f[dMPP] - f[dPMM]) +
+                              (f[dP0P] - f[dM0M]) + (f[dM0P] - f[dP0M]) + (f[d0PP] - f[d0MM]) + (f[d0MP] - f[d0PM]) +
+                              (f[d00P] - f[d00M]));
+    const double cusq = c3o2 * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3);
+    const double a = 1.0 + (-omega);
+    for (int d = 0; d < 27; ++d) {
+        const double wt = weight(d);
+        if (d == d000) { f[d] = f[d] * a - (-omega) * wt * (drho - rho * cusq); continue; }
+        // projected velocity spelled as the reference writes it: components in
+        // x,y,z order with the direction's signs
+        double cu;
+        const int c[3] = {CX[d], CY[d], CZ[d]};
+        const double v[3] = {vx1, vx2, vx3};
+        bool first = true;
+        cu = 0.0;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (c[ax] == 0) continue;
+            if (first) { cu = c[ax] > 0 ? v[ax] : -v[ax]; first = false; }
+            else cu = c[ax] > 0 ? cu + v[ax] : cu - v[ax];
+        }
+        f[d] = f[d] * a - (-omega) * wt * (drho + rho * (3.0 * cu + c9o2 * cu * cu - cusq));
+    }
+    if (macro) { macro[0] = drho; macro[1] = vx1; macro[2] = vx2; macro[3] = vx3; }
+}
+
+// ------------------------------------------------------ shifted equilibrium ----
+// InitNavierStokesCompressible_Device.cu:170-210
+void equilibrium(double* f, double drho, double vx1, double vx2, double vx3)
+{
+    const double cu_sq = c3o2 * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3);
+    const double v[3] = {vx1, vx2, vx3};
+    for (int d = 0; d < 27; ++d) {
+        if (d == d000) { f[d] = c8o27 * (drho - cu_sq * (1.0 + drho)); continue; }
+        const int c[3] = {CX[d], CY[d], CZ[d]};
+        double cu = 0.0;
+        bool first = true;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (c[ax] == 0) continue;
+            if (first) { cu = c[ax] > 0 ? v[ax] : -v[ax]; first = false; }
+            else cu = c[ax] > 0 ? cu + v[ax] : cu - v[ax];
+        }
+        f[d] = weight(d) * (drho + (1.0 + drho) * (3.0 * cu + c9o2 * cu * cu - cu_sq));
+    }
+}
+
+// Wall term of the moving-wall bounce-back for link direction d (towards the
+// wall).  GPU (VelocityBounceBack.cu:159-184): 6 w_d (c_d . u) in working
+// precision.  CPU (BoundaryConditions.h:195-262): the wall velocity is stored
+// as float, the signed component sum is formed in float, scaled by the double
+// constant and rounded to float again.
+double wall_term(int d, double ux, double uy, double uz, bool cpu_float)
+{
+    const int n = std::abs(CX[d]) + std::abs(CY[d]) + std::abs(CZ[d]);
+    const double coef = n == 1 ? c4o9 : n == 2 ? c1o9 : c1o36;
+    const int c[3] = {CX[d], CY[d], CZ[d]};
+    if (cpu_float) {
+        const float u[3] = {(float)ux, (float)uy, (float)uz};
+        float s = 0.0f;
+        bool first = true;
+        for (int ax = 0; ax < 3; ++ax) {
+            if (c[ax] == 0) continue;
+            if (first) { s = c[ax] > 0 ? +u[ax] : -u[ax]; first = false; }
+            else s = c[ax] > 0 ? s + u[ax] : s - u[ax];
+        }
+        return (double)(float)(coef * (double)s);
+    }
+    const double u[3] = {ux, uy, uz};
+    double s = 0.0;
+    bool first = true;
+    for (int ax = 0; ax < 3; ++ax) {
+        if (c[ax] == 0) continue;
+        if (first) { s = c[ax] > 0 ? u[ax] : -u[ax]; first = false; }
+        else s = c[ax] > 0 ? s + u[ax] : s - u[ax];
+    }
+    return coef * s;
+}
+
+} // namespace
+
+// ============================================================== C ABI =====
+extern "C" {
+
+enum { ORC_K17 = 0, ORC_BGK = 1 };
+enum { ORC_BC_VELOCITY_BOUNCE_BACK = 0 };
+
+typedef struct {
+    int32_t kind;             // ORC_BC_*
+    int32_t n;                // number of boundary nodes
+    const int64_t* node;      // dense node index of each boundary node
+    const float* q;           // [27*n] structure of arrays; link d of node i is a wall link iff 0 <= q[d*n+i] <= 1
+    const double* ux;         // [n] wall velocity (NULL = 0)
+    const double* uy;
+    const double* uz;
+    int32_t cpu_float_velocity;
+} orc_bc;
+
+void orc_k17_collide(double* f27, double omega, const double* lim3, const double* force3, double* macro4)
+{
+    static const double zero[3] = {0, 0, 0};
+    k17_collide(f27, omega, lim3, force3 ? force3 : zero, macro4);
+}
+
+void orc_bgk_collide(double* f27, double omega, double* macro4) { bgk_collide(f27, omega, macro4); }
+
+void orc_chimera(int kind, double* abc, double vv, double v2, double kinv, double k)
+{
+    switch (kind) {
+    case 0: fwd_k(abc[0], abc[1], abc[2], vv, v2, kinv, k); break;
+    case 1: bwd_k(abc[0], abc[1], abc[2], vv, v2, kinv, k); break;
+    case 2: fwd(abc[0], abc[1], abc[2], vv, v2); break;
+    default: bwd(abc[0], abc[1], abc[2], vv, v2); break;
+    }
+}
+
+// out4 = {drho, vx, vy, vz} (compressible: velocities divided by 1+drho)
+void orc_macroscopic(const double* f27, double* out4)
+{
+    const double drho = density(f27);
+    const double oor = 1.0 / (drho + 1.0);
+    out4[0] = drho;
+    out4[1] = mom_x(f27) * oor;
+    out4[2] = mom_y(f27) * oor;
+    out4[3] = mom_z(f27) * oor;
+}
+
+void orc_equilibrium(double* f27, double drho, double ux, double uy, double uz) { equilibrium(f27, drho, ux, uy, uz); }
+
+void orc_init_field(double* F, int64_t n, const double* drho, const double* ux, const double* uy, const double* uz)
+{
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) equilibrium(F + 27 * i, drho[i], ux[i], uy[i], uz[i]);
+}
+
+double orc_wall_term(int d, double ux, double uy, double uz, int cpu_float) { return wall_term(d, ux, uy, uz, cpu_float != 0); }
+
+// Run nsteps on a dense nx*ny*nz box (x fastest, periodic index wrap on every
+// axis; walls are solid nodes).  F holds node-major pre-collision populations
+// and is overwritten with the state after nsteps.  macro (optional, 4*N)
+// receives {drho,vx,vy,vz} of the last collision.  Returns 0, or a negative
+// code when a fluid node streams into a solid node through a link no boundary
+// condition covers.
+int orc_run(double* F, int nx, int ny, int nz, const uint8_t* solid, int collision, double omega,
+            const double* lim3, const double* force3, int n_bc, const orc_bc* bcs, int nsteps, double* macro)
+{
+    const int64_t N = (int64_t)nx * ny * nz;
+    std::vector<double> post((size_t)N * 27), G((size_t)N * 27);
+    static const double zero[3] = {0, 0, 0};
+    if (!force3) force3 = zero;
+
+    // link coverage: covered[i*27+d] = 1 if node i has a boundary condition on link d
+    std::vector<uint8_t> covered;
+    if (solid) {
+        covered.assign((size_t)N * 27, 0);
+        for (int b = 0; b < n_bc; ++b)
+            for (int i = 0; i < bcs[b].n; ++i)
+                for (int d = 1; d < 27; ++d) {
+                    const float q = bcs[b].q[(int64_t)d * bcs[b].n + i];
+                    if (q >= 0.0f && q <= 1.0f) covered[(size_t)bcs[b].node[i] * 27 + d] = 1;
+                }
+    }
+    auto neighbour = [&](int64_t i, int d) {
+        int x = (int)(i % nx), y = (int)((i / nx) % ny), z = (int)(i / ((int64_t)nx * ny));
+        x = (x + CX[d] + nx) % nx;
+        y = (y + CY[d] + ny) % ny;
+        z = (z + CZ[d] + nz) % nz;
+        return (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+    };
+
+    for (int s = 0; s < nsteps; ++s) {
+        int bad = 0;
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+        for (int64_t i = 0; i < N; ++i) {
+            if (solid && solid[i]) continue;
+            double f[27];
+            std::memcpy(f, F + 27 * i, sizeof(f));
+            double mv[4];
+            if (collision == ORC_K17) k17_collide(f, omega, lim3, force3, mv);
+            else bgk_collide(f, omega, mv);
+            std::memcpy(&post[(size_t)i * 27], f, sizeof(f));
+            if (macro && s == nsteps - 1) std::memcpy(macro + 4 * i, mv, sizeof(mv));
+        }
+        // push streaming
+#pragma omp parallel for schedule(static) reduction(+ : bad)
+        for (int64_t i = 0; i < N; ++i) {
+            if (solid && solid[i]) continue;
+            for (int d = 0; d < 27; ++d) {
+                const int64_t j = neighbour(i, d);
+                if (solid && solid[j]) {
+                    if (!covered[(size_t)i * 27 + d]) bad += 1;
+                    continue;
+                }
+                G[(size_t)j * 27 + d] = post[(size_t)i * 27 + d];
+            }
+        }
+        if (bad) return -1;
+        // post-collision boundary conditions (writes the population that
+        // re-enters node i against the wall link d)
+        for (int b = 0; b < n_bc; ++b) {
+            const orc_bc& bc = bcs[b];
+            for (int i = 0; i < bc.n; ++i) {
+                const int64_t k = bc.node[i];
+                const double ux = bc.ux ? bc.ux[i] : 0.0, uy = bc.uy ? bc.uy[i] : 0.0, uz = bc.uz ? bc.uz[i] : 0.0;
+                for (int d = 1; d < 27; ++d) {
+                    const float q = bc.q[(int64_t)d * bc.n + i];
+                    if (!(q >= 0.0f && q <= 1.0f)) continue;
+                    const double fd = post[(size_t)k * 27 + d];
+                    G[(size_t)k * 27 + T.inv[d]] = fd - wall_term(d, ux, uy, uz, bc.cpu_float_velocity != 0);
+                }
+            }
+        }
+        // solid nodes keep whatever they had (never read)
+        if (solid)
+            for (int64_t i = 0; i < N; ++i)
+                if (solid[i]) std::memcpy(&G[(size_t)i * 27], F + 27 * i, 27 * sizeof(double));
+        std::memcpy(F, G.data(), (size_t)N * 27 * sizeof(double));
+    }
+    return 0;
+}
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+} // extern "C"
