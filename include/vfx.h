@@ -1,0 +1,144 @@
+/* vfx.h -- C ABI of the MI355X (gfx950) D3Q27 lattice-Boltzmann stepper.
+ *
+ * One GPU, stream ordered, no process aborts: every entry point returns a
+ * vfx_status and vfx_last_error() describes the last failure of the calling
+ * thread.  All array pointers are device pointers owned by the caller; the
+ * library keeps none of them between calls.
+ *
+ * Data layout (identical to VirtualFluids' GPU path so arrays can be handed
+ * over unchanged):
+ *   - distributions: 27 planes of n_nodes values, plane d at d*n_nodes, holding
+ *     shifted ("well-conditioned") populations f_d - w_d in VirtualFluids'
+ *     direction order (src/lbm/constants/D3Q27.h:42-72);
+ *   - in-place esoteric-twist streaming: a step of parity is_even_timestep
+ *     reads and writes the same elements (src/gpu/core/Utilities/
+ *     KernelUtilities.h:66-125, :252-379); the caller flips the parity after
+ *     every step exactly as UpdateGrid27::swapBetweenEvenAndOddTimestep does
+ *     (src/gpu/core/Calculation/UpdateGrid27.cpp:264-268).
+ *
+ * Storage forms (vfx_lattice.storage):
+ *   VFX_FP64          double populations, FP64 arithmetic;
+ *   VFX_FP32          float populations, FP32 arithmetic;
+ *   VFX_FP16_SCALED   half populations h_d = (f_d - w_d) * scale / w_d, FP32
+ *                     arithmetic (scale = vfx_lattice.fp16_scale, one per lattice).
+ */
+#ifndef VFX_H
+#define VFX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* vfx_stream; /* a hipStream_t; NULL = the null stream */
+
+typedef enum { VFX_OK = 0, VFX_EINVAL = 1, VFX_EHIP = 2, VFX_EUNSUPPORTED = 3 } vfx_status;
+
+typedef enum { VFX_FP64 = 0, VFX_FP32 = 1, VFX_FP16_SCALED = 2 } vfx_storage;
+
+/* collision operators: the kernels VirtualFluids registers as
+ * "K17CompressibleNavierStokes" and "BGKCompSP27" (src/gpu/core/Kernel/KernelTypes.h:35-46) */
+typedef enum { VFX_K17 = 0, VFX_BGK = 1 } vfx_collision;
+
+/* CollisionTemplate (src/gpu/core/Calculation/Calculation.h:55-66):
+ * WriteMacroVars additionally stores drho, vx, vy, vz of every collided node. */
+typedef enum { VFX_TEMPLATE_DEFAULT = 0, VFX_TEMPLATE_WRITE_MACRO = 1 } vfx_collision_template;
+
+typedef struct vfx_lattice {
+    void* f;               /* 27 * n_nodes populations of the storage type */
+    int32_t storage;       /* vfx_storage */
+    float fp16_scale;      /* VFX_FP16_SCALED only: per-lattice scale (> 0) */
+    uint64_t n_nodes;      /* nodes per plane */
+
+    /* addressing: direct = 1 -> structured box dims[0]*dims[1]*dims[2], x fastest,
+     * node k = x + dims[0]*(y + dims[1]*z), neighbours implicit with periodic index
+     * wrap on every axis (walls are solid nodes); solid_mask bit k set = node k does
+     * not collide (NULL: every node collides).
+     * direct = 0 -> VirtualFluids sparse layout: neighbor_x/y/z[k] is the node in +x/+y/+z
+     * (src/gpu/GridGenerator/grid/GridImp.cpp:1031-1080), fluid_nodes lists the
+     * nodes that collide (src/gpu/core/Utilities/RunCollision.cuh:79-82). */
+    int32_t direct;
+    uint32_t dims[3];
+    const uint32_t* solid_mask;
+    const uint32_t* neighbor_x;
+    const uint32_t* neighbor_y;
+    const uint32_t* neighbor_z;
+    const uint32_t* fluid_nodes;
+    uint32_t n_fluid;
+
+    /* collision parameters (vf::lbm::CollisionParameter, src/lbm/collision/CollisionParameter.h:42-50) */
+    double omega;
+    double quadric_limiter[3]; /* {P, M, D}; VirtualFluids' default is 0.01 */
+    double force[3];           /* added to the velocity inside the collision (CollisionParameter::forceX..Z) */
+
+    /* VFX_TEMPLATE_WRITE_MACRO output, n_nodes each, double for VFX_FP64 else float */
+    void* rho;
+    void* vx;
+    void* vy;
+    void* vz;
+} vfx_lattice;
+
+/* boundary-condition kinds (src/gpu/core/BoundaryConditions/BoundaryConditionFactory.h:60-132) */
+typedef enum {
+    VFX_BC_VELOCITY_BOUNCE_BACK = 0, /* VelocityBounceBack_Device, Velocity/VelocityBounceBack.cu:44 (u = 0: NoSlipBounceBack) */
+    VFX_BC_NOSLIP_INTERPOLATED = 1,  /* NoSlipInterpolatedCompressible_Device, NoSlip/NoSlipInterpolatedCompressible.cu:44 */
+} vfx_bc_kind;
+
+/* QforBoundaryConditions (src/gpu/core/Calculation/Calculation.h:105-117) */
+typedef struct vfx_bc {
+    int32_t kind;              /* vfx_bc_kind */
+    uint32_t n;                /* numberOfBCnodes */
+    const uint32_t* k;         /* node index of each boundary node */
+    const float* q27;          /* 27*n subgrid distances, SoA; link d of node i is a wall link iff 0 <= q27[d*n+i] <= 1 */
+    const double* vx;          /* wall velocity per boundary node (NULL = 0) */
+    const double* vy;
+    const double* vz;
+    int32_t cpu_float_velocity; /* 1: wall term rounded as VirtualFluids' CPU BoundaryConditions does (float storage) */
+} vfx_bc;
+
+/* ---------------------------------------------------------------- kernels */
+
+/* Kernel::run / runOnIndices of "K17CompressibleNavierStokes" and "BGKCompSP27":
+ * one collide-and-stream pass over every colliding node
+ * (src/gpu/core/Kernel/Compressible/NavierStokes/K17/K17CompressibleNavierStokes.cu:85-145,
+ *  src/gpu/core/Kernel/Compressible/NavierStokes/B92/B92CompressibleNavierStokes.cu:43-57). */
+vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t is_even_timestep,
+                              int32_t collision_template, vfx_stream stream);
+vfx_status vfx_k17_step(const vfx_lattice* lat, int32_t is_even_timestep, int32_t collision_template, vfx_stream stream);
+vfx_status vfx_bgk_step(const vfx_lattice* lat, int32_t is_even_timestep, vfx_stream stream);
+
+/* Post-collision boundary condition on a BC node list, run after the
+ * collision of parity is_even_timestep and before the parity flip
+ * (src/gpu/core/Calculation/UpdateGrid27.cpp:227-262). */
+vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_even_timestep, vfx_stream stream);
+
+/* InitNavierStokesCompressible_Device (src/gpu/core/PreProcessor/PreProcessorStrategy/
+ * InitNavierStokesCompressible/InitNavierStokesCompressible_Device.cu:44-213):
+ * shifted compressible equilibrium of (drho, ux, uy, uz), written for parity
+ * is_even_timestep.  Fields are device arrays of n_nodes doubles; solid nodes
+ * (direct: solid_mask; sparse: all nodes are written) are skipped. */
+vfx_status vfx_init_equilibrium(const vfx_lattice* lat, int32_t is_even_timestep, const double* drho,
+                                const double* ux, const double* uy, const double* uz, vfx_stream stream);
+
+/* Copy the pre-collision populations of every node (parity is_even_timestep)
+ * out to / in from a node-major double array out[node*27 + d] (device). */
+vfx_status vfx_read_populations(const vfx_lattice* lat, int32_t is_even_timestep, double* out, vfx_stream stream);
+vfx_status vfx_write_populations(const vfx_lattice* lat, int32_t is_even_timestep, const double* in, vfx_stream stream);
+
+/* --------------------------------------------------------- diagnostics */
+/* Macroscopic fields of the pre-collision state (calculateMacroscopicQuantitiesCompressible_device,
+ * src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237), doubles, n_nodes each. */
+vfx_status vfx_macroscopic(const vfx_lattice* lat, int32_t is_even_timestep, double* drho, double* vx,
+                           double* vy, double* vz, vfx_stream stream);
+
+/* -------------------------------------------------------------- misc */
+const char* vfx_last_error(void);
+int32_t vfx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VFX_H */

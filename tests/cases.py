@@ -1,0 +1,103 @@
+"""Seeded lattice set-ups shared by the tests (geometry and initial fields only)."""
+from __future__ import annotations
+
+import numpy as np
+
+CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1])
+CY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, -1])
+CZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, 1, 1, -1, -1, -1, -1])
+INV = np.array([0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17, 26, 25, 24, 23, 22, 21, 20, 19])
+W = np.array([8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8)
+
+
+def random_flow(dims, seed, u_amp=0.03, rho_amp=1e-3):
+    """Smooth random velocity/density field (sum of a few periodic modes)."""
+    nx, ny, nz = dims
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    fields = []
+    for amp in (rho_amp, u_amp, u_amp, u_amp):
+        f = np.zeros((nz, ny, nx))
+        for _ in range(4):
+            kx, ky, kz = rng.integers(0, 3, size=3)
+            ph = rng.uniform(0, 2 * np.pi, size=3)
+            f += rng.uniform(-1, 1) * np.sin(2 * np.pi * kx * x / nx + ph[0]) * np.cos(
+                2 * np.pi * ky * y / ny + ph[1]) * np.cos(2 * np.pi * kz * z / nz + ph[2])
+        f += 0.1 * rng.uniform(-1, 1, size=f.shape)
+        fields.append(amp * f / max(1e-30, np.abs(f).max()))
+    return [f.ravel() for f in fields]
+
+
+def taylor_green(n, u0=0.04):
+    """Taylor-Green vortex of apps/gpu/TaylorGreenVortex/TaylorGreenVortex.cpp:138-148
+    on an n^3 periodic box (L = n / 2pi)."""
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    k = 2 * np.pi / n
+    X, Y, Z = k * (x + 0.5), k * (y + 0.5), k * (z + 0.5)
+    ux = u0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+    uy = -u0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+    uz = np.zeros_like(ux)
+    drho = 3.0 * (u0 ** 2 / 16.0) * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+    return [a.ravel() for a in (drho, ux, uy, uz)]
+
+
+def cavity(n, lid=(0.05, 0.0, 0.0)):
+    """Lid-driven cavity of n^3 fluid nodes inside one layer of solid nodes
+    ((n+2)^3 box, x fastest).  Every fluid->solid link is a halfway bounce-back
+    link (q = 0.5); links into the z = n+1 plane move with `lid`.
+
+    Returns (solid[(n+2)^3] bool, [(nodes, q27, velocity) for lid, walls])."""
+    m = n + 2
+    solid = np.zeros((m, m, m), dtype=bool)
+    solid[0, :, :] = solid[-1, :, :] = True
+    solid[:, 0, :] = solid[:, -1, :] = True
+    solid[:, :, 0] = solid[:, :, -1] = True
+    groups = {"lid": ([], []), "wall": ([], [])}
+    fz, fy, fx = np.nonzero(~solid)
+    for z, y, x in zip(fz, fy, fx):
+        ql = np.full(27, -1.0, np.float32)
+        qw = np.full(27, -1.0, np.float32)
+        for d in range(1, 27):
+            tz, ty, tx = z + CZ[d], y + CY[d], x + CX[d]
+            if solid[tz, ty, tx]:
+                (ql if tz == m - 1 else qw)[d] = 0.5
+        k = x + m * (y + m * z)
+        if (ql >= 0).any():
+            groups["lid"][0].append(k)
+            groups["lid"][1].append(ql)
+        if (qw >= 0).any():
+            groups["wall"][0].append(k)
+            groups["wall"][1].append(qw)
+    out = []
+    for name, (nodes, qs) in groups.items():
+        out.append((np.array(nodes, dtype=np.int64), np.stack(qs, axis=1),
+                    lid if name == "lid" else (0.0, 0.0, 0.0)))
+    return solid.ravel(), out
+
+
+def sparse_layout(dims, solid=None, seed=0):
+    """VirtualFluids sparse arrays for a structured periodic box: node k of the
+    box gets sparse index perm[k] + 1, index 0 is the dummy solid node whose
+    neighbours are itself (src/gpu/GridGenerator/grid/GridImp.cpp:2150-2190).
+
+    Returns (nbx, nby, nbz, fluid_nodes, sparse_index_of_box_node)."""
+    nx, ny, nz = dims
+    N = nx * ny * nz
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(N)
+    sidx = perm + 1
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x, y, z = x.ravel(), y.ravel(), z.ravel()
+
+    def k_of(xx, yy, zz):
+        return (xx % nx) + nx * ((yy % ny) + ny * (zz % nz))
+
+    nbx = np.zeros(N + 1, np.uint32)
+    nby = np.zeros(N + 1, np.uint32)
+    nbz = np.zeros(N + 1, np.uint32)
+    nbx[sidx] = sidx[k_of(x + 1, y, z)]
+    nby[sidx] = sidx[k_of(x, y + 1, z)]
+    nbz[sidx] = sidx[k_of(x, y, z + 1)]
+    fluid = np.arange(N) if solid is None else np.nonzero(~np.asarray(solid).ravel())[0]
+    fluid_nodes = np.sort(sidx[fluid]).astype(np.uint32)
+    return nbx, nby, nbz, fluid_nodes, sidx

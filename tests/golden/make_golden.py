@@ -1,0 +1,85 @@
+"""Writes the golden vectors of the K17 hot path from the VirtualFluids reference
+CPU closure (oracle/_ref/libvfref.so, built by `make -C oracle ref` from the
+reference sources).  Run once in a container that has the reference tree:
+
+    python tests/golden/make_golden.py
+
+Fixtures (inputs and reference outputs only; parameters in golden.json):
+  g1_k17_nodes.npz  single-node vf::lbm::runK17CompressibleNavierStokes<None>
+                    (K17CompressibleNavierStokes.h:87-591) on random shifted states
+  g2_periodic.npz   8^3 periodic block, K17CompressibleNavierStokes::calculate +
+                    EsoSplit (K17CompressibleNavierStokes.cpp:77-162), 10 and 100 steps
+  g3_cavity.npz     10^3 lid-driven cavity, VelocityBounceBack strategy
+                    (VelocityBounceBack.cpp:59-79) on every wall link, 50 steps
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from tests import cases  # noqa: E402
+
+W = cases.W
+
+
+def g1(seed=20240601, n=384):
+    rng = np.random.default_rng(seed)
+    omegas = np.array([0.9, 1.2, 1.6778523, 1.9417, 1.998])
+    f_in = np.zeros((n, 27))
+    om = np.zeros(n)
+    force = np.zeros((n, 3))
+    lim = np.zeros((n, 3))
+    for i in range(n):
+        scale = [1e-4, 1e-2, 1e-1][i % 3]
+        f_in[i] = rng.uniform(-1, 1, 27) * scale * W
+        om[i] = omegas[i % len(omegas)]
+        lim[i] = (0.01, 0.01, 0.01) if i % 4 else rng.uniform(1e-3, 1.0, 3)
+        if i % 7 == 0:
+            force[i] = rng.uniform(-1e-5, 1e-5, 3)
+    f_out = np.zeros_like(f_in)
+    macro = np.zeros((n, 4))
+    for i in range(n):
+        f_out[i], macro[i] = oracle.k17_collide(f_in[i], om[i], lim=lim[i], force=force[i], impl="ref")
+    np.savez(os.path.join(HERE, "g1_k17_nodes.npz"), f_in=f_in, omega=om, limiter=lim, force=force,
+             f_out=f_out, macro=macro)
+    return {"seed": seed, "n": n, "omegas": omegas.tolist(), "note": "limiter 0.01 except every 4th node"}
+
+
+def g2(seed=8, n=8, omega=1.6):
+    dims = (n, n, n)
+    fields = cases.random_flow(dims, seed=seed)
+    F0 = oracle.init_field(*fields)
+    F10 = oracle.ref_run_periodic(F0, dims, 10, omega)
+    F100 = oracle.ref_run_periodic(F0, dims, 100, omega)
+    np.savez(os.path.join(HERE, "g2_periodic.npz"), F0=F0, F10=F10, F100=F100)
+    return {"seed": seed, "dims": list(dims), "omega": omega, "steps": [10, 100]}
+
+
+def g3(n=10, omega=1.6778523, lid=(0.05, 0.0, 0.0), steps=50):
+    F0 = np.zeros((n ** 3, 27))
+    F = oracle.ref_run_cavity(F0, (n, n, n), steps, omega, lid=lid)
+    np.savez(os.path.join(HERE, "g3_cavity.npz"), F=F)
+    return {"n_fluid": n, "omega": omega, "lid": list(lid), "steps": steps,
+            "layout": "fluid nodes only, x fastest; walls are the surrounding layer"}
+
+
+def main():
+    if not oracle.ref_available():
+        oracle.build(ref=True)
+    meta = {"generator": "tests/golden/make_golden.py", "reference": "oracle/_ref/libvfref.so",
+            "g1": g1(), "g2": g2(), "g3": g3()}
+    with open(os.path.join(HERE, "golden.json"), "w") as fh:
+        json.dump(meta, fh, indent=1)
+    print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
+
+
+if __name__ == "__main__":
+    main()

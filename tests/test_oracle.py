@@ -1,0 +1,268 @@
+"""CPU tests that pin the oracle (oracle/oracle.cpp) before it is trusted as
+the checker of the GPU stepper.
+
+Pins, each chosen so that a plausible slip (dropped term, wrong sign, wrong
+index, transposed operand) fails at least one of them:
+  * known answers of the reference's own unit tests
+    (tests/unit-tests/lbm/ChimeraTests.cpp:46-165,
+     tests/unit-tests/lbm/MacroscopicQuantitiesTests.cpp:61-134);
+  * golden vectors written by the compiled reference CPU closure
+    (tests/golden/make_golden.py) -- bit exact;
+  * the compiled reference itself where it is built here (oracle/_ref);
+  * invariants the mathematics fixes: exact mass and momentum conservation,
+    the rest state as a fixed point, axis-permutation equivariance, and for the
+    compressible BGK (no CPU twin in the reference) an independent textbook
+    formulation on unshifted populations.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from tests import cases
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+W = cases.W
+
+
+def rnd_state(rng, scale=1e-2):
+    return rng.uniform(-1, 1, 27) * scale * W
+
+
+# ----------------------------------------------------- reference unit tests --
+def test_chimera_known_answers():
+    # ChimeraTests.cpp: forward with K (K = 1): (1,1,1) -> (3,-4,6), backward undoes it
+    assert np.array_equal(oracle.chimera(0, [1, 1, 1], 1.0, 1.0, 1.0, 1.0), [3.0, -4.0, 6.0])
+    assert np.array_equal(oracle.chimera(1, [3, -4, 6], 1.0, 1.0, 1.0, 1.0), [1.0, 1.0, 1.0])
+    # forward without K: (1,1,1) -> (3,-3,5), backward undoes it
+    assert np.array_equal(oracle.chimera(2, [1, 1, 1], 1.0, 1.0), [3.0, -3.0, 5.0])
+    assert np.array_equal(oracle.chimera(3, [3, -3, 5], 1.0, 1.0), [1.0, 1.0, 1.0])
+
+
+@pytest.mark.parametrize("kind,kinv,k", [(0, 36.0, 1 / 36), (0, 9 / 4, 4 / 9), (2, 1.0, 1.0)])
+def test_chimera_round_trip(kind, kinv, k):
+    rng = np.random.default_rng(kind)
+    for _ in range(50):
+        abc = rng.normal(size=3) * 1e-2
+        vv = rng.uniform(-0.1, 0.1)
+        fwd = oracle.chimera(kind, abc, vv, vv * vv, kinv, k)
+        back = oracle.chimera(kind + 1, fwd, vv, vv * vv, kinv, k)
+        np.testing.assert_allclose(back, abc, rtol=0, atol=1e-15)
+
+
+def test_macroscopic_known_answers():
+    f = np.ones(27)
+    r, vx, vy, vz = oracle.macroscopic(f)
+    assert r == 27.0 and vx == vy == vz == 0.0
+    f[1] = 2.0  # dP00
+    r, vx, vy, vz = oracle.macroscopic(f)
+    assert r == 28.0
+    assert vx == 1.0 / 29.0 and vy == 0.0 and vz == 0.0
+    for d, axis in ((3, 2), (5, 3)):  # d0P0 -> vy, d00P -> vz
+        g = np.ones(27)
+        g[d] = 2.0
+        out = oracle.macroscopic(g)
+        assert out[axis] == 1.0 / 29.0
+
+
+# ------------------------------------------------------------- golden vectors --
+def test_k17_single_node_golden():
+    g = np.load(os.path.join(GOLD, "g1_k17_nodes.npz"))
+    for i in range(g["f_in"].shape[0]):
+        f, mv = oracle.k17_collide(g["f_in"][i], g["omega"][i], lim=g["limiter"][i], force=g["force"][i])
+        assert np.array_equal(f, g["f_out"][i]), i
+        assert np.array_equal(mv, g["macro"][i]), i
+
+
+def test_periodic_block_golden():
+    g = np.load(os.path.join(GOLD, "g2_periodic.npz"))
+    dims = (8, 8, 8)
+    assert np.array_equal(oracle.run(g["F0"], dims, 10, 1.6), g["F10"])
+    assert np.array_equal(oracle.run(g["F0"], dims, 100, 1.6), g["F100"])
+
+
+def _oracle_cavity(n, steps, omega, lid=(0.05, 0.0, 0.0), cpu_float=True):
+    solid, groups = cases.cavity(n, lid)
+    m = n + 2
+    bcs = [oracle.BCGroup(nodes, q, u[0], u[1], u[2], cpu_float_velocity=cpu_float) for nodes, q, u in groups]
+    F = oracle.run(np.zeros((m ** 3, 27)), (m, m, m), steps, omega, solid=solid, bcs=bcs)
+    return F.reshape(m, m, m, 27)[1:-1, 1:-1, 1:-1].reshape(-1, 27)
+
+
+def test_cavity_golden():
+    g = np.load(os.path.join(GOLD, "g3_cavity.npz"))
+    assert np.array_equal(_oracle_cavity(10, 50, 1.6778523), g["F"])
+
+
+def test_cavity_needs_cpu_float_rounding():
+    """The GPU formula (wall term in FP64) differs from the CPU strategy's
+    float-rounded term: the golden cavity is matched only with the rounding."""
+    g = np.load(os.path.join(GOLD, "g3_cavity.npz"))
+    F = _oracle_cavity(10, 50, 1.6778523, cpu_float=False)
+    assert not np.array_equal(F, g["F"])
+    assert np.abs(F - g["F"]).max() < 1e-8
+
+
+# --------------------------------------------------- live compiled reference --
+needs_ref = pytest.mark.skipif(not oracle.ref_available(), reason="reference closure not built here")
+
+
+@needs_ref
+def test_k17_matches_reference_random():
+    rng = np.random.default_rng(99)
+    for i in range(300):
+        f = rnd_state(rng, scale=[1e-3, 3e-2, 2e-1][i % 3])
+        om = rng.uniform(0.6, 1.999)
+        a = oracle.k17_collide(f, om)
+        b = oracle.k17_collide(f, om, impl="ref")
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+@needs_ref
+@pytest.mark.parametrize("dims", [(6, 5, 4), (9, 7, 8)])
+def test_periodic_matches_reference(dims):
+    F0 = oracle.init_field(*cases.random_flow(dims, seed=4))
+    assert np.array_equal(oracle.run(F0, dims, 7, 1.85), oracle.ref_run_periodic(F0, dims, 7, 1.85))
+
+
+@needs_ref
+def test_cavity_matches_reference():
+    n = 7
+    F = oracle.ref_run_cavity(np.zeros((n ** 3, 27)), (n, n, n), 30, 1.62, lid=(0.04, 0.01, 0.0))
+    assert np.array_equal(_oracle_cavity(n, 30, 1.62, lid=(0.04, 0.01, 0.0)), F)
+
+
+# ------------------------------------------------------------------ invariants --
+def test_k17_conserves_mass_and_momentum():
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        f = rnd_state(rng, 5e-2)
+        om = rng.uniform(0.5, 1.99)
+        g, mv = oracle.k17_collide(f, om)
+        m0 = oracle.macroscopic(f)
+        m1 = oracle.macroscopic(g)
+        assert abs(m1[0] - m0[0]) <= 1e-15
+        np.testing.assert_allclose(m1[1:], m0[1:], rtol=0, atol=2e-16)
+        assert mv[0] == m0[0]
+
+
+def test_k17_rest_state_is_fixed_point():
+    f, mv = oracle.k17_collide(np.zeros(27), 1.7)
+    assert np.abs(f).max() == 0.0
+    assert np.abs(mv).max() == 0.0
+
+
+def test_k17_equilibrium_relaxes_little():
+    """A shifted equilibrium at small velocity is (nearly) invariant: only the
+    O(u^3) deviation of the second-order equilibrium is relaxed."""
+    feq = oracle.equilibrium(1e-3, 0.01, -0.005, 0.002)
+    f, _ = oracle.k17_collide(feq, 1.5)
+    assert np.abs(f - feq).max() < 1e-6
+
+
+def _perm_xy(f):
+    """Populations under the axis swap x <-> y."""
+    out = np.zeros(27)
+    for d in range(27):
+        e = [j for j in range(27) if cases.CX[j] == cases.CY[d] and cases.CY[j] == cases.CX[d]
+             and cases.CZ[j] == cases.CZ[d]][0]
+        out[e] = f[d]
+    return out
+
+
+def _perm_xz(f):
+    out = np.zeros(27)
+    for d in range(27):
+        e = [j for j in range(27) if cases.CX[j] == cases.CZ[d] and cases.CZ[j] == cases.CX[d]
+             and cases.CY[j] == cases.CY[d]][0]
+        out[e] = f[d]
+    return out
+
+
+@pytest.mark.parametrize("perm", [_perm_xy, _perm_xz])
+def test_k17_axis_permutation_equivariance(perm):
+    rng = np.random.default_rng(17)
+    for _ in range(100):
+        f = rnd_state(rng, 5e-2)
+        om = rng.uniform(0.7, 1.95)
+        a, _ = oracle.k17_collide(perm(f), om)
+        b, _ = oracle.k17_collide(f, om)
+        np.testing.assert_allclose(a, perm(b), rtol=0, atol=1e-14)
+
+
+def test_bgk_matches_textbook_unshifted():
+    """Compressible BGK on unshifted populations F = f + w with
+    feq = w rho (1 + 3 c.u + 9/2 (c.u)^2 - 3/2 u^2): an independent formulation."""
+    rng = np.random.default_rng(3)
+    c = np.stack([cases.CX, cases.CY, cases.CZ], axis=1).astype(float)
+    for _ in range(100):
+        f = rnd_state(rng, 5e-2)
+        om = rng.uniform(0.5, 1.99)
+        Ff = f + W
+        rho = Ff.sum()
+        u = (c * Ff[:, None]).sum(0) / rho
+        cu = c @ u
+        feq = W * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * u @ u)
+        post = Ff - om * (Ff - feq) - W
+        got, mv = oracle.bgk_collide(f, om)
+        np.testing.assert_allclose(got, post, rtol=0, atol=5e-16)
+        np.testing.assert_allclose(mv[1:], u, rtol=0, atol=1e-16)
+
+
+def test_bgk_fixed_point_and_conservation():
+    feq = oracle.equilibrium(2e-3, 0.02, -0.01, 0.005)
+    f, _ = oracle.bgk_collide(feq, 1.3)
+    np.testing.assert_allclose(f, feq, rtol=0, atol=1e-17)
+    rng = np.random.default_rng(8)
+    g = rnd_state(rng, 3e-2)
+    h, _ = oracle.bgk_collide(g, 1.1)
+    np.testing.assert_allclose(oracle.macroscopic(h)[:1], oracle.macroscopic(g)[:1], atol=1e-16)
+    np.testing.assert_allclose(oracle.macroscopic(h)[1:], oracle.macroscopic(g)[1:], atol=1e-16)
+
+
+def test_equilibrium_matches_textbook():
+    c = np.stack([cases.CX, cases.CY, cases.CZ], axis=1).astype(float)
+    drho, u = 3e-3, np.array([0.03, -0.02, 0.01])
+    rho = 1 + drho
+    cu = c @ u
+    feq = W * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * u @ u) - W
+    np.testing.assert_allclose(oracle.equilibrium(drho, *u), feq, rtol=0, atol=5e-17)
+
+
+def test_periodic_run_conserves_mass_exactly_enough():
+    dims = (10, 9, 8)
+    F0 = oracle.init_field(*cases.random_flow(dims, seed=2))
+    F = oracle.run(F0, dims, 40, 1.9)
+    m0 = sum(oracle.macroscopic(f)[0] for f in F0)
+    m1 = sum(oracle.macroscopic(f)[0] for f in F)
+    assert abs(m1 - m0) < 1e-12
+
+
+def test_taylor_green_decay_follows_viscosity():
+    """Kinetic-energy decay rate of the Taylor-Green vortex equals 6 nu k^2 with
+    nu = (1/omega - 1/2)/3 (closed form of
+    AnalyticalResultsTaylorGreenVortexUx.cpp:45-60 in 3D); measured between
+    steps 20 and 100, after the start-up transient of the equilibrium start."""
+    n, omega = 24, 1.8
+    F = oracle.init_field(*cases.taylor_green(n, u0=0.01))
+
+    def energy(Fs):
+        m = np.array([oracle.macroscopic(f) for f in Fs])
+        return (0.5 * (1 + m[:, 0]) * (m[:, 1:] ** 2).sum(1)).sum()
+
+    F = oracle.run(F, (n, n, n), 20, omega)
+    e1 = energy(F)
+    F = oracle.run(F, (n, n, n), 80, omega)
+    e2 = energy(F)
+    rate = np.log(e1 / e2) / 80
+    nu = (1 / omega - 0.5) / 3
+    k = 2 * np.pi / n
+    assert abs(rate / (6 * nu * k * k) - 1) < 0.02, (rate, 6 * nu * k * k)
+
+
+def test_uncovered_wall_link_is_rejected():
+    solid = np.zeros((6, 6, 6), dtype=bool)
+    solid[2, 2, 2] = True
+    with pytest.raises(RuntimeError):
+        oracle.run(np.zeros((216, 27)), (6, 6, 6), 1, 1.5, solid=solid)

@@ -1,0 +1,13 @@
+"""virtualfluids_amd -- MI355X-native D3Q27 lattice-Boltzmann stepper.
+
+The product is ``libvfx.so`` (hand-written HIP for gfx950 behind the C ABI in
+``include/vfx.h``).  This package is its Python host binding: ``Lattice``,
+``BoundaryCondition`` and ``Stepper`` hold device arrays (PyTorch tensors) and
+call the library; nothing here computes populations on the CPU.
+"""
+from ._lib import VfxError, check, lib  # noqa: F401
+from .lattice import (BYTES, CX, CY, CZ, INV, W, BoundaryCondition, Lattice, Stepper,  # noqa: F401
+                      pack_mask)
+
+__all__ = ["Lattice", "BoundaryCondition", "Stepper", "VfxError", "lib", "check", "pack_mask",
+           "CX", "CY", "CZ", "INV", "W", "BYTES"]

@@ -1,0 +1,83 @@
+"""ctypes binding of libvfx.so (the C ABI declared in include/vfx.h).
+
+The binding fails loudly when the HIP library is missing: there is no CPU
+fallback anywhere in the product.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvfx.so")
+
+VFX_OK, VFX_EINVAL, VFX_EHIP, VFX_EUNSUPPORTED = 0, 1, 2, 3
+VFX_FP64, VFX_FP32, VFX_FP16_SCALED = 0, 1, 2
+VFX_K17, VFX_BGK = 0, 1
+VFX_TEMPLATE_DEFAULT, VFX_TEMPLATE_WRITE_MACRO = 0, 1
+VFX_BC_VELOCITY_BOUNCE_BACK, VFX_BC_NOSLIP_INTERPOLATED = 0, 1
+
+_u32p = C.POINTER(C.c_uint32)
+_dp = C.POINTER(C.c_double)
+
+
+class VfxLattice(C.Structure):
+    _fields_ = [
+        ("f", C.c_void_p), ("storage", C.c_int32), ("fp16_scale", C.c_float), ("n_nodes", C.c_uint64),
+        ("direct", C.c_int32), ("dims", C.c_uint32 * 3), ("solid_mask", C.c_void_p),
+        ("neighbor_x", C.c_void_p), ("neighbor_y", C.c_void_p), ("neighbor_z", C.c_void_p),
+        ("fluid_nodes", C.c_void_p), ("n_fluid", C.c_uint32),
+        ("omega", C.c_double), ("quadric_limiter", C.c_double * 3), ("force", C.c_double * 3),
+        ("rho", C.c_void_p), ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
+    ]
+
+
+class VfxBC(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("n", C.c_uint32), ("k", C.c_void_p), ("q27", C.c_void_p),
+        ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p), ("cpu_float_velocity", C.c_int32),
+    ]
+
+
+class VfxError(RuntimeError):
+    pass
+
+
+_lib = None
+
+_SIGNATURES = {
+    "vfx_collide_stream": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_int32, C.c_void_p],
+    "vfx_k17_step": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_void_p],
+    "vfx_bgk_step": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p],
+    "vfx_bc_apply": [C.POINTER(VfxLattice), C.POINTER(VfxBC), C.c_int32, C.c_void_p],
+    "vfx_init_equilibrium": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                             C.c_void_p],
+    "vfx_read_populations": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
+    "vfx_write_populations": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
+    "vfx_macroscopic": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                        C.c_void_p],
+}
+
+
+def lib():
+    """Load libvfx.so; raises if it has not been built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VfxError(f"{LIB_PATH} is missing: build it with `python -m virtualfluids_amd.build` "
+                           "(the MI355X stepper has no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, argt in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = argt
+            fn.restype = C.c_int32
+        L.vfx_last_error.restype = C.c_char_p
+        L.vfx_version.restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != VFX_OK:
+        msg = lib().vfx_last_error().decode(errors="replace")
+        raise VfxError(f"{what or 'vfx call'} failed (status {status}): {msg}")

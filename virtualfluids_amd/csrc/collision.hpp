@@ -1,0 +1,399 @@
+// collision.hpp -- node-local collision operators for the gfx950 stepper.
+//
+// * k17<T>  : cumulant K17 (Geier et al. 2017), the operator VirtualFluids
+//             implements in src/lbm/collision/K17CompressibleNavierStokes.h:87-591.
+// * bgk<T>  : compressible BGK, VirtualFluids' "BGKCompSP27"
+//             (src/gpu/core/Kernel/Compressible/NavierStokes/B92/
+//              B92CompressibleNavierStokes_Device.cu:189-234).
+//
+// Both work on shifted ("well-conditioned") populations f_d - w_d held in a
+// 27-register array indexed by velocity cube position (see cube()).  Every
+// arithmetic expression keeps VirtualFluids' evaluation order, so with
+// T = double and -ffp-contract=off the result is the reference CPU path's bit
+// for bit.  For T = float the same sequence runs in FP32; the relaxation rates
+// that depend only on omega are computed once on the host (Rates<T>).
+#pragma once
+
+#include "d3q27.hpp"
+
+namespace vfx {
+
+// Register slot of the population with velocity (i-1, j-1, k-1), i,j,k in {0,1,2}.
+// Before the forward transform slot (i,j,k) holds f of that direction; after it,
+// slot (a,b,c) holds the central moment of order (a,b,c).
+__host__ __device__ constexpr int cube(int i, int j, int k) { return 9 * i + 3 * j + k; }
+
+// Direction number (VirtualFluids order) stored at cube slot s.
+__host__ __device__ constexpr int dir_at(int s)
+{
+    constexpr int t[27] = {dMMM, dMM0, dMMP, dM0M, dM00, dM0P, dMPM, dMP0, dMPP,
+                           d0MM, d0M0, d0MP, d00M, d000, d00P, d0PM, d0P0, d0PP,
+                           dPMM, dPM0, dPMP, dP0M, dP00, dP0P, dPPM, dPP0, dPPP};
+    return t[s];
+}
+// Cube slot of direction d (inverse of dir_at).
+__host__ __device__ constexpr int slot_of(int d) { return cube(cx(d) + 1, cy(d) + 1, cz(d) + 1); }
+
+// Relaxation rates that depend on omega only
+// (K17CompressibleNavierStokes.h:248-273), computed once per launch.
+template <typename T> struct Rates {
+    T omega, OxyyPxzz, OxyyMxzz, Oxyz, factorA, factorB;
+    T limP, limM, limD;
+    T fx, fy, fz; // half force added to the velocity (:164-166)
+};
+
+template <typename T> inline Rates<T> make_rates(double omega, const double lim[3], const double force[3])
+{
+    // evaluated in T with the reference's expression order, so T=double matches it exactly
+    const T w = (T)omega;
+    Rates<T> r;
+    r.omega = w;
+    r.OxyyPxzz = T(8) * (-T(2) + w) * (T(1) + T(2) * w) / (-T(8) - T(14) * w + T(7) * w * w);
+    r.OxyyMxzz = T(8) * (-T(2) + w) * (-T(7) + T(4) * w) / (T(56) - T(50) * w + T(9) * w * w);
+    r.Oxyz = T(24) * (-T(2) + w) * (-T(2) - T(7) * w + T(3) * w * w) /
+             (T(48) + T(152) * w - T(130) * w * w + T(29) * w * w * w);
+    r.factorA = (T(4) + T(2) * w - T(3) * w * w) / (T(2) - T(7) * w + T(5) * w * w);
+    r.factorB = (T(4) + T(28) * w - T(14) * w * w) / (T(6) - T(21) * w + T(15) * w * w);
+    r.limP = (T)lim[0];
+    r.limM = (T)lim[1];
+    r.limD = (T)lim[2];
+    r.fx = (T)force[0];
+    r.fy = (T)force[1];
+    r.fz = (T)force[2];
+    return r;
+}
+
+// ------------------------------------------------------------ chimera ----
+// Three-point moment transforms along one axis (ChimeraTransformation.h:58-121).
+template <typename T> __device__ __forceinline__ void fwd_k(T& a, T& b, T& c, T vv, T v2, T kinv, T k)
+{
+    const T s = a + c;
+    const T d = c - a;
+    T m0 = s + b;
+    a = m0;
+    m0 *= kinv;
+    m0 += T(1);
+    b = (d * kinv - m0 * vv) * k;
+    c = ((s - T(2) * d * vv) * kinv + v2 * m0) * k;
+}
+template <typename T> __device__ __forceinline__ void bwd_k(T& a, T& b, T& c, T vv, T v2, T kinv, T k)
+{
+    const T h = T(0.5);
+    const T r0 = (((c - b) * h + b * vv) * kinv + (a * kinv + T(1)) * (v2 - vv) * h) * k;
+    const T r1 = (((a - c) - T(2) * b * vv) * kinv + (a * kinv + T(1)) * (-v2)) * k;
+    c = (((c + b) * h + b * vv) * kinv + (a * kinv + T(1)) * (v2 + vv) * h) * k;
+    a = r0;
+    b = r1;
+}
+template <typename T> __device__ __forceinline__ void fwd(T& a, T& b, T& c, T vv, T v2)
+{
+    const T s = (a + c) + b;
+    const T d = c - a;
+    c = (c + a) + (v2 * s - T(2) * vv * d);
+    b = d - vv * s;
+    a = s;
+}
+template <typename T> __device__ __forceinline__ void bwd(T& a, T& b, T& c, T vv, T v2)
+{
+    const T h = T(0.5);
+    const T ra = (c + a * (v2 - vv)) * h + b * (vv - h);
+    const T rb = ((a - c) - a * v2) - T(2) * b * vv;
+    c = (c + a * (v2 + vv)) * h + b * (vv + h);
+    b = rb;
+    a = ra;
+}
+
+// chimera constants of one row; the two axes not being transformed contribute
+// M/P: 1/6, 0: 2/3 before their own sweep and order 0: 1, order 2: 1/3 after it
+template <typename T> struct KPair { T kinv, k; };
+
+#define VFX_M(a, b, c) m[cube(a, b, c)]
+
+// ------------------------------------------------------------------ K17 ----
+// m[27]: populations in cube order on entry, post-collision populations on exit.
+// Returns drho and the (force-shifted) velocity in mv[4].
+template <typename T> __device__ __forceinline__ void k17(T (&m)[27], const Rates<T>& R, T (&mv)[4])
+{
+    constexpr T c1o3 = T(1. / 3.), c2o3 = T(2. / 3.), c1o9 = T(1. / 9.), c1o27 = T(1. / 27.), h = T(0.5);
+    constexpr T c3o2 = T(3. / 2.);
+
+    // macroscopic values, pyramid summation (MacroscopicQuantities.h:55-85,117-127)
+#define F(d) m[slot_of(d)]
+    const T corners = ((F(dPPP) + F(dMMM)) + (F(dMPM) + F(dPMP))) + ((F(dMPP) + F(dPMM)) + (F(dMMP) + F(dPPM)));
+    const T edges = (((F(d0MP) + F(d0PM)) + (F(d0MM) + F(d0PP))) + ((F(dM0P) + F(dP0M)) + (F(dM0M) + F(dP0P)))) +
+                    ((F(dMP0) + F(dPM0)) + (F(dMM0) + F(dPP0)));
+    const T faces = ((F(dM00) + F(dP00)) + (F(d0M0) + F(d0P0))) + (F(d00M) + F(d00P));
+    const T drho = ((corners + edges) + F(d000)) + faces;
+    T vx = (((F(dPPP) - F(dMMM)) + (F(dPMP) - F(dMPM))) + ((F(dPMM) - F(dMPP)) + (F(dPPM) - F(dMMP)))) +
+           (((F(dP0M) - F(dM0P)) + (F(dP0P) - F(dM0M))) + ((F(dPM0) - F(dMP0)) + (F(dPP0) - F(dMM0)))) +
+           (F(dP00) - F(dM00));
+    T vy = (((F(dPPP) - F(dMMM)) + (F(dMPM) - F(dPMP))) + ((F(dMPP) - F(dPMM)) + (F(dPPM) - F(dMMP)))) +
+           (((F(d0PM) - F(d0MP)) + (F(d0PP) - F(d0MM))) + ((F(dMP0) - F(dPM0)) + (F(dPP0) - F(dMM0)))) +
+           (F(d0P0) - F(d0M0));
+    T vz = (((F(dPPP) - F(dMMM)) + (F(dPMP) - F(dMPM))) + ((F(dMPP) - F(dPMM)) + (F(dMMP) - F(dPPM)))) +
+           (((F(d0MP) - F(d0PM)) + (F(d0PP) - F(d0MM))) + ((F(dM0P) - F(dP0M)) + (F(dP0P) - F(dM0M)))) +
+           (F(d00P) - F(d00M));
+#undef F
+    const T oneOverRho = T(1) / (drho + T(1));
+    vx *= oneOverRho;
+    vy *= oneOverRho;
+    vz *= oneOverRho;
+    vx += R.fx;
+    vy += R.fy;
+    vz += R.fz;
+    const T vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+
+    // forward sweeps z, y, x (K17CompressibleNavierStokes.h:190-222)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const T kinv = (i == 1 && j == 1) ? T(9. / 4.) : (i == 1 || j == 1) ? T(9) : T(36);
+            const T k = (i == 1 && j == 1) ? T(4. / 9.) : (i == 1 || j == 1) ? T(1. / 9.) : T(1. / 36.);
+            fwd_k(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), vz, vz2, kinv, k);
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c == 1) {
+                fwd(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), vy, vy2);
+            } else {
+                const T kinv = (i == 1) ? (c == 0 ? T(3. / 2.) : T(9. / 2.)) : (c == 0 ? T(6) : T(18));
+                const T k = (i == 1) ? (c == 0 ? T(2. / 3.) : T(2. / 9.)) : (c == 0 ? T(1. / 6.) : T(1. / 18.));
+                fwd_k(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), vy, vy2, kinv, k);
+            }
+        }
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (b == 1 || c == 1) {
+                fwd(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), vx, vx2);
+            } else {
+                // (b,c)=(2,2) uses K^-1 = 3 with K = 1/9 exactly as the reference (:222)
+                const T kinv = (b == 0 && c == 0) ? T(1) : T(3);
+                const T k = (b == 0 && c == 0) ? T(1) : (b == 2 && c == 2) ? T(1. / 9.) : T(1. / 3.);
+                fwd_k(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), vx, vx2, kinv, k);
+            }
+        }
+
+    T& m000 = VFX_M(0, 0, 0); T& m100 = VFX_M(1, 0, 0); T& m010 = VFX_M(0, 1, 0); T& m001 = VFX_M(0, 0, 1);
+    T& m200 = VFX_M(2, 0, 0); T& m020 = VFX_M(0, 2, 0); T& m002 = VFX_M(0, 0, 2);
+    T& m110 = VFX_M(1, 1, 0); T& m101 = VFX_M(1, 0, 1); T& m011 = VFX_M(0, 1, 1);
+    T& m111 = VFX_M(1, 1, 1);
+    T& m210 = VFX_M(2, 1, 0); T& m012 = VFX_M(0, 1, 2); T& m201 = VFX_M(2, 0, 1);
+    T& m021 = VFX_M(0, 2, 1); T& m120 = VFX_M(1, 2, 0); T& m102 = VFX_M(1, 0, 2);
+    T& m211 = VFX_M(2, 1, 1); T& m121 = VFX_M(1, 2, 1); T& m112 = VFX_M(1, 1, 2);
+    T& m220 = VFX_M(2, 2, 0); T& m202 = VFX_M(2, 0, 2); T& m022 = VFX_M(0, 2, 2);
+    T& m122 = VFX_M(1, 2, 2); T& m212 = VFX_M(2, 1, 2); T& m221 = VFX_M(2, 2, 1);
+    T& m222 = VFX_M(2, 2, 2);
+
+    const T omega = R.omega;
+    const T OxxPyyPzz = T(1), O4 = T(1), O5 = T(1), O6 = T(1);
+
+    // cumulants of order 4..6 (:282-316)
+    T cm211 = m211 - ((m200 + c1o3) * m011 + T(2) * m110 * m101) * oneOverRho;
+    T cm121 = m121 - ((m020 + c1o3) * m101 + T(2) * m110 * m011) * oneOverRho;
+    T cm112 = m112 - ((m002 + c1o3) * m110 + T(2) * m101 * m011) * oneOverRho;
+    T cm220 = m220 - (((m200 * m020 + T(2) * m110 * m110) + c1o3 * (m200 + m020)) * oneOverRho - c1o9 * (drho * oneOverRho));
+    T cm202 = m202 - (((m200 * m002 + T(2) * m101 * m101) + c1o3 * (m200 + m002)) * oneOverRho - c1o9 * (drho * oneOverRho));
+    T cm022 = m022 - (((m002 * m020 + T(2) * m011 * m011) + c1o3 * (m002 + m020)) * oneOverRho - c1o9 * (drho * oneOverRho));
+    T cm122 = m122 - ((m002 * m120 + m020 * m102 + T(4) * m011 * m111 + T(2) * (m101 * m021 + m110 * m012)) +
+                      c1o3 * (m120 + m102)) * oneOverRho;
+    T cm212 = m212 - ((m002 * m210 + m200 * m012 + T(4) * m101 * m111 + T(2) * (m011 * m201 + m110 * m102)) +
+                      c1o3 * (m210 + m012)) * oneOverRho;
+    T cm221 = m221 - ((m200 * m021 + m020 * m201 + T(4) * m110 * m111 + T(2) * (m101 * m120 + m011 * m210)) +
+                      c1o3 * (m021 + m201)) * oneOverRho;
+    T cm222 = m222 + ((-T(4) * m111 * m111 - (m200 * m022 + m020 * m202 + m002 * m220) -
+                       T(4) * (m011 * m211 + m101 * m121 + m110 * m112) -
+                       T(2) * (m120 * m102 + m210 * m012 + m201 * m021)) * oneOverRho +
+                      (T(4) * (m101 * m101 * m020 + m011 * m011 * m200 + m110 * m110 * m002) +
+                       T(2) * (m200 * m020 * m002) + T(16) * m110 * m101 * m011) * oneOverRho * oneOverRho -
+                      c1o3 * (m022 + m202 + m220) * oneOverRho - c1o9 * (m200 + m020 + m002) * oneOverRho +
+                      (T(2) * (m101 * m101 + m011 * m011 + m110 * m110) +
+                       (m002 * m020 + m002 * m200 + m020 * m200) + c1o3 * (m002 + m020 + m200)) *
+                          oneOverRho * oneOverRho * c2o3 +
+                      c1o27 * ((drho * drho - drho) * oneOverRho * oneOverRho));
+
+    // second and third order linear combinations (:323-335)
+    T mxxPyyPzz = m200 + m020 + m002;
+    T mxxMyy = m200 - m020;
+    T mxxMzz = m200 - m002;
+    T mxxyPyzz = m210 + m012, mxxyMyzz = m210 - m012;
+    T mxxzPyyz = m201 + m021, mxxzMyyz = m201 - m021;
+    T mxyyPxzz = m120 + m102, mxyyMxzz = m120 - m102;
+
+    // velocity gradients times rho (:347-352)
+    const T Dxy = -T(3) * omega * m110;
+    const T Dxz = -T(3) * omega * m101;
+    const T Dyz = -T(3) * omega * m011;
+    const T dxux = h * (-omega) * (mxxMyy + mxxMzz) + h * OxxPyyPzz * (m000 - mxxPyyPzz);
+    const T dyuy = dxux + omega * c3o2 * mxxMyy;
+    const T dzuz = dxux + omega * c3o2 * mxxMzz;
+
+    // second order with Galilean correction (:375-387)
+    mxxPyyPzz += OxxPyyPzz * (m000 - mxxPyyPzz) - T(3) * (T(1) - h * OxxPyyPzz) * (vx2 * dxux + vy2 * dyuy + vz2 * dzuz);
+    mxxMyy += omega * (-mxxMyy) - T(3) * (T(1) + h * (-omega)) * (vx2 * dxux - vy2 * dyuy);
+    mxxMzz += omega * (-mxxMzz) - T(3) * (T(1) + h * (-omega)) * (vx2 * dxux - vz2 * dzuz);
+    m011 += omega * (-m011);
+    m101 += omega * (-m101);
+    m110 += omega * (-m110);
+
+    // limited third order (:397-410)
+    T wadj = R.Oxyz + (T(1) - R.Oxyz) * fabs(m111) / (fabs(m111) + R.limD);
+    m111 += wadj * (-m111);
+    wadj = R.OxyyPxzz + (T(1) - R.OxyyPxzz) * fabs(mxxyPyzz) / (fabs(mxxyPyzz) + R.limP);
+    mxxyPyzz += wadj * (-mxxyPyzz);
+    wadj = R.OxyyMxzz + (T(1) - R.OxyyMxzz) * fabs(mxxyMyzz) / (fabs(mxxyMyzz) + R.limM);
+    mxxyMyzz += wadj * (-mxxyMyzz);
+    wadj = R.OxyyPxzz + (T(1) - R.OxyyPxzz) * fabs(mxxzPyyz) / (fabs(mxxzPyyz) + R.limP);
+    mxxzPyyz += wadj * (-mxxzPyyz);
+    wadj = R.OxyyMxzz + (T(1) - R.OxyyMxzz) * fabs(mxxzMyyz) / (fabs(mxxzMyyz) + R.limM);
+    mxxzMyyz += wadj * (-mxxzMyyz);
+    wadj = R.OxyyPxzz + (T(1) - R.OxyyPxzz) * fabs(mxyyPxzz) / (fabs(mxyyPxzz) + R.limP);
+    mxyyPxzz += wadj * (-mxyyPxzz);
+    wadj = R.OxyyMxzz + (T(1) - R.OxyyMxzz) * fabs(mxyyMxzz) / (fabs(mxyyMxzz) + R.limM);
+    mxyyMxzz += wadj * (-mxyyMxzz);
+
+    // back from linear combinations (:424-433)
+    m200 = c1o3 * (mxxMyy + mxxMzz + mxxPyyPzz);
+    m020 = c1o3 * (-T(2) * mxxMyy + mxxMzz + mxxPyyPzz);
+    m002 = c1o3 * (mxxMyy - T(2) * mxxMzz + mxxPyyPzz);
+    m210 = (mxxyMyzz + mxxyPyzz) * h;
+    m012 = (-mxxyMyzz + mxxyPyzz) * h;
+    m201 = (mxxzMyyz + mxxzPyyz) * h;
+    m021 = (-mxxzMyyz + mxxzPyyz) * h;
+    m120 = (mxyyMxzz + mxyyPxzz) * h;
+    m102 = (-mxyyMxzz + mxyyPxzz) * h;
+
+    // fourth order to the modified equilibrium, 5th/6th to zero (:443-459)
+    const T invOmH = T(1) / omega - h;
+    cm022 = -O4 * invOmH * (dyuy + dzuz) * c2o3 * R.factorA + (T(1) - O4) * (cm022);
+    cm202 = -O4 * invOmH * (dxux + dzuz) * c2o3 * R.factorA + (T(1) - O4) * (cm202);
+    cm220 = -O4 * invOmH * (dyuy + dxux) * c2o3 * R.factorA + (T(1) - O4) * (cm220);
+    cm112 = -O4 * invOmH * Dxy * c1o3 * R.factorB + (T(1) - O4) * (cm112);
+    cm121 = -O4 * invOmH * Dxz * c1o3 * R.factorB + (T(1) - O4) * (cm121);
+    cm211 = -O4 * invOmH * Dyz * c1o3 * R.factorB + (T(1) - O4) * (cm211);
+    cm122 += O5 * (-cm122);
+    cm212 += O5 * (-cm212);
+    cm221 += O5 * (-cm221);
+    cm222 += O6 * (-cm222);
+
+    // central moments from post-collision cumulants (:469-505)
+    m211 = cm211 + c1o3 * ((T(3) * m200 + T(1)) * m011 + T(6) * m110 * m101) * oneOverRho;
+    m121 = cm121 + c1o3 * ((T(3) * m020 + T(1)) * m101 + T(6) * m110 * m011) * oneOverRho;
+    m112 = cm112 + c1o3 * ((T(3) * m002 + T(1)) * m110 + T(6) * m101 * m011) * oneOverRho;
+    m220 = cm220 + (((m200 * m020 + T(2) * m110 * m110) * T(9) + T(3) * (m200 + m020)) * oneOverRho - (drho * oneOverRho)) * c1o9;
+    m202 = cm202 + (((m200 * m002 + T(2) * m101 * m101) * T(9) + T(3) * (m200 + m002)) * oneOverRho - (drho * oneOverRho)) * c1o9;
+    m022 = cm022 + (((m002 * m020 + T(2) * m011 * m011) * T(9) + T(3) * (m002 + m020)) * oneOverRho - (drho * oneOverRho)) * c1o9;
+    m122 = cm122 + c1o3 * (T(3) * (m002 * m120 + m020 * m102 + T(4) * m011 * m111 + T(2) * (m101 * m021 + m110 * m012)) +
+                           (m120 + m102)) * oneOverRho;
+    m212 = cm212 + c1o3 * (T(3) * (m002 * m210 + m200 * m012 + T(4) * m101 * m111 + T(2) * (m011 * m201 + m110 * m102)) +
+                           (m210 + m012)) * oneOverRho;
+    m221 = cm221 + c1o3 * (T(3) * (m200 * m021 + m020 * m201 + T(4) * m110 * m111 + T(2) * (m101 * m120 + m011 * m210)) +
+                           (m021 + m201)) * oneOverRho;
+    m222 = cm222 - ((-T(4) * m111 * m111 - (m200 * m022 + m020 * m202 + m002 * m220) -
+                     T(4) * (m011 * m211 + m101 * m121 + m110 * m112) -
+                     T(2) * (m120 * m102 + m210 * m012 + m201 * m021)) * oneOverRho +
+                    (T(4) * (m101 * m101 * m020 + m011 * m011 * m200 + m110 * m110 * m002) +
+                     T(2) * (m200 * m020 * m002) + T(16) * m110 * m101 * m011) * oneOverRho * oneOverRho -
+                    c1o3 * (m022 + m202 + m220) * oneOverRho - c1o9 * (m200 + m020 + m002) * oneOverRho +
+                    (T(2) * (m101 * m101 + m011 * m011 + m110 * m110) +
+                     (m002 * m020 + m002 * m200 + m020 * m200) + c1o3 * (m002 + m020 + m200)) *
+                        oneOverRho * oneOverRho * c2o3 +
+                    c1o27 * ((drho * drho - drho) * oneOverRho * oneOverRho));
+
+    m100 = -m100;
+    m010 = -m010;
+    m001 = -m001;
+
+    // backward sweeps x, y, z (:525-557)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (b == 1 || c == 1) {
+                bwd(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), vx, vx2);
+            } else {
+                const T kinv = (b == 0 && c == 0) ? T(1) : (b == 2 && c == 2) ? T(9) : T(3);
+                const T k = (b == 0 && c == 0) ? T(1) : (b == 2 && c == 2) ? T(1. / 9.) : T(1. / 3.);
+                bwd_k(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), vx, vx2, kinv, k);
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c == 1) {
+                bwd(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), vy, vy2);
+            } else {
+                const T kinv = (i == 1) ? (c == 0 ? T(3. / 2.) : T(9. / 2.)) : (c == 0 ? T(6) : T(18));
+                const T k = (i == 1) ? (c == 0 ? T(2. / 3.) : T(2. / 9.)) : (c == 0 ? T(1. / 6.) : T(1. / 18.));
+                bwd_k(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), vy, vy2, kinv, k);
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const T kinv = (i == 1 && j == 1) ? T(9. / 4.) : (i == 1 || j == 1) ? T(9) : T(36);
+            const T k = (i == 1 && j == 1) ? T(4. / 9.) : (i == 1 || j == 1) ? T(1. / 9.) : T(1. / 36.);
+            bwd_k(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), vz, vz2, kinv, k);
+        }
+
+    mv[0] = drho;
+    mv[1] = vx;
+    mv[2] = vy;
+    mv[3] = vz;
+}
+
+// ------------------------------------------------------------------ BGK ----
+// Compressible BGK on shifted populations, B92CompressibleNavierStokes_Device.cu:189-234.
+template <typename T> __device__ __forceinline__ void bgk(T (&m)[27], T omega, T (&mv)[4])
+{
+#define F(d) m[slot_of(d)]
+    const T drho = (F(dPPP) + F(dMMM)) + (F(dMMP) + F(dPPM)) + (F(dPMP) + F(dMPM)) + (F(dMPP) + F(dPMM)) +
+                   (F(dPP0) + F(dMM0)) + (F(dMP0) + F(dPM0)) + (F(dP0P) + F(dM0M)) + (F(dP0M) + F(dM0P)) +
+                   (F(d0PP) + F(d0MM)) + (F(d0PM) + F(d0MP)) + (F(dP00) + F(dM00)) + (F(d0P0) + F(d0M0)) +
+                   (F(d00P) + F(d00M)) + F(d000);
+    const T rho = drho + T(1);
+    const T oor = T(1) / rho;
+    const T vx1 = oor * ((F(dPPP) - F(dMMM)) + (F(dPPM) - F(dMMP)) + (F(dPMP) - F(dMPM)) + (F(dPMM) - F(dMPP)) +
+                         (F(dPP0) - F(dMM0)) + (F(dPM0) - F(dMP0)) + (F(dP0P) - F(dM0M)) + (F(dP0M) - F(dM0P)) +
+                         (F(dP00) - F(dM00)));
+    const T vx2 = oor * ((F(dPPP) - F(dMMM)) + (F(dPPM) - F(dMMP)) + (F(dMPM) - F(dPMP)) + (F(dMPP) - F(dPMM)) +
+                         (F(dPP0) - F(dMM0)) + (F(dMP0) - F(dPM0)) + (F(d0PP) - F(d0MM)) + (F(d0PM) - F(d0MP)) +
+                         (F(d0P0) - F(d0M0)));
+    const T vx3 = oor * ((F(dPPP) - F(dMMM)) + (F(dMMP) - F(dPPM)) + (F(dPMP) - F(dMPM)) + (F(dMPP) - F(dPMM)) +
+                         (F(dP0P) - F(dM0M)) + (F(dM0P) - F(dP0M)) + (F(d0PP) - F(d0MM)) + (F(d0MP) - F(d0PM)) +
+                         (F(d00P) - F(d00M)));
+    const T cusq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3);
+    const T a = T(1) + (-omega);
+    F(d000) = F(d000) * a - (-omega) * T(8. / 27.) * (drho - rho * cusq);
+#pragma unroll
+    for (int d = 1; d < 27; ++d) {
+        // projected velocity with the reference's left-to-right component order
+        T cu;
+        if (cx(d) != 0) {
+            cu = cx(d) > 0 ? vx1 : -vx1;
+            if (cy(d) != 0) cu = cy(d) > 0 ? cu + vx2 : cu - vx2;
+            if (cz(d) != 0) cu = cz(d) > 0 ? cu + vx3 : cu - vx3;
+        } else if (cy(d) != 0) {
+            cu = cy(d) > 0 ? vx2 : -vx2;
+            if (cz(d) != 0) cu = cz(d) > 0 ? cu + vx3 : cu - vx3;
+        } else {
+            cu = cz(d) > 0 ? vx3 : -vx3;
+        }
+        m[slot_of(d)] = m[slot_of(d)] * a - (-omega) * weight<T>(d) * (drho + rho * (T(3) * cu + T(4.5) * cu * cu - cusq));
+    }
+#undef F
+    mv[0] = drho;
+    mv[1] = vx1;
+    mv[2] = vx2;
+    mv[3] = vx3;
+}
+
+#undef VFX_M
+
+} // namespace vfx

@@ -1,0 +1,477 @@
+// kernels.hpp -- gfx950 kernels of the D3Q27 stepper (templates).
+//
+// Each translation unit includes this header and instantiates the launchers
+// for its storage types; step_fp64.hip is compiled with -ffp-contract=off so
+// the FP64 path reproduces the reference CPU arithmetic exactly, step_fp32.hip
+// (FP32 and scaled-FP16 storage) lets the compiler contract to FMA.
+#pragma once
+
+#include "collision.hpp"
+#include "../../include/vfx.h"
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace vfx {
+
+// ------------------------------------------------------------- storage ----
+// Population storage S with arithmetic type T.  _Float16 holds
+// h_d = (f_d - w_d) * scale / w_d; the per-weight-class factors are kernel
+// arguments (ld[c] = w_c / scale, st[c] = scale / w_c).
+template <typename S, typename T> struct Storage;
+template <> struct Storage<double, double> {
+    static __device__ __forceinline__ double ld(const double* p, uint32_t i, float) { return p[i]; }
+    static __device__ __forceinline__ void st(double* p, uint32_t i, double v, float) { p[i] = v; }
+};
+template <> struct Storage<float, float> {
+    static __device__ __forceinline__ float ld(const float* p, uint32_t i, float) { return p[i]; }
+    static __device__ __forceinline__ void st(float* p, uint32_t i, float v, float) { p[i] = v; }
+};
+template <> struct Storage<_Float16, float> {
+    static __device__ __forceinline__ float ld(const _Float16* p, uint32_t i, float s) { return (float)p[i] * s; }
+    static __device__ __forceinline__ void st(_Float16* p, uint32_t i, float v, float s) { p[i] = (_Float16)(v * s); }
+};
+template <typename S> struct Compute { using T = float; };
+template <> struct Compute<double> { using T = double; };
+
+// Device view of a vfx_lattice.
+struct Lat {
+    void* f;
+    uint64_t N;
+    uint32_t nx, ny, nz, chunks; // direct addressing; chunks = blocks per x row
+    const uint32_t* solid;
+    const uint32_t* nbx;
+    const uint32_t* nby;
+    const uint32_t* nbz;
+    const uint32_t* fluid;
+    uint32_t nfluid;
+    float ld[4], st[4];
+    void* rho;
+    void* vx;
+    void* vy;
+    void* vz;
+    int32_t write_macro;
+};
+
+template <typename S> __device__ __forceinline__ S* plane(const Lat& L, int p) { return (S*)L.f + (uint64_t)p * L.N; }
+
+__device__ __forceinline__ bool is_solid(const Lat& L, uint32_t k)
+{
+    return L.solid != nullptr && ((L.solid[k >> 5] >> (k & 31u)) & 1u);
+}
+
+// The eight nodes a node's populations live on under the esoteric twist:
+// kk[b], bit0 = +x, bit1 = +y, bit2 = +z (VirtualFluids ListIndices,
+// KernelUtilities.h:221-245).
+__device__ __forceinline__ void nbrs_direct_xyz(const Lat& L, uint32_t x, uint32_t y, uint32_t z, uint32_t (&kk)[8])
+{
+    const uint32_t row = y + L.ny * z;
+    const uint32_t xp = (x + 1 == L.nx) ? 0u : x + 1;
+    const uint32_t ryp = (y + 1 == L.ny) ? row - y : row + 1;
+    const uint32_t rzp = (z + 1 == L.nz) ? y : row + L.ny;
+    const uint32_t ryzp = (y + 1 == L.ny) ? rzp - y : rzp + 1;
+    kk[0] = row * L.nx + x;
+    kk[1] = row * L.nx + xp;
+    kk[2] = ryp * L.nx + x;
+    kk[3] = ryp * L.nx + xp;
+    kk[4] = rzp * L.nx + x;
+    kk[5] = rzp * L.nx + xp;
+    kk[6] = ryzp * L.nx + x;
+    kk[7] = ryzp * L.nx + xp;
+}
+__device__ __forceinline__ void nbrs_direct(const Lat& L, uint32_t k, uint32_t (&kk)[8])
+{
+    const uint32_t x = k % L.nx;
+    const uint32_t r = k / L.nx;
+    nbrs_direct_xyz(L, x, r % L.ny, r / L.ny, kk);
+}
+__device__ __forceinline__ void nbrs_sparse(const Lat& L, uint32_t k, uint32_t (&kk)[8])
+{
+    kk[0] = k;
+    kk[1] = L.nbx[k];
+    kk[2] = L.nby[k];
+    kk[4] = L.nbz[k];
+    kk[3] = L.nby[kk[1]];
+    kk[5] = L.nbz[kk[1]];
+    kk[6] = L.nbz[kk[2]];
+    kk[7] = L.nbz[kk[3]];
+}
+
+// pre-collision populations of a node in cube order
+template <typename S, typename T, bool EVEN>
+__device__ __forceinline__ void gather(const Lat& L, const uint32_t (&kk)[8], T (&m)[27])
+{
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+        const int d = dir_at(s);
+        m[s] = Storage<S, T>::ld(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], L.ld[weight_class(d)]);
+    }
+}
+// post-collision populations: the element direction d was read from receives
+// the population of the opposite direction
+template <typename S, typename T, bool EVEN>
+__device__ __forceinline__ void scatter_post(const Lat& L, const uint32_t (&kk)[8], const T (&m)[27])
+{
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+        const int d = dir_at(s);
+        const int e = inv(d);
+        Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], m[slot_of(e)], L.st[weight_class(e)]);
+    }
+}
+// pre-collision write (initialisation, population upload)
+template <typename S, typename T, bool EVEN>
+__device__ __forceinline__ void scatter_pre(const Lat& L, const uint32_t (&kk)[8], const T (&m)[27])
+{
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+        const int d = dir_at(s);
+        Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], m[s], L.st[weight_class(d)]);
+    }
+}
+
+template <typename T, int COLL> __device__ __forceinline__ void collide(T (&m)[27], const Rates<T>& R, T (&mv)[4])
+{
+    if constexpr (COLL == VFX_K17) k17<T>(m, R, mv);
+    else bgk<T>(m, R.omega, mv);
+}
+
+template <typename T> __device__ __forceinline__ void write_macro(const Lat& L, uint32_t k, const T (&mv)[4])
+{
+    ((T*)L.rho)[k] = mv[0];
+    ((T*)L.vx)[k] = mv[1];
+    ((T*)L.vy)[k] = mv[2];
+    ((T*)L.vz)[k] = mv[3];
+}
+
+// ------------------------------------------------------ collide + stream ----
+// Direct (structured) addressing: one block covers `blockDim.x` consecutive
+// nodes of one x row, so every row index below is wave-uniform (scalar ALU).
+template <typename S, int COLL, bool EVEN>
+__global__ void __launch_bounds__(256) k_step_direct(Lat L, Rates<typename Compute<S>::T> R)
+{
+    using T = typename Compute<S>::T;
+    const uint32_t row = blockIdx.x / L.chunks;
+    const uint32_t x = (blockIdx.x - row * L.chunks) * blockDim.x + threadIdx.x;
+    if (x >= L.nx) return;
+    const uint32_t y = row % L.ny, z = row / L.ny;
+    uint32_t kk[8];
+    nbrs_direct_xyz(L, x, y, z, kk);
+    if (is_solid(L, kk[0])) return;
+    T m[27], mv[4];
+    gather<S, T, EVEN>(L, kk, m);
+    collide<T, COLL>(m, R, mv);
+    scatter_post<S, T, EVEN>(L, kk, m);
+    if (L.write_macro) write_macro<T>(L, kk[0], mv);
+}
+
+// VirtualFluids sparse addressing over the fluid-node list
+// (RunCollision.cuh:74-143 semantics).
+template <typename S, int COLL, bool EVEN>
+__global__ void __launch_bounds__(256) k_step_sparse(Lat L, Rates<typename Compute<S>::T> R)
+{
+    using T = typename Compute<S>::T;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.nfluid) return;
+    uint32_t kk[8];
+    const uint32_t k = L.fluid[i];
+    if (k >= L.N) return;
+    nbrs_sparse(L, k, kk);
+    T m[27], mv[4];
+    gather<S, T, EVEN>(L, kk, m);
+    collide<T, COLL>(m, R, mv);
+    scatter_post<S, T, EVEN>(L, kk, m);
+    if (L.write_macro) write_macro<T>(L, kk[0], mv);
+}
+
+// ------------------------------------------------------------ node loops ----
+// Visits every node that collides: direct -> every non-solid node of the box,
+// sparse -> the fluid-node list.
+struct NodeIter {
+    __device__ static bool get(const Lat& L, uint64_t i, uint32_t (&kk)[8], int direct)
+    {
+        if (direct) {
+            if (i >= L.N) return false;
+            const uint32_t k = (uint32_t)i;
+            if (is_solid(L, k)) return false;
+            nbrs_direct(L, k, kk);
+        } else {
+            if (i >= L.nfluid) return false;
+            nbrs_sparse(L, L.fluid[i], kk);
+        }
+        return true;
+    }
+};
+
+// shifted compressible equilibrium (InitNavierStokesCompressible_Device.cu:170-210)
+template <typename T>
+__device__ __forceinline__ void equilibrium(T (&m)[27], T drho, T vx1, T vx2, T vx3)
+{
+    const T cu_sq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3);
+    m[slot_of(d000)] = T(8. / 27.) * (drho - cu_sq * (T(1) + drho));
+#pragma unroll
+    for (int d = 1; d < 27; ++d) {
+        T cu;
+        if (cx(d) != 0) {
+            cu = cx(d) > 0 ? vx1 : -vx1;
+            if (cy(d) != 0) cu = cy(d) > 0 ? cu + vx2 : cu - vx2;
+            if (cz(d) != 0) cu = cz(d) > 0 ? cu + vx3 : cu - vx3;
+        } else if (cy(d) != 0) {
+            cu = cy(d) > 0 ? vx2 : -vx2;
+            if (cz(d) != 0) cu = cz(d) > 0 ? cu + vx3 : cu - vx3;
+        } else {
+            cu = cz(d) > 0 ? vx3 : -vx3;
+        }
+        m[slot_of(d)] = weight<T>(d) * (drho + (T(1) + drho) * (T(3) * cu + T(4.5) * cu * cu - cu_sq));
+    }
+}
+
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) k_init(Lat L, int direct, const double* drho, const double* ux, const double* uy,
+                                              const double* uz)
+{
+    using T = typename Compute<S>::T;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t kk[8];
+    if (!NodeIter::get(L, i, kk, direct)) return;
+    const uint32_t k = kk[0];
+    T m[27];
+    equilibrium<T>(m, (T)drho[k], (T)ux[k], (T)uy[k], (T)uz[k]);
+    scatter_pre<S, T, EVEN>(L, kk, m);
+}
+
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) k_read(Lat L, int direct, double* out)
+{
+    using T = typename Compute<S>::T;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t kk[8];
+    if (!NodeIter::get(L, i, kk, direct)) return;
+    T m[27];
+    gather<S, T, EVEN>(L, kk, m);
+#pragma unroll
+    for (int s = 0; s < 27; ++s) out[(uint64_t)kk[0] * 27 + dir_at(s)] = (double)m[s];
+}
+
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) k_write(Lat L, int direct, const double* in)
+{
+    using T = typename Compute<S>::T;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t kk[8];
+    if (!NodeIter::get(L, i, kk, direct)) return;
+    T m[27];
+#pragma unroll
+    for (int s = 0; s < 27; ++s) m[s] = (T)in[(uint64_t)kk[0] * 27 + dir_at(s)];
+    scatter_pre<S, T, EVEN>(L, kk, m);
+}
+
+// macroscopic values of the pre-collision state (pyramid sums, FP64 output)
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) k_macro(Lat L, int direct, double* drho, double* vx, double* vy, double* vz)
+{
+    using T = typename Compute<S>::T;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t kk[8];
+    if (!NodeIter::get(L, i, kk, direct)) return;
+    T m[27];
+    gather<S, T, EVEN>(L, kk, m);
+    double f[27];
+#pragma unroll
+    for (int s = 0; s < 27; ++s) f[dir_at(s)] = (double)m[s];
+    const double corners = ((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]));
+    const double edges = (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P]))) +
+                         ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]));
+    const double faces = ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0])) + (f[d00M] + f[d00P]);
+    const double r = ((corners + edges) + f[d000]) + faces;
+    const double jx = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dPMM] - f[dMPP]) + (f[dPPM] - f[dMMP]))) +
+                      (((f[dP0M] - f[dM0P]) + (f[dP0P] - f[dM0M])) + ((f[dPM0] - f[dMP0]) + (f[dPP0] - f[dMM0]))) +
+                      (f[dP00] - f[dM00]);
+    const double jy = (((f[dPPP] - f[dMMM]) + (f[dMPM] - f[dPMP])) + ((f[dMPP] - f[dPMM]) + (f[dPPM] - f[dMMP]))) +
+                      (((f[d0PM] - f[d0MP]) + (f[d0PP] - f[d0MM])) + ((f[dMP0] - f[dPM0]) + (f[dPP0] - f[dMM0]))) +
+                      (f[d0P0] - f[d0M0]);
+    const double jz = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dMPP] - f[dPMM]) + (f[dMMP] - f[dPPM]))) +
+                      (((f[d0MP] - f[d0PM]) + (f[d0PP] - f[d0MM])) + ((f[dM0P] - f[dP0M]) + (f[dP0P] - f[dM0M]))) +
+                      (f[d00P] - f[d00M]);
+    const double oor = 1.0 / (r + 1.0);
+    const uint32_t k = kk[0];
+    drho[k] = r;
+    vx[k] = jx * oor;
+    vy[k] = jy * oor;
+    vz[k] = jz * oor;
+}
+
+// -------------------------------------------------- boundary conditions ----
+struct BCDev {
+    uint32_t n;
+    const uint32_t* k;
+    const float* q;
+    const double* vx;
+    const double* vy;
+    const double* vz;
+    int32_t cpu_float;
+    int32_t direct;
+};
+
+// 6 w_d (c_d . u) with the component order VirtualFluids writes it
+// (VelocityBounceBack.cu:159-184); cpu_float reproduces BoundaryConditions.h:195-262
+// (float wall velocity, float component sum, double scale, float result).
+template <typename T>
+__device__ __forceinline__ T wall_term(int d, double ux, double uy, double uz, int cpu_float)
+{
+    const int n = weight_class(d);
+    const double coef = n == 1 ? 4. / 9. : n == 2 ? 1. / 9. : 1. / 36.;
+    if (cpu_float) {
+        const float u[3] = {(float)ux, (float)uy, (float)uz};
+        const int c[3] = {cx(d), cy(d), cz(d)};
+        float s = 0.f;
+        bool first = true;
+        for (int a = 0; a < 3; ++a) {
+            if (c[a] == 0) continue;
+            if (first) { s = c[a] > 0 ? u[a] : -u[a]; first = false; }
+            else s = c[a] > 0 ? s + u[a] : s - u[a];
+        }
+        return (T)(double)(float)(coef * (double)s);
+    }
+    const T u[3] = {(T)ux, (T)uy, (T)uz};
+    const int c[3] = {cx(d), cy(d), cz(d)};
+    T s = T(0);
+    bool first = true;
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] == 0) continue;
+        if (first) { s = c[a] > 0 ? u[a] : -u[a]; first = false; }
+        else s = c[a] > 0 ? s + u[a] : s - u[a];
+    }
+    return (T)coef * s;
+}
+
+// Moving-wall halfway bounce-back (VelocityBounceBack_Device): for every wall
+// link d of a boundary node, the population leaving towards the wall comes back
+// as the opposite direction, minus the wall term.  Runs between the collision
+// of parity EVEN and the parity flip.
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) k_bc_velocity_bb(Lat L, BCDev B)
+{
+    using T = typename Compute<S>::T;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    uint32_t kk[8];
+    const uint32_t k = B.k[i];
+    if (k >= L.N) return; // malformed list entry: never touch memory outside the lattice
+    if (B.direct) nbrs_direct(L, k, kk);
+    else nbrs_sparse(L, k, kk);
+    const double ux = B.vx ? B.vx[i] : 0.0, uy = B.vy ? B.vy[i] : 0.0, uz = B.vz ? B.vz[i] : 0.0;
+#pragma unroll
+    for (int d = 1; d < 27; ++d) {
+        const float q = B.q[(uint64_t)d * B.n + i];
+        if (!(q >= 0.f && q <= 1.f)) continue;
+        const int e = inv(d);
+        // post-collision population of direction d sits where e was read from
+        const T fpost = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)]);
+        const T v = fpost - wall_term<T>(d, ux, uy, uz, B.cpu_float);
+        // next step (parity !EVEN) reads direction e of node k from here
+        Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)]);
+    }
+}
+
+// ------------------------------------------------------------ launchers ----
+constexpr int kBlock = 256;
+
+inline unsigned grid_for(uint64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+template <typename S> hipError_t launch_step(const Lat& L, int direct, int coll, bool even, double omega, const double* lim,
+                                             const double* force, hipStream_t st)
+{
+    using T = typename Compute<S>::T;
+    const Rates<T> R = make_rates<T>(omega, lim, force);
+    if (direct) {
+        const unsigned grid = L.chunks * L.ny * L.nz;
+        const unsigned block = (unsigned)((L.nx + L.chunks - 1) / L.chunks);
+        const unsigned b = block <= 64 ? 64 : block <= 128 ? 128 : 256;
+        if (coll == VFX_K17) {
+            if (even) hipLaunchKernelGGL((k_step_direct<S, VFX_K17, true>), dim3(grid), dim3(b), 0, st, L, R);
+            else hipLaunchKernelGGL((k_step_direct<S, VFX_K17, false>), dim3(grid), dim3(b), 0, st, L, R);
+        } else {
+            if (even) hipLaunchKernelGGL((k_step_direct<S, VFX_BGK, true>), dim3(grid), dim3(b), 0, st, L, R);
+            else hipLaunchKernelGGL((k_step_direct<S, VFX_BGK, false>), dim3(grid), dim3(b), 0, st, L, R);
+        }
+    } else {
+        if (L.nfluid == 0) return hipSuccess;
+        const unsigned grid = grid_for(L.nfluid, kBlock);
+        if (coll == VFX_K17) {
+            if (even) hipLaunchKernelGGL((k_step_sparse<S, VFX_K17, true>), dim3(grid), dim3(kBlock), 0, st, L, R);
+            else hipLaunchKernelGGL((k_step_sparse<S, VFX_K17, false>), dim3(grid), dim3(kBlock), 0, st, L, R);
+        } else {
+            if (even) hipLaunchKernelGGL((k_step_sparse<S, VFX_BGK, true>), dim3(grid), dim3(kBlock), 0, st, L, R);
+            else hipLaunchKernelGGL((k_step_sparse<S, VFX_BGK, false>), dim3(grid), dim3(kBlock), 0, st, L, R);
+        }
+    }
+    return hipGetLastError();
+}
+
+template <typename S> hipError_t launch_init(const Lat& L, int direct, bool even, const double* drho, const double* ux,
+                                             const double* uy, const double* uz, hipStream_t st)
+{
+    const uint64_t n = direct ? L.N : L.nfluid;
+    if (n == 0) return hipSuccess;
+    if (even) hipLaunchKernelGGL((k_init<S, true>), dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, L, direct, drho, ux, uy, uz);
+    else hipLaunchKernelGGL((k_init<S, false>), dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, L, direct, drho, ux, uy, uz);
+    return hipGetLastError();
+}
+
+template <typename S> hipError_t launch_rw(const Lat& L, int direct, bool even, double* out, const double* in, hipStream_t st)
+{
+    const uint64_t n = direct ? L.N : L.nfluid;
+    if (n == 0) return hipSuccess;
+    const dim3 g(grid_for(n, kBlock)), b(kBlock);
+    if (out) {
+        if (even) hipLaunchKernelGGL((k_read<S, true>), g, b, 0, st, L, direct, out);
+        else hipLaunchKernelGGL((k_read<S, false>), g, b, 0, st, L, direct, out);
+    } else {
+        if (even) hipLaunchKernelGGL((k_write<S, true>), g, b, 0, st, L, direct, in);
+        else hipLaunchKernelGGL((k_write<S, false>), g, b, 0, st, L, direct, in);
+    }
+    return hipGetLastError();
+}
+
+template <typename S> hipError_t launch_macro(const Lat& L, int direct, bool even, double* r, double* vx, double* vy,
+                                              double* vz, hipStream_t st)
+{
+    const uint64_t n = direct ? L.N : L.nfluid;
+    if (n == 0) return hipSuccess;
+    const dim3 g(grid_for(n, kBlock)), b(kBlock);
+    if (even) hipLaunchKernelGGL((k_macro<S, true>), g, b, 0, st, L, direct, r, vx, vy, vz);
+    else hipLaunchKernelGGL((k_macro<S, false>), g, b, 0, st, L, direct, r, vx, vy, vz);
+    return hipGetLastError();
+}
+
+template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kind, bool even, hipStream_t st)
+{
+    if (B.n == 0) return hipSuccess;
+    const dim3 g(grid_for(B.n, kBlock)), b(kBlock);
+    if (kind == VFX_BC_VELOCITY_BOUNCE_BACK) {
+        if (even) hipLaunchKernelGGL((k_bc_velocity_bb<S, true>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_velocity_bb<S, false>), g, b, 0, st, L, B);
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+}
+
+#define VFX_INSTANTIATE_LAUNCHERS(S)                                                                                   \
+    template hipError_t launch_step<S>(const Lat&, int, int, bool, double, const double*, const double*, hipStream_t); \
+    template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,             \
+                                       const double*, hipStream_t);                                                    \
+    template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);                      \
+    template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t);       \
+    template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);
+
+#define VFX_EXTERN_LAUNCHERS(S)                                                                                        \
+    extern template hipError_t launch_step<S>(const Lat&, int, int, bool, double, const double*, const double*,        \
+                                              hipStream_t);                                                            \
+    extern template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,      \
+                                              const double*, hipStream_t);                                             \
+    extern template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);               \
+    extern template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t); \
+    extern template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);
+
+} // namespace vfx

@@ -1,0 +1,236 @@
+"""Host-side lattice objects over the libvfx C ABI.
+
+PyTorch only provides device memory and the stream here; every operation on
+the populations is a libvfx kernel.  Mirrors the data VirtualFluids' GPU path
+keeps per level (``LBMSimulationParameter``, src/gpu/core/Parameter/Parameter.h:66-255):
+the distribution array, the neighbour arrays or the structured box, the
+fluid-node list, omega and the quadric limiters.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib as V
+
+STORAGE = {"fp64": V.VFX_FP64, "fp32": V.VFX_FP32, "fp16": V.VFX_FP16_SCALED}
+STORAGE_DTYPE = {"fp64": torch.float64, "fp32": torch.float32, "fp16": torch.float16}
+BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
+COLLISION = {"k17": V.VFX_K17, "bgk": V.VFX_BGK}
+
+# D3Q27 velocity set in VirtualFluids order (src/lbm/constants/D3Q27.h:42-72)
+CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1])
+CY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1, 1, 1, -1, -1, 1, 1, -1, -1])
+CZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1, 1, 1, 1, 1, -1, -1, -1, -1])
+INV = np.array([0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17, 26, 25, 24, 23, 22, 21, 20, 19])
+W = np.array([8 / 27] + [2 / 27] * 6 + [1 / 54] * 12 + [1 / 216] * 8)
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def _stream(stream) -> int | None:
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return stream.cuda_stream if hasattr(stream, "cuda_stream") else stream
+
+
+def pack_mask(solid: np.ndarray) -> np.ndarray:
+    """Bit mask (uint32 words, bit k = node k) from a boolean node array."""
+    flat = np.ascontiguousarray(solid, dtype=bool).ravel()
+    nwords = (flat.size + 31) // 32
+    padded = np.zeros(nwords * 32, dtype=bool)
+    padded[: flat.size] = flat
+    bits = padded.reshape(nwords, 32).astype(np.uint64) << np.arange(32, dtype=np.uint64)
+    return bits.sum(axis=1).astype(np.uint32)
+
+
+class Lattice:
+    """One block of D3Q27 populations on the GPU.
+
+    Structured (``dims=(nx, ny, nz)``, x fastest, periodic index wrap, walls as
+    solid nodes) or VirtualFluids sparse (``neighbors=(nbx, nby, nbz)`` +
+    ``fluid_nodes``)."""
+
+    def __init__(self, dims=None, *, storage="fp64", omega=1.0, limiter=(0.01, 0.01, 0.01),
+                 force=(0.0, 0.0, 0.0), solid=None, fp16_scale=None, neighbors=None, fluid_nodes=None,
+                 n_nodes=None, device="cuda"):
+        if storage not in STORAGE:
+            raise ValueError(f"storage must be one of {sorted(STORAGE)}")
+        self.storage = storage
+        self.device = torch.device(device)
+        self.omega = float(omega)
+        self.limiter = tuple(float(x) for x in limiter)
+        self.force = tuple(float(x) for x in force)
+        self.fp16_scale = float(fp16_scale if fp16_scale is not None else (2.0 ** 10 if storage == "fp16" else 1.0))
+        self.direct = neighbors is None
+        self.solid_mask = None
+        self.nb = None
+        self.fluid = None
+        if self.direct:
+            if dims is None:
+                raise ValueError("structured lattice needs dims")
+            self.dims = tuple(int(d) for d in dims)
+            self.n_nodes = int(np.prod(self.dims))
+            if solid is not None:
+                solid = np.asarray(solid, dtype=bool).ravel()
+                if solid.size != self.n_nodes:
+                    raise ValueError("solid mask size does not match dims")
+                self.solid_mask = torch.from_numpy(pack_mask(solid).view(np.int32)).to(self.device)
+                self.n_fluid = int((~solid).sum())
+            else:
+                self.n_fluid = self.n_nodes
+        else:
+            self.dims = (0, 0, 0)
+            nbx, nby, nbz = (np.ascontiguousarray(a, dtype=np.uint32) for a in neighbors)
+            self.n_nodes = int(n_nodes if n_nodes is not None else nbx.size)
+            if not (nbx.size == nby.size == nbz.size == self.n_nodes):
+                raise ValueError("neighbour arrays must have n_nodes entries")
+            if max(nbx.max(), nby.max(), nbz.max()) >= self.n_nodes:
+                raise ValueError("neighbour index out of range")
+            fl = np.ascontiguousarray(fluid_nodes, dtype=np.uint32)
+            if fl.size and fl.max() >= self.n_nodes:
+                raise ValueError("fluid node index out of range")
+            self.nb = [torch.from_numpy(a.view(np.int32)).to(self.device) for a in (nbx, nby, nbz)]
+            self.fluid = torch.from_numpy(fl.view(np.int32)).to(self.device)
+            self.n_fluid = int(fl.size)
+        self.f = torch.zeros(27 * self.n_nodes, dtype=STORAGE_DTYPE[storage], device=self.device)
+        self.macro = None
+        self._struct = None
+
+    # ------------------------------------------------------------ C view --
+    def struct(self) -> V.VfxLattice:
+        s = V.VfxLattice()
+        s.f = self.f.data_ptr()
+        s.storage = STORAGE[self.storage]
+        s.fp16_scale = self.fp16_scale
+        s.n_nodes = self.n_nodes
+        s.direct = 1 if self.direct else 0
+        s.dims[:] = list(self.dims)
+        s.solid_mask = _ptr(self.solid_mask)
+        if self.nb is not None:
+            s.neighbor_x, s.neighbor_y, s.neighbor_z = (t.data_ptr() for t in self.nb)
+            s.fluid_nodes = _ptr(self.fluid) if self.n_fluid else None
+            s.n_fluid = self.n_fluid
+        s.omega = self.omega
+        s.quadric_limiter[:] = list(self.limiter)
+        s.force[:] = list(self.force)
+        if self.macro is not None:
+            s.rho, s.vx, s.vy, s.vz = (t.data_ptr() for t in self.macro)
+        return s
+
+    @property
+    def bytes_per_population(self) -> int:
+        return BYTES[self.storage]
+
+    # ----------------------------------------------------------- kernels --
+    def init_equilibrium(self, drho, ux, uy, uz, even=True, stream=None):
+        f64 = [torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64).ravel()).to(self.device)
+               if not isinstance(a, torch.Tensor) else a.to(self.device, torch.float64).contiguous().view(-1)
+               for a in (drho, ux, uy, uz)]
+        for a in f64:
+            if a.numel() != self.n_nodes:
+                raise ValueError("initial fields need n_nodes entries")
+        s = self.struct()
+        V.check(V.lib().vfx_init_equilibrium(C.byref(s), int(bool(even)), *(a.data_ptr() for a in f64),
+                                             _stream(stream)), "vfx_init_equilibrium")
+
+    def collide_stream(self, collision="k17", even=True, write_macro=False, stream=None):
+        if write_macro and self.macro is None:
+            dt = torch.float64 if self.storage == "fp64" else torch.float32
+            self.macro = [torch.zeros(self.n_nodes, dtype=dt, device=self.device) for _ in range(4)]
+        s = self.struct()
+        V.check(V.lib().vfx_collide_stream(C.byref(s), COLLISION[collision], int(bool(even)),
+                                           V.VFX_TEMPLATE_WRITE_MACRO if write_macro else V.VFX_TEMPLATE_DEFAULT,
+                                           _stream(stream)), "vfx_collide_stream")
+
+    def apply_bc(self, bc: "BoundaryCondition", even=True, stream=None):
+        if bc.max_node >= self.n_nodes:
+            raise ValueError("boundary node index out of range")
+        s = self.struct()
+        b = bc.struct()
+        V.check(V.lib().vfx_bc_apply(C.byref(s), C.byref(b), int(bool(even)), _stream(stream)), "vfx_bc_apply")
+
+    def read_populations(self, even=True, stream=None) -> torch.Tensor:
+        out = torch.zeros(self.n_nodes, 27, dtype=torch.float64, device=self.device)
+        s = self.struct()
+        V.check(V.lib().vfx_read_populations(C.byref(s), int(bool(even)), out.data_ptr(), _stream(stream)),
+                "vfx_read_populations")
+        return out
+
+    def write_populations(self, F, even=True, stream=None):
+        F = torch.as_tensor(F, dtype=torch.float64).to(self.device).contiguous()
+        if F.shape != (self.n_nodes, 27):
+            raise ValueError("populations must be (n_nodes, 27)")
+        s = self.struct()
+        V.check(V.lib().vfx_write_populations(C.byref(s), int(bool(even)), F.data_ptr(), _stream(stream)),
+                "vfx_write_populations")
+
+    def macroscopic(self, even=True, stream=None):
+        outs = [torch.zeros(self.n_nodes, dtype=torch.float64, device=self.device) for _ in range(4)]
+        s = self.struct()
+        V.check(V.lib().vfx_macroscopic(C.byref(s), int(bool(even)), *(o.data_ptr() for o in outs),
+                                        _stream(stream)), "vfx_macroscopic")
+        return outs
+
+
+class BoundaryCondition:
+    """A boundary-node list in the QforBoundaryConditions shape
+    (src/gpu/core/Calculation/Calculation.h:105-117)."""
+
+    def __init__(self, kind, nodes, q27, vx=None, vy=None, vz=None, cpu_float_velocity=False, device="cuda"):
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+        q = np.ascontiguousarray(q27, dtype=np.float32)
+        n = nodes.size
+        if q.shape != (27, n):
+            raise ValueError("q27 must be (27, n)")
+        self.kind = kind
+        self.n = n
+        dev = torch.device(device)
+        self.max_node = int(nodes.max()) if n else -1
+        self.k = torch.from_numpy(nodes.view(np.int32)).to(dev)
+        self.q = torch.from_numpy(q).to(dev)
+
+        def vel(v):
+            if v is None:
+                return None
+            return torch.from_numpy(np.ascontiguousarray(np.broadcast_to(v, (n,)), dtype=np.float64)).to(dev)
+
+        self.vx, self.vy, self.vz = vel(vx), vel(vy), vel(vz)
+        self.cpu_float_velocity = bool(cpu_float_velocity)
+
+    def struct(self) -> V.VfxBC:
+        b = V.VfxBC()
+        b.kind = self.kind
+        b.n = self.n
+        b.k = self.k.data_ptr() if self.n else None
+        b.q27 = self.q.data_ptr() if self.n else None
+        b.vx, b.vy, b.vz = _ptr(self.vx), _ptr(self.vy), _ptr(self.vz)
+        b.cpu_float_velocity = int(self.cpu_float_velocity)
+        return b
+
+
+@dataclass
+class Stepper:
+    """The per-level time step of UpdateGrid27::updateGrid without refinement
+    (src/gpu/core/Calculation/UpdateGrid27.cpp:54-95): collide+stream, the
+    post-collision boundary conditions, then the parity flip."""
+
+    lattice: Lattice
+    collision: str = "k17"
+    bcs: list = field(default_factory=list)
+    even: bool = True
+
+    def step(self, n=1, stream=None):
+        for _ in range(n):
+            self.lattice.collide_stream(self.collision, self.even, stream=stream)
+            for bc in self.bcs:
+                self.lattice.apply_bc(bc, self.even, stream=stream)
+            self.even = not self.even
+
+    def populations(self) -> torch.Tensor:
+        return self.lattice.read_populations(self.even)
