@@ -1,0 +1,218 @@
+#!/usr/bin/env python3
+"""Throughput of the MI355X D3Q27 K17 stepper.
+
+One step = one collide-and-stream pass of the K17 cumulant kernel over every
+node of a 512 x 256 x 256 lattice (33.5 M nodes), in place (esoteric twist),
+inputs resident in HBM.  Three storage forms run on the same lattice and the
+same initial flow: FP32 arithmetic on per-lattice-scaled FP16 shifted
+populations (the headline form), FP32 on FP32, and plain FP64.
+
+Prints ONE JSON line.  ``value`` is the MLUPS of the headline form;
+``roofline`` prices its kernel against MI355X's 8 TB/s with the algorithmic
+54 * bytes-per-population bytes per lattice update; ``cpu_baseline`` times the
+VirtualFluids reference CPU K17 path (compiled from its sources) on this
+host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+DIMS = (512, 256, 256)
+OMEGA = 1.9417  # Re = 200 on the mean sphere diameter of the channel-with-spheres case
+U0 = 0.05
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=100)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--forms", default="fp16,fp32,fp32:plain,fp64",
+                   help="storage[:plain]; plain = reference operation order")
+    p.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--copy-probe", action="store_true",
+                   help="only run a plain device copy of known size (reference for HBM counters and bandwidth)")
+    return p.parse_args()
+
+
+def copy_probe(torch, nbytes=1 << 30, reps=10):
+    """Plain device-to-device copy: calibrates FETCH_SIZE/WRITE_SIZE counters
+    (known bytes per dispatch) and shows the copy bandwidth this GPU reaches."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes,
+                      "ms": round(ms, 4), "gbs_read_plus_write": round(2 * nbytes / ms / 1e6, 1)}))
+
+
+def initial_flow(torch, dims, device):
+    """Smooth periodic shear-layer flow at |u| ~ 0.05 (lattice units), FP64."""
+    nx, ny, nz = dims
+    z = torch.arange(nz, device=device, dtype=torch.float64).view(nz, 1, 1)
+    y = torch.arange(ny, device=device, dtype=torch.float64).view(1, ny, 1)
+    x = torch.arange(nx, device=device, dtype=torch.float64).view(1, 1, nx)
+    kx, ky, kz = 2 * math.pi / nx, 2 * math.pi / ny, 2 * math.pi / nz
+    ux = U0 * torch.sin(ky * y) * torch.cos(kz * z) + 0.1 * U0 * torch.sin(kx * x)
+    uy = 0.5 * U0 * torch.sin(kx * x) * torch.cos(kz * z) + 0 * y
+    uz = 0.5 * U0 * torch.cos(kx * x) * torch.sin(ky * y) + 0 * z
+    drho = 1e-3 * torch.cos(kx * x) * torch.cos(ky * y) * torch.cos(kz * z)
+    shape = (nz, ny, nx)
+    return [t.expand(shape).contiguous().view(-1) for t in (drho, ux, uy, uz)]
+
+
+def run_form(torch, vfx, form, fields, steps, warmup):
+    storage, _, variant = form.partition(":")
+    lat = vfx.Lattice(DIMS, storage=storage, omega=OMEGA, device="cuda", variant=variant or "default")
+    lat.init_equilibrium(*fields, even=True)
+    st = vfx.Stepper(lat, "k17")
+    st.step(warmup)
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    st.step(steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    macro = lat.macroscopic(st.even)
+    del lat
+    return {"ms_per_step": ms / steps, "wall_s": wall, "macro": macro}
+
+
+def velocity_l2(a, b):
+    _, ax, ay, az = a
+    _, bx, by, bz = b
+    num = ((ax - bx) ** 2 + (ay - by) ** 2 + (az - bz) ** 2).sum()
+    den = (bx ** 2 + by ** 2 + bz ** 2).sum()
+    return float((num / den).sqrt())
+
+
+def density_drift(a, b):
+    return float((a[0] - b[0]).abs().max())
+
+
+def mass_error(a, b):
+    return float((a[0].sum() - b[0].sum()).abs() / a[0].numel())
+
+
+def cpu_baseline(target_s):
+    """VirtualFluids' CPU K17 (K17CompressibleNavierStokes::calculate + EsoSplit,
+    FP64, OpenMP guided over blocks as Simulation::calculateBlocks) on 32^3
+    blocks, two blocks per thread."""
+    import oracle
+
+    if not oracle.ref_available():
+        return None
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    nblocks = max(2, 2 * threads)
+    bs = 32
+    secs, used = oracle.ref_time_blocks(nblocks, bs, OMEGA, 2)
+    per_step = secs / 2
+    steps = max(2, int(target_s / max(per_step, 1e-6)))
+    secs, used = oracle.ref_time_blocks(nblocks, bs, OMEGA, steps)
+    lups = nblocks * bs ** 3 * steps
+    return {"value": round(lups / secs / 1e6, 2), "unit": "MLUPS", "cores": used, "kind": "reference",
+            "sample": f"{nblocks} blocks of {bs}^3 nodes x {steps} steps, K17 FP64, "
+                      f"K17CompressibleNavierStokes::calculate + swap (no block exchange), {secs:.1f} s"}
+
+
+def traffic_from_profiles(form):
+    """HBM bytes per launch of the headline kernel from the committed rocprofv3
+    PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", "k17_fp16_512x256x256_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(f"traffic_bytes_per_launch[{form}]"):
+            return float(line.split("=")[1])
+    return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build has no multi-GPU path.",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import virtualfluids_amd as vfx
+
+    torch.cuda.set_device(0)
+    if args.copy_probe:
+        copy_probe(torch)
+        return
+    n_nodes = DIMS[0] * DIMS[1] * DIMS[2]
+    fields = initial_flow(torch, DIMS, "cuda")
+    forms = [f.strip() for f in args.forms.split(",") if f.strip()]
+    res = {}
+    for form in forms:
+        r = run_form(torch, vfx, form, fields, args.steps, args.warmup)
+        r["mlups"] = n_nodes / (r["ms_per_step"] * 1e-3) / 1e6
+        r["b_alg"] = 54 * vfx.BYTES[form.split(":")[0]]
+        r["gbs"] = r["b_alg"] * n_nodes / (r["ms_per_step"] * 1e-3) / 1e9
+        res[form] = r
+        print(f"# {form}: {r['ms_per_step']:.3f} ms/step  {r['mlups']:.0f} MLUPS  {r['gbs']:.0f} GB/s "
+              f"({100 * r['gbs'] / PEAK_HBM_GBS:.1f}% of peak)", file=sys.stderr, flush=True)
+    head = forms[0]
+    h = res[head]
+    out = {
+        "metric": "D3Q27 K17 MLUPS + %HBM peak; FP16-store/FP32 vs FP64; batched vs per-domain",
+        "value": round(h["mlups"], 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(h["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if not head.startswith("fp64") else "f64",
+        "data": "synthetic smooth periodic shear flow, |u|~0.05, equilibrium start",
+        "config": {"workload": f"K17 collide+stream, {DIMS[0]}x{DIMS[1]}x{DIMS[2]} D3Q27 periodic box, "
+                               f"omega={OMEGA}, storage={head} (shifted f-w)",
+                   "nodes": n_nodes, "storage": head, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(h["gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(h["gbs"] / PEAK_HBM_GBS, 4), "traffic": traffic_from_profiles(head),
+                     "bytes_per_lup": h["b_alg"]},
+        "forms": {f: {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+                      "achieved_gbs": round(r["gbs"], 1), "frac_hbm_peak": round(r["gbs"] / PEAK_HBM_GBS, 4),
+                      "bytes_per_lup": r["b_alg"]} for f, r in res.items()},
+    }
+    if "fp64" in res:
+        ref = res["fp64"]["macro"]
+        for f, r in res.items():
+            if f == "fp64":
+                continue
+            out["forms"][f]["speedup_vs_fp64"] = round(r["mlups"] / res["fp64"]["mlups"], 3)
+            out["forms"][f]["velocity_l2_vs_fp64"] = velocity_l2(r["macro"], ref)
+            out["forms"][f]["density_drift_vs_fp64"] = density_drift(r["macro"], ref)
+            out["forms"][f]["mass_error_per_node_vs_fp64"] = mass_error(r["macro"], ref)
+    out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

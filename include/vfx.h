@@ -78,6 +78,12 @@ typedef struct vfx_lattice {
     void* vx;
     void* vy;
     void* vz;
+
+    /* 0: default -- FP64 keeps the reference operation order (bit-identical to the
+     *    CPU path); FP32/FP16 run the reduced-operation K17 (same collision, folded
+     *    algebra, hardware reciprocal);
+     * 1: reference operation order in every precision ("plain" kernel). */
+    int32_t kernel_variant;
 } vfx_lattice;
 
 /* boundary-condition kinds (src/gpu/core/BoundaryConditions/BoundaryConditionFactory.h:60-132) */
@@ -132,6 +138,33 @@ vfx_status vfx_write_populations(const vfx_lattice* lat, int32_t is_even_timeste
  * src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237), doubles, n_nodes each. */
 vfx_status vfx_macroscopic(const vfx_lattice* lat, int32_t is_even_timestep, double* drho, double* vx,
                            double* vy, double* vz, vfx_stream stream);
+
+/* FP64 reductions (deterministic two-pass: per-block partials in `workspace`,
+ * then one fixed-order pass).  `workspace` needs vfx_reduce_workspace_bytes()
+ * bytes of device memory; results are written to device memory. */
+uint64_t vfx_reduce_workspace_bytes(const vfx_lattice* lat);
+
+/* out4 = {sum drho, sum j_x, sum j_y, sum j_z} over the colliding nodes of the
+ * pre-collision state: the global mass deviation and momentum.  The
+ * reference reduces with thrust (KineticEnergyAnalyzer.cu:99-100); this is the
+ * FP64 mass/momentum conservation monitor of the stepper. */
+vfx_status vfx_reduce_mass_momentum(const vfx_lattice* lat, int32_t is_even_timestep, double* workspace, double* out4,
+                                    vfx_stream stream);
+
+/* KineticEnergyAnalyzer::run (src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:66-154) and
+ * EnstrophyAnalyzer::run (EnstrophyAnalyzer.cu:64-336) on macroscopic fields from
+ * vfx_macroscopic: out2 = {sum, number of fluid nodes with a fluid 7-point stencil};
+ * the analyzers' value is out2[0] / out2[1].  Structured (direct) lattices only. */
+vfx_status vfx_kinetic_energy(const vfx_lattice* lat, const double* drho, const double* vx, const double* vy,
+                              const double* vz, double* workspace, double* out2, vfx_stream stream);
+vfx_status vfx_enstrophy(const vfx_lattice* lat, const double* drho, const double* vx, const double* vy,
+                         const double* vz, double* workspace, double* out2, vfx_stream stream);
+
+/* CalcTurbulenceIntensity (src/gpu/core/PostProcessor/TurbulenceIntensity.cu:45-85):
+ * adds the current velocity to FP64 running sums, sums9 = 9 planes of n_nodes:
+ * u_x, u_y, u_z, u_x^2, u_y^2, u_z^2, u_x u_y, u_x u_z, u_y u_z.  After T samples the
+ * time-averaged mean is sums/T and the Reynolds stress sums_ij/T - mean_i mean_j. */
+vfx_status vfx_accumulate_stats(const vfx_lattice* lat, int32_t is_even_timestep, double* sums9, vfx_stream stream);
 
 /* -------------------------------------------------------------- misc */
 const char* vfx_last_error(void);

@@ -64,6 +64,8 @@ def lib():
                               C.c_int, C.POINTER(OrcBC), C.c_int, _dp]
         L.orc_run.restype = C.c_int
         L.orc_num_threads.restype = C.c_int
+        L.orc_mass_momentum.argtypes = [_dp, C.c_int64, _u8p, _dp]
+        L.orc_energy.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _u8p, C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -199,6 +201,24 @@ def run(F, dims, nsteps, omega, collision="k17", solid=None, bcs=(), lim=LIMITER
     if rc != 0:
         raise RuntimeError(f"orc_run failed ({rc}): a fluid->solid link has no boundary condition")
     return (F, macro) if want_macro else F
+
+
+def mass_momentum(F, solid=None):
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    out = np.zeros(4)
+    sol = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).ravel()
+    lib().orc_mass_momentum(_p(F), F.shape[0], sol.ctypes.data_as(_u8p) if sol is not None else None, _p(out))
+    return out
+
+
+def energy(F, dims, enstrophy=False, solid=None):
+    """(sum, count) of the kinetic-energy or enstrophy analyzer."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    out = np.zeros(2)
+    sol = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).ravel()
+    lib().orc_energy(_p(F), dims[0], dims[1], dims[2], sol.ctypes.data_as(_u8p) if sol is not None else None,
+                     int(bool(enstrophy)), _p(out))
+    return out
 
 
 def ref_run_periodic(F, dims, nsteps, omega):

@@ -632,6 +632,85 @@ int orc_run(double* F, int nx, int ny, int nz, const uint8_t* solid, int collisi
     return 0;
 }
 
+// Global mass deviation and momentum of node-major populations (long double
+// accumulation): out4 = {sum drho, sum jx, sum jy, sum jz} over non-solid nodes.
+void orc_mass_momentum(const double* F, int64_t n, const uint8_t* solid, double* out4)
+{
+    long double a[4] = {0, 0, 0, 0};
+    for (int64_t i = 0; i < n; ++i) {
+        if (solid && solid[i]) continue;
+        const double* f = F + 27 * i;
+        a[0] += density(f);
+        a[1] += mom_x(f);
+        a[2] += mom_y(f);
+        a[3] += mom_z(f);
+    }
+    for (int c = 0; c < 4; ++c) out4[c] = (double)a[c];
+}
+
+// KineticEnergyAnalyzer.cu:137-154 / EnstrophyAnalyzer.cu:134-336 on a
+// structured periodic box: out2 = {sum, count of nodes with a fluid 7-stencil}.
+void orc_energy(const double* F, int nx, int ny, int nz, const uint8_t* solid, int enstrophy, double* out2)
+{
+    const int64_t N = (int64_t)nx * ny * nz;
+    std::vector<double> r(N), u[3];
+    for (int a = 0; a < 3; ++a) u[a].resize(N);
+    for (int64_t i = 0; i < N; ++i) {
+        const double* f = F + 27 * i;
+        r[i] = density(f);
+        const double oor = 1.0 / (r[i] + 1.0);
+        u[0][i] = mom_x(f) * oor;
+        u[1][i] = mom_y(f) * oor;
+        u[2][i] = mom_z(f) * oor;
+    }
+    auto idx = [&](int x, int y, int z) {
+        x = ((x % nx) + nx) % nx; y = ((y % ny) + ny) % ny; z = ((z % nz) + nz) % nz;
+        return (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+    };
+    auto fluid = [&](int64_t k) { return !(solid && solid[k]); };
+    long double sum = 0, cnt = 0;
+    for (int z = 0; z < nz; ++z)
+        for (int y = 0; y < ny; ++y)
+            for (int x = 0; x < nx; ++x) {
+                const int64_t k = idx(x, y, z);
+                if (!fluid(k)) continue;
+                const int p[3] = {x, y, z};
+                auto at = [&](int ax, int s) {
+                    int q[3] = {p[0], p[1], p[2]};
+                    q[ax] += s;
+                    return idx(q[0], q[1], q[2]);
+                };
+                bool ok = true;
+                for (int ax = 0; ax < 3; ++ax)
+                    if (!fluid(at(ax, 1)) || !fluid(at(ax, -1))) ok = false;
+                if (!ok) continue;
+                if (!enstrophy) {
+                    sum += 0.5 * (u[0][k] * u[0][k] + u[1][k] * u[1][k] + u[2][k] * u[2][k]) * (r[k] + 1.0);
+                    cnt += 1;
+                    continue;
+                }
+                auto deriv = [&](int comp, int ax) {
+                    const std::vector<double>& v = u[comp];
+                    int order = 8;
+                    if (!fluid(at(ax, 2)) || !fluid(at(ax, -2))) order = 2;
+                    if ((!fluid(at(ax, 3)) || !fluid(at(ax, -3))) && order > 4) order = 4;
+                    if ((!fluid(at(ax, 4)) || !fluid(at(ax, -4))) && order > 6) order = 6;
+                    auto d = [&](int s) { return v[at(ax, s)] - v[at(ax, -s)]; };
+                    if (order == 2) return d(1) / 2.0;
+                    if (order == 4) return ((8.0 * v[at(ax, 1)] - 8.0 * v[at(ax, -1)]) - d(2)) / 12.0;
+                    if (order == 6) return (45.0 * d(1) - 9.0 * d(2) + d(3)) / 60.0;
+                    return (224.0 * d(1) - 56.0 * d(2) + (32.0 / 3.0) * d(3) - d(4)) / 280.0;
+                };
+                const double wx = deriv(2, 1) - deriv(1, 2);
+                const double wy = deriv(0, 2) - deriv(2, 0);
+                const double wz = deriv(1, 0) - deriv(0, 1);
+                sum += 0.5 * (r[k] + 1.0) * (wx * wx + wy * wy + wz * wz);
+                cnt += 1;
+            }
+    out2[0] = (double)sum;
+    out2[1] = (double)cnt;
+}
+
 int orc_num_threads(void)
 {
 #ifdef _OPENMP

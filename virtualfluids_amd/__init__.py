@@ -6,8 +6,8 @@ The product is ``libvfx.so`` (hand-written HIP for gfx950 behind the C ABI in
 call the library; nothing here computes populations on the CPU.
 """
 from ._lib import VfxError, check, lib  # noqa: F401
-from .lattice import (BYTES, CX, CY, CZ, INV, W, BoundaryCondition, Lattice, Stepper,  # noqa: F401
+from .lattice import (BYTES, CX, CY, CZ, INV, W, BoundaryCondition, Lattice, Statistics, Stepper,  # noqa: F401
                       pack_mask)
 
-__all__ = ["Lattice", "BoundaryCondition", "Stepper", "VfxError", "lib", "check", "pack_mask",
+__all__ = ["Lattice", "BoundaryCondition", "Stepper", "Statistics", "VfxError", "lib", "check", "pack_mask",
            "CX", "CY", "CZ", "INV", "W", "BYTES"]

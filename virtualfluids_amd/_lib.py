@@ -29,6 +29,7 @@ class VfxLattice(C.Structure):
         ("fluid_nodes", C.c_void_p), ("n_fluid", C.c_uint32),
         ("omega", C.c_double), ("quadric_limiter", C.c_double * 3), ("force", C.c_double * 3),
         ("rho", C.c_void_p), ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
+        ("kernel_variant", C.c_int32),
     ]
 
 
@@ -56,6 +57,10 @@ _SIGNATURES = {
     "vfx_write_populations": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
     "vfx_macroscopic": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                         C.c_void_p],
+    "vfx_reduce_mass_momentum": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p],
+    "vfx_kinetic_energy": [C.POINTER(VfxLattice)] + [C.c_void_p] * 7,
+    "vfx_enstrophy": [C.POINTER(VfxLattice)] + [C.c_void_p] * 7,
+    "vfx_accumulate_stats": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
 }
 
 
@@ -71,6 +76,8 @@ def lib():
             fn = getattr(L, name)
             fn.argtypes = argt
             fn.restype = C.c_int32
+        L.vfx_reduce_workspace_bytes.argtypes = [C.POINTER(VfxLattice)]
+        L.vfx_reduce_workspace_bytes.restype = C.c_uint64
         L.vfx_last_error.restype = C.c_char_p
         L.vfx_version.restype = C.c_int32
         _lib = L

@@ -106,6 +106,8 @@ vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t
     vfx::Lat L;
     if (vfx_status s = make_lat(lat, L)) return s;
     if (collision != VFX_K17 && collision != VFX_BGK) return fail(VFX_EINVAL, "unknown collision %d", collision);
+    if (lat->kernel_variant != 0 && lat->kernel_variant != 1)
+        return fail(VFX_EINVAL, "unknown kernel_variant %d", lat->kernel_variant);
     if (tmpl != VFX_TEMPLATE_DEFAULT && tmpl != VFX_TEMPLATE_WRITE_MACRO)
         return fail(VFX_EUNSUPPORTED, "collision template %d not supported", tmpl);
     if (tmpl == VFX_TEMPLATE_WRITE_MACRO) {
@@ -117,13 +119,13 @@ vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t
     hipError_t e;
     switch (lat->storage) {
     case VFX_FP64:
-        e = vfx::launch_step<double>(L, lat->direct, collision, is_even != 0, lat->omega, lat->quadric_limiter, lat->force, st);
+        e = vfx::launch_step<double>(L, lat->direct, collision, is_even != 0, lat->kernel_variant, lat->omega, lat->quadric_limiter, lat->force, st);
         break;
     case VFX_FP32:
-        e = vfx::launch_step<float>(L, lat->direct, collision, is_even != 0, lat->omega, lat->quadric_limiter, lat->force, st);
+        e = vfx::launch_step<float>(L, lat->direct, collision, is_even != 0, lat->kernel_variant, lat->omega, lat->quadric_limiter, lat->force, st);
         break;
     default:
-        e = vfx::launch_step<_Float16>(L, lat->direct, collision, is_even != 0, lat->omega, lat->quadric_limiter, lat->force, st);
+        e = vfx::launch_step<_Float16>(L, lat->direct, collision, is_even != 0, lat->kernel_variant, lat->omega, lat->quadric_limiter, lat->force, st);
         break;
     }
     return hip_status(e, "collide_stream launch");
@@ -214,6 +216,73 @@ vfx_status vfx_macroscopic(const vfx_lattice* lat, int32_t is_even, double* drho
     default: e = vfx::launch_macro<_Float16>(L, lat->direct, is_even != 0, drho, vx, vy, vz, st); break;
     }
     return hip_status(e, "macroscopic launch");
+}
+
+uint64_t vfx_reduce_workspace_bytes(const vfx_lattice* lat)
+{
+    if (!lat) return 0;
+    const uint64_t n = lat->direct ? lat->n_nodes : lat->n_fluid;
+    return (uint64_t)vfx::reduce_blocks(n ? n : 1) * 4 * sizeof(double);
+}
+
+vfx_status vfx_reduce_mass_momentum(const vfx_lattice* lat, int32_t is_even, double* workspace, double* out4,
+                                    vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    if (!workspace || !out4) return fail(VFX_EINVAL, "workspace / out4 is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    switch (lat->storage) {
+    case VFX_FP64: e = vfx::launch_reduce_mass<double>(L, lat->direct, is_even != 0, workspace, out4, st); break;
+    case VFX_FP32: e = vfx::launch_reduce_mass<float>(L, lat->direct, is_even != 0, workspace, out4, st); break;
+    default: e = vfx::launch_reduce_mass<_Float16>(L, lat->direct, is_even != 0, workspace, out4, st); break;
+    }
+    return hip_status(e, "mass/momentum reduction launch");
+}
+
+static vfx_status energy(const vfx_lattice* lat, bool enstrophy, const double* drho, const double* vx, const double* vy,
+                         const double* vz, double* workspace, double* out2, vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    if (!lat->direct) return fail(VFX_EUNSUPPORTED, "energy diagnostics need structured (direct) addressing");
+    if (!drho || !vx || !vy || !vz || !workspace || !out2) return fail(VFX_EINVAL, "NULL field/workspace/output");
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t nb = vfx::reduce_blocks(L.N);
+    if (enstrophy)
+        hipLaunchKernelGGL((vfx::k_reduce_energy<true>), dim3(nb), dim3(vfx::kRedBlock), 0, st, L, drho, vx, vy, vz, workspace);
+    else
+        hipLaunchKernelGGL((vfx::k_reduce_energy<false>), dim3(nb), dim3(vfx::kRedBlock), 0, st, L, drho, vx, vy, vz, workspace);
+    hipLaunchKernelGGL((vfx::k_reduce_finish<2>), dim3(1), dim3(vfx::kRedBlock), 0, st, workspace, nb, out2);
+    return hip_status(hipGetLastError(), enstrophy ? "enstrophy launch" : "kinetic energy launch");
+}
+
+vfx_status vfx_kinetic_energy(const vfx_lattice* lat, const double* drho, const double* vx, const double* vy,
+                              const double* vz, double* workspace, double* out2, vfx_stream stream)
+{
+    return energy(lat, false, drho, vx, vy, vz, workspace, out2, stream);
+}
+
+vfx_status vfx_enstrophy(const vfx_lattice* lat, const double* drho, const double* vx, const double* vy, const double* vz,
+                         double* workspace, double* out2, vfx_stream stream)
+{
+    return energy(lat, true, drho, vx, vy, vz, workspace, out2, stream);
+}
+
+vfx_status vfx_accumulate_stats(const vfx_lattice* lat, int32_t is_even, double* sums9, vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    if (!sums9) return fail(VFX_EINVAL, "sums9 is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    switch (lat->storage) {
+    case VFX_FP64: e = vfx::launch_stats<double>(L, lat->direct, is_even != 0, sums9, st); break;
+    case VFX_FP32: e = vfx::launch_stats<float>(L, lat->direct, is_even != 0, sums9, st); break;
+    default: e = vfx::launch_stats<_Float16>(L, lat->direct, is_even != 0, sums9, st); break;
+    }
+    return hip_status(e, "statistics launch");
 }
 
 } // extern "C"

@@ -40,6 +40,11 @@ template <typename T> struct Rates {
     T omega, OxyyPxzz, OxyyMxzz, Oxyz, factorA, factorB;
     T limP, limM, limD;
     T fx, fy, fz; // half force added to the velocity (:164-166)
+    // folded constants of the reduced-operation form (k17_fast)
+    T om1;                // 1 - omega
+    T c3;                 // 3 (1 - omega/2), Galilean correction weight
+    T limP1, limM1, limD1; // (1 - O3) * lambda of the three limiter groups
+    T A4, B4;             // (1/omega - 1/2) * 2/3 * factorA, (1/omega - 1/2) * 1/3 * factorB
 };
 
 template <typename T> inline Rates<T> make_rates(double omega, const double lim[3], const double force[3])
@@ -60,6 +65,21 @@ template <typename T> inline Rates<T> make_rates(double omega, const double lim[
     r.fx = (T)force[0];
     r.fy = (T)force[1];
     r.fz = (T)force[2];
+    // folded constants, evaluated in double and rounded once
+    const double od = omega;
+    const double oPP = 8. * (-2. + od) * (1. + 2. * od) / (-8. - 14. * od + 7. * od * od);
+    const double oMM = 8. * (-2. + od) * (-7. + 4. * od) / (56. - 50. * od + 9. * od * od);
+    const double oD = 24. * (-2. + od) * (-2. - 7. * od + 3. * od * od) /
+                      (48. + 152. * od - 130. * od * od + 29. * od * od * od);
+    const double fA = (4. + 2. * od - 3. * od * od) / (2. - 7. * od + 5. * od * od);
+    const double fB = (4. + 28. * od - 14. * od * od) / (6. - 21. * od + 15. * od * od);
+    r.om1 = (T)(1. - od);
+    r.c3 = (T)(3. * (1. - 0.5 * od));
+    r.limP1 = (T)((1. - oPP) * lim[0]);
+    r.limM1 = (T)((1. - oMM) * lim[1]);
+    r.limD1 = (T)((1. - oD) * lim[2]);
+    r.A4 = (T)((1. / od - 0.5) * (2. / 3.) * fA);
+    r.B4 = (T)((1. / od - 0.5) * (1. / 3.) * fB);
     return r;
 }
 
@@ -392,6 +412,210 @@ template <typename T> __device__ __forceinline__ void bgk(T (&m)[27], T omega, T
     mv[1] = vx1;
     mv[2] = vx2;
     mv[3] = vx3;
+}
+
+#undef VFX_M
+
+// ------------------------------------------------------ K17, reduced form ----
+// The same collision as k17<T> with the algebra folded for FP32 throughput:
+//  * the pre-collision cumulants of order 4..6 are never formed: K17 relaxes
+//    them with rate 1 (O4 = O5 = O6 = 1, K17CompressibleNavierStokes.h:258-264),
+//    so their post-collision values do not depend on them, and the forward
+//    transform skips every moment of total order >= 4;
+//  * chimera constants enter as K only (K^-1 * K = 1), as in the reference's
+//    forwardChimeraWithK/backwardChimeraWithK (ChimeraTransformation.h:124-147);
+//  * omega-only factors are folded on the host (Rates::om1, c3, A4, B4, lim*1)
+//    and the limiter becomes x (1-O) lambda / (|x| + lambda);
+//  * divisions use the hardware reciprocal (1 ulp).
+// Macroscopic sums go row by row along x (pairwise), which is as well
+// conditioned as the pyramid sums.  Not bit-identical to the reference order;
+// tested against the FP64 oracle by error norms.
+__device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// forward three-point transform (k = 0: no weight constant); `full` = 2 also
+// forms the second-order output, 1 only the first, 0 only the sum
+template <int OUT>
+__device__ __forceinline__ void fwdf(float& a, float& b, float& c, float vv, float v2, float k)
+{
+    const float s = a + c;
+    const float d = c - a;
+    const float m0 = s + b;
+    const float mk = m0 + k;
+    a = m0;
+    if (OUT >= 1) b = d - mk * vv;
+    if (OUT >= 2) c = (s - 2.f * d * vv) + mk * v2;
+}
+// backward three-point transform; hvm = (v2 - vv)/2, hvp = (v2 + vv)/2,
+// vmh = vv - 1/2, vph = vv + 1/2, tv = 2 vv
+__device__ __forceinline__ void bwdf(float& a, float& b, float& c, float k, float v2, float hvm, float hvp, float vmh,
+                                     float vph, float tv)
+{
+    const float ak = a + k;
+    const float ch = 0.5f * c;
+    const float r0 = ch + b * vmh + ak * hvm;
+    const float r1 = (a - c) - b * tv - ak * v2;
+    const float r2 = ch + b * vph + ak * hvp;
+    a = r0;
+    b = r1;
+    c = r2;
+}
+
+#define VFX_M(a, b, c) m[cube(a, b, c)]
+
+__device__ __forceinline__ void k17_fast(float (&m)[27], const Rates<float>& R, float (&mv)[4])
+{
+    constexpr float c1o3 = 1.f / 3.f, c1o9 = 1.f / 9.f, c1o27 = 1.f / 27.f, c2o3 = 2.f / 3.f;
+    // ---- macroscopic values: row sums along x, then pairwise over the rows
+    float rs[3][3], rd[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            rs[j][k] = (VFX_M(0, j, k) + VFX_M(2, j, k)) + VFX_M(1, j, k);
+            rd[j][k] = VFX_M(2, j, k) - VFX_M(0, j, k);
+        }
+    const float drho = (((rs[0][0] + rs[2][2]) + (rs[0][2] + rs[2][0])) + ((rs[0][1] + rs[2][1]) + (rs[1][0] + rs[1][2]))) + rs[1][1];
+    const float jx = (((rd[0][0] + rd[2][2]) + (rd[0][2] + rd[2][0])) + ((rd[0][1] + rd[2][1]) + (rd[1][0] + rd[1][2]))) + rd[1][1];
+    const float jy = ((rs[2][0] - rs[0][0]) + (rs[2][2] - rs[0][2])) + (rs[2][1] - rs[0][1]);
+    const float jz = ((rs[0][2] - rs[0][0]) + (rs[2][2] - rs[2][0])) + (rs[1][2] - rs[1][0]);
+    const float oneOverRho = rcpf(drho + 1.f);
+    const float vx = jx * oneOverRho + R.fx;
+    const float vy = jy * oneOverRho + R.fy;
+    const float vz = jz * oneOverRho + R.fz;
+    const float vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+
+    // ---- forward: z, y, x; moments of total order >= 4 are not formed
+    constexpr float wv[3] = {1.f / 6.f, 2.f / 3.f, 1.f / 6.f};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) fwdf<2>(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), vz, vz2, wv[i] * wv[j]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        fwdf<2>(VFX_M(i, 0, 0), VFX_M(i, 1, 0), VFX_M(i, 2, 0), vy, vy2, wv[i]);
+        fwdf<2>(VFX_M(i, 0, 1), VFX_M(i, 1, 1), VFX_M(i, 2, 1), vy, vy2, 0.f);
+        fwdf<1>(VFX_M(i, 0, 2), VFX_M(i, 1, 2), VFX_M(i, 2, 2), vy, vy2, wv[i] * c1o3);
+    }
+    fwdf<2>(VFX_M(0, 0, 0), VFX_M(1, 0, 0), VFX_M(2, 0, 0), vx, vx2, 1.f);
+    fwdf<2>(VFX_M(0, 1, 0), VFX_M(1, 1, 0), VFX_M(2, 1, 0), vx, vx2, 0.f);
+    fwdf<2>(VFX_M(0, 0, 1), VFX_M(1, 0, 1), VFX_M(2, 0, 1), vx, vx2, 0.f);
+    fwdf<1>(VFX_M(0, 2, 0), VFX_M(1, 2, 0), VFX_M(2, 2, 0), vx, vx2, c1o3);
+    fwdf<1>(VFX_M(0, 0, 2), VFX_M(1, 0, 2), VFX_M(2, 0, 2), vx, vx2, c1o3);
+    fwdf<1>(VFX_M(0, 1, 1), VFX_M(1, 1, 1), VFX_M(2, 1, 1), vx, vx2, 0.f);
+    fwdf<0>(VFX_M(0, 1, 2), VFX_M(1, 1, 2), VFX_M(2, 1, 2), vx, vx2, 0.f);
+    fwdf<0>(VFX_M(0, 2, 1), VFX_M(1, 2, 1), VFX_M(2, 2, 1), vx, vx2, 0.f);
+
+    float& m000 = VFX_M(0, 0, 0); float& m100 = VFX_M(1, 0, 0); float& m010 = VFX_M(0, 1, 0); float& m001 = VFX_M(0, 0, 1);
+    float& m200 = VFX_M(2, 0, 0); float& m020 = VFX_M(0, 2, 0); float& m002 = VFX_M(0, 0, 2);
+    float& m110 = VFX_M(1, 1, 0); float& m101 = VFX_M(1, 0, 1); float& m011 = VFX_M(0, 1, 1);
+    float& m111 = VFX_M(1, 1, 1);
+    float& m210 = VFX_M(2, 1, 0); float& m012 = VFX_M(0, 1, 2); float& m201 = VFX_M(2, 0, 1);
+    float& m021 = VFX_M(0, 2, 1); float& m120 = VFX_M(1, 2, 0); float& m102 = VFX_M(1, 0, 2);
+    float& m211 = VFX_M(2, 1, 1); float& m121 = VFX_M(1, 2, 1); float& m112 = VFX_M(1, 1, 2);
+    float& m220 = VFX_M(2, 2, 0); float& m202 = VFX_M(2, 0, 2); float& m022 = VFX_M(0, 2, 2);
+    float& m122 = VFX_M(1, 2, 2); float& m212 = VFX_M(2, 1, 2); float& m221 = VFX_M(2, 2, 1);
+    float& m222 = VFX_M(2, 2, 2);
+    const float omega = R.omega;
+
+    // ---- second order (:323-387)
+    float mxxPyyPzz = m200 + m020 + m002;
+    float mxxMyy = m200 - m020;
+    float mxxMzz = m200 - m002;
+    const float Dxy = -3.f * omega * m110;
+    const float Dxz = -3.f * omega * m101;
+    const float Dyz = -3.f * omega * m011;
+    const float dxux = -0.5f * omega * (mxxMyy + mxxMzz) + 0.5f * (m000 - mxxPyyPzz);
+    const float dyuy = dxux + 1.5f * omega * mxxMyy;
+    const float dzuz = dxux + 1.5f * omega * mxxMzz;
+    mxxPyyPzz = m000 - 1.5f * (vx2 * dxux + vy2 * dyuy + vz2 * dzuz);
+    mxxMyy = R.om1 * mxxMyy - R.c3 * (vx2 * dxux - vy2 * dyuy);
+    mxxMzz = R.om1 * mxxMzz - R.c3 * (vx2 * dxux - vz2 * dzuz);
+    m011 *= R.om1;
+    m101 *= R.om1;
+    m110 *= R.om1;
+
+    // ---- third order with limiter: x (1-O) lambda / (|x| + lambda) (:397-410)
+    float mxxyPyzz = m210 + m012, mxxyMyzz = m210 - m012;
+    float mxxzPyyz = m201 + m021, mxxzMyyz = m201 - m021;
+    float mxyyPxzz = m120 + m102, mxyyMxzz = m120 - m102;
+    m111 *= R.limD1 * rcpf(fabsf(m111) + R.limD);
+    mxxyPyzz *= R.limP1 * rcpf(fabsf(mxxyPyzz) + R.limP);
+    mxxyMyzz *= R.limM1 * rcpf(fabsf(mxxyMyzz) + R.limM);
+    mxxzPyyz *= R.limP1 * rcpf(fabsf(mxxzPyyz) + R.limP);
+    mxxzMyyz *= R.limM1 * rcpf(fabsf(mxxzMyyz) + R.limM);
+    mxyyPxzz *= R.limP1 * rcpf(fabsf(mxyyPxzz) + R.limP);
+    mxyyMxzz *= R.limM1 * rcpf(fabsf(mxyyMxzz) + R.limM);
+
+    m200 = c1o3 * (mxxMyy + mxxMzz + mxxPyyPzz);
+    m020 = c1o3 * (-2.f * mxxMyy + mxxMzz + mxxPyyPzz);
+    m002 = c1o3 * (mxxMyy - 2.f * mxxMzz + mxxPyyPzz);
+    m210 = 0.5f * (mxxyMyzz + mxxyPyzz);
+    m012 = 0.5f * (-mxxyMyzz + mxxyPyzz);
+    m201 = 0.5f * (mxxzMyyz + mxxzPyyz);
+    m021 = 0.5f * (-mxxzMyyz + mxxzPyyz);
+    m120 = 0.5f * (mxyyMxzz + mxyyPxzz);
+    m102 = 0.5f * (-mxyyMxzz + mxyyPxzz);
+
+    // ---- fourth order at the modified equilibrium, fifth/sixth cumulants zero (:443-505)
+    const float R1 = oneOverRho;
+    const float dr9 = c1o9 * drho;
+    m211 = -R.B4 * Dyz + ((m200 + c1o3) * m011 + 2.f * m110 * m101) * R1;
+    m121 = -R.B4 * Dxz + ((m020 + c1o3) * m101 + 2.f * m110 * m011) * R1;
+    m112 = -R.B4 * Dxy + ((m002 + c1o3) * m110 + 2.f * m101 * m011) * R1;
+    m220 = -R.A4 * (dyuy + dxux) + ((m200 * m020 + 2.f * m110 * m110) + c1o3 * (m200 + m020) - dr9) * R1;
+    m202 = -R.A4 * (dxux + dzuz) + ((m200 * m002 + 2.f * m101 * m101) + c1o3 * (m200 + m002) - dr9) * R1;
+    m022 = -R.A4 * (dyuy + dzuz) + ((m002 * m020 + 2.f * m011 * m011) + c1o3 * (m002 + m020) - dr9) * R1;
+    m122 = ((m002 * m120 + m020 * m102 + 4.f * m011 * m111 + 2.f * (m101 * m021 + m110 * m012)) + c1o3 * (m120 + m102)) * R1;
+    m212 = ((m002 * m210 + m200 * m012 + 4.f * m101 * m111 + 2.f * (m011 * m201 + m110 * m102)) + c1o3 * (m210 + m012)) * R1;
+    m221 = ((m200 * m021 + m020 * m201 + 4.f * m110 * m111 + 2.f * (m101 * m120 + m011 * m210)) + c1o3 * (m021 + m201)) * R1;
+    const float R2 = R1 * R1;
+    const float s2 = m002 + m020 + m200;
+    m222 = -((-4.f * m111 * m111 - (m200 * m022 + m020 * m202 + m002 * m220) -
+              4.f * (m011 * m211 + m101 * m121 + m110 * m112) - 2.f * (m120 * m102 + m210 * m012 + m201 * m021)) * R1 +
+             (4.f * (m101 * m101 * m020 + m011 * m011 * m200 + m110 * m110 * m002) + 2.f * (m200 * m020 * m002) +
+              16.f * m110 * m101 * m011) * R2 -
+             c1o3 * (m022 + m202 + m220) * R1 - c1o9 * s2 * R1 +
+             (2.f * (m101 * m101 + m011 * m011 + m110 * m110) + (m002 * m020 + m002 * m200 + m020 * m200) + c1o3 * s2) *
+                 R2 * c2o3 +
+             c1o27 * ((drho * drho - drho) * R2));
+
+    m100 = -m100;
+    m010 = -m010;
+    m001 = -m001;
+
+    // ---- backward: x, y, z
+    {
+        const float hvm = 0.5f * (vx2 - vx), hvp = 0.5f * (vx2 + vx), vmh = vx - 0.5f, vph = vx + 0.5f, tv = 2.f * vx;
+#pragma unroll
+        for (int b = 0; b < 3; ++b)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float k = (b == 1 || c == 1) ? 0.f : (b == 0 && c == 0) ? 1.f : (b == 2 && c == 2) ? c1o9 : c1o3;
+                bwdf(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), k, vx2, hvm, hvp, vmh, vph, tv);
+            }
+    }
+    {
+        const float hvm = 0.5f * (vy2 - vy), hvp = 0.5f * (vy2 + vy), vmh = vy - 0.5f, vph = vy + 0.5f, tv = 2.f * vy;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float k = (c == 1) ? 0.f : wv[i] * (c == 0 ? 1.f : c1o3);
+                bwdf(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), k, vy2, hvm, hvp, vmh, vph, tv);
+            }
+    }
+    {
+        const float hvm = 0.5f * (vz2 - vz), hvp = 0.5f * (vz2 + vz), vmh = vz - 0.5f, vph = vz + 0.5f, tv = 2.f * vz;
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                bwdf(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), wv[i] * wv[j], vz2, hvm, hvp, vmh, vph, tv);
+    }
+    mv[0] = drho;
+    mv[1] = vx;
+    mv[2] = vy;
+    mv[3] = vz;
 }
 
 #undef VFX_M

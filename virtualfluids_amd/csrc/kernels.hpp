@@ -130,10 +130,16 @@ __device__ __forceinline__ void scatter_pre(const Lat& L, const uint32_t (&kk)[8
     }
 }
 
-template <typename T, int COLL> __device__ __forceinline__ void collide(T (&m)[27], const Rates<T>& R, T (&mv)[4])
+// COLL: VFX_K17 / VFX_BGK; FAST selects the reduced-operation K17 (FP32 only)
+template <typename T, int COLL, bool FAST>
+__device__ __forceinline__ void collide(T (&m)[27], const Rates<T>& R, T (&mv)[4])
 {
-    if constexpr (COLL == VFX_K17) k17<T>(m, R, mv);
-    else bgk<T>(m, R.omega, mv);
+    if constexpr (COLL == VFX_K17) {
+        if constexpr (FAST && sizeof(T) == 4) k17_fast(m, R, mv);
+        else k17<T>(m, R, mv);
+    } else {
+        bgk<T>(m, R.omega, mv);
+    }
 }
 
 template <typename T> __device__ __forceinline__ void write_macro(const Lat& L, uint32_t k, const T (&mv)[4])
@@ -147,7 +153,7 @@ template <typename T> __device__ __forceinline__ void write_macro(const Lat& L, 
 // ------------------------------------------------------ collide + stream ----
 // Direct (structured) addressing: one block covers `blockDim.x` consecutive
 // nodes of one x row, so every row index below is wave-uniform (scalar ALU).
-template <typename S, int COLL, bool EVEN>
+template <typename S, int COLL, bool EVEN, bool FAST>
 __global__ void __launch_bounds__(256) k_step_direct(Lat L, Rates<typename Compute<S>::T> R)
 {
     using T = typename Compute<S>::T;
@@ -160,14 +166,14 @@ __global__ void __launch_bounds__(256) k_step_direct(Lat L, Rates<typename Compu
     if (is_solid(L, kk[0])) return;
     T m[27], mv[4];
     gather<S, T, EVEN>(L, kk, m);
-    collide<T, COLL>(m, R, mv);
+    collide<T, COLL, FAST>(m, R, mv);
     scatter_post<S, T, EVEN>(L, kk, m);
     if (L.write_macro) write_macro<T>(L, kk[0], mv);
 }
 
 // VirtualFluids sparse addressing over the fluid-node list
 // (RunCollision.cuh:74-143 semantics).
-template <typename S, int COLL, bool EVEN>
+template <typename S, int COLL, bool EVEN, bool FAST>
 __global__ void __launch_bounds__(256) k_step_sparse(Lat L, Rates<typename Compute<S>::T> R)
 {
     using T = typename Compute<S>::T;
@@ -179,7 +185,7 @@ __global__ void __launch_bounds__(256) k_step_sparse(Lat L, Rates<typename Compu
     nbrs_sparse(L, k, kk);
     T m[27], mv[4];
     gather<S, T, EVEN>(L, kk, m);
-    collide<T, COLL>(m, R, mv);
+    collide<T, COLL, FAST>(m, R, mv);
     scatter_post<S, T, EVEN>(L, kk, m);
     if (L.write_macro) write_macro<T>(L, kk[0], mv);
 }
@@ -266,7 +272,31 @@ __global__ void __launch_bounds__(256) k_write(Lat L, int direct, const double* 
     scatter_pre<S, T, EVEN>(L, kk, m);
 }
 
-// macroscopic values of the pre-collision state (pyramid sums, FP64 output)
+// FP64 density deviation and momentum of one node (pyramid sums of
+// MacroscopicQuantities.h:55-85 on the populations widened to double)
+template <typename T>
+__device__ __forceinline__ void node_moments(const T (&m)[27], double& r, double& jx, double& jy, double& jz)
+{
+    double f[27];
+#pragma unroll
+    for (int s = 0; s < 27; ++s) f[dir_at(s)] = (double)m[s];
+    const double corners = ((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]));
+    const double edges = (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P]))) +
+                         ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]));
+    const double faces = ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0])) + (f[d00M] + f[d00P]);
+    r = ((corners + edges) + f[d000]) + faces;
+    jx = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dPMM] - f[dMPP]) + (f[dPPM] - f[dMMP]))) +
+         (((f[dP0M] - f[dM0P]) + (f[dP0P] - f[dM0M])) + ((f[dPM0] - f[dMP0]) + (f[dPP0] - f[dMM0]))) +
+         (f[dP00] - f[dM00]);
+    jy = (((f[dPPP] - f[dMMM]) + (f[dMPM] - f[dPMP])) + ((f[dMPP] - f[dPMM]) + (f[dPPM] - f[dMMP]))) +
+         (((f[d0PM] - f[d0MP]) + (f[d0PP] - f[d0MM])) + ((f[dMP0] - f[dPM0]) + (f[dPP0] - f[dMM0]))) +
+         (f[d0P0] - f[d0M0]);
+    jz = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dMPP] - f[dPMM]) + (f[dMMP] - f[dPPM]))) +
+         (((f[d0MP] - f[d0PM]) + (f[d0PP] - f[d0MM])) + ((f[dM0P] - f[dP0M]) + (f[dP0P] - f[dM0M]))) +
+         (f[d00P] - f[d00M]);
+}
+
+// macroscopic values of the pre-collision state (FP64 output)
 template <typename S, bool EVEN>
 __global__ void __launch_bounds__(256) k_macro(Lat L, int direct, double* drho, double* vx, double* vy, double* vz)
 {
@@ -276,29 +306,182 @@ __global__ void __launch_bounds__(256) k_macro(Lat L, int direct, double* drho, 
     if (!NodeIter::get(L, i, kk, direct)) return;
     T m[27];
     gather<S, T, EVEN>(L, kk, m);
-    double f[27];
-#pragma unroll
-    for (int s = 0; s < 27; ++s) f[dir_at(s)] = (double)m[s];
-    const double corners = ((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]));
-    const double edges = (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P]))) +
-                         ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]));
-    const double faces = ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0])) + (f[d00M] + f[d00P]);
-    const double r = ((corners + edges) + f[d000]) + faces;
-    const double jx = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dPMM] - f[dMPP]) + (f[dPPM] - f[dMMP]))) +
-                      (((f[dP0M] - f[dM0P]) + (f[dP0P] - f[dM0M])) + ((f[dPM0] - f[dMP0]) + (f[dPP0] - f[dMM0]))) +
-                      (f[dP00] - f[dM00]);
-    const double jy = (((f[dPPP] - f[dMMM]) + (f[dMPM] - f[dPMP])) + ((f[dMPP] - f[dPMM]) + (f[dPPM] - f[dMMP]))) +
-                      (((f[d0PM] - f[d0MP]) + (f[d0PP] - f[d0MM])) + ((f[dMP0] - f[dPM0]) + (f[dPP0] - f[dMM0]))) +
-                      (f[d0P0] - f[d0M0]);
-    const double jz = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dMPP] - f[dPMM]) + (f[dMMP] - f[dPPM]))) +
-                      (((f[d0MP] - f[d0PM]) + (f[d0PP] - f[d0MM])) + ((f[dM0P] - f[dP0M]) + (f[dP0P] - f[dM0M]))) +
-                      (f[d00P] - f[d00M]);
+    double r, jx, jy, jz;
+    node_moments<T>(m, r, jx, jy, jz);
     const double oor = 1.0 / (r + 1.0);
     const uint32_t k = kk[0];
     drho[k] = r;
     vx[k] = jx * oor;
     vy[k] = jy * oor;
     vz[k] = jz * oor;
+}
+
+// ---------------------------------------------------------- reductions ----
+// Deterministic two-pass FP64 sums: every block sums a fixed chunk of
+// kRedBlock * kRedPerThread nodes (per-thread sequential, then a wave64
+// butterfly and a fixed-order sum of the four waves); a single block then
+// adds the per-block partials in index order.
+constexpr int kRedBlock = 256;
+constexpr int kRedPerThread = 8;
+constexpr int kRedChunk = kRedBlock * kRedPerThread;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV])
+{
+    __shared__ double part[kRedBlock / 64][NV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+        v[c] = wave_sum(v[c]);
+        if (lane == 0) part[wave][c] = v[c];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int c = 0; c < NV; ++c) {
+            double t = part[0][c];
+            for (int w = 1; w < kRedBlock / 64; ++w) t += part[w][c];
+            v[c] = t;
+        }
+    }
+}
+
+// per node: drho, jx, jy, jz (mass deviation and momentum of the pre-collision state)
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(kRedBlock) k_reduce_mass(Lat L, int direct, double* partial)
+{
+    using T = typename Compute<S>::T;
+    double acc[4] = {0, 0, 0, 0};
+    for (int j = 0; j < kRedPerThread; ++j) {
+        const uint64_t i = (uint64_t)blockIdx.x * kRedChunk + (uint64_t)j * kRedBlock + threadIdx.x;
+        uint32_t kk[8];
+        if (!NodeIter::get(L, i, kk, direct)) continue;
+        T m[27];
+        gather<S, T, EVEN>(L, kk, m);
+        double r, jx, jy, jz;
+        node_moments<T>(m, r, jx, jy, jz);
+        acc[0] += r;
+        acc[1] += jx;
+        acc[2] += jy;
+        acc[3] += jz;
+    }
+    block_sum<4>(acc);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 4; ++c) partial[(uint64_t)blockIdx.x * 4 + c] = acc[c];
+}
+
+template <int NV> __global__ void __launch_bounds__(kRedBlock) k_reduce_finish(const double* partial, uint32_t nblocks, double* out)
+{
+    double acc[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) acc[c] = 0.0;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += kRedBlock)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) acc[c] += partial[(uint64_t)b * NV + c];
+    block_sum<NV>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < NV; ++c) out[c] = acc[c];
+}
+
+// Structured-box neighbour along one axis (periodic index wrap), -1 if solid.
+__device__ __forceinline__ int64_t fluid_nbr(const Lat& L, uint32_t x, uint32_t y, uint32_t z, int ax, int step)
+{
+    int64_t c[3] = {x, y, z};
+    const int64_t n[3] = {L.nx, L.ny, L.nz};
+    c[ax] = ((c[ax] + step) % n[ax] + n[ax]) % n[ax];
+    const uint32_t k = (uint32_t)(c[0] + n[0] * (c[1] + n[1] * c[2]));
+    return is_solid(L, k) ? -1 : (int64_t)k;
+}
+
+// Kinetic energy 1/2 rho |u|^2 summed over fluid nodes whose six axis
+// neighbours are fluid, and their count (KineticEnergyAnalyzer.cu:137-154).
+// Enstrophy 1/2 rho |curl u|^2 on the same nodes with central differences of
+// order 8, reduced to 6/4/2 where the stencil meets a wall
+// (EnstrophyAnalyzer.cu:134-336).
+template <bool ENSTROPHY>
+__global__ void __launch_bounds__(kRedBlock) k_reduce_energy(Lat L, const double* rho, const double* vx, const double* vy,
+                                                             const double* vz, double* partial)
+{
+    double acc[2] = {0, 0};
+    for (int j = 0; j < kRedPerThread; ++j) {
+        const uint64_t i = (uint64_t)blockIdx.x * kRedChunk + (uint64_t)j * kRedBlock + threadIdx.x;
+        if (i >= L.N) continue;
+        const uint32_t k = (uint32_t)i;
+        if (is_solid(L, k)) continue;
+        const uint32_t x = k % L.nx, r = k / L.nx, y = r % L.ny, z = r / L.ny;
+        bool ok = true;
+        for (int ax = 0; ax < 3 && ok; ++ax)
+            if (fluid_nbr(L, x, y, z, ax, 1) < 0 || fluid_nbr(L, x, y, z, ax, -1) < 0) ok = false;
+        if (!ok) continue;
+        if (!ENSTROPHY) {
+            acc[0] += 0.5 * (vx[k] * vx[k] + vy[k] * vy[k] + vz[k] * vz[k]) * (rho[k] + 1.0);
+            acc[1] += 1.0;
+            continue;
+        }
+        // derivative of field v along axis ax at node k
+        auto deriv = [&](const double* v, int ax) {
+            int order = 8;
+            int64_t p[5], m[5];
+            for (int s = 1; s <= 4; ++s) {
+                p[s] = fluid_nbr(L, x, y, z, ax, s);
+                m[s] = fluid_nbr(L, x, y, z, ax, -s);
+            }
+            if (p[2] < 0 || m[2] < 0) order = 2;
+            if ((p[3] < 0 || m[3] < 0) && order > 4) order = 4;
+            if ((p[4] < 0 || m[4] < 0) && order > 6) order = 6;
+            auto d = [&](int s) { return v[p[s]] - v[m[s]]; };
+            if (order == 2) return d(1) / 2.0;
+            if (order == 4) return ((8.0 * v[p[1]] - 8.0 * v[m[1]]) - d(2)) / 12.0;
+            if (order == 6) return ((5.0 * 9.0) * d(1) - 9.0 * d(2) + d(3)) / (6.0 * 10.0);
+            return ((28.0 * 8.0) * d(1) - (7.0 * 8.0) * d(2) + (8.0 * 4.0 * (1.0 / 3.0)) * d(3) - d(4)) / (7.0 * 10.0 * 4.0);
+        };
+        const double dyvz = deriv(vz, 1), dzvy = deriv(vy, 2);
+        const double dzvx = deriv(vx, 2), dxvz = deriv(vz, 0);
+        const double dxvy = deriv(vy, 0), dyvx = deriv(vx, 1);
+        const double wx = dyvz - dzvy, wy = dzvx - dxvz, wz = dxvy - dyvx;
+        acc[0] += 0.5 * (rho[k] + 1.0) * (wx * wx + wy * wy + wz * wz);
+        acc[1] += 1.0;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        partial[(uint64_t)blockIdx.x * 2] = acc[0];
+        partial[(uint64_t)blockIdx.x * 2 + 1] = acc[1];
+    }
+}
+
+// Running FP64 sums for time averages (CalcTurbulenceIntensity,
+// TurbulenceIntensity.cu:45-85): planes 0-2 sum u, planes 3-8 sum
+// u_x u_x, u_y u_y, u_z u_z, u_x u_y, u_x u_z, u_y u_z.
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) k_stats(Lat L, int direct, double* sums)
+{
+    using T = typename Compute<S>::T;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t kk[8];
+    if (!NodeIter::get(L, i, kk, direct)) return;
+    T m[27];
+    gather<S, T, EVEN>(L, kk, m);
+    double r, jx, jy, jz;
+    node_moments<T>(m, r, jx, jy, jz);
+    const double oor = 1.0 / (r + 1.0);
+    const double ux = jx * oor, uy = jy * oor, uz = jz * oor;
+    const uint64_t k = kk[0], N = L.N;
+    sums[k] += ux;
+    sums[N + k] += uy;
+    sums[2 * N + k] += uz;
+    sums[3 * N + k] += ux * ux;
+    sums[4 * N + k] += uy * uy;
+    sums[5 * N + k] += uz * uz;
+    sums[6 * N + k] += ux * uy;
+    sums[7 * N + k] += ux * uz;
+    sums[8 * N + k] += uy * uz;
 }
 
 // -------------------------------------------------- boundary conditions ----
@@ -379,34 +562,37 @@ constexpr int kBlock = 256;
 
 inline unsigned grid_for(uint64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-template <typename S> hipError_t launch_step(const Lat& L, int direct, int coll, bool even, double omega, const double* lim,
-                                             const double* force, hipStream_t st)
+template <typename S, int COLL, bool EVEN, bool FAST>
+hipError_t launch_step_t(const Lat& L, int direct, const Rates<typename Compute<S>::T>& R, hipStream_t st)
 {
-    using T = typename Compute<S>::T;
-    const Rates<T> R = make_rates<T>(omega, lim, force);
     if (direct) {
         const unsigned grid = L.chunks * L.ny * L.nz;
         const unsigned block = (unsigned)((L.nx + L.chunks - 1) / L.chunks);
         const unsigned b = block <= 64 ? 64 : block <= 128 ? 128 : 256;
-        if (coll == VFX_K17) {
-            if (even) hipLaunchKernelGGL((k_step_direct<S, VFX_K17, true>), dim3(grid), dim3(b), 0, st, L, R);
-            else hipLaunchKernelGGL((k_step_direct<S, VFX_K17, false>), dim3(grid), dim3(b), 0, st, L, R);
-        } else {
-            if (even) hipLaunchKernelGGL((k_step_direct<S, VFX_BGK, true>), dim3(grid), dim3(b), 0, st, L, R);
-            else hipLaunchKernelGGL((k_step_direct<S, VFX_BGK, false>), dim3(grid), dim3(b), 0, st, L, R);
-        }
+        hipLaunchKernelGGL((k_step_direct<S, COLL, EVEN, FAST>), dim3(grid), dim3(b), 0, st, L, R);
     } else {
         if (L.nfluid == 0) return hipSuccess;
-        const unsigned grid = grid_for(L.nfluid, kBlock);
-        if (coll == VFX_K17) {
-            if (even) hipLaunchKernelGGL((k_step_sparse<S, VFX_K17, true>), dim3(grid), dim3(kBlock), 0, st, L, R);
-            else hipLaunchKernelGGL((k_step_sparse<S, VFX_K17, false>), dim3(grid), dim3(kBlock), 0, st, L, R);
-        } else {
-            if (even) hipLaunchKernelGGL((k_step_sparse<S, VFX_BGK, true>), dim3(grid), dim3(kBlock), 0, st, L, R);
-            else hipLaunchKernelGGL((k_step_sparse<S, VFX_BGK, false>), dim3(grid), dim3(kBlock), 0, st, L, R);
-        }
+        hipLaunchKernelGGL((k_step_sparse<S, COLL, EVEN, FAST>), dim3(grid_for(L.nfluid, kBlock)), dim3(kBlock), 0, st,
+                           L, R);
     }
     return hipGetLastError();
+}
+
+// variant: 0 = default (FP64: reference operation order; FP32/FP16: reduced form), 1 = reference order
+template <typename S> hipError_t launch_step(const Lat& L, int direct, int coll, bool even, int variant, double omega,
+                                             const double* lim, const double* force, hipStream_t st)
+{
+    using T = typename Compute<S>::T;
+    const Rates<T> R = make_rates<T>(omega, lim, force);
+    const bool fast = sizeof(T) == 4 && variant == 0 && coll == VFX_K17;
+    if (coll == VFX_K17) {
+        if (fast) return even ? launch_step_t<S, VFX_K17, true, true>(L, direct, R, st)
+                              : launch_step_t<S, VFX_K17, false, true>(L, direct, R, st);
+        return even ? launch_step_t<S, VFX_K17, true, false>(L, direct, R, st)
+                    : launch_step_t<S, VFX_K17, false, false>(L, direct, R, st);
+    }
+    return even ? launch_step_t<S, VFX_BGK, true, false>(L, direct, R, st)
+                : launch_step_t<S, VFX_BGK, false, false>(L, direct, R, st);
 }
 
 template <typename S> hipError_t launch_init(const Lat& L, int direct, bool even, const double* drho, const double* ux,
@@ -445,6 +631,28 @@ template <typename S> hipError_t launch_macro(const Lat& L, int direct, bool eve
     return hipGetLastError();
 }
 
+inline uint32_t reduce_blocks(uint64_t n) { return (uint32_t)((n + kRedChunk - 1) / kRedChunk); }
+
+template <typename S> hipError_t launch_reduce_mass(const Lat& L, int direct, bool even, double* ws, double* out, hipStream_t st)
+{
+    const uint64_t n = direct ? L.N : L.nfluid;
+    const uint32_t nb = reduce_blocks(n ? n : 1);
+    if (even) hipLaunchKernelGGL((k_reduce_mass<S, true>), dim3(nb), dim3(kRedBlock), 0, st, L, direct, ws);
+    else hipLaunchKernelGGL((k_reduce_mass<S, false>), dim3(nb), dim3(kRedBlock), 0, st, L, direct, ws);
+    hipLaunchKernelGGL((k_reduce_finish<4>), dim3(1), dim3(kRedBlock), 0, st, ws, nb, out);
+    return hipGetLastError();
+}
+
+template <typename S> hipError_t launch_stats(const Lat& L, int direct, bool even, double* sums, hipStream_t st)
+{
+    const uint64_t n = direct ? L.N : L.nfluid;
+    if (n == 0) return hipSuccess;
+    const dim3 g(grid_for(n, kBlock)), b(kBlock);
+    if (even) hipLaunchKernelGGL((k_stats<S, true>), g, b, 0, st, L, direct, sums);
+    else hipLaunchKernelGGL((k_stats<S, false>), g, b, 0, st, L, direct, sums);
+    return hipGetLastError();
+}
+
 template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kind, bool even, hipStream_t st)
 {
     if (B.n == 0) return hipSuccess;
@@ -458,20 +666,25 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
 }
 
 #define VFX_INSTANTIATE_LAUNCHERS(S)                                                                                   \
-    template hipError_t launch_step<S>(const Lat&, int, int, bool, double, const double*, const double*, hipStream_t); \
+    template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*,           \
+                                       hipStream_t);                                                                   \
     template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,             \
                                        const double*, hipStream_t);                                                    \
     template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);                      \
     template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t);       \
-    template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);
+    template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);                            \
+    template hipError_t launch_reduce_mass<S>(const Lat&, int, bool, double*, double*, hipStream_t);              \
+    template hipError_t launch_stats<S>(const Lat&, int, bool, double*, hipStream_t);
 
 #define VFX_EXTERN_LAUNCHERS(S)                                                                                        \
-    extern template hipError_t launch_step<S>(const Lat&, int, int, bool, double, const double*, const double*,        \
+    extern template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*,   \
                                               hipStream_t);                                                            \
     extern template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,      \
                                               const double*, hipStream_t);                                             \
     extern template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);               \
     extern template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t); \
-    extern template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);
+    extern template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);                     \
+    extern template hipError_t launch_reduce_mass<S>(const Lat&, int, bool, double*, double*, hipStream_t);       \
+    extern template hipError_t launch_stats<S>(const Lat&, int, bool, double*, hipStream_t);
 
 } // namespace vfx

@@ -58,10 +58,13 @@ class Lattice:
 
     def __init__(self, dims=None, *, storage="fp64", omega=1.0, limiter=(0.01, 0.01, 0.01),
                  force=(0.0, 0.0, 0.0), solid=None, fp16_scale=None, neighbors=None, fluid_nodes=None,
-                 n_nodes=None, device="cuda"):
+                 n_nodes=None, device="cuda", variant="default"):
         if storage not in STORAGE:
             raise ValueError(f"storage must be one of {sorted(STORAGE)}")
         self.storage = storage
+        if variant not in ("default", "plain"):
+            raise ValueError("variant must be 'default' or 'plain'")
+        self.variant = variant
         self.device = torch.device(device)
         self.omega = float(omega)
         self.limiter = tuple(float(x) for x in limiter)
@@ -121,6 +124,7 @@ class Lattice:
         s.force[:] = list(self.force)
         if self.macro is not None:
             s.rho, s.vx, s.vy, s.vz = (t.data_ptr() for t in self.macro)
+        s.kernel_variant = 0 if self.variant == "default" else 1
         return s
 
     @property
@@ -176,6 +180,69 @@ class Lattice:
         V.check(V.lib().vfx_macroscopic(C.byref(s), int(bool(even)), *(o.data_ptr() for o in outs),
                                         _stream(stream)), "vfx_macroscopic")
         return outs
+
+
+    # ------------------------------------------------------- diagnostics --
+    def _workspace(self):
+        s = self.struct()
+        nbytes = int(V.lib().vfx_reduce_workspace_bytes(C.byref(s)))
+        ws = getattr(self, "_ws", None)
+        if ws is None or ws.numel() * 8 < nbytes:
+            ws = torch.empty((nbytes + 7) // 8, dtype=torch.float64, device=self.device)
+            self._ws = ws
+        return ws
+
+    def mass_momentum(self, even=True, stream=None) -> torch.Tensor:
+        """FP64 {sum drho, sum jx, sum jy, sum jz} of the pre-collision state (device tensor)."""
+        out = torch.zeros(4, dtype=torch.float64, device=self.device)
+        ws = self._workspace()
+        s = self.struct()
+        V.check(V.lib().vfx_reduce_mass_momentum(C.byref(s), int(bool(even)), ws.data_ptr(), out.data_ptr(),
+                                                 _stream(stream)), "vfx_reduce_mass_momentum")
+        return out
+
+    def _energy(self, fn, name, even, fields, stream):
+        if fields is None:
+            fields = self.macroscopic(even, stream)
+        out = torch.zeros(2, dtype=torch.float64, device=self.device)
+        ws = self._workspace()
+        s = self.struct()
+        V.check(fn(C.byref(s), *(f.data_ptr() for f in fields), ws.data_ptr(), out.data_ptr(), _stream(stream)), name)
+        return out
+
+    def kinetic_energy(self, even=True, fields=None, stream=None) -> torch.Tensor:
+        """{sum 1/2 rho |u|^2, count} over fluid nodes with a fluid 7-stencil (KineticEnergyAnalyzer)."""
+        return self._energy(V.lib().vfx_kinetic_energy, "vfx_kinetic_energy", even, fields, stream)
+
+    def enstrophy(self, even=True, fields=None, stream=None) -> torch.Tensor:
+        """{sum 1/2 rho |curl u|^2, count} (EnstrophyAnalyzer)."""
+        return self._energy(V.lib().vfx_enstrophy, "vfx_enstrophy", even, fields, stream)
+
+
+class Statistics:
+    """FP64 running sums of u and u_i u_j (CalcTurbulenceIntensity); mean and
+    Reynolds stress on demand."""
+
+    def __init__(self, lattice: Lattice):
+        self.lattice = lattice
+        self.sums = torch.zeros(9, lattice.n_nodes, dtype=torch.float64, device=lattice.device)
+        self.samples = 0
+
+    def accumulate(self, even, stream=None):
+        s = self.lattice.struct()
+        V.check(V.lib().vfx_accumulate_stats(C.byref(s), int(bool(even)), self.sums.data_ptr(), _stream(stream)),
+                "vfx_accumulate_stats")
+        self.samples += 1
+
+    def mean(self) -> torch.Tensor:
+        return self.sums[:3] / max(self.samples, 1)
+
+    def reynolds_stress(self) -> torch.Tensor:
+        """(6, n): <u'u'> for xx, yy, zz, xy, xz, yz."""
+        m = self.mean()
+        t = max(self.samples, 1)
+        pairs = [(0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2)]
+        return torch.stack([self.sums[3 + c] / t - m[a] * m[b] for c, (a, b) in enumerate(pairs)])
 
 
 class BoundaryCondition:
