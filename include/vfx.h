@@ -88,8 +88,12 @@ typedef struct vfx_lattice {
 
 /* boundary-condition kinds (src/gpu/core/BoundaryConditions/BoundaryConditionFactory.h:60-132) */
 typedef enum {
-    VFX_BC_VELOCITY_BOUNCE_BACK = 0, /* VelocityBounceBack_Device, Velocity/VelocityBounceBack.cu:44 (u = 0: NoSlipBounceBack) */
-    VFX_BC_NOSLIP_INTERPOLATED = 1,  /* NoSlipInterpolatedCompressible_Device, NoSlip/NoSlipInterpolatedCompressible.cu:44 */
+    VFX_BC_VELOCITY_BOUNCE_BACK = 0,  /* VelocityBounceBack_Device, Velocity/VelocityBounceBack.cu:44 (u = 0: NoSlipBounceBack) */
+    VFX_BC_NOSLIP_INTERPOLATED = 1,   /* NoSlipInterpolatedCompressible_Device, NoSlip/NoSlipInterpolatedCompressible.cu:44 */
+    VFX_BC_VELOCITY_INTERPOLATED = 2, /* VelocityInterpolatedCompressible_Device, Velocity/VelocityInterpolatedCompressible.cu:44 */
+    VFX_BC_SLIP = 3,                  /* SlipCompressible_Device, Slip/SlipCompressible.cu:45 */
+    VFX_BC_PRESSURE_NEQ = 4           /* PressureNonEquilibriumCompressible_Device, Pressure/PressureNonEquilibriumCompressible.cu:45
+                                         (pre-collision: apply with the parity of the step it precedes) */
 } vfx_bc_kind;
 
 /* QforBoundaryConditions (src/gpu/core/Calculation/Calculation.h:105-117) */
@@ -102,6 +106,9 @@ typedef struct vfx_bc {
     const double* vy;
     const double* vz;
     int32_t cpu_float_velocity; /* 1: wall term rounded as VirtualFluids' CPU BoundaryConditions does (float storage) */
+    const double* rho;         /* VFX_BC_PRESSURE_NEQ: prescribed density deviation per boundary node (RhoBC) */
+    const uint32_t* k_neighbor;/* VFX_BC_PRESSURE_NEQ: inward neighbour node of each boundary node */
+    int32_t normal;            /* VFX_BC_PRESSURE_NEQ: outward direction of the boundary (e.g. 1 = dP00) */
 } vfx_bc;
 
 /* ---------------------------------------------------------------- kernels */
@@ -115,9 +122,10 @@ vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t
 vfx_status vfx_k17_step(const vfx_lattice* lat, int32_t is_even_timestep, int32_t collision_template, vfx_stream stream);
 vfx_status vfx_bgk_step(const vfx_lattice* lat, int32_t is_even_timestep, vfx_stream stream);
 
-/* Post-collision boundary condition on a BC node list, run after the
+/* Boundary condition on a BC node list.  Post-collision kinds run after the
  * collision of parity is_even_timestep and before the parity flip
- * (src/gpu/core/Calculation/UpdateGrid27.cpp:227-262). */
+ * (src/gpu/core/Calculation/UpdateGrid27.cpp:227-262); VFX_BC_PRESSURE_NEQ is
+ * pre-collision and runs after the flip with the new parity (:288-307). */
 vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_even_timestep, vfx_stream stream);
 
 /* InitNavierStokesCompressible_Device (src/gpu/core/PreProcessor/PreProcessorStrategy/

@@ -29,7 +29,8 @@ _u8p = C.POINTER(C.c_uint8)
 class OrcBC(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n", C.c_int32), ("node", C.POINTER(C.c_int64)),
                 ("q", C.POINTER(C.c_float)), ("ux", _dp), ("uy", _dp), ("uz", _dp),
-                ("cpu_float_velocity", C.c_int32)]
+                ("cpu_float_velocity", C.c_int32), ("rho", _dp), ("nbr", C.POINTER(C.c_int64)),
+                ("normal", C.c_int32)]
 
 
 def build(ref: bool | None = None) -> None:
@@ -158,26 +159,32 @@ def wall_term(d, ux, uy, uz, cpu_float=False):
 
 # ------------------------------------------------------------- lattices ---
 class BCGroup:
-    """Velocity bounce-back group in the VirtualFluids QforBoundaryConditions
-    shape: node indices, q[27][n] (link d is a wall link iff 0<=q<=1), and
-    per-node wall velocity."""
+    """Boundary group in the VirtualFluids QforBoundaryConditions shape: node
+    indices, q[27][n] (link d is a wall link iff 0<=q<=1), per-node wall
+    velocity; kind 0 velocity bounce-back, 1 no-slip interpolated, 2 velocity
+    interpolated, 3 slip, 4 non-equilibrium pressure (rho, nbr, normal)."""
 
-    def __init__(self, node, q, ux=None, uy=None, uz=None, cpu_float_velocity=False, kind=0):
+    def __init__(self, node, q=None, ux=None, uy=None, uz=None, cpu_float_velocity=False, kind=0, rho=None,
+                 nbr=None, normal=0):
         self.node = np.ascontiguousarray(node, dtype=np.int64)
-        self.q = np.ascontiguousarray(q, dtype=np.float32)
         n = self.node.size
+        self.q = np.ascontiguousarray(np.full((27, n), -1.0) if q is None else q, dtype=np.float32)
         assert self.q.shape == (27, n)
         z = np.zeros(n)
         self.ux = np.ascontiguousarray(z if ux is None else np.broadcast_to(ux, (n,)), dtype=np.float64)
         self.uy = np.ascontiguousarray(z if uy is None else np.broadcast_to(uy, (n,)), dtype=np.float64)
         self.uz = np.ascontiguousarray(z if uz is None else np.broadcast_to(uz, (n,)), dtype=np.float64)
+        self.rho = np.ascontiguousarray(z if rho is None else np.broadcast_to(rho, (n,)), dtype=np.float64)
+        self.nbr = np.ascontiguousarray(self.node if nbr is None else nbr, dtype=np.int64)
+        self.normal = int(normal)
         self.cpu_float_velocity = bool(cpu_float_velocity)
         self.kind = kind
 
     def c(self):
         return OrcBC(self.kind, self.node.size, self.node.ctypes.data_as(C.POINTER(C.c_int64)),
                      self.q.ctypes.data_as(C.POINTER(C.c_float)), _p(self.ux), _p(self.uy), _p(self.uz),
-                     int(self.cpu_float_velocity))
+                     int(self.cpu_float_velocity), _p(self.rho), self.nbr.ctypes.data_as(C.POINTER(C.c_int64)),
+                     self.normal)
 
 
 def run(F, dims, nsteps, omega, collision="k17", solid=None, bcs=(), lim=LIMITER, force=(0.0, 0.0, 0.0),

@@ -45,6 +45,14 @@
 #include <omp.h>
 #endif
 
+enum {
+    ORC_BC_VELOCITY_BOUNCE_BACK = 0,
+    ORC_BC_NOSLIP_INTERPOLATED = 1,
+    ORC_BC_VELOCITY_INTERPOLATED = 2,
+    ORC_BC_SLIP = 3,
+    ORC_BC_PRESSURE_NEQ = 4
+};
+
 namespace {
 
 // ---------------------------------------------------------------- D3Q27 ----
@@ -490,13 +498,108 @@ double wall_term(int d, double ux, double uy, double uz, bool cpu_float)
     return coef * s;
 }
 
+// c_d . u spelled as the reference writes it (first non-zero component with
+// its sign, the others added/subtracted in x, y, z order)
+double proj(int d, double ux, double uy, double uz)
+{
+    const int c[3] = {CX[d], CY[d], CZ[d]};
+    const double u[3] = {ux, uy, uz};
+    double v = 0.0;
+    bool first = true;
+    for (int a = 0; a < 3; ++a) {
+        if (c[a] == 0) continue;
+        if (first) { v = c[a] > 0 ? u[a] : -u[a]; first = false; }
+        else v = c[a] > 0 ? v + u[a] : v - u[a];
+    }
+    return v;
+}
+
+// Interpolated bounce-back family on the post-collision populations p of one
+// boundary node (NoSlipInterpolatedCompressible.cu:44-378,
+// VelocityInterpolatedCompressible.cu, SlipCompressible.cu; formulas
+// KernelUtilities.h:171-197).  out[d'] receives the population that re-enters
+// the node along d' = inv(d) for every wall link d; returns a mask of written d'.
+void interp_bc(int kind, const double* p, const float* q, int n, int i, double omega, double VX, double VY, double VZ,
+               double* out, bool* written)
+{
+    const double drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] +
+                        p[d0MP] + p[d0PP] + p[d0MM] + p[dP0M] + p[dM0P] + p[dP0P] + p[dM0M] + p[dPM0] + p[dMP0] +
+                        p[dPP0] + p[dMM0] + p[d00P] + p[d00M] + p[d0P0] + p[d0M0] + p[dP00] + p[dM00] + p[d000];
+    const double vx1 = (((p[dPMP] - p[dMPM]) - (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                        ((p[dP0M] - p[dM0P]) + (p[dP0P] - p[dM0M])) + ((p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
+                        (p[dP00] - p[dM00])) / (1.0 + drho);
+    const double vx2 = ((-(p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                        ((p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + (-(p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
+                        (p[d0P0] - p[d0M0])) / (1.0 + drho);
+    const double vx3 = (((p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) + (p[dMMP] - p[dPPM])) +
+                        (-(p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + ((p[dP0P] - p[dM0M]) - (p[dP0M] - p[dM0P])) +
+                        (p[d00P] - p[d00M])) / (1.0 + drho);
+    const double cu_sq = c3o2 * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3) * (1.0 + drho);
+    bool fx = false, fy = false, fz = false;
+    for (int d = 1; d < 27; ++d) {
+        const double qd = (double)q[(int64_t)d * n + i];
+        if (!(qd >= 0.0 && qd <= 1.0)) continue;
+        const double w = weight(d);
+        const double vlb = proj(d, vx1, vx2, vx3);
+        const double feq = w * (drho + c9o2 * vlb * vlb * (1.0 + drho) - cu_sq);
+        const double f = p[d], fi = p[T.inv[d]];
+        const double a = (1.0 - qd) / (1.0 + qd) * (f - fi + (f + fi - 2.0 * feq * omega) / (1.0 - omega)) * c1o2;
+        double v;
+        if (kind == ORC_BC_NOSLIP_INTERPOLATED) {
+            v = a + (qd * (f + fi)) / (1.0 + qd);
+        } else {
+            double vbc;
+            if (kind == ORC_BC_SLIP) {
+                if (std::abs(CX[d]) + std::abs(CY[d]) + std::abs(CZ[d]) == 1) {
+                    if (CX[d]) fx = true;
+                    if (CY[d]) fy = true;
+                    if (CZ[d]) fz = true;
+                }
+                vbc = proj(d, fx ? 0.0 : vx1, fy ? 0.0 : vx2, fz ? 0.0 : vx3);
+            } else {
+                vbc = proj(d, VX, VY, VZ);
+            }
+            v = a + (qd * (f + fi) - 6.0 * w * vbc) / (1.0 + qd);
+        }
+        out[T.inv[d]] = v;
+        written[T.inv[d]] = true;
+    }
+}
+
+double feq_shift(int d, double drho, double vx1, double vx2, double vx3, double cusq)
+{
+    if (d == d000) return c8o27 * (drho - (drho + 1.0) * cusq);
+    const double cu = proj(d, vx1, vx2, vx3);
+    return weight(d) * (drho + (drho + 1.0) * (3.0 * cu + c9o2 * cu * cu - cusq));
+}
+
+// PressureNonEquilibriumCompressible.cu: neighbour populations with their
+// equilibrium replaced by the one of the prescribed density; returned for
+// every direction with c_d . normal <= 0.
+void pressure_bc(const double* fnb, double rho_bc, int normal, double* out, bool* written)
+{
+    double f[27];
+    std::memcpy(f, fnb, sizeof(f));
+    double drho1 = density(f);
+    const double oor = 1.0 / (drho1 + 1.0);
+    const double vx1 = mom_x(f) * oor, vx2 = mom_y(f) * oor, vx3 = mom_z(f) * oor;
+    const double cusq = c3o2 * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3);
+    for (int d = 0; d < 27; ++d) f[d] -= feq_shift(d, drho1, vx1, vx2, vx3, cusq);
+    drho1 = rho_bc;
+    for (int d = 0; d < 27; ++d) f[d] += feq_shift(d, drho1, vx1, vx2, vx3, cusq);
+    for (int d = 0; d < 27; ++d) {
+        if (CX[d] * CX[normal] + CY[d] * CY[normal] + CZ[d] * CZ[normal] > 0) continue;
+        out[d] = f[d];
+        written[d] = true;
+    }
+}
+
 } // namespace
 
 // ============================================================== C ABI =====
 extern "C" {
 
 enum { ORC_K17 = 0, ORC_BGK = 1 };
-enum { ORC_BC_VELOCITY_BOUNCE_BACK = 0 };
 
 typedef struct {
     int32_t kind;             // ORC_BC_*
@@ -507,6 +610,9 @@ typedef struct {
     const double* uy;
     const double* uz;
     int32_t cpu_float_velocity;
+    const double* rho;        // pressure: prescribed density deviation
+    const int64_t* nbr;       // pressure: inward neighbour node
+    int32_t normal;           // pressure: outward direction
 } orc_bc;
 
 void orc_k17_collide(double* f27, double omega, const double* lim3, const double* force3, double* macro4)
@@ -569,6 +675,13 @@ int orc_run(double* F, int nx, int ny, int nz, const uint8_t* solid, int collisi
         for (int b = 0; b < n_bc; ++b)
             for (int i = 0; i < bcs[b].n; ++i)
                 for (int d = 1; d < 27; ++d) {
+                    if (bcs[b].kind == ORC_BC_PRESSURE_NEQ) {
+                        // populations leaving through the pressure boundary are dropped
+                        const int nm = bcs[b].normal;
+                        if (CX[d] * CX[nm] + CY[d] * CY[nm] + CZ[d] * CZ[nm] > 0)
+                            covered[(size_t)bcs[b].node[i] * 27 + d] = 1;
+                        continue;
+                    }
                     const float q = bcs[b].q[(int64_t)d * bcs[b].n + i];
                     if (q >= 0.0f && q <= 1.0f) covered[(size_t)bcs[b].node[i] * 27 + d] = 1;
                 }
@@ -612,15 +725,24 @@ int orc_run(double* F, int nx, int ny, int nz, const uint8_t* solid, int collisi
         // re-enters node i against the wall link d)
         for (int b = 0; b < n_bc; ++b) {
             const orc_bc& bc = bcs[b];
+            if (bc.kind == ORC_BC_PRESSURE_NEQ) continue;
             for (int i = 0; i < bc.n; ++i) {
                 const int64_t k = bc.node[i];
                 const double ux = bc.ux ? bc.ux[i] : 0.0, uy = bc.uy ? bc.uy[i] : 0.0, uz = bc.uz ? bc.uz[i] : 0.0;
-                for (int d = 1; d < 27; ++d) {
-                    const float q = bc.q[(int64_t)d * bc.n + i];
-                    if (!(q >= 0.0f && q <= 1.0f)) continue;
-                    const double fd = post[(size_t)k * 27 + d];
-                    G[(size_t)k * 27 + T.inv[d]] = fd - wall_term(d, ux, uy, uz, bc.cpu_float_velocity != 0);
+                if (bc.kind == ORC_BC_VELOCITY_BOUNCE_BACK) {
+                    for (int d = 1; d < 27; ++d) {
+                        const float q = bc.q[(int64_t)d * bc.n + i];
+                        if (!(q >= 0.0f && q <= 1.0f)) continue;
+                        const double fd = post[(size_t)k * 27 + d];
+                        G[(size_t)k * 27 + T.inv[d]] = fd - wall_term(d, ux, uy, uz, bc.cpu_float_velocity != 0);
+                    }
+                    continue;
                 }
+                double out[27];
+                bool wr[27] = {false};
+                interp_bc(bc.kind, &post[(size_t)k * 27], bc.q, bc.n, i, omega, ux, uy, uz, out, wr);
+                for (int d = 0; d < 27; ++d)
+                    if (wr[d]) G[(size_t)k * 27 + d] = out[d];
             }
         }
         // solid nodes keep whatever they had (never read)
@@ -628,6 +750,22 @@ int orc_run(double* F, int nx, int ny, int nz, const uint8_t* solid, int collisi
             for (int64_t i = 0; i < N; ++i)
                 if (solid[i]) std::memcpy(&G[(size_t)i * 27], F + 27 * i, 27 * sizeof(double));
         std::memcpy(F, G.data(), (size_t)N * 27 * sizeof(double));
+        // pre-collision boundary conditions of the next step (after the parity
+        // flip, UpdateGrid27.cpp:288-307): read the inward neighbours first
+        for (int b = 0; b < n_bc; ++b) {
+            const orc_bc& bc = bcs[b];
+            if (bc.kind != ORC_BC_PRESSURE_NEQ) continue;
+            std::vector<double> outs((size_t)bc.n * 27);
+            std::vector<uint8_t> wrs((size_t)bc.n * 27);
+            for (int i = 0; i < bc.n; ++i) {
+                bool wr[27] = {false};
+                pressure_bc(F + 27 * bc.nbr[i], bc.rho[i], bc.normal, &outs[(size_t)i * 27], wr);
+                for (int d = 0; d < 27; ++d) wrs[(size_t)i * 27 + d] = wr[d];
+            }
+            for (int i = 0; i < bc.n; ++i)
+                for (int d = 0; d < 27; ++d)
+                    if (wrs[(size_t)i * 27 + d]) F[27 * bc.node[i] + d] = outs[(size_t)i * 27 + d];
+        }
     }
     return 0;
 }

@@ -101,3 +101,72 @@ def sparse_layout(dims, solid=None, seed=0):
     fluid = np.arange(N) if solid is None else np.nonzero(~np.asarray(solid).ravel())[0]
     fluid_nodes = np.sort(sidx[fluid]).astype(np.uint32)
     return nbx, nby, nbz, fluid_nodes, sidx
+
+
+def boundary_groups(solid, dims, classify):
+    """Collect the fluid->solid links of a structured box into boundary groups.
+
+    classify(x, y, z, d, tx, ty, tz) -> (group_name, q) or None for each link d
+    of fluid node (x,y,z) whose target (tx,ty,tz) (periodically wrapped) is
+    solid.  Returns {group: (nodes[int64], q27[27, n] float32)}."""
+    nx, ny, nz = dims
+    sol = np.asarray(solid, dtype=bool).reshape(nz, ny, nx)
+    groups = {}
+    fz, fy, fx = np.nonzero(~sol)
+    for z, y, x in zip(fz, fy, fx):
+        per = {}
+        for d in range(1, 27):
+            tx, ty, tz = (x + CX[d]) % nx, (y + CY[d]) % ny, (z + CZ[d]) % nz
+            if not sol[tz, ty, tx]:
+                continue
+            r = classify(x, y, z, d, tx, ty, tz)
+            if r is None:
+                continue
+            name, q = r
+            per.setdefault(name, np.full(27, -1.0, np.float32))[d] = q
+        k = x + nx * (y + ny * z)
+        for name, q in per.items():
+            groups.setdefault(name, ([], []))
+            groups[name][0].append(k)
+            groups[name][1].append(q)
+    return {name: (np.array(nodes, dtype=np.int64), np.stack(qs, axis=1)) for name, (nodes, qs) in groups.items()}
+
+
+def plate_channel(nx, ny, nz, q_bottom=0.5, q_top=0.5):
+    """Periodic in x and z, solid layers at y = 0 and y = ny-1.  The bottom wall
+    sits at y = 1 - q_bottom, the top wall at y = ny-2 + q_top (every link into
+    a plate crosses it at the same fraction q).  Returns (solid, groups) with
+    groups 'bottom' / 'top'."""
+    solid = np.zeros((nz, ny, nx), dtype=bool)
+    solid[:, 0, :] = True
+    solid[:, ny - 1, :] = True
+
+    def classify(x, y, z, d, tx, ty, tz):
+        return ("bottom", q_bottom) if ty == 0 else ("top", q_top)
+
+    return solid.ravel(), boundary_groups(solid, (nx, ny, nz), classify)
+
+
+def inflow_channel(nx, ny, nz):
+    """Channel along x: solid layers at x = 0 (inlet plane), x = nx-1 (outlet
+    plane), y = 0 / ny-1 and z = 0 / nz-1 (walls).  Links into the inlet plane
+    form group 'inlet', into the outlet plane 'outlet' (handled by the pressure
+    boundary on the last fluid layer), all others 'wall' (q = 0.5).
+    Returns (solid, groups, outlet_nodes, outlet_neighbours)."""
+    solid = np.zeros((nz, ny, nx), dtype=bool)
+    solid[:, :, 0] = solid[:, :, nx - 1] = True
+    solid[:, 0, :] = solid[:, ny - 1, :] = True
+    solid[0, :, :] = solid[nz - 1, :, :] = True
+
+    def classify(x, y, z, d, tx, ty, tz):
+        if tx == 0 and 0 < ty < ny - 1 and 0 < tz < nz - 1:
+            return ("inlet", 0.5)
+        if tx == nx - 1:
+            return None
+        return ("wall", 0.5)
+
+    groups = boundary_groups(solid, (nx, ny, nz), classify)
+    zz, yy = np.meshgrid(np.arange(1, nz - 1), np.arange(1, ny - 1), indexing="ij")
+    out_nodes = ((nx - 2) + nx * (yy + ny * zz)).ravel().astype(np.int64)
+    out_nbrs = out_nodes - 1
+    return solid.ravel(), groups, out_nodes, out_nbrs

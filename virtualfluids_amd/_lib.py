@@ -15,7 +15,8 @@ VFX_OK, VFX_EINVAL, VFX_EHIP, VFX_EUNSUPPORTED = 0, 1, 2, 3
 VFX_FP64, VFX_FP32, VFX_FP16_SCALED = 0, 1, 2
 VFX_K17, VFX_BGK = 0, 1
 VFX_TEMPLATE_DEFAULT, VFX_TEMPLATE_WRITE_MACRO = 0, 1
-VFX_BC_VELOCITY_BOUNCE_BACK, VFX_BC_NOSLIP_INTERPOLATED = 0, 1
+VFX_BC_VELOCITY_BOUNCE_BACK, VFX_BC_NOSLIP_INTERPOLATED, VFX_BC_VELOCITY_INTERPOLATED = 0, 1, 2
+VFX_BC_SLIP, VFX_BC_PRESSURE_NEQ = 3, 4
 
 _u32p = C.POINTER(C.c_uint32)
 _dp = C.POINTER(C.c_double)
@@ -37,6 +38,7 @@ class VfxBC(C.Structure):
     _fields_ = [
         ("kind", C.c_int32), ("n", C.c_uint32), ("k", C.c_void_p), ("q27", C.c_void_p),
         ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p), ("cpu_float_velocity", C.c_int32),
+        ("rho", C.c_void_p), ("k_neighbor", C.c_void_p), ("normal", C.c_int32),
     ]
 
 

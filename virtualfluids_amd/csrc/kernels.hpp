@@ -494,7 +494,29 @@ struct BCDev {
     const double* vz;
     int32_t cpu_float;
     int32_t direct;
+    double omega;          // interpolated kinds
+    const double* rho;     // pressure: density deviation per boundary node
+    const uint32_t* kn;    // pressure: inward neighbour of each boundary node
+    int32_t normal;        // pressure: outward direction of the boundary (d000..dMMM)
 };
+
+// c_d . (ux, uy, uz) spelled as VirtualFluids writes it: first non-zero
+// component with its sign, then the others added or subtracted in x, y, z order
+template <typename T> __device__ __forceinline__ T proj(int d, T ux, T uy, T uz)
+{
+    T v;
+    if (cx(d) != 0) {
+        v = cx(d) > 0 ? ux : -ux;
+        if (cy(d) != 0) v = cy(d) > 0 ? v + uy : v - uy;
+        if (cz(d) != 0) v = cz(d) > 0 ? v + uz : v - uz;
+    } else if (cy(d) != 0) {
+        v = cy(d) > 0 ? uy : -uy;
+        if (cz(d) != 0) v = cz(d) > 0 ? v + uz : v - uz;
+    } else {
+        v = cz(d) > 0 ? uz : -uz;
+    }
+    return v;
+}
 
 // 6 w_d (c_d . u) with the component order VirtualFluids writes it
 // (VelocityBounceBack.cu:159-184); cpu_float reproduces BoundaryConditions.h:195-262
@@ -554,6 +576,148 @@ __global__ void __launch_bounds__(256) k_bc_velocity_bb(Lat L, BCDev B)
         const T v = fpost - wall_term<T>(d, ux, uy, uz, B.cpu_float);
         // next step (parity !EVEN) reads direction e of node k from here
         Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)]);
+    }
+}
+
+// Interpolated bounce-back family (NoSlipInterpolatedCompressible.cu:44-378,
+// VelocityInterpolatedCompressible.cu:44-…, SlipCompressible.cu:45-…, formulas
+// KernelUtilities.h:171-197): from the post-collision populations of the
+// boundary node, density and velocity (with the sum order those kernels use),
+// the symmetric equilibrium part of each wall link, and the Bouzidi-type
+// interpolation with the subgrid distance q.  KIND: 1 no-slip, 2 velocity, 3 slip.
+template <typename S, bool EVEN, int KIND>
+__global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
+{
+    using T = typename Compute<S>::T;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    uint32_t kk[8];
+    const uint32_t k = B.k[i];
+    if (k >= L.N) return;
+    if (B.direct) nbrs_direct(L, k, kk);
+    else nbrs_sparse(L, k, kk);
+    T p[27]; // post-collision population of direction d
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        const int e = inv(d);
+        p[d] = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)]);
+    }
+    const T drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] + p[d0MP] +
+                   p[d0PP] + p[d0MM] + p[dP0M] + p[dM0P] + p[dP0P] + p[dM0M] + p[dPM0] + p[dMP0] + p[dPP0] + p[dMM0] +
+                   p[d00P] + p[d00M] + p[d0P0] + p[d0M0] + p[dP00] + p[dM00] + p[d000];
+    const T vx1 = (((p[dPMP] - p[dMPM]) - (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                   ((p[dP0M] - p[dM0P]) + (p[dP0P] - p[dM0M])) + ((p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
+                   (p[dP00] - p[dM00])) / (T(1) + drho);
+    const T vx2 = ((-(p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                   ((p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + (-(p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
+                   (p[d0P0] - p[d0M0])) / (T(1) + drho);
+    const T vx3 = (((p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) + (p[dMMP] - p[dPPM])) +
+                   (-(p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + ((p[dP0P] - p[dM0M]) - (p[dP0M] - p[dM0P])) +
+                   (p[d00P] - p[d00M])) / (T(1) + drho);
+    const T cu_sq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3) * (T(1) + drho);
+    const T omega = (T)B.omega;
+    T VX = 0, VY = 0, VZ = 0;
+    if (KIND == VFX_BC_VELOCITY_INTERPOLATED) {
+        VX = B.vx ? (T)B.vx[i] : T(0);
+        VY = B.vy ? (T)B.vy[i] : T(0);
+        VZ = B.vz ? (T)B.vz[i] : T(0);
+    }
+    bool fx = false, fy = false, fz = false; // slip: axes normal to a face link
+#pragma unroll
+    for (int d = 1; d < 27; ++d) {
+        const T q = (T)B.q[(uint64_t)d * B.n + i];
+        if (!(q >= T(0) && q <= T(1))) continue;
+        const T w = weight<T>(d);
+        const T vlb = proj<T>(d, vx1, vx2, vx3);
+        const T feq = w * (drho + T(4.5) * vlb * vlb * (T(1) + drho) - cu_sq);
+        const T f = p[d], fi = p[inv(d)];
+        const T a = (T(1) - q) / (T(1) + q) * (f - fi + (f + fi - T(2) * feq * omega) / (T(1) - omega)) * T(0.5);
+        T v;
+        if (KIND == VFX_BC_NOSLIP_INTERPOLATED) {
+            v = a + (q * (f + fi)) / (T(1) + q);
+        } else {
+            T vbc;
+            if (KIND == VFX_BC_SLIP) {
+                if (weight_class(d) == 1) {
+                    if (cx(d) != 0) fx = true;
+                    if (cy(d) != 0) fy = true;
+                    if (cz(d) != 0) fz = true;
+                }
+                const T sx = fx ? T(0) : vx1, sy = fy ? T(0) : vx2, sz = fz ? T(0) : vx3;
+                vbc = proj<T>(d, sx, sy, sz);
+            } else {
+                vbc = proj<T>(d, VX, VY, VZ);
+            }
+            v = a + (q * (f + fi) - T(6) * w * vbc) / (T(1) + q);
+        }
+        const int e = inv(d);
+        Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)]);
+    }
+}
+
+// Non-equilibrium pressure boundary (PressureNonEquilibriumCompressible.cu:45-…):
+// pre-collision, parity EVEN of the step about to run.  The populations of the
+// inward neighbour keep their non-equilibrium part and get the equilibrium of
+// the prescribed density; they replace the boundary node's populations of every
+// direction not pointing out of the domain (c_d . normal <= 0).
+template <typename T> __device__ __forceinline__ T feq_shifted(int d, T drho, T vx1, T vx2, T vx3, T cusq)
+{
+    if (d == d000) return T(8. / 27.) * (drho - (drho + T(1)) * cusq);
+    const T cu = proj<T>(d, vx1, vx2, vx3);
+    return weight<T>(d) * (drho + (drho + T(1)) * (T(3) * cu + T(4.5) * cu * cu - cusq));
+}
+
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
+{
+    using T = typename Compute<S>::T;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    const uint32_t k = B.k[i], kn = B.kn[i];
+    if (k >= L.N || kn >= L.N) return;
+    uint32_t kk[8], kkn[8];
+    if (B.direct) {
+        nbrs_direct(L, k, kk);
+        nbrs_direct(L, kn, kkn);
+    } else {
+        nbrs_sparse(L, k, kk);
+        nbrs_sparse(L, kn, kkn);
+    }
+    T m[27];
+    gather<S, T, EVEN>(L, kkn, m);
+    T f[27];
+#pragma unroll
+    for (int s = 0; s < 27; ++s) f[dir_at(s)] = m[s];
+    // getCompressibleMacroscopicValues (MacroscopicQuantities.h:117-127)
+    const T corners = ((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]));
+    const T edges = (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P]))) +
+                    ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]));
+    const T faces = ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0])) + (f[d00M] + f[d00P]);
+    T drho1 = ((corners + edges) + f[d000]) + faces;
+    T vx1 = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dPMM] - f[dMPP]) + (f[dPPM] - f[dMMP]))) +
+            (((f[dP0M] - f[dM0P]) + (f[dP0P] - f[dM0M])) + ((f[dPM0] - f[dMP0]) + (f[dPP0] - f[dMM0]))) +
+            (f[dP00] - f[dM00]);
+    T vx2 = (((f[dPPP] - f[dMMM]) + (f[dMPM] - f[dPMP])) + ((f[dMPP] - f[dPMM]) + (f[dPPM] - f[dMMP]))) +
+            (((f[d0PM] - f[d0MP]) + (f[d0PP] - f[d0MM])) + ((f[dMP0] - f[dPM0]) + (f[dPP0] - f[dMM0]))) +
+            (f[d0P0] - f[d0M0]);
+    T vx3 = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dMPP] - f[dPMM]) + (f[dMMP] - f[dPPM]))) +
+            (((f[d0MP] - f[d0PM]) + (f[d0PP] - f[d0MM])) + ((f[dM0P] - f[dP0M]) + (f[dP0P] - f[dM0M]))) +
+            (f[d00P] - f[d00M]);
+    const T oor = T(1) / (drho1 + T(1));
+    vx1 *= oor;
+    vx2 *= oor;
+    vx3 *= oor;
+    const T cusq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3);
+#pragma unroll
+    for (int d = 0; d < 27; ++d) f[d] -= feq_shifted<T>(d, drho1, vx1, vx2, vx3, cusq);
+    drho1 = (T)B.rho[i];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) f[d] += feq_shifted<T>(d, drho1, vx1, vx2, vx3, cusq);
+    const int nx = cx(B.normal), ny = cy(B.normal), nz = cz(B.normal);
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        if (cx(d) * nx + cy(d) * ny + cz(d) * nz > 0) continue;
+        Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], f[d], L.st[weight_class(d)]);
     }
 }
 
@@ -657,12 +821,31 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
 {
     if (B.n == 0) return hipSuccess;
     const dim3 g(grid_for(B.n, kBlock)), b(kBlock);
-    if (kind == VFX_BC_VELOCITY_BOUNCE_BACK) {
+    switch (kind) {
+    case VFX_BC_VELOCITY_BOUNCE_BACK:
         if (even) hipLaunchKernelGGL((k_bc_velocity_bb<S, true>), g, b, 0, st, L, B);
         else hipLaunchKernelGGL((k_bc_velocity_bb<S, false>), g, b, 0, st, L, B);
-        return hipGetLastError();
+        break;
+    case VFX_BC_NOSLIP_INTERPOLATED:
+        if (even) hipLaunchKernelGGL((k_bc_interp<S, true, VFX_BC_NOSLIP_INTERPOLATED>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_interp<S, false, VFX_BC_NOSLIP_INTERPOLATED>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_VELOCITY_INTERPOLATED:
+        if (even) hipLaunchKernelGGL((k_bc_interp<S, true, VFX_BC_VELOCITY_INTERPOLATED>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_interp<S, false, VFX_BC_VELOCITY_INTERPOLATED>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_SLIP:
+        if (even) hipLaunchKernelGGL((k_bc_interp<S, true, VFX_BC_SLIP>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_interp<S, false, VFX_BC_SLIP>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_PRESSURE_NEQ:
+        if (even) hipLaunchKernelGGL((k_bc_pressure<S, true>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_pressure<S, false>), g, b, 0, st, L, B);
+        break;
+    default:
+        return hipErrorInvalidValue;
     }
-    return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 #define VFX_INSTANTIATE_LAUNCHERS(S)                                                                                   \

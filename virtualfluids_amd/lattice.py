@@ -247,28 +247,53 @@ class Statistics:
 
 class BoundaryCondition:
     """A boundary-node list in the QforBoundaryConditions shape
-    (src/gpu/core/Calculation/Calculation.h:105-117)."""
+    (src/gpu/core/Calculation/Calculation.h:105-117).
 
-    def __init__(self, kind, nodes, q27, vx=None, vy=None, vz=None, cpu_float_velocity=False, device="cuda"):
+    kind: 0 velocity bounce-back, 1 no-slip interpolated, 2 velocity
+    interpolated, 3 slip, 4 non-equilibrium pressure (pre-collision; needs
+    ``rho``, ``neighbors`` (inward neighbour node) and ``normal`` (outward
+    direction index))."""
+
+    PRE_COLLISION = {V.VFX_BC_PRESSURE_NEQ}
+
+    def __init__(self, kind, nodes, q27=None, vx=None, vy=None, vz=None, cpu_float_velocity=False, rho=None,
+                 neighbors=None, normal=0, device="cuda"):
         nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
-        q = np.ascontiguousarray(q27, dtype=np.float32)
         n = nodes.size
-        if q.shape != (27, n):
-            raise ValueError("q27 must be (27, n)")
-        self.kind = kind
+        self.kind = int(kind)
         self.n = n
         dev = torch.device(device)
         self.max_node = int(nodes.max()) if n else -1
         self.k = torch.from_numpy(nodes.view(np.int32)).to(dev)
+        if self.kind == V.VFX_BC_PRESSURE_NEQ:
+            if rho is None or neighbors is None or not 1 <= int(normal) <= 26:
+                raise ValueError("pressure boundary needs rho, neighbors and normal in 1..26")
+            q27 = np.zeros((27, n), np.float32) if q27 is None else q27
+            kn = np.ascontiguousarray(neighbors, dtype=np.uint32)
+            if kn.size != n:
+                raise ValueError("one inward neighbour per boundary node")
+            self.max_node = max(self.max_node, int(kn.max()) if n else -1)
+            self.kn = torch.from_numpy(kn.view(np.int32)).to(dev)
+        else:
+            self.kn = None
+        q = np.ascontiguousarray(q27, dtype=np.float32)
+        if q.shape != (27, n):
+            raise ValueError("q27 must be (27, n)")
         self.q = torch.from_numpy(q).to(dev)
 
-        def vel(v):
+        def arr(v):
             if v is None:
                 return None
             return torch.from_numpy(np.ascontiguousarray(np.broadcast_to(v, (n,)), dtype=np.float64)).to(dev)
 
-        self.vx, self.vy, self.vz = vel(vx), vel(vy), vel(vz)
+        self.vx, self.vy, self.vz = arr(vx), arr(vy), arr(vz)
+        self.rho = arr(rho)
+        self.normal = int(normal)
         self.cpu_float_velocity = bool(cpu_float_velocity)
+
+    @property
+    def pre_collision(self) -> bool:
+        return self.kind in self.PRE_COLLISION
 
     def struct(self) -> V.VfxBC:
         b = V.VfxBC()
@@ -278,6 +303,9 @@ class BoundaryCondition:
         b.q27 = self.q.data_ptr() if self.n else None
         b.vx, b.vy, b.vz = _ptr(self.vx), _ptr(self.vy), _ptr(self.vz)
         b.cpu_float_velocity = int(self.cpu_float_velocity)
+        b.rho = _ptr(self.rho)
+        b.k_neighbor = _ptr(self.kn) if self.n else None
+        b.normal = self.normal
         return b
 
 
@@ -285,7 +313,8 @@ class BoundaryCondition:
 class Stepper:
     """The per-level time step of UpdateGrid27::updateGrid without refinement
     (src/gpu/core/Calculation/UpdateGrid27.cpp:54-95): collide+stream, the
-    post-collision boundary conditions, then the parity flip."""
+    post-collision boundary conditions, the parity flip, then the
+    pre-collision boundary conditions with the new parity."""
 
     lattice: Lattice
     collision: str = "k17"
@@ -293,11 +322,15 @@ class Stepper:
     even: bool = True
 
     def step(self, n=1, stream=None):
+        post = [b for b in self.bcs if not b.pre_collision]
+        pre = [b for b in self.bcs if b.pre_collision]
         for _ in range(n):
             self.lattice.collide_stream(self.collision, self.even, stream=stream)
-            for bc in self.bcs:
+            for bc in post:
                 self.lattice.apply_bc(bc, self.even, stream=stream)
             self.even = not self.even
+            for bc in pre:
+                self.lattice.apply_bc(bc, self.even, stream=stream)
 
     def populations(self) -> torch.Tensor:
         return self.lattice.read_populations(self.even)
