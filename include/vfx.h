@@ -175,6 +175,14 @@ vfx_status vfx_enstrophy(const vfx_lattice* lat, const double* drho, const doubl
 vfx_status vfx_accumulate_stats(const vfx_lattice* lat, int32_t is_even_timestep, double* sums9, vfx_stream stream);
 
 /* -------------------------------------------------------------- misc */
+/* Device memory and synchronisation for C / C++ callers without their own HIP
+ * runtime code (the host layer of virtualfluids_amd uses these). */
+vfx_status vfx_device_alloc(uint64_t bytes, void** ptr);
+vfx_status vfx_device_free(void* ptr);
+vfx_status vfx_copy_to_device(void* dst, const void* src, uint64_t bytes);
+vfx_status vfx_copy_to_host(void* dst, const void* src, uint64_t bytes);
+vfx_status vfx_synchronize(vfx_stream stream);
+
 const char* vfx_last_error(void);
 int32_t vfx_version(void);
 

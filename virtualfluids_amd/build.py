@@ -34,8 +34,9 @@ UNITS = {
     "api.hip": ["-ffp-contract=off"],
 }
 HOST_UNITS = {
-    "host/kernel_factory.cpp": ["-ffp-contract=off"],
+    "host/vfx_host.cpp": [],
 }
+GXX = shutil.which("g++") or "g++"
 
 
 def _compile(src: str, flags: list[str]) -> str:
@@ -49,14 +50,21 @@ def _compile(src: str, flags: list[str]) -> str:
     return out
 
 
+def _compile_host(src: str, flags: list[str]) -> str:
+    out = os.path.join(OBJ, src.replace("/", "_") + ".o")
+    cmd = [GXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off"] + flags + ["-c", os.path.join(CSRC, src),
+                                                                                    "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{' '.join(cmd)}\n{r.stderr}")
+    return out
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
-    units = dict(UNITS)
-    for src, fl in HOST_UNITS.items():
-        if os.path.exists(os.path.join(CSRC, src)):
-            units[src] = fl + ["-x", "hip"]
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(units))) as ex:
-        objs = list(ex.map(lambda kv: _compile(*kv), units.items()))
+    jobs = [(_compile, src, fl) for src, fl in UNITS.items()] + [(_compile_host, src, fl) for src, fl in HOST_UNITS.items()]
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        objs = list(ex.map(lambda j: j[0](j[1], j[2]), jobs))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)

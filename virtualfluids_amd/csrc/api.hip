@@ -99,6 +99,28 @@ extern "C" {
 
 const char* vfx_last_error(void) { return g_err.c_str(); }
 
+vfx_status vfx_device_alloc(uint64_t bytes, void** ptr)
+{
+    if (!ptr) return fail(VFX_EINVAL, "ptr is NULL");
+    *ptr = nullptr;
+    if (bytes == 0) return VFX_OK;
+    return hip_status(hipMalloc(ptr, bytes), "hipMalloc");
+}
+
+vfx_status vfx_device_free(void* ptr) { return hip_status(hipFree(ptr), "hipFree"); }
+
+vfx_status vfx_copy_to_device(void* dst, const void* src, uint64_t bytes)
+{
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+vfx_status vfx_copy_to_host(void* dst, const void* src, uint64_t bytes)
+{
+    return hip_status(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+vfx_status vfx_synchronize(vfx_stream stream) { return hip_status(hipStreamSynchronize((hipStream_t)stream), "sync"); }
+
 int32_t vfx_version(void) { return 1; }
 
 vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t is_even, int32_t tmpl, vfx_stream stream)

@@ -1,0 +1,161 @@
+// vfx_host.cpp -- implementation of the C++ host layer (see vfx_host.hpp).
+#include "vfx_host.hpp"
+
+#include <string>
+
+namespace vfx::host {
+
+void check(vfx_status s, const char* what)
+{
+    if (s != VFX_OK) throw std::runtime_error(std::string(what) + ": " + vfx_last_error());
+}
+
+static int32_t vfx_template(CollisionTemplate t)
+{
+    switch (t) {
+    case CollisionTemplate::Default:
+    case CollisionTemplate::SubDomainBorder: return VFX_TEMPLATE_DEFAULT;
+    case CollisionTemplate::WriteMacroVars: return VFX_TEMPLATE_WRITE_MACRO;
+    default: throw std::runtime_error("collision template with body force is not supported by the MI355X stepper");
+    }
+}
+
+void KernelImp::run()
+{
+    LBMSimulationParameter* p = para->getParD(level);
+    vfx_lattice lat = p->lattice;
+    const double* lim = para->getQuadricLimiters();
+    for (int i = 0; i < 3; ++i) lat.quadric_limiter[i] = lim[i];
+    check(vfx_collide_stream(&lat, collision, p->isEvenTimestep ? 1 : 0, VFX_TEMPLATE_DEFAULT, p->stream),
+          "Kernel::run");
+}
+
+// K17CompressibleNavierStokes.cu:102-145: one launch over a list of fluid nodes
+void KernelImp::runOnIndices(const unsigned int* indices, unsigned int size_indices, CollisionTemplate collisionTemplate,
+                             CudaStreamIndex)
+{
+    LBMSimulationParameter* p = para->getParD(level);
+    vfx_lattice lat = p->lattice;
+    if (lat.direct) throw std::runtime_error("runOnIndices needs the sparse (neighbour array) layout");
+    lat.fluid_nodes = indices;
+    lat.n_fluid = size_indices;
+    const double* lim = para->getQuadricLimiters();
+    for (int i = 0; i < 3; ++i) lat.quadric_limiter[i] = lim[i];
+    check(vfx_collide_stream(&lat, collision, p->isEvenTimestep ? 1 : 0, vfx_template(collisionTemplate), p->stream),
+          "Kernel::runOnIndices");
+}
+
+std::vector<std::shared_ptr<Kernel>> KernelFactoryImp::makeKernels(std::shared_ptr<Parameter> para)
+{
+    std::vector<std::shared_ptr<Kernel>> kernels;
+    for (int level = 0; level <= para->getMaxLevel(); level++) kernels.push_back(makeKernel(para, para->getMainKernel(), level));
+    return kernels;
+}
+
+std::shared_ptr<Kernel> KernelFactoryImp::makeKernel(std::shared_ptr<Parameter> para, const std::string& kernel, int level)
+{
+    if (kernel == "BGKCompSP27") return B92CompressibleNavierStokes::getNewInstance(para, level);
+    if (kernel == "K17CompressibleNavierStokes") return K17CompressibleNavierStokes::getNewInstance(para, level);
+    throw std::runtime_error("KernelFactory does not know the KernelType.");
+}
+
+static boundaryCondition make_bc(int32_t kind)
+{
+    return [kind](LBMSimulationParameter* p, QforBoundaryConditions* q) {
+        vfx_bc bc{};
+        bc.kind = kind;
+        bc.n = q->numberOfBCnodes;
+        bc.k = q->k;
+        bc.q27 = q->q27;
+        bc.vx = q->Vx;
+        bc.vy = q->Vy;
+        bc.vz = q->Vz;
+        bc.rho = q->RhoBC;
+        bc.k_neighbor = q->kN;
+        bc.normal = q->direction;
+        check(vfx_bc_apply(&p->lattice, &bc, p->isEvenTimestep ? 1 : 0, p->stream), "boundary condition");
+    };
+}
+
+boundaryCondition BoundaryConditionFactory::getVelocityBoundaryConditionPost() const
+{
+    switch (velocity) {
+    case VelocityBC::VelocityBounceBack: return make_bc(VFX_BC_VELOCITY_BOUNCE_BACK);
+    case VelocityBC::VelocityInterpolatedCompressible: return make_bc(VFX_BC_VELOCITY_INTERPOLATED);
+    default: return nullptr;
+    }
+}
+boundaryCondition BoundaryConditionFactory::getNoSlipBoundaryConditionPost() const
+{
+    switch (noSlip) {
+    case NoSlipBC::NoSlipBounceBack: return make_bc(VFX_BC_VELOCITY_BOUNCE_BACK); // zero wall velocity
+    case NoSlipBC::NoSlipInterpolatedCompressible: return make_bc(VFX_BC_NOSLIP_INTERPOLATED);
+    default: return nullptr;
+    }
+}
+boundaryCondition BoundaryConditionFactory::getSlipBoundaryConditionPost() const
+{
+    return slip == SlipBC::SlipCompressible ? make_bc(VFX_BC_SLIP) : nullptr;
+}
+boundaryCondition BoundaryConditionFactory::getPressureBoundaryConditionPre() const
+{
+    return pressure == PressureBC::PressureNonEquilibriumCompressible ? make_bc(VFX_BC_PRESSURE_NEQ) : nullptr;
+}
+
+// BoundaryConditionKernelManager.cpp:53-83: boundary nodes without a selected
+// boundary condition are an error
+static void require(const boundaryCondition& f, const QforBoundaryConditions& q, const char* name)
+{
+    if (q.numberOfBCnodes > 0 && !f)
+        throw std::runtime_error(std::string(name) + " boundary nodes exist but no boundary condition was selected");
+}
+
+UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared_ptr<Kernel>> k,
+                           const BoundaryConditionFactory& f)
+    : para(std::move(p)), kernels(std::move(k)), velocityBC(f.getVelocityBoundaryConditionPost()),
+      noSlipBC(f.getNoSlipBoundaryConditionPost()), slipBC(f.getSlipBoundaryConditionPost()),
+      pressureBC(f.getPressureBoundaryConditionPre())
+{
+    if ((int)kernels.size() != para->getMaxLevel() + 1) throw std::runtime_error("one kernel per level expected");
+    for (int l = 0; l <= para->getMaxLevel(); ++l) {
+        LBMSimulationParameter* d = para->getParD(l);
+        require(velocityBC, d->velocityBC, "velocity");
+        require(noSlipBC, d->noSlipBC, "no-slip");
+        require(slipBC, d->slipBC, "slip");
+        require(pressureBC, d->pressureBC, "pressure");
+    }
+}
+
+void UpdateGrid27::postCollisionBC(int level)
+{
+    LBMSimulationParameter* p = para->getParD(level);
+    if (p->velocityBC.numberOfBCnodes) velocityBC(p, &p->velocityBC);
+    if (p->noSlipBC.numberOfBCnodes) noSlipBC(p, &p->noSlipBC);
+    if (p->slipBC.numberOfBCnodes) slipBC(p, &p->slipBC);
+}
+
+void UpdateGrid27::preCollisionBC(int level)
+{
+    LBMSimulationParameter* p = para->getParD(level);
+    if (p->pressureBC.numberOfBCnodes) pressureBC(p, &p->pressureBC);
+}
+
+void UpdateGrid27::updateGrid(int level, unsigned int)
+{
+    LBMSimulationParameter* p = para->getParD(level);
+    auto* k = dynamic_cast<KernelImp*>(kernels.at(level).get());
+    if (k && k->getKernelUsesFluidNodeIndices() && !p->lattice.direct) {
+        for (CollisionTemplate tag : p->allocatedBulkFluidNodeTags) {
+            auto it = p->taggedFluidNodeIndices.find(tag);
+            if (it == p->taggedFluidNodeIndices.end() || it->second.second == 0) continue;
+            kernels[level]->runOnIndices(it->second.first, it->second.second, tag);
+        }
+    } else {
+        kernels[level]->run();
+    }
+    postCollisionBC(level);
+    p->isEvenTimestep = !p->isEvenTimestep; // UpdateGrid27.cpp:264-268
+    preCollisionBC(level);
+}
+
+} // namespace vfx::host

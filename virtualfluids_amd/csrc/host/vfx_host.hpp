@@ -1,0 +1,184 @@
+// vfx_host.hpp -- C++ host layer over the libvfx C ABI.
+//
+// Mirrors the plug-in interface VirtualFluids' GPU solver uses for this path,
+// so a caller written against it finds the same names and behaviour:
+//   Kernel / KernelImp           src/gpu/core/Kernel/Kernel.h:44-52, KernelImp.h:45-65
+//   KernelFactory(Imp)           src/gpu/core/Kernel/KernelFactory/KernelFactory.h:47-55,
+//                                KernelFactoryImp.cpp:61-135 (string keys of KernelTypes.h:35-46)
+//   BoundaryConditionFactory     src/gpu/core/BoundaryConditions/BoundaryConditionFactory.h:50-174
+//   UpdateGrid27 step order      src/gpu/core/Calculation/UpdateGrid27.cpp:54-95, :227-307
+// Every device operation goes through include/vfx.h; errors become
+// std::runtime_error (the reference aborts the process instead).
+#pragma once
+
+#include "../../../include/vfx.h"
+
+#include <functional>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace vfx::host {
+
+// src/gpu/core/Calculation/Calculation.h:55-66
+enum class CollisionTemplate { Default, WriteMacroVars, ApplyBodyForce, AllFeatures, SubDomainBorder };
+// src/gpu/core/Cuda/CudaStreamManager.h:41-46
+enum class CudaStreamIndex { Legacy, Bulk, SubDomainBorder, Precursor };
+enum class PreProcessorType { InitNavierStokesCompressible };
+
+// Calculation.h:105-117 (plus the pressure boundary's inward neighbours)
+struct QforBoundaryConditions {
+    uint32_t* k = nullptr;
+    float* q27 = nullptr;       // 27 * numberOfBCnodes
+    double* Vx = nullptr;
+    double* Vy = nullptr;
+    double* Vz = nullptr;
+    double* RhoBC = nullptr;
+    uint32_t* kN = nullptr;
+    uint32_t numberOfBCnodes = 0;
+    int direction = 0;          // pressure boundary: outward direction (dP00 = 1 ...)
+};
+
+// The per-level data of src/gpu/core/Parameter/Parameter.h:66-255 this path uses.
+struct LBMSimulationParameter {
+    vfx_lattice lattice{};      // distributions, addressing, omega, limiters, force
+    bool isEvenTimestep = true;
+    // fluid node lists per collision template (Parameter::taggedFluidNodeIndices)
+    std::map<CollisionTemplate, std::pair<uint32_t*, uint32_t>> taggedFluidNodeIndices;
+    std::vector<CollisionTemplate> allocatedBulkFluidNodeTags{CollisionTemplate::Default};
+    QforBoundaryConditions velocityBC, noSlipBC, slipBC, pressureBC;
+    vfx_stream stream = nullptr;
+};
+
+class Parameter {
+public:
+    explicit Parameter(int maxLevel = 0) : levels(maxLevel + 1) {}
+    int getMaxLevel() const { return (int)levels.size() - 1; }
+    LBMSimulationParameter* getParD(int level) { return &levels.at(level); }
+    void setMainKernel(const std::string& k) { mainKernel = k; }
+    const std::string& getMainKernel() const { return mainKernel; }
+    void setQuadricLimiters(double p, double m, double d) { limiters[0] = p; limiters[1] = m; limiters[2] = d; }
+    const double* getQuadricLimiters() const { return limiters; }
+
+private:
+    std::vector<LBMSimulationParameter> levels;
+    std::string mainKernel = "K17CompressibleNavierStokes";
+    double limiters[3] = {0.01, 0.01, 0.01};
+};
+
+void check(vfx_status s, const char* what);
+
+class Kernel {
+public:
+    virtual ~Kernel() = default;
+    virtual void run() = 0;
+    virtual void runOnIndices(const unsigned int* indices, unsigned int size_indices, CollisionTemplate collisionTemplate,
+                              CudaStreamIndex streamIndex = CudaStreamIndex::Legacy) = 0;
+    virtual std::vector<PreProcessorType> getPreProcessorTypes() = 0;
+};
+
+class KernelImp : public Kernel {
+public:
+    void run() override;
+    void runOnIndices(const unsigned int* indices, unsigned int size_indices, CollisionTemplate collisionTemplate,
+                      CudaStreamIndex streamIndex = CudaStreamIndex::Legacy) override;
+    std::vector<PreProcessorType> getPreProcessorTypes() override { return {PreProcessorType::InitNavierStokesCompressible}; }
+    bool getKernelUsesFluidNodeIndices() const { return kernelUsesFluidNodeIndices; }
+
+protected:
+    KernelImp(std::shared_ptr<Parameter> para, int level, int32_t collision, bool usesIndices)
+        : para(std::move(para)), level(level), collision(collision), kernelUsesFluidNodeIndices(usesIndices) {}
+    std::shared_ptr<Parameter> para;
+    int level;
+    int32_t collision;
+    bool kernelUsesFluidNodeIndices;
+};
+
+// "K17CompressibleNavierStokes" (K17CompressibleNavierStokes.cu:57-156), TurbulenceModel::None
+class K17CompressibleNavierStokes : public KernelImp {
+public:
+    static std::shared_ptr<K17CompressibleNavierStokes> getNewInstance(std::shared_ptr<Parameter> para, int level)
+    {
+        return std::shared_ptr<K17CompressibleNavierStokes>(new K17CompressibleNavierStokes(std::move(para), level));
+    }
+
+private:
+    K17CompressibleNavierStokes(std::shared_ptr<Parameter> p, int l) : KernelImp(std::move(p), l, VFX_K17, true) {}
+};
+
+// "BGKCompSP27" (B92CompressibleNavierStokes.cu:43-57)
+class B92CompressibleNavierStokes : public KernelImp {
+public:
+    static std::shared_ptr<B92CompressibleNavierStokes> getNewInstance(std::shared_ptr<Parameter> para, int level)
+    {
+        return std::shared_ptr<B92CompressibleNavierStokes>(new B92CompressibleNavierStokes(std::move(para), level));
+    }
+    void runOnIndices(const unsigned int*, unsigned int, CollisionTemplate, CudaStreamIndex) override
+    {
+        throw std::runtime_error("B92CompressibleNavierStokes runs over all non-solid nodes: use run()");
+    }
+
+private:
+    B92CompressibleNavierStokes(std::shared_ptr<Parameter> p, int l) : KernelImp(std::move(p), l, VFX_BGK, false) {}
+};
+
+class KernelFactory {
+public:
+    virtual ~KernelFactory() = default;
+    virtual std::vector<std::shared_ptr<Kernel>> makeKernels(std::shared_ptr<Parameter> para) = 0;
+};
+
+class KernelFactoryImp : public KernelFactory {
+public:
+    std::vector<std::shared_ptr<Kernel>> makeKernels(std::shared_ptr<Parameter> para) override;
+    std::shared_ptr<Kernel> makeKernel(std::shared_ptr<Parameter> para, const std::string& kernel, int level);
+};
+
+// BoundaryConditionFactory.h:50-53
+using boundaryCondition = std::function<void(LBMSimulationParameter*, QforBoundaryConditions*)>;
+
+class BoundaryConditionFactory {
+public:
+    // subsets of the reference enums (BoundaryConditionFactory.h:60-132) this stepper implements
+    enum class VelocityBC { VelocityBounceBack, VelocityInterpolatedCompressible, NotSpecified };
+    enum class NoSlipBC { NoSlipBounceBack, NoSlipInterpolatedCompressible, NotSpecified };
+    enum class SlipBC { SlipCompressible, NotSpecified };
+    enum class PressureBC { PressureNonEquilibriumCompressible, NotSpecified };
+
+    void setVelocityBoundaryCondition(VelocityBC t) { velocity = t; }
+    void setNoSlipBoundaryCondition(NoSlipBC t) { noSlip = t; }
+    void setSlipBoundaryCondition(SlipBC t) { slip = t; }
+    void setPressureBoundaryCondition(PressureBC t) { pressure = t; }
+
+    boundaryCondition getVelocityBoundaryConditionPost() const;
+    boundaryCondition getNoSlipBoundaryConditionPost() const;
+    boundaryCondition getSlipBoundaryConditionPost() const;
+    boundaryCondition getPressureBoundaryConditionPre() const;
+
+private:
+    VelocityBC velocity = VelocityBC::NotSpecified;
+    NoSlipBC noSlip = NoSlipBC::NotSpecified;
+    SlipBC slip = SlipBC::NotSpecified;
+    PressureBC pressure = PressureBC::NotSpecified;
+};
+
+// One level of UpdateGrid27::updateGrid without refinement: collision over the
+// tagged fluid nodes (or the whole box), post-collision boundary conditions,
+// parity swap, pre-collision boundary conditions.
+class UpdateGrid27 {
+public:
+    UpdateGrid27(std::shared_ptr<Parameter> para, std::vector<std::shared_ptr<Kernel>> kernels,
+                 const BoundaryConditionFactory& bcFactory);
+    void updateGrid(int level, unsigned int timestep);
+
+private:
+    void postCollisionBC(int level);
+    void preCollisionBC(int level);
+    std::shared_ptr<Parameter> para;
+    std::vector<std::shared_ptr<Kernel>> kernels;
+    boundaryCondition velocityBC, noSlipBC, slipBC, pressureBC;
+};
+
+} // namespace vfx::host
