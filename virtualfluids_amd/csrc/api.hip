@@ -52,6 +52,10 @@ vfx_status make_lat(const vfx_lattice* in, vfx::Lat& L)
     if (in->n_nodes >= (1ull << 32)) return fail(VFX_EUNSUPPORTED, "n_nodes %llu needs 64-bit node indices",
                                                  (unsigned long long)in->n_nodes);
     if (!(in->omega > 0.0 && in->omega < 2.0)) return fail(VFX_EINVAL, "omega %g outside (0,2)", in->omega);
+    const uint64_t esz = in->storage == VFX_FP64 ? 8 : in->storage == VFX_FP32 ? 4 : 2;
+    if (in->n_nodes * esz >= (1ull << 32))
+        return fail(VFX_EUNSUPPORTED, "a plane of %llu nodes exceeds 4 GiB (32-bit in-plane offsets)",
+                    (unsigned long long)in->n_nodes);
     std::memset(&L, 0, sizeof(L));
     L.f = in->f;
     L.N = in->n_nodes;

@@ -18,18 +18,30 @@ namespace vfx {
 // Population storage S with arithmetic type T.  _Float16 holds
 // h_d = (f_d - w_d) * scale / w_d; the per-weight-class factors are kernel
 // arguments (ld[c] = w_c / scale, st[c] = scale / w_c).
+// Element i of a plane is addressed as plane base (wave-uniform, scalar
+// registers) plus a 32-bit byte offset, so the loads and stores use the
+// scalar-base + 32-bit-offset form and no 64-bit address arithmetic per lane.
+// The host guarantees n_nodes * sizeof(S) < 2^32.
+template <typename S> __device__ __forceinline__ const S& elem(const S* p, uint32_t i)
+{
+    return *(const S*)((const char*)p + i * (uint32_t)sizeof(S));
+}
+template <typename S> __device__ __forceinline__ S& elem(S* p, uint32_t i)
+{
+    return *(S*)((char*)p + i * (uint32_t)sizeof(S));
+}
 template <typename S, typename T> struct Storage;
 template <> struct Storage<double, double> {
-    static __device__ __forceinline__ double ld(const double* p, uint32_t i, float) { return p[i]; }
-    static __device__ __forceinline__ void st(double* p, uint32_t i, double v, float) { p[i] = v; }
+    static __device__ __forceinline__ double ld(const double* p, uint32_t i, float) { return elem(p, i); }
+    static __device__ __forceinline__ void st(double* p, uint32_t i, double v, float) { elem(p, i) = v; }
 };
 template <> struct Storage<float, float> {
-    static __device__ __forceinline__ float ld(const float* p, uint32_t i, float) { return p[i]; }
-    static __device__ __forceinline__ void st(float* p, uint32_t i, float v, float) { p[i] = v; }
+    static __device__ __forceinline__ float ld(const float* p, uint32_t i, float) { return elem(p, i); }
+    static __device__ __forceinline__ void st(float* p, uint32_t i, float v, float) { elem(p, i) = v; }
 };
 template <> struct Storage<_Float16, float> {
-    static __device__ __forceinline__ float ld(const _Float16* p, uint32_t i, float s) { return (float)p[i] * s; }
-    static __device__ __forceinline__ void st(_Float16* p, uint32_t i, float v, float s) { p[i] = (_Float16)(v * s); }
+    static __device__ __forceinline__ float ld(const _Float16* p, uint32_t i, float s) { return (float)elem(p, i) * s; }
+    static __device__ __forceinline__ void st(_Float16* p, uint32_t i, float v, float s) { elem(p, i) = (_Float16)(v * s); }
 };
 template <typename S> struct Compute { using T = float; };
 template <> struct Compute<double> { using T = double; };
