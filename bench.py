@@ -40,6 +40,9 @@ def parse():
                    help="storage[:plain]; plain = reference operation order")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--ensemble", type=int, default=512,
+                   help="cavities in the batched-ensemble leg (0 skips it)")
+    p.add_argument("--ensemble-steps", type=int, default=100, help="steps per captured graph of the ensemble leg")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain device copy of known size (reference for HBM counters and bandwidth)")
     return p.parse_args()
@@ -138,6 +141,58 @@ def cpu_baseline(target_s):
                       f"K17CompressibleNavierStokes::calculate + swap (no block exchange), {secs:.1f} s"}
 
 
+def ensemble_leg(torch, vfx, n_domains, graph_steps, reps=2):
+    """Lid-driven cavity ensemble (edges log-uniform in [16, 96] from seed 7,
+    aspect 1, 1:1:2 or 2:1:1, Re uniform in [50, 1000], lid 0.05), FP16 shifted
+    storage, FP32 arithmetic.  Times `graph_steps`-step HIP graphs of (a) the
+    batched launch with the autotuned shape and (b) one launch per domain, and
+    the same per-domain launches issued eagerly; aggregate MLUPS counts the
+    interior (colliding) nodes."""
+    dims, omegas, lids = vfx.cavity_ensemble(n_domains, seed=7)
+    boxes = [(a + 2, b + 2, c + 2) for a, b, c in dims]
+    ens = vfx.Ensemble(boxes, omegas, lids, storage="fp16", device="cuda")
+    ens.init_rest()
+    block, grid, tune_ms = ens.autotune(steps=4)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn, n):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(n):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    g_batched = ens.capture(graph_steps)
+    g_domain = ens.capture(graph_steps, per_domain=True)
+    ens.replay(g_batched, graph_steps)  # warm up both graphs
+    ens.replay(g_domain, graph_steps)
+    ms_b = timed(lambda: ens.replay(g_batched, graph_steps), reps) / (reps * graph_steps)
+    ms_d = timed(lambda: ens.replay(g_domain, graph_steps), reps) / (reps * graph_steps)
+    eager_steps = 10
+    ms_e = timed(lambda: ens.step_per_domain(1), eager_steps) / eager_steps
+    steps_run = 4 + 2 * graph_steps + 2 * reps * graph_steps + eager_steps
+    finite = sum(bool(torch.isfinite(ens.lattice(i).f.float()).all()) for i in range(n_domains))
+    lups = ens.fluid_nodes
+    out = {
+        "domains": n_domains, "interior_nodes": lups, "graph_steps": graph_steps,
+        "launch": {"block": block, "grid": grid, "autotune_ms_per_step": round(tune_ms, 4)},
+        "batched_ms_per_step": round(ms_b, 4), "batched_mlups": round(lups / ms_b / 1e3, 1),
+        "per_domain_graph_ms_per_step": round(ms_d, 4), "per_domain_graph_mlups": round(lups / ms_d / 1e3, 1),
+        "per_domain_eager_ms_per_step": round(ms_e, 4), "per_domain_eager_mlups": round(lups / ms_e / 1e3, 1),
+        "speedup_vs_per_domain_graph": round(ms_d / ms_b, 3),
+        "speedup_vs_per_domain_eager": round(ms_e / ms_b, 3),
+        "achieved_gbs": round(54 * 2 * lups / ms_b / 1e6, 1),
+        "domains_finite_after_run": finite, "steps_run": steps_run,
+        "omega_range": [round(float(omegas.min()), 4), round(float(omegas.max()), 4)],
+    }
+    del g_batched, g_domain, ens
+    return out
+
+
 def traffic_from_profiles(form):
     """HBM bytes per launch of the headline kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
@@ -210,6 +265,15 @@ def main():
             out["forms"][f]["velocity_l2_vs_fp64"] = velocity_l2(r["macro"], ref)
             out["forms"][f]["density_drift_vs_fp64"] = density_drift(r["macro"], ref)
             out["forms"][f]["mass_error_per_node_vs_fp64"] = mass_error(r["macro"], ref)
+    for r in res.values():
+        r.pop("macro", None)
+    torch.cuda.empty_cache()
+    if args.ensemble > 0:
+        out["ensemble"] = ensemble_leg(torch, vfx, args.ensemble, args.ensemble_steps)
+        e = out["ensemble"]
+        print(f"# ensemble of {e['domains']} cavities: batched {e['batched_mlups']:.0f} MLUPS, per-domain graph "
+              f"{e['per_domain_graph_mlups']:.0f}, per-domain eager {e['per_domain_eager_mlups']:.0f}",
+              file=sys.stderr, flush=True)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
 

@@ -174,6 +174,90 @@ vfx_status vfx_enstrophy(const vfx_lattice* lat, const double* drho, const doubl
  * time-averaged mean is sums/T and the Reynolds stress sums_ij/T - mean_i mean_j. */
 vfx_status vfx_accumulate_stats(const vfx_lattice* lat, int32_t is_even_timestep, double* sums9, vfx_stream stream);
 
+/* ---------------------------------------------------- batched domains */
+/* ---------------------------------------------------------- refinement --
+ * Interface cells between a coarse level and its 2:1 fine level: ICells and
+ * ICellNeigh of src/gpu/core/Calculation/Calculation.h:72-89.  A cell is the
+ * 2x2x2 block of source nodes whose corner with the smallest coordinates
+ * (MMM) is listed; the other seven follow through +x, +y, +z neighbours
+ * (neighbor_x/y/z, or the box for direct addressing).
+ *   coarse to fine: coarse_cells = MMM corner of the coarse source cell,
+ *                   fine_cells   = MMM corner of the 8 fine nodes written;
+ *   fine to coarse: fine_cells   = MMM corner of the fine source cell,
+ *                   coarse_cells = the coarse node written (at the cell centre).
+ * offset_x/y/z (may be NULL) shift the interpolation point for cells next to
+ * walls, in source-cell units. */
+typedef struct {
+    const uint32_t* fine_cells;
+    const uint32_t* coarse_cells;
+    uint32_t n_cells;
+    const double* offset_x;
+    const double* offset_y;
+    const double* offset_z;
+} vfx_icells;
+
+/* scaleCoarseToFineCompressible (src/gpu/core/GridScaling/Scaling.cu:105-138,
+ * kernel scaleCoarseToFineCompressible.cu:305-529): coarse pre-collision
+ * populations at the coarse parity -> the 8 fine nodes of every cell,
+ * pre-collision at the fine parity (the reference passes the coarse level's
+ * isEvenTimestep and writes the fine level as even).  Both lattices share one
+ * storage form; omega of each lattice is its level's rate. */
+vfx_status vfx_scale_cf(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells,
+                        int32_t coarse_is_even, int32_t fine_is_even, vfx_stream stream);
+/* scaleFineToCoarseCompressible (Scaling.cu:155-188, kernel
+ * scaleFineToCoarseCompressible.cu:76-124): fine pre-collision populations at
+ * the fine parity (the reference reads the fine level as even) -> the coarse
+ * node of every cell, pre-collision at the coarse parity. */
+vfx_status vfx_scale_fc(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells,
+                        int32_t coarse_is_even, int32_t fine_is_even, vfx_stream stream);
+
+/* An ensemble of independent box domains (for example many lid-driven
+ * cavities) advanced by one launch from a per-domain work list.  Every
+ * domain is a box of dims[0]*dims[1]*dims[2] nodes with its own 27 planes at
+ * `f_offset` elements into a caller-owned pool; the outer layer is wall and
+ * every wall link is closed by moving-wall halfway bounce-back
+ * (VelocityBounceBack_Device, Velocity/VelocityBounceBack.cu:159-184) fused into
+ * the collide-stream kernel; links into the z = dims[2]-1 plane move with
+ * lid_velocity.  Each domain has its own relaxation rate.  No counterpart
+ * exists in the reference, which launches one kernel per level and block. */
+typedef struct vfx_domain {
+    uint64_t f_offset;
+    uint32_t dims[3];
+    double omega;
+    double lid_velocity[3];
+} vfx_domain;
+
+typedef struct vfx_batch_s* vfx_batch;
+typedef struct vfx_graph_s* vfx_graph;
+
+/* pool holds pool_elements populations of the storage type; every domain's 27
+ * planes [f_offset, f_offset + 27*nx*ny*nz) must lie inside it and must not
+ * overlap another domain's */
+vfx_status vfx_batch_create(void* pool, uint64_t pool_elements, int32_t storage, float fp16_scale,
+                            const vfx_domain* domains, uint32_t n_domains, const double* quadric_limiter,
+                            int32_t kernel_variant, vfx_batch* out);
+vfx_status vfx_batch_destroy(vfx_batch batch);
+/* zero the pool (rest state f - w = 0 in every storage form) */
+vfx_status vfx_batch_init_rest(vfx_batch batch, vfx_stream stream);
+/* launch shape: block 64/128/256 nodes per workgroup; grid 0 = one workgroup per
+ * tile, otherwise a persistent grid that strides over the tiles */
+vfx_status vfx_batch_set_launch(vfx_batch batch, uint32_t block, uint32_t grid);
+vfx_status vfx_batch_get_launch(vfx_batch batch, uint32_t* block, uint32_t* grid);
+/* times every launch shape on `steps` (even) steps from parity is_even_timestep
+ * and keeps the fastest; the state advances, its parity does not change */
+vfx_status vfx_batch_autotune(vfx_batch batch, int32_t is_even_timestep, int32_t steps, vfx_stream stream,
+                              uint32_t* block, uint32_t* grid, float* ms_per_step);
+/* one step of every domain in one launch / of one domain (the per-domain baseline) */
+vfx_status vfx_batch_step(vfx_batch batch, int32_t is_even_timestep, vfx_stream stream);
+vfx_status vfx_batch_step_domain(vfx_batch batch, uint32_t domain, int32_t is_even_timestep, vfx_stream stream);
+/* a vfx_lattice view of one domain (direct addressing over its box) for the
+ * population and diagnostic entry points */
+vfx_status vfx_batch_lattice(vfx_batch batch, uint32_t domain, vfx_lattice* out);
+/* n_steps steps captured into a HIP graph (per_domain = 1: as one launch per domain) */
+vfx_status vfx_batch_capture(vfx_batch batch, int32_t n_steps, int32_t is_even_start, int32_t per_domain, vfx_graph* out);
+vfx_status vfx_graph_launch(vfx_graph graph, vfx_stream stream);
+vfx_status vfx_graph_destroy(vfx_graph graph);
+
 /* -------------------------------------------------------------- misc */
 /* Device memory and synchronisation for C / C++ callers without their own HIP
  * runtime code (the host layer of virtualfluids_amd uses these). */

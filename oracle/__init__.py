@@ -67,6 +67,14 @@ def lib():
         L.orc_num_threads.restype = C.c_int
         L.orc_mass_momentum.argtypes = [_dp, C.c_int64, _u8p, _dp]
         L.orc_energy.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _u8p, C.c_int, _dp]
+        L.orc_source_moments.argtypes = [_dp, C.c_double, _dp]
+        L.orc_interp_coefficients.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_interpolate_cf.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        L.orc_interpolate_fc.argtypes = [_dp, C.c_double, C.c_double, _dp]
+        _i64p = C.POINTER(C.c_int64)
+        _ip = C.POINTER(C.c_int)
+        for fn in (L.orc_scale_cf, L.orc_scale_fc):
+            fn.argtypes = [_dp, _dp, _ip, _ip, C.c_int64, _i64p, _i64p, _dp, _dp, _dp, C.c_double, C.c_double]
         _lib = L
     return _lib
 
@@ -87,6 +95,8 @@ def ref():
                                      C.c_double, C.c_int]
         L.ref_time_blocks.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]
         L.ref_time_blocks.restype = C.c_double
+        L.ref_interp_cell.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      _dp, _dp]
         _ref = L
     return _ref
 
@@ -247,3 +257,83 @@ def ref_time_blocks(nblocks, bs, omega, nsteps):
     t = C.c_int(0)
     secs = ref().ref_time_blocks(int(nblocks), int(bs), float(omega), int(nsteps), C.byref(t))
     return secs, t.value
+
+
+# ---------------------------------------------------------------- refinement
+def source_moments(f27, omega):
+    """(drho, vx, vy, vz, kxy, kyz, kxz, kxxMyy, kxxMzz) of one source node."""
+    f = np.ascontiguousarray(f27, dtype=np.float64)
+    out = np.zeros(9)
+    lib().orc_source_moments(_p(f), float(omega), _p(out))
+    return out
+
+
+def interp_coefficients(f8x27, omega, offset=(0.0, 0.0, 0.0)):
+    """42 interpolation coefficients of one cell (8 corners, bit 0 = +x)."""
+    f = np.ascontiguousarray(f8x27, dtype=np.float64).reshape(8, 27)
+    out = np.zeros(42)
+    lib().orc_interp_coefficients(_p(f), float(omega), *(float(o) for o in offset), _p(out))
+    return out
+
+
+def interpolate_cf(coeff42, omega_fine, x, y, z, eps=0.5):
+    k = np.ascontiguousarray(coeff42, dtype=np.float64)
+    out = np.zeros(27)
+    lib().orc_interpolate_cf(_p(k), float(omega_fine), float(eps), float(x), float(y), float(z), _p(out))
+    return out
+
+
+def interpolate_fc(coeff42, omega_coarse, eps=2.0):
+    k = np.ascontiguousarray(coeff42, dtype=np.float64)
+    out = np.zeros(27)
+    lib().orc_interpolate_fc(_p(k), float(omega_coarse), float(eps), _p(out))
+    return out
+
+
+def interp_cell(kind, f8x27, omega_src, omega_dst, offset=(0.0, 0.0, 0.0), impl="oracle"):
+    """One interface cell end to end: kind "cf" -> (coeff42, 8 x 27 fine nodes),
+    kind "fc" -> (coeff42, 27 coarse populations)."""
+    f = np.ascontiguousarray(f8x27, dtype=np.float64).reshape(8, 27)
+    if impl == "ref":
+        k = np.zeros(42)
+        out = np.zeros((8, 27) if kind == "cf" else 27)
+        ref().ref_interp_cell(0 if kind == "cf" else 1, _p(f), float(omega_src), float(omega_dst),
+                              *(float(o) for o in offset), _p(k), _p(out))
+        return k, out
+    k = interp_coefficients(f, omega_src, offset)
+    if kind == "cf":
+        out = np.stack([interpolate_cf(k, omega_dst, (0.25 if c & 1 else -0.25), (0.25 if c & 2 else -0.25),
+                                       (0.25 if c & 4 else -0.25)) for c in range(8)])
+    else:
+        out = interpolate_fc(k, omega_dst)
+    return k, out
+
+
+def _scale(fn, Fc, Ff, dims_c, dims_f, coarse, fine, offsets, omega_c, omega_f):
+    n = len(coarse)
+    co = np.ascontiguousarray(coarse, dtype=np.int64)
+    fi = np.ascontiguousarray(fine, dtype=np.int64)
+    dc = (C.c_int * 3)(*dims_c)
+    df = (C.c_int * 3)(*dims_f)
+    offs = [None, None, None]
+    if offsets is not None:
+        offs = [np.ascontiguousarray(o, dtype=np.float64) for o in offsets]
+    i64 = C.POINTER(C.c_int64)
+    fn(_p(Fc), _p(Ff), dc, df, n, co.ctypes.data_as(i64), fi.ctypes.data_as(i64),
+       *(_p(o) if o is not None else None for o in offs), float(omega_c), float(omega_f))
+
+
+def scale_cf(Fc, Ff, dims_c, dims_f, coarse_cells, fine_cells, omega_c, omega_f, offsets=None):
+    """Coarse-to-fine over cells; returns the updated fine populations."""
+    Fc = np.ascontiguousarray(Fc, dtype=np.float64)
+    Ff = np.ascontiguousarray(Ff, dtype=np.float64).copy()
+    _scale(lib().orc_scale_cf, Fc, Ff, dims_c, dims_f, coarse_cells, fine_cells, offsets, omega_c, omega_f)
+    return Ff
+
+
+def scale_fc(Fc, Ff, dims_c, dims_f, coarse_nodes, fine_cells, omega_c, omega_f, offsets=None):
+    """Fine-to-coarse over cells; returns the updated coarse populations."""
+    Fc = np.ascontiguousarray(Fc, dtype=np.float64).copy()
+    Ff = np.ascontiguousarray(Ff, dtype=np.float64)
+    _scale(lib().orc_scale_fc, Fc, Ff, dims_c, dims_f, coarse_nodes, fine_cells, offsets, omega_c, omega_f)
+    return Fc

@@ -177,6 +177,44 @@ double mom_z(const double* f)
            (f[d00P] - f[d00M]);
 }
 
+// Central moments -> shifted populations: backward chimera sweeps x, y, z with
+// the well-conditioning constants of K17CompressibleNavierStokes.h:525-557
+// (the same sweeps close InterpolationCF.h:255-293 and InterpolationFC.h:210-248).
+// m[i][j][k] holds the moment of orders (i, j, k); on return the population of
+// direction (i-1, j-1, k-1).
+void backward_sweeps(double (&m)[3][3][3], double vx, double vy, double vz)
+{
+    const double vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+    // x sweep (:525-533): rows (cy,cz)
+    for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+            if (b == 1 || c == 1) { bwd(m[0][b][c], m[1][b][c], m[2][b][c], vx, vx2); continue; }
+            double kinv, k;
+            if (b == 0 && c == 0) { kinv = 1.0; k = 1.0; }
+            else if (b == 2 && c == 2) { kinv = 9.0; k = c1o9; }
+            else { kinv = 3.0; k = c1o3; }
+            bwd_k(m[0][b][c], m[1][b][c], m[2][b][c], vx, vx2, kinv, k);
+        }
+    // y sweep (:537-545): rows (i, cz)
+    for (int i = 0; i < 3; ++i)
+        for (int c = 0; c < 3; ++c) {
+            if (c == 1) { bwd(m[i][0][c], m[i][1][c], m[i][2][c], vy, vy2); continue; }
+            double kinv, k;
+            if (i == 1) { kinv = (c == 0) ? c3o2 : c9o2; k = (c == 0) ? c2o3 : c2o9; }
+            else        { kinv = (c == 0) ? 6.0 : 18.0; k = (c == 0) ? c1o6 : c1o18; }
+            bwd_k(m[i][0][c], m[i][1][c], m[i][2][c], vy, vy2, kinv, k);
+        }
+    // z sweep (:549-557): rows (i,j)
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            double kinv, k;
+            if (i == 1 && j == 1) { kinv = c9o4; k = c4o9; }
+            else if (i == 1 || j == 1) { kinv = 9.0; k = c1o9; }
+            else { kinv = 36.0; k = c1o36; }
+            bwd_k(m[i][j][0], m[i][j][1], m[i][j][2], vz, vz2, kinv, k);
+        }
+}
+
 // ------------------------------------------------------------- K17 -------
 // Weight of a velocity-space slot along one axis before that axis is
 // transformed: M/P -> 1/6, 0 -> 2/3; after it is transformed the slot holds a
@@ -368,35 +406,7 @@ void k17_collide(double* f, double omega, const double* lim, const double* force
     m010 = -m010;
     m001 = -m001;
 
-    // backward x sweep (:525-533): rows (cy,cz)
-    for (int b = 0; b < 3; ++b)
-        for (int c = 0; c < 3; ++c) {
-            if (b == 1 || c == 1) { bwd(m[0][b][c], m[1][b][c], m[2][b][c], vx, vx2); continue; }
-            double kinv, k;
-            if (b == 0 && c == 0) { kinv = 1.0; k = 1.0; }
-            else if (b == 2 && c == 2) { kinv = 9.0; k = c1o9; }
-            else { kinv = 3.0; k = c1o3; }
-            bwd_k(m[0][b][c], m[1][b][c], m[2][b][c], vx, vx2, kinv, k);
-        }
-    // backward y sweep (:537-545): rows (i, cz)
-    for (int i = 0; i < 3; ++i)
-        for (int c = 0; c < 3; ++c) {
-            if (c == 1) { bwd(m[i][0][c], m[i][1][c], m[i][2][c], vy, vy2); continue; }
-            double kinv, k;
-            if (i == 1) { kinv = (c == 0) ? c3o2 : c9o2; k = (c == 0) ? c2o3 : c2o9; }
-            else        { kinv = (c == 0) ? 6.0 : 18.0; k = (c == 0) ? c1o6 : c1o18; }
-            bwd_k(m[i][0][c], m[i][1][c], m[i][2][c], vy, vy2, kinv, k);
-        }
-    // backward z sweep (:549-557): rows (i,j)
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            double kinv, k;
-            if (i == 1 && j == 1) { kinv = c9o4; k = c4o9; }
-            else if (i == 1 || j == 1) { kinv = 9.0; k = c1o9; }
-            else { kinv = 36.0; k = c1o36; }
-            bwd_k(m[i][j][0], m[i][j][1], m[i][j][2], vz, vz2, kinv, k);
-        }
-
+    backward_sweeps(m, vx, vy, vz);
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
             for (int k = 0; k < 3; ++k) f[T.cube[i][j][k]] = m[i][j][k];
@@ -594,6 +604,240 @@ void pressure_bc(const double* fnb, double rho_bc, int normal, double* out, bool
     }
 }
 
+
+// ------------------------------------------------------------ refinement --
+// Interface interpolation between a coarse level and its 2:1 fine level
+// (src/lbm/interpolation/InterpolationCoefficients.h:74-432,
+// src/lbm/refinement/InterpolationCF.h:57-306, InterpolationFC.h:56-263).
+//
+// The 8 source nodes of an interpolation cell sit at the corners
+// (sx/2, sy/2, sz/2), s = +-1, of a unit cube centred on the cell; corner c
+// has bit 0 = +x, bit 1 = +y, bit 2 = +z (c = 0 is the reference's MMM, 7 is
+// PPP).  From each corner the reference takes drho, u and the five
+// non-equilibrium second moments k.. = -3 omega (sum c_a c_b f / rho - u_a u_b)
+// (kxx-yy, kxx-zz with -3/2 omega), :89-128.
+//
+// Each velocity component is then a polynomial
+//   v(x,y,z) = v000 + v100 x + v010 y + v001 z + v200 x^2 + v020 y^2 + v002 z^2
+//            + v110 xy + v101 xz + v011 yz + v111 xyz
+// whose trilinear part interpolates the 8 corner values and whose curvature
+// terms follow from the corner strain rates; drho is trilinear.  Written with
+// the corner sums  S q = sum_c q_c  and the signed sums  D_x q = sum_c s_x q_c,
+// D_xy q = sum_c s_x s_y q_c, ... the coefficients of :242-319 are:
+//   v100 = D_x v / 4,  v110 = D_xy v / 2,  v111 = D_xyz v          (all v, drho)
+//   a200 = (2 D_xy vy + 2 D_xz vz + D_x kxxMyy + D_x kxxMzz) / 16
+//   a020 = (D_y kxy - 2 D_xy vy) / 8,   a002 = (D_z kxz - 2 D_xz vz) / 8
+//   b200 = (D_x kxy - 2 D_xy vx) / 8,   b002 = (D_z kyz - 2 D_yz vz) / 8
+//   b020 = (2 D_xy vx + 2 D_yz vz - 2 D_y kxxMyy + D_y kxxMzz) / 16
+//   c200 = (D_x kxz - 2 D_xz vx) / 8,   c020 = (D_y kyz - 2 D_yz vy) / 8
+//   c002 = (2 D_xz vx + 2 D_yz vy - 2 D_z kxxMzz + D_z kxxMyy) / 16
+//   v000 = S v / 8 - (v200 + v020 + v002) / 4,   d000 = S drho / 8
+// (a, b, c: the x, y, z velocity).  These are algebraically the reference's
+// expressions; the association of the sums differs, so results agree to
+// rounding (checked against the compiled reference header).
+struct SourceMoments {
+    double drho, vx, vy, vz, kxy, kyz, kxz, kxxMyy, kxxMzz;
+};
+
+void source_moments(const double* f, double omega, SourceMoments& s)
+{
+    s.drho = density(f);
+    const double oneOverRho = 1.0 / (s.drho + 1.0);
+    s.vx = mom_x(f) * oneOverRho;
+    s.vy = mom_y(f) * oneOverRho;
+    s.vz = mom_z(f) * oneOverRho;
+    double pxy = 0, pyz = 0, pxz = 0, pxxMyy = 0, pxxMzz = 0;
+    for (int d = 1; d < 27; ++d) {
+        pxy += CX[d] * CY[d] * f[d];
+        pyz += CY[d] * CZ[d] * f[d];
+        pxz += CX[d] * CZ[d] * f[d];
+        pxxMyy += (CX[d] * CX[d] - CY[d] * CY[d]) * f[d];
+        pxxMzz += (CX[d] * CX[d] - CZ[d] * CZ[d]) * f[d];
+    }
+    s.kxy = -3.0 * omega * (pxy * oneOverRho - s.vx * s.vy);
+    s.kyz = -3.0 * omega * (pyz * oneOverRho - s.vy * s.vz);
+    s.kxz = -3.0 * omega * (pxz * oneOverRho - s.vx * s.vz);
+    s.kxxMyy = -1.5 * omega * (pxxMyy * oneOverRho - (s.vx * s.vx - s.vy * s.vy));
+    s.kxxMzz = -1.5 * omega * (pxxMzz * oneOverRho - (s.vx * s.vx - s.vz * s.vz));
+}
+
+// coefficient slots of one polynomial
+enum { P000, P100, P010, P001, P200, P020, P002, P110, P101, P011, P111, NP };
+
+struct Coefficients {
+    double a[NP], b[NP], c[NP], d[NP]; // vx, vy, vz, drho (d has no square terms)
+    double laplace_rho;
+};
+
+struct CornerSums {
+    double S, Dx, Dy, Dz, Dxy, Dxz, Dyz, Dxyz;
+};
+
+CornerSums corner_sums(const double q[8])
+{
+    CornerSums r{0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 8; ++c) {
+        const double sx = (c & 1) ? 1.0 : -1.0, sy = (c & 2) ? 1.0 : -1.0, sz = (c & 4) ? 1.0 : -1.0;
+        r.S += q[c];
+        r.Dx += sx * q[c];
+        r.Dy += sy * q[c];
+        r.Dz += sz * q[c];
+        r.Dxy += sx * sy * q[c];
+        r.Dxz += sx * sz * q[c];
+        r.Dyz += sy * sz * q[c];
+        r.Dxyz += sx * sy * sz * q[c];
+    }
+    return r;
+}
+
+void linear_part(const CornerSums& s, double* p)
+{
+    p[P100] = s.Dx / 4;
+    p[P010] = s.Dy / 4;
+    p[P001] = s.Dz / 4;
+    p[P110] = s.Dxy / 2;
+    p[P101] = s.Dxz / 2;
+    p[P011] = s.Dyz / 2;
+    p[P111] = s.Dxyz;
+}
+
+// MomentsOnSourceNodeSet::calculateCoefficients (:174-428), including the
+// Taylor shift to the offset point (xoff, yoff, zoff) of :389-424, which, as
+// in the reference, leaves the second-order coefficients and the xyz terms
+// unchanged and drops the xyz contributions to the shifted first-order terms.
+void interpolation_coefficients(const SourceMoments src[8], double xoff, double yoff, double zoff, Coefficients& k)
+{
+    double q[8];
+    auto sums = [&](double SourceMoments::*field) {
+        for (int c = 0; c < 8; ++c) q[c] = src[c].*field;
+        return corner_sums(q);
+    };
+    const CornerSums vx = sums(&SourceMoments::vx), vy = sums(&SourceMoments::vy), vz = sums(&SourceMoments::vz);
+    const CornerSums rho = sums(&SourceMoments::drho);
+    const CornerSums kxy = sums(&SourceMoments::kxy), kyz = sums(&SourceMoments::kyz), kxz = sums(&SourceMoments::kxz);
+    const CornerSums kMyy = sums(&SourceMoments::kxxMyy), kMzz = sums(&SourceMoments::kxxMzz);
+
+    double* a = k.a;
+    double* b = k.b;
+    double* c = k.c;
+    double* d = k.d;
+    linear_part(vx, a);
+    linear_part(vy, b);
+    linear_part(vz, c);
+    linear_part(rho, d);
+    a[P200] = (2 * vy.Dxy + 2 * vz.Dxz + kMyy.Dx + kMzz.Dx) / 16;
+    a[P020] = (kxy.Dy - 2 * vy.Dxy) / 8;
+    a[P002] = (kxz.Dz - 2 * vz.Dxz) / 8;
+    b[P200] = (kxy.Dx - 2 * vx.Dxy) / 8;
+    b[P020] = (2 * vx.Dxy + 2 * vz.Dyz - 2 * kMyy.Dy + kMzz.Dy) / 16;
+    b[P002] = (kyz.Dz - 2 * vz.Dyz) / 8;
+    c[P200] = (kxz.Dx - 2 * vx.Dxz) / 8;
+    c[P020] = (kyz.Dy - 2 * vy.Dyz) / 8;
+    c[P002] = (2 * vx.Dxz + 2 * vy.Dyz - 2 * kMzz.Dz + kMyy.Dz) / 16;
+    a[P000] = vx.S / 8 - (a[P200] + a[P020] + a[P002]) / 4;
+    b[P000] = vy.S / 8 - (b[P200] + b[P020] + b[P002]) / 4;
+    c[P000] = vz.S / 8 - (c[P200] + c[P020] + c[P002]) / 4;
+    d[P000] = rho.S / 8;
+    d[P200] = d[P020] = d[P002] = 0.0;
+
+    // :378-380: the pressure curvature is used only for an unshifted cell
+    k.laplace_rho = (xoff != 0.0 || yoff != 0.0 || zoff != 0.0)
+                        ? 0.0
+                        : -3.0 * (a[P100] * a[P100] + b[P010] * b[P010] + c[P001] * c[P001]) -
+                              6.0 * (b[P100] * a[P010] + c[P100] * a[P001] + c[P010] * b[P001]);
+
+    for (double* p : {a, b, c, d}) {
+        p[P000] += xoff * p[P100] + yoff * p[P010] + zoff * p[P001] + xoff * xoff * p[P200] + yoff * yoff * p[P020] +
+                   zoff * zoff * p[P002] + xoff * yoff * p[P110] + xoff * zoff * p[P101] + yoff * zoff * p[P011];
+        p[P100] += 2 * xoff * p[P200] + yoff * p[P110] + zoff * p[P101];
+        p[P010] += 2 * yoff * p[P020] + xoff * p[P110] + zoff * p[P011];
+        p[P001] += 2 * zoff * p[P002] + xoff * p[P101] + yoff * p[P011];
+    }
+}
+
+double poly_value(const double* p, double x, double y, double z)
+{
+    return p[P000] + x * p[P100] + y * p[P010] + z * p[P001] + x * x * p[P200] + y * y * p[P020] + z * z * p[P002] +
+           x * y * p[P110] + x * z * p[P101] + y * z * p[P011] + x * y * z * p[P111];
+}
+void poly_gradient(const double* p, double x, double y, double z, double g[3])
+{
+    g[0] = p[P100] + 2 * x * p[P200] + y * p[P110] + z * p[P101] + y * z * p[P111];
+    g[1] = p[P010] + 2 * y * p[P020] + x * p[P110] + z * p[P011] + x * z * p[P111];
+    g[2] = p[P001] + 2 * z * p[P002] + x * p[P101] + y * p[P011] + x * y * p[P111];
+}
+
+// Destination node from (drho, u, grad u): the second-order central moments
+// carry the non-equilibrium stress of the destination level, -(eps/omega)
+// (1+drho) S_ab with the source-level strain rate S and eps the ratio of grid
+// spacings (1/2 coarse->fine, 2 fine->coarse); third order is zero, fourth
+// and sixth order take their equilibrium values drho/9 and drho/27
+// (InterpolationCF.h:180-236, InterpolationFC.h:148-202); then back to
+// shifted populations (backward sweeps).
+void destination_populations(double drho, const double u[3], const double gx[3], const double gy[3], const double gz[3],
+                             double eps, double omega, double* f)
+{
+    // gx = grad vx, gy = grad vy, gz = grad vz
+    const double s = eps / omega * (1.0 + drho);
+    const double mxxMyy = -c2o3 * (gx[0] - gy[1]) * s;
+    const double mxxMzz = -c2o3 * (gx[0] - gz[2]) * s;
+    double m[3][3][3] = {};
+    m[0][0][0] = drho;
+    m[0][1][1] = -c1o3 * (gy[2] + gz[1]) * s;
+    m[1][0][1] = -c1o3 * (gx[2] + gz[0]) * s;
+    m[1][1][0] = -c1o3 * (gx[1] + gy[0]) * s;
+    m[2][0][0] = c1o3 * (mxxMyy + mxxMzz + drho);
+    m[0][2][0] = c1o3 * (-2.0 * mxxMyy + mxxMzz + drho);
+    m[0][0][2] = c1o3 * (mxxMyy - 2.0 * mxxMzz + drho);
+    m[0][2][2] = m[2][0][2] = m[2][2][0] = drho * c1o9;
+    m[2][2][2] = drho * c1o27;
+    backward_sweeps(m, u[0], u[1], u[2]);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int kk = 0; kk < 3; ++kk) f[T.cube[i][j][kk]] = m[i][j][kk];
+}
+
+// interpolateCF at (x, y, z) in cell units (fine nodes sit at +-1/4).  The
+// pressure carries the curvature term 3 x^2 laplace_rho of :168-169.
+void interpolate_cf(const Coefficients& k, double omega_fine, double eps, double x, double y, double z, double* f)
+{
+    const double drho = poly_value(k.d, x, y, z) + 3.0 * x * x * k.laplace_rho;
+    const double u[3] = {poly_value(k.a, x, y, z), poly_value(k.b, x, y, z), poly_value(k.c, x, y, z)};
+    double gx[3], gy[3], gz[3];
+    poly_gradient(k.a, x, y, z, gx);
+    poly_gradient(k.b, x, y, z, gy);
+    poly_gradient(k.c, x, y, z, gz);
+    destination_populations(drho, u, gx, gy, gz, eps, omega_fine, f);
+}
+
+// interpolateFC at the cell centre, pressure d000 - laplace_rho / 4 (:138)
+void interpolate_fc(const Coefficients& k, double omega_coarse, double eps, double* f)
+{
+    const double drho = k.d[P000] - 2.0 * k.laplace_rho * 0.125;
+    const double u[3] = {k.a[P000], k.b[P000], k.c[P000]};
+    const double gx[3] = {k.a[P100], k.a[P010], k.a[P001]};
+    const double gy[3] = {k.b[P100], k.b[P010], k.b[P001]};
+    const double gz[3] = {k.c[P100], k.c[P010], k.c[P001]};
+    destination_populations(drho, u, gx, gy, gz, eps, omega_coarse, f);
+}
+
+int64_t box_node(int x, int y, int z, int nx, int ny, int nz)
+{
+    x = ((x % nx) + nx) % nx;
+    y = ((y % ny) + ny) % ny;
+    z = ((z % nz) + nz) % nz;
+    return (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
+}
+
+void cell_moments(const double* F, int64_t base, int nx, int ny, int nz, double omega, SourceMoments src[8])
+{
+    const int bx = (int)(base % nx), by = (int)((base / nx) % ny), bz = (int)(base / ((int64_t)nx * ny));
+    for (int c = 0; c < 8; ++c) {
+        const int64_t k = box_node(bx + (c & 1), by + ((c >> 1) & 1), bz + ((c >> 2) & 1), nx, ny, nz);
+        source_moments(F + 27 * k, omega, src[c]);
+    }
+}
+
 } // namespace
 
 // ============================================================== C ABI =====
@@ -714,7 +958,9 @@ int orc_run(double* F, int nx, int ny, int nz, const uint8_t* solid, int collisi
             for (int d = 0; d < 27; ++d) {
                 const int64_t j = neighbour(i, d);
                 if (solid && solid[j]) {
-                    if (!covered[(size_t)i * 27 + d]) bad += 1;
+                    // solid = 2 marks an inactive node (e.g. a coarse node under a
+                    // finer level): links into it are dropped without a boundary
+                    if (solid[j] == 1 && !covered[(size_t)i * 27 + d]) bad += 1;
                     continue;
                 }
                 G[(size_t)j * 27 + d] = post[(size_t)i * 27 + d];
@@ -856,6 +1102,102 @@ int orc_num_threads(void)
 #else
     return 1;
 #endif
+}
+
+
+// ---- refinement (test infrastructure)
+void orc_source_moments(const double* f27, double omega, double* out9)
+{
+    SourceMoments s;
+    source_moments(f27, omega, s);
+    const double v[9] = {s.drho, s.vx, s.vy, s.vz, s.kxy, s.kyz, s.kxz, s.kxxMyy, s.kxxMzz};
+    std::memcpy(out9, v, sizeof(v));
+}
+
+// out42: a[11], b[11], c[11] in the slot order 000,100,010,001,200,020,002,
+// 110,101,011,111; d[8] in 000,100,010,001,110,101,011,111; laplace_rho
+static void pack_coefficients(const Coefficients& k, double* out42)
+{
+    int n = 0;
+    for (const double* p : {k.a, k.b, k.c})
+        for (int i = 0; i < NP; ++i) out42[n++] = p[i];
+    for (int i : {P000, P100, P010, P001, P110, P101, P011, P111}) out42[n++] = k.d[i];
+    out42[n] = k.laplace_rho;
+}
+static void unpack_coefficients(const double* in42, Coefficients& k)
+{
+    int n = 0;
+    for (double* p : {k.a, k.b, k.c})
+        for (int i = 0; i < NP; ++i) p[i] = in42[n++];
+    for (int i : {P000, P100, P010, P001, P110, P101, P011, P111}) k.d[i] = in42[n++];
+    k.d[P200] = k.d[P020] = k.d[P002] = 0.0;
+    k.laplace_rho = in42[n];
+}
+
+// f8x27: the 8 corner nodes (corner bit order) of one cell
+void orc_interp_coefficients(const double* f8x27, double omega, double xoff, double yoff, double zoff, double* out42)
+{
+    SourceMoments src[8];
+    for (int c = 0; c < 8; ++c) source_moments(f8x27 + 27 * c, omega, src[c]);
+    Coefficients k;
+    interpolation_coefficients(src, xoff, yoff, zoff, k);
+    pack_coefficients(k, out42);
+}
+
+void orc_interpolate_cf(const double* coeff42, double omega_fine, double eps, double x, double y, double z, double* f27)
+{
+    Coefficients k;
+    unpack_coefficients(coeff42, k);
+    interpolate_cf(k, omega_fine, eps, x, y, z, f27);
+}
+
+void orc_interpolate_fc(const double* coeff42, double omega_coarse, double eps, double* f27)
+{
+    Coefficients k;
+    unpack_coefficients(coeff42, k);
+    interpolate_fc(k, omega_coarse, eps, f27);
+}
+
+// Coarse-to-fine over a list of cells (scaleCoarseToFineCompressible.cu:305-529):
+// node-major pre-collision populations of periodic boxes; coarse cell with
+// corner MMM coarse_cells[i]; its 8 fine nodes form the fine cell with corner
+// MMM fine_cells[i] and receive the polynomial at (+-1/4, +-1/4, +-1/4)
+// (eps = 1/2, fine omega).  Offsets may be NULL (zero).
+void orc_scale_cf(const double* Fc, double* Ff, const int* dims_c, const int* dims_f, int64_t n_cells,
+                  const int64_t* coarse_cells, const int64_t* fine_cells, const double* ox, const double* oy,
+                  const double* oz, double omega_c, double omega_f)
+{
+    for (int64_t i = 0; i < n_cells; ++i) {
+        SourceMoments src[8];
+        cell_moments(Fc, coarse_cells[i], dims_c[0], dims_c[1], dims_c[2], omega_c, src);
+        Coefficients k;
+        interpolation_coefficients(src, ox ? ox[i] : 0.0, oy ? oy[i] : 0.0, oz ? oz[i] : 0.0, k);
+        const int64_t b = fine_cells[i];
+        const int nx = dims_f[0], ny = dims_f[1], nz = dims_f[2];
+        const int bx = (int)(b % nx), by = (int)((b / nx) % ny), bz = (int)(b / ((int64_t)nx * ny));
+        for (int c = 0; c < 8; ++c) {
+            const double x = (c & 1) ? 0.25 : -0.25, y = (c & 2) ? 0.25 : -0.25, z = (c & 4) ? 0.25 : -0.25;
+            const int64_t kf = box_node(bx + (c & 1), by + ((c >> 1) & 1), bz + ((c >> 2) & 1), nx, ny, nz);
+            interpolate_cf(k, omega_f, 0.5, x, y, z, Ff + 27 * kf);
+        }
+    }
+}
+
+// Fine-to-coarse (scaleFineToCoarseCompressible.cu:76-124): fine cell with
+// corner MMM fine_cells[i] -> coarse node coarse_nodes[i] at the cell centre
+// (eps = 2, coarse omega).
+void orc_scale_fc(double* Fc, const double* Ff, const int* dims_c, const int* dims_f, int64_t n_cells,
+                  const int64_t* coarse_nodes, const int64_t* fine_cells, const double* ox, const double* oy,
+                  const double* oz, double omega_c, double omega_f)
+{
+    (void)dims_c;
+    for (int64_t i = 0; i < n_cells; ++i) {
+        SourceMoments src[8];
+        cell_moments(Ff, fine_cells[i], dims_f[0], dims_f[1], dims_f[2], omega_f, src);
+        Coefficients k;
+        interpolation_coefficients(src, ox ? ox[i] : 0.0, oy ? oy[i] : 0.0, oz ? oz[i] : 0.0, k);
+        interpolate_fc(k, omega_c, 2.0, Fc + 27 * coarse_nodes[i]);
+    }
 }
 
 } // extern "C"

@@ -24,6 +24,9 @@
 #include <lbm/ChimeraTransformation.h>
 #include <lbm/MacroscopicQuantities.h>
 #include <lbm/collision/K17CompressibleNavierStokes.h>
+#include <lbm/interpolation/InterpolationCoefficients.h>
+#include <lbm/refinement/InterpolationCF.h>
+#include <lbm/refinement/InterpolationFC.h>
 
 #include <chrono>
 #include <cstdint>
@@ -239,3 +242,42 @@ double ref_time_blocks(int nblocks, int bs, double omega, int nsteps, int* threa
 }
 
 } // extern "C"
+
+// Refinement interpolation of one cell with the reference's header-only
+// src/lbm/interpolation/InterpolationCoefficients.h and
+// src/lbm/refinement/InterpolationCF.h / InterpolationFC.h.
+// f8x27: the 8 source nodes, corner c with bit 0 = +x, bit 1 = +y, bit 2 = +z
+// (c = 0 is MMM).  kind 0: coarse-to-fine, out = 8 x 27 (destination nodes at
+// (+-1/4, +-1/4, +-1/4), same corner order, eps = 1/2); kind 1: fine-to-coarse,
+// out = 27 (eps = 2).  coeff42 receives a, b, c (000,100,010,001,200,020,002,
+// 110,101,011,111), d (000,100,010,001,110,101,011,111), LaplaceRho.
+extern "C" void ref_interp_cell(int kind, const double* f8x27, double omega_src, double omega_dst, double xoff,
+                                double yoff, double zoff, double* coeff42, double* out)
+{
+    vf::lbm::MomentsOnSourceNodeSet set;
+    const double* f = f8x27;
+    set.calculateMMM(f + 27 * 0, omega_src);
+    set.calculatePMM(f + 27 * 1, omega_src);
+    set.calculateMPM(f + 27 * 2, omega_src);
+    set.calculatePPM(f + 27 * 3, omega_src);
+    set.calculateMMP(f + 27 * 4, omega_src);
+    set.calculatePMP(f + 27 * 5, omega_src);
+    set.calculateMPP(f + 27 * 6, omega_src);
+    set.calculatePPP(f + 27 * 7, omega_src);
+    vf::lbm::InterpolationCoefficients k;
+    set.calculateCoefficients(k, xoff, yoff, zoff);
+    const double packed[42] = {k.a000, k.a100, k.a010, k.a001, k.a200, k.a020, k.a002, k.a110, k.a101, k.a011, k.a111,
+                               k.b000, k.b100, k.b010, k.b001, k.b200, k.b020, k.b002, k.b110, k.b101, k.b011, k.b111,
+                               k.c000, k.c100, k.c010, k.c001, k.c200, k.c020, k.c002, k.c110, k.c101, k.c011, k.c111,
+                               k.d000, k.d100, k.d010, k.d001, k.d110, k.d101, k.d011, k.d111, k.LaplaceRho};
+    for (int i = 0; i < 42; ++i) coeff42[i] = packed[i];
+    if (kind == 0) {
+        for (int c = 0; c < 8; ++c) {
+            const double x = (c & 1) ? 0.25 : -0.25, y = (c & 2) ? 0.25 : -0.25, z = (c & 4) ? 0.25 : -0.25;
+            vf::lbm::interpolateCF(out + 27 * c, omega_dst, 0.5, k, x, y, z);
+        }
+    } else {
+        vf::lbm::interpolateFC(out, 2.0, omega_dst, k);
+    }
+}
+

@@ -42,13 +42,14 @@ def taylor_green(n, u0=0.04):
 
 
 def cavity(n, lid=(0.05, 0.0, 0.0)):
-    """Lid-driven cavity of n^3 fluid nodes inside one layer of solid nodes
-    ((n+2)^3 box, x fastest).  Every fluid->solid link is a halfway bounce-back
-    link (q = 0.5); links into the z = n+1 plane move with `lid`.
+    """Lid-driven cavity of n^3 (or nx*ny*nz for a tuple n) fluid nodes inside
+    one layer of solid nodes ((nx+2)(ny+2)(nz+2) box, x fastest).  Every
+    fluid->solid link is a halfway bounce-back link (q = 0.5); links into the
+    z = nz+1 plane move with `lid`.
 
-    Returns (solid[(n+2)^3] bool, [(nodes, q27, velocity) for lid, walls])."""
-    m = n + 2
-    solid = np.zeros((m, m, m), dtype=bool)
+    Returns (solid[box] bool, [(nodes, q27, velocity) for lid, walls])."""
+    nx, ny, nz = (n, n, n) if np.isscalar(n) else tuple(n)
+    solid = np.zeros((nz + 2, ny + 2, nx + 2), dtype=bool)
     solid[0, :, :] = solid[-1, :, :] = True
     solid[:, 0, :] = solid[:, -1, :] = True
     solid[:, :, 0] = solid[:, :, -1] = True
@@ -60,8 +61,8 @@ def cavity(n, lid=(0.05, 0.0, 0.0)):
         for d in range(1, 27):
             tz, ty, tx = z + CZ[d], y + CY[d], x + CX[d]
             if solid[tz, ty, tx]:
-                (ql if tz == m - 1 else qw)[d] = 0.5
-        k = x + m * (y + m * z)
+                (ql if tz == nz + 1 else qw)[d] = 0.5
+        k = x + (nx + 2) * (y + (ny + 2) * z)
         if (ql >= 0).any():
             groups["lid"][0].append(k)
             groups["lid"][1].append(ql)
@@ -73,6 +74,9 @@ def cavity(n, lid=(0.05, 0.0, 0.0)):
         out.append((np.array(nodes, dtype=np.int64), np.stack(qs, axis=1),
                     lid if name == "lid" else (0.0, 0.0, 0.0)))
     return solid.ravel(), out
+
+
+from virtualfluids_amd.batch import cavity_ensemble  # noqa: E402,F401
 
 
 def sparse_layout(dims, solid=None, seed=0):

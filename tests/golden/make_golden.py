@@ -11,6 +11,12 @@ Fixtures (inputs and reference outputs only; parameters in golden.json):
                     EsoSplit (K17CompressibleNavierStokes.cpp:77-162), 10 and 100 steps
   g3_cavity.npz     10^3 lid-driven cavity, VelocityBounceBack strategy
                     (VelocityBounceBack.cpp:59-79) on every wall link, 50 steps
+  g4_interp.npz     refinement interpolation of 128 random 8-node cells with the
+                    header-only MomentsOnSourceNodeSet / interpolateCF / interpolateFC
+                    (InterpolationCoefficients.h:74-432, InterpolationCF.h:57-306,
+                    InterpolationFC.h:56-263)
+
+`python tests/golden/make_golden.py g4` writes only the named fixtures.
 """
 from __future__ import annotations
 
@@ -71,12 +77,51 @@ def g3(n=10, omega=1.6778523, lid=(0.05, 0.0, 0.0), steps=50):
             "layout": "fluid nodes only, x fastest; walls are the surrounding layer"}
 
 
+def g4(seed=4242, n=128):
+    """Cells of near-equilibrium corner states with random drho, velocity and
+    non-equilibrium parts; every other cell carries a random offset (the
+    extrapolation used next to walls)."""
+    rng = np.random.default_rng(seed)
+    f8 = np.zeros((n, 8, 27))
+    offsets = np.zeros((n, 3))
+    omega = np.zeros((n, 2))  # (coarse, fine)
+    for i in range(n):
+        u0 = rng.uniform(-0.05, 0.05, 3)
+        grad = rng.uniform(-0.02, 0.02, (3, 3))
+        for c in range(8):
+            s = np.array([(1 if c & 1 else -1), (1 if c & 2 else -1), (1 if c & 4 else -1)]) * 0.5
+            u = u0 + grad @ s
+            f8[i, c] = oracle.equilibrium(rng.uniform(-0.01, 0.01), *u) + rng.normal(0, 1e-3, 27) * W
+        if i % 2:
+            offsets[i] = rng.uniform(-0.5, 0.5, 3)
+        wc = rng.uniform(1.6, 1.99)
+        omega[i] = (wc, 1.0 / (2.0 / wc - 0.5))  # fine omega for nu_f = 2 nu_c
+    coeff_cf = np.zeros((n, 42))
+    coeff_fc = np.zeros((n, 42))
+    cf = np.zeros((n, 8, 27))
+    fc = np.zeros((n, 27))
+    for i in range(n):
+        coeff_cf[i], cf[i] = oracle.interp_cell("cf", f8[i], omega[i, 0], omega[i, 1], offsets[i], impl="ref")
+        coeff_fc[i], fc[i] = oracle.interp_cell("fc", f8[i], omega[i, 1], omega[i, 0], offsets[i], impl="ref")
+    np.savez(os.path.join(HERE, "g4_interp.npz"), f8=f8, offsets=offsets, omega=omega, coeff_cf=coeff_cf,
+             coeff_fc=coeff_fc, cf=cf, fc=fc)
+    return {"seed": seed, "n": n, "omega": "coarse U[1.6,1.99], fine 1/(2/omega_c - 1/2)",
+            "cf": "coarse moments (omega_c) -> 8 fine nodes at +-1/4, eps 1/2, omega_f",
+            "fc": "fine moments (omega_f) -> cell centre, eps 2, omega_c"}
+
+
+GENERATORS = {"g1": g1, "g2": g2, "g3": g3, "g4": g4}
+
+
 def main():
     if not oracle.ref_available():
         oracle.build(ref=True)
-    meta = {"generator": "tests/golden/make_golden.py", "reference": "oracle/_ref/libvfref.so",
-            "g1": g1(), "g2": g2(), "g3": g3()}
-    with open(os.path.join(HERE, "golden.json"), "w") as fh:
+    path = os.path.join(HERE, "golden.json")
+    meta = json.load(open(path)) if os.path.exists(path) else {}
+    meta.update({"generator": "tests/golden/make_golden.py", "reference": "oracle/_ref/libvfref.so"})
+    for name in sys.argv[1:] or list(GENERATORS):
+        meta[name] = GENERATORS[name]()
+    with open(path, "w") as fh:
         json.dump(meta, fh, indent=1)
     print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
 

@@ -50,6 +50,36 @@ def test_invalid_arguments_fail_without_device():
     assert L.vfx_version() >= 1
 
 
+def test_batch_arguments_fail_without_device():
+    L = vfx.lib()
+    V = vfx._lib
+    doms = (V.VfxDomain * 2)()
+    for d in doms:
+        d.dims[:] = [8, 8, 8]
+        d.omega = 1.8
+    doms[1].f_offset = 27 * 512
+    h = C.c_void_p()
+    assert L.vfx_batch_create(None, 27 * 1024, 2, 1024.0, doms, 2, None, 0, C.byref(h)) == V.VFX_EINVAL  # NULL pool
+    assert L.vfx_batch_create(1, 27 * 1024, 2, 1024.0, doms, 0, None, 0, C.byref(h)) == V.VFX_EINVAL  # no domains
+    assert L.vfx_batch_create(1, 27 * 1024, 2, 0.0, doms, 2, None, 0, C.byref(h)) == V.VFX_EINVAL  # FP16 scale
+    assert L.vfx_batch_create(1, 27 * 1024, 2, 1024.0, doms, 2, None, 5, C.byref(h)) == V.VFX_EINVAL  # variant
+    doms[1].dims[:] = [8, 2, 8]
+    assert L.vfx_batch_create(1, 27 * 1024, 2, 1024.0, doms, 2, None, 0, C.byref(h)) == V.VFX_EINVAL
+    assert b"domain 1" in L.vfx_last_error()
+    doms[1].dims[:] = [8, 8, 8]
+    doms[0].omega = 2.0
+    assert L.vfx_batch_create(1, 27 * 1024, 2, 1024.0, doms, 2, None, 0, C.byref(h)) == V.VFX_EINVAL
+    doms[0].omega = 1.8
+    assert L.vfx_batch_create(1, 27 * 1024 - 1, 2, 1024.0, doms, 2, None, 0, C.byref(h)) == V.VFX_EINVAL
+    assert b"beyond the pool" in L.vfx_last_error()
+    doms[1].f_offset = 27 * 511
+    assert L.vfx_batch_create(1, 27 * 1024, 2, 1024.0, doms, 2, None, 0, C.byref(h)) == V.VFX_EINVAL
+    assert b"overlap" in L.vfx_last_error()
+    assert h.value is None
+    assert L.vfx_batch_step(None, 1, None) == V.VFX_EINVAL
+    assert L.vfx_graph_launch(None, None) == V.VFX_EINVAL
+
+
 def build_host_test():
     cmd = ["g++", "-O1", "-std=c++17", TEST_SRC, "-o", TEST_BIN, "-L" + os.path.dirname(vfx._lib.LIB_PATH), "-lvfx",
            "-Wl,-rpath," + os.path.dirname(vfx._lib.LIB_PATH)]

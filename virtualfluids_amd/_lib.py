@@ -42,6 +42,16 @@ class VfxBC(C.Structure):
     ]
 
 
+class VfxDomain(C.Structure):
+    _fields_ = [("f_offset", C.c_uint64), ("dims", C.c_uint32 * 3), ("omega", C.c_double),
+                ("lid_velocity", C.c_double * 3)]
+
+
+class VfxICells(C.Structure):
+    _fields_ = [("fine_cells", C.c_void_p), ("coarse_cells", C.c_void_p), ("n_cells", C.c_uint32),
+                ("offset_x", C.c_void_p), ("offset_y", C.c_void_p), ("offset_z", C.c_void_p)]
+
+
 class VfxError(RuntimeError):
     pass
 
@@ -63,6 +73,24 @@ _SIGNATURES = {
     "vfx_kinetic_energy": [C.POINTER(VfxLattice)] + [C.c_void_p] * 7,
     "vfx_enstrophy": [C.POINTER(VfxLattice)] + [C.c_void_p] * 7,
     "vfx_accumulate_stats": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
+    "vfx_scale_cf": [C.POINTER(VfxLattice), C.POINTER(VfxLattice), C.POINTER(VfxICells), C.c_int32, C.c_int32,
+                     C.c_void_p],
+    "vfx_scale_fc": [C.POINTER(VfxLattice), C.POINTER(VfxLattice), C.POINTER(VfxICells), C.c_int32, C.c_int32,
+                     C.c_void_p],
+    "vfx_batch_create": [C.c_void_p, C.c_uint64, C.c_int32, C.c_float, C.POINTER(VfxDomain), C.c_uint32, C.c_void_p, C.c_int32,
+                         C.POINTER(C.c_void_p)],
+    "vfx_batch_destroy": [C.c_void_p],
+    "vfx_batch_init_rest": [C.c_void_p, C.c_void_p],
+    "vfx_batch_set_launch": [C.c_void_p, C.c_uint32, C.c_uint32],
+    "vfx_batch_get_launch": [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
+    "vfx_batch_autotune": [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                           C.POINTER(C.c_float)],
+    "vfx_batch_step": [C.c_void_p, C.c_int32, C.c_void_p],
+    "vfx_batch_step_domain": [C.c_void_p, C.c_uint32, C.c_int32, C.c_void_p],
+    "vfx_batch_lattice": [C.c_void_p, C.c_uint32, C.POINTER(VfxLattice)],
+    "vfx_batch_capture": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)],
+    "vfx_graph_launch": [C.c_void_p, C.c_void_p],
+    "vfx_graph_destroy": [C.c_void_p],
 }
 
 

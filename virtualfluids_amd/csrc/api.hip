@@ -6,10 +6,13 @@
 // vfx_status; the message is kept per host thread for vfx_last_error().
 #include "kernels.hpp"
 
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
+#include <vector>
 
 namespace vfx {
 // the kernels live in step_fp64.hip / step_fp32.hip (built with their own FP flags)
@@ -316,6 +319,406 @@ vfx_status vfx_accumulate_stats(const vfx_lattice* lat, int32_t is_even, double*
     default: e = vfx::launch_stats<_Float16>(L, lat->direct, is_even != 0, sums9, st); break;
     }
     return hip_status(e, "statistics launch");
+}
+
+} // extern "C"
+
+// --------------------------------------------------------------- refinement --
+namespace {
+
+vfx_status scale(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells, int32_t even_c,
+                 int32_t even_f, vfx_stream stream, bool coarse_to_fine)
+{
+    vfx::Lat C, F;
+    if (vfx_status s = make_lat(coarse, C)) return s;
+    if (vfx_status s = make_lat(fine, F)) return s;
+    if (coarse->storage != fine->storage) return fail(VFX_EINVAL, "coarse and fine storage differ");
+    if (!cells) return fail(VFX_EINVAL, "cells is NULL");
+    if (cells->n_cells == 0) return VFX_OK;
+    if (!cells->fine_cells || !cells->coarse_cells) return fail(VFX_EINVAL, "cell index arrays are NULL");
+    vfx::ICellsDev I;
+    I.coarse = cells->coarse_cells;
+    I.fine = cells->fine_cells;
+    I.ox = cells->offset_x;
+    I.oy = cells->offset_y;
+    I.oz = cells->offset_z;
+    I.n = cells->n_cells;
+    I.direct_c = coarse->direct;
+    I.direct_f = fine->direct;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    switch (coarse->storage) {
+    case VFX_FP64:
+        e = vfx::launch_scale<double>(C, F, I, coarse_to_fine, even_c != 0, even_f != 0, coarse->omega, fine->omega, st);
+        break;
+    case VFX_FP32:
+        e = vfx::launch_scale<float>(C, F, I, coarse_to_fine, even_c != 0, even_f != 0, coarse->omega, fine->omega, st);
+        break;
+    default:
+        e = vfx::launch_scale<_Float16>(C, F, I, coarse_to_fine, even_c != 0, even_f != 0, coarse->omega, fine->omega,
+                                        st);
+        break;
+    }
+    return hip_status(e, coarse_to_fine ? "scale coarse to fine" : "scale fine to coarse");
+}
+
+} // namespace
+
+extern "C" {
+
+vfx_status vfx_scale_cf(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells,
+                        int32_t coarse_is_even, int32_t fine_is_even, vfx_stream stream)
+{
+    return scale(coarse, fine, cells, coarse_is_even, fine_is_even, stream, true);
+}
+
+vfx_status vfx_scale_fc(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells,
+                        int32_t coarse_is_even, int32_t fine_is_even, vfx_stream stream)
+{
+    return scale(coarse, fine, cells, coarse_is_even, fine_is_even, stream, false);
+}
+
+} // extern "C"
+
+// ------------------------------------------------------------------ batches --
+struct vfx_batch_s {
+    int32_t storage;
+    int32_t variant;
+    void* pool;
+    uint64_t pool_elements;
+    uint32_t n;
+    std::vector<vfx_domain> domains;
+    void* dev_domains = nullptr;               // DomainDev<T>[n]
+    uint2* dev_tiles[3] = {nullptr, nullptr, nullptr}; // tile tables for blocks 64/128/256
+    std::vector<uint32_t> first_tile[3];       // per domain, n+1 entries
+    uint32_t ntiles[3] = {0, 0, 0};
+    unsigned block = 256, grid = 0;
+    float ld[4], st[4];
+};
+
+struct vfx_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int block_index(unsigned b) { return b == 64 ? 0 : b == 128 ? 1 : b == 256 ? 2 : -1; }
+
+template <typename T> void fill_domains(const vfx_batch_s& b, const double lim[3], std::vector<char>& out)
+{
+    out.resize(sizeof(vfx::DomainDev<T>) * b.n);
+    auto* d = (vfx::DomainDev<T>*)out.data();
+    const double zero[3] = {0, 0, 0};
+    for (uint32_t i = 0; i < b.n; ++i) {
+        const vfx_domain& s = b.domains[i];
+        vfx::DomainDev<T> e{};
+        e.f_off = s.f_offset;
+        e.nx = s.dims[0];
+        e.ny = s.dims[1];
+        e.nz = s.dims[2];
+        e.N = s.dims[0] * s.dims[1] * s.dims[2];
+        e.nfx = s.dims[0] - 2;
+        e.nfy = s.dims[1] - 2;
+        e.nfluid = (s.dims[0] - 2) * (s.dims[1] - 2) * (s.dims[2] - 2);
+        for (int c = 0; c < 3; ++c) e.lid[c] = (T)s.lid_velocity[c];
+        e.R = vfx::make_rates<T>(s.omega, lim, zero);
+        d[i] = e;
+    }
+}
+
+vfx::BatchDev batch_view(const vfx_batch_s* b, int bi, uint32_t first, uint32_t count)
+{
+    vfx::BatchDev B;
+    B.pool = b->pool;
+    B.domains = b->dev_domains;
+    B.tiles = b->dev_tiles[bi] + first;
+    B.ntiles = count;
+    for (int c = 0; c < 4; ++c) {
+        B.ld[c] = b->ld[c];
+        B.st[c] = b->st[c];
+    }
+    return B;
+}
+
+hipError_t launch_batch(const vfx_batch_s* b, const vfx::BatchDev& B, bool even, unsigned block, unsigned grid,
+                        hipStream_t st)
+{
+    if (B.ntiles == 0) return hipSuccess;
+    const unsigned g = grid == 0 || grid > B.ntiles ? B.ntiles : grid;
+    switch (b->storage) {
+    case VFX_FP64: return vfx::launch_batched<double>(B, even, b->variant, block, g, st);
+    case VFX_FP32: return vfx::launch_batched<float>(B, even, b->variant, block, g, st);
+    default: return vfx::launch_batched<_Float16>(B, even, b->variant, block, g, st);
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+vfx_status vfx_batch_create(void* pool, uint64_t pool_elements, int32_t storage, float fp16_scale,
+                            const vfx_domain* domains, uint32_t n_domains, const double* quadric_limiter,
+                            int32_t kernel_variant, vfx_batch* out)
+{
+    if (!out) return fail(VFX_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (!pool || !domains || n_domains == 0) return fail(VFX_EINVAL, "pool/domains empty");
+    if (storage < VFX_FP64 || storage > VFX_FP16_SCALED) return fail(VFX_EINVAL, "unknown storage");
+    if (storage == VFX_FP16_SCALED && !(fp16_scale > 0.f)) return fail(VFX_EINVAL, "fp16_scale must be > 0");
+    if (kernel_variant != 0 && kernel_variant != 1) return fail(VFX_EINVAL, "unknown kernel_variant");
+    const uint64_t esz = storage == VFX_FP64 ? 8 : storage == VFX_FP32 ? 4 : 2;
+    for (uint32_t i = 0; i < n_domains; ++i) {
+        const vfx_domain& d = domains[i];
+        if (d.dims[0] < 3 || d.dims[1] < 3 || d.dims[2] < 3) return fail(VFX_EINVAL, "domain %u: dims must be >= 3", i);
+        const uint64_t N = (uint64_t)d.dims[0] * d.dims[1] * d.dims[2];
+        if (N * esz >= (1ull << 32)) return fail(VFX_EUNSUPPORTED, "domain %u too large", i);
+        if (!(d.omega > 0.0 && d.omega < 2.0)) return fail(VFX_EINVAL, "domain %u: omega %g outside (0,2)", i, d.omega);
+        if (d.f_offset > pool_elements || 27 * N > pool_elements - d.f_offset)
+            return fail(VFX_EINVAL, "domain %u: planes end beyond the pool", i);
+    }
+    {
+        std::vector<std::pair<uint64_t, uint64_t>> r(n_domains);
+        for (uint32_t i = 0; i < n_domains; ++i)
+            r[i] = {domains[i].f_offset,
+                    domains[i].f_offset + 27ull * domains[i].dims[0] * domains[i].dims[1] * domains[i].dims[2]};
+        std::sort(r.begin(), r.end());
+        for (uint32_t i = 1; i < n_domains; ++i)
+            if (r[i].first < r[i - 1].second) return fail(VFX_EINVAL, "domains overlap in the pool");
+    }
+    const double defl[3] = {0.01, 0.01, 0.01};
+    const double* lim = quadric_limiter ? quadric_limiter : defl;
+    auto* b = new vfx_batch_s();
+    b->storage = storage;
+    b->variant = kernel_variant;
+    b->pool = pool;
+    b->pool_elements = pool_elements;
+    b->n = n_domains;
+    b->domains.assign(domains, domains + n_domains);
+    for (int c = 0; c < 4; ++c) {
+        if (storage == VFX_FP16_SCALED) {
+            b->ld[c] = (float)(kW[c] / (double)fp16_scale);
+            b->st[c] = (float)((double)fp16_scale / kW[c]);
+        } else {
+            b->ld[c] = b->st[c] = 1.f;
+        }
+    }
+    std::vector<char> host;
+    if (storage == VFX_FP64) fill_domains<double>(*b, lim, host);
+    else fill_domains<float>(*b, lim, host);
+    hipError_t e = hipMalloc(&b->dev_domains, host.size());
+    if (e == hipSuccess) e = hipMemcpy(b->dev_domains, host.data(), host.size(), hipMemcpyHostToDevice);
+    const unsigned blocks[3] = {64, 128, 256};
+    for (int bi = 0; bi < 3 && e == hipSuccess; ++bi) {
+        std::vector<uint2> tiles;
+        b->first_tile[bi].assign(n_domains + 1, 0);
+        for (uint32_t i = 0; i < n_domains; ++i) {
+            b->first_tile[bi][i] = (uint32_t)tiles.size();
+            const vfx_domain& d = domains[i];
+            const uint32_t nf = (d.dims[0] - 2) * (d.dims[1] - 2) * (d.dims[2] - 2);
+            for (uint32_t s = 0; s < nf; s += blocks[bi]) tiles.push_back(make_uint2(i, s));
+        }
+        b->first_tile[bi][n_domains] = (uint32_t)tiles.size();
+        b->ntiles[bi] = (uint32_t)tiles.size();
+        e = hipMalloc((void**)&b->dev_tiles[bi], tiles.size() * sizeof(uint2));
+        if (e == hipSuccess) e = hipMemcpy(b->dev_tiles[bi], tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        vfx_batch_destroy(b);
+        return hip_status(e, "batch tables");
+    }
+    *out = b;
+    return VFX_OK;
+}
+
+vfx_status vfx_batch_destroy(vfx_batch b)
+{
+    if (!b) return VFX_OK;
+    if (b->dev_domains) (void)hipFree(b->dev_domains);
+    for (auto* p : b->dev_tiles)
+        if (p) (void)hipFree(p);
+    delete b;
+    return VFX_OK;
+}
+
+vfx_status vfx_batch_set_launch(vfx_batch b, uint32_t block, uint32_t grid)
+{
+    if (!b) return fail(VFX_EINVAL, "batch is NULL");
+    if (block_index(block) < 0) return fail(VFX_EINVAL, "block must be 64, 128 or 256");
+    b->block = block;
+    b->grid = grid;
+    return VFX_OK;
+}
+
+vfx_status vfx_batch_get_launch(vfx_batch b, uint32_t* block, uint32_t* grid)
+{
+    if (!b || !block || !grid) return fail(VFX_EINVAL, "NULL argument");
+    *block = b->block;
+    *grid = b->grid;
+    return VFX_OK;
+}
+
+vfx_status vfx_batch_init_rest(vfx_batch b, vfx_stream stream)
+{
+    if (!b) return fail(VFX_EINVAL, "batch is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    switch (b->storage) {
+    case VFX_FP64: e = vfx::launch_batch_zero<double>(b->pool, b->pool_elements, st); break;
+    case VFX_FP32: e = vfx::launch_batch_zero<float>(b->pool, b->pool_elements, st); break;
+    default: e = vfx::launch_batch_zero<_Float16>(b->pool, b->pool_elements, st); break;
+    }
+    return hip_status(e, "batch init");
+}
+
+vfx_status vfx_batch_step(vfx_batch b, int32_t is_even, vfx_stream stream)
+{
+    if (!b) return fail(VFX_EINVAL, "batch is NULL");
+    const int bi = block_index(b->block);
+    return hip_status(launch_batch(b, batch_view(b, bi, 0, b->ntiles[bi]), is_even != 0, b->block, b->grid,
+                                   (hipStream_t)stream),
+                      "batch step");
+}
+
+vfx_status vfx_batch_step_domain(vfx_batch b, uint32_t domain, int32_t is_even, vfx_stream stream)
+{
+    if (!b) return fail(VFX_EINVAL, "batch is NULL");
+    if (domain >= b->n) return fail(VFX_EINVAL, "domain %u out of range", domain);
+    const int bi = block_index(b->block);
+    const uint32_t first = b->first_tile[bi][domain], count = b->first_tile[bi][domain + 1] - first;
+    return hip_status(launch_batch(b, batch_view(b, bi, first, count), is_even != 0, b->block, 0, (hipStream_t)stream),
+                      "batch domain step");
+}
+
+vfx_status vfx_batch_lattice(vfx_batch b, uint32_t domain, vfx_lattice* out)
+{
+    if (!b || !out) return fail(VFX_EINVAL, "NULL argument");
+    if (domain >= b->n) return fail(VFX_EINVAL, "domain out of range");
+    const vfx_domain& d = b->domains[domain];
+    const uint64_t esz = b->storage == VFX_FP64 ? 8 : b->storage == VFX_FP32 ? 4 : 2;
+    std::memset(out, 0, sizeof(*out));
+    out->f = (char*)b->pool + d.f_offset * esz;
+    out->storage = b->storage;
+    out->fp16_scale = b->storage == VFX_FP16_SCALED ? (float)(kW[0] / (double)b->ld[0]) : 1.f;
+    out->n_nodes = (uint64_t)d.dims[0] * d.dims[1] * d.dims[2];
+    out->direct = 1;
+    for (int c = 0; c < 3; ++c) out->dims[c] = d.dims[c];
+    out->omega = d.omega;
+    out->quadric_limiter[0] = out->quadric_limiter[1] = out->quadric_limiter[2] = 0.01;
+    return VFX_OK;
+}
+
+// Times `steps` batched steps per candidate launch shape (blocks 64/128/256,
+// one workgroup per tile or a persistent grid of 2/4/8 workgroups per CU-slot)
+// with HIP events and keeps the fastest.  The state advances during tuning,
+// starting from the parity is_even; every trial runs an even number of steps,
+// so the parity is the same afterwards.
+vfx_status vfx_batch_autotune(vfx_batch b, int32_t is_even, int32_t steps, vfx_stream stream, uint32_t* block_out,
+                              uint32_t* grid_out, float* ms_out)
+{
+    if (!b) return fail(VFX_EINVAL, "batch is NULL");
+    if (steps < 2) steps = 2;
+    steps += steps & 1;
+    const bool even0 = is_even != 0;
+    hipStream_t st = (hipStream_t)stream;
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return fail(VFX_EHIP, "event create");
+    }
+    const unsigned keep_block = b->block, keep_grid = b->grid;
+    float best = 1e30f;
+    unsigned bb = keep_block, bg = keep_grid;
+    vfx_status r = VFX_OK;
+    const unsigned blocks[3] = {64, 128, 256};
+    for (unsigned block : blocks) {
+        const unsigned grids[4] = {0, (unsigned)cus * 8, (unsigned)cus * 16, (unsigned)cus * 32};
+        for (unsigned grid : grids) {
+            b->block = block;
+            b->grid = grid;
+            for (int s = 0; s < 2 && r == VFX_OK; ++s) r = vfx_batch_step(b, (s % 2 == 0) == even0, stream); // warm-up
+            if (r == VFX_OK) r = hip_status(hipEventRecord(e0, st), "event record");
+            for (int s = 0; s < steps && r == VFX_OK; ++s) r = vfx_batch_step(b, (s % 2 == 0) == even0, stream);
+            if (r == VFX_OK) r = hip_status(hipEventRecord(e1, st), "event record");
+            if (r == VFX_OK) r = hip_status(hipEventSynchronize(e1), "event synchronize");
+            float ms = 0;
+            if (r == VFX_OK) r = hip_status(hipEventElapsedTime(&ms, e0, e1), "event elapsed");
+            if (r != VFX_OK) break;
+            ms /= steps;
+            if (ms < best) {
+                best = ms;
+                bb = block;
+                bg = grid;
+            }
+        }
+        if (r != VFX_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r != VFX_OK) {
+        b->block = keep_block;
+        b->grid = keep_grid;
+        return r;
+    }
+    b->block = bb;
+    b->grid = bg;
+    if (block_out) *block_out = bb;
+    if (grid_out) *grid_out = bg;
+    if (ms_out) *ms_out = best;
+    return VFX_OK;
+}
+
+// Captures n_steps batched steps (parity alternating from is_even_start) --
+// or, with per_domain = 1, n_steps x n_domains single-domain launches -- into
+// a HIP graph.  Capture runs on a private stream; the graph may be launched on
+// any stream.
+vfx_status vfx_batch_capture(vfx_batch b, int32_t n_steps, int32_t is_even_start, int32_t per_domain, vfx_graph* out)
+{
+    if (!b || !out) return fail(VFX_EINVAL, "NULL argument");
+    if (n_steps < 1) return fail(VFX_EINVAL, "n_steps must be >= 1");
+    *out = nullptr;
+    hipStream_t cs;
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return fail(VFX_EHIP, "stream create");
+    auto* g = new vfx_graph_s();
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    bool even = is_even_start != 0;
+    vfx_status s = VFX_OK;
+    for (int i = 0; i < n_steps && e == hipSuccess && s == VFX_OK; ++i) {
+        if (per_domain) {
+            for (uint32_t d = 0; d < b->n && s == VFX_OK; ++d) s = vfx_batch_step_domain(b, d, even, cs);
+        } else {
+            s = vfx_batch_step(b, even, cs);
+        }
+        even = !even;
+    }
+    hipError_t e2 = hipStreamEndCapture(cs, &g->graph);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess && s == VFX_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    if (s != VFX_OK || e != hipSuccess) {
+        vfx_graph_destroy(g);
+        return s != VFX_OK ? s : hip_status(e, "graph capture");
+    }
+    *out = g;
+    return VFX_OK;
+}
+
+vfx_status vfx_graph_launch(vfx_graph g, vfx_stream stream)
+{
+    if (!g || !g->exec) return fail(VFX_EINVAL, "graph is NULL");
+    return hip_status(hipGraphLaunch(g->exec, (hipStream_t)stream), "graph launch");
+}
+
+vfx_status vfx_graph_destroy(vfx_graph g)
+{
+    if (!g) return VFX_OK;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return VFX_OK;
 }
 
 } // extern "C"

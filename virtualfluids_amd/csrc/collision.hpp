@@ -129,6 +129,70 @@ template <typename T> struct KPair { T kinv, k; };
 
 #define VFX_M(a, b, c) m[cube(a, b, c)]
 
+// density deviation and momentum of cube-ordered populations in the
+// reference's pyramid order (MacroscopicQuantities.h:55-85,117-127)
+template <typename T>
+__device__ __forceinline__ void pyramid_sums(const T (&m)[27], T& drho, T& jx, T& jy, T& jz)
+{
+#define F(d) m[slot_of(d)]
+    const T corners = ((F(dPPP) + F(dMMM)) + (F(dMPM) + F(dPMP))) + ((F(dMPP) + F(dPMM)) + (F(dMMP) + F(dPPM)));
+    const T edges = (((F(d0MP) + F(d0PM)) + (F(d0MM) + F(d0PP))) + ((F(dM0P) + F(dP0M)) + (F(dM0M) + F(dP0P)))) +
+                    ((F(dMP0) + F(dPM0)) + (F(dMM0) + F(dPP0)));
+    const T faces = ((F(dM00) + F(dP00)) + (F(d0M0) + F(d0P0))) + (F(d00M) + F(d00P));
+    drho = ((corners + edges) + F(d000)) + faces;
+    jx = (((F(dPPP) - F(dMMM)) + (F(dPMP) - F(dMPM))) + ((F(dPMM) - F(dMPP)) + (F(dPPM) - F(dMMP)))) +
+         (((F(dP0M) - F(dM0P)) + (F(dP0P) - F(dM0M))) + ((F(dPM0) - F(dMP0)) + (F(dPP0) - F(dMM0)))) +
+         (F(dP00) - F(dM00));
+    jy = (((F(dPPP) - F(dMMM)) + (F(dMPM) - F(dPMP))) + ((F(dMPP) - F(dPMM)) + (F(dPPM) - F(dMMP)))) +
+         (((F(d0PM) - F(d0MP)) + (F(d0PP) - F(d0MM))) + ((F(dMP0) - F(dPM0)) + (F(dPP0) - F(dMM0)))) +
+         (F(d0P0) - F(d0M0));
+    jz = (((F(dPPP) - F(dMMM)) + (F(dPMP) - F(dMPM))) + ((F(dMPP) - F(dPMM)) + (F(dMMP) - F(dPPM)))) +
+         (((F(d0MP) - F(d0PM)) + (F(d0PP) - F(d0MM))) + ((F(dM0P) - F(dP0M)) + (F(dP0P) - F(dM0M)))) +
+         (F(d00P) - F(d00M));
+#undef F
+}
+
+// central moments (orders (i,j,k) at cube(i,j,k)) -> cube-ordered shifted
+// populations: backward chimera sweeps x, y, z with the well-conditioning
+// constants of K17CompressibleNavierStokes.h:525-557 (the same sweeps close
+// InterpolationCF.h:255-293 and InterpolationFC.h:210-248)
+template <typename T> __device__ __forceinline__ void backward_sweeps(T (&m)[27], T vx, T vy, T vz)
+{
+    const T vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (b == 1 || c == 1) {
+                bwd(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), vx, vx2);
+            } else {
+                const T kinv = (b == 0 && c == 0) ? T(1) : (b == 2 && c == 2) ? T(9) : T(3);
+                const T k = (b == 0 && c == 0) ? T(1) : (b == 2 && c == 2) ? T(1. / 9.) : T(1. / 3.);
+                bwd_k(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), vx, vx2, kinv, k);
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            if (c == 1) {
+                bwd(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), vy, vy2);
+            } else {
+                const T kinv = (i == 1) ? (c == 0 ? T(3. / 2.) : T(9. / 2.)) : (c == 0 ? T(6) : T(18));
+                const T k = (i == 1) ? (c == 0 ? T(2. / 3.) : T(2. / 9.)) : (c == 0 ? T(1. / 6.) : T(1. / 18.));
+                bwd_k(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), vy, vy2, kinv, k);
+            }
+        }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const T kinv = (i == 1 && j == 1) ? T(9. / 4.) : (i == 1 || j == 1) ? T(9) : T(36);
+            const T k = (i == 1 && j == 1) ? T(4. / 9.) : (i == 1 || j == 1) ? T(1. / 9.) : T(1. / 36.);
+            bwd_k(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), vz, vz2, kinv, k);
+        }
+}
+
 // ------------------------------------------------------------------ K17 ----
 // m[27]: populations in cube order on entry, post-collision populations on exit.
 // Returns drho and the (force-shifted) velocity in mv[4].
@@ -138,22 +202,8 @@ template <typename T> __device__ __forceinline__ void k17(T (&m)[27], const Rate
     constexpr T c3o2 = T(3. / 2.);
 
     // macroscopic values, pyramid summation (MacroscopicQuantities.h:55-85,117-127)
-#define F(d) m[slot_of(d)]
-    const T corners = ((F(dPPP) + F(dMMM)) + (F(dMPM) + F(dPMP))) + ((F(dMPP) + F(dPMM)) + (F(dMMP) + F(dPPM)));
-    const T edges = (((F(d0MP) + F(d0PM)) + (F(d0MM) + F(d0PP))) + ((F(dM0P) + F(dP0M)) + (F(dM0M) + F(dP0P)))) +
-                    ((F(dMP0) + F(dPM0)) + (F(dMM0) + F(dPP0)));
-    const T faces = ((F(dM00) + F(dP00)) + (F(d0M0) + F(d0P0))) + (F(d00M) + F(d00P));
-    const T drho = ((corners + edges) + F(d000)) + faces;
-    T vx = (((F(dPPP) - F(dMMM)) + (F(dPMP) - F(dMPM))) + ((F(dPMM) - F(dMPP)) + (F(dPPM) - F(dMMP)))) +
-           (((F(dP0M) - F(dM0P)) + (F(dP0P) - F(dM0M))) + ((F(dPM0) - F(dMP0)) + (F(dPP0) - F(dMM0)))) +
-           (F(dP00) - F(dM00));
-    T vy = (((F(dPPP) - F(dMMM)) + (F(dMPM) - F(dPMP))) + ((F(dMPP) - F(dPMM)) + (F(dPPM) - F(dMMP)))) +
-           (((F(d0PM) - F(d0MP)) + (F(d0PP) - F(d0MM))) + ((F(dMP0) - F(dPM0)) + (F(dPP0) - F(dMM0)))) +
-           (F(d0P0) - F(d0M0));
-    T vz = (((F(dPPP) - F(dMMM)) + (F(dPMP) - F(dMPM))) + ((F(dMPP) - F(dPMM)) + (F(dMMP) - F(dPPM)))) +
-           (((F(d0MP) - F(d0PM)) + (F(d0PP) - F(d0MM))) + ((F(dM0P) - F(dP0M)) + (F(dP0P) - F(dM0M)))) +
-           (F(d00P) - F(d00M));
-#undef F
+    T drho, vx, vy, vz;
+    pyramid_sums(m, drho, vx, vy, vz);
     const T oneOverRho = T(1) / (drho + T(1));
     vx *= oneOverRho;
     vy *= oneOverRho;
@@ -328,39 +378,7 @@ template <typename T> __device__ __forceinline__ void k17(T (&m)[27], const Rate
     m010 = -m010;
     m001 = -m001;
 
-    // backward sweeps x, y, z (:525-557)
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (b == 1 || c == 1) {
-                bwd(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), vx, vx2);
-            } else {
-                const T kinv = (b == 0 && c == 0) ? T(1) : (b == 2 && c == 2) ? T(9) : T(3);
-                const T k = (b == 0 && c == 0) ? T(1) : (b == 2 && c == 2) ? T(1. / 9.) : T(1. / 3.);
-                bwd_k(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), vx, vx2, kinv, k);
-            }
-        }
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            if (c == 1) {
-                bwd(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), vy, vy2);
-            } else {
-                const T kinv = (i == 1) ? (c == 0 ? T(3. / 2.) : T(9. / 2.)) : (c == 0 ? T(6) : T(18));
-                const T k = (i == 1) ? (c == 0 ? T(2. / 3.) : T(2. / 9.)) : (c == 0 ? T(1. / 6.) : T(1. / 18.));
-                bwd_k(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), vy, vy2, kinv, k);
-            }
-        }
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            const T kinv = (i == 1 && j == 1) ? T(9. / 4.) : (i == 1 || j == 1) ? T(9) : T(36);
-            const T k = (i == 1 && j == 1) ? T(4. / 9.) : (i == 1 || j == 1) ? T(1. / 9.) : T(1. / 36.);
-            bwd_k(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), vz, vz2, kinv, k);
-        }
+    backward_sweeps(m, vx, vy, vz);
 
     mv[0] = drho;
     mv[1] = vx;

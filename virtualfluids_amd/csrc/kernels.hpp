@@ -733,6 +733,109 @@ __global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
     }
 }
 
+// ------------------------------------------------------------- batched ----
+// Many independent box domains (an ensemble of cavities) advanced by one launch.
+// Every domain is a box of nx*ny*nz nodes whose outer layer is wall; only the
+// (nx-2)(ny-2)(nz-2) interior nodes collide.  Wall links are closed by the
+// moving-wall halfway bounce-back of VelocityBounceBack_Device
+// (VelocityBounceBack.cu:159-184) fused into the scatter: a population leaving
+// towards a wall node is written, minus the wall term, straight into the element
+// the node reads the opposite direction from in the next step.  Links into the
+// top plane (z = nz-1) move with the domain's lid velocity.
+template <typename T> struct DomainDev {
+    uint64_t f_off;             // element offset of the domain's 27 planes in the pool
+    uint32_t N, nx, ny, nz;     // box nodes / dims (wall layer included)
+    uint32_t nfx, nfy, nfluid;  // interior dims and count
+    uint32_t pad;
+    T lid[3];
+    Rates<T> R;
+};
+
+struct BatchDev {
+    void* pool;
+    const void* domains;        // DomainDev<T>[n]
+    const uint2* tiles;         // (domain, first interior node) per tile
+    uint32_t ntiles;
+    float ld[4], st[4];
+};
+
+template <typename S, bool EVEN, bool FAST>
+__global__ void __launch_bounds__(256) k_step_batched(BatchDev B)
+{
+    using T = typename Compute<S>::T;
+    const DomainDev<T>* doms = (const DomainDev<T>*)B.domains;
+    for (uint32_t t = blockIdx.x; t < B.ntiles; t += gridDim.x) {
+        const uint2 tile = B.tiles[t];
+        const DomainDev<T>& D = doms[tile.x];
+        const uint32_t idx = tile.y + threadIdx.x;
+        if (idx >= D.nfluid) continue;
+        const uint32_t i = idx % D.nfx, r = idx / D.nfx;
+        const uint32_t x = i + 1, y = r % D.nfy + 1, z = r / D.nfy + 1;
+        Lat L;
+        L.f = (S*)B.pool + D.f_off;
+        L.N = D.N;
+        L.nx = D.nx;
+        L.ny = D.ny;
+        L.nz = D.nz;
+        uint32_t kk[8];
+        nbrs_direct_xyz(L, x, y, z, kk); // interior nodes: no index wraps
+        T m[27], mv[4];
+#pragma unroll
+        for (int s = 0; s < 27; ++s) {
+            const int d = dir_at(s);
+            m[s] = Storage<S, T>::ld(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], B.ld[weight_class(d)]);
+        }
+        collide<T, VFX_K17, FAST>(m, D.R, mv);
+        const bool near = x == 1 || y == 1 || z == 1 || x == D.nx - 2 || y == D.ny - 2 || z == D.nz - 2;
+        if (!near) {
+#pragma unroll
+            for (int s = 0; s < 27; ++s) {
+                const int d = dir_at(s);
+                const int e = inv(d);
+                Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], m[slot_of(e)], B.st[weight_class(e)]);
+            }
+            continue;
+        }
+#pragma unroll
+        for (int s = 0; s < 27; ++s) {
+            const int e = dir_at(s); // direction of the post-collision population
+            const uint32_t tx = x + cx(e), ty = y + cy(e), tz = z + cz(e);
+            const bool wall = tx == 0 || ty == 0 || tz == 0 || tx == D.nx - 1 || ty == D.ny - 1 || tz == D.nz - 1;
+            const int d = inv(e);
+            if (!wall) {
+                Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], m[s], B.st[weight_class(e)]);
+            } else {
+                T u0 = 0, u1 = 0, u2 = 0;
+                if (tz == D.nz - 1) {
+                    u0 = D.lid[0];
+                    u1 = D.lid[1];
+                    u2 = D.lid[2];
+                }
+                const int n = weight_class(e);
+                const T coef = n == 1 ? T(4. / 9.) : n == 2 ? T(1. / 9.) : T(1. / 36.);
+                const T v = m[s] - coef * proj<T>(e, u0, u1, u2);
+                // the element node k reads direction d from in the next step
+                Storage<S, T>::st(plane<S>(L, slot(d, !EVEN)), kk[nbr(d)], v, B.st[weight_class(d)]);
+            }
+        }
+    }
+}
+
+// Initialise every domain's interior to the rest state f - w = 0 (both parities
+// read the same zeros, walls included).
+template <typename S>
+__global__ void __launch_bounds__(256) k_batch_zero(void* pool, uint64_t n)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) ((S*)pool)[i] = S(0);
+}
+
+} // namespace vfx
+
+#include "refinement.hpp"
+
+namespace vfx {
+
 // ------------------------------------------------------------ launchers ----
 constexpr int kBlock = 256;
 
@@ -807,6 +910,29 @@ template <typename S> hipError_t launch_macro(const Lat& L, int direct, bool eve
     return hipGetLastError();
 }
 
+template <typename S>
+hipError_t launch_batched(const BatchDev& B, bool even, int variant, unsigned block, unsigned grid, hipStream_t st)
+{
+    const bool fast = sizeof(typename Compute<S>::T) == 4 && variant == 0;
+    const dim3 g(grid), b(block);
+    if (fast) {
+        if (even) hipLaunchKernelGGL((k_step_batched<S, true, true>), g, b, 0, st, B);
+        else hipLaunchKernelGGL((k_step_batched<S, false, true>), g, b, 0, st, B);
+    } else {
+        if (even) hipLaunchKernelGGL((k_step_batched<S, true, false>), g, b, 0, st, B);
+        else hipLaunchKernelGGL((k_step_batched<S, false, false>), g, b, 0, st, B);
+    }
+    return hipGetLastError();
+}
+
+template <typename S> hipError_t launch_batch_zero(void* pool, uint64_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const uint64_t g = (n + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL((k_batch_zero<S>), dim3((unsigned)(g < 65536 ? g : 65536)), dim3(kBlock), 0, st, pool, n);
+    return hipGetLastError();
+}
+
 inline uint32_t reduce_blocks(uint64_t n) { return (uint32_t)((n + kRedChunk - 1) / kRedChunk); }
 
 template <typename S> hipError_t launch_reduce_mass(const Lat& L, int direct, bool even, double* ws, double* out, hipStream_t st)
@@ -869,7 +995,11 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t);       \
     template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);                            \
     template hipError_t launch_reduce_mass<S>(const Lat&, int, bool, double*, double*, hipStream_t);              \
-    template hipError_t launch_stats<S>(const Lat&, int, bool, double*, hipStream_t);
+    template hipError_t launch_stats<S>(const Lat&, int, bool, double*, hipStream_t);                             \
+    template hipError_t launch_batched<S>(const BatchDev&, bool, int, unsigned, unsigned, hipStream_t);           \
+    template hipError_t launch_batch_zero<S>(void*, uint64_t, hipStream_t);                                        \
+    template hipError_t launch_scale<S>(const Lat&, const Lat&, const ICellsDev&, bool, bool, bool, double, double,     \
+                                        hipStream_t);
 
 #define VFX_EXTERN_LAUNCHERS(S)                                                                                        \
     extern template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*,   \
@@ -880,6 +1010,10 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     extern template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t); \
     extern template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);                     \
     extern template hipError_t launch_reduce_mass<S>(const Lat&, int, bool, double*, double*, hipStream_t);       \
-    extern template hipError_t launch_stats<S>(const Lat&, int, bool, double*, hipStream_t);
+    extern template hipError_t launch_stats<S>(const Lat&, int, bool, double*, hipStream_t);                      \
+    extern template hipError_t launch_batched<S>(const BatchDev&, bool, int, unsigned, unsigned, hipStream_t);    \
+    extern template hipError_t launch_batch_zero<S>(void*, uint64_t, hipStream_t);                                 \
+    extern template hipError_t launch_scale<S>(const Lat&, const Lat&, const ICellsDev&, bool, bool, bool, double,     \
+                                               double, hipStream_t);
 
 } // namespace vfx
