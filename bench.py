@@ -22,6 +22,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -43,6 +45,10 @@ def parse():
     p.add_argument("--ensemble", type=int, default=512,
                    help="cavities in the batched-ensemble leg (0 skips it)")
     p.add_argument("--ensemble-steps", type=int, default=100, help="steps per captured graph of the ensemble leg")
+    p.add_argument("--tgv", type=int, default=256, help="Taylor-Green vortex leg edge (0 skips it)")
+    p.add_argument("--tgv-steps", type=int, default=2000)
+    p.add_argument("--channel-steps", type=int, default=5000,
+                   help="steps of the flow-past-spheres channel leg (0 skips it)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain device copy of known size (reference for HBM counters and bandwidth)")
     return p.parse_args()
@@ -193,6 +199,120 @@ def ensemble_leg(torch, vfx, n_domains, graph_steps, reps=2):
     return out
 
 
+def _timed_loop(torch, stream, chunks, body):
+    """Run body(i) for every chunk, timing the launches with HIP events on the
+    stream they go to; returns milliseconds."""
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for i in range(chunks):
+        body(i)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1)
+
+
+def tgv_leg(torch, vfx, n, steps, every=100):
+    """Taylor-Green vortex n^3 (u0 = 0.04, Re = 1600), K17 and BGK each in FP64
+    and FP32, `steps` steps: kinetic energy and enstrophy every `every` steps
+    (FP64 GPU reductions), decay against the FP64 run and against the analytic
+    early-time decay E0 exp(-6 nu k^2 t), k = 2 pi / n."""
+    from virtualfluids_amd import geometry
+
+    fields, omega, nu = geometry.taylor_green(n)
+    k = 2 * math.pi / n
+    stream = torch.cuda.current_stream()
+    out = {"grid": f"{n}^3", "omega": round(omega, 6), "steps": steps, "sample_every": every}
+    curves = {}
+    for coll in ("k17", "bgk"):
+        for storage in ("fp64", "fp32"):
+            lat = vfx.Lattice((n, n, n), storage=storage, omega=omega)
+            lat.init_equilibrium(*fields)
+            st = vfx.Stepper(lat, coll)
+            ke, en = [], []
+
+            def sample():
+                a = lat.kinetic_energy(st.even)
+                b = lat.enstrophy(st.even)
+                ke.append(a)
+                en.append(b)
+
+            sample()
+            ms = 0.0
+            for _ in range(steps // every):
+                ms += _timed_loop(torch, stream, 1, lambda i: st.step(every))
+                sample()
+            kev = torch.stack(ke).cpu().numpy()
+            env = torch.stack(en).cpu().numpy()
+            curves[(coll, storage)] = (kev[:, 0] / kev[:, 1], env[:, 0] / env[:, 1])
+            out[f"{coll}_{storage}_mlups"] = round(n ** 3 * (steps // every) * every / (ms * 1e-3) / 1e6, 1)
+            del lat, st
+    t = np.arange(steps // every + 1) * every
+    for coll in ("k17", "bgk"):
+        e64, z64 = curves[(coll, "fp64")]
+        e32, z32 = curves[(coll, "fp32")]
+        out[f"{coll}_fp32_ke_rel_err_vs_fp64"] = float(np.abs(e32 / e64 - 1).max())
+        out[f"{coll}_fp32_enstrophy_rel_err_vs_fp64"] = float(np.abs(z32 / z64 - 1).max())
+        early = t <= max(every * 2, steps // 10)
+        rate = -np.polyfit(t[early], np.log(e64[early]), 1)[0]
+        out[f"{coll}_fp64_early_decay_rate"] = float(rate)
+        out[f"{coll}_fp64_early_decay_rel_err_vs_analytic"] = float(rate / (6 * nu * k * k) - 1)
+    out["analytic_early_decay_rate"] = 6 * nu * k * k
+    return out
+
+
+def channel_leg(torch, vfx, steps, sample_every=10, forms=("fp64", "fp32", "fp16")):
+    """Flow past 64 spheres in a 512 x 256 x 256 channel (velocity inlet 0.05
+    with 1% noise, pressure outlet, no-slip walls, interpolated bounce-back with
+    ray-sphere q), K17 at Re = 200 on the mean sphere diameter, started from a
+    uniform stream; `steps` steps with FP64 running mean/Reynolds-stress sums
+    every `sample_every` steps.  Reports whole-step MLUPS (fluid nodes, all
+    boundary kernels and statistics included), the time-averaged mean velocity
+    error of FP32/FP16 storage vs FP64, and the FP64 mass/momentum monitor."""
+    from virtualfluids_amd import geometry
+
+    ch = geometry.spheres_in_channel()
+    bcs = geometry.boundary_conditions(ch.bcs)
+    N = int(np.prod(ch.dims))
+    fl = torch.from_numpy(~ch.solid).cuda()
+    stream = torch.cuda.current_stream()
+    zero = torch.zeros(N, dtype=torch.float64, device="cuda")
+    u0 = torch.full((N,), ch.u_in, dtype=torch.float64, device="cuda")
+    out = {"dims": list(ch.dims), "fluid_nodes": ch.n_fluid, "spheres": len(ch.spheres), "omega": round(ch.omega, 6),
+           "steps": steps, "sample_every": sample_every,
+           "bc_nodes": {str(b.kind): int(b.nodes.size) for b in ch.bcs}}
+    ref_mean = None
+    for storage in forms:
+        lat = vfx.Lattice(ch.dims, storage=storage, omega=ch.omega, solid=ch.solid, device="cuda")
+        lat.init_equilibrium(zero, u0, zero, zero)
+        st = vfx.Stepper(lat, "k17", bcs)
+        stats = vfx.Statistics(lat)
+        mm0 = lat.mass_momentum(st.even).cpu().numpy()
+
+        def chunk(i):
+            st.step(sample_every)
+            stats.accumulate(st.even)
+
+        ms = _timed_loop(torch, stream, steps // sample_every, chunk)
+        mm1 = lat.mass_momentum(st.even).cpu().numpy()
+        mean = stats.mean()[:, fl]
+        r = {"mlups": round(ch.n_fluid * (steps // sample_every) * sample_every / (ms * 1e-3) / 1e6, 1),
+             "ms_per_step": round(ms / ((steps // sample_every) * sample_every), 4),
+             "mass_change_per_fluid_node": float((mm1[0] - mm0[0]) / ch.n_fluid),
+             "mean_ux": float(mean[0].mean())}
+        if ref_mean is None:
+            ref_mean = mean
+        else:
+            r["mean_velocity_l2_vs_fp64"] = float(torch.linalg.vector_norm(mean - ref_mean) /
+                                                  torch.linalg.vector_norm(ref_mean))
+        out[storage] = r
+        print(f"# channel {storage}: {r['mlups']:.0f} MLUPS (whole step)", file=sys.stderr, flush=True)
+        del lat, st, stats, mean
+        torch.cuda.empty_cache()
+    return out
+
+
 def traffic_from_profiles(form):
     """HBM bytes per launch of the headline kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
@@ -274,6 +394,11 @@ def main():
         print(f"# ensemble of {e['domains']} cavities: batched {e['batched_mlups']:.0f} MLUPS, per-domain graph "
               f"{e['per_domain_graph_mlups']:.0f}, per-domain eager {e['per_domain_eager_mlups']:.0f}",
               file=sys.stderr, flush=True)
+    if args.tgv > 0:
+        out["taylor_green"] = tgv_leg(torch, vfx, args.tgv, args.tgv_steps)
+        torch.cuda.empty_cache()
+    if args.channel_steps > 0:
+        out["channel"] = channel_leg(torch, vfx, args.channel_steps)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
 

@@ -258,6 +258,16 @@ vfx_status vfx_batch_capture(vfx_batch batch, int32_t n_steps, int32_t is_even_s
 vfx_status vfx_graph_launch(vfx_graph graph, vfx_stream stream);
 vfx_status vfx_graph_destroy(vfx_graph graph);
 
+/* General capture: every libvfx call issued on `stream` between begin and end
+ * is recorded (not executed) into a graph, e.g. one refined coarse step (two
+ * fine steps, the coarse step, fine-to-coarse, coarse-to-fine).  The stream
+ * must not be the legacy NULL stream; vfx_stream_create gives a non-blocking
+ * stream of default priority.  No libvfx entry point records a memset node. */
+vfx_status vfx_stream_create(vfx_stream* out);
+vfx_status vfx_stream_destroy(vfx_stream stream);
+vfx_status vfx_capture_begin(vfx_stream stream);
+vfx_status vfx_capture_end(vfx_stream stream, vfx_graph* out);
+
 /* -------------------------------------------------------------- misc */
 /* Device memory and synchronisation for C / C++ callers without their own HIP
  * runtime code (the host layer of virtualfluids_amd uses these). */

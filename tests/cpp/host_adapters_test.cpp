@@ -55,6 +55,26 @@ static void cpu_tests()
         threw = true;
     }
     EXPECT(threw);
+    // more than one level needs a scaling factory (GridScalingKernelManager.cpp:45-50)
+    auto p2 = std::make_shared<Parameter>(1);
+    threw = false;
+    try {
+        UpdateGrid27 u(p2, factory.makeKernels(p2), BoundaryConditionFactory());
+    } catch (const std::runtime_error& e) {
+        threw = std::strcmp(e.what(), "There is more than one level, but no scalingFactory was provided.") == 0;
+    }
+    EXPECT(threw);
+    GridScalingFactory gsf;
+    EXPECT(!gsf.getGridScalingFC(false) && !gsf.getGridScalingCF(false));
+    gsf.setScalingFactory(GridScalingFactory::GridScaling::ScaleCompressible);
+    EXPECT((bool)gsf.getGridScalingFC(false) && (bool)gsf.getGridScalingCF(false));
+    threw = false;
+    try {
+        gsf.getGridScalingCF(true);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    EXPECT(threw);
     // the C ABI rejects bad arguments without touching a device
     vfx_lattice lat{};
     EXPECT(vfx_collide_stream(&lat, VFX_K17, 1, 0, nullptr) == VFX_EINVAL);
@@ -146,11 +166,107 @@ static void gpu_tests()
     for (void* d : {dnbx, dnby, dnbz, dfl, fa, fb, fields[0], fields[1], fields[2], fields[3]}) vfx_device_free(d);
 }
 
+// Two levels through UpdateGrid27 (fine twice, coarse, fine-to-coarse,
+// coarse-to-fine) equal the same launches made through the C ABI directly.
+static void gpu_refinement_tests()
+{
+    const uint32_t n = 8, N = n * n * n;
+    std::vector<double> drho(N), ux(N), uy(N), uz(N);
+    for (uint32_t k = 0; k < N; ++k) {
+        drho[k] = 1e-3 * std::cos(0.2 * k);
+        ux[k] = 0.02 * std::sin(0.13 * k);
+        uy[k] = 0.015 * std::cos(0.05 * k);
+        uz[k] = 0.01 * std::sin(0.09 * k + 0.5);
+    }
+    auto up = [&](const void* src, size_t bytes) {
+        void* d = nullptr;
+        check(vfx_device_alloc(bytes, &d), "alloc");
+        check(vfx_copy_to_device(d, src, bytes), "h2d");
+        return d;
+    };
+    void* fields[4] = {up(drho.data(), N * 8), up(ux.data(), N * 8), up(uy.data(), N * 8), up(uz.data(), N * 8)};
+    // disjoint cells: coarse-to-fine writes the 2x2x2 fine blocks at even
+    // coordinates, fine-to-coarse writes distinct coarse nodes
+    std::vector<uint32_t> cf_c, cf_f, fc_c, fc_f;
+    for (uint32_t i = 0; i < 16; ++i) {
+        const uint32_t bx = 2 * (i % 4), by = 2 * ((i / 4) % 4), bz = 2 * (i % 3);
+        cf_f.push_back(bx + n * (by + n * bz));
+        cf_c.push_back((7 * i + 3) % N);
+        fc_c.push_back((11 * i + 5) % N);
+        fc_f.push_back((13 * i + 1) % N);
+    }
+    void* d_cf_c = up(cf_c.data(), 64);
+    void* d_cf_f = up(cf_f.data(), 64);
+    void* d_fc_c = up(fc_c.data(), 64);
+    void* d_fc_f = up(fc_f.data(), 64);
+    vfx_lattice lat{};
+    lat.storage = VFX_FP64;
+    lat.n_nodes = N;
+    lat.direct = 1;
+    lat.dims[0] = lat.dims[1] = lat.dims[2] = n;
+    lat.quadric_limiter[0] = lat.quadric_limiter[1] = lat.quadric_limiter[2] = 0.01;
+    void* f[4];
+    for (auto& a : f) check(vfx_device_alloc((uint64_t)N * 27 * 8, &a), "alloc");
+    auto level = [&](void* fp, double omega) {
+        vfx_lattice l = lat;
+        l.f = fp;
+        l.omega = omega;
+        check(vfx_init_equilibrium(&l, 1, (double*)fields[0], (double*)fields[1], (double*)fields[2],
+                                   (double*)fields[3], nullptr),
+              "init");
+        return l;
+    };
+    const double oc = 1.9, of = 1.0 / (6.0 * (1.0 / oc - 0.5) / 3.0 + 0.5);
+
+    // (a) host layer
+    auto para = std::make_shared<Parameter>(1);
+    LBMSimulationParameter* c = para->getParD(0);
+    LBMSimulationParameter* fi = para->getParD(1);
+    c->lattice = level(f[0], oc);
+    fi->lattice = level(f[1], of);
+    c->coarseToFine = {(uint32_t*)d_cf_f, (uint32_t*)d_cf_c, 16};
+    c->fineToCoarse = {(uint32_t*)d_fc_f, (uint32_t*)d_fc_c, 16};
+    GridScalingFactory gsf;
+    gsf.setScalingFactory(GridScalingFactory::GridScaling::ScaleCompressible);
+    KernelFactoryImp factory;
+    UpdateGrid27 ug(para, factory.makeKernels(para), BoundaryConditionFactory(), &gsf);
+    for (unsigned t = 0; t < 6; ++t) ug.updateGrid(0, t);
+
+    // (b) C ABI
+    vfx_lattice lc = level(f[2], oc), lf = level(f[3], of);
+    vfx_icells cf{(const uint32_t*)d_cf_f, (const uint32_t*)d_cf_c, 16, nullptr, nullptr, nullptr};
+    vfx_icells fc{(const uint32_t*)d_fc_f, (const uint32_t*)d_fc_c, 16, nullptr, nullptr, nullptr};
+    bool ec = true, ef = true;
+    for (int t = 0; t < 6; ++t) {
+        for (int s = 0; s < 2; ++s) {
+            check(vfx_collide_stream(&lf, VFX_K17, ef ? 1 : 0, VFX_TEMPLATE_DEFAULT, nullptr), "fine");
+            ef = !ef;
+        }
+        check(vfx_collide_stream(&lc, VFX_K17, ec ? 1 : 0, VFX_TEMPLATE_DEFAULT, nullptr), "coarse");
+        ec = !ec;
+        check(vfx_scale_fc(&lc, &lf, &fc, ec ? 1 : 0, 1, nullptr), "fc");
+        check(vfx_scale_cf(&lc, &lf, &cf, ec ? 1 : 0, 1, nullptr), "cf");
+    }
+    check(vfx_synchronize(nullptr), "sync");
+    std::vector<double> a((size_t)N * 27), b((size_t)N * 27);
+    for (int lvl = 0; lvl < 2; ++lvl) {
+        check(vfx_copy_to_host(a.data(), f[lvl], a.size() * 8), "d2h");
+        check(vfx_copy_to_host(b.data(), f[lvl + 2], b.size() * 8), "d2h");
+        EXPECT(std::memcmp(a.data(), b.data(), a.size() * 8) == 0);
+    }
+    EXPECT(c->isEvenTimestep == ec && fi->isEvenTimestep == ef);
+    for (void* d : {d_cf_c, d_cf_f, d_fc_c, d_fc_f, fields[0], fields[1], fields[2], fields[3], f[0], f[1], f[2], f[3]})
+        vfx_device_free(d);
+}
+
 int main(int argc, char** argv)
 {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     try {
-        if (gpu) gpu_tests();
+        if (gpu) {
+            gpu_tests();
+            gpu_refinement_tests();
+        }
         else cpu_tests();
     } catch (const std::exception& e) {
         std::printf("exception: %s\n", e.what());

@@ -36,16 +36,21 @@ def _interior(d):
     return s.ravel()
 
 
-def _ensemble(storage, **kw):
-    e = vfx.Ensemble([_box(d) for d in DOMS], OMEGAS, LIDS, storage=storage, **kw)
+# degenerate interiors (a single node along an axis) exercise the index
+# decomposition's division by 1
+DOMS_THIN = [(1, 2, 3), (3, 1, 7), (6, 4, 1)]
+
+
+def _ensemble(storage, doms=DOMS, omegas=OMEGAS, lids=LIDS, **kw):
+    e = vfx.Ensemble([_box(d) for d in doms], omegas, lids, storage=storage, **kw)
     e.init_rest()
     return e
 
 
-def _state(e):
+def _state(e, doms=DOMS):
     """Interior populations of every domain, node-major, as float64 host arrays."""
     out = []
-    for i, d in enumerate(DOMS):
+    for i, d in enumerate(doms):
         f = e.lattice(i).read_populations(e.even).cpu().numpy().astype(np.float64)
         out.append(f[_interior(d)])
     return out
@@ -53,16 +58,19 @@ def _state(e):
 
 def test_batched_fp64_matches_oracle_cavities(cuda):
     steps = 40
-    print(f"[batched fp64 {len(DOMS)} cavities x {steps}]", flush=True)
-    e = _ensemble("fp64")
+    doms = DOMS + DOMS_THIN
+    omegas = OMEGAS + [1.8, 1.85, 1.9]
+    lids = LIDS + [(0.05, 0.0, 0.0), (0.02, 0.03, 0.0), (0.0, 0.04, 0.0)]
+    print(f"[batched fp64 {len(doms)} cavities x {steps}]", flush=True)
+    e = _ensemble("fp64", doms, omegas, lids)
     e.step(steps)
     torch.cuda.synchronize()
-    got = _state(e)
-    for i, d in enumerate(DOMS):
-        solid, groups = cases.cavity(d, lid=LIDS[i])
+    got = _state(e, doms)
+    for i, d in enumerate(doms):
+        solid, groups = cases.cavity(d, lid=lids[i])
         obcs = [oracle.BCGroup(nodes, q, u[0], u[1], u[2], cpu_float_velocity=False) for nodes, q, u in groups]
         m = int(np.prod(_box(d)))
-        ref = oracle.run(np.zeros((m, 27)), _box(d), steps, OMEGAS[i], solid=solid, bcs=obcs)[~solid]
+        ref = oracle.run(np.zeros((m, 27)), _box(d), steps, omegas[i], solid=solid, bcs=obcs)[~solid]
         err = float(np.abs(got[i] - ref).max() / np.abs(ref).max())
         print(f"domain {i} {d}: max rel err {err:.3e} exact={np.array_equal(got[i], ref)}")
         assert err <= REL

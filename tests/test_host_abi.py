@@ -50,6 +50,28 @@ def test_invalid_arguments_fail_without_device():
     assert L.vfx_version() >= 1
 
 
+def test_refinement_arguments_fail_without_device():
+    L = vfx.lib()
+    V = vfx._lib
+    a = V.VfxLattice()
+    b = V.VfxLattice()
+    cells = V.VfxICells()
+    assert L.vfx_scale_cf(C.byref(a), C.byref(b), C.byref(cells), 1, 1, None) == V.VFX_EINVAL  # f NULL
+    for s in (a, b):
+        s.f = 1
+        s.n_nodes = 8
+        s.omega = 1.9
+        s.direct = 1
+        s.dims[:] = [2, 2, 2]
+    b.storage = 1
+    assert L.vfx_scale_fc(C.byref(a), C.byref(b), C.byref(cells), 1, 1, None) == V.VFX_EINVAL
+    assert b"storage differ" in L.vfx_last_error()
+    b.storage = 0
+    assert L.vfx_scale_cf(C.byref(a), C.byref(b), None, 1, 1, None) == V.VFX_EINVAL
+    cells.n_cells = 3
+    assert L.vfx_scale_cf(C.byref(a), C.byref(b), C.byref(cells), 1, 1, None) == V.VFX_EINVAL  # NULL index arrays
+
+
 def test_batch_arguments_fail_without_device():
     L = vfx.lib()
     V = vfx._lib
@@ -78,6 +100,8 @@ def test_batch_arguments_fail_without_device():
     assert h.value is None
     assert L.vfx_batch_step(None, 1, None) == V.VFX_EINVAL
     assert L.vfx_graph_launch(None, None) == V.VFX_EINVAL
+    assert L.vfx_capture_begin(None) == V.VFX_EINVAL  # the NULL stream cannot be captured
+    assert L.vfx_stream_destroy(None) == V.VFX_EINVAL
 
 
 def build_host_test():

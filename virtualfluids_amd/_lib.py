@@ -90,6 +90,10 @@ _SIGNATURES = {
     "vfx_batch_lattice": [C.c_void_p, C.c_uint32, C.POINTER(VfxLattice)],
     "vfx_batch_capture": [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_void_p)],
     "vfx_graph_launch": [C.c_void_p, C.c_void_p],
+    "vfx_stream_create": [C.POINTER(C.c_void_p)],
+    "vfx_stream_destroy": [C.c_void_p],
+    "vfx_capture_begin": [C.c_void_p],
+    "vfx_capture_end": [C.c_void_p, C.POINTER(C.c_void_p)],
     "vfx_graph_destroy": [C.c_void_p],
 }
 

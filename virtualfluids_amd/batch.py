@@ -34,10 +34,13 @@ def cavity_ensemble(n_domains, seed=7, lmin=16, lmax=96, u_lid=0.05, re=(50.0, 1
 
 
 class Graph:
-    """An instantiated HIP graph of captured steps."""
+    """An instantiated HIP graph of captured steps.  `parity` records the time
+    step parity the recording started from: a graph of an odd number of steps
+    replays correctly only from that parity."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, parity=None):
         self.handle = handle
+        self.parity = parity
 
     def launch(self, stream=None):
         V.check(V.lib().vfx_graph_launch(self.handle, _stream(stream)), "vfx_graph_launch")
@@ -46,6 +49,25 @@ class Graph:
         if getattr(self, "handle", None):
             V.lib().vfx_graph_destroy(self.handle)
             self.handle = None
+
+
+def capture(record) -> Graph:
+    """Record the libvfx launches that `record(stream)` issues on a private
+    stream into a HIP graph (nothing executes while recording)."""
+    L = V.lib()
+    s = C.c_void_p()
+    V.check(L.vfx_stream_create(C.byref(s)), "vfx_stream_create")
+    g = C.c_void_p()
+    try:
+        V.check(L.vfx_capture_begin(s), "vfx_capture_begin")
+        try:
+            record(s.value)
+        finally:
+            rc = L.vfx_capture_end(s, C.byref(g))
+        V.check(rc, "vfx_capture_end")
+    finally:
+        L.vfx_stream_destroy(s)
+    return Graph(g.value)
 
 
 class Ensemble:
@@ -126,10 +148,12 @@ class Ensemble:
         g = C.c_void_p()
         V.check(V.lib().vfx_batch_capture(self.handle, int(n_steps), int(self.even), int(per_domain), C.byref(g)),
                 "vfx_batch_capture")
-        return Graph(g.value)
+        return Graph(g.value, parity=self.even)
 
     def replay(self, graph: Graph, n_steps, stream=None):
         """Launch a graph captured from the current parity; advances the parity."""
+        if graph.parity is not None and graph.parity != self.even:
+            raise ValueError("graph was recorded from the other time-step parity")
         graph.launch(stream)
         if n_steps % 2:
             self.even = not self.even

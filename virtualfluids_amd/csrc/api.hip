@@ -421,7 +421,26 @@ template <typename T> void fill_domains(const vfx_batch_s& b, const double lim[3
         e.nfx = s.dims[0] - 2;
         e.nfy = s.dims[1] - 2;
         e.nfluid = (s.dims[0] - 2) * (s.dims[1] - 2) * (s.dims[2] - 2);
-        for (int c = 0; c < 3; ++c) e.lid[c] = (T)s.lid_velocity[c];
+        e.div_x = vfx::make_fastdiv(e.nfx);
+        e.div_y = vfx::make_fastdiv(e.nfy);
+        // wall term of VelocityBounceBack.cu:159-184 in the kernel's arithmetic
+        // type, component order of proj() (first non-zero component with its
+        // sign, the others added or subtracted in x, y, z order)
+        const T u[3] = {(T)s.lid_velocity[0], (T)s.lid_velocity[1], (T)s.lid_velocity[2]};
+        for (int dir = 0; dir < 27; ++dir) {
+            const int c[3] = {vfx::cx(dir), vfx::cy(dir), vfx::cz(dir)};
+            const int wc = vfx::weight_class(dir);
+            const T coef = wc == 1 ? T(4. / 9.) : wc == 2 ? T(1. / 9.) : T(1. / 36.);
+            T pr = T(0);
+            bool first = true;
+            for (int a = 0; a < 3; ++a) {
+                if (c[a] == 0) continue;
+                if (first) pr = c[a] > 0 ? u[a] : -u[a];
+                else pr = c[a] > 0 ? pr + u[a] : pr - u[a];
+                first = false;
+            }
+            e.lid_term[dir] = dir == 0 ? T(0) : coef * pr;
+        }
         e.R = vfx::make_rates<T>(s.omega, lim, zero);
         d[i] = e;
     }
@@ -472,7 +491,8 @@ vfx_status vfx_batch_create(void* pool, uint64_t pool_elements, int32_t storage,
         const vfx_domain& d = domains[i];
         if (d.dims[0] < 3 || d.dims[1] < 3 || d.dims[2] < 3) return fail(VFX_EINVAL, "domain %u: dims must be >= 3", i);
         const uint64_t N = (uint64_t)d.dims[0] * d.dims[1] * d.dims[2];
-        if (N * esz >= (1ull << 32)) return fail(VFX_EUNSUPPORTED, "domain %u too large", i);
+        // one buffer descriptor spans a domain's 27 planes
+        if (27 * N * esz >= (1ull << 32)) return fail(VFX_EUNSUPPORTED, "domain %u too large (27 planes >= 4 GiB)", i);
         if (!(d.omega > 0.0 && d.omega < 2.0)) return fail(VFX_EINVAL, "domain %u: omega %g outside (0,2)", i, d.omega);
         if (d.f_offset > pool_elements || 27 * N > pool_elements - d.f_offset)
             return fail(VFX_EINVAL, "domain %u: planes end beyond the pool", i);
@@ -701,6 +721,42 @@ vfx_status vfx_batch_capture(vfx_batch b, int32_t n_steps, int32_t is_even_start
     if (s != VFX_OK || e != hipSuccess) {
         vfx_graph_destroy(g);
         return s != VFX_OK ? s : hip_status(e, "graph capture");
+    }
+    *out = g;
+    return VFX_OK;
+}
+
+vfx_status vfx_stream_create(vfx_stream* out)
+{
+    if (!out) return fail(VFX_EINVAL, "out is NULL");
+    hipStream_t s = nullptr;
+    hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    *out = e == hipSuccess ? (vfx_stream)s : nullptr;
+    return hip_status(e, "stream create");
+}
+
+vfx_status vfx_stream_destroy(vfx_stream stream)
+{
+    if (!stream) return fail(VFX_EINVAL, "stream is NULL");
+    return hip_status(hipStreamDestroy((hipStream_t)stream), "stream destroy");
+}
+
+vfx_status vfx_capture_begin(vfx_stream stream)
+{
+    if (!stream) return fail(VFX_EINVAL, "capture needs a created stream, not the NULL stream");
+    return hip_status(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal), "begin capture");
+}
+
+vfx_status vfx_capture_end(vfx_stream stream, vfx_graph* out)
+{
+    if (!stream || !out) return fail(VFX_EINVAL, "NULL argument");
+    *out = nullptr;
+    auto* g = new vfx_graph_s();
+    hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g->graph);
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        vfx_graph_destroy(g);
+        return hip_status(e, "end capture");
     }
     *out = g;
     return VFX_OK;

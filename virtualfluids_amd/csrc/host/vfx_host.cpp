@@ -110,13 +110,59 @@ static void require(const boundaryCondition& f, const QforBoundaryConditions& q,
         throw std::runtime_error(std::string(name) + " boundary nodes exist but no boundary condition was selected");
 }
 
+// Scaling.cu:105-188: the interface kernels take the coarse level's parity and
+// treat the fine level as even (both levels are back at their starting parity
+// when the interpolation runs)
+static gridScaling make_scaling(bool coarse_to_fine)
+{
+    return [coarse_to_fine](LBMSimulationParameter* c, LBMSimulationParameter* f, ICells* cells, ICellNeigh& nb,
+                            vfx_stream stream) {
+        vfx_icells ic{};
+        ic.fine_cells = cells->fineCellIndices;
+        ic.coarse_cells = cells->coarseCellIndices;
+        ic.n_cells = cells->numberOfCells;
+        ic.offset_x = nb.x;
+        ic.offset_y = nb.y;
+        ic.offset_z = nb.z;
+        const int32_t even_c = c->isEvenTimestep ? 1 : 0;
+        if (coarse_to_fine) check(vfx_scale_cf(&c->lattice, &f->lattice, &ic, even_c, 1, stream), "scaleCoarseToFine");
+        else check(vfx_scale_fc(&c->lattice, &f->lattice, &ic, even_c, 1, stream), "scaleFineToCoarse");
+    };
+}
+
+// GridScalingFactory.cpp: ScaleCompressible selects the compressible scaling
+gridScaling GridScalingFactory::getGridScalingFC(bool hasTurbulentViscosity) const
+{
+    if (type != GridScaling::ScaleCompressible) return nullptr;
+    if (hasTurbulentViscosity) throw std::runtime_error("scaling with turbulent viscosity is not supported");
+    return make_scaling(false);
+}
+gridScaling GridScalingFactory::getGridScalingCF(bool hasTurbulentViscosity) const
+{
+    if (type != GridScaling::ScaleCompressible) return nullptr;
+    if (hasTurbulentViscosity) throw std::runtime_error("scaling with turbulent viscosity is not supported");
+    return make_scaling(true);
+}
+
 UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared_ptr<Kernel>> k,
-                           const BoundaryConditionFactory& f)
+                           const BoundaryConditionFactory& f, const GridScalingFactory* s)
     : para(std::move(p)), kernels(std::move(k)), velocityBC(f.getVelocityBoundaryConditionPost()),
       noSlipBC(f.getNoSlipBoundaryConditionPost()), slipBC(f.getSlipBoundaryConditionPost()),
       pressureBC(f.getPressureBoundaryConditionPre())
 {
     if ((int)kernels.size() != para->getMaxLevel() + 1) throw std::runtime_error("one kernel per level expected");
+    // GridScalingKernelManager.cpp:45-60
+    if (para->getMaxLevel() != 0) {
+        if (!s) throw std::runtime_error("There is more than one level, but no scalingFactory was provided.");
+        scalingFineToCoarse = s->getGridScalingFC(false);
+        scalingCoarseToFine = s->getGridScalingCF(false);
+        for (int l = 0; l < para->getMaxLevel(); ++l) {
+            if (para->getParD(l)->fineToCoarse.numberOfCells && !scalingFineToCoarse)
+                throw std::runtime_error("scalingFineToCoarse: interface cells exist but no scaling function was selected");
+            if (para->getParD(l)->coarseToFine.numberOfCells && !scalingCoarseToFine)
+                throw std::runtime_error("scalingCoarseToFine: interface cells exist but no scaling function was selected");
+        }
+    }
     for (int l = 0; l <= para->getMaxLevel(); ++l) {
         LBMSimulationParameter* d = para->getParD(l);
         require(velocityBC, d->velocityBC, "velocity");
@@ -140,8 +186,23 @@ void UpdateGrid27::preCollisionBC(int level)
     if (p->pressureBC.numberOfBCnodes) pressureBC(p, &p->pressureBC);
 }
 
-void UpdateGrid27::updateGrid(int level, unsigned int)
+void UpdateGrid27::refinement(int level)
 {
+    LBMSimulationParameter* c = para->getParD(level);
+    LBMSimulationParameter* f = para->getParD(level + 1);
+    if (scalingFineToCoarse && c->fineToCoarse.numberOfCells)
+        scalingFineToCoarse(c, f, &c->fineToCoarse, c->neighborFineToCoarse, c->stream);
+    if (scalingCoarseToFine && c->coarseToFine.numberOfCells)
+        scalingCoarseToFine(c, f, &c->coarseToFine, c->neighborCoarseToFine, c->stream);
+}
+
+void UpdateGrid27::updateGrid(int level, unsigned int timestep)
+{
+    const bool finest = level == para->getMaxLevel();
+    if (!finest) {
+        updateGrid(level + 1, timestep);
+        updateGrid(level + 1, timestep);
+    }
     LBMSimulationParameter* p = para->getParD(level);
     auto* k = dynamic_cast<KernelImp*>(kernels.at(level).get());
     if (k && k->getKernelUsesFluidNodeIndices() && !p->lattice.direct) {
@@ -156,6 +217,7 @@ void UpdateGrid27::updateGrid(int level, unsigned int)
     postCollisionBC(level);
     p->isEvenTimestep = !p->isEvenTimestep; // UpdateGrid27.cpp:264-268
     preCollisionBC(level);
+    if (!finest) refinement(level);
 }
 
 } // namespace vfx::host

@@ -41,10 +41,25 @@ struct QforBoundaryConditions {
     int direction = 0;          // pressure boundary: outward direction (dP00 = 1 ...)
 };
 
+// Calculation.h:72-89: interface cells between a level and the next finer one
+struct ICells {
+    uint32_t* fineCellIndices = nullptr;
+    uint32_t* coarseCellIndices = nullptr;
+    uint32_t numberOfCells = 0;
+};
+struct ICellNeigh {
+    double* x = nullptr;
+    double* y = nullptr;
+    double* z = nullptr;
+};
+
 // The per-level data of src/gpu/core/Parameter/Parameter.h:66-255 this path uses.
 struct LBMSimulationParameter {
     vfx_lattice lattice{};      // distributions, addressing, omega, limiters, force
     bool isEvenTimestep = true;
+    // interface to the next finer level (held by the coarser of the two)
+    ICells fineToCoarse, coarseToFine;
+    ICellNeigh neighborFineToCoarse, neighborCoarseToFine;
     // fluid node lists per collision template (Parameter::taggedFluidNodeIndices)
     std::map<CollisionTemplate, std::pair<uint32_t*, uint32_t>> taggedFluidNodeIndices;
     std::vector<CollisionTemplate> allocatedBulkFluidNodeTags{CollisionTemplate::Default};
@@ -164,21 +179,42 @@ private:
     PressureBC pressure = PressureBC::NotSpecified;
 };
 
-// One level of UpdateGrid27::updateGrid without refinement: collision over the
+// GridScalingFactory.h:46-71
+using gridScaling =
+    std::function<void(LBMSimulationParameter* coarse, LBMSimulationParameter* fine, ICells*, ICellNeigh&, vfx_stream)>;
+
+class GridScalingFactory {
+public:
+    enum class GridScaling { ScaleCompressible, NotSpecified };
+    void setScalingFactory(GridScaling t) { type = t; }
+    // the turbulent-viscosity variants of the reference are not part of this stepper
+    gridScaling getGridScalingFC(bool hasTurbulentViscosity) const;
+    gridScaling getGridScalingCF(bool hasTurbulentViscosity) const;
+
+private:
+    GridScaling type = GridScaling::NotSpecified;
+};
+
+// UpdateGrid27::updateGrid (UpdateGrid27.cpp:54-95): for a level below the
+// finest, the next finer level is updated twice first; then collision over the
 // tagged fluid nodes (or the whole box), post-collision boundary conditions,
-// parity swap, pre-collision boundary conditions.
+// parity swap, pre-collision boundary conditions and, below the finest level,
+// fine-to-coarse then coarse-to-fine interpolation (Refinement_noExchange,
+// RefinementStrategy.cpp:169-178).
 class UpdateGrid27 {
 public:
     UpdateGrid27(std::shared_ptr<Parameter> para, std::vector<std::shared_ptr<Kernel>> kernels,
-                 const BoundaryConditionFactory& bcFactory);
+                 const BoundaryConditionFactory& bcFactory, const GridScalingFactory* scalingFactory = nullptr);
     void updateGrid(int level, unsigned int timestep);
 
 private:
     void postCollisionBC(int level);
     void preCollisionBC(int level);
+    void refinement(int level);
     std::shared_ptr<Parameter> para;
     std::vector<std::shared_ptr<Kernel>> kernels;
     boundaryCondition velocityBC, noSlipBC, slipBC, pressureBC;
+    gridScaling scalingFineToCoarse, scalingCoarseToFine;
 };
 
 } // namespace vfx::host

@@ -34,15 +34,50 @@ template <typename S, typename T> struct Storage;
 template <> struct Storage<double, double> {
     static __device__ __forceinline__ double ld(const double* p, uint32_t i, float) { return elem(p, i); }
     static __device__ __forceinline__ void st(double* p, uint32_t i, double v, float) { elem(p, i) = v; }
+    static __device__ __forceinline__ double enc(double v, float) { return v; }
+    static __device__ __forceinline__ double dec(double h, float) { return h; }
 };
 template <> struct Storage<float, float> {
     static __device__ __forceinline__ float ld(const float* p, uint32_t i, float) { return elem(p, i); }
     static __device__ __forceinline__ void st(float* p, uint32_t i, float v, float) { elem(p, i) = v; }
+    static __device__ __forceinline__ float enc(float v, float) { return v; }
+    static __device__ __forceinline__ float dec(float h, float) { return h; }
 };
 template <> struct Storage<_Float16, float> {
     static __device__ __forceinline__ float ld(const _Float16* p, uint32_t i, float s) { return (float)elem(p, i) * s; }
-    static __device__ __forceinline__ void st(_Float16* p, uint32_t i, float v, float s) { elem(p, i) = (_Float16)(v * s); }
+    static __device__ __forceinline__ void st(_Float16* p, uint32_t i, float v, float s) { elem(p, i) = enc(v, s); }
+    static __device__ __forceinline__ _Float16 enc(float v, float s) { return (_Float16)(v * s); }
+    static __device__ __forceinline__ float dec(_Float16 h, float s) { return (float)h * s; }
 };
+// Raw buffer stores of one element type: descriptor = base address and size
+// in bytes (dword 3 = 0x00020000, the gfx9-family raw-buffer format), VGPR
+// byte offset + SGPR byte offset.  Used where a lane chooses between planes:
+// the lane offset stays one 32-bit VGPR and the plane offset an SGPR.
+typedef unsigned int vfx_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(void* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, (int)bytes, 0x00020000);
+}
+template <typename S> struct Buf;
+template <> struct Buf<double> {
+    static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, double v)
+    {
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(vfx_u32x2, v), r, vo, so, 0);
+    }
+};
+template <> struct Buf<float> {
+    static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, float v)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, vo, so, 0);
+    }
+};
+template <> struct Buf<_Float16> {
+    static __device__ __forceinline__ void st(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so, _Float16 v)
+    {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, vo, so, 0);
+    }
+};
+
 template <typename S> struct Compute { using T = float; };
 template <> struct Compute<double> { using T = double; };
 
@@ -742,12 +777,50 @@ __global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
 // towards a wall node is written, minus the wall term, straight into the element
 // the node reads the opposite direction from in the next step.  Links into the
 // top plane (z = nz-1) move with the domain's lid velocity.
+//
+// Both destinations of a population (its streaming slot, or the bounce-back
+// slot) sit at the same element of two different planes, so the wall case is a
+// per-direction select of plane and value, without divergent branches.
+
+// Exact unsigned division by a run-time invariant d (Granlund & Montgomery,
+// "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1):
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+// q = (t + ((n - t) >> 1)) >> (l - 1) with t = mulhi(m, n); d = 1 is passed
+// through.
+struct FastDiv {
+    uint32_t d, m, l;
+};
+inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f{d, 0, 0};
+    if (d <= 1) return f;
+    uint32_t l = 0;
+    while ((1ull << l) < d) ++l;
+    f.l = l;
+    f.m = (uint32_t)((((1ull << l) - d) << 32) / d + 1);
+    return f;
+}
+// a value the caller knows to be uniform across the wavefront, as a scalar
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint64_t uni64(uint64_t v)
+{
+    return (uint64_t)uni((uint32_t)v) | ((uint64_t)uni((uint32_t)(v >> 32)) << 32);
+}
+
+__device__ __forceinline__ uint32_t fastdiv(uint32_t n, const FastDiv& f)
+{
+    if (f.d <= 1) return n;
+    const uint32_t t = __umulhi(f.m, n);
+    return (t + ((n - t) >> 1)) >> (f.l - 1);
+}
+
 template <typename T> struct DomainDev {
     uint64_t f_off;             // element offset of the domain's 27 planes in the pool
     uint32_t N, nx, ny, nz;     // box nodes / dims (wall layer included)
     uint32_t nfx, nfy, nfluid;  // interior dims and count
     uint32_t pad;
-    T lid[3];
+    FastDiv div_x, div_y;       // division by nfx, nfy
+    T lid_term[27];             // 6 w_e (c_e . u_lid) per direction (used where c_e,z > 0)
     Rates<T> R;
 };
 
@@ -759,18 +832,23 @@ struct BatchDev {
     float ld[4], st[4];
 };
 
+// tiles and domains are read-only for the whole launch (restrict), so their
+// workgroup-uniform loads become scalar loads into SGPRs and every plane
+// address stays a scalar base plus a 32-bit lane offset
 template <typename S, bool EVEN, bool FAST>
-__global__ void __launch_bounds__(256) k_step_batched(BatchDev B)
+__global__ void __launch_bounds__(256)
+    k_step_batched(BatchDev B, const uint2* __restrict__ tiles, const DomainDev<typename Compute<S>::T>* __restrict__ doms)
 {
     using T = typename Compute<S>::T;
-    const DomainDev<T>* doms = (const DomainDev<T>*)B.domains;
     for (uint32_t t = blockIdx.x; t < B.ntiles; t += gridDim.x) {
-        const uint2 tile = B.tiles[t];
+        const uint2 tile = tiles[t];
         const DomainDev<T>& D = doms[tile.x];
         const uint32_t idx = tile.y + threadIdx.x;
+        const uint32_t nfx = D.nfx, nfy = D.nfy;
         if (idx >= D.nfluid) continue;
-        const uint32_t i = idx % D.nfx, r = idx / D.nfx;
-        const uint32_t x = i + 1, y = r % D.nfy + 1, z = r / D.nfy + 1;
+        const uint32_t r = fastdiv(idx, D.div_x);
+        const uint32_t zi = fastdiv(r, D.div_y);
+        const uint32_t x = idx - r * nfx + 1, y = r - zi * nfy + 1, z = zi + 1;
         Lat L;
         L.f = (S*)B.pool + D.f_off;
         L.N = D.N;
@@ -786,37 +864,44 @@ __global__ void __launch_bounds__(256) k_step_batched(BatchDev B)
             m[s] = Storage<S, T>::ld(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], B.ld[weight_class(d)]);
         }
         collide<T, VFX_K17, FAST>(m, D.R, mv);
-        const bool near = x == 1 || y == 1 || z == 1 || x == D.nx - 2 || y == D.ny - 2 || z == D.nz - 2;
-        if (!near) {
+        // which faces of the box the node touches (lane masks: scalar logic)
+        const bool xl = x == 1, xh = x == L.nx - 2, yl = y == 1, yh = y == L.ny - 2, zl = z == 1, zh = z == L.nz - 2;
+        // one buffer descriptor over the domain's planes: a lane's byte offset
+        // is one VGPR per neighbour node, the plane's byte offset an SGPR
+        const uint32_t pb = L.N * (uint32_t)sizeof(S);
+        const __amdgpu_buffer_rsrc_t rs = buffer_rsrc(L.f, 27u * pb);
+        uint32_t vo[8];
 #pragma unroll
-            for (int s = 0; s < 27; ++s) {
-                const int d = dir_at(s);
-                const int e = inv(d);
-                Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], m[slot_of(e)], B.st[weight_class(e)]);
-            }
-            continue;
+        for (int b = 0; b < 8; ++b) vo[b] = kk[b] * (uint32_t)sizeof(S);
+        // links into the lid plane carry the moving-wall term (only waves
+        // with a lane next to the lid do the arithmetic; top = 0 leaves the
+        // value unchanged exactly)
+        if (__ballot(zh)) {
+            const T top = zh ? T(1) : T(0);
+#pragma unroll
+            for (int s = 0; s < 27; ++s)
+                if (cz(dir_at(s)) > 0) m[s] = m[s] - top * D.lid_term[dir_at(s)];
         }
+        S sv[27];
+#pragma unroll
+        for (int s = 0; s < 27; ++s) sv[s] = Storage<S, T>::enc(m[s], B.st[weight_class(dir_at(s))]);
+        // streaming lanes: the slot direction d = inv(e) was read from ...
 #pragma unroll
         for (int s = 0; s < 27; ++s) {
-            const int e = dir_at(s); // direction of the post-collision population
-            const uint32_t tx = x + cx(e), ty = y + cy(e), tz = z + cz(e);
-            const bool wall = tx == 0 || ty == 0 || tz == 0 || tx == D.nx - 1 || ty == D.ny - 1 || tz == D.nz - 1;
+            const int e = dir_at(s);
             const int d = inv(e);
-            if (!wall) {
-                Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], m[s], B.st[weight_class(e)]);
-            } else {
-                T u0 = 0, u1 = 0, u2 = 0;
-                if (tz == D.nz - 1) {
-                    u0 = D.lid[0];
-                    u1 = D.lid[1];
-                    u2 = D.lid[2];
-                }
-                const int n = weight_class(e);
-                const T coef = n == 1 ? T(4. / 9.) : n == 2 ? T(1. / 9.) : T(1. / 36.);
-                const T v = m[s] - coef * proj<T>(e, u0, u1, u2);
-                // the element node k reads direction d from in the next step
-                Storage<S, T>::st(plane<S>(L, slot(d, !EVEN)), kk[nbr(d)], v, B.st[weight_class(d)]);
-            }
+            const bool wall = (cx(e) < 0 && xl) || (cx(e) > 0 && xh) || (cy(e) < 0 && yl) || (cy(e) > 0 && yh) ||
+                              (cz(e) < 0 && zl) || (cz(e) > 0 && zh);
+            if (!wall) Buf<S>::st(rs, vo[nbr(d)], slot(d, EVEN) * pb, sv[s]);
+        }
+        // ... bounce-back lanes: the slot node k reads direction d from next step
+#pragma unroll
+        for (int s = 0; s < 27; ++s) {
+            const int e = dir_at(s);
+            const int d = inv(e);
+            const bool wall = (cx(e) < 0 && xl) || (cx(e) > 0 && xh) || (cy(e) < 0 && yl) || (cy(e) > 0 && yh) ||
+                              (cz(e) < 0 && zl) || (cz(e) > 0 && zh);
+            if (wall) Buf<S>::st(rs, vo[nbr(d)], slot(d, !EVEN) * pb, sv[s]);
         }
     }
 }
@@ -913,14 +998,15 @@ template <typename S> hipError_t launch_macro(const Lat& L, int direct, bool eve
 template <typename S>
 hipError_t launch_batched(const BatchDev& B, bool even, int variant, unsigned block, unsigned grid, hipStream_t st)
 {
-    const bool fast = sizeof(typename Compute<S>::T) == 4 && variant == 0;
+    using T = typename Compute<S>::T;
+    const bool fast = sizeof(T) == 4 && variant == 0;
     const dim3 g(grid), b(block);
     if (fast) {
-        if (even) hipLaunchKernelGGL((k_step_batched<S, true, true>), g, b, 0, st, B);
-        else hipLaunchKernelGGL((k_step_batched<S, false, true>), g, b, 0, st, B);
+        if (even) hipLaunchKernelGGL((k_step_batched<S, true, true>), g, b, 0, st, B, B.tiles, (const DomainDev<T>*)B.domains);
+        else hipLaunchKernelGGL((k_step_batched<S, false, true>), g, b, 0, st, B, B.tiles, (const DomainDev<T>*)B.domains);
     } else {
-        if (even) hipLaunchKernelGGL((k_step_batched<S, true, false>), g, b, 0, st, B);
-        else hipLaunchKernelGGL((k_step_batched<S, false, false>), g, b, 0, st, B);
+        if (even) hipLaunchKernelGGL((k_step_batched<S, true, false>), g, b, 0, st, B, B.tiles, (const DomainDev<T>*)B.domains);
+        else hipLaunchKernelGGL((k_step_batched<S, false, false>), g, b, 0, st, B, B.tiles, (const DomainDev<T>*)B.domains);
     }
     return hipGetLastError();
 }

@@ -32,6 +32,7 @@ import numpy as np
 import torch
 
 from . import _lib as V
+from .batch import Graph, capture
 from .lattice import Lattice, _stream
 
 INACTIVE = 2  # value in the coarse node-state array
@@ -179,6 +180,25 @@ class RefinedStepper:
             self.even_c = not self.even_c
             scale_fc(self.coarse, self.fine, self.fc, self.even_c, self.even_f, stream)
             scale_cf(self.coarse, self.fine, self.cf, self.even_c, self.even_f, stream)
+
+    def capture(self, n=1) -> Graph:
+        """n coarse steps from the current parities as one HIP graph."""
+        saved = (self.even_c, self.even_f)
+        try:
+            g = capture(lambda s: self.step(n, stream=s))
+        finally:
+            self.even_c, self.even_f = saved
+        g.parity = saved
+        return g
+
+    def replay(self, graph: Graph, n, stream=None):
+        """Launch a graph of n coarse steps captured at the current parities
+        (an odd n therefore alternates between two graphs)."""
+        if graph.parity is not None and graph.parity != (self.even_c, self.even_f):
+            raise ValueError("graph was recorded from other time-step parities")
+        graph.launch(stream)
+        if n % 2:
+            self.even_c = not self.even_c
 
     @property
     def lups_per_step(self) -> int:
