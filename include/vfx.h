@@ -84,6 +84,13 @@ typedef struct vfx_lattice {
      *    algebra, hardware reciprocal);
      * 1: reference operation order in every precision ("plain" kernel). */
     int32_t kernel_variant;
+
+    /* VFX_FP16_SCALED only: the stored value of direction d is the deviation of
+     * f_d - w_d from the equilibrium at this velocity (drho = 0), scaled by
+     * fp16_scale / w_d.  Choosing the mean flow (e.g. the inflow velocity)
+     * keeps the stored deviations small, so FP16 rounds them less.  {0,0,0}:
+     * plain shifted populations.  Must be zero for FP32/FP64 storage. */
+    double fp16_reference_velocity[3];
 } vfx_lattice;
 
 /* boundary-condition kinds (src/gpu/core/BoundaryConditions/BoundaryConditionFactory.h:60-132) */

@@ -47,6 +47,10 @@ def test_invalid_arguments_fail_without_device():
     assert L.vfx_collide_stream(C.byref(s), 7, 1, 0, None) == vfx._lib.VFX_EINVAL  # unknown collision
     s.omega = 2.0
     assert L.vfx_collide_stream(C.byref(s), 0, 1, 0, None) == vfx._lib.VFX_EINVAL
+    s.omega = 1.5
+    s.fp16_reference_velocity[:] = [0.05, 0.0, 0.0]  # FP16 storage only
+    assert L.vfx_collide_stream(C.byref(s), 0, 1, 0, None) == vfx._lib.VFX_EINVAL
+    assert b"fp16_reference_velocity" in L.vfx_last_error()
     assert L.vfx_version() >= 1
 
 

@@ -7,6 +7,7 @@
 #include "kernels.hpp"
 
 #include <chrono>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -92,6 +93,21 @@ vfx_status make_lat(const vfx_lattice* in, vfx::Lat& L)
         }
     } else {
         for (int c = 0; c < 4; ++c) L.ld[c] = L.st[c] = 1.f;
+    }
+    const double* ur = in->fp16_reference_velocity;
+    const bool shifted = ur[0] != 0.0 || ur[1] != 0.0 || ur[2] != 0.0;
+    if (shifted && in->storage != VFX_FP16_SCALED)
+        return fail(VFX_EINVAL, "fp16_reference_velocity needs VFX_FP16_SCALED storage");
+    if (!(std::fabs(ur[0]) < 0.5 && std::fabs(ur[1]) < 0.5 && std::fabs(ur[2]) < 0.5))
+        return fail(VFX_EINVAL, "fp16_reference_velocity out of range");
+    for (int d = 0; d < 27; ++d) {
+        // shifted compressible equilibrium at drho = 0 (the offset only has to
+        // be a fixed per-direction constant; this one is close to the flow)
+        const double cu = vfx::cx(d) * ur[0] + vfx::cy(d) * ur[1] + vfx::cz(d) * ur[2];
+        const double u2 = ur[0] * ur[0] + ur[1] * ur[1] + ur[2] * ur[2];
+        const double sh = shifted ? kW[vfx::weight_class(d)] * (3.0 * cu + 4.5 * cu * cu - 1.5 * u2) : 0.0;
+        L.sh[d] = (float)sh;
+        L.ss[d] = (float)(-sh * (double)L.st[vfx::weight_class(d)]);
     }
     L.rho = in->rho;
     L.vx = in->vx;

@@ -453,26 +453,27 @@ __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x)
 // forward three-point transform (k = 0: no weight constant); `full` = 2 also
 // forms the second-order output, 1 only the first, 0 only the sum
 template <int OUT>
-__device__ __forceinline__ void fwdf(float& a, float& b, float& c, float vv, float v2, float k)
+__device__ __forceinline__ void fwdf(float& a, float& b, float& c, float vv, float v2, float tv, float k)
 {
     const float s = a + c;
     const float d = c - a;
     const float m0 = s + b;
-    const float mk = m0 + k;
+    const float mk = k == 0.f ? m0 : m0 + k; // k is a compile-time constant after unrolling
     a = m0;
     if (OUT >= 1) b = d - mk * vv;
-    if (OUT >= 2) c = (s - 2.f * d * vv) + mk * v2;
+    if (OUT >= 2) c = (s - tv * d) + mk * v2;
 }
 // backward three-point transform; hvm = (v2 - vv)/2, hvp = (v2 + vv)/2,
-// vmh = vv - 1/2, vph = vv + 1/2, tv = 2 vv
-__device__ __forceinline__ void bwdf(float& a, float& b, float& c, float k, float v2, float hvm, float hvp, float vmh,
-                                     float vph, float tv)
+// vmh = vv - 1/2, vph = vv + 1/2.  The centre output uses r0 + r1 + r2 = a
+// (the zeroth moment): r1 = a - (r0 + r2), two operations instead of four.
+__device__ __forceinline__ void bwdf(float& a, float& b, float& c, float k, float hvm, float hvp, float vmh,
+                                     float vph)
 {
-    const float ak = a + k;
+    const float ak = k == 0.f ? a : a + k; // k is a compile-time constant after unrolling
     const float ch = 0.5f * c;
     const float r0 = ch + b * vmh + ak * hvm;
-    const float r1 = (a - c) - b * tv - ak * v2;
     const float r2 = ch + b * vph + ak * hvp;
+    const float r1 = a - (r0 + r2);
     a = r0;
     b = r1;
     c = r2;
@@ -483,45 +484,54 @@ __device__ __forceinline__ void bwdf(float& a, float& b, float& c, float k, floa
 __device__ __forceinline__ void k17_fast(float (&m)[27], const Rates<float>& R, float (&mv)[4])
 {
     constexpr float c1o3 = 1.f / 3.f, c1o9 = 1.f / 9.f, c1o27 = 1.f / 27.f, c2o3 = 2.f / 3.f;
-    // ---- macroscopic values: row sums along x, then pairwise over the rows
-    float rs[3][3], rd[3][3];
+    // ---- macroscopic values from the sums the z sweep needs anyway: per (x,y)
+    // row s = f(z-) + f(z+), d = f(z+) - f(z-), m0 = s + f(z0); then
+    // pairwise over the nine rows
+    float zs[3][3], zd[3][3], zm[3][3];
 #pragma unroll
-    for (int j = 0; j < 3; ++j)
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            rs[j][k] = (VFX_M(0, j, k) + VFX_M(2, j, k)) + VFX_M(1, j, k);
-            rd[j][k] = VFX_M(2, j, k) - VFX_M(0, j, k);
+        for (int j = 0; j < 3; ++j) {
+            zs[i][j] = VFX_M(i, j, 0) + VFX_M(i, j, 2);
+            zd[i][j] = VFX_M(i, j, 2) - VFX_M(i, j, 0);
+            zm[i][j] = zs[i][j] + VFX_M(i, j, 1);
         }
-    const float drho = (((rs[0][0] + rs[2][2]) + (rs[0][2] + rs[2][0])) + ((rs[0][1] + rs[2][1]) + (rs[1][0] + rs[1][2]))) + rs[1][1];
-    const float jx = (((rd[0][0] + rd[2][2]) + (rd[0][2] + rd[2][0])) + ((rd[0][1] + rd[2][1]) + (rd[1][0] + rd[1][2]))) + rd[1][1];
-    const float jy = ((rs[2][0] - rs[0][0]) + (rs[2][2] - rs[0][2])) + (rs[2][1] - rs[0][1]);
-    const float jz = ((rs[0][2] - rs[0][0]) + (rs[2][2] - rs[2][0])) + (rs[1][2] - rs[1][0]);
+    const float drho = (((zm[0][0] + zm[2][2]) + (zm[0][2] + zm[2][0])) + ((zm[0][1] + zm[2][1]) + (zm[1][0] + zm[1][2]))) + zm[1][1];
+    const float jz = (((zd[0][0] + zd[2][2]) + (zd[0][2] + zd[2][0])) + ((zd[0][1] + zd[2][1]) + (zd[1][0] + zd[1][2]))) + zd[1][1];
+    const float jx = ((zm[2][0] - zm[0][0]) + (zm[2][2] - zm[0][2])) + (zm[2][1] - zm[0][1]);
+    const float jy = ((zm[0][2] - zm[0][0]) + (zm[2][2] - zm[2][0])) + (zm[1][2] - zm[1][0]);
     const float oneOverRho = rcpf(drho + 1.f);
     const float vx = jx * oneOverRho + R.fx;
     const float vy = jy * oneOverRho + R.fy;
     const float vz = jz * oneOverRho + R.fz;
     const float vx2 = vx * vx, vy2 = vy * vy, vz2 = vz * vz;
+    const float tx = 2.f * vx, ty = 2.f * vy, tz = 2.f * vz;
 
     // ---- forward: z, y, x; moments of total order >= 4 are not formed
     constexpr float wv[3] = {1.f / 6.f, 2.f / 3.f, 1.f / 6.f};
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) fwdf<2>(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), vz, vz2, wv[i] * wv[j]);
+        for (int j = 0; j < 3; ++j) {
+            const float mk = zm[i][j] + wv[i] * wv[j];
+            VFX_M(i, j, 0) = zm[i][j];
+            VFX_M(i, j, 1) = zd[i][j] - mk * vz;
+            VFX_M(i, j, 2) = (zs[i][j] - tz * zd[i][j]) + mk * vz2;
+        }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        fwdf<2>(VFX_M(i, 0, 0), VFX_M(i, 1, 0), VFX_M(i, 2, 0), vy, vy2, wv[i]);
-        fwdf<2>(VFX_M(i, 0, 1), VFX_M(i, 1, 1), VFX_M(i, 2, 1), vy, vy2, 0.f);
-        fwdf<1>(VFX_M(i, 0, 2), VFX_M(i, 1, 2), VFX_M(i, 2, 2), vy, vy2, wv[i] * c1o3);
+        fwdf<2>(VFX_M(i, 0, 0), VFX_M(i, 1, 0), VFX_M(i, 2, 0), vy, vy2, ty, wv[i]);
+        fwdf<2>(VFX_M(i, 0, 1), VFX_M(i, 1, 1), VFX_M(i, 2, 1), vy, vy2, ty, 0.f);
+        fwdf<1>(VFX_M(i, 0, 2), VFX_M(i, 1, 2), VFX_M(i, 2, 2), vy, vy2, ty, wv[i] * c1o3);
     }
-    fwdf<2>(VFX_M(0, 0, 0), VFX_M(1, 0, 0), VFX_M(2, 0, 0), vx, vx2, 1.f);
-    fwdf<2>(VFX_M(0, 1, 0), VFX_M(1, 1, 0), VFX_M(2, 1, 0), vx, vx2, 0.f);
-    fwdf<2>(VFX_M(0, 0, 1), VFX_M(1, 0, 1), VFX_M(2, 0, 1), vx, vx2, 0.f);
-    fwdf<1>(VFX_M(0, 2, 0), VFX_M(1, 2, 0), VFX_M(2, 2, 0), vx, vx2, c1o3);
-    fwdf<1>(VFX_M(0, 0, 2), VFX_M(1, 0, 2), VFX_M(2, 0, 2), vx, vx2, c1o3);
-    fwdf<1>(VFX_M(0, 1, 1), VFX_M(1, 1, 1), VFX_M(2, 1, 1), vx, vx2, 0.f);
-    fwdf<0>(VFX_M(0, 1, 2), VFX_M(1, 1, 2), VFX_M(2, 1, 2), vx, vx2, 0.f);
-    fwdf<0>(VFX_M(0, 2, 1), VFX_M(1, 2, 1), VFX_M(2, 2, 1), vx, vx2, 0.f);
+    fwdf<2>(VFX_M(0, 0, 0), VFX_M(1, 0, 0), VFX_M(2, 0, 0), vx, vx2, tx, 1.f);
+    fwdf<2>(VFX_M(0, 1, 0), VFX_M(1, 1, 0), VFX_M(2, 1, 0), vx, vx2, tx, 0.f);
+    fwdf<2>(VFX_M(0, 0, 1), VFX_M(1, 0, 1), VFX_M(2, 0, 1), vx, vx2, tx, 0.f);
+    fwdf<1>(VFX_M(0, 2, 0), VFX_M(1, 2, 0), VFX_M(2, 2, 0), vx, vx2, tx, c1o3);
+    fwdf<1>(VFX_M(0, 0, 2), VFX_M(1, 0, 2), VFX_M(2, 0, 2), vx, vx2, tx, c1o3);
+    fwdf<1>(VFX_M(0, 1, 1), VFX_M(1, 1, 1), VFX_M(2, 1, 1), vx, vx2, tx, 0.f);
+    fwdf<0>(VFX_M(0, 1, 2), VFX_M(1, 1, 2), VFX_M(2, 1, 2), vx, vx2, tx, 0.f);
+    fwdf<0>(VFX_M(0, 2, 1), VFX_M(1, 2, 1), VFX_M(2, 2, 1), vx, vx2, tx, 0.f);
 
     float& m000 = VFX_M(0, 0, 0); float& m100 = VFX_M(1, 0, 0); float& m010 = VFX_M(0, 1, 0); float& m001 = VFX_M(0, 0, 1);
     float& m200 = VFX_M(2, 0, 0); float& m020 = VFX_M(0, 2, 0); float& m002 = VFX_M(0, 0, 2);
@@ -603,32 +613,32 @@ __device__ __forceinline__ void k17_fast(float (&m)[27], const Rates<float>& R, 
 
     // ---- backward: x, y, z
     {
-        const float hvm = 0.5f * (vx2 - vx), hvp = 0.5f * (vx2 + vx), vmh = vx - 0.5f, vph = vx + 0.5f, tv = 2.f * vx;
+        const float hvm = 0.5f * (vx2 - vx), hvp = 0.5f * (vx2 + vx), vmh = vx - 0.5f, vph = vx + 0.5f;
 #pragma unroll
         for (int b = 0; b < 3; ++b)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float k = (b == 1 || c == 1) ? 0.f : (b == 0 && c == 0) ? 1.f : (b == 2 && c == 2) ? c1o9 : c1o3;
-                bwdf(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), k, vx2, hvm, hvp, vmh, vph, tv);
+                bwdf(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), k, hvm, hvp, vmh, vph);
             }
     }
     {
-        const float hvm = 0.5f * (vy2 - vy), hvp = 0.5f * (vy2 + vy), vmh = vy - 0.5f, vph = vy + 0.5f, tv = 2.f * vy;
+        const float hvm = 0.5f * (vy2 - vy), hvp = 0.5f * (vy2 + vy), vmh = vy - 0.5f, vph = vy + 0.5f;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
                 const float k = (c == 1) ? 0.f : wv[i] * (c == 0 ? 1.f : c1o3);
-                bwdf(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), k, vy2, hvm, hvp, vmh, vph, tv);
+                bwdf(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), k, hvm, hvp, vmh, vph);
             }
     }
     {
-        const float hvm = 0.5f * (vz2 - vz), hvp = 0.5f * (vz2 + vz), vmh = vz - 0.5f, vph = vz + 0.5f, tv = 2.f * vz;
+        const float hvm = 0.5f * (vz2 - vz), hvp = 0.5f * (vz2 + vz), vmh = vz - 0.5f, vph = vz + 0.5f;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int j = 0; j < 3; ++j)
-                bwdf(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), wv[i] * wv[j], vz2, hvm, hvp, vmh, vph, tv);
+                bwdf(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), wv[i] * wv[j], hvm, hvp, vmh, vph);
     }
     mv[0] = drho;
     mv[1] = vx;

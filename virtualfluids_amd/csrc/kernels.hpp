@@ -32,20 +32,28 @@ template <typename S> __device__ __forceinline__ S& elem(S* p, uint32_t i)
 }
 template <typename S, typename T> struct Storage;
 template <> struct Storage<double, double> {
-    static __device__ __forceinline__ double ld(const double* p, uint32_t i, float) { return elem(p, i); }
-    static __device__ __forceinline__ void st(double* p, uint32_t i, double v, float) { elem(p, i) = v; }
+    static __device__ __forceinline__ double ld(const double* p, uint32_t i, float, float = 0.f) { return elem(p, i); }
+    static __device__ __forceinline__ void st(double* p, uint32_t i, double v, float, float = 0.f) { elem(p, i) = v; }
     static __device__ __forceinline__ double enc(double v, float) { return v; }
     static __device__ __forceinline__ double dec(double h, float) { return h; }
 };
 template <> struct Storage<float, float> {
-    static __device__ __forceinline__ float ld(const float* p, uint32_t i, float) { return elem(p, i); }
-    static __device__ __forceinline__ void st(float* p, uint32_t i, float v, float) { elem(p, i) = v; }
+    static __device__ __forceinline__ float ld(const float* p, uint32_t i, float, float = 0.f) { return elem(p, i); }
+    static __device__ __forceinline__ void st(float* p, uint32_t i, float v, float, float = 0.f) { elem(p, i) = v; }
     static __device__ __forceinline__ float enc(float v, float) { return v; }
     static __device__ __forceinline__ float dec(float h, float) { return h; }
 };
+// sh: offset of the stored deviation (the reference-velocity equilibrium, in
+// f - w units); st takes ss = -sh * s, so both are one mixed-precision FMA
 template <> struct Storage<_Float16, float> {
-    static __device__ __forceinline__ float ld(const _Float16* p, uint32_t i, float s) { return (float)elem(p, i) * s; }
-    static __device__ __forceinline__ void st(_Float16* p, uint32_t i, float v, float s) { elem(p, i) = enc(v, s); }
+    static __device__ __forceinline__ float ld(const _Float16* p, uint32_t i, float s, float sh = 0.f)
+    {
+        return (float)elem(p, i) * s + sh;
+    }
+    static __device__ __forceinline__ void st(_Float16* p, uint32_t i, float v, float s, float ss = 0.f)
+    {
+        elem(p, i) = (_Float16)(v * s + ss);
+    }
     static __device__ __forceinline__ _Float16 enc(float v, float s) { return (_Float16)(v * s); }
     static __device__ __forceinline__ float dec(_Float16 h, float s) { return (float)h * s; }
 };
@@ -93,6 +101,7 @@ struct Lat {
     const uint32_t* fluid;
     uint32_t nfluid;
     float ld[4], st[4];
+    float sh[27], ss[27]; // FP16 reference-velocity offsets per direction (0 otherwise)
     void* rho;
     void* vx;
     void* vy;
@@ -144,14 +153,37 @@ __device__ __forceinline__ void nbrs_sparse(const Lat& L, uint32_t k, uint32_t (
     kk[7] = L.nbz[kk[3]];
 }
 
+// A wave-uniform float held in a VGPR.  FP16 loads and stores are one
+// mixed-precision FMA with two lattice constants (scale and offset); a VALU
+// instruction reads at most one SGPR, so the four per-class scales live in
+// VGPRs (four moves per thread instead of one per access).
+__device__ __forceinline__ float in_vgpr(float x)
+{
+    float r;
+    asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
+    return r;
+}
+template <typename S> struct Scales {
+    float ld[4], st[4];
+    __device__ __forceinline__ explicit Scales(const Lat& L)
+    {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            ld[c] = sizeof(S) == 2 ? in_vgpr(L.ld[c]) : L.ld[c];
+            st[c] = sizeof(S) == 2 ? in_vgpr(L.st[c]) : L.st[c];
+        }
+    }
+};
+
 // pre-collision populations of a node in cube order
 template <typename S, typename T, bool EVEN>
 __device__ __forceinline__ void gather(const Lat& L, const uint32_t (&kk)[8], T (&m)[27])
 {
+    const Scales<S> sc(L);
 #pragma unroll
     for (int s = 0; s < 27; ++s) {
         const int d = dir_at(s);
-        m[s] = Storage<S, T>::ld(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], L.ld[weight_class(d)]);
+        m[s] = Storage<S, T>::ld(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], sc.ld[weight_class(d)], L.sh[d]);
     }
 }
 // post-collision populations: the element direction d was read from receives
@@ -159,11 +191,12 @@ __device__ __forceinline__ void gather(const Lat& L, const uint32_t (&kk)[8], T 
 template <typename S, typename T, bool EVEN>
 __device__ __forceinline__ void scatter_post(const Lat& L, const uint32_t (&kk)[8], const T (&m)[27])
 {
+    const Scales<S> sc(L);
 #pragma unroll
     for (int s = 0; s < 27; ++s) {
         const int d = dir_at(s);
         const int e = inv(d);
-        Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], m[slot_of(e)], L.st[weight_class(e)]);
+        Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], m[slot_of(e)], sc.st[weight_class(e)], L.ss[e]);
     }
 }
 // pre-collision write (initialisation, population upload)
@@ -173,7 +206,7 @@ __device__ __forceinline__ void scatter_pre(const Lat& L, const uint32_t (&kk)[8
 #pragma unroll
     for (int s = 0; s < 27; ++s) {
         const int d = dir_at(s);
-        Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], m[s], L.st[weight_class(d)]);
+        Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], m[s], L.st[weight_class(d)], L.ss[d]);
     }
 }
 
@@ -619,10 +652,10 @@ __global__ void __launch_bounds__(256) k_bc_velocity_bb(Lat L, BCDev B)
         if (!(q >= 0.f && q <= 1.f)) continue;
         const int e = inv(d);
         // post-collision population of direction d sits where e was read from
-        const T fpost = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)]);
+        const T fpost = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)], L.sh[d]);
         const T v = fpost - wall_term<T>(d, ux, uy, uz, B.cpu_float);
         // next step (parity !EVEN) reads direction e of node k from here
-        Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)]);
+        Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)], L.ss[e]);
     }
 }
 
@@ -647,7 +680,7 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
         const int e = inv(d);
-        p[d] = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)]);
+        p[d] = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)], L.sh[d]);
     }
     const T drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] + p[d0MP] +
                    p[d0PP] + p[d0MM] + p[dP0M] + p[dM0P] + p[dP0P] + p[dM0M] + p[dPM0] + p[dMP0] + p[dPP0] + p[dMM0] +
@@ -698,7 +731,7 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
             v = a + (q * (f + fi) - T(6) * w * vbc) / (T(1) + q);
         }
         const int e = inv(d);
-        Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)]);
+        Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)], L.ss[e]);
     }
 }
 
@@ -764,7 +797,7 @@ __global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
         if (cx(d) * nx + cy(d) * ny + cz(d) * nz > 0) continue;
-        Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], f[d], L.st[weight_class(d)]);
+        Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], f[d], L.st[weight_class(d)], L.ss[d]);
     }
 }
 

@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(256) k_scale_fc(Lat C, Lat F, ICellsDev cells,
     for (int s = 0; s < 27; ++s) {
         if (s % kCellLanes != lane) continue;
         const int d = dir_at(s);
-        Storage<S, T>::st(plane<S>(C, slot(d, EVEN_C)), kk[nbr(d)], m[s], C.st[weight_class(d)]);
+        Storage<S, T>::st(plane<S>(C, slot(d, EVEN_C)), kk[nbr(d)], m[s], C.st[weight_class(d)], C.ss[d]);
     }
 }
 

@@ -58,7 +58,7 @@ class Lattice:
 
     def __init__(self, dims=None, *, storage="fp64", omega=1.0, limiter=(0.01, 0.01, 0.01),
                  force=(0.0, 0.0, 0.0), solid=None, fp16_scale=None, neighbors=None, fluid_nodes=None,
-                 n_nodes=None, device="cuda", variant="default"):
+                 n_nodes=None, device="cuda", variant="default", reference_velocity=(0.0, 0.0, 0.0)):
         if storage not in STORAGE:
             raise ValueError(f"storage must be one of {sorted(STORAGE)}")
         self.storage = storage
@@ -70,6 +70,9 @@ class Lattice:
         self.limiter = tuple(float(x) for x in limiter)
         self.force = tuple(float(x) for x in force)
         self.fp16_scale = float(fp16_scale if fp16_scale is not None else (2.0 ** 10 if storage == "fp16" else 1.0))
+        # FP16 storage only: populations are stored as deviations from the
+        # equilibrium at this velocity (pick the mean flow)
+        self.reference_velocity = tuple(float(x) for x in reference_velocity)
         self.direct = neighbors is None
         self.solid_mask = None
         self.nb = None
@@ -125,6 +128,7 @@ class Lattice:
         if self.macro is not None:
             s.rho, s.vx, s.vy, s.vz = (t.data_ptr() for t in self.macro)
         s.kernel_variant = 0 if self.variant == "default" else 1
+        s.fp16_reference_velocity[:] = list(getattr(self, "reference_velocity", (0.0, 0.0, 0.0)))
         return s
 
     @property
