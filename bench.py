@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--tgv-steps", type=int, default=2000)
     p.add_argument("--channel-steps", type=int, default=5000,
                    help="steps of the flow-past-spheres channel leg (0 skips it)")
+    p.add_argument("--cube-steps", type=int, default=10000,
+                   help="coarse steps of the refined wall-mounted-cube leg (0 skips it)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain device copy of known size (reference for HBM counters and bandwidth)")
     return p.parse_args()
@@ -319,6 +321,74 @@ def channel_leg(torch, vfx, steps, sample_every=10, forms=("fp64", "fp32", "fp16
     return out
 
 
+def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp16")):
+    """Wall-mounted cube in a boundary layer on a 256 x 128 x 128 coarse grid
+    with a 2:1 nest of 192 x 128 x 96 fine nodes around the cube (power-law
+    inflow with 2% noise, pressure outlet, no-slip floor, slip top, periodic
+    spanwise), K17 at Re = 10^4 on the cube height.  Every coarse step (two
+    fine steps, one coarse step, both levels' boundary kernels, fine-to-coarse
+    and coarse-to-fine) is recorded into a HIP graph of `sample_every` coarse
+    steps.  Time-averaged velocity over the second half on both levels; the
+    FP16 form (deviations from the u_ref equilibrium) against FP64.  MLUPS
+    counts coarse fluid nodes once and fine fluid nodes twice per coarse step."""
+    from virtualfluids_amd import geometry
+
+    case = geometry.wall_mounted_cube()
+    nest = case.nest
+    H = float(case.cube[3])
+    stream = torch.cuda.current_stream()
+    cbcs = geometry.boundary_conditions(case.coarse_bcs)
+    fbcs = geometry.boundary_conditions(case.fine_bcs)
+
+    def profile(z):
+        return case.u_ref * np.clip((z - 0.5) / H, 0.0, None) ** 0.2
+
+    nx, ny, nz = nest.coarse_dims
+    zc = np.repeat(np.arange(nz, dtype=np.float64), nx * ny)
+    zf = nest.fine_positions()[2]
+    out = {"coarse_dims": list(nest.coarse_dims), "fine_dims": list(nest.fine_dims), "omega_coarse": case.omega,
+           "coarse_steps": coarse_steps, "fine_steps": 2 * coarse_steps, "graph_coarse_steps": sample_every,
+           "interface_cells": {"coarse_to_fine": int(nest.cf_coarse.size), "fine_to_coarse": int(nest.fc_coarse.size)}}
+    means = {}
+    for storage in forms:
+        uref = (case.u_ref, 0.0, 0.0) if storage == "fp16" else (0.0, 0.0, 0.0)
+        st = vfx.RefinedStepper(nest, case.omega, storage=storage, coarse_solid=case.coarse_solid,
+                                fine_solid=case.fine_solid, coarse_bcs=cbcs, fine_bcs=fbcs, reference_velocity=uref)
+        for lat, z in ((st.coarse, zc), (st.fine, zf)):
+            n = z.size
+            lat.init_equilibrium(np.zeros(n), profile(z), np.zeros(n), np.zeros(n))
+        graph = st.capture(sample_every)
+        sc, sf = vfx.Statistics(st.coarse), vfx.Statistics(st.fine)
+        chunks = coarse_steps // sample_every
+
+        def chunk(i):
+            st.replay(graph, sample_every)
+            if i >= chunks // 2:
+                sc.accumulate(st.even_c)
+                sf.accumulate(st.even_f)
+
+        ms = _timed_loop(torch, stream, chunks, chunk)
+        lups = st.lups_per_step * chunks * sample_every
+        act = torch.from_numpy((nest.coarse_state == 0) & ~case.coarse_solid).cuda()
+        ffl = torch.from_numpy(~case.fine_solid).cuda()
+        means[storage] = (sc.mean()[:, act], sf.mean()[:, ffl])
+        r = {"mlups": round(lups / (ms * 1e-3) / 1e6, 1), "ms_per_coarse_step": round(ms / (chunks * sample_every), 4),
+             "finite": bool(torch.isfinite(means[storage][1]).all())}
+        out[storage] = r
+        print(f"# cube {storage}: {r['mlups']:.0f} MLUPS (refined, fine counted twice)", file=sys.stderr, flush=True)
+        del st, sc, sf, graph
+        torch.cuda.empty_cache()
+    if "fp64" in means:
+        for s in forms:
+            if s == "fp64":
+                continue
+            for lvl, name in ((0, "coarse"), (1, "fine")):
+                a, b = means[s][lvl], means["fp64"][lvl]
+                out[s][f"mean_velocity_l2_vs_fp64_{name}"] = float(torch.linalg.vector_norm(a - b) /
+                                                                    torch.linalg.vector_norm(b))
+    return out
+
+
 def traffic_from_profiles(form):
     """HBM bytes per launch of the headline kernel from the committed rocprofv3
     PMC summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel factor)."""
@@ -405,6 +475,8 @@ def main():
         torch.cuda.empty_cache()
     if args.channel_steps > 0:
         out["channel"] = channel_leg(torch, vfx, args.channel_steps)
+    if args.cube_steps > 0:
+        out["refined_cube"] = cube_leg(torch, vfx, args.cube_steps)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
 

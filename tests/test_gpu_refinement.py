@@ -178,3 +178,37 @@ def test_refined_graph_replay_matches_eager(cuda):
     st.replay(odd, 1)
     with pytest.raises(ValueError):
         st.replay(odd, 1)
+
+
+def test_wall_mounted_cube_fp64_matches_oracle(cuda):
+    """The nest standing on the floor with a cube resolved on the fine level:
+    inlet/outlet/floor/slip-top boundaries on the coarse level, floor and cube
+    on the fine level, coarse-to-fine cells next to the floor moved up with
+    offset (0, 0, -1).  FP64, 15 coarse steps vs the oracle on every fine fluid
+    node and every active coarse fluid node."""
+    from tests.test_oracle_refinement import refined_oracle_run_walls, small_cube_case
+    from virtualfluids_amd import geometry
+
+    steps = 15
+    print(f"[wall nest fp64 x {steps}]", flush=True)
+    case = small_cube_case()
+    nest = case.nest
+    Nc, Nf = int(np.prod(nest.coarse_dims)), int(np.prod(nest.fine_dims))
+    rng = np.random.default_rng(12)
+    Fc = np.tile(oracle.equilibrium(0.0, case.u_ref, 0.0, 0.0), (Nc, 1)) + rng.normal(0, 1e-4, (Nc, 27)) * cases.W
+    Ff = np.tile(oracle.equilibrium(0.0, case.u_ref, 0.0, 0.0), (Nf, 1)) + rng.normal(0, 1e-4, (Nf, 27)) * cases.W
+    st = vfx.RefinedStepper(nest, case.omega, storage="fp64", coarse_solid=case.coarse_solid,
+                            fine_solid=case.fine_solid, coarse_bcs=geometry.boundary_conditions(case.coarse_bcs),
+                            fine_bcs=geometry.boundary_conditions(case.fine_bcs))
+    st.coarse.write_populations(Fc, even=True)
+    st.fine.write_populations(Ff, even=True)
+    st.step(steps)
+    torch.cuda.synchronize()
+    got_c = st.coarse.read_populations(st.even_c).cpu().numpy()
+    got_f = st.fine.read_populations(st.even_f).cpu().numpy()
+    ref_c, ref_f = refined_oracle_run_walls(case, Fc, Ff, steps, case.omega)
+    act = (nest.coarse_state == 0) & ~case.coarse_solid
+    ffl = ~case.fine_solid
+    ec, ef = rel(got_c[act], ref_c[act]), rel(got_f[ffl], ref_f[ffl])
+    print(f"coarse {ec:.3e} fine {ef:.3e}")
+    assert ec <= REL and ef <= REL

@@ -266,3 +266,105 @@ def test_shear_wave_through_nest_decays_at_viscous_rate():
     assert abs(fit_f / amp - 1) < 0.01
     # the fine level carries the same wave: residual small against the amplitude
     assert np.abs(uxf - fit_f * np.sin(k * yf)).max() < 0.02 * amp
+
+
+# ---------------------------------------------------------------- wall nest
+SMALL_CUBE = dict(coarse_dims=(40, 24, 20), nest_cells=(20, 14, 12), cube=6, cube_x=14, upstream=6,
+                  u_ref=0.05, re=200.0)
+
+
+def small_cube_case():
+    from virtualfluids_amd import geometry
+    return geometry.wall_mounted_cube(**SMALL_CUBE)
+
+
+def cube_oracle_groups(tables):
+    out = []
+    for b in tables:
+        if b.kind == 4:
+            out.append(oracle.BCGroup(b.nodes, kind=4, rho=b.rho, nbr=b.neighbors, normal=b.normal))
+        elif b.velocity is not None:
+            out.append(oracle.BCGroup(b.nodes, b.q27, *b.velocity, kind=b.kind))
+        else:
+            out.append(oracle.BCGroup(b.nodes, b.q27, kind=b.kind))
+    return out
+
+
+def cube_masks(case):
+    """Oracle node states: 1 solid with boundary links, 2 not computed (inactive
+    coarse core; the fine floor layer, which the fine box's periodic wrap
+    touches from the top layers that the coarse-to-fine step rebuilds)."""
+    nest = case.nest
+    cm = np.where(nest.coarse_state != 0, 2, 0).astype(np.uint8)
+    cm[case.coarse_solid] = np.maximum(cm[case.coarse_solid], 1)
+    fx, fy, fz = nest.fine_dims
+    fm = case.fine_solid.astype(np.uint8)
+    fm.reshape(fz, fy, fx)[0] = 2
+    return cm, fm
+
+
+def refined_oracle_run_walls(case, Fc, Ff, steps, omega_c):
+    nest = case.nest
+    omega_f = vfx.fine_omega(omega_c)
+    cm, fm = cube_masks(case)
+    cb, fb = cube_oracle_groups(case.coarse_bcs), cube_oracle_groups(case.fine_bcs)
+    for _ in range(steps):
+        Ff = oracle.run(Ff, nest.fine_dims, 1, omega_f, solid=fm, bcs=fb)
+        Ff = oracle.run(Ff, nest.fine_dims, 1, omega_f, solid=fm, bcs=fb)
+        Fc = oracle.run(Fc, nest.coarse_dims, 1, omega_c, solid=cm, bcs=cb)
+        Fc = oracle.scale_fc(Fc, Ff, nest.coarse_dims, nest.fine_dims, nest.fc_coarse, nest.fc_fine, omega_c, omega_f,
+                             nest.fc_offset)
+        Ff = oracle.scale_cf(Fc, Ff, nest.coarse_dims, nest.fine_dims, nest.cf_coarse, nest.cf_fine, omega_c, omega_f,
+                             nest.cf_offset)
+    return Fc, Ff
+
+
+def test_wall_nest_tables():
+    """The nest standing on the floor: no interface on the floor face, the
+    coarse-to-fine cells that touch the floor move up one cell with offset
+    (0, 0, -1), every interpolation source cell is all fluid and valid."""
+    case = small_cube_case()
+    nest = case.nest
+    nx, ny, nz = nest.coarse_dims
+    fx, fy, fz = nest.fine_dims
+    csol = case.coarse_solid.reshape(nz, ny, nx)
+    fsol = case.fine_solid.reshape(fz, fy, fx)
+    depth = nest.coarse_depth.reshape(nz, ny, nx)
+    off = np.stack(nest.cf_offset)
+    moved = np.abs(off).sum(axis=0) > 0
+    assert moved.any() and (off[:, moved] == np.array([[0.0], [0.0], [-1.0]])).all()
+    for base in nest.cf_coarse:
+        x, y, z = base % nx, (base // nx) % ny, base // (nx * ny)
+        for c in range(8):
+            cx, cy, cz = x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1)
+            assert not csol[cz, cy, cx] and depth[cz, cy, cx] <= 1
+    for base in nest.fc_fine:
+        x, y, z = base % fx, (base // fx) % fy, base // (fx * fy)
+        for c in range(8):
+            assert not fsol[z + ((c >> 2) & 1), y + ((c >> 1) & 1), x + (c & 1)]
+    # coarse-to-fine cells sit on the five interface faces only: each one is on
+    # an x or y face or the top face, and the floor row is covered through the
+    # side faces
+    mx, my, mz = nest.cf_fine % fx, (nest.cf_fine // fx) % fy, nest.cf_fine // (fx * fy)
+    cx, cy, cz = nest.lo[0] + mx // 2, nest.lo[1] + my // 2, nest.lo[2] + mz // 2
+    on_face = ((cx == nest.lo[0]) | (cx == nest.hi[0] - 1) | (cy == nest.lo[1]) | (cy == nest.hi[1] - 1) |
+               (cz == nest.hi[2] - 1))
+    assert on_face.all() and (cz == 0).any()
+
+
+def test_wall_nest_oracle_runs():
+    """The oracle advances the small wall-mounted cube case on both levels
+    (every wall link closed, no interface link left open) and the flow stays
+    bounded and attached to the inflow."""
+    case = small_cube_case()
+    nest = case.nest
+    Nc, Nf = int(np.prod(nest.coarse_dims)), int(np.prod(nest.fine_dims))
+    u0 = case.u_ref
+    Fc = np.tile(oracle.equilibrium(0.0, u0, 0.0, 0.0), (Nc, 1))
+    Ff = np.tile(oracle.equilibrium(0.0, u0, 0.0, 0.0), (Nf, 1))
+    Fc, Ff = refined_oracle_run_walls(case, Fc, Ff, 40, case.omega)
+    mc = np.array([oracle.macroscopic(f) for f in Fc])
+    act = (nest.coarse_state == 0) & ~case.coarse_solid
+    assert np.isfinite(mc).all() and np.abs(mc[act, 1:]).max() < 0.2
+    mf = np.array([oracle.macroscopic(f) for f in Ff])
+    assert np.isfinite(mf).all() and np.abs(mf[~case.fine_solid, 1:]).max() < 0.2

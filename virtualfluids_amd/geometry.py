@@ -200,3 +200,112 @@ def taylor_green(n, u0=0.04, re=1600.0):
     nu = (n / (2.0 * np.pi)) * u0 / re
     omega = 1.0 / (3.0 * nu + 0.5)
     return [a.ravel() for a in (drho, ux, uy, uz)], omega, nu
+
+
+@dataclass
+class RefinedCase:
+    """A two-level setup: nest tables, per-level solids and boundary tables."""
+    nest: object
+    coarse_solid: np.ndarray
+    fine_solid: np.ndarray
+    coarse_bcs: list
+    fine_bcs: list
+    omega: float
+    u_ref: float
+    cube: tuple
+
+
+def _links(solid, dims, target_mask, q=0.5):
+    """Fluid nodes with links into `target_mask` nodes (periodic wrap):
+    (nodes, q27) with q on those links and -1 elsewhere."""
+    nx, ny, nz = dims
+    fluid = ~solid
+    per = {}
+    for d in range(1, 27):
+        hit = _shift(target_mask, d) & fluid
+        if hit.any():
+            per[d] = np.flatnonzero(hit)
+    if not per:
+        return np.empty(0, np.int64), np.full((27, 0), -1.0, np.float32)
+    nodes = np.unique(np.concatenate(list(per.values())))
+    q27 = np.full((27, nodes.size), -1.0, np.float32)
+    for d, idx in per.items():
+        q27[d, np.searchsorted(nodes, idx)] = q
+    return nodes.astype(np.int64), q27
+
+
+def wall_mounted_cube(coarse_dims=(256, 128, 128), nest_cells=(96, 64, 48), cube=32, cube_x=64, upstream=24,
+                      u_ref=0.05, re=1.0e4, noise=0.02, noise_seed=2024):
+    """Wall-mounted cube in a boundary layer, coarse level + one 2:1 nest.
+
+    Coarse box (coarse units, x streamwise, z up): inlet plane x = 0 with the
+    power-law profile u(z) = u_ref (z'/H)^0.2 (z' = height above the floor
+    wall at z = 1/2, H = cube height) plus a uniform +-noise*u(z) perturbation
+    of every component (seeded); pressure outlet on the last fluid layer before
+    x = nx-1; no-slip floor (solid z = 0, halfway links); slip top (solid
+    z = nz-1); periodic in y.  The cube occupies coarse cells
+    [cube_x, cube_x+cube) x [centred in y) x [0, cube) above the floor; the
+    nest covers coarse cells x in [cube_x-upstream, +nest_cells[0]), centred in
+    y, z in [0, nest_cells[2]) and stands on the floor (no interface there).
+    The cube is resolved on the fine level only (it lies in the inactive core
+    of the coarse level).  Re on the cube height at u_ref."""
+    from ._lib import (VFX_BC_NOSLIP_INTERPOLATED, VFX_BC_PRESSURE_NEQ, VFX_BC_SLIP, VFX_BC_VELOCITY_INTERPOLATED)
+    from .refinement import NestedBox
+
+    nx, ny, nz = coarse_dims
+    H = float(cube)
+    cy0 = (ny - cube) // 2
+    lo = (cube_x - upstream, (ny - nest_cells[1]) // 2, 0)
+    hi = (lo[0] + nest_cells[0], lo[1] + nest_cells[1], nest_cells[2])
+    # coarse walls
+    csol = np.zeros((nz, ny, nx), bool)
+    inlet = np.zeros_like(csol)
+    outlet = np.zeros_like(csol)
+    floor = np.zeros_like(csol)
+    top = np.zeros_like(csol)
+    inlet[1:-1, :, 0] = True
+    outlet[1:-1, :, nx - 1] = True
+    floor[0, :, :] = True
+    top[nz - 1, :, :] = True
+    csol = inlet | outlet | floor | top
+    nest = NestedBox(coarse_dims, lo, hi, wall_faces=("z-",), coarse_solid=None)
+    fx, fy, fz = nest.fine_dims
+    # fine solids: floor layer (z = 1/4 < 1/2) and the cube [x0 - 1/2, x0 + cube - 1/2] (coarse units)
+    px, py, pz = [p.reshape(fz, fy, fx) for p in nest.fine_positions()]
+    fcube = ((px > cube_x - 0.5) & (px < cube_x + cube - 0.5) & (py > cy0 - 0.5) & (py < cy0 + cube - 0.5) &
+             (pz < cube + 0.5))
+    ffloor = pz < 0.5
+    fsol = fcube | ffloor
+    nest = NestedBox(coarse_dims, lo, hi, wall_faces=("z-",), coarse_solid=csol, fine_solid=fsol)
+
+    # coarse boundary tables
+    cfl = ~csol
+    zc = np.arange(nz, dtype=np.float64)
+    prof = u_ref * np.clip((zc - 0.5) / H, 0.0, None) ** 0.2
+    in_nodes, in_q = _links(csol, coarse_dims, inlet)
+    rng = np.random.default_rng(noise_seed)
+    zin = in_nodes // (nx * ny)
+    vel = np.zeros((3, in_nodes.size))
+    vel[0] = prof[zin]
+    vel += noise * prof[zin] * rng.uniform(-1.0, 1.0, vel.shape)
+    fl_nodes, fl_q = _links(csol, coarse_dims, floor)
+    tp_nodes, tp_q = _links(csol, coarse_dims, top)
+    zz, yy = np.meshgrid(np.arange(1, nz - 1), np.arange(ny), indexing="ij")
+    out_nodes = ((nx - 2) + nx * (yy + ny * zz)).ravel().astype(np.int64)
+    coarse_bcs = [
+        BCTable(VFX_BC_VELOCITY_INTERPOLATED, in_nodes, in_q, velocity=vel),
+        BCTable(VFX_BC_NOSLIP_INTERPOLATED, fl_nodes, fl_q),
+        BCTable(VFX_BC_SLIP, tp_nodes, tp_q),
+        BCTable(VFX_BC_PRESSURE_NEQ, out_nodes, rho=np.zeros(out_nodes.size), neighbors=out_nodes - 1, normal=D_P00),
+    ]
+    # links into the outlet plane are dropped (pressure boundary); links into
+    # the inlet/outlet plane edges belong to the floor/top groups above
+    # fine boundary tables: floor and cube, halfway
+    ff_nodes, ff_q = _links(fsol, nest.fine_dims, ffloor)
+    fc_nodes, fc_q = _links(fsol, nest.fine_dims, fcube)
+    fine_bcs = [BCTable(VFX_BC_NOSLIP_INTERPOLATED, ff_nodes, ff_q),
+                BCTable(VFX_BC_NOSLIP_INTERPOLATED, fc_nodes, fc_q)]
+    nu = u_ref * H / re
+    omega = 1.0 / (3.0 * nu + 0.5)
+    return RefinedCase(nest=nest, coarse_solid=csol.ravel(), fine_solid=fsol.ravel(), coarse_bcs=coarse_bcs,
+                       fine_bcs=fine_bcs, omega=omega, u_ref=u_ref, cube=(cube_x, cy0, 0, cube))

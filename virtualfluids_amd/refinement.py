@@ -38,32 +38,61 @@ from .lattice import Lattice, _stream
 INACTIVE = 2  # value in the coarse node-state array
 
 
-class NestedBox:
-    """Index tables of one fine nest inside a coarse box (x fastest)."""
+FACES = ("x-", "x+", "y-", "y+", "z-", "z+")
 
-    def __init__(self, coarse_dims, lo, hi):
+
+class NestedBox:
+    """Index tables of one fine nest inside a coarse box (x fastest).
+
+    wall_faces: nest faces that lie on a wall shared by both levels (for
+    example the floor under a wall-mounted obstacle): no interface there, the
+    nest reaches the wall layer.  coarse_solid / fine_solid (bool, box-shaped):
+    solid nodes of each level; an interface cell with a solid source node is
+    moved to the first all-fluid neighbour cell in D3Q27 direction order and
+    interpolated back with the offset -c_d, as VirtualFluids' grid generator
+    does (GridInterface.cpp:350-414, LevelGridBuilder.cpp:400-422)."""
+
+    def __init__(self, coarse_dims, lo, hi, wall_faces=(), coarse_solid=None, fine_solid=None):
         self.coarse_dims = tuple(int(v) for v in coarse_dims)
         self.lo = np.array(lo, dtype=np.int64)
         self.hi = np.array(hi, dtype=np.int64)
+        self.wall_faces = tuple(wall_faces)
+        for f in self.wall_faces:
+            if f not in FACES:
+                raise ValueError(f"unknown face {f}")
         if np.any(self.hi - self.lo < 6):
             raise ValueError("the nest needs at least 6 coarse cells per axis")
-        if np.any(self.lo < 1) or np.any(self.hi > np.array(self.coarse_dims) - 2):
-            raise ValueError("the nest must stay one coarse node clear of the box faces")
+        dims = np.array(self.coarse_dims)
+        for a, name in enumerate("xyz"):
+            if f"{name}-" not in self.wall_faces and self.lo[a] < 1:
+                raise ValueError("an interface face must stay one coarse node clear of the box faces")
+            if f"{name}+" not in self.wall_faces and self.hi[a] > dims[a] - 2:
+                raise ValueError("an interface face must stay one coarse node clear of the box faces")
+            if self.lo[a] < 0 or self.hi[a] > dims[a] - 1:
+                raise ValueError("the nest must lie inside the coarse box")
         self.fine_dims = tuple(int(v) for v in 2 * (self.hi - self.lo))
         nx, ny, nz = self.coarse_dims
         fx, fy, fz = self.fine_dims
+        csol = np.zeros((nz, ny, nx), bool) if coarse_solid is None else np.asarray(coarse_solid, bool).reshape(nz, ny, nx)
+        fsol = np.zeros((fz, fy, fx), bool) if fine_solid is None else np.asarray(fine_solid, bool).reshape(fz, fy, fx)
 
-        # coarse depth inside the nest (-1 outside)
+        # coarse depth inside the nest, measured from the interface faces only (-1 outside)
         z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
         pos = (x, y, z)
         inside = np.ones(x.shape, dtype=bool)
-        depth = np.full(x.shape, np.iinfo(np.int64).max, dtype=np.int64)
-        for a in range(3):
+        big = np.iinfo(np.int64).max
+        depth = np.full(x.shape, big, dtype=np.int64)
+        for a, name in enumerate("xyz"):
             inside &= (pos[a] >= self.lo[a]) & (pos[a] <= self.hi[a])
-            depth = np.minimum(depth, np.minimum(pos[a] - self.lo[a], self.hi[a] - pos[a]))
+            if f"{name}-" not in self.wall_faces:
+                depth = np.minimum(depth, pos[a] - self.lo[a])
+            if f"{name}+" not in self.wall_faces:
+                depth = np.minimum(depth, self.hi[a] - pos[a])
         depth = np.where(inside, depth, -1)
         self.coarse_state = np.where(depth >= 3, INACTIVE, 0).astype(np.uint8).ravel()
         self.coarse_depth = depth.ravel()
+        cvalid = (depth <= 1) & ~csol  # coarse nodes that hold valid data when the interface is rebuilt
+        fvalid = ~fsol
 
         def cidx(cx, cy, cz):
             return cx + nx * (cy + ny * cz)
@@ -71,20 +100,25 @@ class NestedBox:
         def fidx(mx, my, mz):
             return mx + fx * (my + fy * mz)
 
-        # fine to coarse: depth-2 coarse nodes
-        fz_, fy_, fx_ = np.nonzero(depth == 2)
+        # fine to coarse: depth-2 coarse nodes (fluid ones)
+        fz_, fy_, fx_ = np.nonzero((depth == 2) & ~csol)
         self.fc_coarse = cidx(fx_, fy_, fz_).astype(np.int64)
-        self.fc_fine = fidx(2 * (fx_ - self.lo[0]) - 1, 2 * (fy_ - self.lo[1]) - 1,
-                            2 * (fz_ - self.lo[2]) - 1).astype(np.int64)
-        # coarse to fine: coarse cells of the outermost nest layer
+        fine_base = np.stack([2 * (fx_ - self.lo[0]) - 1, 2 * (fy_ - self.lo[1]) - 1, 2 * (fz_ - self.lo[2]) - 1])
+        fine_base, self.fc_offset = _move_off_solid(fine_base, fvalid, (fx, fy, fz))
+        self.fc_fine = fidx(*fine_base).astype(np.int64)
+        # coarse to fine: coarse cells of the outermost layer of every interface face
         cz_, cy_, cx_ = np.meshgrid(np.arange(self.lo[2], self.hi[2]), np.arange(self.lo[1], self.hi[1]),
                                     np.arange(self.lo[0], self.hi[0]), indexing="ij")
         shell = np.zeros(cx_.shape, dtype=bool)
-        for a, c in enumerate((cx_, cy_, cz_)):
-            shell |= (c == self.lo[a]) | (c == self.hi[a] - 1)
+        for a, (c, name) in enumerate(zip((cx_, cy_, cz_), "xyz")):
+            if f"{name}-" not in self.wall_faces:
+                shell |= c == self.lo[a]
+            if f"{name}+" not in self.wall_faces:
+                shell |= c == self.hi[a] - 1
         cx_, cy_, cz_ = cx_[shell], cy_[shell], cz_[shell]
-        self.cf_coarse = cidx(cx_, cy_, cz_).astype(np.int64)
         self.cf_fine = fidx(2 * (cx_ - self.lo[0]), 2 * (cy_ - self.lo[1]), 2 * (cz_ - self.lo[2])).astype(np.int64)
+        coarse_base, self.cf_offset = _move_off_solid(np.stack([cx_, cy_, cz_]), cvalid, (nx, ny, nz))
+        self.cf_coarse = cidx(*coarse_base).astype(np.int64)
 
     @property
     def n_coarse_active(self) -> int:
@@ -95,6 +129,47 @@ class NestedBox:
         fx, fy, fz = self.fine_dims
         mz, my, mx = np.meshgrid(np.arange(fz), np.arange(fy), np.arange(fx), indexing="ij")
         return [(self.lo[a] + 0.25 + 0.5 * m).ravel() for a, m in enumerate((mx, my, mz))]
+
+
+def _cell_valid(base, valid, dims):
+    """All 8 corners of the cells with corner MMM `base` (3 x n) are valid."""
+    nx, ny, nz = dims
+    ok = np.ones(base.shape[1], dtype=bool)
+    for c in range(8):
+        x = base[0] + (c & 1)
+        y = base[1] + ((c >> 1) & 1)
+        z = base[2] + ((c >> 2) & 1)
+        inb = (x >= 0) & (x < nx) & (y >= 0) & (y < ny) & (z >= 0) & (z < nz)
+        ok &= inb
+        ok[inb] &= valid[z[inb], y[inb], x[inb]]
+    return ok
+
+
+def _move_off_solid(base, valid, dims):
+    """Cells with an invalid corner move to the first all-valid neighbour cell
+    in D3Q27 direction order (GridInterface::findOffsetCF/FC); returns the
+    moved bases and the offsets (-c_d; 0 for cells that stay) as 3 arrays or
+    None when no cell moved."""
+    from .lattice import CX, CY, CZ
+
+    base = base.copy()
+    ok = _cell_valid(base, valid, dims)
+    if ok.all():
+        return base, None
+    off = np.zeros((3, base.shape[1]))
+    todo = np.nonzero(~ok)[0]
+    for d in range(1, 27):
+        if todo.size == 0:
+            break
+        cand = base[:, todo] + np.array([CX[d], CY[d], CZ[d]])[:, None]
+        good = _cell_valid(cand, valid, dims)
+        sel = todo[good]
+        base[:, sel] = cand[:, good]
+        off[:, sel] = -np.array([CX[d], CY[d], CZ[d]], dtype=float)[:, None]
+        todo = todo[~good]
+    if todo.size:
+        raise ValueError(f"{todo.size} interface cells have no all-fluid neighbour cell")
+    return base, [off[0], off[1], off[2]]
 
 
 class ICells:
@@ -153,31 +228,49 @@ def fine_omega(omega_coarse: float) -> float:
 
 class RefinedStepper:
     """One coarse step = two fine steps, one coarse step, fine-to-coarse, then
-    coarse-to-fine, on a NestedBox (both levels direct-addressed periodic
-    boxes; inactive coarse nodes masked)."""
+    coarse-to-fine, on a NestedBox (both levels direct-addressed boxes;
+    inactive coarse nodes masked).  Each level's step is collide+stream, its
+    post-collision boundary conditions, the parity flip and its pre-collision
+    boundary conditions (UpdateGrid27.cpp:54-95)."""
 
     def __init__(self, nest: NestedBox, omega_coarse, storage="fp64", collision="k17", device="cuda",
-                 fp16_scale=None, variant="default"):
+                 fp16_scale=None, variant="default", coarse_solid=None, fine_solid=None, coarse_bcs=(),
+                 fine_bcs=(), reference_velocity=(0.0, 0.0, 0.0)):
         self.nest = nest
         self.collision = collision
         self.omega_c = float(omega_coarse)
         self.omega_f = fine_omega(self.omega_c)
-        self.coarse = Lattice(nest.coarse_dims, storage=storage, omega=self.omega_c,
-                              solid=nest.coarse_state != 0, device=device, fp16_scale=fp16_scale, variant=variant)
+        cmask = nest.coarse_state != 0
+        if coarse_solid is not None:
+            cmask = cmask | np.asarray(coarse_solid, bool).ravel()
+        self.coarse = Lattice(nest.coarse_dims, storage=storage, omega=self.omega_c, solid=cmask, device=device,
+                              fp16_scale=fp16_scale, variant=variant, reference_velocity=reference_velocity)
         self.fine = Lattice(nest.fine_dims, storage=storage, omega=self.omega_f, device=device,
-                            fp16_scale=fp16_scale, variant=variant)
-        self.fc = ICells(nest.fc_coarse, nest.fc_fine, device)
-        self.cf = ICells(nest.cf_coarse, nest.cf_fine, device)
+                            solid=None if fine_solid is None else np.asarray(fine_solid, bool).ravel(),
+                            fp16_scale=fp16_scale, variant=variant, reference_velocity=reference_velocity)
+        self.fc = ICells(nest.fc_coarse, nest.fc_fine, device, offsets=nest.fc_offset)
+        self.cf = ICells(nest.cf_coarse, nest.cf_fine, device, offsets=nest.cf_offset)
+        self.coarse_bcs = list(coarse_bcs)
+        self.fine_bcs = list(fine_bcs)
         self.even_c = True
         self.even_f = True
+
+    @staticmethod
+    def _level_step(lat, bcs, collision, even, stream):
+        lat.collide_stream(collision, even, stream=stream)
+        for bc in bcs:
+            if not bc.pre_collision:
+                lat.apply_bc(bc, even, stream=stream)
+        for bc in bcs:
+            if bc.pre_collision:
+                lat.apply_bc(bc, not even, stream=stream)
+        return not even
 
     def step(self, n=1, stream=None):
         for _ in range(n):
             for _ in range(2):
-                self.fine.collide_stream(self.collision, self.even_f, stream=stream)
-                self.even_f = not self.even_f
-            self.coarse.collide_stream(self.collision, self.even_c, stream=stream)
-            self.even_c = not self.even_c
+                self.even_f = self._level_step(self.fine, self.fine_bcs, self.collision, self.even_f, stream)
+            self.even_c = self._level_step(self.coarse, self.coarse_bcs, self.collision, self.even_c, stream)
             scale_fc(self.coarse, self.fine, self.fc, self.even_c, self.even_f, stream)
             scale_cf(self.coarse, self.fine, self.cf, self.even_c, self.even_f, stream)
 
@@ -203,4 +296,4 @@ class RefinedStepper:
     @property
     def lups_per_step(self) -> int:
         """Lattice updates per coarse step, fine counted twice (two fine steps)."""
-        return self.nest.n_coarse_active + 2 * int(np.prod(self.nest.fine_dims))
+        return self.coarse.n_fluid + 2 * self.fine.n_fluid
