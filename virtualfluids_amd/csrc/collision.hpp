@@ -445,8 +445,8 @@ template <typename T> __device__ __forceinline__ void bgk(T (&m)[27], T omega, T
 //  * omega-only factors are folded on the host (Rates::om1, c3, A4, B4, lim*1)
 //    and the limiter becomes x (1-O) lambda / (|x| + lambda);
 //  * divisions use the hardware reciprocal (1 ulp).
-// Macroscopic sums go row by row along x (pairwise), which is as well
-// conditioned as the pyramid sums.  Not bit-identical to the reference order;
+// Macroscopic sums are the partial sums of the forward sweeps (z rows, then
+// y, then x), which are as well conditioned as the pyramid sums.  Not bit-identical to the reference order;
 // tested against the FP64 oracle by error norms.
 __device__ __forceinline__ float rcpf(float x) { return __builtin_amdgcn_rcpf(x); }
 
@@ -459,6 +459,16 @@ __device__ __forceinline__ void fwdf(float& a, float& b, float& c, float vv, flo
     const float d = c - a;
     const float m0 = s + b;
     const float mk = k == 0.f ? m0 : m0 + k; // k is a compile-time constant after unrolling
+    a = m0;
+    if (OUT >= 1) b = d - mk * vv;
+    if (OUT >= 2) c = (s - tv * d) + mk * v2;
+}
+// the same transform with the sums s = a + c, d = c - a, m0 = s + b already formed
+template <int OUT>
+__device__ __forceinline__ void fwdf_sums(float& a, float& b, float& c, float s, float d, float m0, float vv, float v2,
+                                          float tv, float k)
+{
+    const float mk = k == 0.f ? m0 : m0 + k;
     a = m0;
     if (OUT >= 1) b = d - mk * vv;
     if (OUT >= 2) c = (s - tv * d) + mk * v2;
@@ -479,14 +489,41 @@ __device__ __forceinline__ void bwdf(float& a, float& b, float& c, float k, floa
     c = r2;
 }
 
+// The same transform returning the outer outputs doubled, (2 r0, r1, 2 r2):
+// with hv2m = v2 - vv, hv2p = v2 + vv, v2mh = 2 vv - 1, v2ph = 2 vv + 1 the
+// factor 1/2 of the outer rows disappears and the centre output becomes
+// a - (2 r0 + 2 r2)/2.  Every value is exactly twice the one bwdf forms
+// (scaling by 2 commutes with rounding), so a sweep over doubled inputs, with
+// its weight constant doubled too, is exact bookkeeping: after x, y and z a
+// population of direction d comes out as 2^n f_d, n = number of non-zero
+// components of c_d, and the store folds 2^-n into its scale factor.
+__device__ __forceinline__ void bwdf2(float& a, float& b, float& c, float k, float hv2m, float hv2p, float v2mh,
+                                      float v2ph)
+{
+    const float ak = k == 0.f ? a : a + k; // k is a compile-time constant after unrolling
+    const float r0 = c + b * v2mh + ak * hv2m;
+    const float r2 = c + b * v2ph + ak * hv2p;
+    const float r1 = a - 0.5f * (r0 + r2);
+    a = r0;
+    b = r1;
+    c = r2;
+}
+
 #define VFX_M(a, b, c) m[cube(a, b, c)]
 
+// OUTER2: run the backward sweeps with bwdf2, so that population d leaves
+// scaled by 2^weight_class(d) (the caller's store removes the factor; used
+// where the store multiplies by a scale anyway, i.e. FP16 storage).
+template <bool OUTER2>
 __device__ __forceinline__ void k17_fast(float (&m)[27], const Rates<float>& R, float (&mv)[4])
 {
     constexpr float c1o3 = 1.f / 3.f, c1o9 = 1.f / 9.f, c1o27 = 1.f / 27.f, c2o3 = 2.f / 3.f;
-    // ---- macroscopic values from the sums the z sweep needs anyway: per (x,y)
-    // row s = f(z-) + f(z+), d = f(z+) - f(z-), m0 = s + f(z0); then
-    // pairwise over the nine rows
+    // ---- macroscopic values from the sums the sweeps form anyway.  Per (x,y)
+    // row of the z sweep: s = f(z-) + f(z+), d = f(z+) - f(z-), m0 = s + f(z0).
+    // The zeroth-moment rows enter the y sweep unchanged (their sums do not
+    // depend on the velocity), so its sums ys, yd, ym per x plane are formed
+    // here too: drho = (ym0 + ym2) + ym1 and jx = ym2 - ym0 are the sums of the
+    // x sweep's first line, jy = sum of yd; jz sums the nine z differences.
     float zs[3][3], zd[3][3], zm[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -496,10 +533,18 @@ __device__ __forceinline__ void k17_fast(float (&m)[27], const Rates<float>& R, 
             zd[i][j] = VFX_M(i, j, 2) - VFX_M(i, j, 0);
             zm[i][j] = zs[i][j] + VFX_M(i, j, 1);
         }
-    const float drho = (((zm[0][0] + zm[2][2]) + (zm[0][2] + zm[2][0])) + ((zm[0][1] + zm[2][1]) + (zm[1][0] + zm[1][2]))) + zm[1][1];
+    float ys[3], yd[3], ym[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        ys[i] = zm[i][0] + zm[i][2];
+        yd[i] = zm[i][2] - zm[i][0];
+        ym[i] = ys[i] + zm[i][1];
+    }
+    const float xs = ym[0] + ym[2];
+    const float jx = ym[2] - ym[0];
+    const float drho = xs + ym[1];
+    const float jy = (yd[0] + yd[2]) + yd[1];
     const float jz = (((zd[0][0] + zd[2][2]) + (zd[0][2] + zd[2][0])) + ((zd[0][1] + zd[2][1]) + (zd[1][0] + zd[1][2]))) + zd[1][1];
-    const float jx = ((zm[2][0] - zm[0][0]) + (zm[2][2] - zm[0][2])) + (zm[2][1] - zm[0][1]);
-    const float jy = ((zm[0][2] - zm[0][0]) + (zm[2][2] - zm[2][0])) + (zm[1][2] - zm[1][0]);
     const float oneOverRho = rcpf(drho + 1.f);
     const float vx = jx * oneOverRho + R.fx;
     const float vy = jy * oneOverRho + R.fy;
@@ -520,11 +565,11 @@ __device__ __forceinline__ void k17_fast(float (&m)[27], const Rates<float>& R, 
         }
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-        fwdf<2>(VFX_M(i, 0, 0), VFX_M(i, 1, 0), VFX_M(i, 2, 0), vy, vy2, ty, wv[i]);
+        fwdf_sums<2>(VFX_M(i, 0, 0), VFX_M(i, 1, 0), VFX_M(i, 2, 0), ys[i], yd[i], ym[i], vy, vy2, ty, wv[i]);
         fwdf<2>(VFX_M(i, 0, 1), VFX_M(i, 1, 1), VFX_M(i, 2, 1), vy, vy2, ty, 0.f);
         fwdf<1>(VFX_M(i, 0, 2), VFX_M(i, 1, 2), VFX_M(i, 2, 2), vy, vy2, ty, wv[i] * c1o3);
     }
-    fwdf<2>(VFX_M(0, 0, 0), VFX_M(1, 0, 0), VFX_M(2, 0, 0), vx, vx2, tx, 1.f);
+    fwdf_sums<2>(VFX_M(0, 0, 0), VFX_M(1, 0, 0), VFX_M(2, 0, 0), xs, jx, drho, vx, vx2, tx, 1.f);
     fwdf<2>(VFX_M(0, 1, 0), VFX_M(1, 1, 0), VFX_M(2, 1, 0), vx, vx2, tx, 0.f);
     fwdf<2>(VFX_M(0, 0, 1), VFX_M(1, 0, 1), VFX_M(2, 0, 1), vx, vx2, tx, 0.f);
     fwdf<1>(VFX_M(0, 2, 0), VFX_M(1, 2, 0), VFX_M(2, 2, 0), vx, vx2, tx, c1o3);
@@ -612,33 +657,68 @@ __device__ __forceinline__ void k17_fast(float (&m)[27], const Rates<float>& R, 
     m001 = -m001;
 
     // ---- backward: x, y, z
-    {
-        const float hvm = 0.5f * (vx2 - vx), hvp = 0.5f * (vx2 + vx), vmh = vx - 0.5f, vph = vx + 0.5f;
+    if constexpr (OUTER2) {
+        // a line's inputs carry the factor 2 of every outer index already
+        // swept (lam), so its weight constant is scaled alike
+        constexpr float lam[3] = {2.f, 1.f, 2.f};
+        {
+            const float hm = vx2 - vx, hp = vx2 + vx, bm = tx - 1.f, bp = tx + 1.f;
 #pragma unroll
-        for (int b = 0; b < 3; ++b)
+            for (int b = 0; b < 3; ++b)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float k = (b == 1 || c == 1) ? 0.f : (b == 0 && c == 0) ? 1.f : (b == 2 && c == 2) ? c1o9 : c1o3;
-                bwdf(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), k, hvm, hvp, vmh, vph);
-            }
-    }
-    {
-        const float hvm = 0.5f * (vy2 - vy), hvp = 0.5f * (vy2 + vy), vmh = vy - 0.5f, vph = vy + 0.5f;
+                for (int c = 0; c < 3; ++c) {
+                    const float k = (b == 1 || c == 1) ? 0.f : (b == 0 && c == 0) ? 1.f : (b == 2 && c == 2) ? c1o9 : c1o3;
+                    bwdf2(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), k, hm, hp, bm, bp);
+                }
+        }
+        {
+            const float hm = vy2 - vy, hp = vy2 + vy, bm = ty - 1.f, bp = ty + 1.f;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+            for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float k = (c == 1) ? 0.f : wv[i] * (c == 0 ? 1.f : c1o3);
-                bwdf(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), k, hvm, hvp, vmh, vph);
-            }
-    }
-    {
-        const float hvm = 0.5f * (vz2 - vz), hvp = 0.5f * (vz2 + vz), vmh = vz - 0.5f, vph = vz + 0.5f;
+                for (int c = 0; c < 3; ++c) {
+                    const float k = (c == 1) ? 0.f : wv[i] * (c == 0 ? 1.f : c1o3) * lam[i];
+                    bwdf2(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), k, hm, hp, bm, bp);
+                }
+        }
+        {
+            const float hm = vz2 - vz, hp = vz2 + vz, bm = tz - 1.f, bp = tz + 1.f;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+            for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j)
-                bwdf(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), wv[i] * wv[j], hvm, hvp, vmh, vph);
+                for (int j = 0; j < 3; ++j)
+                    bwdf2(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), wv[i] * wv[j] * (lam[i] * lam[j]), hm, hp, bm,
+                          bp);
+        }
+    } else {
+        {
+            const float hvm = 0.5f * (vx2 - vx), hvp = 0.5f * (vx2 + vx), vmh = vx - 0.5f, vph = vx + 0.5f;
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float k = (b == 1 || c == 1) ? 0.f : (b == 0 && c == 0) ? 1.f : (b == 2 && c == 2) ? c1o9 : c1o3;
+                    bwdf(VFX_M(0, b, c), VFX_M(1, b, c), VFX_M(2, b, c), k, hvm, hvp, vmh, vph);
+                }
+        }
+        {
+            const float hvm = 0.5f * (vy2 - vy), hvp = 0.5f * (vy2 + vy), vmh = vy - 0.5f, vph = vy + 0.5f;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const float k = (c == 1) ? 0.f : wv[i] * (c == 0 ? 1.f : c1o3);
+                    bwdf(VFX_M(i, 0, c), VFX_M(i, 1, c), VFX_M(i, 2, c), k, hvm, hvp, vmh, vph);
+                }
+        }
+        {
+            const float hvm = 0.5f * (vz2 - vz), hvp = 0.5f * (vz2 + vz), vmh = vz - 0.5f, vph = vz + 0.5f;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    bwdf(VFX_M(i, j, 0), VFX_M(i, j, 1), VFX_M(i, j, 2), wv[i] * wv[j], hvm, hvp, vmh, vph);
+        }
     }
     mv[0] = drho;
     mv[1] = vx;

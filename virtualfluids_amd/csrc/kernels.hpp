@@ -135,6 +135,20 @@ __device__ __forceinline__ void nbrs_direct_xyz(const Lat& L, uint32_t x, uint32
     kk[6] = ryzp * L.nx + x;
     kk[7] = ryzp * L.nx + xp;
 }
+// interior node (no index wraps): one 24-bit multiply-add chain, then adds
+// of wave-uniform strides
+__device__ __forceinline__ void nbrs_interior(const Lat& L, uint32_t x, uint32_t y, uint32_t z, uint32_t (&kk)[8])
+{
+    const uint32_t sy = L.nx, sz = L.nx * L.ny;
+    kk[0] = __umul24(__umul24(z, L.ny) + y, L.nx) + x;
+    kk[1] = kk[0] + 1;
+    kk[2] = kk[0] + sy;
+    kk[3] = kk[2] + 1;
+    kk[4] = kk[0] + sz;
+    kk[5] = kk[4] + 1;
+    kk[6] = kk[4] + sy;
+    kk[7] = kk[6] + 1;
+}
 __device__ __forceinline__ void nbrs_direct(const Lat& L, uint32_t k, uint32_t (&kk)[8])
 {
     const uint32_t x = k % L.nx;
@@ -163,14 +177,16 @@ __device__ __forceinline__ float in_vgpr(float x)
     asm("v_mov_b32 %0, %1" : "=v"(r) : "s"(x));
     return r;
 }
-template <typename S> struct Scales {
+// OUTER2: the collision hands over 2^c times the populations of weight class
+// c (k17_fast<true>), so the store scales carry 2^-c (exact)
+template <typename S, bool OUTER2 = false> struct Scales {
     float ld[4], st[4];
     __device__ __forceinline__ explicit Scales(const Lat& L)
     {
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             ld[c] = sizeof(S) == 2 ? in_vgpr(L.ld[c]) : L.ld[c];
-            st[c] = sizeof(S) == 2 ? in_vgpr(L.st[c]) : L.st[c];
+            st[c] = sizeof(S) != 2 ? L.st[c] : (OUTER2 && c > 0) ? L.st[c] * (1.f / float(1 << c)) : in_vgpr(L.st[c]);
         }
     }
 };
@@ -188,10 +204,10 @@ __device__ __forceinline__ void gather(const Lat& L, const uint32_t (&kk)[8], T 
 }
 // post-collision populations: the element direction d was read from receives
 // the population of the opposite direction
-template <typename S, typename T, bool EVEN>
+template <typename S, typename T, bool EVEN, bool OUTER2 = false>
 __device__ __forceinline__ void scatter_post(const Lat& L, const uint32_t (&kk)[8], const T (&m)[27])
 {
-    const Scales<S> sc(L);
+    const Scales<S, OUTER2> sc(L);
 #pragma unroll
     for (int s = 0; s < 27; ++s) {
         const int d = dir_at(s);
@@ -210,12 +226,13 @@ __device__ __forceinline__ void scatter_pre(const Lat& L, const uint32_t (&kk)[8
     }
 }
 
-// COLL: VFX_K17 / VFX_BGK; FAST selects the reduced-operation K17 (FP32 only)
-template <typename T, int COLL, bool FAST>
+// COLL: VFX_K17 / VFX_BGK; FAST selects the reduced-operation K17 (FP32 only);
+// OUTER2 (with FAST) leaves class-c populations scaled by 2^c for a scaling store
+template <typename T, int COLL, bool FAST, bool OUTER2 = false>
 __device__ __forceinline__ void collide(T (&m)[27], const Rates<T>& R, T (&mv)[4])
 {
     if constexpr (COLL == VFX_K17) {
-        if constexpr (FAST && sizeof(T) == 4) k17_fast(m, R, mv);
+        if constexpr (FAST && sizeof(T) == 4) k17_fast<OUTER2>(m, R, mv);
         else k17<T>(m, R, mv);
     } else {
         bgk<T>(m, R.omega, mv);
@@ -231,23 +248,44 @@ template <typename T> __device__ __forceinline__ void write_macro(const Lat& L, 
 }
 
 // ------------------------------------------------------ collide + stream ----
+// The command processor hands the workgroups of a launch to the 8 XCDs round
+// robin (workgroup b to XCD b mod 8), and every XCD has its own L2.  Blocks
+// next to each other in the lattice share cache lines at their ends (the
+// +1-shifted reads of the esoteric twist, rows that do not start on a line), so
+// the logical block of hardware workgroup b is chosen to give every XCD runs of
+// C consecutive logical blocks: groups of 8C logical blocks, XCD x taking
+// blocks x*C .. x*C+C-1 of each group (a tail shorter than 8C keeps the plain
+// order).  Shared lines inside a run meet in one L2.  Runs much longer than
+// this spread the XCDs over distant parts of each plane at once, which costs
+// more than it saves for the 2-byte form (measured on MI355X: 16 blocks for
+// 2- and 4-byte populations, 256 for 8-byte ones).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G, uint32_t C)
+{
+    const uint32_t span = kXcds * C, full = G / span * span;
+    return b >= full ? b : (b / span) * span + (b % kXcds) * C + (b / kXcds) % C;
+}
+template <typename S> constexpr uint32_t xcd_run() { return sizeof(S) == 8 ? 256u : 16u; }
+
 // Direct (structured) addressing: one block covers `blockDim.x` consecutive
 // nodes of one x row, so every row index below is wave-uniform (scalar ALU).
 template <typename S, int COLL, bool EVEN, bool FAST>
 __global__ void __launch_bounds__(256) k_step_direct(Lat L, Rates<typename Compute<S>::T> R)
 {
     using T = typename Compute<S>::T;
-    const uint32_t row = blockIdx.x / L.chunks;
-    const uint32_t x = (blockIdx.x - row * L.chunks) * blockDim.x + threadIdx.x;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, xcd_run<S>());
+    const uint32_t row = blk / L.chunks;
+    const uint32_t x = (blk - row * L.chunks) * blockDim.x + threadIdx.x;
     if (x >= L.nx) return;
     const uint32_t y = row % L.ny, z = row / L.ny;
     uint32_t kk[8];
     nbrs_direct_xyz(L, x, y, z, kk);
     if (is_solid(L, kk[0])) return;
     T m[27], mv[4];
+    constexpr bool O2 = FAST && COLL == VFX_K17 && sizeof(S) == 2;
     gather<S, T, EVEN>(L, kk, m);
-    collide<T, COLL, FAST>(m, R, mv);
-    scatter_post<S, T, EVEN>(L, kk, m);
+    collide<T, COLL, FAST, O2>(m, R, mv);
+    scatter_post<S, T, EVEN, O2>(L, kk, m);
     if (L.write_macro) write_macro<T>(L, kk[0], mv);
 }
 
@@ -257,16 +295,17 @@ template <typename S, int COLL, bool EVEN, bool FAST>
 __global__ void __launch_bounds__(256) k_step_sparse(Lat L, Rates<typename Compute<S>::T> R)
 {
     using T = typename Compute<S>::T;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = xcd_block(blockIdx.x, gridDim.x, xcd_run<S>()) * blockDim.x + threadIdx.x;
     if (i >= L.nfluid) return;
     uint32_t kk[8];
     const uint32_t k = L.fluid[i];
     if (k >= L.N) return;
     nbrs_sparse(L, k, kk);
     T m[27], mv[4];
+    constexpr bool O2 = FAST && COLL == VFX_K17 && sizeof(S) == 2;
     gather<S, T, EVEN>(L, kk, m);
-    collide<T, COLL, FAST>(m, R, mv);
-    scatter_post<S, T, EVEN>(L, kk, m);
+    collide<T, COLL, FAST, O2>(m, R, mv);
+    scatter_post<S, T, EVEN, O2>(L, kk, m);
     if (L.write_macro) write_macro<T>(L, kk[0], mv);
 }
 
@@ -812,8 +851,11 @@ __global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
 // top plane (z = nz-1) move with the domain's lid velocity.
 //
 // Both destinations of a population (its streaming slot, or the bounce-back
-// slot) sit at the same element of two different planes, so the wall case is a
-// per-direction select of plane and value, without divergent branches.
+// slot) sit at the same element of two different planes.  Every population is
+// streamed unconditionally (a wall link's streaming slot belongs to the wall
+// node and is never read); only the bounce-back copy of a wall link is a
+// lane-masked store.  The wall layer therefore holds stale values after a
+// step, which nothing reads.
 
 // Exact unsigned division by a run-time invariant d (Granlund & Montgomery,
 // "Division by invariant integers using multiplication", PLDI 1994, fig. 4.1):
@@ -873,6 +915,8 @@ __global__ void __launch_bounds__(256)
     k_step_batched(BatchDev B, const uint2* __restrict__ tiles, const DomainDev<typename Compute<S>::T>* __restrict__ doms)
 {
     using T = typename Compute<S>::T;
+    // tiles in plain order: the ensemble's domains are small and their tiles
+    // start off line boundaries anyway; runs per XCD measured slower here
     for (uint32_t t = blockIdx.x; t < B.ntiles; t += gridDim.x) {
         const uint2 tile = tiles[t];
         const DomainDev<T>& D = doms[tile.x];
@@ -889,14 +933,15 @@ __global__ void __launch_bounds__(256)
         L.ny = D.ny;
         L.nz = D.nz;
         uint32_t kk[8];
-        nbrs_direct_xyz(L, x, y, z, kk); // interior nodes: no index wraps
+        nbrs_interior(L, x, y, z, kk);
         T m[27], mv[4];
 #pragma unroll
         for (int s = 0; s < 27; ++s) {
             const int d = dir_at(s);
             m[s] = Storage<S, T>::ld(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], B.ld[weight_class(d)]);
         }
-        collide<T, VFX_K17, FAST>(m, D.R, mv);
+        constexpr bool O2 = FAST && sizeof(S) == 2; // class-c populations come out as 2^c f
+        collide<T, VFX_K17, FAST, O2>(m, D.R, mv);
         // which faces of the box the node touches (lane masks: scalar logic)
         const bool xl = x == 1, xh = x == L.nx - 2, yl = y == 1, yh = y == L.ny - 2, zl = z == 1, zh = z == L.nz - 2;
         // one buffer descriptor over the domain's planes: a lane's byte offset
@@ -913,21 +958,27 @@ __global__ void __launch_bounds__(256)
             const T top = zh ? T(1) : T(0);
 #pragma unroll
             for (int s = 0; s < 27; ++s)
-                if (cz(dir_at(s)) > 0) m[s] = m[s] - top * D.lid_term[dir_at(s)];
+                if (cz(dir_at(s)) > 0)
+                    m[s] = m[s] - (top * T(O2 ? 1 << weight_class(dir_at(s)) : 1)) * D.lid_term[dir_at(s)];
         }
+        T stc[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) stc[c] = (O2 && c > 0) ? B.st[c] * (1.f / float(1 << c)) : B.st[c];
         S sv[27];
 #pragma unroll
-        for (int s = 0; s < 27; ++s) sv[s] = Storage<S, T>::enc(m[s], B.st[weight_class(dir_at(s))]);
-        // streaming lanes: the slot direction d = inv(e) was read from ...
+        for (int s = 0; s < 27; ++s) sv[s] = Storage<S, T>::enc(m[s], stc[weight_class(dir_at(s))]);
+        // streaming: population e goes to the slot direction d = inv(e) was
+        // read from.  Every lane stores, wall links included: a wall link's
+        // streaming slot is the one the wall node would read e from, and the
+        // wall layer never collides, so nothing reads it; it is not the
+        // bounce-back destination of any node either (that would be the wall
+        // node itself).  Only the bounce-back stores below need a lane mask.
 #pragma unroll
         for (int s = 0; s < 27; ++s) {
-            const int e = dir_at(s);
-            const int d = inv(e);
-            const bool wall = (cx(e) < 0 && xl) || (cx(e) > 0 && xh) || (cy(e) < 0 && yl) || (cy(e) > 0 && yh) ||
-                              (cz(e) < 0 && zl) || (cz(e) > 0 && zh);
-            if (!wall) Buf<S>::st(rs, vo[nbr(d)], slot(d, EVEN) * pb, sv[s]);
+            const int d = inv(dir_at(s));
+            Buf<S>::st(rs, vo[nbr(d)], slot(d, EVEN) * pb, sv[s]);
         }
-        // ... bounce-back lanes: the slot node k reads direction d from next step
+        // bounce-back lanes: the slot node k reads direction d from next step
 #pragma unroll
         for (int s = 0; s < 27; ++s) {
             const int e = dir_at(s);
