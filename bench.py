@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--ensemble", type=int, default=512,
                    help="cavities in the batched-ensemble leg (0 skips it)")
     p.add_argument("--ensemble-steps", type=int, default=100, help="steps per captured graph of the ensemble leg")
+    p.add_argument("--sparse-steps", type=int, default=50,
+                   help="timed steps of the VirtualFluids sparse-layout leg (0 skips it)")
     p.add_argument("--tgv", type=int, default=256, help="Taylor-Green vortex leg edge (0 skips it)")
     p.add_argument("--tgv-steps", type=int, default=2000)
     p.add_argument("--channel-steps", type=int, default=5000,
@@ -213,6 +215,50 @@ def _timed_loop(torch, stream, chunks, body):
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1)
+
+
+def sparse_leg(torch, vfx, steps=50, warmup=10, forms=("fp16", "fp32", "fp64")):
+    """The same 512 x 256 x 256 periodic flow in VirtualFluids' own sparse
+    layout: neighborX/Y/Z arrays (+x/+y/+z neighbour per node, index 0 = dummy
+    node) and the fluid-node list, numbered x-fastest as VirtualFluids' grid
+    generator numbers a box.  This is the layout the reference's
+    runCollision/K17 kernel reads (RunCollision.cuh:74-143); per node it adds the
+    fluid index and three neighbour indices (16 B) to the populations, and four
+    second-level neighbour reads that mostly hit in cache."""
+    nx, ny, nz = DIMS
+    N = nx * ny * nz
+    k = np.arange(N, dtype=np.int64)
+    x, y, z = k % nx, (k // nx) % ny, k // (nx * ny)
+
+    def kof(xx, yy, zz):
+        return (xx % nx) + nx * ((yy % ny) + ny * (zz % nz)) + 1
+
+    nb = [np.zeros(N + 1, np.uint32) for _ in range(3)]
+    nb[0][1:] = kof(x + 1, y, z)
+    nb[1][1:] = kof(x, y + 1, z)
+    nb[2][1:] = kof(x, y, z + 1)
+    fluid = (k + 1).astype(np.uint32)
+    del k, x, y, z
+    fields = initial_flow(torch, DIMS, "cuda")
+    zero = torch.zeros(1, dtype=torch.float64, device="cuda")
+    fields = [torch.cat([zero, f]) for f in fields]
+    stream = torch.cuda.current_stream()
+    out = {"layout": "VirtualFluids sparse (neighbour arrays + fluid-node list), x-fastest numbering",
+           "nodes": N, "steps": steps}
+    for form in forms:
+        lat = vfx.Lattice(storage=form, omega=OMEGA, device="cuda", neighbors=tuple(nb), fluid_nodes=fluid)
+        lat.init_equilibrium(*fields, even=True)
+        st = vfx.Stepper(lat, "k17")
+        st.step(warmup)
+        ms = _timed_loop(torch, stream, steps, lambda i: st.step(1)) / steps
+        s = {"fp16": 2, "fp32": 4, "fp64": 8}[form]
+        out[form] = {"mlups": round(N / ms / 1e3, 1), "ms_per_step": round(ms, 4),
+                     "population_gbs": round(54 * s * N / ms / 1e6, 1),
+                     "gbs_with_index_streams": round((54 * s + 16) * N / ms / 1e6, 1)}
+        print(f"# sparse layout {form}: {out[form]['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        del st, lat
+        torch.cuda.empty_cache()
+    return out
 
 
 def tgv_leg(torch, vfx, n, steps, every=100):
@@ -473,6 +519,8 @@ def main():
         print(f"# ensemble of {e['domains']} cavities: batched {e['batched_mlups']:.0f} MLUPS, per-domain graph "
               f"{e['per_domain_graph_mlups']:.0f}, per-domain eager {e['per_domain_eager_mlups']:.0f}",
               file=sys.stderr, flush=True)
+    if args.sparse_steps > 0:
+        out["sparse_layout"] = sparse_leg(torch, vfx, args.sparse_steps)
     if args.tgv > 0:
         out["taylor_green"] = tgv_leg(torch, vfx, args.tgv, args.tgv_steps)
         torch.cuda.empty_cache()
