@@ -12,7 +12,7 @@ import numpy as np
 import torch
 
 from . import _lib as V
-from .lattice import BYTES, STORAGE, STORAGE_DTYPE, Lattice, _stream
+from .lattice import BYTES, STORAGE, STORAGE_DTYPE, Lattice, _stream, pack_mask
 
 
 def cavity_ensemble(n_domains, seed=7, lmin=16, lmax=96, u_lid=0.05, re=(50.0, 1000.0)):
@@ -159,7 +159,10 @@ class Ensemble:
             self.even = not self.even
 
     def lattice(self, domain) -> Lattice:
-        """A Lattice view over one domain's box (shares the pool memory)."""
+        """A Lattice view over one domain's box (shares the pool memory), for
+        reading populations and macroscopic fields and for the diagnostics.
+        The wall layer is marked solid, so the diagnostics skip it (the batched
+        step leaves values there that belong to no node)."""
         s = V.VfxLattice()
         V.check(V.lib().vfx_batch_lattice(self.handle, int(domain), C.byref(s)), "vfx_batch_lattice")
         lat = Lattice.__new__(Lattice)
@@ -171,12 +174,15 @@ class Ensemble:
         lat.force = (0.0, 0.0, 0.0)
         lat.fp16_scale = float(s.fp16_scale)
         lat.direct = True
-        lat.solid_mask = None
+        nx, ny, nz = self.dims[domain]
+        wall = np.ones((nz, ny, nx), dtype=bool)
+        wall[1:-1, 1:-1, 1:-1] = False
+        lat.solid_mask = torch.from_numpy(pack_mask(wall).view(np.int32)).to(self.device)
         lat.nb = None
         lat.fluid = None
         lat.dims = self.dims[domain]
         lat.n_nodes = int(self.sizes[domain])
-        lat.n_fluid = lat.n_nodes
+        lat.n_fluid = (nx - 2) * (ny - 2) * (nz - 2)
         off = int(self.offsets[domain])
         lat.f = self.pool[off: off + 27 * lat.n_nodes]
         lat.macro = None
