@@ -724,17 +724,31 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
     const T drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] + p[d0MP] +
                    p[d0PP] + p[d0MM] + p[dP0M] + p[dM0P] + p[dP0P] + p[dM0M] + p[dPM0] + p[dMP0] + p[dPP0] + p[dMM0] +
                    p[d00P] + p[d00M] + p[d0P0] + p[d0M0] + p[dP00] + p[dM00] + p[d000];
-    const T vx1 = (((p[dPMP] - p[dMPM]) - (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+    // division by 1 + drho: exact in FP64 (reference order), the hardware
+    // reciprocal in FP32
+    const T ovr = sizeof(T) == 4 ? T(__builtin_amdgcn_rcpf(float(T(1) + drho))) : T(0);
+    auto per_rho = [&](T x) -> T {
+        if constexpr (sizeof(T) == 4) return x * ovr;
+        else return x / (T(1) + drho);
+    };
+    const T vx1 = per_rho((((p[dPMP] - p[dMPM]) - (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
                    ((p[dP0M] - p[dM0P]) + (p[dP0P] - p[dM0M])) + ((p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
-                   (p[dP00] - p[dM00])) / (T(1) + drho);
-    const T vx2 = ((-(p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                   (p[dP00] - p[dM00])));
+    const T vx2 = per_rho(((-(p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
                    ((p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + (-(p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
-                   (p[d0P0] - p[d0M0])) / (T(1) + drho);
-    const T vx3 = (((p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) + (p[dMMP] - p[dPPM])) +
+                   (p[d0P0] - p[d0M0])));
+    const T vx3 = per_rho((((p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) + (p[dMMP] - p[dPPM])) +
                    (-(p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + ((p[dP0P] - p[dM0M]) - (p[dP0M] - p[dM0P])) +
-                   (p[d00P] - p[d00M])) / (T(1) + drho);
+                   (p[d00P] - p[d00M])));
     const T cu_sq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3) * (T(1) + drho);
     const T omega = (T)B.omega;
+    const T inv1mo = sizeof(T) == 4 ? T(1) / (T(1) - omega) : T(0);
+    // all link fractions first: loaded inside the link loop, each q waited for
+    // the previous link's store as well (one memory counter covers both) and
+    // the 26 links ran as 26 round trips
+    T qs[27];
+#pragma unroll
+    for (int d = 1; d < 27; ++d) qs[d] = (T)B.q[(uint64_t)d * B.n + i];
     T VX = 0, VY = 0, VZ = 0;
     if (KIND == VFX_BC_VELOCITY_INTERPOLATED) {
         VX = B.vx ? (T)B.vx[i] : T(0);
@@ -744,16 +758,27 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
     bool fx = false, fy = false, fz = false; // slip: axes normal to a face link
 #pragma unroll
     for (int d = 1; d < 27; ++d) {
-        const T q = (T)B.q[(uint64_t)d * B.n + i];
+        const T q = qs[d];
         if (!(q >= T(0) && q <= T(1))) continue;
         const T w = weight<T>(d);
         const T vlb = proj<T>(d, vx1, vx2, vx3);
         const T feq = w * (drho + T(4.5) * vlb * vlb * (T(1) + drho) - cu_sq);
         const T f = p[d], fi = p[inv(d)];
-        const T a = (T(1) - q) / (T(1) + q) * (f - fi + (f + fi - T(2) * feq * omega) / (T(1) - omega)) * T(0.5);
+        // FP64 keeps the reference's divisions (bit parity with the CPU
+        // restatement); the FP32 arithmetic of the reduced forms uses one
+        // hardware reciprocal per link and 1/(1 - omega) once per node
+        T a, r1q;
+        if constexpr (sizeof(T) == 4) {
+            r1q = __builtin_amdgcn_rcpf(T(1) + q);
+            a = (T(1) - q) * r1q * (f - fi + (f + fi - T(2) * feq * omega) * inv1mo) * T(0.5);
+        } else {
+            r1q = T(0);
+            a = (T(1) - q) / (T(1) + q) * (f - fi + (f + fi - T(2) * feq * omega) / (T(1) - omega)) * T(0.5);
+        }
         T v;
         if (KIND == VFX_BC_NOSLIP_INTERPOLATED) {
-            v = a + (q * (f + fi)) / (T(1) + q);
+            if constexpr (sizeof(T) == 4) v = a + q * (f + fi) * r1q;
+            else v = a + (q * (f + fi)) / (T(1) + q);
         } else {
             T vbc;
             if (KIND == VFX_BC_SLIP) {
@@ -767,7 +792,8 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
             } else {
                 vbc = proj<T>(d, VX, VY, VZ);
             }
-            v = a + (q * (f + fi) - T(6) * w * vbc) / (T(1) + q);
+            if constexpr (sizeof(T) == 4) v = a + (q * (f + fi) - T(6) * w * vbc) * r1q;
+            else v = a + (q * (f + fi) - T(6) * w * vbc) / (T(1) + q);
         }
         const int e = inv(d);
         Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)], L.ss[e]);
