@@ -148,6 +148,19 @@ vfx_status vfx_init_equilibrium(const vfx_lattice* lat, int32_t is_even_timestep
 vfx_status vfx_read_populations(const vfx_lattice* lat, int32_t is_even_timestep, double* out, vfx_stream stream);
 vfx_status vfx_write_populations(const vfx_lattice* lat, int32_t is_even_timestep, const double* in, vfx_stream stream);
 
+/* The distribution array in VirtualFluids' own layout -- 27 planes of n_nodes
+ * shifted populations at parity is_even_timestep, as distributions.f[0] holds
+ * it -- copied to / from a device array of 27*n_nodes values of `precision`
+ * (VFX_FP32 for VirtualFluids' default `real`, VFX_FP64 with
+ * VF_DOUBLE_ACCURACY).  This is the array RestartObject::serialize writes and
+ * RestartObject::deserialize reads back (src/gpu/core/Restart/RestartObject.cpp:
+ * 19-48); export writes NaN as 0 as serialize does.  FP32/FP64 storage is
+ * copied, FP16 storage decoded on export and encoded on import. */
+vfx_status vfx_export_distributions(const vfx_lattice* lat, int32_t is_even_timestep, void* out, int32_t precision,
+                                    vfx_stream stream);
+vfx_status vfx_import_distributions(const vfx_lattice* lat, int32_t is_even_timestep, const void* in,
+                                    int32_t precision, vfx_stream stream);
+
 /* --------------------------------------------------------- diagnostics */
 /* Macroscopic fields of the pre-collision state (calculateMacroscopicQuantitiesCompressible_device,
  * src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237), doubles, n_nodes each. */

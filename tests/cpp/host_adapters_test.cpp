@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 using namespace vfx::host;
@@ -163,6 +164,22 @@ static void gpu_tests()
     double amax = 0;
     for (double v : a) amax = std::fmax(amax, std::fabs(v));
     EXPECT(amax > 1e-4 && amax < 1.0);
+
+    // (c) restart: serialize the level (double, as with VF_DOUBLE_ACCURACY),
+    // clear the array, deserialize -- the distributions come back bit for bit
+    const std::string rs = "host_adapters_restart";
+    BinaryRestartObjectT<double> out;
+    out.serialize(rs, para);
+    EXPECT(out.fs.size() == 1 && out.fs[0].size() == a.size());
+    EXPECT(out.fs.size() == 1 && std::memcmp(out.fs[0].data(), a.data(), a.size() * 8) == 0);
+    std::vector<double> zeros(a.size(), 0.0);
+    check(vfx_copy_to_device(fa, zeros.data(), zeros.size() * 8), "h2d");
+    BinaryRestartObjectT<double> in;
+    in.deserialize(rs, para);
+    std::vector<double> c(a.size());
+    check(vfx_copy_to_host(c.data(), fa, c.size() * 8), "d2h");
+    EXPECT(std::memcmp(a.data(), c.data(), a.size() * 8) == 0);
+    EXPECT(in.delete_restart_file(rs));
     for (void* d : {dnbx, dnby, dnbz, dfl, fa, fb, fields[0], fields[1], fields[2], fields[3]}) vfx_device_free(d);
 }
 
@@ -259,9 +276,53 @@ static void gpu_refinement_tests()
         vfx_device_free(d);
 }
 
+// Restart files: write the reference unit test's data (RestartObjectTests.cpp,
+// saveAndLoad) plus values that exercise the text format, and read back the
+// files the Python writer made for the same data (<prefix>_py.bin/.txt).
+static void restart_tests(const std::string& prefix)
+{
+    const std::vector<std::vector<float>> data = {{1, 2, 3}, {4, 5, 6}, {0.123456789f, -1e-7f, 12345678.f, -0.f}};
+    BinaryRestartObject b;
+    b.fs = data;
+    b.serialize_internal(prefix + "_cpp");
+    ASCIIRestartObject a;
+    a.fs = data;
+    a.serialize_internal(prefix + "_cpp");
+    BinaryRestartObjectT<double> d;
+    d.fs = {{1, 2, 3}, {0.1, -2.5e-300}};
+    d.serialize_internal(prefix + "_cpp64");
+    BinaryRestartObject rb;
+    rb.deserialize_internal(prefix + "_py");
+    EXPECT(rb.fs == data);
+    ASCIIRestartObject ra;
+    ra.deserialize_internal(prefix + "_py");
+    EXPECT(ra.fs.size() == data.size());
+    for (size_t i = 0; i < ra.fs.size() && i < data.size(); ++i) {
+        EXPECT(ra.fs[i].size() == data[i].size());
+        for (size_t j = 0; j < ra.fs[i].size() && j < data[i].size(); ++j)
+            EXPECT(std::fabs(ra.fs[i][j] - data[i][j]) <= 5e-6f * std::fabs(data[i][j])); // six digits
+    }
+    BinaryRestartObjectT<double> rd;
+    rd.deserialize_internal(prefix + "_py64");
+    EXPECT(rd.fs == d.fs);
+    b.serialize_internal(prefix + "_del");
+    EXPECT(b.delete_restart_file(prefix + "_del"));
+    EXPECT(!b.delete_restart_file(prefix + "_does_not_exist"));
+}
+
 int main(int argc, char** argv)
 {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
+    if (argc > 2 && std::strcmp(argv[1], "restart") == 0) {
+        try {
+            restart_tests(argv[2]);
+        } catch (const std::exception& e) {
+            std::printf("exception: %s\n", e.what());
+            return 2;
+        }
+        std::printf("restart file tests: %d failure(s)\n", failures);
+        return failures ? 1 : 0;
+    }
     try {
         if (gpu) {
             gpu_tests();

@@ -67,6 +67,8 @@ _SIGNATURES = {
                              C.c_void_p],
     "vfx_read_populations": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
     "vfx_write_populations": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
+    "vfx_export_distributions": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_int32, C.c_void_p],
+    "vfx_import_distributions": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_int32, C.c_void_p],
     "vfx_macroscopic": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                         C.c_void_p],
     "vfx_reduce_mass_momentum": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p],

@@ -254,6 +254,38 @@ vfx_status vfx_write_populations(const vfx_lattice* lat, int32_t is_even, const 
     return rw(lat, is_even, nullptr, in, stream);
 }
 
+static vfx_status vf_array(const vfx_lattice* lat, int32_t is_even, void* out, const void* in, int32_t precision,
+                           vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    if (precision != VFX_FP32 && precision != VFX_FP64)
+        return fail(VFX_EINVAL, "precision %d: the VirtualFluids array is float or double", precision);
+    hipStream_t st = (hipStream_t)stream;
+    const bool fp64 = precision == VFX_FP64;
+    hipError_t e;
+    switch (lat->storage) {
+    case VFX_FP64: e = vfx::launch_vf_array<double>(L, is_even != 0, fp64, out, in, st); break;
+    case VFX_FP32: e = vfx::launch_vf_array<float>(L, is_even != 0, fp64, out, in, st); break;
+    default: e = vfx::launch_vf_array<_Float16>(L, is_even != 0, fp64, out, in, st); break;
+    }
+    return hip_status(e, "distribution array copy launch");
+}
+
+vfx_status vfx_export_distributions(const vfx_lattice* lat, int32_t is_even, void* out, int32_t precision,
+                                    vfx_stream stream)
+{
+    if (!out) return fail(VFX_EINVAL, "out is NULL");
+    return vf_array(lat, is_even, out, nullptr, precision, stream);
+}
+
+vfx_status vfx_import_distributions(const vfx_lattice* lat, int32_t is_even, const void* in, int32_t precision,
+                                    vfx_stream stream)
+{
+    if (!in) return fail(VFX_EINVAL, "in is NULL");
+    return vf_array(lat, is_even, nullptr, in, precision, stream);
+}
+
 vfx_status vfx_macroscopic(const vfx_lattice* lat, int32_t is_even, double* drho, double* vx, double* vy, double* vz,
                            vfx_stream stream)
 {

@@ -1,6 +1,10 @@
 // vfx_host.cpp -- implementation of the C++ host layer (see vfx_host.hpp).
 #include "vfx_host.hpp"
 
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
 #include <string>
 
 namespace vfx::host {
@@ -219,5 +223,127 @@ void UpdateGrid27::updateGrid(int level, unsigned int timestep)
     preCollisionBC(level);
     if (!finest) refinement(level);
 }
+
+// -------------------------------------------------------------- restart ----
+template <typename Real>
+void RestartObjectT<Real>::serialize(const std::string& filename, const std::shared_ptr<Parameter>& para)
+{
+    fs.clear();
+    constexpr int32_t precision = sizeof(Real) == 8 ? VFX_FP64 : VFX_FP32;
+    for (int level = 0; level <= para->getMaxLevel(); ++level) {
+        LBMSimulationParameter* p = para->getParD(level);
+        const uint64_t n = 27 * p->lattice.n_nodes;
+        void* dev = nullptr;
+        check(vfx_device_alloc(n * sizeof(Real), &dev), "RestartObject::serialize alloc");
+        std::vector<Real> host(n);
+        vfx_status st = vfx_export_distributions(&p->lattice, p->isEvenTimestep ? 1 : 0, dev, precision, p->stream);
+        if (st == VFX_OK) st = vfx_synchronize(p->stream);
+        if (st == VFX_OK) st = vfx_copy_to_host(host.data(), dev, n * sizeof(Real));
+        vfx_device_free(dev);
+        check(st, "RestartObject::serialize");
+        fs.push_back(std::move(host));  // NaN already written as 0 by the export
+    }
+    serialize_internal(filename);
+}
+
+template <typename Real>
+void RestartObjectT<Real>::deserialize(const std::string& filename, std::shared_ptr<Parameter>& para)
+{
+    fs.clear();
+    deserialize_internal(filename);
+    if ((int)fs.size() != para->getMaxLevel() + 1)
+        throw std::runtime_error("restart file " + filename + " holds " + std::to_string(fs.size()) + " levels, the parameter " +
+                                 std::to_string(para->getMaxLevel() + 1));
+    constexpr int32_t precision = sizeof(Real) == 8 ? VFX_FP64 : VFX_FP32;
+    for (int level = 0; level <= para->getMaxLevel(); ++level) {
+        LBMSimulationParameter* p = para->getParD(level);
+        const uint64_t n = 27 * p->lattice.n_nodes;
+        if (fs[level].size() != n)
+            throw std::runtime_error("restart file " + filename + ": level " + std::to_string(level) + " holds " +
+                                     std::to_string(fs[level].size()) + " values, the lattice needs " + std::to_string(n));
+        void* dev = nullptr;
+        check(vfx_device_alloc(n * sizeof(Real), &dev), "RestartObject::deserialize alloc");
+        vfx_status st = vfx_copy_to_device(dev, fs[level].data(), n * sizeof(Real));
+        if (st == VFX_OK)
+            st = vfx_import_distributions(&p->lattice, p->isEvenTimestep ? 1 : 0, dev, precision, p->stream);
+        if (st == VFX_OK) st = vfx_synchronize(p->stream);
+        vfx_device_free(dev);
+        check(st, "RestartObject::deserialize");
+    }
+}
+
+template <typename Real> bool RestartObjectT<Real>::delete_restart_file(const std::string& filename)
+{
+    const std::string file = filename + getFileExtension();
+    if (std::remove(file.c_str()) != 0) {
+        std::fprintf(stdout, "[warning] Could not delete the restart object file \"%s\".\n", file.c_str());
+        return false;
+    }
+    return true;
+}
+
+// one line per level, each value followed by a space, default stream format
+template <typename Real> void ASCIIRestartObjectT<Real>::serialize_internal(const std::string& filename)
+{
+    std::ofstream stream(filename + getFileExtension());
+    if (!stream) throw std::runtime_error("couldn't open file " + filename);
+    for (const auto& level : this->fs) {
+        for (Real v : level) stream << v << " ";
+        stream << "\n";
+    }
+}
+
+template <typename Real> void ASCIIRestartObjectT<Real>::deserialize_internal(const std::string& filename)
+{
+    std::ifstream stream(filename + getFileExtension(), std::ios_base::in);
+    if (!stream.is_open()) throw std::runtime_error("couldn't open check point file " + filename);
+    std::string line;
+    while (std::getline(stream, line)) {
+        std::vector<Real> values;
+        std::stringstream ls(line);
+        Real v;
+        while (ls >> v) values.push_back(v);
+        this->fs.push_back(std::move(values));
+    }
+}
+
+// int32 level count, then per level an int32 count and the raw values
+template <typename Real> void BinaryRestartObjectT<Real>::serialize_internal(const std::string& filename)
+{
+    std::ofstream stream(filename + getFileExtension(), std::ios_base::binary);
+    if (!stream) throw std::runtime_error("couldn't open file " + filename);
+    const int32_t levels = (int32_t)this->fs.size();
+    stream.write(reinterpret_cast<const char*>(&levels), sizeof(levels));
+    for (const auto& v : this->fs) {
+        const int32_t size = (int32_t)v.size();
+        stream.write(reinterpret_cast<const char*>(&size), sizeof(size));
+        if (size) stream.write(reinterpret_cast<const char*>(v.data()), (std::streamsize)(v.size() * sizeof(Real)));
+    }
+}
+
+template <typename Real> void BinaryRestartObjectT<Real>::deserialize_internal(const std::string& filename)
+{
+    std::ifstream stream(filename + getFileExtension(), std::ios_base::in | std::ios_base::binary);
+    if (!stream.is_open()) throw std::runtime_error("couldn't open check point file " + filename);
+    int32_t levels = 0;
+    stream.read(reinterpret_cast<char*>(&levels), sizeof(levels));
+    if (!stream || levels < 0) throw std::runtime_error("check point file " + filename + " is not a restart object");
+    this->fs.assign((size_t)levels, {});
+    for (int32_t n = 0; n < levels; ++n) {
+        int32_t size = 0;
+        stream.read(reinterpret_cast<char*>(&size), sizeof(size));
+        if (!stream || size < 0) throw std::runtime_error("check point file " + filename + " is truncated");
+        this->fs[n].resize((size_t)size);
+        if (size) stream.read(reinterpret_cast<char*>(this->fs[n].data()), (std::streamsize)(size * sizeof(Real)));
+        if (!stream) throw std::runtime_error("check point file " + filename + " is truncated");
+    }
+}
+
+template class RestartObjectT<float>;
+template class RestartObjectT<double>;
+template class ASCIIRestartObjectT<float>;
+template class ASCIIRestartObjectT<double>;
+template class BinaryRestartObjectT<float>;
+template class BinaryRestartObjectT<double>;
 
 } // namespace vfx::host

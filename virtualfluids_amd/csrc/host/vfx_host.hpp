@@ -217,4 +217,51 @@ private:
     gridScaling scalingFineToCoarse, scalingCoarseToFine;
 };
 
+// -------------------------------------------------------------- restart ----
+// RestartObject / ASCIIRestartObject / BinaryRestartObject
+// (src/gpu/core/Restart/RestartObject.h:40-80, RestartObject.cpp): one
+// distribution array per level, coarse to fine, in VirtualFluids' layout at
+// each level's current parity.  Real is VirtualFluids' `real` (float by
+// default, double with VF_DOUBLE_ACCURACY): it fixes the file's value type.
+template <typename Real> class RestartObjectT {
+public:
+    virtual ~RestartObjectT() = default;
+
+    void deserialize(const std::string& filename, std::shared_ptr<Parameter>& para);
+    void serialize(const std::string& filename, const std::shared_ptr<Parameter>& para);
+
+    std::vector<std::vector<Real>> fs;
+
+    virtual void serialize_internal(const std::string& filename) = 0;
+    virtual void deserialize_internal(const std::string& filename) = 0;
+
+    // returns false (and prints a warning) if there was no file to delete
+    bool delete_restart_file(const std::string& filename);
+
+protected:
+    virtual std::string getFileExtension() const = 0;
+};
+
+template <typename Real> class ASCIIRestartObjectT : public RestartObjectT<Real> {
+public:
+    void serialize_internal(const std::string& filename) override;
+    void deserialize_internal(const std::string& filename) override;
+
+protected:
+    std::string getFileExtension() const override { return ".txt"; }
+};
+
+template <typename Real> class BinaryRestartObjectT : public RestartObjectT<Real> {
+public:
+    void serialize_internal(const std::string& filename) override;
+    void deserialize_internal(const std::string& filename) override;
+
+protected:
+    std::string getFileExtension() const override { return ".bin"; }
+};
+
+using RestartObject = RestartObjectT<float>;
+using ASCIIRestartObject = ASCIIRestartObjectT<float>;
+using BinaryRestartObject = BinaryRestartObjectT<float>;
+
 } // namespace vfx::host

@@ -1031,6 +1031,43 @@ __global__ void __launch_bounds__(256) k_batch_zero(void* pool, uint64_t n)
 
 namespace vfx {
 
+// ---------------------------------------------- VirtualFluids array export ----
+// The distribution array exactly as VirtualFluids holds it (distributions.f[0]:
+// 27 planes of n_nodes shifted populations at the current parity) in a chosen
+// floating type: the array RestartObject::serialize/deserialize write and read
+// (src/gpu/core/Restart/RestartObject.cpp:19-48).  Element j of plane p holds
+// direction d = slot^-1(p) (p itself at even parity, inv(p) at odd), which
+// fixes the FP16 scale and reference offset; FP32/FP64 storage is copied.
+// Export writes NaN as 0, as serialize does.
+template <typename S, bool EVEN, typename O>
+__global__ void __launch_bounds__(256) k_export(Lat L, O* out)
+{
+    using T = typename Compute<S>::T;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t n = 27ull * L.N;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int p = (int)(i / L.N);
+        const uint32_t j = (uint32_t)(i - (uint64_t)p * L.N);
+        const int d = EVEN ? p : inv(p);
+        T v = Storage<S, T>::ld(plane<S>(L, p), j, L.ld[weight_class(d)], L.sh[d]);
+        if (v != v) v = T(0);
+        out[i] = (O)v;
+    }
+}
+template <typename S, bool EVEN, typename O>
+__global__ void __launch_bounds__(256) k_import(Lat L, const O* in)
+{
+    using T = typename Compute<S>::T;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t n = 27ull * L.N;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int p = (int)(i / L.N);
+        const uint32_t j = (uint32_t)(i - (uint64_t)p * L.N);
+        const int d = EVEN ? p : inv(p);
+        Storage<S, T>::st(plane<S>(L, p), j, (T)in[i], L.st[weight_class(d)], L.ss[d]);
+    }
+}
+
 // ------------------------------------------------------------ launchers ----
 constexpr int kBlock = 256;
 
@@ -1090,6 +1127,33 @@ template <typename S> hipError_t launch_rw(const Lat& L, int direct, bool even, 
     } else {
         if (even) hipLaunchKernelGGL((k_write<S, true>), g, b, 0, st, L, direct, in);
         else hipLaunchKernelGGL((k_write<S, false>), g, b, 0, st, L, direct, in);
+    }
+    return hipGetLastError();
+}
+
+// out_fp64: the VirtualFluids array in double (else float); exactly one of
+// out / in is non-null
+template <typename S>
+hipError_t launch_vf_array(const Lat& L, bool even, bool fp64, void* out, const void* in, hipStream_t st)
+{
+    const uint64_t n = 27ull * L.N;
+    const unsigned g = (unsigned)std::min<uint64_t>((n + kBlock - 1) / kBlock, 65536), b = kBlock;
+    if (out) {
+        if (fp64) {
+            if (even) hipLaunchKernelGGL((k_export<S, true, double>), g, b, 0, st, L, (double*)out);
+            else hipLaunchKernelGGL((k_export<S, false, double>), g, b, 0, st, L, (double*)out);
+        } else {
+            if (even) hipLaunchKernelGGL((k_export<S, true, float>), g, b, 0, st, L, (float*)out);
+            else hipLaunchKernelGGL((k_export<S, false, float>), g, b, 0, st, L, (float*)out);
+        }
+    } else {
+        if (fp64) {
+            if (even) hipLaunchKernelGGL((k_import<S, true, double>), g, b, 0, st, L, (const double*)in);
+            else hipLaunchKernelGGL((k_import<S, false, double>), g, b, 0, st, L, (const double*)in);
+        } else {
+            if (even) hipLaunchKernelGGL((k_import<S, true, float>), g, b, 0, st, L, (const float*)in);
+            else hipLaunchKernelGGL((k_import<S, false, float>), g, b, 0, st, L, (const float*)in);
+        }
     }
     return hipGetLastError();
 }
@@ -1188,6 +1252,7 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,             \
                                        const double*, hipStream_t);                                                    \
     template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);                      \
+    template hipError_t launch_vf_array<S>(const Lat&, bool, bool, void*, const void*, hipStream_t);                  \
     template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t);       \
     template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);                            \
     template hipError_t launch_reduce_mass<S>(const Lat&, int, bool, double*, double*, hipStream_t);              \
@@ -1203,6 +1268,7 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     extern template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,      \
                                               const double*, hipStream_t);                                             \
     extern template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);               \
+    extern template hipError_t launch_vf_array<S>(const Lat&, bool, bool, void*, const void*, hipStream_t);           \
     extern template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t); \
     extern template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);                     \
     extern template hipError_t launch_reduce_mass<S>(const Lat&, int, bool, double*, double*, hipStream_t);       \

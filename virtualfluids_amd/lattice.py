@@ -178,6 +178,33 @@ class Lattice:
         V.check(V.lib().vfx_write_populations(C.byref(s), int(bool(even)), F.data_ptr(), _stream(stream)),
                 "vfx_write_populations")
 
+    def export_distributions(self, even=True, dtype=torch.float32, stream=None) -> torch.Tensor:
+        """The distribution array as VirtualFluids holds it (27 planes of
+        n_nodes shifted populations at this parity) in float32 (VirtualFluids'
+        default `real`) or float64, on the device."""
+        if dtype not in (torch.float32, torch.float64):
+            raise ValueError("VirtualFluids' real is float32 or float64")
+        out = torch.empty(27 * self.n_nodes, dtype=dtype, device=self.device)
+        s = self.struct()
+        V.check(V.lib().vfx_export_distributions(C.byref(s), int(bool(even)), out.data_ptr(),
+                                                 STORAGE["fp64" if dtype == torch.float64 else "fp32"], _stream(stream)),
+                "vfx_export_distributions")
+        return out
+
+    def import_distributions(self, f, even=True, stream=None):
+        """Load a VirtualFluids-layout distribution array (float32 or float64,
+        27 * n_nodes values) into this lattice at the given parity."""
+        f = torch.as_tensor(f)
+        if f.dtype not in (torch.float32, torch.float64):
+            raise ValueError("VirtualFluids' real is float32 or float64")
+        if f.numel() != 27 * self.n_nodes:
+            raise ValueError(f"expected {27 * self.n_nodes} values, got {f.numel()}")
+        f = f.to(self.device).contiguous().view(-1)
+        s = self.struct()
+        V.check(V.lib().vfx_import_distributions(C.byref(s), int(bool(even)), f.data_ptr(),
+                                                 STORAGE["fp64" if f.dtype == torch.float64 else "fp32"], _stream(stream)),
+                "vfx_import_distributions")
+
     def macroscopic(self, even=True, stream=None):
         outs = [torch.zeros(self.n_nodes, dtype=torch.float64, device=self.device) for _ in range(4)]
         s = self.struct()
