@@ -91,6 +91,15 @@ typedef struct vfx_lattice {
      * keeps the stored deviations small, so FP16 rounds them less.  {0,0,0}:
      * plain shifted populations.  Must be zero for FP32/FP64 storage. */
     double fp16_reference_velocity[3];
+
+    /* Sparse addressing only: VirtualFluids' typeOfGridNode (device, n_nodes
+     * entries).  When set, every node whose type is neither GEO_SOLID (15) nor
+     * GEO_VOID (16) collides (and is initialised, read and reduced), as
+     * B92CompressibleNavierStokes_Device selects them
+     * (src/gpu/core/Kernel/Compressible/NavierStokes/B92/B92CompressibleNavierStokes_Device.cu:58-66,
+     * constants src/gpu/core/Calculation/Calculation.h:43-45); fluid_nodes is
+     * then not used.  NULL: the fluid_nodes list selects the nodes. */
+    const uint32_t* type_of_grid_node;
 } vfx_lattice;
 
 /* boundary-condition kinds (src/gpu/core/BoundaryConditions/BoundaryConditionFactory.h:60-132) */

@@ -102,6 +102,37 @@ def test_sparse_layout_matches_oracle(cuda):
     assert rel_err(got, ref) <= REL
 
 
+@pytest.mark.parametrize("collision", ["bgk", "k17"])
+def test_sparse_node_types_select_like_fluid_list(cuda, collision):
+    """VirtualFluids' typeOfGridNode instead of the fluid-node list (the way
+    B92CompressibleNavierStokes_Device selects its nodes): every node that is
+    neither GEO_SOLID (15) nor GEO_VOID (16) collides -- fluid (19) and
+    boundary-tagged nodes alike -- and the result equals the fluid-list run and
+    the oracle (periodic box, permuted order, dummy node 0 typed solid)."""
+    dims = (12, 10, 8)
+    omega = 1.62 if collision == "k17" else 1.3
+    nbx, nby, nbz, fluid, sidx = cases.sparse_layout(dims, seed=5)
+    types = np.full(sidx.size + 1, 19, np.uint32)
+    types[0] = 15
+    rng = np.random.default_rng(2)
+    types[rng.choice(np.arange(1, sidx.size + 1), 40, replace=False)] = 7  # a non-fluid, non-solid tag
+    fields = cases.random_flow(dims, seed=7)
+    F0 = oracle.init_field(*fields)
+    Fs = np.zeros((sidx.size + 1, 27))
+    Fs[sidx] = F0
+    a = vfx.Lattice(storage="fp64", omega=omega, neighbors=(nbx, nby, nbz), fluid_nodes=fluid)
+    b = vfx.Lattice(storage="fp64", omega=omega, neighbors=(nbx, nby, nbz), node_types=types)
+    assert b.n_fluid == sidx.size
+    for lat in (a, b):
+        lat.write_populations(Fs, even=True)
+        vfx.Stepper(lat, collision).step(9)
+    torch.cuda.synchronize()
+    assert torch.equal(a.f, b.f)
+    got = b.read_populations(False).cpu().numpy()[sidx]
+    ref = oracle.run(F0, dims, 9, omega, collision=collision)
+    assert rel_err(got, ref) <= REL
+
+
 def _cavity_lattice(n, omega, storage="fp64", cpu_float=True):
     solid, groups = cases.cavity(n)
     m = n + 2

@@ -51,6 +51,13 @@ def test_invalid_arguments_fail_without_device():
     s.fp16_reference_velocity[:] = [0.05, 0.0, 0.0]  # FP16 storage only
     assert L.vfx_collide_stream(C.byref(s), 0, 1, 0, None) == vfx._lib.VFX_EINVAL
     assert b"fp16_reference_velocity" in L.vfx_last_error()
+    s.fp16_reference_velocity[:] = [0.0, 0.0, 0.0]
+    s.type_of_grid_node = 1  # typeOfGridNode belongs to the sparse layout
+    assert L.vfx_collide_stream(C.byref(s), 0, 1, 0, None) == vfx._lib.VFX_EINVAL
+    assert b"type_of_grid_node" in L.vfx_last_error()
+    s.type_of_grid_node = None
+    assert L.vfx_export_distributions(C.byref(s), 1, 1, 9, None) == vfx._lib.VFX_EINVAL  # float or double only
+    assert L.vfx_export_distributions(C.byref(s), 1, None, 1, None) == vfx._lib.VFX_EINVAL
     assert L.vfx_version() >= 1
 
 

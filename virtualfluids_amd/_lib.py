@@ -31,6 +31,7 @@ class VfxLattice(C.Structure):
         ("omega", C.c_double), ("quadric_limiter", C.c_double * 3), ("force", C.c_double * 3),
         ("rho", C.c_void_p), ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
         ("kernel_variant", C.c_int32), ("fp16_reference_velocity", C.c_double * 3),
+        ("type_of_grid_node", C.c_void_p),
     ]
 
 

@@ -78,13 +78,16 @@ vfx_status make_lat(const vfx_lattice* in, vfx::Lat& L)
     } else {
         if (!in->neighbor_x || !in->neighbor_y || !in->neighbor_z)
             return fail(VFX_EINVAL, "sparse addressing needs neighbor_x/y/z");
-        if (in->n_fluid && !in->fluid_nodes) return fail(VFX_EINVAL, "fluid_nodes is NULL");
+        if (in->n_fluid && !in->fluid_nodes && !in->type_of_grid_node) return fail(VFX_EINVAL, "fluid_nodes is NULL");
         L.nbx = in->neighbor_x;
         L.nby = in->neighbor_y;
         L.nbz = in->neighbor_z;
         L.fluid = in->fluid_nodes;
         L.nfluid = in->n_fluid;
+        L.types = in->type_of_grid_node;
     }
+    if (in->direct && in->type_of_grid_node)
+        return fail(VFX_EINVAL, "type_of_grid_node selects nodes of the sparse layout; direct addressing uses solid_mask");
     if (in->storage == VFX_FP16_SCALED) {
         if (!(in->fp16_scale > 0.f)) return fail(VFX_EINVAL, "fp16_scale must be > 0");
         for (int c = 0; c < 4; ++c) {

@@ -100,6 +100,7 @@ struct Lat {
     const uint32_t* nbz;
     const uint32_t* fluid;
     uint32_t nfluid;
+    const uint32_t* types; // sparse: typeOfGridNode (nodes not GEO_SOLID/GEO_VOID collide), or NULL
     float ld[4], st[4];
     float sh[27], ss[27]; // FP16 reference-velocity offsets per direction (0 otherwise)
     void* rho;
@@ -114,6 +115,29 @@ template <typename S> __device__ __forceinline__ S* plane(const Lat& L, int p) {
 __device__ __forceinline__ bool is_solid(const Lat& L, uint32_t k)
 {
     return L.solid != nullptr && ((L.solid[k >> 5] >> (k & 31u)) & 1u);
+}
+
+// VirtualFluids node types (src/gpu/core/Calculation/Calculation.h:43-45)
+constexpr uint32_t kGeoSolid = 15, kGeoVoid = 16;
+
+// sparse layout: the node of work item i (the i-th fluid node, or node i when
+// typeOfGridNode selects), false if it does not collide
+__device__ __forceinline__ bool sparse_node(const Lat& L, uint64_t i, uint32_t& k)
+{
+    if (L.types) {
+        if (i >= L.N) return false;
+        k = (uint32_t)i;
+        const uint32_t t = L.types[k];
+        return t != kGeoSolid && t != kGeoVoid;
+    }
+    if (i >= L.nfluid) return false;
+    k = L.fluid[i];
+    return k < L.N;
+}
+// work items of a node loop
+__host__ __device__ inline uint64_t node_items(const Lat& L, int direct)
+{
+    return direct || L.types ? L.N : L.nfluid;
 }
 
 // The eight nodes a node's populations live on under the esoteric twist:
@@ -296,10 +320,8 @@ __global__ void __launch_bounds__(256) k_step_sparse(Lat L, Rates<typename Compu
 {
     using T = typename Compute<S>::T;
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x, xcd_run<S>()) * blockDim.x + threadIdx.x;
-    if (i >= L.nfluid) return;
-    uint32_t kk[8];
-    const uint32_t k = L.fluid[i];
-    if (k >= L.N) return;
+    uint32_t kk[8], k;
+    if (!sparse_node(L, i, k)) return;
     nbrs_sparse(L, k, kk);
     T m[27], mv[4];
     constexpr bool O2 = FAST && COLL == VFX_K17 && sizeof(S) == 2;
@@ -321,8 +343,9 @@ struct NodeIter {
             if (is_solid(L, k)) return false;
             nbrs_direct(L, k, kk);
         } else {
-            if (i >= L.nfluid) return false;
-            nbrs_sparse(L, L.fluid[i], kk);
+            uint32_t k;
+            if (!sparse_node(L, i, k)) return false;
+            nbrs_sparse(L, k, kk);
         }
         return true;
     }
@@ -1082,9 +1105,9 @@ hipError_t launch_step_t(const Lat& L, int direct, const Rates<typename Compute<
         const unsigned b = block <= 64 ? 64 : block <= 128 ? 128 : 256;
         hipLaunchKernelGGL((k_step_direct<S, COLL, EVEN, FAST>), dim3(grid), dim3(b), 0, st, L, R);
     } else {
-        if (L.nfluid == 0) return hipSuccess;
-        hipLaunchKernelGGL((k_step_sparse<S, COLL, EVEN, FAST>), dim3(grid_for(L.nfluid, kBlock)), dim3(kBlock), 0, st,
-                           L, R);
+        const uint64_t n = node_items(L, 0);
+        if (n == 0) return hipSuccess;
+        hipLaunchKernelGGL((k_step_sparse<S, COLL, EVEN, FAST>), dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, L, R);
     }
     return hipGetLastError();
 }
@@ -1109,7 +1132,7 @@ template <typename S> hipError_t launch_step(const Lat& L, int direct, int coll,
 template <typename S> hipError_t launch_init(const Lat& L, int direct, bool even, const double* drho, const double* ux,
                                              const double* uy, const double* uz, hipStream_t st)
 {
-    const uint64_t n = direct ? L.N : L.nfluid;
+    const uint64_t n = node_items(L, direct);
     if (n == 0) return hipSuccess;
     if (even) hipLaunchKernelGGL((k_init<S, true>), dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, L, direct, drho, ux, uy, uz);
     else hipLaunchKernelGGL((k_init<S, false>), dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, L, direct, drho, ux, uy, uz);
@@ -1118,7 +1141,7 @@ template <typename S> hipError_t launch_init(const Lat& L, int direct, bool even
 
 template <typename S> hipError_t launch_rw(const Lat& L, int direct, bool even, double* out, const double* in, hipStream_t st)
 {
-    const uint64_t n = direct ? L.N : L.nfluid;
+    const uint64_t n = node_items(L, direct);
     if (n == 0) return hipSuccess;
     const dim3 g(grid_for(n, kBlock)), b(kBlock);
     if (out) {
@@ -1161,7 +1184,7 @@ hipError_t launch_vf_array(const Lat& L, bool even, bool fp64, void* out, const 
 template <typename S> hipError_t launch_macro(const Lat& L, int direct, bool even, double* r, double* vx, double* vy,
                                               double* vz, hipStream_t st)
 {
-    const uint64_t n = direct ? L.N : L.nfluid;
+    const uint64_t n = node_items(L, direct);
     if (n == 0) return hipSuccess;
     const dim3 g(grid_for(n, kBlock)), b(kBlock);
     if (even) hipLaunchKernelGGL((k_macro<S, true>), g, b, 0, st, L, direct, r, vx, vy, vz);
@@ -1197,7 +1220,7 @@ inline uint32_t reduce_blocks(uint64_t n) { return (uint32_t)((n + kRedChunk - 1
 
 template <typename S> hipError_t launch_reduce_mass(const Lat& L, int direct, bool even, double* ws, double* out, hipStream_t st)
 {
-    const uint64_t n = direct ? L.N : L.nfluid;
+    const uint64_t n = node_items(L, direct);
     const uint32_t nb = reduce_blocks(n ? n : 1);
     if (even) hipLaunchKernelGGL((k_reduce_mass<S, true>), dim3(nb), dim3(kRedBlock), 0, st, L, direct, ws);
     else hipLaunchKernelGGL((k_reduce_mass<S, false>), dim3(nb), dim3(kRedBlock), 0, st, L, direct, ws);
@@ -1207,7 +1230,7 @@ template <typename S> hipError_t launch_reduce_mass(const Lat& L, int direct, bo
 
 template <typename S> hipError_t launch_stats(const Lat& L, int direct, bool even, double* sums, hipStream_t st)
 {
-    const uint64_t n = direct ? L.N : L.nfluid;
+    const uint64_t n = node_items(L, direct);
     if (n == 0) return hipSuccess;
     const dim3 g(grid_for(n, kBlock)), b(kBlock);
     if (even) hipLaunchKernelGGL((k_stats<S, true>), g, b, 0, st, L, direct, sums);

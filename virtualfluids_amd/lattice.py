@@ -58,7 +58,8 @@ class Lattice:
 
     def __init__(self, dims=None, *, storage="fp64", omega=1.0, limiter=(0.01, 0.01, 0.01),
                  force=(0.0, 0.0, 0.0), solid=None, fp16_scale=None, neighbors=None, fluid_nodes=None,
-                 n_nodes=None, device="cuda", variant="default", reference_velocity=(0.0, 0.0, 0.0)):
+                 n_nodes=None, device="cuda", variant="default", reference_velocity=(0.0, 0.0, 0.0),
+                 node_types=None):
         if storage not in STORAGE:
             raise ValueError(f"storage must be one of {sorted(STORAGE)}")
         self.storage = storage
@@ -77,7 +78,10 @@ class Lattice:
         self.solid_mask = None
         self.nb = None
         self.fluid = None
+        self.types = None
         if self.direct:
+            if node_types is not None:
+                raise ValueError("node_types (typeOfGridNode) belongs to the sparse layout; use solid")
             if dims is None:
                 raise ValueError("structured lattice needs dims")
             self.dims = tuple(int(d) for d in dims)
@@ -98,12 +102,21 @@ class Lattice:
                 raise ValueError("neighbour arrays must have n_nodes entries")
             if max(nbx.max(), nby.max(), nbz.max()) >= self.n_nodes:
                 raise ValueError("neighbour index out of range")
-            fl = np.ascontiguousarray(fluid_nodes, dtype=np.uint32)
-            if fl.size and fl.max() >= self.n_nodes:
-                raise ValueError("fluid node index out of range")
             self.nb = [torch.from_numpy(a.view(np.int32)).to(self.device) for a in (nbx, nby, nbz)]
-            self.fluid = torch.from_numpy(fl.view(np.int32)).to(self.device)
-            self.n_fluid = int(fl.size)
+            if node_types is not None:
+                # VirtualFluids' typeOfGridNode: every node that is not
+                # GEO_SOLID (15) or GEO_VOID (16) collides
+                ty = np.ascontiguousarray(node_types, dtype=np.uint32).ravel()
+                if ty.size != self.n_nodes:
+                    raise ValueError("node_types needs n_nodes entries")
+                self.types = torch.from_numpy(ty.view(np.int32)).to(self.device)
+                self.n_fluid = int(((ty != 15) & (ty != 16)).sum())
+            else:
+                fl = np.ascontiguousarray(fluid_nodes, dtype=np.uint32)
+                if fl.size and fl.max() >= self.n_nodes:
+                    raise ValueError("fluid node index out of range")
+                self.fluid = torch.from_numpy(fl.view(np.int32)).to(self.device)
+                self.n_fluid = int(fl.size)
         self.f = torch.zeros(27 * self.n_nodes, dtype=STORAGE_DTYPE[storage], device=self.device)
         self.macro = None
         self._struct = None
@@ -120,8 +133,11 @@ class Lattice:
         s.solid_mask = _ptr(self.solid_mask)
         if self.nb is not None:
             s.neighbor_x, s.neighbor_y, s.neighbor_z = (t.data_ptr() for t in self.nb)
-            s.fluid_nodes = _ptr(self.fluid) if self.n_fluid else None
-            s.n_fluid = self.n_fluid
+            if self.types is not None:
+                s.type_of_grid_node = self.types.data_ptr()
+            else:
+                s.fluid_nodes = _ptr(self.fluid) if self.n_fluid else None
+                s.n_fluid = self.n_fluid
         s.omega = self.omega
         s.quadric_limiter[:] = list(self.limiter)
         s.force[:] = list(self.force)
