@@ -100,7 +100,35 @@ typedef struct vfx_lattice {
      * constants src/gpu/core/Calculation/Calculation.h:43-45); fluid_nodes is
      * then not used.  NULL: the fluid_nodes list selects the nodes. */
     const uint32_t* type_of_grid_node;
+
+    /* Eddy-viscosity model of the K17 collision (vf::lbm::TurbulenceModel,
+     * src/lbm/collision/TurbulentViscosity.h:50-61; run by runCollision,
+     * src/gpu/core/Utilities/RunCollision.cuh:116-141).  Anything but
+     * VFX_TURB_NONE: node k relaxes with omega / (1 + 3 omega nu_t[k])
+     * (calculateOmegaWithturbulentViscosity, TurbulentViscosity.h:86-89);
+     * VFX_TURB_SMAGORINSKY / VFX_TURB_QR store the model's nu_t of the new
+     * velocity gradients back into turbulent_viscosity
+     * (K17CompressibleNavierStokes.h:355-368); VFX_TURB_AMD takes nu_t from
+     * vfx_turbulent_viscosity_amd and makes the collision write rho/vx/vy/vz,
+     * which must then be set.  The reference-order K17 kernel runs (any
+     * kernel_variant).  BGK ignores the model. */
+    int32_t turbulence_model;     /* vfx_turbulence_model */
+    double sgs_constant;          /* Parameter::SGSConstant (Smagorinsky Cs, QR / AMD constant) */
+    void* turbulent_viscosity;    /* n_nodes values, double for VFX_FP64 else float (Parameter turbViscosity) */
+    /* Sparse addressing with VFX_TURB_AMD: the node in -x-y-z
+     * (LBMSimulationParameter::neighborInverse, src/gpu/GridGenerator/grid/GridImp.cpp:1034, 1112-1130); the
+     * AMD kernel then takes GEO_FLUID (19) from type_of_grid_node, which must
+     * be set.  Direct addressing: neighbours implicit, non-solid = fluid. */
+    const uint32_t* neighbor_inverse;
 } vfx_lattice;
+
+/* vf::lbm::TurbulenceModel */
+typedef enum {
+    VFX_TURB_NONE = 0,
+    VFX_TURB_SMAGORINSKY = 1,
+    VFX_TURB_AMD = 2, /* Rozema et al., Phys. Fluids 27, 085107 (2015) */
+    VFX_TURB_QR = 3   /* Verstappen's QR model */
+} vfx_turbulence_model;
 
 /* boundary-condition kinds (src/gpu/core/BoundaryConditions/BoundaryConditionFactory.h:60-132) */
 typedef enum {
@@ -137,6 +165,14 @@ vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t
                               int32_t collision_template, vfx_stream stream);
 vfx_status vfx_k17_step(const vfx_lattice* lat, int32_t is_even_timestep, int32_t collision_template, vfx_stream stream);
 vfx_status vfx_bgk_step(const vfx_lattice* lat, int32_t is_even_timestep, vfx_stream stream);
+
+/* calcTurbulentViscosityAMD (src/gpu/core/PostProcessor/TurbulentViscosityKernels.cu:62-130):
+ * the AMD eddy viscosity of every fluid node from vx/vy/vz into
+ * turbulent_viscosity, with constant sgs_constant.  UpdateGrid27 runs it after
+ * the parity flip and before the pre-collision boundary conditions
+ * (src/gpu/core/Calculation/UpdateGrid27.cpp:77-85, TurbulenceModelFactory).
+ * Sparse layout: needs type_of_grid_node and neighbor_inverse. */
+vfx_status vfx_turbulent_viscosity_amd(const vfx_lattice* lat, vfx_stream stream);
 
 /* Boundary condition on a BC node list.  Post-collision kinds run after the
  * collision of parity is_even_timestep and before the parity flip
@@ -230,13 +266,17 @@ typedef struct {
  * populations at the coarse parity -> the 8 fine nodes of every cell,
  * pre-collision at the fine parity (the reference passes the coarse level's
  * isEvenTimestep and writes the fine level as even).  Both lattices share one
- * storage form; omega of each lattice is its level's rate. */
+ * storage form; omega of each lattice is its level's rate.  With a turbulence
+ * model on both levels (hasTurbulentViscosity, GridScalingFactory) every source
+ * corner and every destination node takes omega / (1 + 3 omega nu_t) with nu_t
+ * of its own node and level (scaleCoarseToFineCompressible.cu:94, 369). */
 vfx_status vfx_scale_cf(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells,
                         int32_t coarse_is_even, int32_t fine_is_even, vfx_stream stream);
 /* scaleFineToCoarseCompressible (Scaling.cu:155-188, kernel
  * scaleFineToCoarseCompressible.cu:76-124): fine pre-collision populations at
  * the fine parity (the reference reads the fine level as even) -> the coarse
- * node of every cell, pre-collision at the coarse parity. */
+ * node of every cell, pre-collision at the coarse parity.  Turbulence models:
+ * per-node omega as in vfx_scale_cf (scaleFineToCoarseCompressible.cu:64). */
 vfx_status vfx_scale_fc(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells,
                         int32_t coarse_is_even, int32_t fine_is_even, vfx_stream stream);
 

@@ -64,6 +64,11 @@ def lib():
         L.orc_run.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _u8p, C.c_int, C.c_double, _dp, _dp,
                               C.c_int, C.POINTER(OrcBC), C.c_int, _dp]
         L.orc_run.restype = C.c_int
+        L.orc_run_turb.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _u8p, C.c_double, _dp, _dp,
+                                   C.c_int, C.POINTER(OrcBC), C.c_int, _dp, C.c_int, C.c_double, _dp]
+        L.orc_run_turb.restype = C.c_int
+        L.orc_k17_collide_turb.argtypes = [_dp, C.c_double, _dp, _dp, _dp, C.c_int, C.c_double, _dp]
+        L.orc_turbulent_viscosity_amd.argtypes = [C.c_int, C.c_int, C.c_int, _u8p, _dp, _dp, _dp, C.c_double, _dp]
         L.orc_num_threads.restype = C.c_int
         L.orc_mass_momentum.argtypes = [_dp, C.c_int64, _u8p, _dp]
         L.orc_energy.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _u8p, C.c_int, _dp]
@@ -75,6 +80,10 @@ def lib():
         _ip = C.POINTER(C.c_int)
         for fn in (L.orc_scale_cf, L.orc_scale_fc):
             fn.argtypes = [_dp, _dp, _ip, _ip, C.c_int64, _i64p, _i64p, _dp, _dp, _dp, C.c_double, C.c_double]
+        for fn in (L.orc_scale_cf_turb, L.orc_scale_fc_turb):
+            fn.argtypes = [_dp, _dp, _ip, _ip, C.c_int64, _i64p, _i64p, _dp, _dp, _dp, C.c_double, C.c_double,
+                           _dp, _dp]
+        L.orc_interp_coefficients_omegas.argtypes = [_dp, _dp, C.c_double, C.c_double, C.c_double, _dp]
         _lib = L
     return _lib
 
@@ -88,6 +97,7 @@ def ref():
     if _ref is None:
         L = C.CDLL(REF_PATH)
         L.ref_k17_collide.argtypes = [_dp, C.c_double, _dp, _dp, _dp]
+        L.ref_k17_collide_turb.argtypes = [_dp, C.c_double, _dp, _dp, _dp, C.c_int, C.c_double, _dp]
         L.ref_chimera.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.ref_macroscopic.argtypes = [_dp, _dp]
         L.ref_run_periodic.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]
@@ -119,6 +129,34 @@ def k17_collide(f27, omega, lim=LIMITER, force=(0.0, 0.0, 0.0), impl="oracle"):
     fn = L.orc_k17_collide if impl == "oracle" else L.ref_k17_collide
     fn(_p(f), float(omega), _p(lim), _p(fo), _p(mv))
     return f, mv
+
+
+TURBULENCE = {"none": 0, "smagorinsky": 1, "amd": 2, "qr": 3}
+
+
+def k17_collide_turb(f27, omega, turb, sgs, nut, lim=LIMITER, force=(0.0, 0.0, 0.0), impl="oracle"):
+    """One node with an eddy-viscosity model: (post-collision populations,
+    (drho, vx, vy, vz), the node's turbulent viscosity after the collision)."""
+    f = np.ascontiguousarray(f27, dtype=np.float64).copy()
+    lim = np.ascontiguousarray(lim, dtype=np.float64)
+    fo = np.ascontiguousarray(force, dtype=np.float64)
+    mv = np.zeros(4)
+    nu = np.array([float(nut)])
+    L = lib() if impl == "oracle" else ref()
+    fn = L.orc_k17_collide_turb if impl == "oracle" else L.ref_k17_collide_turb
+    fn(_p(f), float(omega), _p(lim), _p(fo), _p(mv), TURBULENCE[turb], float(sgs), _p(nu))
+    return f, mv, float(nu[0])
+
+
+def turbulent_viscosity_amd(dims, vx, vy, vz, C_amd, solid=None):
+    """AMD eddy viscosity of every non-solid node of a dense periodic box."""
+    nx, ny, nz = dims
+    v = [np.ascontiguousarray(a, dtype=np.float64).ravel() for a in (vx, vy, vz)]
+    nut = np.zeros(nx * ny * nz)
+    sol = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).ravel()
+    lib().orc_turbulent_viscosity_amd(nx, ny, nz, sol.ctypes.data_as(_u8p) if sol is not None else None,
+                                      _p(v[0]), _p(v[1]), _p(v[2]), float(C_amd), _p(nut))
+    return nut
 
 
 def bgk_collide(f27, omega):
@@ -220,6 +258,28 @@ def run(F, dims, nsteps, omega, collision="k17", solid=None, bcs=(), lim=LIMITER
     return (F, macro) if want_macro else F
 
 
+def run_turb(F, dims, nsteps, omega, turb, sgs, nut=None, solid=None, bcs=(), lim=LIMITER,
+             force=(0.0, 0.0, 0.0), want_macro=False):
+    """`run` with the K17 collision and an eddy-viscosity model; returns
+    (F, nut[, macro]) with the turbulent viscosity of every node."""
+    nx, ny, nz = dims
+    F = np.ascontiguousarray(F, dtype=np.float64).copy()
+    assert F.shape == (nx * ny * nz, 27)
+    nut = np.zeros(nx * ny * nz) if nut is None else np.ascontiguousarray(nut, dtype=np.float64).copy()
+    sol = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).ravel()
+    bcl = list(bcs)
+    arr = (OrcBC * max(1, len(bcl)))(*[b.c() for b in bcl]) if bcl else None
+    macro = np.zeros((nx * ny * nz, 4)) if want_macro else None
+    lim = np.ascontiguousarray(lim, dtype=np.float64)
+    fo = np.ascontiguousarray(force, dtype=np.float64)
+    rc = lib().orc_run_turb(_p(F), nx, ny, nz, sol.ctypes.data_as(_u8p) if sol is not None else None, float(omega),
+                            _p(lim), _p(fo), len(bcl), arr, int(nsteps), _p(macro) if macro is not None else None,
+                            TURBULENCE[turb], float(sgs), _p(nut))
+    if rc != 0:
+        raise RuntimeError(f"orc_run_turb failed ({rc}): a fluid->solid link has no boundary condition")
+    return (F, nut, macro) if want_macro else (F, nut)
+
+
 def mass_momentum(F, solid=None):
     F = np.ascontiguousarray(F, dtype=np.float64)
     out = np.zeros(4)
@@ -309,7 +369,16 @@ def interp_cell(kind, f8x27, omega_src, omega_dst, offset=(0.0, 0.0, 0.0), impl=
     return k, out
 
 
-def _scale(fn, Fc, Ff, dims_c, dims_f, coarse, fine, offsets, omega_c, omega_f):
+def interp_coefficients_omegas(f8x27, omega8, offset=(0.0, 0.0, 0.0)):
+    """interp_coefficients with one omega per corner (corner bit order)."""
+    f = np.ascontiguousarray(f8x27, dtype=np.float64)
+    om = np.ascontiguousarray(omega8, dtype=np.float64)
+    out = np.zeros(42)
+    lib().orc_interp_coefficients_omegas(_p(f), _p(om), *(float(o) for o in offset), _p(out))
+    return out
+
+
+def _scale(fn, Fc, Ff, dims_c, dims_f, coarse, fine, offsets, omega_c, omega_f, nut=None):
     n = len(coarse)
     co = np.ascontiguousarray(coarse, dtype=np.int64)
     fi = np.ascontiguousarray(fine, dtype=np.int64)
@@ -319,21 +388,29 @@ def _scale(fn, Fc, Ff, dims_c, dims_f, coarse, fine, offsets, omega_c, omega_f):
     if offsets is not None:
         offs = [np.ascontiguousarray(o, dtype=np.float64) for o in offsets]
     i64 = C.POINTER(C.c_int64)
+    extra = ()
+    if nut is not None:
+        nc, nf = (np.ascontiguousarray(a, dtype=np.float64) for a in nut)
+        extra = (_p(nc), _p(nf))
     fn(_p(Fc), _p(Ff), dc, df, n, co.ctypes.data_as(i64), fi.ctypes.data_as(i64),
-       *(_p(o) if o is not None else None for o in offs), float(omega_c), float(omega_f))
+       *(_p(o) if o is not None else None for o in offs), float(omega_c), float(omega_f), *extra)
 
 
-def scale_cf(Fc, Ff, dims_c, dims_f, coarse_cells, fine_cells, omega_c, omega_f, offsets=None):
-    """Coarse-to-fine over cells; returns the updated fine populations."""
+def scale_cf(Fc, Ff, dims_c, dims_f, coarse_cells, fine_cells, omega_c, omega_f, offsets=None, nut=None):
+    """Coarse-to-fine over cells; returns the updated fine populations.
+    nut = (coarse, fine) turbulent viscosity arrays: per-node omegas."""
     Fc = np.ascontiguousarray(Fc, dtype=np.float64)
     Ff = np.ascontiguousarray(Ff, dtype=np.float64).copy()
-    _scale(lib().orc_scale_cf, Fc, Ff, dims_c, dims_f, coarse_cells, fine_cells, offsets, omega_c, omega_f)
+    fn = lib().orc_scale_cf if nut is None else lib().orc_scale_cf_turb
+    _scale(fn, Fc, Ff, dims_c, dims_f, coarse_cells, fine_cells, offsets, omega_c, omega_f, nut)
     return Ff
 
 
-def scale_fc(Fc, Ff, dims_c, dims_f, coarse_nodes, fine_cells, omega_c, omega_f, offsets=None):
-    """Fine-to-coarse over cells; returns the updated coarse populations."""
+def scale_fc(Fc, Ff, dims_c, dims_f, coarse_nodes, fine_cells, omega_c, omega_f, offsets=None, nut=None):
+    """Fine-to-coarse over cells; returns the updated coarse populations.
+    nut = (coarse, fine) turbulent viscosity arrays: per-node omegas."""
     Fc = np.ascontiguousarray(Fc, dtype=np.float64).copy()
     Ff = np.ascontiguousarray(Ff, dtype=np.float64)
-    _scale(lib().orc_scale_fc, Fc, Ff, dims_c, dims_f, coarse_nodes, fine_cells, offsets, omega_c, omega_f)
+    fn = lib().orc_scale_fc if nut is None else lib().orc_scale_fc_turb
+    _scale(fn, Fc, Ff, dims_c, dims_f, coarse_nodes, fine_cells, offsets, omega_c, omega_f, nut)
     return Fc

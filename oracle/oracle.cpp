@@ -223,8 +223,27 @@ void backward_sweeps(double (&m)[3][3][3], double vx, double vy, double vz)
 // Reproduces the K arguments of K17CompressibleNavierStokes.h:190-222 and
 // :525-557, including the (K^-1 = 3, K = 1/9) pair the reference passes for
 // the (y,z) = (2,2) row of the forward x sweep (:222).
-void k17_collide(double* f, double omega, const double* lim, const double* force, double* macro)
+// Eddy-viscosity models (vf::lbm::TurbulenceModel, src/lbm/collision/TurbulentViscosity.h:50-89)
+enum { TURB_NONE = 0, TURB_SMAGORINSKY = 1, TURB_AMD = 2, TURB_QR = 3 };
+
+// :355-368 with TurbulentViscosity.h:63-84
+double smagorinsky_viscosity(double Cs, double dxux, double dyuy, double dzuz, double Dxy, double Dxz, double Dyz)
 {
+    return Cs * Cs * std::sqrt(2.0 * (dxux * dxux + dyuy * dyuy + dzuz * dzuz) + Dxy * Dxy + Dxz * Dxz + Dyz * Dyz);
+}
+double qr_viscosity(double C, double dxux, double dyuy, double dzuz, double Dxy, double Dxz, double Dyz)
+{
+    const double Q = c1o2 * (dxux * dxux + dyuy * dyuy + dzuz * dzuz) + 0.25 * (Dxy * Dxy + Dxz * Dxz + Dyz * Dyz);
+    const double R = -dxux * dyuy * dzuz + 0.25 * (-Dxy * Dxz * Dyz + dxux * Dyz * Dyz + dyuy * Dxz * Dxz + dzuz * Dxy * Dxy);
+    return C * (R > 0.0 ? R : 0.0) / Q;
+}
+
+// turb != TURB_NONE: the node relaxes with omega / (1 + 3 omega nu_t) (:245);
+// Smagorinsky / QR replace *nut by the model's value of the new gradients.
+void k17_collide(double* f, double omega_in, const double* lim, const double* force, double* macro,
+                 int turb = TURB_NONE, double sgs = 0.0, double* nut = nullptr)
+{
+    const double omega = turb == TURB_NONE ? omega_in : omega_in / (1.0 + 3.0 * omega_in * *nut);
     double m[3][3][3];
     for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
@@ -285,7 +304,7 @@ void k17_collide(double* f, double omega, const double* lim, const double* force
     double& m122 = m[1][2][2]; double& m212 = m[2][1][2]; double& m221 = m[2][2][1];
     double& m222 = m[2][2][2];
 
-    // :245-273 relaxation rates (TurbulenceModel::None)
+    // :245-273 relaxation rates
     const double OxxPyyPzz = 1.0;
     const double OxyyPxzz = 8.0 * (-2.0 + omega) * (1.0 + 2.0 * omega) / (-8.0 - 14.0 * omega + 7.0 * omega * omega);
     const double OxyyMxzz = 8.0 * (-2.0 + omega) * (-7.0 + 4.0 * omega) / (56.0 - 50.0 * omega + 9.0 * omega * omega);
@@ -334,6 +353,8 @@ void k17_collide(double* f, double omega, const double* lim, const double* force
     const double dxux = c1o2 * (-omega) * (mxxMyy + mxxMzz) + c1o2 * OxxPyyPzz * (m000 - mxxPyyPzz);
     const double dyuy = dxux + omega * c3o2 * mxxMyy;
     const double dzuz = dxux + omega * c3o2 * mxxMzz;
+    if (turb == TURB_SMAGORINSKY) *nut = smagorinsky_viscosity(sgs, dxux, dyuy, dzuz, Dxy, Dxz, Dyz);
+    if (turb == TURB_QR) *nut = qr_viscosity(sgs, dxux, dyuy, dzuz, Dxy, Dxz, Dyz);
 
     // :375-387 second order with correction
     mxxPyyPzz += OxxPyyPzz * (m000 - mxxPyyPzz) - 3.0 * (1.0 - c1o2 * OxxPyyPzz) * (vx2 * dxux + vy2 * dyuy + vz2 * dzuz);
@@ -829,12 +850,20 @@ int64_t box_node(int x, int y, int z, int nx, int ny, int nz)
     return (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
 }
 
-void cell_moments(const double* F, int64_t base, int nx, int ny, int nz, double omega, SourceMoments src[8])
+// calculateOmegaWithturbulentViscosity (TurbulentViscosity.h:86-89) where a
+// turbulent viscosity array is given (hasTurbulentViscosity of the scaling)
+double node_omega(double omega, const double* nut, int64_t k)
+{
+    return nut ? omega / (1.0 + 3.0 * omega * nut[k]) : omega;
+}
+
+void cell_moments(const double* F, int64_t base, int nx, int ny, int nz, double omega, SourceMoments src[8],
+                  const double* nut = nullptr)
 {
     const int bx = (int)(base % nx), by = (int)((base / nx) % ny), bz = (int)(base / ((int64_t)nx * ny));
     for (int c = 0; c < 8; ++c) {
         const int64_t k = box_node(bx + (c & 1), by + ((c >> 1) & 1), bz + ((c >> 2) & 1), nx, ny, nz);
-        source_moments(F + 27 * k, omega, src[c]);
+        source_moments(F + 27 * k, node_omega(omega, nut, k), src[c]);
     }
 }
 
@@ -858,6 +887,13 @@ typedef struct {
     const int64_t* nbr;       // pressure: inward neighbour node
     int32_t normal;           // pressure: outward direction
 } orc_bc;
+
+void orc_k17_collide_turb(double* f27, double omega, const double* lim3, const double* force3, double* macro4,
+                          int turb, double sgs, double* nut)
+{
+    static const double zero[3] = {0, 0, 0};
+    k17_collide(f27, omega, lim3, force3 ? force3 : zero, macro4, turb, sgs, nut);
+}
 
 void orc_k17_collide(double* f27, double omega, const double* lim3, const double* force3, double* macro4)
 {
@@ -904,10 +940,73 @@ double orc_wall_term(int d, double ux, double uy, double uz, int cpu_float) { re
 // receives {drho,vx,vy,vz} of the last collision.  Returns 0, or a negative
 // code when a fluid node streams into a solid node through a link no boundary
 // condition covers.
+static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int collision, double omega,
+                    const double* lim3, const double* force3, int n_bc, const orc_bc* bcs, int nsteps, double* macro,
+                    int turb, double sgs, double* nut);
+
 int orc_run(double* F, int nx, int ny, int nz, const uint8_t* solid, int collision, double omega,
             const double* lim3, const double* force3, int n_bc, const orc_bc* bcs, int nsteps, double* macro)
 {
+    return run_impl(F, nx, ny, nz, solid, collision, omega, lim3, force3, n_bc, bcs, nsteps, macro, TURB_NONE, 0.0,
+                    nullptr);
+}
+
+// orc_run with an eddy-viscosity model of the K17 collision: nut (N values)
+// holds the turbulent viscosity of every node in and out.  Smagorinsky / QR
+// update it inside the collision; AMD recomputes it from the velocities the
+// collision wrote after the streaming and the post-collision boundary
+// conditions, before the pre-collision ones (UpdateGrid27.cpp:69-85).
+int orc_run_turb(double* F, int nx, int ny, int nz, const uint8_t* solid, double omega, const double* lim3,
+                 const double* force3, int n_bc, const orc_bc* bcs, int nsteps, double* macro, int turb, double sgs,
+                 double* nut)
+{
+    return run_impl(F, nx, ny, nz, solid, ORC_K17, omega, lim3, force3, n_bc, bcs, nsteps, macro, turb, sgs, nut);
+}
+
+// calcAMD (src/gpu/core/PostProcessor/TurbulentViscosityKernels.cu:52-113) on
+// a dense periodic box: fluid = not solid; derivatives central where both
+// neighbours along an axis are fluid, one-sided where one is, 0 where none is.
+void orc_turbulent_viscosity_amd(int nx, int ny, int nz, const uint8_t* solid, const double* vx, const double* vy,
+                                 const double* vz, double C, double* nut)
+{
     const int64_t N = (int64_t)nx * ny * nz;
+#pragma omp parallel for schedule(static)
+    for (int64_t k = 0; k < N; ++k) {
+        if (solid && solid[k]) continue;
+        const int x = (int)(k % nx), y = (int)((k / nx) % ny), z = (int)(k / ((int64_t)nx * ny));
+        auto at = [&](int a, int b, int c) {
+            return (int64_t)((a + nx) % nx) + (int64_t)nx * ((int64_t)((b + ny) % ny) + (int64_t)ny * ((c + nz) % nz));
+        };
+        const int64_t kp[3] = {at(x + 1, y, z), at(x, y + 1, z), at(x, y, z + 1)};
+        const int64_t km[3] = {at(x - 1, y, z), at(x, y - 1, z), at(x, y, z - 1)};
+        double g[3][3]; // g[i][a] = d u_i / d x_a
+        for (int a = 0; a < 3; ++a) {
+            const bool fp = !(solid && solid[kp[a]]), fm = !(solid && solid[km[a]]);
+            const double div = (fm && fp) ? c1o2 : 1.0;
+            const double* v[3] = {vx, vy, vz};
+            for (int i = 0; i < 3; ++i) g[i][a] = ((fp ? v[i][kp[a]] : v[i][k]) - (fm ? v[i][km[a]] : v[i][k])) * div;
+        }
+        const double dvxdx = g[0][0], dvxdy = g[0][1], dvxdz = g[0][2];
+        const double dvydx = g[1][0], dvydy = g[1][1], dvydz = g[1][2];
+        const double dvzdx = g[2][0], dvzdy = g[2][1], dvzdz = g[2][2];
+        const double den = dvxdx * dvxdx + dvydx * dvydx + dvzdx * dvzdx + dvxdy * dvxdy + dvydy * dvydy +
+                           dvzdy * dvzdy + dvxdz * dvxdz + dvydz * dvydz + dvzdz * dvzdz;
+        const double num = (dvxdx * dvxdx + dvxdy * dvxdy + dvxdz * dvxdz) * dvxdx +
+                           (dvydx * dvydx + dvydy * dvydy + dvydz * dvydz) * dvydy +
+                           (dvzdx * dvzdx + dvzdy * dvzdy + dvzdz * dvzdz) * dvzdz +
+                           (dvxdx * dvydx + dvxdy * dvydy + dvxdz * dvydz) * (dvxdy + dvydx) +
+                           (dvxdx * dvzdx + dvxdy * dvzdy + dvxdz * dvzdz) * (dvxdz + dvzdx) +
+                           (dvydx * dvzdx + dvydy * dvzdy + dvydz * dvzdz) * (dvydz + dvzdy);
+        nut[k] = den != 0.0 ? std::max(0.0, -C * num) / den : 0.0;
+    }
+}
+
+static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int collision, double omega,
+                    const double* lim3, const double* force3, int n_bc, const orc_bc* bcs, int nsteps, double* macro,
+                    int turb, double sgs, double* nut)
+{
+    const int64_t N = (int64_t)nx * ny * nz;
+    std::vector<double> vel(turb == TURB_AMD ? (size_t)N * 3 : 0);
     std::vector<double> post((size_t)N * 27), G((size_t)N * 27);
     static const double zero[3] = {0, 0, 0};
     if (!force3) force3 = zero;
@@ -946,8 +1045,10 @@ int orc_run(double* F, int nx, int ny, int nz, const uint8_t* solid, int collisi
             double f[27];
             std::memcpy(f, F + 27 * i, sizeof(f));
             double mv[4];
-            if (collision == ORC_K17) k17_collide(f, omega, lim3, force3, mv);
+            if (collision == ORC_K17) k17_collide(f, omega, lim3, force3, mv, turb, sgs, nut ? nut + i : nullptr);
             else bgk_collide(f, omega, mv);
+            if (turb == TURB_AMD)
+                for (int a = 0; a < 3; ++a) vel[(size_t)a * N + i] = mv[1 + a];
             std::memcpy(&post[(size_t)i * 27], f, sizeof(f));
             if (macro && s == nsteps - 1) std::memcpy(macro + 4 * i, mv, sizeof(mv));
         }
@@ -996,6 +1097,8 @@ int orc_run(double* F, int nx, int ny, int nz, const uint8_t* solid, int collisi
             for (int64_t i = 0; i < N; ++i)
                 if (solid[i]) std::memcpy(&G[(size_t)i * 27], F + 27 * i, 27 * sizeof(double));
         std::memcpy(F, G.data(), (size_t)N * 27 * sizeof(double));
+        if (turb == TURB_AMD)
+            orc_turbulent_viscosity_amd(nx, ny, nz, solid, vel.data(), vel.data() + N, vel.data() + 2 * N, sgs, nut);
         // pre-collision boundary conditions of the next step (after the parity
         // flip, UpdateGrid27.cpp:288-307): read the inward neighbours first
         for (int b = 0; b < n_bc; ++b) {
@@ -1144,6 +1247,17 @@ void orc_interp_coefficients(const double* f8x27, double omega, double xoff, dou
     pack_coefficients(k, out42);
 }
 
+// as orc_interp_coefficients with one omega per corner
+void orc_interp_coefficients_omegas(const double* f8x27, const double* omega8, double xoff, double yoff, double zoff,
+                                    double* out42)
+{
+    SourceMoments src[8];
+    for (int c = 0; c < 8; ++c) source_moments(f8x27 + 27 * c, omega8[c], src[c]);
+    Coefficients k;
+    interpolation_coefficients(src, xoff, yoff, zoff, k);
+    pack_coefficients(k, out42);
+}
+
 void orc_interpolate_cf(const double* coeff42, double omega_fine, double eps, double x, double y, double z, double* f27)
 {
     Coefficients k;
@@ -1163,13 +1277,16 @@ void orc_interpolate_fc(const double* coeff42, double omega_coarse, double eps, 
 // corner MMM coarse_cells[i]; its 8 fine nodes form the fine cell with corner
 // MMM fine_cells[i] and receive the polynomial at (+-1/4, +-1/4, +-1/4)
 // (eps = 1/2, fine omega).  Offsets may be NULL (zero).
-void orc_scale_cf(const double* Fc, double* Ff, const int* dims_c, const int* dims_f, int64_t n_cells,
-                  const int64_t* coarse_cells, const int64_t* fine_cells, const double* ox, const double* oy,
-                  const double* oz, double omega_c, double omega_f)
+// nut_c / nut_f (may be NULL): turbulent viscosity of the coarse / fine level;
+// given, every source corner and destination node uses its own node's omega
+// (scaleCoarseToFineCompressible.cu:94, 369).
+void orc_scale_cf_turb(const double* Fc, double* Ff, const int* dims_c, const int* dims_f, int64_t n_cells,
+                       const int64_t* coarse_cells, const int64_t* fine_cells, const double* ox, const double* oy,
+                       const double* oz, double omega_c, double omega_f, const double* nut_c, const double* nut_f)
 {
     for (int64_t i = 0; i < n_cells; ++i) {
         SourceMoments src[8];
-        cell_moments(Fc, coarse_cells[i], dims_c[0], dims_c[1], dims_c[2], omega_c, src);
+        cell_moments(Fc, coarse_cells[i], dims_c[0], dims_c[1], dims_c[2], omega_c, src, nut_c);
         Coefficients k;
         interpolation_coefficients(src, ox ? ox[i] : 0.0, oy ? oy[i] : 0.0, oz ? oz[i] : 0.0, k);
         const int64_t b = fine_cells[i];
@@ -1178,26 +1295,42 @@ void orc_scale_cf(const double* Fc, double* Ff, const int* dims_c, const int* di
         for (int c = 0; c < 8; ++c) {
             const double x = (c & 1) ? 0.25 : -0.25, y = (c & 2) ? 0.25 : -0.25, z = (c & 4) ? 0.25 : -0.25;
             const int64_t kf = box_node(bx + (c & 1), by + ((c >> 1) & 1), bz + ((c >> 2) & 1), nx, ny, nz);
-            interpolate_cf(k, omega_f, 0.5, x, y, z, Ff + 27 * kf);
+            interpolate_cf(k, node_omega(omega_f, nut_f, kf), 0.5, x, y, z, Ff + 27 * kf);
         }
     }
+}
+
+void orc_scale_cf(const double* Fc, double* Ff, const int* dims_c, const int* dims_f, int64_t n_cells,
+                  const int64_t* coarse_cells, const int64_t* fine_cells, const double* ox, const double* oy,
+                  const double* oz, double omega_c, double omega_f)
+{
+    orc_scale_cf_turb(Fc, Ff, dims_c, dims_f, n_cells, coarse_cells, fine_cells, ox, oy, oz, omega_c, omega_f,
+                      nullptr, nullptr);
 }
 
 // Fine-to-coarse (scaleFineToCoarseCompressible.cu:76-124): fine cell with
 // corner MMM fine_cells[i] -> coarse node coarse_nodes[i] at the cell centre
 // (eps = 2, coarse omega).
-void orc_scale_fc(double* Fc, const double* Ff, const int* dims_c, const int* dims_f, int64_t n_cells,
-                  const int64_t* coarse_nodes, const int64_t* fine_cells, const double* ox, const double* oy,
-                  const double* oz, double omega_c, double omega_f)
+void orc_scale_fc_turb(double* Fc, const double* Ff, const int* dims_c, const int* dims_f, int64_t n_cells,
+                       const int64_t* coarse_nodes, const int64_t* fine_cells, const double* ox, const double* oy,
+                       const double* oz, double omega_c, double omega_f, const double* nut_c, const double* nut_f)
 {
     (void)dims_c;
     for (int64_t i = 0; i < n_cells; ++i) {
         SourceMoments src[8];
-        cell_moments(Ff, fine_cells[i], dims_f[0], dims_f[1], dims_f[2], omega_f, src);
+        cell_moments(Ff, fine_cells[i], dims_f[0], dims_f[1], dims_f[2], omega_f, src, nut_f);
         Coefficients k;
         interpolation_coefficients(src, ox ? ox[i] : 0.0, oy ? oy[i] : 0.0, oz ? oz[i] : 0.0, k);
-        interpolate_fc(k, omega_c, 2.0, Fc + 27 * coarse_nodes[i]);
+        interpolate_fc(k, node_omega(omega_c, nut_c, coarse_nodes[i]), 2.0, Fc + 27 * coarse_nodes[i]);
     }
+}
+
+void orc_scale_fc(double* Fc, const double* Ff, const int* dims_c, const int* dims_f, int64_t n_cells,
+                  const int64_t* coarse_nodes, const int64_t* fine_cells, const double* ox, const double* oy,
+                  const double* oz, double omega_c, double omega_f)
+{
+    orc_scale_fc_turb(Fc, Ff, dims_c, dims_f, n_cells, coarse_nodes, fine_cells, ox, oy, oz, omega_c, omega_f,
+                      nullptr, nullptr);
 }
 
 } // extern "C"

@@ -130,6 +130,34 @@ void ref_k17_collide(double* f27, double omega, const double* lim3, const double
     if (macro4) { macro4[0] = mv.rho; macro4[1] = mv.vx; macro4[2] = mv.vy; macro4[3] = mv.vz; }
 }
 
+// the reference K17 with an eddy-viscosity model (turb: 1 Smagorinsky, 2 AMD,
+// 3 QR): nut is the node's turbulent viscosity in and the model's value out
+void ref_k17_collide_turb(double* f27, double omega, const double* lim3, const double* force3, double* macro4,
+                          int turb, double sgs, double* nut)
+{
+    vf::lbm::CollisionParameter p;
+    for (int d = 0; d < 27; ++d) p.distribution[d] = f27[d];
+    double lim[3] = {lim3[0], lim3[1], lim3[2]};
+    p.omega = omega;
+    p.quadricLimiter = lim;
+    p.forceX = force3 ? force3[0] : 0.0;
+    p.forceY = force3 ? force3[1] : 0.0;
+    p.forceZ = force3 ? force3[2] : 0.0;
+    vf::lbm::MacroscopicValues mv;
+    vf::lbm::TurbulentViscosity tv;
+    tv.value = *nut;
+    tv.SGSconstant = sgs;
+    using TM = vf::lbm::TurbulenceModel;
+    switch (turb) {
+    case 1: vf::lbm::runK17CompressibleNavierStokes<TM::Smagorinsky>(p, mv, tv); break;
+    case 2: vf::lbm::runK17CompressibleNavierStokes<TM::AMD>(p, mv, tv); break;
+    default: vf::lbm::runK17CompressibleNavierStokes<TM::QR>(p, mv, tv); break;
+    }
+    *nut = tv.value;
+    for (int d = 0; d < 27; ++d) f27[d] = p.distribution[d];
+    if (macro4) { macro4[0] = mv.rho; macro4[1] = mv.vx; macro4[2] = mv.vy; macro4[3] = mv.vz; }
+}
+
 void ref_chimera(int kind, double* abc, double vv, double v2, double kinv, double k)
 {
     switch (kind) {

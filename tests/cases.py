@@ -107,6 +107,17 @@ def sparse_layout(dims, solid=None, seed=0):
     return nbx, nby, nbz, fluid_nodes, sidx
 
 
+def sparse_inverse(dims, sidx):
+    """neighborInverse of sparse_layout's arrays: the node in -x-y-z
+    (GridImp.cpp:1034, 1112-1130); the dummy node 0 points to itself."""
+    nx, ny, nz = dims
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    x, y, z = x.ravel(), y.ravel(), z.ravel()
+    inv = np.zeros(sidx.size + 1, np.uint32)
+    inv[sidx] = sidx[((x - 1) % nx) + nx * (((y - 1) % ny) + ny * ((z - 1) % nz))]
+    return inv
+
+
 def boundary_groups(solid, dims, classify):
     """Collect the fluid->solid links of a structured box into boundary groups.
 

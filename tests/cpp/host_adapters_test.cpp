@@ -69,11 +69,21 @@ static void cpu_tests()
     EXPECT(!gsf.getGridScalingFC(false) && !gsf.getGridScalingCF(false));
     gsf.setScalingFactory(GridScalingFactory::GridScaling::ScaleCompressible);
     EXPECT((bool)gsf.getGridScalingFC(false) && (bool)gsf.getGridScalingCF(false));
+    // with a turbulent viscosity the same scaling runs with per-node rates
+    EXPECT((bool)gsf.getGridScalingFC(true) && (bool)gsf.getGridScalingCF(true));
+    // TurbulenceModelFactory::readConfigFile keys
+    auto pt = std::make_shared<Parameter>(0);
+    TurbulenceModelFactory tmf(pt);
+    EXPECT(pt->getTurbulenceModel() == TurbulenceModel::None && !pt->getUseTurbulentViscosity());
+    tmf.readConfig({{"TurbulenceModel", "QR"}, {"SGSconstant", "0.3"}});
+    EXPECT(pt->getTurbulenceModel() == TurbulenceModel::QR && pt->getUseTurbulentViscosity());
+    EXPECT(pt->getSGSConstant() == 0.3);
+    EXPECT(vfx_turbulence_model_of(TurbulenceModel::AMD) == VFX_TURB_AMD);
     threw = false;
     try {
-        gsf.getGridScalingCF(true);
-    } catch (const std::runtime_error&) {
-        threw = true;
+        tmf.readConfig({{"TurbulenceModel", "WALE"}});
+    } catch (const std::runtime_error& e) {
+        threw = std::strcmp(e.what(), "TurbulenceModelFactory: Invalid turbulence model!") == 0;
     }
     EXPECT(threw);
     // the C ABI rejects bad arguments without touching a device
@@ -276,6 +286,88 @@ static void gpu_refinement_tests()
         vfx_device_free(d);
 }
 
+// The AMD model through UpdateGrid27 + TurbulenceModelFactory (collision with
+// the turbulent viscosity, swap, calcTurbulentViscosityAMD) equals the same
+// launches through the C ABI directly, populations and viscosity bit for bit.
+static void gpu_turbulence_tests()
+{
+    const uint32_t nx = 10, ny = 8, nz = 6, N = nx * ny * nz;
+    std::vector<double> drho(N), ux(N), uy(N), uz(N), zero(N, 0.0);
+    for (uint32_t k = 0; k < N; ++k) {
+        drho[k] = 1e-3 * std::sin(0.21 * k);
+        ux[k] = 0.04 * std::sin(0.17 * k);
+        uy[k] = 0.03 * std::cos(0.09 * k);
+        uz[k] = 0.02 * std::sin(0.07 * k + 0.3);
+    }
+    auto up = [&](const void* src, size_t bytes) {
+        void* d = nullptr;
+        check(vfx_device_alloc(bytes, &d), "alloc");
+        check(vfx_copy_to_device(d, src, bytes), "h2d");
+        return d;
+    };
+    void* fields[4] = {up(drho.data(), N * 8), up(ux.data(), N * 8), up(uy.data(), N * 8), up(uz.data(), N * 8)};
+    // per run: populations, turbulent viscosity, rho/vx/vy/vz
+    void* arr[2][6];
+    for (auto& run : arr) {
+        check(vfx_device_alloc((uint64_t)N * 27 * 8, &run[0]), "alloc");
+        for (int i = 1; i < 6; ++i) run[i] = up(zero.data(), N * 8);
+    }
+    auto lattice = [&](void* const* run) {
+        vfx_lattice l{};
+        l.storage = VFX_FP64;
+        l.n_nodes = N;
+        l.direct = 1;
+        l.dims[0] = nx;
+        l.dims[1] = ny;
+        l.dims[2] = nz;
+        l.omega = 1.95;
+        l.quadric_limiter[0] = l.quadric_limiter[1] = l.quadric_limiter[2] = 0.01;
+        l.f = run[0];
+        l.turbulent_viscosity = run[1];
+        l.rho = run[2];
+        l.vx = run[3];
+        l.vy = run[4];
+        l.vz = run[5];
+        check(vfx_init_equilibrium(&l, 1, (double*)fields[0], (double*)fields[1], (double*)fields[2],
+                                   (double*)fields[3], nullptr),
+              "init");
+        return l;
+    };
+    // (a) host layer
+    auto para = std::make_shared<Parameter>(0);
+    para->getParD(0)->lattice = lattice(arr[0]);
+    auto tm = std::make_shared<TurbulenceModelFactory>(para);
+    tm->setTurbulenceModel(TurbulenceModel::AMD);
+    tm->setModelConstant(0.33);
+    KernelFactoryImp factory;
+    UpdateGrid27 ug(para, factory.makeKernels(para), BoundaryConditionFactory(), nullptr, tm);
+    for (unsigned t = 0; t < 10; ++t) ug.updateGrid(0, t);
+    // (b) C ABI
+    vfx_lattice lb = lattice(arr[1]);
+    lb.turbulence_model = VFX_TURB_AMD;
+    lb.sgs_constant = 0.33;
+    bool even = true;
+    for (int t = 0; t < 10; ++t) {
+        check(vfx_k17_step(&lb, even ? 1 : 0, VFX_TEMPLATE_DEFAULT, nullptr), "step");
+        even = !even;
+        check(vfx_turbulent_viscosity_amd(&lb, nullptr), "amd");
+    }
+    check(vfx_synchronize(nullptr), "sync");
+    std::vector<double> a((size_t)N * 27), b((size_t)N * 27), na(N), nb(N);
+    check(vfx_copy_to_host(a.data(), arr[0][0], a.size() * 8), "d2h");
+    check(vfx_copy_to_host(b.data(), arr[1][0], b.size() * 8), "d2h");
+    check(vfx_copy_to_host(na.data(), arr[0][1], N * 8), "d2h");
+    check(vfx_copy_to_host(nb.data(), arr[1][1], N * 8), "d2h");
+    EXPECT(std::memcmp(a.data(), b.data(), a.size() * 8) == 0);
+    EXPECT(std::memcmp(na.data(), nb.data(), N * 8) == 0);
+    double nmax = 0;
+    for (double v : na) nmax = std::fmax(nmax, v);
+    EXPECT(nmax > 0.0);
+    for (auto& run : arr)
+        for (void* d : run) vfx_device_free(d);
+    for (void* d : fields) vfx_device_free(d);
+}
+
 // Restart files: write the reference unit test's data (RestartObjectTests.cpp,
 // saveAndLoad) plus values that exercise the text format, and read back the
 // files the Python writer made for the same data (<prefix>_py.bin/.txt).
@@ -327,6 +419,7 @@ int main(int argc, char** argv)
         if (gpu) {
             gpu_tests();
             gpu_refinement_tests();
+            gpu_turbulence_tests();
         }
         else cpu_tests();
     } catch (const std::exception& e) {

@@ -15,6 +15,10 @@ Fixtures (inputs and reference outputs only; parameters in golden.json):
                     header-only MomentsOnSourceNodeSet / interpolateCF / interpolateFC
                     (InterpolationCoefficients.h:74-432, InterpolationCF.h:57-306,
                     InterpolationFC.h:56-263)
+  g6_k17_turbulence.npz  single-node runK17CompressibleNavierStokes<Smagorinsky / AMD / QR>
+                    (K17CompressibleNavierStokes.h:245, 355-368, TurbulentViscosity.h:63-89)
+                    with random turbulent viscosity in; populations, macroscopic
+                    values and turbulent viscosity out
 
 `python tests/golden/make_golden.py g4` writes only the named fixtures.
 """
@@ -110,7 +114,37 @@ def g4(seed=4242, n=128):
             "fc": "fine moments (omega_f) -> cell centre, eps 2, omega_c"}
 
 
-GENERATORS = {"g1": g1, "g2": g2, "g3": g3, "g4": g4}
+def g6(seed=6606, n=192):
+    """Per model (smagorinsky, amd, qr): random shifted states at three
+    amplitudes, omega in [0.6, 1.999], turbulent viscosity in [0, 0.05) (0 for
+    every 8th node), SGS constant in [0.05, 0.5]."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for model in ("smagorinsky", "amd", "qr"):
+        f_in = np.zeros((n, 27))
+        om = np.zeros(n)
+        nut_in = np.zeros(n)
+        sgs = np.zeros(n)
+        for i in range(n):
+            f_in[i] = rng.uniform(-1, 1, 27) * [1e-4, 1e-2, 1e-1][i % 3] * W
+            om[i] = rng.uniform(0.6, 1.999)
+            nut_in[i] = 0.0 if i % 8 == 0 else rng.uniform(0.0, 0.05)
+            sgs[i] = rng.uniform(0.05, 0.5)
+        f_out = np.zeros_like(f_in)
+        macro = np.zeros((n, 4))
+        nut_out = np.zeros(n)
+        for i in range(n):
+            f_out[i], macro[i], nut_out[i] = oracle.k17_collide_turb(f_in[i], om[i], model, sgs[i], nut_in[i],
+                                                                     impl="ref")
+        out.update({f"{model}_f_in": f_in, f"{model}_omega": om, f"{model}_nut_in": nut_in,
+                    f"{model}_sgs": sgs, f"{model}_f_out": f_out, f"{model}_macro": macro,
+                    f"{model}_nut_out": nut_out})
+    np.savez(os.path.join(HERE, "g6_k17_turbulence.npz"), **out)
+    return {"seed": seed, "n_per_model": n, "models": ["smagorinsky", "amd", "qr"], "limiter": 0.01,
+            "note": "nut_in 0 for every 8th node, else U[0, 0.05); SGS constant U[0.05, 0.5]"}
+
+
+GENERATORS = {"g1": g1, "g2": g2, "g3": g3, "g4": g4, "g6": g6}
 
 
 def main():

@@ -56,6 +56,27 @@ def test_invalid_arguments_fail_without_device():
     assert L.vfx_collide_stream(C.byref(s), 0, 1, 0, None) == vfx._lib.VFX_EINVAL
     assert b"type_of_grid_node" in L.vfx_last_error()
     s.type_of_grid_node = None
+    s.turbulence_model = 4  # unknown model
+    assert L.vfx_collide_stream(C.byref(s), 0, 1, 0, None) == vfx._lib.VFX_EINVAL
+    assert b"turbulence_model" in L.vfx_last_error()
+    s.turbulence_model = vfx._lib.VFX_TURB_SMAGORINSKY  # no viscosity array
+    assert L.vfx_collide_stream(C.byref(s), 0, 1, 0, None) == vfx._lib.VFX_EINVAL
+    assert b"turbulent_viscosity" in L.vfx_last_error()
+    s.turbulence_model = vfx._lib.VFX_TURB_AMD
+    s.turbulent_viscosity = 1
+    assert L.vfx_collide_stream(C.byref(s), 0, 1, 0, None) == vfx._lib.VFX_EINVAL  # AMD writes the velocities
+    assert b"AMD" in L.vfx_last_error()
+    assert L.vfx_turbulent_viscosity_amd(C.byref(s), None) == vfx._lib.VFX_EINVAL
+    s.vx = s.vy = s.vz = 1
+    s.direct = 0
+    s.neighbor_x = s.neighbor_y = s.neighbor_z = 1
+    assert L.vfx_turbulent_viscosity_amd(C.byref(s), None) == vfx._lib.VFX_EINVAL  # sparse needs the node types
+    assert b"neighbor_inverse" in L.vfx_last_error()
+    s.direct = 1
+    s.neighbor_x = s.neighbor_y = s.neighbor_z = None
+    s.vx = s.vy = s.vz = None
+    s.turbulence_model = 0
+    s.turbulent_viscosity = None
     assert L.vfx_export_distributions(C.byref(s), 1, 1, 9, None) == vfx._lib.VFX_EINVAL  # float or double only
     assert L.vfx_export_distributions(C.byref(s), 1, None, 1, None) == vfx._lib.VFX_EINVAL
     assert L.vfx_version() >= 1

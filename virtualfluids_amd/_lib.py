@@ -15,6 +15,7 @@ VFX_OK, VFX_EINVAL, VFX_EHIP, VFX_EUNSUPPORTED = 0, 1, 2, 3
 VFX_FP64, VFX_FP32, VFX_FP16_SCALED = 0, 1, 2
 VFX_K17, VFX_BGK = 0, 1
 VFX_TEMPLATE_DEFAULT, VFX_TEMPLATE_WRITE_MACRO = 0, 1
+VFX_TURB_NONE, VFX_TURB_SMAGORINSKY, VFX_TURB_AMD, VFX_TURB_QR = 0, 1, 2, 3
 VFX_BC_VELOCITY_BOUNCE_BACK, VFX_BC_NOSLIP_INTERPOLATED, VFX_BC_VELOCITY_INTERPOLATED = 0, 1, 2
 VFX_BC_SLIP, VFX_BC_PRESSURE_NEQ = 3, 4
 
@@ -32,6 +33,8 @@ class VfxLattice(C.Structure):
         ("rho", C.c_void_p), ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p),
         ("kernel_variant", C.c_int32), ("fp16_reference_velocity", C.c_double * 3),
         ("type_of_grid_node", C.c_void_p),
+        ("turbulence_model", C.c_int32), ("sgs_constant", C.c_double), ("turbulent_viscosity", C.c_void_p),
+        ("neighbor_inverse", C.c_void_p),
     ]
 
 
@@ -69,6 +72,7 @@ _SIGNATURES = {
     "vfx_read_populations": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
     "vfx_write_populations": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
     "vfx_export_distributions": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_int32, C.c_void_p],
+    "vfx_turbulent_viscosity_amd": [C.POINTER(VfxLattice), C.c_void_p],
     "vfx_import_distributions": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_int32, C.c_void_p],
     "vfx_macroscopic": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                         C.c_void_p],

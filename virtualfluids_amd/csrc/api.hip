@@ -116,6 +116,14 @@ vfx_status make_lat(const vfx_lattice* in, vfx::Lat& L)
     L.vx = in->vx;
     L.vy = in->vy;
     L.vz = in->vz;
+    if (in->turbulence_model < VFX_TURB_NONE || in->turbulence_model > VFX_TURB_QR)
+        return fail(VFX_EINVAL, "unknown turbulence_model %d", in->turbulence_model);
+    // the turbulent viscosity is read where a model is active (collision,
+    // refinement interpolation)
+    if (in->turbulence_model != VFX_TURB_NONE && !in->turbulent_viscosity)
+        return fail(VFX_EINVAL, "turbulence_model %d needs turbulent_viscosity", in->turbulence_model);
+    L.nut = in->turbulence_model != VFX_TURB_NONE ? in->turbulent_viscosity : nullptr;
+    L.nbinv = in->direct ? nullptr : in->neighbor_inverse;
     return VFX_OK;
 }
 
@@ -158,25 +166,52 @@ vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t
         return fail(VFX_EINVAL, "unknown kernel_variant %d", lat->kernel_variant);
     if (tmpl != VFX_TEMPLATE_DEFAULT && tmpl != VFX_TEMPLATE_WRITE_MACRO)
         return fail(VFX_EUNSUPPORTED, "collision template %d not supported", tmpl);
-    if (tmpl == VFX_TEMPLATE_WRITE_MACRO) {
+    // the eddy-viscosity models belong to the K17 collision (BGK has none)
+    const int turb = collision == VFX_K17 ? lat->turbulence_model : VFX_TURB_NONE;
+    // runCollision writes the macroscopic values for the AMD model (RunCollision.cuh:133-138)
+    if (tmpl == VFX_TEMPLATE_WRITE_MACRO || turb == VFX_TURB_AMD) {
         if (!lat->rho || !lat->vx || !lat->vy || !lat->vz)
-            return fail(VFX_EINVAL, "WriteMacroVars needs rho/vx/vy/vz");
+            return fail(VFX_EINVAL, turb == VFX_TURB_AMD ? "the AMD model needs rho/vx/vy/vz"
+                                                         : "WriteMacroVars needs rho/vx/vy/vz");
         L.write_macro = 1;
     }
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
+    const bool even = is_even != 0;
     switch (lat->storage) {
     case VFX_FP64:
-        e = vfx::launch_step<double>(L, lat->direct, collision, is_even != 0, lat->kernel_variant, lat->omega, lat->quadric_limiter, lat->force, st);
+        e = vfx::launch_step<double>(L, lat->direct, collision, even, lat->kernel_variant, lat->omega,
+                                     lat->quadric_limiter, lat->force, turb, lat->sgs_constant, st);
         break;
     case VFX_FP32:
-        e = vfx::launch_step<float>(L, lat->direct, collision, is_even != 0, lat->kernel_variant, lat->omega, lat->quadric_limiter, lat->force, st);
+        e = vfx::launch_step<float>(L, lat->direct, collision, even, lat->kernel_variant, lat->omega,
+                                    lat->quadric_limiter, lat->force, turb, lat->sgs_constant, st);
         break;
     default:
-        e = vfx::launch_step<_Float16>(L, lat->direct, collision, is_even != 0, lat->kernel_variant, lat->omega, lat->quadric_limiter, lat->force, st);
+        e = vfx::launch_step<_Float16>(L, lat->direct, collision, even, lat->kernel_variant, lat->omega,
+                                       lat->quadric_limiter, lat->force, turb, lat->sgs_constant, st);
         break;
     }
     return hip_status(e, "collide_stream launch");
+}
+
+vfx_status vfx_turbulent_viscosity_amd(const vfx_lattice* lat, vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    if (!lat->turbulent_viscosity) return fail(VFX_EINVAL, "the AMD model needs turbulent_viscosity");
+    if (!lat->vx || !lat->vy || !lat->vz) return fail(VFX_EINVAL, "the AMD model needs vx/vy/vz");
+    L.nut = lat->turbulent_viscosity;
+    if (!lat->direct && (!lat->type_of_grid_node || !lat->neighbor_inverse))
+        return fail(VFX_EINVAL, "the AMD model on the sparse layout needs type_of_grid_node and neighbor_inverse");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    switch (lat->storage) {
+    case VFX_FP64: e = vfx::launch_turb_amd<double>(L, lat->direct, lat->sgs_constant, st); break;
+    case VFX_FP32: e = vfx::launch_turb_amd<float>(L, lat->direct, lat->sgs_constant, st); break;
+    default: e = vfx::launch_turb_amd<_Float16>(L, lat->direct, lat->sgs_constant, st); break;
+    }
+    return hip_status(e, "turbulent_viscosity_amd launch");
 }
 
 vfx_status vfx_k17_step(const vfx_lattice* lat, int32_t is_even, int32_t tmpl, vfx_stream stream)
@@ -384,6 +419,9 @@ vfx_status scale(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_i
     if (vfx_status s = make_lat(coarse, C)) return s;
     if (vfx_status s = make_lat(fine, F)) return s;
     if (coarse->storage != fine->storage) return fail(VFX_EINVAL, "coarse and fine storage differ");
+    // Parameter::getUseTurbulentViscosity is one switch for every level
+    if ((coarse->turbulence_model != VFX_TURB_NONE) != (fine->turbulence_model != VFX_TURB_NONE))
+        return fail(VFX_EINVAL, "coarse and fine levels differ in using a turbulence model");
     if (!cells) return fail(VFX_EINVAL, "cells is NULL");
     if (cells->n_cells == 0) return VFX_OK;
     if (!cells->fine_cells || !cells->coarse_cells) return fail(VFX_EINVAL, "cell index arrays are NULL");

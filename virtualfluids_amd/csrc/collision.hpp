@@ -38,6 +38,7 @@ __host__ __device__ constexpr int slot_of(int d) { return cube(cx(d) + 1, cy(d) 
 // (K17CompressibleNavierStokes.h:248-273), computed once per launch.
 template <typename T> struct Rates {
     T omega, OxyyPxzz, OxyyMxzz, Oxyz, factorA, factorB;
+    T sgs; // SGSconstant of the eddy-viscosity models
     T limP, limM, limD;
     T fx, fy, fz; // half force added to the velocity (:164-166)
     // folded constants of the reduced-operation form (k17_fast)
@@ -47,11 +48,11 @@ template <typename T> struct Rates {
     T A4, B4;             // (1/omega - 1/2) * 2/3 * factorA, (1/omega - 1/2) * 1/3 * factorB
 };
 
-template <typename T> inline Rates<T> make_rates(double omega, const double lim[3], const double force[3])
+// the rates of one omega, in T with the reference's expression order (so
+// T=double matches it exactly), on the host per launch or on the device per
+// node (eddy-viscosity models)
+template <typename T> __host__ __device__ inline void omega_rates(T w, Rates<T>& r)
 {
-    // evaluated in T with the reference's expression order, so T=double matches it exactly
-    const T w = (T)omega;
-    Rates<T> r;
     r.omega = w;
     r.OxyyPxzz = T(8) * (-T(2) + w) * (T(1) + T(2) * w) / (-T(8) - T(14) * w + T(7) * w * w);
     r.OxyyMxzz = T(8) * (-T(2) + w) * (-T(7) + T(4) * w) / (T(56) - T(50) * w + T(9) * w * w);
@@ -59,6 +60,14 @@ template <typename T> inline Rates<T> make_rates(double omega, const double lim[
              (T(48) + T(152) * w - T(130) * w * w + T(29) * w * w * w);
     r.factorA = (T(4) + T(2) * w - T(3) * w * w) / (T(2) - T(7) * w + T(5) * w * w);
     r.factorB = (T(4) + T(28) * w - T(14) * w * w) / (T(6) - T(21) * w + T(15) * w * w);
+}
+
+template <typename T>
+inline Rates<T> make_rates(double omega, const double lim[3], const double force[3], double sgs = 0.0)
+{
+    Rates<T> r;
+    omega_rates<T>((T)omega, r);
+    r.sgs = (T)sgs;
     r.limP = (T)lim[0];
     r.limM = (T)lim[1];
     r.limD = (T)lim[2];
@@ -196,8 +205,38 @@ template <typename T> __device__ __forceinline__ void backward_sweeps(T (&m)[27]
 // ------------------------------------------------------------------ K17 ----
 // m[27]: populations in cube order on entry, post-collision populations on exit.
 // Returns drho and the (force-shifted) velocity in mv[4].
-template <typename T> __device__ __forceinline__ void k17(T (&m)[27], const Rates<T>& R, T (&mv)[4])
+// Eddy-viscosity models of the K17 kernel (vf::lbm::TurbulenceModel,
+// src/lbm/collision/TurbulentViscosity.h): the node relaxes with
+// omega / (1 + 3 omega nu_t) (calculateOmegaWithturbulentViscosity), and the
+// Smagorinsky and QR models replace nu_t by their value from the new velocity
+// gradients (K17CompressibleNavierStokes.h:245, 355-368).  AMD's nu_t comes
+// from a separate kernel between steps.
+enum TurbModel : int { kTurbNone = 0, kTurbSmagorinsky = 1, kTurbAMD = 2, kTurbQR = 3 };
+
+template <typename T> __device__ __forceinline__ T turbulent_viscosity_smagorinsky(T Cs, T dxux, T dyuy, T dzuz, T Dxy,
+                                                                                   T Dxz, T Dyz)
 {
+    return Cs * Cs * sqrt(T(2) * (dxux * dxux + dyuy * dyuy + dzuz * dzuz) + Dxy * Dxy + Dxz * Dxz + Dyz * Dyz);
+}
+// Verstappen's QR model: C max(R, 0) / Q with the second and third invariants
+// of the strain rate (0/0 at rest, as in the reference)
+template <typename T>
+__device__ __forceinline__ T turbulent_viscosity_qr(T C, T dxux, T dyuy, T dzuz, T Dxy, T Dxz, T Dyz)
+{
+    const T Q = T(0.5) * (dxux * dxux + dyuy * dyuy + dzuz * dzuz) + T(0.25) * (Dxy * Dxy + Dxz * Dxz + Dyz * Dyz);
+    const T Rr = -dxux * dyuy * dzuz + T(0.25) * (-Dxy * Dxz * Dyz + dxux * Dyz * Dyz + dyuy * Dxz * Dxz + dzuz * Dxy * Dxy);
+    return C * (Rr > T(0) ? Rr : T(0)) / Q;
+}
+
+template <typename T, int TM = kTurbNone>
+__device__ __forceinline__ void k17(T (&m)[27], const Rates<T>& R0, T (&mv)[4], T& nut)
+{
+    Rates<T> Rt;
+    if constexpr (TM != kTurbNone) {
+        Rt = R0;
+        omega_rates<T>(R0.omega / (T(1) + T(3) * R0.omega * nut), Rt);
+    }
+    const Rates<T>& R = TM != kTurbNone ? Rt : R0;
     constexpr T c1o3 = T(1. / 3.), c2o3 = T(2. / 3.), c1o9 = T(1. / 9.), c1o27 = T(1. / 27.), h = T(0.5);
     constexpr T c3o2 = T(3. / 2.);
 
@@ -301,6 +340,8 @@ template <typename T> __device__ __forceinline__ void k17(T (&m)[27], const Rate
     const T dxux = h * (-omega) * (mxxMyy + mxxMzz) + h * OxxPyyPzz * (m000 - mxxPyyPzz);
     const T dyuy = dxux + omega * c3o2 * mxxMyy;
     const T dzuz = dxux + omega * c3o2 * mxxMzz;
+    if constexpr (TM == kTurbSmagorinsky) nut = turbulent_viscosity_smagorinsky<T>(R.sgs, dxux, dyuy, dzuz, Dxy, Dxz, Dyz);
+    if constexpr (TM == kTurbQR) nut = turbulent_viscosity_qr<T>(R.sgs, dxux, dyuy, dzuz, Dxy, Dxz, Dyz);
 
     // second order with Galilean correction (:375-387)
     mxxPyyPzz += OxxPyyPzz * (m000 - mxxPyyPzz) - T(3) * (T(1) - h * OxxPyyPzz) * (vx2 * dxux + vy2 * dyuy + vz2 * dzuz);

@@ -24,12 +24,63 @@ static int32_t vfx_template(CollisionTemplate t)
     }
 }
 
+int32_t vfx_turbulence_model_of(TurbulenceModel m)
+{
+    switch (m) {
+    case TurbulenceModel::Smagorinsky: return VFX_TURB_SMAGORINSKY;
+    case TurbulenceModel::AMD: return VFX_TURB_AMD;
+    case TurbulenceModel::QR: return VFX_TURB_QR;
+    default: return VFX_TURB_NONE;
+    }
+}
+
+// the collision parameters the kernels take from Parameter
+static void set_collision_parameters(vfx_lattice& lat, const Parameter& para)
+{
+    const double* lim = para.getQuadricLimiters();
+    for (int i = 0; i < 3; ++i) lat.quadric_limiter[i] = lim[i];
+    lat.turbulence_model = vfx_turbulence_model_of(para.getTurbulenceModel());
+    lat.sgs_constant = para.getSGSConstant();
+}
+
+void TurbulenceModelFactory::setTurbulenceModel(TurbulenceModel m)
+{
+    turbulenceModel = m;
+    para->setTurbulenceModel(m);
+    if (m != TurbulenceModel::None) para->setUseTurbulentViscosity(true);
+}
+
+void TurbulenceModelFactory::readConfig(const std::map<std::string, std::string>& config)
+{
+    auto it = config.find("TurbulenceModel");
+    if (it != config.end()) {
+        const std::string& v = it->second;
+        if (v == "Smagorinsky") setTurbulenceModel(TurbulenceModel::Smagorinsky);
+        else if (v == "AMD") setTurbulenceModel(TurbulenceModel::AMD);
+        else if (v == "QR") setTurbulenceModel(TurbulenceModel::QR);
+        else if (v == "None") setTurbulenceModel(TurbulenceModel::None);
+        else throw std::runtime_error("TurbulenceModelFactory: Invalid turbulence model!");
+    }
+    it = config.find("SGSconstant");
+    if (it != config.end()) para->setSGSConstant(std::stod(it->second));
+}
+
+// calcTurbulentViscosityAMD (TurbulentViscosityKernels.cu:115-130) for the AMD
+// model; the other models need no kernel between steps
+void TurbulenceModelFactory::runTurbulenceModelKernel(int level) const
+{
+    if (turbulenceModel != TurbulenceModel::AMD) return;
+    LBMSimulationParameter* p = para->getParD(level);
+    vfx_lattice lat = p->lattice;
+    set_collision_parameters(lat, *para);
+    check(vfx_turbulent_viscosity_amd(&lat, p->stream), "calcTurbulentViscosityAMD");
+}
+
 void KernelImp::run()
 {
     LBMSimulationParameter* p = para->getParD(level);
     vfx_lattice lat = p->lattice;
-    const double* lim = para->getQuadricLimiters();
-    for (int i = 0; i < 3; ++i) lat.quadric_limiter[i] = lim[i];
+    set_collision_parameters(lat, *para);
     check(vfx_collide_stream(&lat, collision, p->isEvenTimestep ? 1 : 0, VFX_TEMPLATE_DEFAULT, p->stream),
           "Kernel::run");
 }
@@ -43,8 +94,7 @@ void KernelImp::runOnIndices(const unsigned int* indices, unsigned int size_indi
     if (lat.direct) throw std::runtime_error("runOnIndices needs the sparse (neighbour array) layout");
     lat.fluid_nodes = indices;
     lat.n_fluid = size_indices;
-    const double* lim = para->getQuadricLimiters();
-    for (int i = 0; i < 3; ++i) lat.quadric_limiter[i] = lim[i];
+    set_collision_parameters(lat, *para);
     check(vfx_collide_stream(&lat, collision, p->isEvenTimestep ? 1 : 0, vfx_template(collisionTemplate), p->stream),
           "Kernel::runOnIndices");
 }
@@ -134,23 +184,24 @@ static gridScaling make_scaling(bool coarse_to_fine)
     };
 }
 
-// GridScalingFactory.cpp: ScaleCompressible selects the compressible scaling
-gridScaling GridScalingFactory::getGridScalingFC(bool hasTurbulentViscosity) const
+// GridScalingFactory.cpp: ScaleCompressible selects the compressible scaling;
+// hasTurbulentViscosity selects the per-node rates, which the C ABI takes from
+// the levels' turbulence_model / turbulent_viscosity fields
+gridScaling GridScalingFactory::getGridScalingFC(bool) const
 {
     if (type != GridScaling::ScaleCompressible) return nullptr;
-    if (hasTurbulentViscosity) throw std::runtime_error("scaling with turbulent viscosity is not supported");
     return make_scaling(false);
 }
-gridScaling GridScalingFactory::getGridScalingCF(bool hasTurbulentViscosity) const
+gridScaling GridScalingFactory::getGridScalingCF(bool) const
 {
     if (type != GridScaling::ScaleCompressible) return nullptr;
-    if (hasTurbulentViscosity) throw std::runtime_error("scaling with turbulent viscosity is not supported");
     return make_scaling(true);
 }
 
 UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared_ptr<Kernel>> k,
-                           const BoundaryConditionFactory& f, const GridScalingFactory* s)
-    : para(std::move(p)), kernels(std::move(k)), velocityBC(f.getVelocityBoundaryConditionPost()),
+                           const BoundaryConditionFactory& f, const GridScalingFactory* s,
+                           std::shared_ptr<TurbulenceModelFactory> tm)
+    : tmFactory(std::move(tm)), para(std::move(p)), kernels(std::move(k)), velocityBC(f.getVelocityBoundaryConditionPost()),
       noSlipBC(f.getNoSlipBoundaryConditionPost()), slipBC(f.getSlipBoundaryConditionPost()),
       pressureBC(f.getPressureBoundaryConditionPre())
 {
@@ -158,8 +209,8 @@ UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared
     // GridScalingKernelManager.cpp:45-60
     if (para->getMaxLevel() != 0) {
         if (!s) throw std::runtime_error("There is more than one level, but no scalingFactory was provided.");
-        scalingFineToCoarse = s->getGridScalingFC(false);
-        scalingCoarseToFine = s->getGridScalingCF(false);
+        scalingFineToCoarse = s->getGridScalingFC(para->getUseTurbulentViscosity());
+        scalingCoarseToFine = s->getGridScalingCF(para->getUseTurbulentViscosity());
         for (int l = 0; l < para->getMaxLevel(); ++l) {
             if (para->getParD(l)->fineToCoarse.numberOfCells && !scalingFineToCoarse)
                 throw std::runtime_error("scalingFineToCoarse: interface cells exist but no scaling function was selected");
@@ -208,6 +259,9 @@ void UpdateGrid27::updateGrid(int level, unsigned int timestep)
         updateGrid(level + 1, timestep);
     }
     LBMSimulationParameter* p = para->getParD(level);
+    // the level's lattice carries the model for the interface interpolation
+    // (per-node rates with a turbulent viscosity)
+    set_collision_parameters(p->lattice, *para);
     auto* k = dynamic_cast<KernelImp*>(kernels.at(level).get());
     if (k && k->getKernelUsesFluidNodeIndices() && !p->lattice.direct) {
         for (CollisionTemplate tag : p->allocatedBulkFluidNodeTags) {
@@ -220,6 +274,7 @@ void UpdateGrid27::updateGrid(int level, unsigned int timestep)
     }
     postCollisionBC(level);
     p->isEvenTimestep = !p->isEvenTimestep; // UpdateGrid27.cpp:264-268
+    if (tmFactory) tmFactory->runTurbulenceModelKernel(level); // :81
     preCollisionBC(level);
     if (!finest) refinement(level);
 }

@@ -7,6 +7,7 @@
 //                                KernelFactoryImp.cpp:61-135 (string keys of KernelTypes.h:35-46)
 //   BoundaryConditionFactory     src/gpu/core/BoundaryConditions/BoundaryConditionFactory.h:50-174
 //   UpdateGrid27 step order      src/gpu/core/Calculation/UpdateGrid27.cpp:54-95, :227-307
+//   TurbulenceModelFactory       src/gpu/core/TurbulenceModels/TurbulenceModelFactory.cpp:44-100
 // Every device operation goes through include/vfx.h; errors become
 // std::runtime_error (the reference aborts the process instead).
 #pragma once
@@ -27,6 +28,8 @@ enum class CollisionTemplate { Default, WriteMacroVars, ApplyBodyForce, AllFeatu
 // src/gpu/core/Cuda/CudaStreamManager.h:41-46
 enum class CudaStreamIndex { Legacy, Bulk, SubDomainBorder, Precursor };
 enum class PreProcessorType { InitNavierStokesCompressible };
+// vf::lbm::TurbulenceModel (src/lbm/collision/TurbulentViscosity.h:50-61)
+enum class TurbulenceModel { Smagorinsky, AMD, QR, None };
 
 // Calculation.h:105-117 (plus the pressure boundary's inward neighbours)
 struct QforBoundaryConditions {
@@ -76,11 +79,42 @@ public:
     const std::string& getMainKernel() const { return mainKernel; }
     void setQuadricLimiters(double p, double m, double d) { limiters[0] = p; limiters[1] = m; limiters[2] = d; }
     const double* getQuadricLimiters() const { return limiters; }
+    // Parameter.h:486-491, 773-779: the eddy-viscosity model of the K17 kernel;
+    // each level's lattice then needs turbulent_viscosity (and, for AMD,
+    // rho/vx/vy/vz; on the sparse layout also type_of_grid_node and
+    // neighbor_inverse)
+    void setTurbulenceModel(TurbulenceModel m) { turbulenceModel = m; }
+    TurbulenceModel getTurbulenceModel() const { return turbulenceModel; }
+    void setSGSConstant(double c) { sgsConstant = c; }
+    double getSGSConstant() const { return sgsConstant; }
+    void setUseTurbulentViscosity(bool u) { useTurbulentViscosity = u; }
+    bool getUseTurbulentViscosity() const { return useTurbulentViscosity; }
 
 private:
     std::vector<LBMSimulationParameter> levels;
     std::string mainKernel = "K17CompressibleNavierStokes";
     double limiters[3] = {0.01, 0.01, 0.01};
+    TurbulenceModel turbulenceModel = TurbulenceModel::None;
+    double sgsConstant = 0.0;
+    bool useTurbulentViscosity = false;
+};
+
+int32_t vfx_turbulence_model_of(TurbulenceModel m);
+
+// TurbulenceModelFactory (TurbulenceModelFactory.cpp:44-100): selects the model
+// (Parameter), and runs the AMD viscosity kernel between steps
+class TurbulenceModelFactory {
+public:
+    explicit TurbulenceModelFactory(std::shared_ptr<Parameter> para) : para(std::move(para)) {}
+    void setTurbulenceModel(TurbulenceModel m);
+    void setModelConstant(double c) { para->setSGSConstant(c); }
+    // "TurbulenceModel" = Smagorinsky | AMD | QR | None, "SGSconstant" = number
+    void readConfig(const std::map<std::string, std::string>& config);
+    void runTurbulenceModelKernel(int level) const;
+
+private:
+    std::shared_ptr<Parameter> para;
+    TurbulenceModel turbulenceModel = TurbulenceModel::None;
 };
 
 void check(vfx_status s, const char* what);
@@ -198,16 +232,19 @@ private:
 // UpdateGrid27::updateGrid (UpdateGrid27.cpp:54-95): for a level below the
 // finest, the next finer level is updated twice first; then collision over the
 // tagged fluid nodes (or the whole box), post-collision boundary conditions,
-// parity swap, pre-collision boundary conditions and, below the finest level,
+// parity swap, the turbulence model's viscosity kernel (AMD), pre-collision
+// boundary conditions and, below the finest level,
 // fine-to-coarse then coarse-to-fine interpolation (Refinement_noExchange,
 // RefinementStrategy.cpp:169-178).
 class UpdateGrid27 {
 public:
     UpdateGrid27(std::shared_ptr<Parameter> para, std::vector<std::shared_ptr<Kernel>> kernels,
-                 const BoundaryConditionFactory& bcFactory, const GridScalingFactory* scalingFactory = nullptr);
+                 const BoundaryConditionFactory& bcFactory, const GridScalingFactory* scalingFactory = nullptr,
+                 std::shared_ptr<TurbulenceModelFactory> tmFactory = nullptr);
     void updateGrid(int level, unsigned int timestep);
 
 private:
+    std::shared_ptr<TurbulenceModelFactory> tmFactory;
     void postCollisionBC(int level);
     void preCollisionBC(int level);
     void refinement(int level);

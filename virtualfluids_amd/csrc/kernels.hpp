@@ -108,6 +108,8 @@ struct Lat {
     void* vy;
     void* vz;
     int32_t write_macro;
+    void* nut;             // turbulent viscosity per node (T), eddy-viscosity models
+    const uint32_t* nbinv; // sparse: the node in -x-y-z (AMD model)
 };
 
 template <typename S> __device__ __forceinline__ S* plane(const Lat& L, int p) { return (S*)L.f + (uint64_t)p * L.N; }
@@ -118,7 +120,7 @@ __device__ __forceinline__ bool is_solid(const Lat& L, uint32_t k)
 }
 
 // VirtualFluids node types (src/gpu/core/Calculation/Calculation.h:43-45)
-constexpr uint32_t kGeoSolid = 15, kGeoVoid = 16;
+constexpr uint32_t kGeoSolid = 15, kGeoVoid = 16, kGeoFluid = 19;
 
 // sparse layout: the node of work item i (the i-th fluid node, or node i when
 // typeOfGridNode selects), false if it does not collide
@@ -250,17 +252,34 @@ __device__ __forceinline__ void scatter_pre(const Lat& L, const uint32_t (&kk)[8
     }
 }
 
-// COLL: VFX_K17 / VFX_BGK; FAST selects the reduced-operation K17 (FP32 only);
-// OUTER2 (with FAST) leaves class-c populations scaled by 2^c for a scaling store
+// COLL: VFX_K17 / VFX_BGK in the low 4 bits, the eddy-viscosity model
+// (TurbModel) of the reference-order K17 above them; FAST selects the
+// reduced-operation K17 (FP32 only); OUTER2 (with FAST) leaves class-c
+// populations scaled by 2^c for a scaling store.  nut: the node's turbulent
+// viscosity in, the model's new value out.
+constexpr int coll_op(int coll) { return coll & 15; }
+constexpr int coll_turb(int coll) { return coll >> 4; }
 template <typename T, int COLL, bool FAST, bool OUTER2 = false>
-__device__ __forceinline__ void collide(T (&m)[27], const Rates<T>& R, T (&mv)[4])
+__device__ __forceinline__ void collide(T (&m)[27], const Rates<T>& R, T (&mv)[4], T& nut)
 {
-    if constexpr (COLL == VFX_K17) {
-        if constexpr (FAST && sizeof(T) == 4) k17_fast<OUTER2>(m, R, mv);
-        else k17<T>(m, R, mv);
+    if constexpr (coll_op(COLL) == VFX_K17) {
+        if constexpr (FAST && sizeof(T) == 4 && coll_turb(COLL) == kTurbNone) k17_fast<OUTER2>(m, R, mv);
+        else k17<T, coll_turb(COLL)>(m, R, mv, nut);
     } else {
         bgk<T>(m, R.omega, mv);
     }
+}
+// runCollision's handling of the turbulent viscosity (RunCollision.cuh:116-141):
+// read before the collision, stored after it; the AMD model needs the
+// velocities of every node
+template <typename T, int COLL> __device__ __forceinline__ T load_nut(const Lat& L, uint32_t k)
+{
+    if constexpr (coll_turb(COLL) != kTurbNone) return ((const T*)L.nut)[k];
+    return T(0);
+}
+template <typename T, int COLL> __device__ __forceinline__ void store_nut(const Lat& L, uint32_t k, T nut)
+{
+    if constexpr (coll_turb(COLL) == kTurbSmagorinsky || coll_turb(COLL) == kTurbQR) ((T*)L.nut)[k] = nut;
 }
 
 template <typename T> __device__ __forceinline__ void write_macro(const Lat& L, uint32_t k, const T (&mv)[4])
@@ -307,10 +326,12 @@ __global__ void __launch_bounds__(256) k_step_direct(Lat L, Rates<typename Compu
     if (is_solid(L, kk[0])) return;
     T m[27], mv[4];
     constexpr bool O2 = FAST && COLL == VFX_K17 && sizeof(S) == 2;
+    T nut = load_nut<T, COLL>(L, kk[0]);
     gather<S, T, EVEN>(L, kk, m);
-    collide<T, COLL, FAST, O2>(m, R, mv);
+    collide<T, COLL, FAST, O2>(m, R, mv, nut);
     scatter_post<S, T, EVEN, O2>(L, kk, m);
     if (L.write_macro) write_macro<T>(L, kk[0], mv);
+    store_nut<T, COLL>(L, kk[0], nut);
 }
 
 // VirtualFluids sparse addressing over the fluid-node list
@@ -325,10 +346,12 @@ __global__ void __launch_bounds__(256) k_step_sparse(Lat L, Rates<typename Compu
     nbrs_sparse(L, k, kk);
     T m[27], mv[4];
     constexpr bool O2 = FAST && COLL == VFX_K17 && sizeof(S) == 2;
+    T nut = load_nut<T, COLL>(L, kk[0]);
     gather<S, T, EVEN>(L, kk, m);
-    collide<T, COLL, FAST, O2>(m, R, mv);
+    collide<T, COLL, FAST, O2>(m, R, mv, nut);
     scatter_post<S, T, EVEN, O2>(L, kk, m);
     if (L.write_macro) write_macro<T>(L, kk[0], mv);
+    store_nut<T, COLL>(L, kk[0], nut);
 }
 
 // ------------------------------------------------------------ node loops ----
@@ -983,14 +1006,14 @@ __global__ void __launch_bounds__(256)
         L.nz = D.nz;
         uint32_t kk[8];
         nbrs_interior(L, x, y, z, kk);
-        T m[27], mv[4];
+        T m[27], mv[4], nut = T(0);
 #pragma unroll
         for (int s = 0; s < 27; ++s) {
             const int d = dir_at(s);
             m[s] = Storage<S, T>::ld(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], B.ld[weight_class(d)]);
         }
         constexpr bool O2 = FAST && sizeof(S) == 2; // class-c populations come out as 2^c f
-        collide<T, VFX_K17, FAST, O2>(m, D.R, mv);
+        collide<T, VFX_K17, FAST, O2>(m, D.R, mv, nut);
         // which faces of the box the node touches (lane masks: scalar logic)
         const bool xl = x == 1, xh = x == L.nx - 2, yl = y == 1, yh = y == L.ny - 2, zl = z == 1, zh = z == L.nz - 2;
         // one buffer descriptor over the domain's planes: a lane's byte offset
@@ -1091,6 +1114,81 @@ __global__ void __launch_bounds__(256) k_import(Lat L, const O* in)
     }
 }
 
+// ------------------------------------------------------- AMD viscosity ----
+// Anisotropic minimum-dissipation eddy viscosity of every fluid node from the
+// velocity fields the collision wrote (calcAMD,
+// src/gpu/core/PostProcessor/TurbulentViscosityKernels.cu:62-113): central
+// differences where both neighbours along an axis are fluid, one-sided where
+// one is, zero where none is; nu_t = max(0, -C * numerator) / |grad u|^2
+// (0 where the gradient vanishes).  Sparse layout: the -x/-y/-z neighbours
+// come from the -x-y-z node (neighborInverse) and two + steps, fluid means
+// typeOfGridNode == GEO_FLUID; direct layout: periodic index wrap, fluid =
+// not solid.  Expression order as the reference's (FP64 reproduces it).
+template <typename T>
+__device__ __forceinline__ void amd_derivatives(uint32_t k, uint32_t km, uint32_t kp, bool fm, bool fp, const T* vx,
+                                                const T* vy, const T* vz, T& dvx, T& dvy, T& dvz)
+{
+    const T div = (fm && fp) ? T(0.5) : T(1);
+    dvx = ((fp ? vx[kp] : vx[k]) - (fm ? vx[km] : vx[k])) * div;
+    dvy = ((fp ? vy[kp] : vy[k]) - (fm ? vy[km] : vy[k])) * div;
+    dvz = ((fp ? vz[kp] : vz[k]) - (fm ? vz[km] : vz[k])) * div;
+}
+
+template <typename T> __global__ void __launch_bounds__(256) k_turb_amd(Lat L, int direct, T C)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.N) return;
+    const uint32_t k = (uint32_t)i;
+    uint32_t kp[3], km[3];
+    bool fp[3], fm[3];
+    if (direct) {
+        if (is_solid(L, k)) return;
+        const uint32_t x = k % L.nx, r = k / L.nx, y = r % L.ny, z = r / L.ny;
+        const uint32_t sy = L.nx, sz = L.nx * L.ny;
+        kp[0] = x + 1 == L.nx ? k - x : k + 1;
+        km[0] = x == 0 ? k + L.nx - 1 : k - 1;
+        kp[1] = y + 1 == L.ny ? k - y * sy : k + sy;
+        km[1] = y == 0 ? k + (L.ny - 1) * sy : k - sy;
+        kp[2] = z + 1 == L.nz ? k - z * sz : k + sz;
+        km[2] = z == 0 ? k + (L.nz - 1) * sz : k - sz;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            fp[a] = !is_solid(L, kp[a]);
+            fm[a] = !is_solid(L, km[a]);
+        }
+    } else {
+        if (L.types[k] != kGeoFluid) return;
+        kp[0] = L.nbx[k];
+        kp[1] = L.nby[k];
+        kp[2] = L.nbz[k];
+        const uint32_t w = L.nbinv[k];
+        km[0] = L.nbz[L.nby[w]];
+        km[1] = L.nbz[L.nbx[w]];
+        km[2] = L.nby[L.nbx[w]];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            fp[a] = L.types[kp[a]] == kGeoFluid;
+            fm[a] = L.types[km[a]] == kGeoFluid;
+        }
+    }
+    const T* vx = (const T*)L.vx;
+    const T* vy = (const T*)L.vy;
+    const T* vz = (const T*)L.vz;
+    T dvxdx, dvxdy, dvxdz, dvydx, dvydy, dvydz, dvzdx, dvzdy, dvzdz;
+    amd_derivatives<T>(k, km[0], kp[0], fm[0], fp[0], vx, vy, vz, dvxdx, dvydx, dvzdx);
+    amd_derivatives<T>(k, km[1], kp[1], fm[1], fp[1], vx, vy, vz, dvxdy, dvydy, dvzdy);
+    amd_derivatives<T>(k, km[2], kp[2], fm[2], fp[2], vx, vy, vz, dvxdz, dvydz, dvzdz);
+    const T den = dvxdx * dvxdx + dvydx * dvydx + dvzdx * dvzdx + dvxdy * dvxdy + dvydy * dvydy + dvzdy * dvzdy +
+                  dvxdz * dvxdz + dvydz * dvydz + dvzdz * dvzdz;
+    const T num = (dvxdx * dvxdx + dvxdy * dvxdy + dvxdz * dvxdz) * dvxdx +
+                  (dvydx * dvydx + dvydy * dvydy + dvydz * dvydz) * dvydy +
+                  (dvzdx * dvzdx + dvzdy * dvzdy + dvzdz * dvzdz) * dvzdz +
+                  (dvxdx * dvydx + dvxdy * dvydy + dvxdz * dvydz) * (dvxdy + dvydx) +
+                  (dvxdx * dvzdx + dvxdy * dvzdy + dvxdz * dvzdz) * (dvxdz + dvzdx) +
+                  (dvydx * dvzdx + dvydy * dvzdy + dvydz * dvzdz) * (dvydz + dvzdy);
+    ((T*)L.nut)[k] = den != T(0) ? fmax(T(0), -C * num) / den : T(0);
+}
+
 // ------------------------------------------------------------ launchers ----
 constexpr int kBlock = 256;
 
@@ -1112,13 +1210,29 @@ hipError_t launch_step_t(const Lat& L, int direct, const Rates<typename Compute<
     return hipGetLastError();
 }
 
-// variant: 0 = default (FP64: reference operation order; FP32/FP16: reduced form), 1 = reference order
+template <typename S, int COLL>
+hipError_t launch_step_p(const Lat& L, int direct, bool even, const Rates<typename Compute<S>::T>& R, hipStream_t st)
+{
+    return even ? launch_step_t<S, COLL, true, false>(L, direct, R, st)
+                : launch_step_t<S, COLL, false, false>(L, direct, R, st);
+}
+
+// variant: 0 = default (FP64: reference operation order; FP32/FP16: reduced form), 1 = reference order;
+// turb: TurbModel of the K17 collision (reference operation order in every variant)
 template <typename S> hipError_t launch_step(const Lat& L, int direct, int coll, bool even, int variant, double omega,
-                                             const double* lim, const double* force, hipStream_t st)
+                                             const double* lim, const double* force, int turb, double sgs,
+                                             hipStream_t st)
 {
     using T = typename Compute<S>::T;
-    const Rates<T> R = make_rates<T>(omega, lim, force);
+    const Rates<T> R = make_rates<T>(omega, lim, force, sgs);
     const bool fast = sizeof(T) == 4 && variant == 0 && coll == VFX_K17;
+    if (coll == VFX_K17 && turb != kTurbNone) {
+        switch (turb) {
+        case kTurbSmagorinsky: return launch_step_p<S, VFX_K17 | (kTurbSmagorinsky << 4)>(L, direct, even, R, st);
+        case kTurbAMD: return launch_step_p<S, VFX_K17 | (kTurbAMD << 4)>(L, direct, even, R, st);
+        default: return launch_step_p<S, VFX_K17 | (kTurbQR << 4)>(L, direct, even, R, st);
+        }
+    }
     if (coll == VFX_K17) {
         if (fast) return even ? launch_step_t<S, VFX_K17, true, true>(L, direct, R, st)
                               : launch_step_t<S, VFX_K17, false, true>(L, direct, R, st);
@@ -1127,6 +1241,13 @@ template <typename S> hipError_t launch_step(const Lat& L, int direct, int coll,
     }
     return even ? launch_step_t<S, VFX_BGK, true, false>(L, direct, R, st)
                 : launch_step_t<S, VFX_BGK, false, false>(L, direct, R, st);
+}
+
+template <typename S> hipError_t launch_turb_amd(const Lat& L, int direct, double sgs, hipStream_t st)
+{
+    using T = typename Compute<S>::T;
+    hipLaunchKernelGGL((k_turb_amd<T>), dim3(grid_for(L.N, kBlock)), dim3(kBlock), 0, st, L, direct, (T)sgs);
+    return hipGetLastError();
 }
 
 template <typename S> hipError_t launch_init(const Lat& L, int direct, bool even, const double* drho, const double* ux,
@@ -1270,8 +1391,9 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
 }
 
 #define VFX_INSTANTIATE_LAUNCHERS(S)                                                                                   \
-    template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*,           \
-                                       hipStream_t);                                                                   \
+    template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*, int,      \
+                                       double, hipStream_t);                                                           \
+    template hipError_t launch_turb_amd<S>(const Lat&, int, double, hipStream_t);                                      \
     template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,             \
                                        const double*, hipStream_t);                                                    \
     template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);                      \
@@ -1287,7 +1409,8 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
 
 #define VFX_EXTERN_LAUNCHERS(S)                                                                                        \
     extern template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*,   \
-                                              hipStream_t);                                                            \
+                                              int, double, hipStream_t);                                               \
+    extern template hipError_t launch_turb_amd<S>(const Lat&, int, double, hipStream_t);                               \
     extern template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,      \
                                               const double*, hipStream_t);                                             \
     extern template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);               \

@@ -235,6 +235,15 @@ __device__ __forceinline__ void destination(T drho, T ux, T uy, T uz, const T (&
     backward_sweeps(m, ux, uy, uz);
 }
 
+// hasTurbulentViscosity: a level with an eddy-viscosity model relaxes node k at
+// omega / (1 + 3 omega nu_t[k]) (calculateOmegaWithturbulentViscosity), in the
+// source moments of each corner and at each destination node
+// (scaleCoarseToFineCompressible.cu:94, 369; scaleFineToCoarseCompressible.cu:64)
+template <typename T> __device__ __forceinline__ T node_omega(const Lat& L, uint32_t k, T omega)
+{
+    return L.nut ? omega / (T(1) + T(3) * omega * ((const T*)L.nut)[k]) : omega;
+}
+
 // moments of this lane's source node into LDS, polynomial of the cell
 template <typename S, bool EVEN_SRC>
 __device__ __forceinline__ void load_cell(const Lat& L, int direct, uint32_t base, int lane, bool active,
@@ -249,7 +258,7 @@ __device__ __forceinline__ void load_cell(const Lat& L, int direct, uint32_t bas
         T m[27];
         gather<S, T, EVEN_SRC>(L, kk, m);
         T s[kMoments];
-        source_moments(m, omega, s);
+        source_moments(m, node_omega<T>(L, k, omega), s);
 #pragma unroll
         for (int q = 0; q < kMoments; ++q) sh[lane * kMoments + q] = s[q];
     }
@@ -286,9 +295,9 @@ __global__ void __launch_bounds__(256) k_scale_cf(Lat C, Lat F, ICellsDev cells,
     poly_gradient(k.b, x, y, z, gy);
     poly_gradient(k.c, x, y, z, gz);
     T m[27];
-    destination(drho, poly_value(k.a, x, y, z), poly_value(k.b, x, y, z), poly_value(k.c, x, y, z), gx, gy, gz,
-                T(0.5), (T)omega_f, m);
     const uint32_t kf = cell_corner(F, cells.direct_f, cells.fine[cell], lane);
+    destination(drho, poly_value(k.a, x, y, z), poly_value(k.b, x, y, z), poly_value(k.c, x, y, z), gx, gy, gz,
+                T(0.5), node_omega<T>(F, kf, (T)omega_f), m);
     uint32_t kk[8];
     if (cells.direct_f) nbrs_direct(F, kf, kk);
     else nbrs_sparse(F, kf, kk);
@@ -321,8 +330,8 @@ __global__ void __launch_bounds__(256) k_scale_fc(Lat C, Lat F, ICellsDev cells,
     const T gy[3] = {k.b[P100], k.b[P010], k.b[P001]};
     const T gz[3] = {k.c[P100], k.c[P010], k.c[P001]};
     T m[27];
-    destination(drho, k.a[P000], k.b[P000], k.c[P000], gx, gy, gz, T(2), (T)omega_c, m);
     const uint32_t kc = cells.coarse[cell];
+    destination(drho, k.a[P000], k.b[P000], k.c[P000], gx, gy, gz, T(2), node_omega<T>(C, kc, (T)omega_c), m);
     uint32_t kk[8];
     if (cells.direct_c) nbrs_direct(C, kc, kk);
     else nbrs_sparse(C, kc, kk);

@@ -20,6 +20,9 @@ STORAGE = {"fp64": V.VFX_FP64, "fp32": V.VFX_FP32, "fp16": V.VFX_FP16_SCALED}
 STORAGE_DTYPE = {"fp64": torch.float64, "fp32": torch.float32, "fp16": torch.float16}
 BYTES = {"fp64": 8, "fp32": 4, "fp16": 2}
 COLLISION = {"k17": V.VFX_K17, "bgk": V.VFX_BGK}
+# vf::lbm::TurbulenceModel of the K17 collision
+TURBULENCE = {"none": V.VFX_TURB_NONE, "smagorinsky": V.VFX_TURB_SMAGORINSKY, "amd": V.VFX_TURB_AMD,
+              "qr": V.VFX_TURB_QR}
 
 # D3Q27 velocity set in VirtualFluids order (src/lbm/constants/D3Q27.h:42-72)
 CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1])
@@ -59,7 +62,7 @@ class Lattice:
     def __init__(self, dims=None, *, storage="fp64", omega=1.0, limiter=(0.01, 0.01, 0.01),
                  force=(0.0, 0.0, 0.0), solid=None, fp16_scale=None, neighbors=None, fluid_nodes=None,
                  n_nodes=None, device="cuda", variant="default", reference_velocity=(0.0, 0.0, 0.0),
-                 node_types=None):
+                 node_types=None, turbulence="none", sgs_constant=0.0, neighbor_inverse=None):
         if storage not in STORAGE:
             raise ValueError(f"storage must be one of {sorted(STORAGE)}")
         self.storage = storage
@@ -120,6 +123,33 @@ class Lattice:
         self.f = torch.zeros(27 * self.n_nodes, dtype=STORAGE_DTYPE[storage], device=self.device)
         self.macro = None
         self._struct = None
+        # eddy-viscosity model of the K17 collision (vf::lbm::TurbulenceModel):
+        # the turbulent viscosity of every node starts at 0, as the reference's
+        # zero-filled turbViscosity array
+        if turbulence not in TURBULENCE:
+            raise ValueError(f"turbulence must be one of {sorted(TURBULENCE)}")
+        self.turbulence = turbulence
+        self.sgs_constant = float(sgs_constant)
+        self.nut = None
+        self.nbinv = None
+        if turbulence != "none":
+            self.nut = torch.zeros(self.n_nodes, dtype=self._macro_dtype(), device=self.device)
+        if turbulence == "amd":
+            self._alloc_macro()
+            if not self.direct:
+                if node_types is None or neighbor_inverse is None:
+                    raise ValueError("the AMD model on the sparse layout needs node_types and neighbor_inverse")
+                inv = np.ascontiguousarray(neighbor_inverse, dtype=np.uint32).ravel()
+                if inv.size != self.n_nodes or (inv.size and inv.max() >= self.n_nodes):
+                    raise ValueError("neighbor_inverse needs n_nodes entries in range")
+                self.nbinv = torch.from_numpy(inv.view(np.int32)).to(self.device)
+
+    def _macro_dtype(self):
+        return torch.float64 if self.storage == "fp64" else torch.float32
+
+    def _alloc_macro(self):
+        if self.macro is None:
+            self.macro = [torch.zeros(self.n_nodes, dtype=self._macro_dtype(), device=self.device) for _ in range(4)]
 
     # ------------------------------------------------------------ C view --
     def struct(self) -> V.VfxLattice:
@@ -145,6 +175,11 @@ class Lattice:
             s.rho, s.vx, s.vy, s.vz = (t.data_ptr() for t in self.macro)
         s.kernel_variant = 0 if self.variant == "default" else 1
         s.fp16_reference_velocity[:] = list(getattr(self, "reference_velocity", (0.0, 0.0, 0.0)))
+        if getattr(self, "nut", None) is not None:
+            s.turbulence_model = TURBULENCE[self.turbulence]
+            s.sgs_constant = self.sgs_constant
+            s.turbulent_viscosity = self.nut.data_ptr()
+            s.neighbor_inverse = _ptr(self.nbinv)
         return s
 
     @property
@@ -164,13 +199,20 @@ class Lattice:
                                              _stream(stream)), "vfx_init_equilibrium")
 
     def collide_stream(self, collision="k17", even=True, write_macro=False, stream=None):
-        if write_macro and self.macro is None:
-            dt = torch.float64 if self.storage == "fp64" else torch.float32
-            self.macro = [torch.zeros(self.n_nodes, dtype=dt, device=self.device) for _ in range(4)]
+        if write_macro:
+            self._alloc_macro()
         s = self.struct()
         V.check(V.lib().vfx_collide_stream(C.byref(s), COLLISION[collision], int(bool(even)),
                                            V.VFX_TEMPLATE_WRITE_MACRO if write_macro else V.VFX_TEMPLATE_DEFAULT,
                                            _stream(stream)), "vfx_collide_stream")
+
+    def turbulent_viscosity_amd(self, stream=None):
+        """calcTurbulentViscosityAMD: the AMD eddy viscosity of every fluid
+        node from the velocities the last collision wrote."""
+        if self.nut is None or self.macro is None:
+            raise ValueError("lattice has no turbulent viscosity / velocity fields")
+        s = self.struct()
+        V.check(V.lib().vfx_turbulent_viscosity_amd(C.byref(s), _stream(stream)), "vfx_turbulent_viscosity_amd")
 
     def apply_bc(self, bc: "BoundaryCondition", even=True, stream=None):
         if bc.max_node >= self.n_nodes:
@@ -360,8 +402,9 @@ class BoundaryCondition:
 class Stepper:
     """The per-level time step of UpdateGrid27::updateGrid without refinement
     (src/gpu/core/Calculation/UpdateGrid27.cpp:54-95): collide+stream, the
-    post-collision boundary conditions, the parity flip, then the
-    pre-collision boundary conditions with the new parity."""
+    post-collision boundary conditions, the parity flip, the AMD viscosity
+    kernel when the lattice uses that model, then the pre-collision boundary
+    conditions with the new parity."""
 
     lattice: Lattice
     collision: str = "k17"
@@ -376,6 +419,8 @@ class Stepper:
             for bc in post:
                 self.lattice.apply_bc(bc, self.even, stream=stream)
             self.even = not self.even
+            if self.collision == "k17" and getattr(self.lattice, "turbulence", "none") == "amd":
+                self.lattice.turbulent_viscosity_amd(stream=stream)
             for bc in pre:
                 self.lattice.apply_bc(bc, self.even, stream=stream)
 

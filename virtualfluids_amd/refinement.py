@@ -230,12 +230,15 @@ class RefinedStepper:
     """One coarse step = two fine steps, one coarse step, fine-to-coarse, then
     coarse-to-fine, on a NestedBox (both levels direct-addressed boxes;
     inactive coarse nodes masked).  Each level's step is collide+stream, its
-    post-collision boundary conditions, the parity flip and its pre-collision
-    boundary conditions (UpdateGrid27.cpp:54-95)."""
+    post-collision boundary conditions, the parity flip, the AMD viscosity
+    kernel (AMD model) and its pre-collision boundary conditions
+    (UpdateGrid27.cpp:54-95).  A turbulence model applies to both levels with
+    one SGS constant; the interface then uses per-node rates
+    (hasTurbulentViscosity)."""
 
     def __init__(self, nest: NestedBox, omega_coarse, storage="fp64", collision="k17", device="cuda",
                  fp16_scale=None, variant="default", coarse_solid=None, fine_solid=None, coarse_bcs=(),
-                 fine_bcs=(), reference_velocity=(0.0, 0.0, 0.0)):
+                 fine_bcs=(), reference_velocity=(0.0, 0.0, 0.0), turbulence="none", sgs_constant=0.0):
         self.nest = nest
         self.collision = collision
         self.omega_c = float(omega_coarse)
@@ -243,11 +246,12 @@ class RefinedStepper:
         cmask = nest.coarse_state != 0
         if coarse_solid is not None:
             cmask = cmask | np.asarray(coarse_solid, bool).ravel()
+        turb = dict(turbulence=turbulence, sgs_constant=sgs_constant)
         self.coarse = Lattice(nest.coarse_dims, storage=storage, omega=self.omega_c, solid=cmask, device=device,
-                              fp16_scale=fp16_scale, variant=variant, reference_velocity=reference_velocity)
+                              fp16_scale=fp16_scale, variant=variant, reference_velocity=reference_velocity, **turb)
         self.fine = Lattice(nest.fine_dims, storage=storage, omega=self.omega_f, device=device,
                             solid=None if fine_solid is None else np.asarray(fine_solid, bool).ravel(),
-                            fp16_scale=fp16_scale, variant=variant, reference_velocity=reference_velocity)
+                            fp16_scale=fp16_scale, variant=variant, reference_velocity=reference_velocity, **turb)
         self.fc = ICells(nest.fc_coarse, nest.fc_fine, device, offsets=nest.fc_offset)
         self.cf = ICells(nest.cf_coarse, nest.cf_fine, device, offsets=nest.cf_offset)
         self.coarse_bcs = list(coarse_bcs)
@@ -261,6 +265,8 @@ class RefinedStepper:
         for bc in bcs:
             if not bc.pre_collision:
                 lat.apply_bc(bc, even, stream=stream)
+        if collision == "k17" and lat.turbulence == "amd":
+            lat.turbulent_viscosity_amd(stream=stream)
         for bc in bcs:
             if bc.pre_collision:
                 lat.apply_bc(bc, not even, stream=stream)
