@@ -136,8 +136,15 @@ typedef enum {
     VFX_BC_NOSLIP_INTERPOLATED = 1,   /* NoSlipInterpolatedCompressible_Device, NoSlip/NoSlipInterpolatedCompressible.cu:44 */
     VFX_BC_VELOCITY_INTERPOLATED = 2, /* VelocityInterpolatedCompressible_Device, Velocity/VelocityInterpolatedCompressible.cu:44 */
     VFX_BC_SLIP = 3,                  /* SlipCompressible_Device, Slip/SlipCompressible.cu:45 */
-    VFX_BC_PRESSURE_NEQ = 4           /* PressureNonEquilibriumCompressible_Device, Pressure/PressureNonEquilibriumCompressible.cu:45
+    VFX_BC_PRESSURE_NEQ = 4,          /* PressureNonEquilibriumCompressible_Device, Pressure/PressureNonEquilibriumCompressible.cu:45
                                          (pre-collision: apply with the parity of the step it precedes) */
+    VFX_BC_VELOCITY_PRESSURE_INTERPOLATED = 5, /* VelocityWithPressureInterpolatedCompressible_Device,
+                                                  Velocity/VelocityWithPressureInterpolatedCompressible.cu:44 */
+    VFX_BC_OUTFLOW_NON_REFLECTING = 6, /* OutflowNonReflecting_Device, Outflow/OutflowNonReflecting.cu:50 (pre-collision;
+                                          k_neighbor, normal) */
+    VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7 /* OutflowNonReflectingPressureCorrection_Device,
+                                          Outflow/OutflowNonReflectingPressureCorrection.cu:51 (pre-collision;
+                                          k_neighbor, normal, pressure_correction_factor) */
 } vfx_bc_kind;
 
 /* QforBoundaryConditions (src/gpu/core/Calculation/Calculation.h:105-117) */
@@ -151,8 +158,12 @@ typedef struct vfx_bc {
     const double* vz;
     int32_t cpu_float_velocity; /* 1: wall term rounded as VirtualFluids' CPU BoundaryConditions does (float storage) */
     const double* rho;         /* VFX_BC_PRESSURE_NEQ: prescribed density deviation per boundary node (RhoBC) */
-    const uint32_t* k_neighbor;/* VFX_BC_PRESSURE_NEQ: inward neighbour node of each boundary node */
-    int32_t normal;            /* VFX_BC_PRESSURE_NEQ: outward direction of the boundary (e.g. 1 = dP00) */
+    const uint32_t* k_neighbor;/* VFX_BC_PRESSURE_NEQ / outflow: inward neighbour node of each boundary node */
+    int32_t normal;            /* VFX_BC_PRESSURE_NEQ / outflow: outward direction of the boundary (e.g. 1 = dP00);
+                                  the outflow kinds act on the six face directions only */
+    /* VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION: Parameter::outflowPressureCorrectionFactor
+       (Parameter.h:761, default 0) */
+    double pressure_correction_factor;
 } vfx_bc;
 
 /* ---------------------------------------------------------------- kernels */

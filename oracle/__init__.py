@@ -30,7 +30,7 @@ class OrcBC(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n", C.c_int32), ("node", C.POINTER(C.c_int64)),
                 ("q", C.POINTER(C.c_float)), ("ux", _dp), ("uy", _dp), ("uz", _dp),
                 ("cpu_float_velocity", C.c_int32), ("rho", _dp), ("nbr", C.POINTER(C.c_int64)),
-                ("normal", C.c_int32)]
+                ("normal", C.c_int32), ("correction", C.c_double)]
 
 
 def build(ref: bool | None = None) -> None:
@@ -67,6 +67,10 @@ def lib():
         L.orc_run_turb.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _u8p, C.c_double, _dp, _dp,
                                    C.c_int, C.POINTER(OrcBC), C.c_int, _dp, C.c_int, C.c_double, _dp]
         L.orc_run_turb.restype = C.c_int
+        _i32p = C.POINTER(C.c_int32)
+        L.orc_interp_bc.argtypes = [C.c_int, _dp, C.POINTER(C.c_float), C.c_double, C.c_double, C.c_double,
+                                    C.c_double, _dp, _i32p]
+        L.orc_outflow_bc.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, _i32p]
         L.orc_k17_collide_turb.argtypes = [_dp, C.c_double, _dp, _dp, _dp, C.c_int, C.c_double, _dp]
         L.orc_turbulent_viscosity_amd.argtypes = [C.c_int, C.c_int, C.c_int, _u8p, _dp, _dp, _dp, C.c_double, _dp]
         L.orc_num_threads.restype = C.c_int
@@ -148,6 +152,32 @@ def k17_collide_turb(f27, omega, turb, sgs, nut, lim=LIMITER, force=(0.0, 0.0, 0
     return f, mv, float(nu[0])
 
 
+def interp_bc(kind, p27, q27, omega, u=(0.0, 0.0, 0.0)):
+    """One boundary node of an interpolated kind (1, 2, 3, 5): post-collision
+    populations p27, link fractions q27 -> (values, written mask) indexed by the
+    direction written (the opposite of the wall link)."""
+    p = np.ascontiguousarray(p27, dtype=np.float64)
+    q = np.ascontiguousarray(q27, dtype=np.float32)
+    out = np.zeros(27)
+    wr = np.zeros(27, np.int32)
+    lib().orc_interp_bc(int(kind), _p(p), q.ctypes.data_as(C.POINTER(C.c_float)), float(omega),
+                        *(float(x) for x in u), _p(out), wr.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out, wr.astype(bool)
+
+
+def outflow_bc(kind, f27, fn27, normal, correction=0.0):
+    """One node of the non-reflecting outflow (kind 6, or 7 with pressure
+    correction): pre-collision f27 of the node and fn27 of its inward
+    neighbour -> (entering populations, written mask)."""
+    f = np.ascontiguousarray(f27, dtype=np.float64)
+    fn = np.ascontiguousarray(fn27, dtype=np.float64)
+    out = np.zeros(27)
+    wr = np.zeros(27, np.int32)
+    lib().orc_outflow_bc(int(kind), _p(f), _p(fn), int(normal), float(correction), _p(out),
+                         wr.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out, wr.astype(bool)
+
+
 def turbulent_viscosity_amd(dims, vx, vy, vz, C_amd, solid=None):
     """AMD eddy viscosity of every non-solid node of a dense periodic box."""
     nx, ny, nz = dims
@@ -210,10 +240,13 @@ class BCGroup:
     """Boundary group in the VirtualFluids QforBoundaryConditions shape: node
     indices, q[27][n] (link d is a wall link iff 0<=q<=1), per-node wall
     velocity; kind 0 velocity bounce-back, 1 no-slip interpolated, 2 velocity
-    interpolated, 3 slip, 4 non-equilibrium pressure (rho, nbr, normal)."""
+    interpolated, 3 slip, 4 non-equilibrium pressure (rho, nbr, normal),
+    5 velocity interpolated with pressure, 6 non-reflecting outflow (nbr,
+    normal), 7 non-reflecting outflow with pressure correction (nbr, normal,
+    correction)."""
 
     def __init__(self, node, q=None, ux=None, uy=None, uz=None, cpu_float_velocity=False, kind=0, rho=None,
-                 nbr=None, normal=0):
+                 nbr=None, normal=0, correction=0.0):
         self.node = np.ascontiguousarray(node, dtype=np.int64)
         n = self.node.size
         self.q = np.ascontiguousarray(np.full((27, n), -1.0) if q is None else q, dtype=np.float32)
@@ -225,6 +258,7 @@ class BCGroup:
         self.rho = np.ascontiguousarray(z if rho is None else np.broadcast_to(rho, (n,)), dtype=np.float64)
         self.nbr = np.ascontiguousarray(self.node if nbr is None else nbr, dtype=np.int64)
         self.normal = int(normal)
+        self.correction = float(correction)
         self.cpu_float_velocity = bool(cpu_float_velocity)
         self.kind = kind
 
@@ -232,7 +266,7 @@ class BCGroup:
         return OrcBC(self.kind, self.node.size, self.node.ctypes.data_as(C.POINTER(C.c_int64)),
                      self.q.ctypes.data_as(C.POINTER(C.c_float)), _p(self.ux), _p(self.uy), _p(self.uz),
                      int(self.cpu_float_velocity), _p(self.rho), self.nbr.ctypes.data_as(C.POINTER(C.c_int64)),
-                     self.normal)
+                     self.normal, self.correction)
 
 
 def run(F, dims, nsteps, omega, collision="k17", solid=None, bcs=(), lim=LIMITER, force=(0.0, 0.0, 0.0),

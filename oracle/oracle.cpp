@@ -50,7 +50,10 @@ enum {
     ORC_BC_NOSLIP_INTERPOLATED = 1,
     ORC_BC_VELOCITY_INTERPOLATED = 2,
     ORC_BC_SLIP = 3,
-    ORC_BC_PRESSURE_NEQ = 4
+    ORC_BC_PRESSURE_NEQ = 4,
+    ORC_BC_VELOCITY_PRESSURE_INTERPOLATED = 5,
+    ORC_BC_OUTFLOW_NON_REFLECTING = 6,
+    ORC_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7
 };
 
 namespace {
@@ -591,9 +594,39 @@ void interp_bc(int kind, const double* p, const float* q, int n, int i, double o
                 vbc = proj(d, VX, VY, VZ);
             }
             v = a + (qd * (f + fi) - 6.0 * w * vbc) / (1.0 + qd);
+            // getInterpolatedDistributionForVeloWithPressureBC (KernelUtilities.h:207-213)
+            if (kind == ORC_BC_VELOCITY_PRESSURE_INTERPOLATED) v = v - w * drho;
         }
         out[T.inv[d]] = v;
         written[T.inv[d]] = true;
+    }
+}
+
+// Non-reflecting outflow (OutflowNonReflecting.cu:44-280,
+// OutflowNonReflectingPressureCorrection.cu:46-245): from the pre-collision
+// populations f of the boundary node and fn of its inward neighbour, the
+// population entering through the boundary in every direction d with
+// c_d . normal < 0 (normal one of the six face directions) becomes
+//   fn[d] cs + (1 - cs) f[d]  [- w_d * factor * drho(f)],  cs = 1 / sqrtf(3).
+// The pressure-corrected kernel reads its "neighbour" populations of the
+// directions with a negative component from the boundary node's own slots
+// (its neighbour indices are built from the boundary node), so for those d
+// fn[d] is f[d]; restated as is.  Returns the values in out[d], written[d].
+void outflow_bc(int kind, const double* f, const double* fn_true, int normal, double factor, double* out,
+                bool* written)
+{
+    if (std::abs(CX[normal]) + std::abs(CY[normal]) + std::abs(CZ[normal]) != 1) return;
+    const double cs = 1.0 / (double)std::sqrt(3.0f);
+    const bool corr = kind == ORC_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION;
+    const double rc = corr ? factor * density(f) : 0.0;
+    for (int d = 1; d < 27; ++d) {
+        if (CX[d] * CX[normal] + CY[d] * CY[normal] + CZ[d] * CZ[normal] >= 0) continue;
+        const bool own = corr && (CX[d] < 0 || CY[d] < 0 || CZ[d] < 0);
+        const double fn = own ? f[d] : fn_true[d];
+        double v = fn * cs + (1.0 - cs) * f[d];
+        if (corr) v = v - weight(d) * rc;
+        out[d] = v;
+        written[d] = true;
     }
 }
 
@@ -884,15 +917,39 @@ typedef struct {
     const double* uz;
     int32_t cpu_float_velocity;
     const double* rho;        // pressure: prescribed density deviation
-    const int64_t* nbr;       // pressure: inward neighbour node
-    int32_t normal;           // pressure: outward direction
+    const int64_t* nbr;       // pressure / outflow: inward neighbour node
+    int32_t normal;           // pressure / outflow: outward direction
+    double correction;        // outflow with pressure correction: density correction factor
 } orc_bc;
+
+static bool pre_collision_kind(int kind)
+{
+    return kind == ORC_BC_PRESSURE_NEQ || kind == ORC_BC_OUTFLOW_NON_REFLECTING ||
+           kind == ORC_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION;
+}
 
 void orc_k17_collide_turb(double* f27, double omega, const double* lim3, const double* force3, double* macro4,
                           int turb, double sgs, double* nut)
 {
     static const double zero[3] = {0, 0, 0};
     k17_collide(f27, omega, lim3, force3 ? force3 : zero, macro4, turb, sgs, nut);
+}
+
+// one boundary node: post-collision populations p27 (interpolated kinds) or
+// pre-collision f27 / neighbour fn27 (outflow kinds); out27 / written27 (0/1)
+void orc_interp_bc(int kind, const double* p27, const float* q27, double omega, double vx, double vy, double vz,
+                   double* out27, int32_t* written27)
+{
+    bool wr[27] = {false};
+    interp_bc(kind, p27, q27, 1, 0, omega, vx, vy, vz, out27, wr);
+    for (int d = 0; d < 27; ++d) written27[d] = wr[d];
+}
+void orc_outflow_bc(int kind, const double* f27, const double* fn27, int normal, double factor, double* out27,
+                    int32_t* written27)
+{
+    bool wr[27] = {false};
+    outflow_bc(kind, f27, fn27, normal, factor, out27, wr);
+    for (int d = 0; d < 27; ++d) written27[d] = wr[d];
 }
 
 void orc_k17_collide(double* f27, double omega, const double* lim3, const double* force3, double* macro4)
@@ -1018,8 +1075,8 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
         for (int b = 0; b < n_bc; ++b)
             for (int i = 0; i < bcs[b].n; ++i)
                 for (int d = 1; d < 27; ++d) {
-                    if (bcs[b].kind == ORC_BC_PRESSURE_NEQ) {
-                        // populations leaving through the pressure boundary are dropped
+                    if (pre_collision_kind(bcs[b].kind)) {
+                        // populations leaving through the pressure / outflow boundary are dropped
                         const int nm = bcs[b].normal;
                         if (CX[d] * CX[nm] + CY[d] * CY[nm] + CZ[d] * CZ[nm] > 0)
                             covered[(size_t)bcs[b].node[i] * 27 + d] = 1;
@@ -1038,6 +1095,25 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
     };
 
     for (int s = 0; s < nsteps; ++s) {
+        // non-reflecting outflow: the populations entering the boundary nodes
+        // in the next state come from this pre-collision state (the kernel
+        // runs after a parity flip and writes the other parity's slots, which
+        // the following collision neither reads nor writes)
+        std::vector<std::vector<double>> pend(n_bc);
+        std::vector<std::vector<uint8_t>> pend_wr(n_bc);
+        for (int b = 0; b < n_bc; ++b) {
+            const orc_bc& bc = bcs[b];
+            if (bc.kind != ORC_BC_OUTFLOW_NON_REFLECTING && bc.kind != ORC_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION)
+                continue;
+            pend[b].assign((size_t)bc.n * 27, 0.0);
+            pend_wr[b].assign((size_t)bc.n * 27, 0);
+            for (int i = 0; i < bc.n; ++i) {
+                bool wr[27] = {false};
+                outflow_bc(bc.kind, F + 27 * bc.node[i], F + 27 * bc.nbr[i], bc.normal, bc.correction,
+                           &pend[b][(size_t)i * 27], wr);
+                for (int d = 0; d < 27; ++d) pend_wr[b][(size_t)i * 27 + d] = wr[d];
+            }
+        }
         int bad = 0;
 #pragma omp parallel for schedule(static) reduction(+ : bad)
         for (int64_t i = 0; i < N; ++i) {
@@ -1068,11 +1144,15 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
             }
         }
         if (bad) return -1;
+        for (int b = 0; b < n_bc; ++b)
+            for (int i = 0; i < (int)(pend_wr[b].size() / 27); ++i)
+                for (int d = 0; d < 27; ++d)
+                    if (pend_wr[b][(size_t)i * 27 + d]) G[(size_t)bcs[b].node[i] * 27 + d] = pend[b][(size_t)i * 27 + d];
         // post-collision boundary conditions (writes the population that
         // re-enters node i against the wall link d)
         for (int b = 0; b < n_bc; ++b) {
             const orc_bc& bc = bcs[b];
-            if (bc.kind == ORC_BC_PRESSURE_NEQ) continue;
+            if (pre_collision_kind(bc.kind)) continue;
             for (int i = 0; i < bc.n; ++i) {
                 const int64_t k = bc.node[i];
                 const double ux = bc.ux ? bc.ux[i] : 0.0, uy = bc.uy ? bc.uy[i] : 0.0, uz = bc.uz ? bc.uz[i] : 0.0;

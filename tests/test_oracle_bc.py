@@ -109,3 +109,147 @@ def test_inflow_outflow_channel_reaches_prescribed_outlet_density():
     assert np.abs(inner[:, :, nx - 2, 0]).max() < 2e-3
     flux = ((1 + inner[..., 0]) * inner[..., 1]).sum(axis=(0, 1))[1:-1]
     assert np.abs(flux / flux.mean() - 1).max() < 2e-2
+
+
+# ------------------------------------ velocity with pressure, outflow ------
+W = cases.W
+C27 = np.stack([cases.CX, cases.CY, cases.CZ], axis=1)
+FACES = [1, 2, 3, 4, 5, 6]  # dP00, dM00, d0P0, d0M0, d00P, d00M
+
+
+def _random_links(rng):
+    q = np.full(27, -1.0, np.float32)
+    links = rng.choice(np.arange(1, 27), 9, replace=False)
+    q[links] = rng.uniform(0.0, 1.0, links.size)
+    return q
+
+
+def test_velocity_with_pressure_closed_forms():
+    """VelocityWithPressureInterpolatedCompressible (KernelUtilities.h:207-213):
+    at rest with density deviation drho0 the plain velocity boundary writes
+    the equilibrium w_d drho0 and the pressure variant the bulk state 0; a
+    moving equilibrium at drho = 0 with the wall velocity is written back
+    unchanged by both; in general they differ by exactly -w_d drho."""
+    rng = np.random.default_rng(40)
+    for _ in range(50):
+        q = _random_links(rng)
+        om = rng.uniform(1.2, 1.95)
+        drho0 = rng.uniform(-0.02, 0.02)
+        rest = W * drho0
+        a, wa = oracle.interp_bc(2, rest, q, om)
+        b, wb = oracle.interp_bc(5, rest, q, om)
+        assert np.array_equal(wa, wb) and wa.sum() == 9
+        np.testing.assert_allclose(a[wa], rest[wa], rtol=0, atol=1e-17)
+        np.testing.assert_allclose(b[wb], 0.0, rtol=0, atol=1e-17)
+        u = rng.uniform(-0.05, 0.05, 3)
+        feq = oracle.equilibrium(0.0, *u)
+        for kind in (2, 5):
+            v, w = oracle.interp_bc(kind, feq, q, om, u)
+            np.testing.assert_allclose(v[w], feq[w], rtol=0, atol=2e-17)
+        p = feq + rng.normal(0, 1e-3, 27) * W + W * drho0
+        dr = oracle.macroscopic(p)[0]
+        a, wa = oracle.interp_bc(2, p, q, om, u)
+        b, wb = oracle.interp_bc(5, p, q, om, u)
+        np.testing.assert_allclose(b[wb] - a[wa], -W[wb] * dr, rtol=1e-12, atol=1e-19)
+
+
+@pytest.mark.parametrize("kind", [6, 7])
+def test_outflow_writes_the_entering_directions(kind):
+    rng = np.random.default_rng(41)
+    f = rng.normal(0, 1e-3, 27) * W
+    fn = rng.normal(0, 1e-3, 27) * W
+    for normal in FACES:
+        _, w = oracle.outflow_bc(kind, f, fn, normal)
+        assert np.array_equal(w, C27 @ C27[normal] < 0), normal
+        assert w.sum() == 9
+    for normal in (0, 7, 19):  # not a face direction: nothing (the kernels' default branch)
+        assert not oracle.outflow_bc(kind, f, fn, normal)[1].any()
+
+
+@pytest.mark.parametrize("kind", [6, 7])
+def test_outflow_keeps_a_uniform_state(kind):
+    """Node and neighbour in the same equilibrium: every entering population
+    stays what it is (the boundary passes a uniform stream through)."""
+    rng = np.random.default_rng(42)
+    for _ in range(30):
+        f = oracle.equilibrium(rng.uniform(-0.01, 0.01), *rng.uniform(-0.05, 0.05, 3))
+        for normal in FACES:
+            v, w = oracle.outflow_bc(kind, f, f, normal)
+            np.testing.assert_allclose(v[w], f[w], rtol=0, atol=1e-17)
+
+
+def test_outflow_characteristic_and_pressure_correction():
+    """Entering population = f + cs (fn - f) with cs = 1/sqrt(3) (upwind
+    advection at the sound speed); the corrected kernel subtracts w_d factor
+    drho, so a node at rest with density drho0 and factor 1 lets the bulk
+    state 0 enter; its neighbour values of directions with a negative
+    component are the node's own (the kernel's neighbour indices)."""
+    rng = np.random.default_rng(43)
+    cs = 1.0 / np.sqrt(3.0)
+    for normal in FACES:
+        f = rng.normal(0, 1e-3, 27) * W
+        fn = rng.normal(0, 1e-3, 27) * W
+        v, w = oracle.outflow_bc(6, f, fn, normal)
+        # cs is 1 / sqrtf(3): single-precision root, exact only to ~3e-8
+        np.testing.assert_allclose(v[w], (f + cs * (fn - f))[w], rtol=0, atol=1e-7 * np.abs(fn - f).max())
+        drho0 = rng.uniform(-0.02, 0.02)
+        v, w = oracle.outflow_bc(7, W * drho0, W * drho0, normal, correction=1.0)
+        np.testing.assert_allclose(v[w], 0.0, rtol=0, atol=1e-17)
+        neg = (C27 < 0).any(axis=1)
+        a, w = oracle.outflow_bc(7, f, fn, normal, correction=0.3)
+        fn2 = fn.copy()
+        fn2[neg] += 1e-3
+        b, _ = oracle.outflow_bc(7, f, fn2, normal, correction=0.3)
+        assert np.array_equal(a, b)
+        fn3 = fn.copy()
+        fn3[~neg] += 1e-3
+        c, _ = oracle.outflow_bc(7, f, fn3, normal, correction=0.3)
+        assert np.array_equal(c[w & neg], a[w & neg])
+        if normal in (1, 3, 5):  # outflow on a + face: every entering direction has a negative component
+            assert not (w & ~neg).any()
+        else:
+            assert (w & ~neg).sum() == 4 and not np.any(c[w & ~neg] == a[w & ~neg])
+
+
+def _pulse_run(right_kind, steps, nx=100, x0=70, amp=1e-3):
+    """Density pulse in a tube along x (2 x 2 periodic cross section): no-slip
+    bounce-back wall on the left, `right_kind` on the right (6 outflow, 0
+    bounce-back wall)."""
+    ny = nz = 2
+    solid = np.zeros((nz, ny, nx), bool)
+    solid[:, :, 0] = solid[:, :, nx - 1] = True
+    x = np.arange(nx)
+    drho = np.tile(amp * np.exp(-0.5 * ((x - x0) / 3.0) ** 2), ny * nz)
+    z = np.zeros(nx * ny * nz)
+    F = oracle.init_field(drho, z, z, z)
+
+    def face(xf, toward):
+        nodes = np.array([xf + nx * (y + ny * zz) for zz in range(nz) for y in range(ny)], np.int64)
+        q = np.full((27, nodes.size), -1.0, np.float32)
+        for d in range(1, 27):
+            if cases.CX[d] == toward:
+                q[d, :] = 0.5
+        return nodes, q
+
+    ln, lq = face(1, -1)
+    bcs = [oracle.BCGroup(ln, lq)]
+    rn, rq = face(nx - 2, 1)
+    if right_kind == 0:
+        bcs.append(oracle.BCGroup(rn, rq))
+    else:
+        bcs.append(oracle.BCGroup(rn, kind=right_kind, nbr=rn - 1, normal=1))
+    F = oracle.run(F, (nx, ny, nz), steps, 1.9, solid=solid.ravel(), bcs=bcs)
+    return np.array([oracle.macroscopic(f)[0] for f in F]).reshape(nz, ny, nx)[0, 0]
+
+
+def test_outflow_absorbs_a_pressure_pulse():
+    """The right-going half of the pulse meets the boundary after ~50 steps; at
+    step 100 its reflection would sit near x = 70.  A bounce-back wall reflects
+    it whole; the non-reflecting outflow lets it leave."""
+    wall = _pulse_run(0, 100)
+    out = _pulse_run(6, 100)
+    r_wall = np.abs(wall[45:95]).max()
+    r_out = np.abs(out[45:95]).max()
+    print(f"reflected amplitude: wall {r_wall:.3e}, outflow {r_out:.3e}")
+    assert r_wall > 3e-4
+    assert r_out < 0.15 * r_wall

@@ -18,6 +18,8 @@ VFX_TEMPLATE_DEFAULT, VFX_TEMPLATE_WRITE_MACRO = 0, 1
 VFX_TURB_NONE, VFX_TURB_SMAGORINSKY, VFX_TURB_AMD, VFX_TURB_QR = 0, 1, 2, 3
 VFX_BC_VELOCITY_BOUNCE_BACK, VFX_BC_NOSLIP_INTERPOLATED, VFX_BC_VELOCITY_INTERPOLATED = 0, 1, 2
 VFX_BC_SLIP, VFX_BC_PRESSURE_NEQ = 3, 4
+VFX_BC_VELOCITY_PRESSURE_INTERPOLATED, VFX_BC_OUTFLOW_NON_REFLECTING = 5, 6
+VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7
 
 _u32p = C.POINTER(C.c_uint32)
 _dp = C.POINTER(C.c_double)
@@ -43,6 +45,7 @@ class VfxBC(C.Structure):
         ("kind", C.c_int32), ("n", C.c_uint32), ("k", C.c_void_p), ("q27", C.c_void_p),
         ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p), ("cpu_float_velocity", C.c_int32),
         ("rho", C.c_void_p), ("k_neighbor", C.c_void_p), ("normal", C.c_int32),
+        ("pressure_correction_factor", C.c_double),
     ]
 
 

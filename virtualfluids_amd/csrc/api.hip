@@ -229,16 +229,21 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
     vfx::Lat L;
     if (vfx_status s = make_lat(lat, L)) return s;
     if (!bc) return fail(VFX_EINVAL, "bc is NULL");
-    if (bc->kind < VFX_BC_VELOCITY_BOUNCE_BACK || bc->kind > VFX_BC_PRESSURE_NEQ)
+    if (bc->kind < VFX_BC_VELOCITY_BOUNCE_BACK || bc->kind > VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION)
         return fail(VFX_EUNSUPPORTED, "bc kind %d not supported", bc->kind);
+    const bool outflow = bc->kind == VFX_BC_OUTFLOW_NON_REFLECTING ||
+                         bc->kind == VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION;
     if (bc->kind == VFX_BC_PRESSURE_NEQ) {
         if (bc->n && (!bc->k || !bc->k_neighbor || !bc->rho)) return fail(VFX_EINVAL, "pressure bc needs k, k_neighbor, rho");
         if (bc->normal < 1 || bc->normal > 26) return fail(VFX_EINVAL, "pressure bc normal %d invalid", bc->normal);
+    } else if (outflow) {
+        if (bc->n && (!bc->k || !bc->k_neighbor)) return fail(VFX_EINVAL, "outflow bc needs k, k_neighbor");
+        if (bc->normal < 0 || bc->normal > 26) return fail(VFX_EINVAL, "outflow bc normal %d invalid", bc->normal);
     } else if (bc->n && (!bc->k || !bc->q27)) {
         return fail(VFX_EINVAL, "bc.k / bc.q27 is NULL");
     }
     vfx::BCDev B{bc->n, bc->k, bc->q27, bc->vx, bc->vy, bc->vz, bc->cpu_float_velocity, lat->direct,
-                 lat->omega, bc->rho, bc->k_neighbor, bc->normal};
+                 lat->omega, bc->rho, bc->k_neighbor, bc->normal, bc->pressure_correction_factor};
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     switch (lat->storage) {

@@ -127,6 +127,7 @@ static boundaryCondition make_bc(int32_t kind)
         bc.rho = q->RhoBC;
         bc.k_neighbor = q->kN;
         bc.normal = q->direction;
+        bc.pressure_correction_factor = p->outflowPressureCorrectionFactor;
         check(vfx_bc_apply(&p->lattice, &bc, p->isEvenTimestep ? 1 : 0, p->stream), "boundary condition");
     };
 }
@@ -136,6 +137,7 @@ boundaryCondition BoundaryConditionFactory::getVelocityBoundaryConditionPost() c
     switch (velocity) {
     case VelocityBC::VelocityBounceBack: return make_bc(VFX_BC_VELOCITY_BOUNCE_BACK);
     case VelocityBC::VelocityInterpolatedCompressible: return make_bc(VFX_BC_VELOCITY_INTERPOLATED);
+    case VelocityBC::VelocityWithPressureInterpolatedCompressible: return make_bc(VFX_BC_VELOCITY_PRESSURE_INTERPOLATED);
     default: return nullptr;
     }
 }
@@ -153,7 +155,13 @@ boundaryCondition BoundaryConditionFactory::getSlipBoundaryConditionPost() const
 }
 boundaryCondition BoundaryConditionFactory::getPressureBoundaryConditionPre() const
 {
-    return pressure == PressureBC::PressureNonEquilibriumCompressible ? make_bc(VFX_BC_PRESSURE_NEQ) : nullptr;
+    switch (pressure) {
+    case PressureBC::PressureNonEquilibriumCompressible: return make_bc(VFX_BC_PRESSURE_NEQ);
+    case PressureBC::OutflowNonReflective: return make_bc(VFX_BC_OUTFLOW_NON_REFLECTING);
+    case PressureBC::OutflowNonReflectivePressureCorrection:
+        return make_bc(VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION);
+    default: return nullptr;
+    }
 }
 
 // BoundaryConditionKernelManager.cpp:53-83: boundary nodes without a selected

@@ -67,6 +67,7 @@ struct LBMSimulationParameter {
     std::map<CollisionTemplate, std::pair<uint32_t*, uint32_t>> taggedFluidNodeIndices;
     std::vector<CollisionTemplate> allocatedBulkFluidNodeTags{CollisionTemplate::Default};
     QforBoundaryConditions velocityBC, noSlipBC, slipBC, pressureBC;
+    double outflowPressureCorrectionFactor = 0.0; // Parameter.h:158
     vfx_stream stream = nullptr;
 };
 
@@ -88,6 +89,11 @@ public:
     void setSGSConstant(double c) { sgsConstant = c; }
     double getSGSConstant() const { return sgsConstant; }
     void setUseTurbulentViscosity(bool u) { useTurbulentViscosity = u; }
+    // Parameter::setOutflowPressureCorrectionFactor (Parameter.cpp:669-672), every level
+    void setOutflowPressureCorrectionFactor(double f)
+    {
+        for (auto& l : levels) l.outflowPressureCorrectionFactor = f;
+    }
     bool getUseTurbulentViscosity() const { return useTurbulentViscosity; }
 
 private:
@@ -191,10 +197,20 @@ using boundaryCondition = std::function<void(LBMSimulationParameter*, QforBounda
 class BoundaryConditionFactory {
 public:
     // subsets of the reference enums (BoundaryConditionFactory.h:60-132) this stepper implements
-    enum class VelocityBC { VelocityBounceBack, VelocityInterpolatedCompressible, NotSpecified };
+    enum class VelocityBC {
+        VelocityBounceBack,
+        VelocityInterpolatedCompressible,
+        VelocityWithPressureInterpolatedCompressible, // combine with PressureBC::OutflowNonReflective
+        NotSpecified
+    };
     enum class NoSlipBC { NoSlipBounceBack, NoSlipInterpolatedCompressible, NotSpecified };
     enum class SlipBC { SlipCompressible, NotSpecified };
-    enum class PressureBC { PressureNonEquilibriumCompressible, NotSpecified };
+    enum class PressureBC {
+        PressureNonEquilibriumCompressible,
+        OutflowNonReflective,
+        OutflowNonReflectivePressureCorrection,
+        NotSpecified
+    };
 
     void setVelocityBoundaryCondition(VelocityBC t) { velocity = t; }
     void setNoSlipBoundaryCondition(NoSlipBC t) { noSlip = t; }

@@ -661,8 +661,9 @@ struct BCDev {
     int32_t direct;
     double omega;          // interpolated kinds
     const double* rho;     // pressure: density deviation per boundary node
-    const uint32_t* kn;    // pressure: inward neighbour of each boundary node
-    int32_t normal;        // pressure: outward direction of the boundary (d000..dMMM)
+    const uint32_t* kn;    // pressure / outflow: inward neighbour of each boundary node
+    int32_t normal;        // pressure / outflow: outward direction of the boundary (d000..dMMM)
+    double corr;           // outflow with pressure correction: density correction factor
 };
 
 // c_d . (ux, uy, uz) spelled as VirtualFluids writes it: first non-zero
@@ -796,7 +797,7 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
 #pragma unroll
     for (int d = 1; d < 27; ++d) qs[d] = (T)B.q[(uint64_t)d * B.n + i];
     T VX = 0, VY = 0, VZ = 0;
-    if (KIND == VFX_BC_VELOCITY_INTERPOLATED) {
+    if (KIND == VFX_BC_VELOCITY_INTERPOLATED || KIND == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED) {
         VX = B.vx ? (T)B.vx[i] : T(0);
         VY = B.vy ? (T)B.vy[i] : T(0);
         VZ = B.vz ? (T)B.vz[i] : T(0);
@@ -840,6 +841,9 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
             }
             if constexpr (sizeof(T) == 4) v = a + (q * (f + fi) - T(6) * w * vbc) * r1q;
             else v = a + (q * (f + fi) - T(6) * w * vbc) / (T(1) + q);
+            // getInterpolatedDistributionForVeloWithPressureBC (KernelUtilities.h:207-213):
+            // the entering population carries the bulk pressure
+            if (KIND == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED) v = v - w * drho;
         }
         const int e = inv(d);
         Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)], L.ss[e]);
@@ -909,6 +913,65 @@ __global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
     for (int d = 0; d < 27; ++d) {
         if (cx(d) * nx + cy(d) * ny + cz(d) * nz > 0) continue;
         Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], f[d], L.st[weight_class(d)], L.ss[d]);
+    }
+}
+
+// Non-reflecting outflow (OutflowNonReflecting.cu:44-280; with CORR
+// OutflowNonReflectingPressureCorrection.cu:46-245): pre-collision, parity EVEN.
+// From the node's populations f and its inward neighbour's fn, every direction
+// d entering through the boundary face (c_d . normal < 0) gets
+// fn[d] cs + (1 - cs) f[d] [- w_d corr drho(f)], cs = 1 / sqrtf(3), written
+// into the element the next collision of the other parity reads as direction
+// d of the node: slot (d, !EVEN) at the node's own neighbour index.  That
+// element belongs to the node across the boundary, which does not collide, so
+// the collision of parity EVEN leaves it alone.  The corrected kernel builds
+// its "neighbour" indices from the boundary node, so for directions with a
+// negative component fn[d] is the node's own value; kept as is.  A normal that
+// is not a face direction writes nothing (the kernels' default branch).
+template <typename S, bool EVEN, bool CORR>
+__global__ void __launch_bounds__(256) k_bc_outflow(Lat L, BCDev B)
+{
+    using T = typename Compute<S>::T;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    const uint32_t k = B.k[i], kn = B.kn[i];
+    if (k >= L.N || kn >= L.N) return;
+    const int nx = cx(B.normal), ny = cy(B.normal), nz = cz(B.normal);
+    if (nx * nx + ny * ny + nz * nz != 1) return;
+    uint32_t kk[8], kkn[8];
+    if (B.direct) {
+        nbrs_direct(L, k, kk);
+        nbrs_direct(L, kn, kkn);
+    } else {
+        nbrs_sparse(L, k, kk);
+        nbrs_sparse(L, kn, kkn);
+    }
+    T m[27], mn[27];
+    gather<S, T, EVEN>(L, kk, m);
+    gather<S, T, EVEN>(L, kkn, mn);
+    T f[27], fn[27];
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+        const int d = dir_at(s);
+        f[d] = m[s];
+        fn[d] = (CORR && nbr(d) != 0) ? m[s] : mn[s];
+    }
+    T rc = T(0);
+    if constexpr (CORR) {
+        // vf::lbm::getDensity (MacroscopicQuantities.h:55-64)
+        const T corners = ((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]));
+        const T edges = (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P]))) +
+                        ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]));
+        const T faces = ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0])) + (f[d00M] + f[d00P]);
+        rc = (T)B.corr * (((corners + edges) + f[d000]) + faces);
+    }
+    const T cs = T(1) / (T)sqrtf(3.f);
+#pragma unroll
+    for (int d = 1; d < 27; ++d) {
+        if (cx(d) * nx + cy(d) * ny + cz(d) * nz >= 0) continue;
+        T v = fn[d] * cs + (T(1) - cs) * f[d];
+        if constexpr (CORR) v = v - weight<T>(d) * rc;
+        Storage<S, T>::st(plane<S>(L, slot(d, !EVEN)), kk[nbr(d)], v, L.st[weight_class(d)], L.ss[d]);
     }
 }
 
@@ -1383,6 +1446,18 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     case VFX_BC_PRESSURE_NEQ:
         if (even) hipLaunchKernelGGL((k_bc_pressure<S, true>), g, b, 0, st, L, B);
         else hipLaunchKernelGGL((k_bc_pressure<S, false>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_VELOCITY_PRESSURE_INTERPOLATED:
+        if (even) hipLaunchKernelGGL((k_bc_interp<S, true, VFX_BC_VELOCITY_PRESSURE_INTERPOLATED>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_interp<S, false, VFX_BC_VELOCITY_PRESSURE_INTERPOLATED>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_OUTFLOW_NON_REFLECTING:
+        if (even) hipLaunchKernelGGL((k_bc_outflow<S, true, false>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_outflow<S, false, false>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION:
+        if (even) hipLaunchKernelGGL((k_bc_outflow<S, true, true>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_outflow<S, false, true>), g, b, 0, st, L, B);
         break;
     default:
         return hipErrorInvalidValue;

@@ -341,12 +341,23 @@ class BoundaryCondition:
     kind: 0 velocity bounce-back, 1 no-slip interpolated, 2 velocity
     interpolated, 3 slip, 4 non-equilibrium pressure (pre-collision; needs
     ``rho``, ``neighbors`` (inward neighbour node) and ``normal`` (outward
-    direction index))."""
+    direction index)), 5 velocity interpolated with pressure, 6 non-reflecting
+    outflow and 7 non-reflecting outflow with pressure correction
+    (pre-collision; ``neighbors``, ``normal`` a face direction 1..6, kind 7
+    also ``pressure_correction_factor``).
 
-    PRE_COLLISION = {V.VFX_BC_PRESSURE_NEQ}
+    The outflow kinds write the populations the collision after next reads
+    (the kernel runs after a parity flip and fills the other parity's slots):
+    applied only after every step, as VirtualFluids does, the second step
+    reads whatever those elements held before.  Apply it once with the
+    starting parity before the first step to define them."""
+
+    PRE_COLLISION = {V.VFX_BC_PRESSURE_NEQ, V.VFX_BC_OUTFLOW_NON_REFLECTING,
+                     V.VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION}
+    OUTFLOW = {V.VFX_BC_OUTFLOW_NON_REFLECTING, V.VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION}
 
     def __init__(self, kind, nodes, q27=None, vx=None, vy=None, vz=None, cpu_float_velocity=False, rho=None,
-                 neighbors=None, normal=0, device="cuda"):
+                 neighbors=None, normal=0, device="cuda", pressure_correction_factor=0.0):
         nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
         n = nodes.size
         self.kind = int(kind)
@@ -354,9 +365,11 @@ class BoundaryCondition:
         dev = torch.device(device)
         self.max_node = int(nodes.max()) if n else -1
         self.k = torch.from_numpy(nodes.view(np.int32)).to(dev)
-        if self.kind == V.VFX_BC_PRESSURE_NEQ:
-            if rho is None or neighbors is None or not 1 <= int(normal) <= 26:
+        if self.kind == V.VFX_BC_PRESSURE_NEQ or self.kind in self.OUTFLOW:
+            if self.kind == V.VFX_BC_PRESSURE_NEQ and (rho is None or neighbors is None or not 1 <= int(normal) <= 26):
                 raise ValueError("pressure boundary needs rho, neighbors and normal in 1..26")
+            if self.kind in self.OUTFLOW and (neighbors is None or not 1 <= int(normal) <= 6):
+                raise ValueError("outflow boundary needs neighbors and a face normal in 1..6")
             q27 = np.zeros((27, n), np.float32) if q27 is None else q27
             kn = np.ascontiguousarray(neighbors, dtype=np.uint32)
             if kn.size != n:
@@ -373,11 +386,12 @@ class BoundaryCondition:
         def arr(v):
             if v is None:
                 return None
-            return torch.from_numpy(np.ascontiguousarray(np.broadcast_to(v, (n,)), dtype=np.float64)).to(dev)
+            return torch.from_numpy(np.array(np.broadcast_to(v, (n,)), dtype=np.float64)).to(dev)
 
         self.vx, self.vy, self.vz = arr(vx), arr(vy), arr(vz)
         self.rho = arr(rho)
         self.normal = int(normal)
+        self.pressure_correction_factor = float(pressure_correction_factor)
         self.cpu_float_velocity = bool(cpu_float_velocity)
 
     @property
@@ -395,6 +409,7 @@ class BoundaryCondition:
         b.rho = _ptr(self.rho)
         b.k_neighbor = _ptr(self.kn) if self.n else None
         b.normal = self.normal
+        b.pressure_correction_factor = self.pressure_correction_factor
         return b
 
 
