@@ -142,9 +142,12 @@ typedef enum {
                                                   Velocity/VelocityWithPressureInterpolatedCompressible.cu:44 */
     VFX_BC_OUTFLOW_NON_REFLECTING = 6, /* OutflowNonReflecting_Device, Outflow/OutflowNonReflecting.cu:50 (pre-collision;
                                           k_neighbor, normal) */
-    VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7 /* OutflowNonReflectingPressureCorrection_Device,
+    VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7, /* OutflowNonReflectingPressureCorrection_Device,
                                           Outflow/OutflowNonReflectingPressureCorrection.cu:51 (pre-collision;
                                           k_neighbor, normal, pressure_correction_factor) */
+    VFX_BC_SLIP_TURBULENT_VISCOSITY = 8 /* SlipTurbulentViscosityCompressible_Device, Slip/SlipTurbulentViscosityCompressible.cu:45:
+                                          slip with omega / (1 + 3 omega max(0, nu_t[k])); the lattice needs a
+                                          turbulence model and its turbulent_viscosity */
 } vfx_bc_kind;
 
 /* QforBoundaryConditions (src/gpu/core/Calculation/Calculation.h:105-117) */

@@ -243,7 +243,7 @@ class BCGroup:
     interpolated, 3 slip, 4 non-equilibrium pressure (rho, nbr, normal),
     5 velocity interpolated with pressure, 6 non-reflecting outflow (nbr,
     normal), 7 non-reflecting outflow with pressure correction (nbr, normal,
-    correction)."""
+    correction), 8 slip at the node's turbulent-viscosity rate (run_turb)."""
 
     def __init__(self, node, q=None, ux=None, uy=None, uz=None, cpu_float_velocity=False, kind=0, rho=None,
                  nbr=None, normal=0, correction=0.0):

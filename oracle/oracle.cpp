@@ -53,7 +53,8 @@ enum {
     ORC_BC_PRESSURE_NEQ = 4,
     ORC_BC_VELOCITY_PRESSURE_INTERPOLATED = 5,
     ORC_BC_OUTFLOW_NON_REFLECTING = 6,
-    ORC_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7
+    ORC_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7,
+    ORC_BC_SLIP_TURBULENT_VISCOSITY = 8
 };
 
 namespace {
@@ -583,7 +584,7 @@ void interp_bc(int kind, const double* p, const float* q, int n, int i, double o
             v = a + (qd * (f + fi)) / (1.0 + qd);
         } else {
             double vbc;
-            if (kind == ORC_BC_SLIP) {
+            if (kind == ORC_BC_SLIP || kind == ORC_BC_SLIP_TURBULENT_VISCOSITY) {
                 if (std::abs(CX[d]) + std::abs(CY[d]) + std::abs(CZ[d]) == 1) {
                     if (CX[d]) fx = true;
                     if (CY[d]) fy = true;
@@ -1167,7 +1168,11 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
                 }
                 double out[27];
                 bool wr[27] = {false};
-                interp_bc(bc.kind, &post[(size_t)k * 27], bc.q, bc.n, i, omega, ux, uy, uz, out, wr);
+                // SlipTurbulentViscosityCompressible.cu:130-134: the node's own rate
+                const double om = bc.kind == ORC_BC_SLIP_TURBULENT_VISCOSITY && nut
+                                      ? omega / (1.0 + 3.0 * omega * std::max(0.0, nut[k]))
+                                      : omega;
+                interp_bc(bc.kind, &post[(size_t)k * 27], bc.q, bc.n, i, om, ux, uy, uz, out, wr);
                 for (int d = 0; d < 27; ++d)
                     if (wr[d]) G[(size_t)k * 27 + d] = out[d];
             }

@@ -243,3 +243,37 @@ def test_refined_steps_match_oracle(cuda, model):
     print(f"  coarse {ec:.2e} fine {ef:.2e} fine nu_t {en:.2e}", flush=True)
     assert nf.max() > 0.0
     assert ec <= REL and ef <= REL and en <= REL
+
+
+# ------------------------------------------- slip with turbulent viscosity --
+@pytest.mark.parametrize("model", ["smagorinsky", "qr"])
+def test_slip_turbulent_viscosity_matches_oracle(cuda, model):
+    """SlipTurbulentViscosityCompressible walls (q 0.3 / 0.6) with the model's
+    viscosity, FP64 vs the oracle after 40 steps."""
+    print(f"[slip turbulent viscosity {model}]", flush=True)
+    dims = (6, 12, 5)
+    solid, groups = cases.plate_channel(*dims, q_bottom=0.3, q_top=0.6)
+    F0 = oracle.init_field(*cases.random_flow(dims, seed=15, u_amp=0.03))
+    lat = vfx.Lattice(dims, storage="fp64", omega=1.9, solid=solid, turbulence=model, sgs_constant=SGS[model])
+    lat.write_populations(F0)
+    bcs = [vfx.BoundaryCondition(8, nodes, q) for nodes, q in groups.values()]
+    st = vfx.Stepper(lat, "k17", bcs)
+    st.step(40)
+    torch.cuda.synchronize()
+    got = st.populations().cpu().numpy()
+    obcs = [oracle.BCGroup(nodes, q, kind=8) for nodes, q in groups.values()]
+    ref, rnut = oracle.run_turb(F0, dims, 40, 1.9, model, SGS[model], solid=solid, bcs=obcs)
+    fl = ~solid
+    e_f, e_n = rel_err(got[fl], ref[fl]), rel_err(lat.nut.cpu().numpy()[fl], rnut[fl])
+    print(f"  populations {e_f:.2e}, nu_t {e_n:.2e}", flush=True)
+    assert rnut[fl].max() > 0.0
+    assert e_f <= REL and e_n <= REL
+
+
+def test_slip_turbulent_viscosity_needs_a_model(cuda):
+    dims = (6, 12, 5)
+    solid, groups = cases.plate_channel(*dims, q_bottom=0.3, q_top=0.6)
+    lat = vfx.Lattice(dims, storage="fp64", omega=1.9, solid=solid)
+    nodes, q = next(iter(groups.values()))
+    with pytest.raises(vfx.VfxError, match="turbulence model"):
+        lat.apply_bc(vfx.BoundaryCondition(8, nodes, q))

@@ -275,3 +275,27 @@ def test_scaling_uses_each_nodes_viscosity():
         k = oracle.interp_coefficients_omegas(Ff[fc], [rate(of, nut_f[j]) for j in fc])
         want = oracle.interpolate_fc(k, rate(oc, nut_c[nest.fc_coarse[i]]))
         assert np.array_equal(got[nest.fc_coarse[i]], want), i
+
+
+# ------------------------------------------- slip with turbulent viscosity --
+def _slip_channel(kind):
+    dims = (6, 12, 5)
+    solid, groups = cases.plate_channel(*dims, q_bottom=0.3, q_top=0.6)
+    bcs = [oracle.BCGroup(nodes, q, kind=kind) for nodes, q in groups.values()]
+    F0 = oracle.init_field(*cases.random_flow(dims, seed=14, u_amp=0.03))
+    return dims, solid, bcs, F0
+
+
+def test_slip_turbulent_viscosity_rate():
+    """SlipTurbulentViscosityCompressible: the slip formula at the node's rate
+    omega / (1 + 3 omega max(0, nu_t)).  With a fixed nu_t (AMD: the collision
+    does not change it) one step equals the plain slip run at the effective
+    rate; with nu_t = 0 (Smagorinsky constant 0) it is the plain slip run."""
+    dims, solid, bcs8, F0 = _slip_channel(8)
+    _, _, bcs3, _ = _slip_channel(3)
+    om, c = 1.9, 0.012
+    F, _ = oracle.run_turb(F0, dims, 1, om, "amd", 0.3, nut=np.full(len(F0), c), solid=solid, bcs=bcs8)
+    want = oracle.run(F0, dims, 1, om / (1.0 + 3.0 * om * c), solid=solid, bcs=bcs3)
+    assert np.array_equal(F[~solid], want[~solid])
+    F, nut = oracle.run_turb(F0, dims, 20, om, "smagorinsky", 0.0, solid=solid, bcs=bcs8)
+    assert np.array_equal(F[~solid], oracle.run(F0, dims, 20, om, solid=solid, bcs=bcs3)[~solid])

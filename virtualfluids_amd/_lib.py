@@ -20,6 +20,7 @@ VFX_BC_VELOCITY_BOUNCE_BACK, VFX_BC_NOSLIP_INTERPOLATED, VFX_BC_VELOCITY_INTERPO
 VFX_BC_SLIP, VFX_BC_PRESSURE_NEQ = 3, 4
 VFX_BC_VELOCITY_PRESSURE_INTERPOLATED, VFX_BC_OUTFLOW_NON_REFLECTING = 5, 6
 VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7
+VFX_BC_SLIP_TURBULENT_VISCOSITY = 8
 
 _u32p = C.POINTER(C.c_uint32)
 _dp = C.POINTER(C.c_double)

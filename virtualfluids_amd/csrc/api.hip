@@ -229,8 +229,10 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
     vfx::Lat L;
     if (vfx_status s = make_lat(lat, L)) return s;
     if (!bc) return fail(VFX_EINVAL, "bc is NULL");
-    if (bc->kind < VFX_BC_VELOCITY_BOUNCE_BACK || bc->kind > VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION)
+    if (bc->kind < VFX_BC_VELOCITY_BOUNCE_BACK || bc->kind > VFX_BC_SLIP_TURBULENT_VISCOSITY)
         return fail(VFX_EUNSUPPORTED, "bc kind %d not supported", bc->kind);
+    if (bc->kind == VFX_BC_SLIP_TURBULENT_VISCOSITY && !L.nut)
+        return fail(VFX_EINVAL, "slip with turbulent viscosity needs a turbulence model and turbulent_viscosity");
     const bool outflow = bc->kind == VFX_BC_OUTFLOW_NON_REFLECTING ||
                          bc->kind == VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION;
     if (bc->kind == VFX_BC_PRESSURE_NEQ) {

@@ -151,7 +151,11 @@ boundaryCondition BoundaryConditionFactory::getNoSlipBoundaryConditionPost() con
 }
 boundaryCondition BoundaryConditionFactory::getSlipBoundaryConditionPost() const
 {
-    return slip == SlipBC::SlipCompressible ? make_bc(VFX_BC_SLIP) : nullptr;
+    switch (slip) {
+    case SlipBC::SlipCompressible: return make_bc(VFX_BC_SLIP);
+    case SlipBC::SlipTurbulentViscosityCompressible: return make_bc(VFX_BC_SLIP_TURBULENT_VISCOSITY);
+    default: return nullptr;
+    }
 }
 boundaryCondition BoundaryConditionFactory::getPressureBoundaryConditionPre() const
 {

@@ -204,7 +204,7 @@ public:
         NotSpecified
     };
     enum class NoSlipBC { NoSlipBounceBack, NoSlipInterpolatedCompressible, NotSpecified };
-    enum class SlipBC { SlipCompressible, NotSpecified };
+    enum class SlipBC { SlipCompressible, SlipTurbulentViscosityCompressible, NotSpecified };
     enum class PressureBC {
         PressureNonEquilibriumCompressible,
         OutflowNonReflective,

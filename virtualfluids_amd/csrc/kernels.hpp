@@ -788,7 +788,10 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
                    (-(p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + ((p[dP0P] - p[dM0M]) - (p[dP0M] - p[dM0P])) +
                    (p[d00P] - p[d00M])));
     const T cu_sq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3) * (T(1) + drho);
-    const T omega = (T)B.omega;
+    T omega = (T)B.omega;
+    // SlipTurbulentViscosityCompressible.cu:130-134: the node's rate with its turbulent viscosity
+    if constexpr (KIND == VFX_BC_SLIP_TURBULENT_VISCOSITY)
+        omega = omega / (T(1) + T(3) * omega * fmax(T(0), ((const T*)L.nut)[k]));
     const T inv1mo = sizeof(T) == 4 ? T(1) / (T(1) - omega) : T(0);
     // all link fractions first: loaded inside the link loop, each q waited for
     // the previous link's store as well (one memory counter covers both) and
@@ -828,7 +831,7 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
             else v = a + (q * (f + fi)) / (T(1) + q);
         } else {
             T vbc;
-            if (KIND == VFX_BC_SLIP) {
+            if (KIND == VFX_BC_SLIP || KIND == VFX_BC_SLIP_TURBULENT_VISCOSITY) {
                 if (weight_class(d) == 1) {
                     if (cx(d) != 0) fx = true;
                     if (cy(d) != 0) fy = true;
@@ -1450,6 +1453,10 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     case VFX_BC_VELOCITY_PRESSURE_INTERPOLATED:
         if (even) hipLaunchKernelGGL((k_bc_interp<S, true, VFX_BC_VELOCITY_PRESSURE_INTERPOLATED>), g, b, 0, st, L, B);
         else hipLaunchKernelGGL((k_bc_interp<S, false, VFX_BC_VELOCITY_PRESSURE_INTERPOLATED>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_SLIP_TURBULENT_VISCOSITY:
+        if (even) hipLaunchKernelGGL((k_bc_interp<S, true, VFX_BC_SLIP_TURBULENT_VISCOSITY>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_interp<S, false, VFX_BC_SLIP_TURBULENT_VISCOSITY>), g, b, 0, st, L, B);
         break;
     case VFX_BC_OUTFLOW_NON_REFLECTING:
         if (even) hipLaunchKernelGGL((k_bc_outflow<S, true, false>), g, b, 0, st, L, B);

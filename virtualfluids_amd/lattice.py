@@ -344,7 +344,8 @@ class BoundaryCondition:
     direction index)), 5 velocity interpolated with pressure, 6 non-reflecting
     outflow and 7 non-reflecting outflow with pressure correction
     (pre-collision; ``neighbors``, ``normal`` a face direction 1..6, kind 7
-    also ``pressure_correction_factor``).
+    also ``pressure_correction_factor``), 8 slip with the node's turbulent
+    viscosity (the lattice needs a turbulence model).
 
     The outflow kinds write the populations the collision after next reads
     (the kernel runs after a parity flip and fills the other parity's slots):
