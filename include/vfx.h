@@ -221,6 +221,18 @@ vfx_status vfx_import_distributions(const vfx_lattice* lat, int32_t is_even_time
                                     int32_t precision, vfx_stream stream);
 
 /* --------------------------------------------------------- diagnostics */
+/* calculateMacroscopicQuantitiesCompressible (src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237,
+ * run before every output, Simulation::run): rho (density deviation), vx, vy,
+ * vz and pressure of every node in the lattice's real type (double for VFX_FP64,
+ * float otherwise), n_nodes each, in the reference's arithmetic
+ * (vf::lbm::getDensity, getCompressibleVelocityX1..3, getPressure,
+ * src/lbm/MacroscopicQuantities.h:55-149).  Every node is zeroed first; the
+ * fluid nodes -- direct: not solid; sparse: isValidFluidNode of
+ * type_of_grid_node (types 19, 5, 6, 7), or the fluid_nodes list when no types
+ * are given -- get their values. */
+vfx_status vfx_macroscopic_quantities(const vfx_lattice* lat, int32_t is_even_timestep, void* rho, void* vx, void* vy,
+                                      void* vz, void* pressure, vfx_stream stream);
+
 /* Macroscopic fields of the pre-collision state (calculateMacroscopicQuantitiesCompressible_device,
  * src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237), doubles, n_nodes each. */
 vfx_status vfx_macroscopic(const vfx_lattice* lat, int32_t is_even_timestep, double* drho, double* vx,

@@ -71,6 +71,7 @@ def lib():
         L.orc_interp_bc.argtypes = [C.c_int, _dp, C.POINTER(C.c_float), C.c_double, C.c_double, C.c_double,
                                     C.c_double, _dp, _i32p]
         L.orc_outflow_bc.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, _i32p]
+        L.orc_macroscopic_quantities.argtypes = [_dp, _dp]
         L.orc_k17_collide_turb.argtypes = [_dp, C.c_double, _dp, _dp, _dp, C.c_int, C.c_double, _dp]
         L.orc_turbulent_viscosity_amd.argtypes = [C.c_int, C.c_int, C.c_int, _u8p, _dp, _dp, _dp, C.c_double, _dp]
         L.orc_num_threads.restype = C.c_int
@@ -101,6 +102,7 @@ def ref():
     if _ref is None:
         L = C.CDLL(REF_PATH)
         L.ref_k17_collide.argtypes = [_dp, C.c_double, _dp, _dp, _dp]
+        L.ref_macroscopic_quantities.argtypes = [_dp, _dp]
         L.ref_k17_collide_turb.argtypes = [_dp, C.c_double, _dp, _dp, _dp, C.c_int, C.c_double, _dp]
         L.ref_chimera.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.ref_macroscopic.argtypes = [_dp, _dp]
@@ -150,6 +152,16 @@ def k17_collide_turb(f27, omega, turb, sgs, nut, lim=LIMITER, force=(0.0, 0.0, 0
     fn = L.orc_k17_collide_turb if impl == "oracle" else L.ref_k17_collide_turb
     fn(_p(f), float(omega), _p(lim), _p(fo), _p(mv), TURBULENCE[turb], float(sgs), _p(nu))
     return f, mv, float(nu[0])
+
+
+def macroscopic_quantities(f27, impl="oracle"):
+    """(rho, vx, vy, vz, pressure) of one node as the reference's output
+    kernel computes them."""
+    f = np.ascontiguousarray(f27, dtype=np.float64)
+    out = np.zeros(5)
+    fn = lib().orc_macroscopic_quantities if impl == "oracle" else ref().ref_macroscopic_quantities
+    fn(_p(f), _p(out))
+    return out
 
 
 def interp_bc(kind, p27, q27, omega, u=(0.0, 0.0, 0.0)):

@@ -953,6 +953,27 @@ void orc_outflow_bc(int kind, const double* f27, const double* fn27, int normal,
     for (int d = 0; d < 27; ++d) written27[d] = wr[d];
 }
 
+// calculateMacroscopicQuantitiesCompressible_device per node
+// (MacroscopicQuantities.cu:222-236 with MacroscopicQuantities.h:55-149):
+// out5 = {rho, vx, vy, vz, pressure}
+void orc_macroscopic_quantities(const double* f, double* out5)
+{
+    const double rho = density(f);
+    const double vx = mom_x(f) / (rho + 1.0), vy = mom_y(f) / (rho + 1.0), vz = mom_z(f) / (rho + 1.0);
+    const double press = (f[dP00] + f[dM00] + f[d0P0] + f[d0M0] + f[d00P] + f[d00M] +
+                          2.0 * (f[dPP0] + f[dMM0] + f[dPM0] + f[dMP0] + f[dP0P] + f[dM0M] + f[dP0M] + f[dM0P] +
+                                 f[d0PP] + f[d0MM] + f[d0PM] + f[d0MP]) +
+                          3.0 * (f[dPPP] + f[dMMP] + f[dPMP] + f[dMPP] + f[dPPM] + f[dMMM] + f[dPMM] + f[dMPM]) -
+                          rho - (vx * vx + vy * vy + vz * vz) * (1.0 + rho)) *
+                             c1o2 +
+                         rho;
+    out5[0] = rho;
+    out5[1] = vx;
+    out5[2] = vy;
+    out5[3] = vz;
+    out5[4] = press;
+}
+
 void orc_k17_collide(double* f27, double omega, const double* lim3, const double* force3, double* macro4)
 {
     static const double zero[3] = {0, 0, 0};

@@ -158,6 +158,20 @@ void ref_k17_collide_turb(double* f27, double omega, const double* lim3, const d
     if (macro4) { macro4[0] = mv.rho; macro4[1] = mv.vx; macro4[2] = mv.vy; macro4[3] = mv.vz; }
 }
 
+// the reference's macroscopic functions of calculateMacroscopicQuantitiesCompressible_device
+void ref_macroscopic_quantities(const double* f27, double* out5)
+{
+    const double rho = vf::lbm::getDensity(f27);
+    const double vx = vf::lbm::getCompressibleVelocityX1(f27, rho);
+    const double vy = vf::lbm::getCompressibleVelocityX2(f27, rho);
+    const double vz = vf::lbm::getCompressibleVelocityX3(f27, rho);
+    out5[0] = rho;
+    out5[1] = vx;
+    out5[2] = vy;
+    out5[3] = vz;
+    out5[4] = vf::lbm::getPressure(f27, rho, vx, vy, vz);
+}
+
 void ref_chimera(int kind, double* abc, double vv, double v2, double kinv, double k)
 {
     switch (kind) {

@@ -9,6 +9,10 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <iterator>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -402,8 +406,179 @@ static void restart_tests(const std::string& prefix)
     EXPECT(!b.delete_restart_file(prefix + "_does_not_exist"));
 }
 
+// FileWriter through the host layer: the written point data of an output step
+// equal vfx_macroscopic_quantities of the level (press and rho / 3).
+static void gpu_output_tests()
+{
+    const uint32_t nx = 6, ny = 5, nz = 4, N = nx * ny * nz;
+    std::vector<double> drho(N), ux(N), uy(N), uz(N);
+    for (uint32_t k = 0; k < N; ++k) {
+        drho[k] = 1e-3 * std::sin(0.4 * k);
+        ux[k] = 0.03 * std::cos(0.2 * k);
+        uy[k] = 0.02 * std::sin(0.3 * k);
+        uz[k] = 0.01 * std::cos(0.1 * k + 0.2);
+    }
+    auto up = [&](const void* src, size_t bytes) {
+        void* d = nullptr;
+        check(vfx_device_alloc(bytes, &d), "alloc");
+        check(vfx_copy_to_device(d, src, bytes), "h2d");
+        return d;
+    };
+    void* fields[4] = {up(drho.data(), N * 8), up(ux.data(), N * 8), up(uy.data(), N * 8), up(uz.data(), N * 8)};
+    void* f = nullptr;
+    check(vfx_device_alloc((uint64_t)N * 27 * 8, &f), "alloc");
+    auto para = std::make_shared<Parameter>(0);
+    LBMSimulationParameter* p = para->getParD(0);
+    vfx_lattice& lat = p->lattice;
+    lat.f = f;
+    lat.storage = VFX_FP64;
+    lat.n_nodes = N;
+    lat.direct = 1;
+    lat.dims[0] = nx;
+    lat.dims[1] = ny;
+    lat.dims[2] = nz;
+    lat.omega = 1.9;
+    check(vfx_init_equilibrium(&lat, 1, (double*)fields[0], (double*)fields[1], (double*)fields[2], (double*)fields[3],
+                               nullptr),
+          "init");
+    HostGrid& g = p->host;
+    for (uint32_t k = 0; k < N; ++k) {
+        const uint32_t x = k % nx, y = (k / nx) % ny, z = k / (nx * ny);
+        g.coordinateX.push_back(x);
+        g.coordinateY.push_back(y);
+        g.coordinateZ.push_back(z);
+        g.typeOfGridNode.push_back(19);
+        g.neighborX.push_back((x + 1) % nx + nx * (y + ny * z));
+        g.neighborY.push_back(x + nx * ((y + 1) % ny + ny * z));
+        g.neighborZ.push_back(x + nx * (y + ny * ((z + 1) % nz)));
+    }
+    const std::string prefix = "host_adapters_output/run";
+    FileWriter w(prefix);
+    w.writeTimestep(para, 0, true);
+    // expected fields
+    void* q[5];
+    for (auto& a : q) check(vfx_device_alloc(N * 8, &a), "alloc");
+    check(vfx_macroscopic_quantities(&lat, 1, q[0], q[1], q[2], q[3], q[4], nullptr), "macro");
+    std::vector<std::vector<double>> want(5, std::vector<double>(N));
+    for (int i = 0; i < 5; ++i) check(vfx_copy_to_host(want[i].data(), q[i], N * 8), "d2h");
+    std::ifstream in(prefix + "_bin_lev_0_ID_0_Part_1_t_0.vtk.bin.vtu", std::ios::binary);
+    std::string raw((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    const size_t at = raw.find("<AppendedData encoding=\"raw\">\n_");
+    EXPECT(at != std::string::npos);
+    const size_t m = (nx - 1) * (ny - 1) * (nz - 1);
+    size_t pos = at + std::strlen("<AppendedData encoding=\"raw\">\n_") + (4 + 12 * N) + (4 + 32 * m) + (4 + 4 * m) +
+                 (4 + m);
+    const int order[5] = {4, 0, 1, 2, 3}; // press, rho, vx1, vx2, vx3
+    for (int s = 0; s < 5 && at != std::string::npos; ++s) {
+        std::vector<double> got(N);
+        std::memcpy(got.data(), raw.data() + pos + 4, N * 8);
+        pos += 4 + 8 * N;
+        bool same = true;
+        for (uint32_t k = 0; k < N; ++k) {
+            const double e = s < 2 ? want[order[s]][k] / 3.0 : want[order[s]][k];
+            same = same && got[k] == e;
+        }
+        EXPECT(same);
+    }
+    EXPECT(std::ifstream(prefix + "_bin_ID_0_t_0.vtk.pvtu").good());
+    EXPECT(std::ifstream(prefix + "_bin_ID_0.pvd").good());
+    std::filesystem::remove_all("host_adapters_output");
+    for (void* d : q) vfx_device_free(d);
+    for (void* d : fields) vfx_device_free(d);
+    vfx_device_free(f);
+}
+
+// Output: the FilePartCalculator / WriterUtilities answers of the reference's
+// unit tests (FilePartCalculatorTest.cpp, WriterUtilitiesTest.cpp)
+static void output_unit_tests()
+{
+    const uint32_t L = FilePartCalculator::limitOfNodesForVTK;
+    EXPECT(FilePartCalculator::calculateNumberOfParts(L * 2) == 3);
+    EXPECT(FilePartCalculator::calculateNumberOfParts(L) == 2);
+    EXPECT(FilePartCalculator::calculateNumberOfParts(L - 1) == 1);
+    EXPECT(FilePartCalculator::calculateNumberOfParts(1) == 1);
+    EXPECT(FilePartCalculator::calculateNumberOfNodesInPart(L + 13, 0) == L);
+    EXPECT(FilePartCalculator::calculateNumberOfNodesInPart(L + 13, 1) == 13);
+    bool threw = false;
+    try {
+        FilePartCalculator::calculateNumberOfNodesInPart(L + 13, 2);
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    EXPECT(threw);
+    EXPECT(FilePartCalculator::calculateStartingPostionOfPart(0) == 0);
+    EXPECT(FilePartCalculator::calculateStartingPostionOfPart(1) == L);
+    HostGrid g;
+    g.gridSpacing = 1.0;
+    g.coordinateX = g.coordinateY = g.coordinateZ = {0.0, 1.0};
+    EXPECT(!WriterUtilities::isPeriodicCell(g, 0, 1) && !WriterUtilities::isPeriodicCell(g, 1, 0));
+    for (int axes = 1; axes < 8; ++axes) {
+        HostGrid h = g;
+        if (axes & 1) h.coordinateX[1] = 2.0;
+        if (axes & 2) h.coordinateY[1] = 2.0;
+        if (axes & 4) h.coordinateZ[1] = 2.0;
+        EXPECT(WriterUtilities::isPeriodicCell(h, 0, 1) && WriterUtilities::isPeriodicCell(h, 1, 0));
+    }
+    // an oct of the node indices 7..0 (reversed), neighbours as in the reference's fixture
+    HostGrid o;
+    o.neighborX.assign(8, 0);
+    o.neighborY.assign(8, 0);
+    o.neighborZ.assign(8, 0);
+    const std::array<uint32_t, 8> idx = {7, 6, 5, 4, 3, 2, 1, 0};
+    for (uint32_t i = 0; i < 8; i++) {
+        const uint32_t c = idx[i];
+        o.neighborZ[c] = i < 4 ? idx[i + 4] : 99;
+        o.neighborY[c] = (i == 0 || i == 4) ? idx[i + 3] : (i == 1 || i == 5) ? idx[i + 1] : 999;
+        o.neighborX[c] = (i == 0 || i == 4) ? idx[i + 1] : (i == 3 || i == 7) ? idx[i - 1] : 9999;
+    }
+    std::array<uint32_t, 8> got;
+    WriterUtilities::getIndicesOfAllNodesInOct(got, idx[0], o);
+    EXPECT(got == idx);
+    std::array<uint32_t, 8> rel;
+    WriterUtilities::calculateRelativeNodeIndexInPart(rel, got, 0);
+    EXPECT(rel == idx);
+    WriterUtilities::calculateRelativeNodeIndexInPart(rel, got, 2);
+    EXPECT(rel[0] == 5 && rel[7] == (uint32_t)-2);
+}
+
+// vtu <input> <prefix>: the parts, .pvtu and .pvd of one level from a binary
+// input the Python side wrote (the two writers must give identical files)
+static int vtu_mode(const std::string& input, const std::string& prefix)
+{
+    std::ifstream in(input, std::ios::binary);
+    if (!in) return 2;
+    auto rd = [&](void* p, size_t n) { in.read((char*)p, (std::streamsize)n); };
+    int32_t n = 0, nf = 0, t = 0;
+    rd(&n, 4);
+    HostGrid g;
+    for (auto* v : {&g.coordinateX, &g.coordinateY, &g.coordinateZ}) {
+        v->resize(n);
+        rd(v->data(), 8 * (size_t)n);
+    }
+    for (auto* v : {&g.typeOfGridNode, &g.neighborX, &g.neighborY, &g.neighborZ}) {
+        v->resize(n);
+        rd(v->data(), 4 * (size_t)n);
+    }
+    rd(&g.gridSpacing, 8);
+    rd(&nf, 4);
+    std::vector<std::vector<double>> data(nf, std::vector<double>(n));
+    for (auto& d : data) rd(d.data(), 8 * (size_t)n);
+    rd(&t, 4);
+    if (!in) return 3;
+    std::vector<std::string> names = {"press", "rho", "vx1", "vx2", "vx3", "geo"};
+    if (nf == 7) names.push_back("nut");
+    FileWriter w(prefix, 0);
+    std::vector<std::string> files = w.writeUnstructuredGridLT(g, names, data, 0, (unsigned)t);
+    for (auto& f : files) f = f.substr(f.find_last_of('/') + 1);
+    WbWriterVtkXmlBinary::writeParallelFile(prefix + "_bin_ID_0_t_" + std::to_string(t) + ".vtk", files, names, {});
+    std::map<uint32_t, std::vector<std::string>> series = {{(uint32_t)t, files}, {1000000u, files}};
+    WbWriterVtkXmlBinary::writeCollectionForTimeSeries(prefix + "_bin_ID_0", series, false);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
+    if (argc > 3 && std::strcmp(argv[1], "vtu") == 0) return vtu_mode(argv[2], argv[3]);
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     if (argc > 2 && std::strcmp(argv[1], "restart") == 0) {
         try {
@@ -420,8 +595,12 @@ int main(int argc, char** argv)
             gpu_tests();
             gpu_refinement_tests();
             gpu_turbulence_tests();
+            gpu_output_tests();
         }
-        else cpu_tests();
+        else {
+            cpu_tests();
+            output_unit_tests();
+        }
     } catch (const std::exception& e) {
         std::printf("exception: %s\n", e.what());
         return 2;

@@ -77,6 +77,8 @@ _SIGNATURES = {
     "vfx_write_populations": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
     "vfx_export_distributions": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_int32, C.c_void_p],
     "vfx_turbulent_viscosity_amd": [C.POINTER(VfxLattice), C.c_void_p],
+    "vfx_macroscopic_quantities": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p, C.c_void_p],
     "vfx_import_distributions": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_int32, C.c_void_p],
     "vfx_macroscopic": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                         C.c_void_p],

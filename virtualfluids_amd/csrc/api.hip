@@ -347,6 +347,25 @@ vfx_status vfx_macroscopic(const vfx_lattice* lat, int32_t is_even, double* drho
     return hip_status(e, "macroscopic launch");
 }
 
+vfx_status vfx_macroscopic_quantities(const vfx_lattice* lat, int32_t is_even, void* rho, void* vx, void* vy,
+                                      void* vz, void* pressure, vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    if (!rho || !vx || !vy || !vz || !pressure) return fail(VFX_EINVAL, "macroscopic outputs are NULL");
+    if (!lat->direct && !lat->type_of_grid_node && lat->n_fluid && !lat->fluid_nodes)
+        return fail(VFX_EINVAL, "fluid_nodes is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    const bool even = is_even != 0;
+    switch (lat->storage) {
+    case VFX_FP64: e = vfx::launch_macro_vf<double>(L, lat->direct, even, rho, vx, vy, vz, pressure, st); break;
+    case VFX_FP32: e = vfx::launch_macro_vf<float>(L, lat->direct, even, rho, vx, vy, vz, pressure, st); break;
+    default: e = vfx::launch_macro_vf<_Float16>(L, lat->direct, even, rho, vx, vy, vz, pressure, st); break;
+    }
+    return hip_status(e, "macroscopic quantities launch");
+}
+
 uint64_t vfx_reduce_workspace_bytes(const vfx_lattice* lat)
 {
     if (!lat) return 0;

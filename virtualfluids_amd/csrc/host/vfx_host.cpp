@@ -2,6 +2,7 @@
 #include "vfx_host.hpp"
 
 #include <cmath>
+#include <filesystem>
 #include <cstdio>
 #include <fstream>
 #include <sstream>
@@ -412,5 +413,314 @@ template class ASCIIRestartObjectT<float>;
 template class ASCIIRestartObjectT<double>;
 template class BinaryRestartObjectT<float>;
 template class BinaryRestartObjectT<double>;
+
+} // namespace vfx::host
+
+// ---------------------------------------------------------------- output ----
+namespace vfx::host {
+
+uint32_t FilePartCalculator::calculateNumberOfNodesInPart(uint32_t numberOfNodes, uint32_t indexOfFilePart)
+{
+    const uint32_t last = calculateNumberOfParts(numberOfNodes) - 1;
+    if (indexOfFilePart > last)
+        throw std::runtime_error("The number of nodes for a non-existing part can not be calculated");
+    if (indexOfFilePart == last) return numberOfNodes - indexOfFilePart * limitOfNodesForVTK;
+    return limitOfNodesForVTK;
+}
+uint32_t FilePartCalculator::calculateNumberOfParts(uint32_t numberOfNodes)
+{
+    return numberOfNodes / limitOfNodesForVTK + 1;
+}
+uint32_t FilePartCalculator::calculateStartingPostionOfPart(uint32_t indexOfPart) { return indexOfPart * limitOfNodesForVTK; }
+
+std::string WriterUtilities::makePartFileNameEnding(uint32_t level, int ID, int part, int timestep)
+{
+    return "_lev_" + std::to_string((int)level) + "_ID_" + std::to_string(ID) + "_Part_" + std::to_string(part) +
+           "_t_" + std::to_string(timestep) + ".vtk";
+}
+
+bool WriterUtilities::isPeriodicCell(const HostGrid& g, uint32_t base, uint32_t other)
+{
+    const double distance = std::sqrt(std::pow(g.coordinateX[other] - g.coordinateX[base], 2.) +
+                                      std::pow(g.coordinateY[other] - g.coordinateY[base], 2.) +
+                                      std::pow(g.coordinateZ[other] - g.coordinateZ[base], 2.));
+    return distance > 1.01 * std::sqrt(3 * std::pow(g.gridSpacing, 2.));
+}
+
+void WriterUtilities::getIndicesOfAllNodesInOct(std::array<uint32_t, 8>& n, uint32_t base, const HostGrid& g)
+{
+    n[0] = base;
+    n[1] = g.neighborX[n[0]];
+    n[2] = g.neighborY[n[1]];
+    n[3] = g.neighborY[n[0]];
+    n[4] = g.neighborZ[n[0]];
+    n[5] = g.neighborZ[n[1]];
+    n[6] = g.neighborZ[n[2]];
+    n[7] = g.neighborZ[n[3]];
+}
+
+void WriterUtilities::calculateRelativeNodeIndexInPart(std::array<uint32_t, 8>& rel,
+                                                       const std::array<uint32_t, 8>& octs, uint32_t start)
+{
+    for (size_t i = 0; i < rel.size(); i++) rel[i] = octs[i] - start;
+}
+
+bool WriterUtilities::areAllNodesInOctValidForWriting(const std::array<uint32_t, 8>& octs, const HostGrid& g,
+                                                      uint32_t end)
+{
+    bool fluid = true, out = false;
+    for (uint32_t i : octs) {
+        fluid = fluid && g.typeOfGridNode[i] == 19; // GEO_FLUID
+        out = out || i > end;
+    }
+    return fluid && !out;
+}
+
+static std::ofstream open_output(const std::string& path)
+{
+    std::ofstream out(path, std::ios::out | std::ios::binary);
+    if (!out) {
+        // createFileStream: make the directory and try again
+        const std::filesystem::path dir = std::filesystem::path(path).parent_path();
+        if (!dir.empty()) {
+            std::error_code ec;
+            std::filesystem::create_directories(dir, ec);
+            out.open(path, std::ios::out | std::ios::binary);
+        }
+        if (!out) throw std::runtime_error("couldn't open file " + path);
+    }
+    return out;
+}
+
+std::string WbWriterVtkXmlBinary::writeOctsWithNodeData(const std::string& filename,
+                                                        const std::vector<std::array<float, 3>>& nodes,
+                                                        const std::vector<std::array<uint32_t, 8>>& cells,
+                                                        const std::vector<std::string>& names,
+                                                        const std::vector<std::vector<double>>& data)
+{
+    const std::string path = filename + ".bin.vtu";
+    std::ofstream out = open_output(path);
+    const int n = (int)nodes.size(), m = (int)cells.size();
+    const int bPoints = 3 * n * (int)sizeof(float), bConn = 8 * m * (int)sizeof(int), bOff = m * (int)sizeof(int);
+    const int bTypes = m, bScalar = n * (int)sizeof(double);
+    unsigned long long offset = 0;
+    out << "<?xml version=\"2.0\"?>\n";
+    out << "<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\" >\n";
+    out << "   <UnstructuredGrid>\n";
+    out << "      <Piece NumberOfPoints=\"" << n << "\" NumberOfCells=\"" << m << "\">\n";
+    out << "         <Points>\n";
+    out << "            <DataArray type=\"Float32\" NumberOfComponents=\"3\" format=\"appended\" offset=\"" << offset
+        << "\"  />\n";
+    out << "         </Points>\n";
+    offset += 4 + bPoints;
+    out << "         <Cells>\n";
+    out << "            <DataArray type=\"Int32\" Name=\"connectivity\" format=\"appended\" offset=\"" << offset << "\" />\n";
+    offset += 4 + bConn;
+    out << "            <DataArray type=\"Int32\" Name=\"offsets\" format=\"appended\" offset=\"" << offset << "\" />\n";
+    offset += 4 + bOff;
+    out << "            <DataArray type=\"UInt8\" Name=\"types\" format=\"appended\" offset=\"" << offset << "\" />\n ";
+    offset += 4 + bTypes;
+    out << "         </Cells>\n";
+    out << "         <PointData>\n";
+    for (const std::string& s : names) {
+        out << "            <DataArray type=\"Float64\" Name=\"" << s << "\" format=\"appended\" offset=\"" << offset
+            << "\" /> \n";
+        offset += 4 + bScalar;
+    }
+    out << "         </PointData>\n";
+    out << "      </Piece>\n";
+    out << "   </UnstructuredGrid>\n";
+    out << "   <AppendedData encoding=\"raw\">\n";
+    out << "_";
+    out.write((const char*)&bPoints, 4);
+    for (const auto& p : nodes) out.write((const char*)p.data(), 3 * sizeof(float));
+    out.write((const char*)&bConn, 4);
+    static const int order[8] = {0, 1, 3, 2, 4, 5, 7, 6}; // VTK_VOXEL corner order
+    for (const auto& c : cells)
+        for (int i : order) out.write((const char*)&c[i], 4);
+    out.write((const char*)&bOff, 4);
+    for (int c = 1; c <= m; c++) {
+        const int v = 8 * c;
+        out.write((const char*)&v, 4);
+    }
+    out.write((const char*)&bTypes, 4);
+    const unsigned char voxel = 11;
+    for (int c = 0; c < m; c++) out.write((const char*)&voxel, 1);
+    for (const auto& d : data) {
+        out.write((const char*)&bScalar, 4);
+        out.write((const char*)d.data(), (std::streamsize)(d.size() * sizeof(double)));
+    }
+    out << "\n</AppendedData>\n";
+    out << "</VTKFile>";
+    out << std::endl;
+    return path;
+}
+
+std::string WbWriterVtkXmlBinary::writeParallelFile(const std::string& filename, const std::vector<std::string>& pieces,
+                                                    const std::vector<std::string>& pointNames,
+                                                    const std::vector<std::string>& cellNames)
+{
+    const std::string path = filename + ".pvtu";
+    std::ofstream out = open_output(path);
+    out << "<?xml version=\"1.0\"?>\n";
+    out << "<VTKFile type=\"PUnstructuredGrid\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
+    out << "  <PUnstructuredGrid GhostLevel=\"1\">\n";
+    out << "    <PPointData>\n";
+    for (const auto& s : pointNames) out << "      <PDataArray type=\"Float64\" Name=\"" << s << "\"/>\n";
+    out << "    </PPointData>\n";
+    if (!cellNames.empty()) {
+        out << "    <PCellData>\n";
+        for (const auto& s : cellNames) out << "      <PDataArray type=\"Float64\" Name=\"" << s << "\"/>\n";
+        out << "    </PCellData>\n";
+    }
+    out << "    <PPoints>\n";
+    out << "      <PDataArray type=\"Float32\" NumberOfComponents=\"3\"/>\n";
+    out << "    </PPoints>\n";
+    for (const auto& s : pieces) out << "    <Piece Source=\"" << s << "\"/>\n";
+    out << "  </PUnstructuredGrid>\n";
+    out << "</VTKFile>";
+    out << std::endl;
+    return path;
+}
+
+std::string WbWriterVtkXmlBinary::writeCollectionForTimeSeries(const std::string& filename,
+                                                               const std::map<uint32_t, std::vector<std::string>>& files,
+                                                               bool separateGroups)
+{
+    const std::string path = filename + ".pvd";
+    std::ofstream out = open_output(path);
+    out << "<VTKFile type=\"Collection\" version=\"0.1\" byte_order=\"LittleEndian\" >" << std::endl;
+    out << "   <Collection>" << std::endl;
+    for (const auto& [step, names] : files) {
+        int group = 0, part = 0;
+        for (const auto& f : names) {
+            out << "       <DataSet timestep=\"" << (double)step << "\" group=\"" << group << "\" part=\"" << part
+                << "\" file=\"" << f << "\"/>" << std::endl;
+            if (separateGroups) group++;
+            else part++;
+        }
+    }
+    out << "   </Collection>\n</VTKFile>";
+    return path;
+}
+
+std::vector<std::string> FileWriter::getNodeDataNames(const Parameter& para) const
+{
+    std::vector<std::string> names = {"press", "rho", "vx1", "vx2", "vx3", "geo"};
+    if (para.getUseTurbulentViscosity()) names.push_back("nut");
+    return names;
+}
+
+std::vector<std::string> FileWriter::writeUnstructuredGridLT(const HostGrid& g, const std::vector<std::string>& names,
+                                                             const std::vector<std::vector<double>>& nodeData,
+                                                             int level, unsigned int timestep)
+{
+    const uint32_t numberOfNodes = (uint32_t)g.typeOfGridNode.size();
+    const uint32_t parts = FilePartCalculator::calculateNumberOfParts(numberOfNodes);
+    std::vector<std::array<float, 3>> nodes;
+    std::vector<std::array<uint32_t, 8>> cells;
+    std::vector<std::vector<double>> data(names.size());
+    std::vector<std::string> outNames;
+    for (uint32_t part = 0; part < parts; part++) {
+        const uint32_t size = FilePartCalculator::calculateNumberOfNodesInPart(numberOfNodes, part);
+        const uint32_t start = FilePartCalculator::calculateStartingPostionOfPart(part);
+        const uint32_t end = start + size;
+        cells.clear();
+        nodes.resize(size); // keeps what a previous part left, as the reference
+        for (auto& d : data) d.resize(size);
+        for (uint32_t pos = start; pos < end; pos++) {
+            if (g.typeOfGridNode[pos] != 19) continue; // GEO_FLUID
+            const uint32_t rel = pos - start;
+            nodes[rel] = {(float)g.coordinateX[pos], (float)g.coordinateY[pos], (float)g.coordinateZ[pos]};
+            for (size_t i = 0; i < data.size(); ++i) data[i][rel] = nodeData[i][pos];
+            std::array<uint32_t, 8> octs, relPos;
+            WriterUtilities::getIndicesOfAllNodesInOct(octs, pos, g);
+            if (WriterUtilities::isPeriodicCell(g, octs[0], octs[6])) continue;
+            if (WriterUtilities::areAllNodesInOctValidForWriting(octs, g, end)) {
+                WriterUtilities::calculateRelativeNodeIndexInPart(relPos, octs, start);
+                cells.push_back(relPos);
+            }
+        }
+        const std::string fname =
+            prefix + "_bin" + WriterUtilities::makePartFileNameEnding((uint32_t)level, ID, (int)part + 1, (int)timestep);
+        outNames.push_back(WbWriterVtkXmlBinary::writeOctsWithNodeData(fname, nodes, cells, names, data));
+    }
+    return outNames;
+}
+
+// cudaCopyPrint + writeUnstructuredGridLT of one level
+void FileWriter::writeLevel(const std::shared_ptr<Parameter>& para, unsigned int timestep, int level)
+{
+    LBMSimulationParameter* p = para->getParD(level);
+    const HostGrid& g = p->host;
+    const uint64_t n = p->lattice.n_nodes;
+    if (g.typeOfGridNode.size() != n || g.coordinateX.size() != n || g.neighborX.size() != n)
+        throw std::runtime_error("FileWriter: the level's host grid does not match its lattice");
+    const size_t real = p->lattice.storage == VFX_FP64 ? 8 : 4;
+    void* dev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    std::vector<std::vector<double>> fields(5, std::vector<double>(n));
+    try {
+        for (auto& d : dev) check(vfx_device_alloc(n * real, &d), "FileWriter alloc");
+        check(vfx_macroscopic_quantities(&p->lattice, p->isEvenTimestep ? 1 : 0, dev[0], dev[1], dev[2], dev[3], dev[4],
+                                         p->stream),
+              "calculateMacroscopicQuantities");
+        check(vfx_synchronize(p->stream), "FileWriter sync");
+        std::vector<char> buf(n * real);
+        for (int i = 0; i < 5; ++i) {
+            check(vfx_copy_to_host(buf.data(), dev[i], n * real), "FileWriter d2h");
+            for (uint64_t k = 0; k < n; ++k)
+                fields[i][k] = real == 8 ? ((const double*)buf.data())[k] : (double)((const float*)buf.data())[k];
+        }
+    } catch (...) {
+        for (void* d : dev) vfx_device_free(d);
+        throw;
+    }
+    for (void* d : dev) vfx_device_free(d);
+    // FileWriter.cpp:268-287: press, rho, vx1, vx2, vx3, geo [, nut]
+    std::vector<std::vector<double>> data;
+    for (int i : {4, 0}) {
+        std::vector<double> v(n);
+        for (uint64_t k = 0; k < n; ++k)
+            v[k] = fields[i][k] / (double)3.0 * densityRatio * velocityRatio * velocityRatio;
+        data.push_back(std::move(v));
+    }
+    for (int i : {1, 2, 3}) {
+        std::vector<double> v(n);
+        for (uint64_t k = 0; k < n; ++k) v[k] = fields[i][k] * velocityRatio;
+        data.push_back(std::move(v));
+    }
+    std::vector<double> geo(n);
+    for (uint64_t k = 0; k < n; ++k) geo[k] = (double)g.typeOfGridNode[k];
+    data.push_back(std::move(geo));
+    if (para->getUseTurbulentViscosity()) {
+        if (!p->lattice.turbulent_viscosity) throw std::runtime_error("FileWriter: no turbulent viscosity array");
+        std::vector<char> buf(n * real);
+        check(vfx_copy_to_host(buf.data(), p->lattice.turbulent_viscosity, n * real), "FileWriter d2h");
+        std::vector<double> v(n);
+        for (uint64_t k = 0; k < n; ++k)
+            v[k] = (real == 8 ? ((const double*)buf.data())[k] : (double)((const float*)buf.data())[k]) * viscosityRatio;
+        data.push_back(std::move(v));
+    }
+    for (const std::string& f : writeUnstructuredGridLT(g, getNodeDataNames(*para), data, level, timestep))
+        fileNamesForCollectionFile.push_back(f.substr(f.find_last_of('/') + 1));
+}
+
+void FileWriter::writeInit(const std::shared_ptr<Parameter>& para, unsigned int timestep)
+{
+    writeTimestep(para, timestep, false);
+}
+
+void FileWriter::writeTimestep(const std::shared_ptr<Parameter>& para, unsigned int timestep, bool lastTimestep)
+{
+    for (int level = 0; level <= para->getMaxLevel(); level++) writeLevel(para, timestep, level);
+    fileNamesForCollectionFileTimeSeries[timestep] = fileNamesForCollectionFile;
+    if (lastTimestep)
+        WbWriterVtkXmlBinary::writeCollectionForTimeSeries(prefix + "_bin_ID_" + std::to_string(ID),
+                                                           fileNamesForCollectionFileTimeSeries, false);
+    WbWriterVtkXmlBinary::writeParallelFile(prefix + "_bin_ID_" + std::to_string(ID) + "_t_" + std::to_string(timestep) +
+                                                ".vtk",
+                                            fileNamesForCollectionFile, getNodeDataNames(*para), {});
+    fileNamesForCollectionFile.clear();
+}
 
 } // namespace vfx::host

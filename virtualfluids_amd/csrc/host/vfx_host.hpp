@@ -14,6 +14,7 @@
 
 #include "../../../include/vfx.h"
 
+#include <array>
 #include <functional>
 #include <map>
 #include <memory>
@@ -56,6 +57,14 @@ struct ICellNeigh {
     double* z = nullptr;
 };
 
+// Host copies of a level's grid the output needs (LBMSimulationParameter's
+// coordinateX/Y/Z, typeOfGridNode, neighborX/Y/Z, gridSpacing on the host side)
+struct HostGrid {
+    std::vector<double> coordinateX, coordinateY, coordinateZ;
+    std::vector<uint32_t> typeOfGridNode, neighborX, neighborY, neighborZ;
+    double gridSpacing = 1.0;
+};
+
 // The per-level data of src/gpu/core/Parameter/Parameter.h:66-255 this path uses.
 struct LBMSimulationParameter {
     vfx_lattice lattice{};      // distributions, addressing, omega, limiters, force
@@ -68,6 +77,7 @@ struct LBMSimulationParameter {
     std::vector<CollisionTemplate> allocatedBulkFluidNodeTags{CollisionTemplate::Default};
     QforBoundaryConditions velocityBC, noSlipBC, slipBC, pressureBC;
     double outflowPressureCorrectionFactor = 0.0; // Parameter.h:158
+    HostGrid host;                                // for the output (FileWriter)
     vfx_stream stream = nullptr;
 };
 
@@ -316,5 +326,68 @@ protected:
 using RestartObject = RestartObjectT<float>;
 using ASCIIRestartObject = ASCIIRestartObjectT<float>;
 using BinaryRestartObject = BinaryRestartObjectT<float>;
+
+// ---------------------------------------------------------------- output ----
+// FilePartCalculator (src/gpu/core/Output/FilePartCalculator.cpp:36-58)
+struct FilePartCalculator {
+    static constexpr uint32_t limitOfNodesForVTK = 30000000; // FilePartCalculator.h:53
+    static uint32_t calculateNumberOfNodesInPart(uint32_t numberOfNodes, uint32_t indexOfFilePart);
+    static uint32_t calculateNumberOfParts(uint32_t numberOfNodes);
+    static uint32_t calculateStartingPostionOfPart(uint32_t indexOfPart);
+};
+
+// WriterUtilities (src/gpu/core/Output/WriterUtilities.cpp:38-86)
+struct WriterUtilities {
+    static std::string makePartFileNameEnding(uint32_t level, int ID, int part, int timestep);
+    static bool isPeriodicCell(const HostGrid& grid, uint32_t baseNodeOfCell, uint32_t otherNodeInCell);
+    static void getIndicesOfAllNodesInOct(std::array<uint32_t, 8>& nodeIndices, uint32_t baseNodeOfOct,
+                                          const HostGrid& grid);
+    static void calculateRelativeNodeIndexInPart(std::array<uint32_t, 8>& relativePositionInPart,
+                                                 const std::array<uint32_t, 8>& indicesOfOct, uint32_t startPositionOfPart);
+    static bool areAllNodesInOctValidForWriting(const std::array<uint32_t, 8>& indicesOfOct, const HostGrid& grid,
+                                                uint32_t endPositionOfPart);
+};
+
+// The three writers of WbWriterVtkXmlBinary the output uses
+// (src/basics/writer/WbWriterVtkXmlBinary.cpp:88-177, 1306-1425)
+struct WbWriterVtkXmlBinary {
+    static std::string writeOctsWithNodeData(const std::string& filename, const std::vector<std::array<float, 3>>& nodes,
+                                             const std::vector<std::array<uint32_t, 8>>& cells,
+                                             const std::vector<std::string>& datanames,
+                                             const std::vector<std::vector<double>>& nodedata);
+    static std::string writeParallelFile(const std::string& filename, const std::vector<std::string>& pieceSources,
+                                         const std::vector<std::string>& pointDataNames,
+                                         const std::vector<std::string>& cellDataNames);
+    static std::string writeCollectionForTimeSeries(const std::string& filename,
+                                                    const std::map<uint32_t, std::vector<std::string>>& files,
+                                                    bool separateGroups);
+};
+
+// FileWriter (src/gpu/core/Output/FileWriter.cpp:102-330): at an output step
+// every level's macroscopic quantities (vfx_macroscopic_quantities, plus the
+// turbulent viscosity with a turbulence model) are copied to the host and
+// written as parts of at most 30 M nodes, a .pvtu per step and, at the last
+// step, a .pvd of the time series.  The ratios scale to physical units as the
+// reference's Parameter does (1 = lattice units).
+class FileWriter {
+public:
+    explicit FileWriter(std::string prefix, int ID = 0) : prefix(std::move(prefix)), ID(ID) {}
+    double densityRatio = 1.0, velocityRatio = 1.0, viscosityRatio = 1.0;
+
+    void writeInit(const std::shared_ptr<Parameter>& para, unsigned int timestep);
+    void writeTimestep(const std::shared_ptr<Parameter>& para, unsigned int timestep, bool lastTimestep = false);
+    std::vector<std::string> getNodeDataNames(const Parameter& para) const;
+    // the parts of one level from host data (already scaled), as writeUnstructuredGridLT
+    std::vector<std::string> writeUnstructuredGridLT(const HostGrid& grid, const std::vector<std::string>& names,
+                                                     const std::vector<std::vector<double>>& nodeData, int level,
+                                                     unsigned int timestep);
+
+private:
+    void writeLevel(const std::shared_ptr<Parameter>& para, unsigned int timestep, int level);
+    std::string prefix;
+    int ID;
+    std::vector<std::string> fileNamesForCollectionFile;
+    std::map<uint32_t, std::vector<std::string>> fileNamesForCollectionFileTimeSeries;
+};
 
 } // namespace vfx::host

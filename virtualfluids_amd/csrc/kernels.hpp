@@ -481,6 +481,93 @@ __global__ void __launch_bounds__(256) k_macro(Lat L, int direct, double* drho, 
     vz[k] = jz * oor;
 }
 
+// calculateMacroscopicQuantitiesCompressible_device
+// (PostProcessor/MacroscopicQuantities.cu:194-237): every node's outputs are
+// zeroed, then the fluid nodes get vf::lbm::getDensity, the compressible
+// velocities (numerator / (rho + 1)) and getPressure (MacroscopicQuantities.h:
+// 55-64, 100-115, 139-149) in the lattice's arithmetic type, in the
+// reference's operation order.  LIST = 0 walks every node (direct: not solid;
+// sparse with typeOfGridNode: isValidFluidNode, KernelUtilities.h:216-219, types
+// 19, 5, 6, 7); LIST = 1 computes the nodes of the fluid-node list after a
+// LIST = 0 pass has zeroed everything.
+template <typename T> __device__ __forceinline__ void vf_macroscopic(const T (&f)[27], T& rho, T& vx, T& vy, T& vz,
+                                                                     T& press)
+{
+    rho = ((((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]))) +
+           (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P])) +
+            ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]))) +
+           f[d000]) +
+          ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0]) + (f[d00M] + f[d00P]));
+    const T jx = ((((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dPMM] - f[dMPP]) + (f[dPPM] - f[dMMP]))) +
+                  (((f[dP0M] - f[dM0P]) + (f[dP0P] - f[dM0M])) + ((f[dPM0] - f[dMP0]) + (f[dPP0] - f[dMM0]))) +
+                  (f[dP00] - f[dM00]));
+    const T jy = ((((f[dPPP] - f[dMMM]) + (f[dMPM] - f[dPMP])) + ((f[dMPP] - f[dPMM]) + (f[dPPM] - f[dMMP]))) +
+                  (((f[d0PM] - f[d0MP]) + (f[d0PP] - f[d0MM])) + ((f[dMP0] - f[dPM0]) + (f[dPP0] - f[dMM0]))) +
+                  (f[d0P0] - f[d0M0]));
+    const T jz = ((((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dMPP] - f[dPMM]) + (f[dMMP] - f[dPPM]))) +
+                  (((f[d0MP] - f[d0PM]) + (f[d0PP] - f[d0MM])) + ((f[dM0P] - f[dP0M]) + (f[dP0P] - f[dM0M]))) +
+                  (f[d00P] - f[d00M]));
+    vx = jx / (rho + T(1));
+    vy = jy / (rho + T(1));
+    vz = jz / (rho + T(1));
+    press = (f[dP00] + f[dM00] + f[d0P0] + f[d0M0] + f[d00P] + f[d00M] +
+             T(2) * (f[dPP0] + f[dMM0] + f[dPM0] + f[dMP0] + f[dP0P] + f[dM0M] + f[dP0M] + f[dM0P] + f[d0PP] +
+                     f[d0MM] + f[d0PM] + f[d0MP]) +
+             T(3) * (f[dPPP] + f[dMMP] + f[dPMP] + f[dMPP] + f[dPPM] + f[dMMM] + f[dPMM] + f[dMPM]) - rho -
+             (vx * vx + vy * vy + vz * vz) * (T(1) + rho)) *
+                T(0.5) +
+            rho;
+}
+
+template <typename S, bool EVEN, int LIST>
+__global__ void __launch_bounds__(256) k_macro_vf(Lat L, int direct, typename Compute<S>::T* rho,
+                                                  typename Compute<S>::T* vx, typename Compute<S>::T* vy,
+                                                  typename Compute<S>::T* vz, typename Compute<S>::T* press)
+{
+    using T = typename Compute<S>::T;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t k;
+    bool fluid;
+    if constexpr (LIST) {
+        if (i >= L.nfluid) return;
+        k = L.fluid[i];
+        if (k >= L.N) return;
+        fluid = true;
+    } else {
+        if (i >= L.N) return;
+        k = (uint32_t)i;
+        press[k] = T(0);
+        rho[k] = T(0);
+        vx[k] = T(0);
+        vy[k] = T(0);
+        vz[k] = T(0);
+        if (direct) {
+            fluid = !is_solid(L, k);
+        } else if (L.types) {
+            const uint32_t t = L.types[k];
+            fluid = t == 19u || t == 5u || t == 6u || t == 7u;
+        } else {
+            fluid = false;
+        }
+    }
+    if (!fluid) return;
+    uint32_t kk[8];
+    if (direct) nbrs_direct(L, k, kk);
+    else nbrs_sparse(L, k, kk);
+    T m[27];
+    gather<S, T, EVEN>(L, kk, m);
+    T f[27];
+#pragma unroll
+    for (int s = 0; s < 27; ++s) f[dir_at(s)] = m[s];
+    T r, ux, uy, uz, p;
+    vf_macroscopic<T>(f, r, ux, uy, uz, p);
+    rho[k] = r;
+    vx[k] = ux;
+    vy[k] = uy;
+    vz[k] = uz;
+    press[k] = p;
+}
+
 // ---------------------------------------------------------- reductions ----
 // Deterministic two-pass FP64 sums: every block sums a fixed chunk of
 // kRedBlock * kRedPerThread nodes (per-thread sequential, then a wave64
@@ -1380,6 +1467,22 @@ template <typename S> hipError_t launch_macro(const Lat& L, int direct, bool eve
 }
 
 template <typename S>
+hipError_t launch_macro_vf(const Lat& L, int direct, bool even, void* rho, void* vx, void* vy, void* vz, void* press,
+                           hipStream_t st)
+{
+    using T = typename Compute<S>::T;
+    T *r = (T*)rho, *x = (T*)vx, *y = (T*)vy, *z = (T*)vz, *p = (T*)press;
+    const dim3 b(kBlock);
+    if (even) hipLaunchKernelGGL((k_macro_vf<S, true, 0>), dim3(grid_for(L.N, kBlock)), b, 0, st, L, direct, r, x, y, z, p);
+    else hipLaunchKernelGGL((k_macro_vf<S, false, 0>), dim3(grid_for(L.N, kBlock)), b, 0, st, L, direct, r, x, y, z, p);
+    if (!direct && !L.types && L.nfluid) {
+        if (even) hipLaunchKernelGGL((k_macro_vf<S, true, 1>), dim3(grid_for(L.nfluid, kBlock)), b, 0, st, L, direct, r, x, y, z, p);
+        else hipLaunchKernelGGL((k_macro_vf<S, false, 1>), dim3(grid_for(L.nfluid, kBlock)), b, 0, st, L, direct, r, x, y, z, p);
+    }
+    return hipGetLastError();
+}
+
+template <typename S>
 hipError_t launch_batched(const BatchDev& B, bool even, int variant, unsigned block, unsigned grid, hipStream_t st)
 {
     using T = typename Compute<S>::T;
@@ -1481,6 +1584,7 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);                      \
     template hipError_t launch_vf_array<S>(const Lat&, bool, bool, void*, const void*, hipStream_t);                  \
     template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t);       \
+    template hipError_t launch_macro_vf<S>(const Lat&, int, bool, void*, void*, void*, void*, void*, hipStream_t);     \
     template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);                            \
     template hipError_t launch_reduce_mass<S>(const Lat&, int, bool, double*, double*, hipStream_t);              \
     template hipError_t launch_stats<S>(const Lat&, int, bool, double*, hipStream_t);                             \
@@ -1498,6 +1602,7 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     extern template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);               \
     extern template hipError_t launch_vf_array<S>(const Lat&, bool, bool, void*, const void*, hipStream_t);           \
     extern template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t); \
+    extern template hipError_t launch_macro_vf<S>(const Lat&, int, bool, void*, void*, void*, void*, void*, hipStream_t); \
     extern template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);                     \
     extern template hipError_t launch_reduce_mass<S>(const Lat&, int, bool, double*, double*, hipStream_t);       \
     extern template hipError_t launch_stats<S>(const Lat&, int, bool, double*, hipStream_t);                      \

@@ -206,6 +206,15 @@ class Lattice:
                                            V.VFX_TEMPLATE_WRITE_MACRO if write_macro else V.VFX_TEMPLATE_DEFAULT,
                                            _stream(stream)), "vfx_collide_stream")
 
+    def macroscopic_quantities(self, even=True, stream=None):
+        """calculateMacroscopicQuantitiesCompressible: (rho, vx, vy, vz, press)
+        of every node in the lattice's real type, zero off the fluid nodes."""
+        out = [torch.empty(self.n_nodes, dtype=self._macro_dtype(), device=self.device) for _ in range(5)]
+        s = self.struct()
+        V.check(V.lib().vfx_macroscopic_quantities(C.byref(s), int(bool(even)), *(t.data_ptr() for t in out),
+                                                   _stream(stream)), "vfx_macroscopic_quantities")
+        return out
+
     def turbulent_viscosity_amd(self, stream=None):
         """calcTurbulentViscosityAMD: the AMD eddy viscosity of every fluid
         node from the velocities the last collision wrote."""
