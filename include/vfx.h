@@ -145,9 +145,16 @@ typedef enum {
     VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7, /* OutflowNonReflectingPressureCorrection_Device,
                                           Outflow/OutflowNonReflectingPressureCorrection.cu:51 (pre-collision;
                                           k_neighbor, normal, pressure_correction_factor) */
-    VFX_BC_SLIP_TURBULENT_VISCOSITY = 8 /* SlipTurbulentViscosityCompressible_Device, Slip/SlipTurbulentViscosityCompressible.cu:45:
+    VFX_BC_SLIP_TURBULENT_VISCOSITY = 8, /* SlipTurbulentViscosityCompressible_Device, Slip/SlipTurbulentViscosityCompressible.cu:45:
                                           slip with omega / (1 + 3 omega max(0, nu_t[k])); the lattice needs a
                                           turbulence model and its turbulent_viscosity */
+    /* wall models (BoundaryConditions/Stress/, iMEM.cuh): need k_neighbor (the
+       sampling node), the wm_* arrays and the lattice's vx/vy arrays holding the
+       velocities of this step's collision (collide with VFX_TEMPLATE_WRITE_MACRO) */
+    VFX_BC_STRESS_INTERPOLATED = 9,    /* StressCompressible_Device, Stress/StressCompressible.cu:46 */
+    VFX_BC_STRESS_BOUNCE_BACK = 10,    /* StressBounceBackCompressible_Device, Stress/StressBounceBackCompressible.cu:46 */
+    VFX_BC_STRESS_BOUNCE_BACK_PRESSURE = 11 /* StressBounceBackPressureCompressible_Device,
+                                               Stress/StressBounceBackPressureCompressible.cu:46 */
 } vfx_bc_kind;
 
 /* QforBoundaryConditions (src/gpu/core/Calculation/Calculation.h:105-117) */
@@ -167,6 +174,20 @@ typedef struct vfx_bc {
     /* VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION: Parameter::outflowPressureCorrectionFactor
        (Parameter.h:761, default 0) */
     double pressure_correction_factor;
+    /* Stress kinds: per boundary node, arrays of the lattice's real type (double
+       for VFX_FP64 else float): QforBoundaryConditions normalX/Y/Z, Vx/Vy/Vz
+       (filtered velocity at the sampling node, updated), Vx1/Vy1/Vz1 (filtered
+       velocity at the wall node, updated), WallModelParameters samplingOffset
+       and z0, and the optional monitors u_star, Fx/Fy/Fz (NULL: not written).
+       As the reference's launcher (Stress.cu:57-59), the sampling node's z
+       velocity is read from the lattice's vy array. */
+    const void* wm_normal[3];
+    void* wm_exchange_velocity[3];
+    void* wm_wall_velocity[3];
+    const int32_t* wm_sampling_offset;
+    const void* wm_z0;
+    void* wm_u_star;
+    void* wm_force[3];
 } vfx_bc;
 
 /* ---------------------------------------------------------------- kernels */

@@ -30,7 +30,9 @@ class OrcBC(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n", C.c_int32), ("node", C.POINTER(C.c_int64)),
                 ("q", C.POINTER(C.c_float)), ("ux", _dp), ("uy", _dp), ("uz", _dp),
                 ("cpu_float_velocity", C.c_int32), ("rho", _dp), ("nbr", C.POINTER(C.c_int64)),
-                ("normal", C.c_int32), ("correction", C.c_double)]
+                ("normal", C.c_int32), ("correction", C.c_double),
+                ("wm_normal", _dp * 3), ("wm_el", _dp * 3), ("wm_wall", _dp * 3), ("wm_offset", C.POINTER(C.c_int32)),
+                ("wm_z0", _dp), ("wm_ustar", _dp), ("wm_force", _dp * 3)]
 
 
 def build(ref: bool | None = None) -> None:
@@ -71,6 +73,8 @@ def lib():
         L.orc_interp_bc.argtypes = [C.c_int, _dp, C.POINTER(C.c_float), C.c_double, C.c_double, C.c_double,
                                     C.c_double, _dp, _i32p]
         L.orc_outflow_bc.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, _i32p]
+        L.orc_stress_bc.argtypes = [C.c_int, _dp, C.POINTER(C.c_float), C.c_double, _dp, _dp, C.c_int32, C.c_double,
+                                    _dp, _dp, _dp, _i32p, _dp, _dp]
         L.orc_macroscopic_quantities.argtypes = [_dp, _dp]
         L.orc_k17_collide_turb.argtypes = [_dp, C.c_double, _dp, _dp, _dp, C.c_int, C.c_double, _dp]
         L.orc_turbulent_viscosity_amd.argtypes = [C.c_int, C.c_int, C.c_int, _u8p, _dp, _dp, _dp, C.c_double, _dp]
@@ -177,6 +181,27 @@ def interp_bc(kind, p27, q27, omega, u=(0.0, 0.0, 0.0)):
     return out, wr.astype(bool)
 
 
+def stress_bc(kind, p27, q27, omega, sample_velocity, normal, offset, z0, exchange_velocity=(0, 0, 0),
+              wall_velocity=(0, 0, 0)):
+    """One node of a wall-model boundary (9, 10, 11).  sample_velocity: what
+    the kernel reads at the sampling node (its z from the y array).  Returns
+    (values, written, new exchange velocity, new wall velocity, u_star, force)."""
+    p = np.ascontiguousarray(p27, dtype=np.float64)
+    q = np.ascontiguousarray(q27, dtype=np.float32)
+    vs = np.ascontiguousarray(sample_velocity, dtype=np.float64)
+    nrm = np.ascontiguousarray(normal, dtype=np.float64)
+    el = np.array(exchange_velocity, dtype=np.float64)
+    wm = np.array(wall_velocity, dtype=np.float64)
+    out = np.zeros(27)
+    wr = np.zeros(27, np.int32)
+    us = np.zeros(1)
+    fo = np.zeros(3)
+    lib().orc_stress_bc(int(kind), _p(p), q.ctypes.data_as(C.POINTER(C.c_float)), float(omega), _p(vs), _p(nrm),
+                        int(offset), float(z0), _p(el), _p(wm), _p(out), wr.ctypes.data_as(C.POINTER(C.c_int32)),
+                        _p(us), _p(fo))
+    return out, wr.astype(bool), el, wm, float(us[0]), fo
+
+
 def outflow_bc(kind, f27, fn27, normal, correction=0.0):
     """One node of the non-reflecting outflow (kind 6, or 7 with pressure
     correction): pre-collision f27 of the node and fn27 of its inward
@@ -255,10 +280,15 @@ class BCGroup:
     interpolated, 3 slip, 4 non-equilibrium pressure (rho, nbr, normal),
     5 velocity interpolated with pressure, 6 non-reflecting outflow (nbr,
     normal), 7 non-reflecting outflow with pressure correction (nbr, normal,
-    correction), 8 slip at the node's turbulent-viscosity rate (run_turb)."""
+    correction), 8 slip at the node's turbulent-viscosity rate (run_turb),
+    9/10/11 wall models (interpolated, bounce-back, bounce-back with pressure:
+    nbr = sampling node, wall_normal (3, n), sampling_offset, z0, optional
+    initial exchange / wall velocities (3, n); the filtered velocities and the
+    monitors u_star, force are updated in place)."""
 
     def __init__(self, node, q=None, ux=None, uy=None, uz=None, cpu_float_velocity=False, kind=0, rho=None,
-                 nbr=None, normal=0, correction=0.0):
+                 nbr=None, normal=0, correction=0.0, wall_normal=None, sampling_offset=None, z0=None,
+                 exchange_velocity=None, wall_velocity=None):
         self.node = np.ascontiguousarray(node, dtype=np.int64)
         n = self.node.size
         self.q = np.ascontiguousarray(np.full((27, n), -1.0) if q is None else q, dtype=np.float32)
@@ -271,6 +301,18 @@ class BCGroup:
         self.nbr = np.ascontiguousarray(self.node if nbr is None else nbr, dtype=np.int64)
         self.normal = int(normal)
         self.correction = float(correction)
+        self.stress = kind >= 9
+        if self.stress:
+            self.wall_normal = np.ascontiguousarray(np.broadcast_to(np.asarray(wall_normal, np.float64).reshape(3, -1),
+                                                                    (3, n)), dtype=np.float64).copy()
+            self.sampling_offset = np.ascontiguousarray(np.broadcast_to(sampling_offset, (n,)), dtype=np.int32)
+            self.z0 = np.ascontiguousarray(np.broadcast_to(z0, (n,)), dtype=np.float64)
+            self.exchange_velocity = np.zeros((3, n)) if exchange_velocity is None else \
+                np.array(np.broadcast_to(np.asarray(exchange_velocity, np.float64).reshape(3, -1), (3, n)))
+            self.wall_velocity = np.zeros((3, n)) if wall_velocity is None else \
+                np.array(np.broadcast_to(np.asarray(wall_velocity, np.float64).reshape(3, -1), (3, n)))
+            self.u_star = np.zeros(n)
+            self.force = np.zeros((3, n))
         self.cpu_float_velocity = bool(cpu_float_velocity)
         self.kind = kind
 
@@ -278,7 +320,16 @@ class BCGroup:
         return OrcBC(self.kind, self.node.size, self.node.ctypes.data_as(C.POINTER(C.c_int64)),
                      self.q.ctypes.data_as(C.POINTER(C.c_float)), _p(self.ux), _p(self.uy), _p(self.uz),
                      int(self.cpu_float_velocity), _p(self.rho), self.nbr.ctypes.data_as(C.POINTER(C.c_int64)),
-                     self.normal, self.correction)
+                     self.normal, self.correction, *self._wm())
+
+    def _wm(self):
+        z3 = (_dp * 3)(None, None, None)
+        if not self.stress:
+            return z3, z3, z3, None, None, None, z3
+        rows = lambda a: (_dp * 3)(*(_p(a[j]) for j in range(3)))  # noqa: E731
+        return (rows(self.wall_normal), rows(self.exchange_velocity), rows(self.wall_velocity),
+                self.sampling_offset.ctypes.data_as(C.POINTER(C.c_int32)), _p(self.z0), _p(self.u_star),
+                rows(self.force))
 
 
 def run(F, dims, nsteps, omega, collision="k17", solid=None, bcs=(), lim=LIMITER, force=(0.0, 0.0, 0.0),

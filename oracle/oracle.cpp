@@ -54,7 +54,10 @@ enum {
     ORC_BC_VELOCITY_PRESSURE_INTERPOLATED = 5,
     ORC_BC_OUTFLOW_NON_REFLECTING = 6,
     ORC_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7,
-    ORC_BC_SLIP_TURBULENT_VISCOSITY = 8
+    ORC_BC_SLIP_TURBULENT_VISCOSITY = 8,
+    ORC_BC_STRESS_INTERPOLATED = 9,
+    ORC_BC_STRESS_BOUNCE_BACK = 10,
+    ORC_BC_STRESS_BOUNCE_BACK_PRESSURE = 11
 };
 
 namespace {
@@ -921,7 +924,131 @@ typedef struct {
     const int64_t* nbr;       // pressure / outflow: inward neighbour node
     int32_t normal;           // pressure / outflow: outward direction
     double correction;        // outflow with pressure correction: density correction factor
+    // stress kinds (per boundary node): wall normal, filtered velocities (updated),
+    // sampling offset, roughness length, monitors (NULL: none)
+    const double* wm_normal[3];
+    double* wm_el[3];
+    double* wm_wall[3];
+    const int32_t* wm_offset;
+    const double* wm_z0;
+    double* wm_ustar;
+    double* wm_force[3];
 } orc_bc;
+
+// Wall-model boundaries (StressCompressible.cu, StressBounceBackCompressible.cu,
+// StressBounceBackPressureCompressible.cu with iMEM.cuh:43-141), restated:
+// links visited E W N S T B NE SW SE NW TE BW BE TW TN BS BN TS TNE BSW BNE TSW
+// TSE BNW BSE TNW; the entering population of a wall link is the interpolated
+// no-slip with pressure (mode 0, the NE link's pressure weight 2/27), the
+// bounce-back (1) or the bounce-back minus w drho (2); the wall model (the B
+// link's q, eps = 0.001f, kappa 0.4, log law, clip at twice the sampled
+// tangential velocity) gives a wall velocity whose term 6 w c.U (/ (1 + q) in
+// mode 0) is subtracted.  vel: 3*N velocities of this step's collision; the
+// sampling node's z velocity is taken from the y array (Stress.cu:57-59).
+static const int STRESS_ORDER[26] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 26, 23, 22, 21, 24, 25, 20};
+
+void stress_bc(const orc_bc& bc, int i, const double* p, double omega, const double* vel, int64_t N, double* out,
+               bool* written)
+{
+    const int mode = bc.kind - ORC_BC_STRESS_INTERPOLATED;
+    const double drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] +
+                        p[d0MP] + p[d0PP] + p[d0MM] + p[dP0M] + p[dM0P] + p[dP0P] + p[dM0M] + p[dPM0] + p[dMP0] +
+                        p[dPP0] + p[dMM0] + p[d00P] + p[d00M] + p[d0P0] + p[d0M0] + p[dP00] + p[dM00] + p[d000];
+    const double vx1 = (((p[dPMP] - p[dMPM]) - (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                        ((p[dP0M] - p[dM0P]) + (p[dP0P] - p[dM0M])) + ((p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
+                        (p[dP00] - p[dM00])) / (1.0 + drho);
+    const double vx2 = ((-(p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                        ((p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + (-(p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
+                        (p[d0P0] - p[d0M0])) / (1.0 + drho);
+    const double vx3 = (((p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) + (p[dMMP] - p[dPPM])) +
+                        (-(p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + ((p[dP0P] - p[dM0M]) - (p[dP0M] - p[dM0P])) +
+                        (p[d00P] - p[d00M])) / (1.0 + drho);
+    const double cu_sq = c3o2 * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3) * (1.0 + drho);
+    const float* q = bc.q;
+    auto qd = [&](int d) { return (double)q[(int64_t)d * bc.n + i]; };
+    double fin[27] = {0};
+    double wm[3] = {0, 0, 0};
+    auto add = [&](int d, double s) {
+        const int c[3] = {CX[d], CY[d], CZ[d]};
+        for (int a = 0; a < 3; ++a) {
+            if (c[a] > 0) wm[a] += s;
+            if (c[a] < 0) wm[a] -= s;
+        }
+    };
+    for (int j = 0; j < 26; ++j) {
+        const int d = STRESS_ORDER[j];
+        const double qq = qd(d);
+        if (!(qq >= 0.0 && qq <= 1.0)) continue;
+        const double f = p[d], fi = p[T.inv[d]];
+        double v;
+        if (mode == 0) {
+            const double w = weight(d);
+            const double vlb = proj(d, vx1, vx2, vx3);
+            const double feq = w * (drho + c9o2 * vlb * vlb * (1.0 + drho) - cu_sq);
+            const double wp = d == dPP0 ? 2.0 / 27.0 : w;
+            v = (1.0 - qq) / (1.0 + qq) * (f - fi + (f + fi - 2.0 * feq * omega) / (1.0 - omega)) * c1o2 +
+                (qq * (f + fi)) / (1.0 + qq) - wp * drho;
+        } else if (mode == 1) {
+            v = f;
+        } else {
+            v = f - weight(d) * drho;
+        }
+        fin[T.inv[d]] = v;
+        add(d, f + v);
+    }
+    // iMEM
+    const int64_t kN = bc.nbr[i];
+    const double qB = qd(d00M);
+    const double forceFactor = mode == 0 ? 1.0 + qB : 1.0;
+    const double eps = (double)0.001f;
+    const double nrm[3] = {bc.wm_normal[0][i], bc.wm_normal[1][i], bc.wm_normal[2][i]};
+    const double vs[3] = {vel[kN], vel[N + kN], vel[N + kN]}; // z from the y array
+    const double vw[3] = {vx1, vx2, vx3};
+    double el[3], mw[3];
+    for (int a = 0; a < 3; ++a) {
+        el[a] = eps * vs[a] + (1.0 - eps) * bc.wm_el[a][i];
+        bc.wm_el[a][i] = el[a];
+    }
+    for (int a = 0; a < 3; ++a) {
+        mw[a] = eps * vw[a] + (1.0 - eps) * bc.wm_wall[a][i];
+        bc.wm_wall[a][i] = mw[a];
+    }
+    const double dn_el = el[0] * nrm[0] + el[1] * nrm[1] + el[2] * nrm[2];
+    for (int a = 0; a < 3; ++a) el[a] -= dn_el * nrm[a];
+    const double mag_el = std::sqrt(el[0] * el[0] + el[1] * el[1] + el[2] * el[2]);
+    const double dn_mw = mw[0] * nrm[0] + mw[1] * nrm[1] + mw[2] * nrm[2];
+    for (int a = 0; a < 3; ++a) mw[a] -= dn_mw * nrm[a];
+    const double mag_mw = std::sqrt(mw[0] * mw[0] + mw[1] * mw[1] + mw[2] * mw[2]);
+    const double dn_w = vw[0] * nrm[0] + vw[1] * nrm[1] + vw[2] * nrm[2];
+    const double wt[3] = {vw[0] - dn_w * nrm[0], vw[1] - dn_w * nrm[1], vw[2] - dn_w * nrm[2]};
+    const double z = (double)bc.wm_offset[i] + qB;
+    const double u_star = mag_el * 0.4 / (std::log(z / bc.wm_z0[i]));
+    if (bc.wm_ustar) bc.wm_ustar[i] = u_star;
+    const double tau_w = u_star * u_star;
+    const double mdn = wm[0] * nrm[0] + wm[1] * nrm[1] + wm[2] * nrm[2];
+    const double rho = 1.0 + drho;
+    double U[3];
+    for (int a = 0; a < 3; ++a) {
+        const double Fw = (tau_w * 1.0) * (wt[a] / mag_mw);
+        const double F = Fw - (wm[a] - mdn * nrm[a]) / rho;
+        const double c = 2.0 * el[a];
+        const double v = -3.0 * F * forceFactor;
+        U[a] = c > -c ? std::fmin(c, std::fmax(-c, v)) : std::fmax(c, std::fmin(-c, v));
+    }
+    for (int j = 0; j < 26; ++j) {
+        const int d = STRESS_ORDER[j];
+        const double qq = qd(d);
+        if (!(qq >= 0.0 && qq <= 1.0)) continue;
+        double t = 6.0 * weight(d) * proj(d, U[0], U[1], U[2]);
+        if (mode == 0) t = t / (1.0 + qq);
+        const int e = T.inv[d];
+        out[e] = fin[e] - t;
+        written[e] = true;
+        add(d, -t);
+    }
+    if (bc.wm_force[0])
+        for (int a = 0; a < 3; ++a) bc.wm_force[a][i] = wm[a];
+}
 
 static bool pre_collision_kind(int kind)
 {
@@ -972,6 +1099,35 @@ void orc_macroscopic_quantities(const double* f, double* out5)
     out5[2] = vy;
     out5[3] = vz;
     out5[4] = press;
+}
+
+// one node of a wall-model boundary: post-collision p27, q27, the rate, the
+// sampling node's velocity vs3 (its z component as the kernels read it),
+// normal3, offset, z0; el3 / wall3 filtered velocities in and out; out27,
+// written27; ustar1 and force3 monitors (may be NULL)
+void orc_stress_bc(int kind, const double* p27, const float* q27, double omega, const double* vs3, const double* normal3,
+                   int32_t offset, double z0, double* el3, double* wall3, double* out27, int32_t* written27,
+                   double* ustar1, double* force3)
+{
+    orc_bc bc{};
+    bc.kind = kind;
+    bc.n = 1;
+    bc.q = q27;
+    int64_t kN = 0;
+    bc.nbr = &kN;
+    for (int a = 0; a < 3; ++a) {
+        bc.wm_normal[a] = normal3 + a;
+        bc.wm_el[a] = el3 + a;
+        bc.wm_wall[a] = wall3 + a;
+        bc.wm_force[a] = force3 ? force3 + a : nullptr;
+    }
+    bc.wm_offset = &offset;
+    bc.wm_z0 = &z0;
+    bc.wm_ustar = ustar1;
+    const double vel[3] = {vs3[0], vs3[1], vs3[2]};
+    bool wr[27] = {false};
+    stress_bc(bc, 0, p27, omega, vel, 1, out27, wr);
+    for (int d = 0; d < 27; ++d) written27[d] = wr[d];
 }
 
 void orc_k17_collide(double* f27, double omega, const double* lim3, const double* force3, double* macro4)
@@ -1085,7 +1241,9 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
                     int turb, double sgs, double* nut)
 {
     const int64_t N = (int64_t)nx * ny * nz;
-    std::vector<double> vel(turb == TURB_AMD ? (size_t)N * 3 : 0);
+    bool stress = false;
+    for (int b = 0; b < n_bc; ++b) stress = stress || bcs[b].kind >= ORC_BC_STRESS_INTERPOLATED;
+    std::vector<double> vel(turb == TURB_AMD || stress ? (size_t)N * 3 : 0);
     std::vector<double> post((size_t)N * 27), G((size_t)N * 27);
     static const double zero[3] = {0, 0, 0};
     if (!force3) force3 = zero;
@@ -1145,7 +1303,7 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
             double mv[4];
             if (collision == ORC_K17) k17_collide(f, omega, lim3, force3, mv, turb, sgs, nut ? nut + i : nullptr);
             else bgk_collide(f, omega, mv);
-            if (turb == TURB_AMD)
+            if (!vel.empty())
                 for (int a = 0; a < 3; ++a) vel[(size_t)a * N + i] = mv[1 + a];
             std::memcpy(&post[(size_t)i * 27], f, sizeof(f));
             if (macro && s == nsteps - 1) std::memcpy(macro + 4 * i, mv, sizeof(mv));
@@ -1189,6 +1347,13 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
                 }
                 double out[27];
                 bool wr[27] = {false};
+                if (bc.kind >= ORC_BC_STRESS_INTERPOLATED) {
+                    const double om = nut ? omega / (1.0 + 3.0 * omega * std::max(0.0, nut[k])) : omega;
+                    stress_bc(bc, i, &post[(size_t)k * 27], om, vel.data(), N, out, wr);
+                    for (int d = 0; d < 27; ++d)
+                        if (wr[d]) G[(size_t)k * 27 + d] = out[d];
+                    continue;
+                }
                 // SlipTurbulentViscosityCompressible.cu:130-134: the node's own rate
                 const double om = bc.kind == ORC_BC_SLIP_TURBULENT_VISCOSITY && nut
                                       ? omega / (1.0 + 3.0 * omega * std::max(0.0, nut[k]))

@@ -119,3 +119,61 @@ def test_sparse_layout_with_boundaries(cuda):
     obcs = [oracle.BCGroup(n, q, kind=1) for n, q in groups.values()]
     ref = oracle.run(F0, dims, 25, 1.85, solid=solid, bcs=obcs)
     assert rel_err(got[~solid], ref[~solid]) <= REL
+
+
+# ------------------------------------------------------------- wall models --
+def _wall_channel():
+    """Periodic in x and y, solid layers at z = 0 and z = nz-1; the wall model on
+    the links into the floor (q = 0.5, sampling node 2 above), bounce-back on
+    the lid links."""
+    nx, ny, nz = 6, 5, 12
+    dims = (nx, ny, nz)
+    solid = np.zeros((nz, ny, nx), bool)
+    solid[0] = solid[-1] = True
+    fz, fy, fx = np.nonzero(~solid)
+    floor, lid = {}, {}
+    for z, y, x in zip(fz, fy, fx):
+        k = x + nx * (y + ny * z)
+        for d in range(1, 27):
+            tz = z + cases.CZ[d]
+            if tz == 0:
+                floor.setdefault(k, np.full(27, -1.0, np.float32))[d] = 0.5
+            elif tz == nz - 1:
+                lid.setdefault(k, np.full(27, -1.0, np.float32))[d] = 0.5
+    fn = np.array(sorted(floor), np.int64)
+    ln = np.array(sorted(lid), np.int64)
+    fq = np.stack([floor[k] for k in fn], axis=1)
+    lq = np.stack([lid[k] for k in ln], axis=1)
+    sample = fn + 2 * nx * ny
+    return dims, solid.ravel(), (fn, fq, sample), (ln, lq)
+
+
+@pytest.mark.parametrize("kind", [9, 10, 11])
+def test_wall_model_channel_matches_oracle(cuda, kind):
+    print(f"[wall model kind {kind}]", flush=True)
+    dims, solid, (fn, fq, sample), (ln, lq) = _wall_channel()
+    N = int(np.prod(dims))
+    drho, ux, uy, uz = cases.random_flow(dims, seed=kind, u_amp=0.01)
+    F0 = oracle.init_field(drho, ux + 0.04, uy, uz)
+    force = (2e-6, 0.0, 0.0)
+    normal = np.array([0.0, 0.0, 1.0])[:, None]
+    ex0 = np.array([0.04, 0.0, 0.0])[:, None]
+    lat = vfx.Lattice(dims, storage="fp64", omega=1.9, solid=solid, force=force)
+    lat.write_populations(F0)
+    wall = vfx.BoundaryCondition(kind, fn, fq, neighbors=sample, wall_normal=normal, sampling_offset=2, z0=0.1,
+                                 exchange_velocity=ex0, wall_velocity=ex0)
+    st = vfx.Stepper(lat, "k17", [wall, vfx.BoundaryCondition(0, ln, lq)])
+    st.step(40)
+    got = st.populations().cpu().numpy()
+    og = oracle.BCGroup(fn, fq, kind=kind, nbr=sample, wall_normal=normal, sampling_offset=2, z0=0.1,
+                        exchange_velocity=ex0, wall_velocity=ex0)
+    ref = oracle.run(F0, dims, 40, 1.9, solid=solid, bcs=[og, oracle.BCGroup(ln, lq)], force=force)
+    fl = ~solid
+    errs = [rel_err(got[fl], ref[fl]),
+            rel_err(wall.wm["exchange"].cpu().numpy(), og.exchange_velocity),
+            rel_err(wall.wm["wall"].cpu().numpy(), og.wall_velocity),
+            rel_err(wall.wm["u_star"].cpu().numpy(), og.u_star),
+            rel_err(wall.wm["force"].cpu().numpy(), og.force)]
+    print(f"  populations / exchange / wall / u* / force: {' '.join(f'{e:.1e}' for e in errs)}", flush=True)
+    assert og.u_star.max() > 0
+    assert max(errs) <= REL

@@ -253,3 +253,63 @@ def test_outflow_absorbs_a_pressure_pulse():
     print(f"reflected amplitude: wall {r_wall:.3e}, outflow {r_out:.3e}")
     assert r_wall > 3e-4
     assert r_out < 0.15 * r_wall
+
+
+# ------------------------------------------------------------- wall models --
+def _wall_node(rng):
+    """A node above a floor (links with cz < 0 are wall links, q = 0.5) in a
+    random post-collision state."""
+    p = oracle.equilibrium(rng.uniform(-0.01, 0.01), *rng.uniform(-0.05, 0.05, 3)) + rng.normal(0, 1e-3, 27) * W
+    q = np.full(27, -1.0, np.float32)
+    q[C27[:, 2] < 0] = 0.5
+    return p, q
+
+
+def test_wall_model_without_sampled_velocity_is_no_slip():
+    """Sampled velocity 0 and filtered state 0: the clip bounds are 0, so the
+    wall velocity is 0 -- StressBounceBack is the plain bounce-back, the
+    pressure variant the bounce-back minus w drho, StressCompressible the
+    interpolated no-slip minus w drho (the NE link with 2/27); the force
+    monitor is the momentum exchange 2 sum c_d p_d over the wall links."""
+    rng = np.random.default_rng(50)
+    for _ in range(20):
+        p, q = _wall_node(rng)
+        om = rng.uniform(1.5, 1.95)
+        dr = sum(p)  # plain sum as the kernels take it (difference to the pyramid sum is rounding)
+        links = np.nonzero(q >= 0)[0]
+        v10, w10, *_, f10 = oracle.stress_bc(10, p, q, om, (0, 0, 0), (0, 0, 1), 2, 0.1)
+        v11, w11, *_ = oracle.stress_bc(11, p, q, om, (0, 0, 0), (0, 0, 1), 2, 0.1)
+        v9, w9, *_ = oracle.stress_bc(9, p, q, om, (0, 0, 0), (0, 0, 1), 2, 0.1)
+        inv = cases.INV
+        assert sorted(np.nonzero(w10)[0]) == sorted(inv[links])
+        for d in links:
+            assert v10[inv[d]] == p[d]
+            assert abs(v11[inv[d]] - (p[d] - W[d] * dr)) < 1e-17
+        ns, _ = oracle.interp_bc(1, p, q, om)
+        for d in links:
+            wp = 2 / 27 if d == 7 else W[d]
+            assert abs(v9[inv[d]] - (ns[inv[d]] - wp * dr)) < 1e-16
+        np.testing.assert_allclose(f10, 2 * (C27[links].T * p[links]).sum(axis=1), rtol=0, atol=1e-15)
+
+
+def test_wall_model_log_law_and_clip():
+    """Filtered exchange velocity (U0, V0, 0) over a floor: u* = kappa |U| /
+    ln((offset + q_B) / z0); the wall velocity is -3 F, clipped to twice the
+    filtered tangential velocity; the filters move by eps = 0.001f."""
+    rng = np.random.default_rng(51)
+    for trial in range(20):
+        p, q = _wall_node(rng)
+        U = np.array([rng.uniform(0.01, 0.05), rng.uniform(-0.02, 0.02), 0.0])
+        z0 = [1e-3, 0.5, 2.4][trial % 3]
+        v, w, el, wm, us, force = oracle.stress_bc(10, p, q, 1.9, U, (0, 0, 1), 2, z0, exchange_velocity=U,
+                                                   wall_velocity=(0.01, 0, 0))
+        eps = float(np.float32(0.001))
+        # the sampled z velocity is read from the y array (Stress.cu:57-59)
+        np.testing.assert_allclose(el, eps * U[[0, 1, 1]] + (1 - eps) * U, rtol=1e-15)
+        assert abs(us - 0.4 * np.hypot(U[0], U[1]) / np.log(2.5 / z0)) <= 1e-14 * us
+        # wall velocity from the BW link (-x, -z): out = p - 6 w (-Ux - Uz), Uz = 0
+        bw, te = 12, 11
+        Ux = (v[te] - p[bw]) / (6 * W[bw])
+        assert abs(Ux) <= 2 * el[0] * (1 + 1e-12)
+        if trial % 3 == 2:  # z0 close to z: large stress, the clip holds
+            assert abs(abs(Ux) - 2 * el[0]) < 1e-12

@@ -21,6 +21,7 @@ VFX_BC_SLIP, VFX_BC_PRESSURE_NEQ = 3, 4
 VFX_BC_VELOCITY_PRESSURE_INTERPOLATED, VFX_BC_OUTFLOW_NON_REFLECTING = 5, 6
 VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7
 VFX_BC_SLIP_TURBULENT_VISCOSITY = 8
+VFX_BC_STRESS_INTERPOLATED, VFX_BC_STRESS_BOUNCE_BACK, VFX_BC_STRESS_BOUNCE_BACK_PRESSURE = 9, 10, 11
 
 _u32p = C.POINTER(C.c_uint32)
 _dp = C.POINTER(C.c_double)
@@ -47,6 +48,9 @@ class VfxBC(C.Structure):
         ("vx", C.c_void_p), ("vy", C.c_void_p), ("vz", C.c_void_p), ("cpu_float_velocity", C.c_int32),
         ("rho", C.c_void_p), ("k_neighbor", C.c_void_p), ("normal", C.c_int32),
         ("pressure_correction_factor", C.c_double),
+        ("wm_normal", C.c_void_p * 3), ("wm_exchange_velocity", C.c_void_p * 3), ("wm_wall_velocity", C.c_void_p * 3),
+        ("wm_sampling_offset", C.c_void_p), ("wm_z0", C.c_void_p), ("wm_u_star", C.c_void_p),
+        ("wm_force", C.c_void_p * 3),
     ]
 
 

@@ -229,7 +229,7 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
     vfx::Lat L;
     if (vfx_status s = make_lat(lat, L)) return s;
     if (!bc) return fail(VFX_EINVAL, "bc is NULL");
-    if (bc->kind < VFX_BC_VELOCITY_BOUNCE_BACK || bc->kind > VFX_BC_SLIP_TURBULENT_VISCOSITY)
+    if (bc->kind < VFX_BC_VELOCITY_BOUNCE_BACK || bc->kind > VFX_BC_STRESS_BOUNCE_BACK_PRESSURE)
         return fail(VFX_EUNSUPPORTED, "bc kind %d not supported", bc->kind);
     if (bc->kind == VFX_BC_SLIP_TURBULENT_VISCOSITY && !L.nut)
         return fail(VFX_EINVAL, "slip with turbulent viscosity needs a turbulence model and turbulent_viscosity");
@@ -238,6 +238,15 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
     if (bc->kind == VFX_BC_PRESSURE_NEQ) {
         if (bc->n && (!bc->k || !bc->k_neighbor || !bc->rho)) return fail(VFX_EINVAL, "pressure bc needs k, k_neighbor, rho");
         if (bc->normal < 1 || bc->normal > 26) return fail(VFX_EINVAL, "pressure bc normal %d invalid", bc->normal);
+    } else if (bc->kind >= VFX_BC_STRESS_INTERPOLATED) {
+        if (bc->n && (!bc->k || !bc->q27 || !bc->k_neighbor)) return fail(VFX_EINVAL, "stress bc needs k, q27, k_neighbor");
+        if (bc->n && (!bc->wm_normal[0] || !bc->wm_normal[1] || !bc->wm_normal[2] || !bc->wm_exchange_velocity[0] ||
+                      !bc->wm_exchange_velocity[1] || !bc->wm_exchange_velocity[2] || !bc->wm_wall_velocity[0] ||
+                      !bc->wm_wall_velocity[1] || !bc->wm_wall_velocity[2] || !bc->wm_sampling_offset || !bc->wm_z0))
+            return fail(VFX_EINVAL, "stress bc needs the wall-model arrays");
+        if (bc->wm_force[0] && (!bc->wm_force[1] || !bc->wm_force[2]))
+            return fail(VFX_EINVAL, "stress bc monitors need all three force arrays");
+        if (!lat->vx || !lat->vy) return fail(VFX_EINVAL, "stress bc needs the lattice's velocity arrays");
     } else if (outflow) {
         if (bc->n && (!bc->k || !bc->k_neighbor)) return fail(VFX_EINVAL, "outflow bc needs k, k_neighbor");
         if (bc->normal < 0 || bc->normal > 26) return fail(VFX_EINVAL, "outflow bc normal %d invalid", bc->normal);
@@ -245,7 +254,12 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
         return fail(VFX_EINVAL, "bc.k / bc.q27 is NULL");
     }
     vfx::BCDev B{bc->n, bc->k, bc->q27, bc->vx, bc->vy, bc->vz, bc->cpu_float_velocity, lat->direct,
-                 lat->omega, bc->rho, bc->k_neighbor, bc->normal, bc->pressure_correction_factor};
+                 lat->omega, bc->rho, bc->k_neighbor, bc->normal, bc->pressure_correction_factor,
+                 {bc->wm_normal[0], bc->wm_normal[1], bc->wm_normal[2]},
+                 {bc->wm_exchange_velocity[0], bc->wm_exchange_velocity[1], bc->wm_exchange_velocity[2]},
+                 {bc->wm_wall_velocity[0], bc->wm_wall_velocity[1], bc->wm_wall_velocity[2]},
+                 bc->wm_sampling_offset, bc->wm_z0, bc->wm_u_star,
+                 {bc->wm_force[0], bc->wm_force[1], bc->wm_force[2]}};
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     switch (lat->storage) {

@@ -82,8 +82,9 @@ void KernelImp::run()
     LBMSimulationParameter* p = para->getParD(level);
     vfx_lattice lat = p->lattice;
     set_collision_parameters(lat, *para);
-    check(vfx_collide_stream(&lat, collision, p->isEvenTimestep ? 1 : 0, VFX_TEMPLATE_DEFAULT, p->stream),
-          "Kernel::run");
+    // the wall models sample the velocities of this step's collision
+    const int32_t tmpl = p->stressBC.numberOfBCnodes ? VFX_TEMPLATE_WRITE_MACRO : VFX_TEMPLATE_DEFAULT;
+    check(vfx_collide_stream(&lat, collision, p->isEvenTimestep ? 1 : 0, tmpl, p->stream), "Kernel::run");
 }
 
 // K17CompressibleNavierStokes.cu:102-145: one launch over a list of fluid nodes
@@ -129,6 +130,17 @@ static boundaryCondition make_bc(int32_t kind)
         bc.k_neighbor = q->kN;
         bc.normal = q->direction;
         bc.pressure_correction_factor = p->outflowPressureCorrectionFactor;
+        bc.wm_normal[0] = q->normalX;
+        bc.wm_normal[1] = q->normalY;
+        bc.wm_normal[2] = q->normalZ;
+        for (int a = 0; a < 3; ++a) {
+            bc.wm_exchange_velocity[a] = q->exchangeVelocity[a];
+            bc.wm_wall_velocity[a] = q->wallVelocity[a];
+            bc.wm_force[a] = q->wallForce[a];
+        }
+        bc.wm_sampling_offset = q->samplingOffset;
+        bc.wm_z0 = q->z0;
+        bc.wm_u_star = q->u_star;
         check(vfx_bc_apply(&p->lattice, &bc, p->isEvenTimestep ? 1 : 0, p->stream), "boundary condition");
     };
 }
@@ -158,6 +170,16 @@ boundaryCondition BoundaryConditionFactory::getSlipBoundaryConditionPost() const
     default: return nullptr;
     }
 }
+boundaryCondition BoundaryConditionFactory::getStressBoundaryConditionPost() const
+{
+    switch (stress) {
+    case StressBC::StressCompressible: return make_bc(VFX_BC_STRESS_INTERPOLATED);
+    case StressBC::StressBounceBackCompressible: return make_bc(VFX_BC_STRESS_BOUNCE_BACK);
+    case StressBC::StressBounceBackPressureCompressible: return make_bc(VFX_BC_STRESS_BOUNCE_BACK_PRESSURE);
+    default: return nullptr;
+    }
+}
+
 boundaryCondition BoundaryConditionFactory::getPressureBoundaryConditionPre() const
 {
     switch (pressure) {
@@ -216,7 +238,7 @@ UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared
                            std::shared_ptr<TurbulenceModelFactory> tm)
     : tmFactory(std::move(tm)), para(std::move(p)), kernels(std::move(k)), velocityBC(f.getVelocityBoundaryConditionPost()),
       noSlipBC(f.getNoSlipBoundaryConditionPost()), slipBC(f.getSlipBoundaryConditionPost()),
-      pressureBC(f.getPressureBoundaryConditionPre())
+      pressureBC(f.getPressureBoundaryConditionPre()), stressBC(f.getStressBoundaryConditionPost())
 {
     if ((int)kernels.size() != para->getMaxLevel() + 1) throw std::runtime_error("one kernel per level expected");
     // GridScalingKernelManager.cpp:45-60
@@ -237,6 +259,7 @@ UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared
         require(noSlipBC, d->noSlipBC, "no-slip");
         require(slipBC, d->slipBC, "slip");
         require(pressureBC, d->pressureBC, "pressure");
+        require(stressBC, d->stressBC, "stress");
     }
 }
 
@@ -246,6 +269,7 @@ void UpdateGrid27::postCollisionBC(int level)
     if (p->velocityBC.numberOfBCnodes) velocityBC(p, &p->velocityBC);
     if (p->noSlipBC.numberOfBCnodes) noSlipBC(p, &p->noSlipBC);
     if (p->slipBC.numberOfBCnodes) slipBC(p, &p->slipBC);
+    if (p->stressBC.numberOfBCnodes) stressBC(p, &p->stressBC); // UpdateGrid27.cpp:241-243
 }
 
 void UpdateGrid27::preCollisionBC(int level)

@@ -43,6 +43,18 @@ struct QforBoundaryConditions {
     uint32_t* kN = nullptr;
     uint32_t numberOfBCnodes = 0;
     int direction = 0;          // pressure boundary: outward direction (dP00 = 1 ...)
+    // wall models (stressBC): normalX/Y/Z, Vx/Vy/Vz = filtered exchange velocity,
+    // Vx1/Vy1/Vz1 = filtered wall velocity (lattice real type), and the level's
+    // WallModelParameters samplingOffset, z0 and monitors
+    void* normalX = nullptr;
+    void* normalY = nullptr;
+    void* normalZ = nullptr;
+    void* exchangeVelocity[3] = {nullptr, nullptr, nullptr};
+    void* wallVelocity[3] = {nullptr, nullptr, nullptr};
+    int32_t* samplingOffset = nullptr;
+    void* z0 = nullptr;
+    void* u_star = nullptr;
+    void* wallForce[3] = {nullptr, nullptr, nullptr};
 };
 
 // Calculation.h:72-89: interface cells between a level and the next finer one
@@ -75,7 +87,7 @@ struct LBMSimulationParameter {
     // fluid node lists per collision template (Parameter::taggedFluidNodeIndices)
     std::map<CollisionTemplate, std::pair<uint32_t*, uint32_t>> taggedFluidNodeIndices;
     std::vector<CollisionTemplate> allocatedBulkFluidNodeTags{CollisionTemplate::Default};
-    QforBoundaryConditions velocityBC, noSlipBC, slipBC, pressureBC;
+    QforBoundaryConditions velocityBC, noSlipBC, slipBC, pressureBC, stressBC;
     double outflowPressureCorrectionFactor = 0.0; // Parameter.h:158
     HostGrid host;                                // for the output (FileWriter)
     vfx_stream stream = nullptr;
@@ -215,6 +227,12 @@ public:
     };
     enum class NoSlipBC { NoSlipBounceBack, NoSlipInterpolatedCompressible, NotSpecified };
     enum class SlipBC { SlipCompressible, SlipTurbulentViscosityCompressible, NotSpecified };
+    enum class StressBC {
+        StressCompressible,
+        StressBounceBackCompressible,
+        StressBounceBackPressureCompressible,
+        NotSpecified
+    };
     enum class PressureBC {
         PressureNonEquilibriumCompressible,
         OutflowNonReflective,
@@ -226,17 +244,20 @@ public:
     void setNoSlipBoundaryCondition(NoSlipBC t) { noSlip = t; }
     void setSlipBoundaryCondition(SlipBC t) { slip = t; }
     void setPressureBoundaryCondition(PressureBC t) { pressure = t; }
+    void setStressBoundaryCondition(StressBC t) { stress = t; }
 
     boundaryCondition getVelocityBoundaryConditionPost() const;
     boundaryCondition getNoSlipBoundaryConditionPost() const;
     boundaryCondition getSlipBoundaryConditionPost() const;
     boundaryCondition getPressureBoundaryConditionPre() const;
+    boundaryCondition getStressBoundaryConditionPost() const;
 
 private:
     VelocityBC velocity = VelocityBC::NotSpecified;
     NoSlipBC noSlip = NoSlipBC::NotSpecified;
     SlipBC slip = SlipBC::NotSpecified;
     PressureBC pressure = PressureBC::NotSpecified;
+    StressBC stress = StressBC::NotSpecified;
 };
 
 // GridScalingFactory.h:46-71
@@ -276,7 +297,7 @@ private:
     void refinement(int level);
     std::shared_ptr<Parameter> para;
     std::vector<std::shared_ptr<Kernel>> kernels;
-    boundaryCondition velocityBC, noSlipBC, slipBC, pressureBC;
+    boundaryCondition velocityBC, noSlipBC, slipBC, pressureBC, stressBC;
     gridScaling scalingFineToCoarse, scalingCoarseToFine;
 };
 

@@ -751,6 +751,14 @@ struct BCDev {
     const uint32_t* kn;    // pressure / outflow: inward neighbour of each boundary node
     int32_t normal;        // pressure / outflow: outward direction of the boundary (d000..dMMM)
     double corr;           // outflow with pressure correction: density correction factor
+    // stress (wall-model) kinds, arrays of the lattice's arithmetic type
+    const void* wm_normal[3];     // wall normal per boundary node
+    void* wm_el[3];               // filtered velocity at the exchange location (state)
+    void* wm_wall[3];             // filtered velocity at the wall node (state)
+    const int32_t* wm_offset;     // sampling offset (nodes from the wall)
+    const void* wm_z0;            // roughness length
+    void* wm_ustar;               // monitors (NULL: none)
+    void* wm_force[3];
 };
 
 // c_d . (ux, uy, uz) spelled as VirtualFluids writes it: first non-zero
@@ -1003,6 +1011,179 @@ __global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
     for (int d = 0; d < 27; ++d) {
         if (cx(d) * nx + cy(d) * ny + cz(d) * nz > 0) continue;
         Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], f[d], L.st[weight_class(d)], L.ss[d]);
+    }
+}
+
+// Wall-model (stress) boundaries (BoundaryConditions/Stress/*.cu with the
+// instantaneous wall model iMEM, iMEM.cuh:43-141): post-collision, parity EVEN.
+// MODE 0 StressCompressible (interpolated no-slip with pressure at the node's
+// turbulent-viscosity rate, wall term / (1 + q)), 1 StressBounceBackCompressible,
+// 2 StressBounceBackPressureCompressible.  The links are visited in the
+// kernels' order (E W N S T B NE SW SE NW TE BW BE TW TN BS BN TS TNE BSW BNE
+// TSW TSE BNW BSE TNW), which fixes the wall-momentum sums.  Kept from the
+// reference: the wall model takes q of the B (00M) link, forceFactor 1 + q(B)
+// in MODE 0, the NE link's pressure term in MODE 0 weighs 2/27, and the
+// launcher hands velocityY in for the z velocity (Stress.cu:57-59), which the
+// launch below reproduces by passing the y array twice.
+__host__ __device__ constexpr int stress_link(int j)
+{
+    constexpr int order[26] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 26, 23, 22, 21, 24, 25, 20};
+    return order[j];
+}
+
+template <typename T>
+__device__ __forceinline__ void wall_model(const BCDev& B, uint32_t i, uint32_t kN, const T* vx, const T* vy, const T* vz,
+                                           T vxw, T vyw, T vzw, T rho, T q, T forceFactor, T wmx, T wmy, T wmz,
+                                           T& Ux, T& Uy, T& Uz)
+{
+    const T* nrm[3] = {(const T*)B.wm_normal[0], (const T*)B.wm_normal[1], (const T*)B.wm_normal[2]};
+    T* el[3] = {(T*)B.wm_el[0], (T*)B.wm_el[1], (T*)B.wm_el[2]};
+    T* wm[3] = {(T*)B.wm_wall[0], (T*)B.wm_wall[1], (T*)B.wm_wall[2]};
+    const T eps = T(0.001f);
+    const T nx = nrm[0][i], ny = nrm[1][i], nz = nrm[2][i];
+    T ex = eps * vx[kN] + (T(1.0) - eps) * el[0][i];
+    T ey = eps * vy[kN] + (T(1.0) - eps) * el[1][i];
+    T ez = eps * vz[kN] + (T(1.0) - eps) * el[2][i];
+    el[0][i] = ex;
+    el[1][i] = ey;
+    el[2][i] = ez;
+    T mx = eps * vxw + (T(1.0) - eps) * wm[0][i];
+    T my = eps * vyw + (T(1.0) - eps) * wm[1][i];
+    T mz = eps * vzw + (T(1.0) - eps) * wm[2][i];
+    wm[0][i] = mx;
+    wm[1][i] = my;
+    wm[2][i] = mz;
+    const T dn_el = ex * nx + ey * ny + ez * nz;
+    ex -= dn_el * nx;
+    ey -= dn_el * ny;
+    ez -= dn_el * nz;
+    const T mag_el = sqrt(ex * ex + ey * ey + ez * ez);
+    const T dn_wm = mx * nx + my * ny + mz * nz;
+    mx -= dn_wm * nx;
+    my -= dn_wm * ny;
+    mz -= dn_wm * nz;
+    const T mag_wm = sqrt(mx * mx + my * my + mz * mz);
+    const T dn_w = vxw * nx + vyw * ny + vzw * nz;
+    const T wx = vxw - dn_w * nx, wy = vyw - dn_w * ny, wz = vzw - dn_w * nz;
+    const T z = (T)B.wm_offset[i] + q;
+    const T kappa = T(0.4);
+    const T u_star = mag_el * kappa / (log(z / ((const T*)B.wm_z0)[i]));
+    if (B.wm_ustar) ((T*)B.wm_ustar)[i] = u_star;
+    const T tau_w = u_star * u_star;
+    const T A = T(1.0);
+    const T Fwx = (tau_w * A) * (wx / mag_wm);
+    const T Fwy = (tau_w * A) * (wy / mag_wm);
+    const T Fwz = (tau_w * A) * (wz / mag_wm);
+    const T mdn = wmx * nx + wmy * ny + wmz * nz;
+    const T Fx = Fwx - (wmx - mdn * nx) / rho;
+    const T Fy = Fwy - (wmy - mdn * ny) / rho;
+    const T Fz = Fwz - (wmz - mdn * nz) / rho;
+    const T clip = T(2.0);
+    const T cx_ = clip * ex, cy_ = clip * ey, cz_ = clip * ez;
+    auto clamp = [&](T c, T F) -> T {
+        const T v = T(-3.0) * F * forceFactor;
+        return c > -c ? fmin(c, fmax(-c, v)) : fmax(c, fmin(-c, v));
+    };
+    Ux = clamp(cx_, Fx);
+    Uy = clamp(cy_, Fy);
+    Uz = clamp(cz_, Fz);
+}
+
+template <typename S, bool EVEN, int MODE>
+__global__ void __launch_bounds__(256) k_bc_stress(Lat L, BCDev B)
+{
+    using T = typename Compute<S>::T;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    const uint32_t k = B.k[i], kN = B.kn[i];
+    if (k >= L.N || kN >= L.N) return;
+    uint32_t kk[8];
+    if (B.direct) nbrs_direct(L, k, kk);
+    else nbrs_sparse(L, k, kk);
+    T p[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        const int e = inv(d);
+        p[d] = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)], L.sh[d]);
+    }
+    const T drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] + p[d0MP] +
+                   p[d0PP] + p[d0MM] + p[dP0M] + p[dM0P] + p[dP0P] + p[dM0M] + p[dPM0] + p[dMP0] + p[dPP0] + p[dMM0] +
+                   p[d00P] + p[d00M] + p[d0P0] + p[d0M0] + p[dP00] + p[dM00] + p[d000];
+    const T vx1 = (((p[dPMP] - p[dMPM]) - (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                   ((p[dP0M] - p[dM0P]) + (p[dP0P] - p[dM0M])) + ((p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
+                   (p[dP00] - p[dM00])) /
+                  (T(1) + drho);
+    const T vx2 = ((-(p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                   ((p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + (-(p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
+                   (p[d0P0] - p[d0M0])) /
+                  (T(1) + drho);
+    const T vx3 = (((p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) + (p[dMMP] - p[dPPM])) +
+                   (-(p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + ((p[dP0P] - p[dM0M]) - (p[dP0M] - p[dM0P])) +
+                   (p[d00P] - p[d00M])) /
+                  (T(1) + drho);
+    const T cu_sq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3) * (T(1) + drho);
+    T om = (T)B.omega;
+    if constexpr (MODE == 0) {
+        const T nut = L.nut ? ((const T*)L.nut)[k] : T(0);
+        om = om / (T(1) + T(3) * om * fmax(T(0), nut));
+    }
+    T qs[27];
+#pragma unroll
+    for (int d = 1; d < 27; ++d) qs[d] = (T)B.q[(uint64_t)d * B.n + i];
+    T fin[27] = {};
+    T wmx = T(0), wmy = T(0), wmz = T(0);
+#pragma unroll
+    for (int j = 0; j < 26; ++j) {
+        const int d = stress_link(j);
+        const T q = qs[d];
+        if (!(q >= T(0) && q <= T(1))) continue;
+        const T f = p[d], fi = p[inv(d)];
+        T v;
+        if constexpr (MODE == 0) {
+            const T w = weight<T>(d);
+            const T vlb = proj<T>(d, vx1, vx2, vx3);
+            const T feq = w * (drho + T(4.5) * vlb * vlb * (T(1) + drho) - cu_sq);
+            const T wp = d == dPP0 ? T(2. / 27.) : w; // the NE link's pressure weight
+            v = (T(1) - q) / (T(1) + q) * (f - fi + (f + fi - T(2) * feq * om) / (T(1) - om)) * T(0.5) +
+                (q * (f + fi)) / (T(1) + q) - wp * drho;
+        } else if constexpr (MODE == 1) {
+            v = f;
+        } else {
+            v = f - weight<T>(d) * drho;
+        }
+        fin[inv(d)] = v;
+        const T s = f + v;
+        if (cx(d) > 0) wmx += s;
+        if (cx(d) < 0) wmx -= s;
+        if (cy(d) > 0) wmy += s;
+        if (cy(d) < 0) wmy -= s;
+        if (cz(d) > 0) wmz += s;
+        if (cz(d) < 0) wmz -= s;
+    }
+    const T qB = qs[d00M];
+    T Ux, Uy, Uz;
+    wall_model<T>(B, i, kN, (const T*)L.vx, (const T*)L.vy, (const T*)L.vy, vx1, vx2, vx3, T(1) + drho, qB,
+                  MODE == 0 ? T(1.0) + qB : T(1.0), wmx, wmy, wmz, Ux, Uy, Uz);
+#pragma unroll
+    for (int j = 0; j < 26; ++j) {
+        const int d = stress_link(j);
+        const T q = qs[d];
+        if (!(q >= T(0) && q <= T(1))) continue;
+        T t = T(6) * weight<T>(d) * proj<T>(d, Ux, Uy, Uz);
+        if constexpr (MODE == 0) t = t / (T(1) + q);
+        const int e = inv(d);
+        Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], fin[e] - t, L.st[weight_class(e)], L.ss[e]);
+        if (cx(d) > 0) wmx += -t;
+        if (cx(d) < 0) wmx -= -t;
+        if (cy(d) > 0) wmy += -t;
+        if (cy(d) < 0) wmy -= -t;
+        if (cz(d) > 0) wmz += -t;
+        if (cz(d) < 0) wmz -= -t;
+    }
+    if (B.wm_force[0]) {
+        ((T*)B.wm_force[0])[i] = wmx;
+        ((T*)B.wm_force[1])[i] = wmy;
+        ((T*)B.wm_force[2])[i] = wmz;
     }
 }
 
@@ -1560,6 +1741,18 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     case VFX_BC_SLIP_TURBULENT_VISCOSITY:
         if (even) hipLaunchKernelGGL((k_bc_interp<S, true, VFX_BC_SLIP_TURBULENT_VISCOSITY>), g, b, 0, st, L, B);
         else hipLaunchKernelGGL((k_bc_interp<S, false, VFX_BC_SLIP_TURBULENT_VISCOSITY>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_STRESS_INTERPOLATED:
+        if (even) hipLaunchKernelGGL((k_bc_stress<S, true, 0>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_stress<S, false, 0>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_STRESS_BOUNCE_BACK:
+        if (even) hipLaunchKernelGGL((k_bc_stress<S, true, 1>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_stress<S, false, 1>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_STRESS_BOUNCE_BACK_PRESSURE:
+        if (even) hipLaunchKernelGGL((k_bc_stress<S, true, 2>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_stress<S, false, 2>), g, b, 0, st, L, B);
         break;
     case VFX_BC_OUTFLOW_NON_REFLECTING:
         if (even) hipLaunchKernelGGL((k_bc_outflow<S, true, false>), g, b, 0, st, L, B);

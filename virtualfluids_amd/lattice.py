@@ -226,6 +226,10 @@ class Lattice:
     def apply_bc(self, bc: "BoundaryCondition", even=True, stream=None):
         if bc.max_node >= self.n_nodes:
             raise ValueError("boundary node index out of range")
+        if bc.wm is not None:
+            if bc.real != self._macro_dtype():
+                raise ValueError("wall-model arrays must have the lattice's arithmetic type")
+            self._alloc_macro()
         s = self.struct()
         b = bc.struct()
         V.check(V.lib().vfx_bc_apply(C.byref(s), C.byref(b), int(bool(even)), _stream(stream)), "vfx_bc_apply")
@@ -354,7 +358,14 @@ class BoundaryCondition:
     outflow and 7 non-reflecting outflow with pressure correction
     (pre-collision; ``neighbors``, ``normal`` a face direction 1..6, kind 7
     also ``pressure_correction_factor``), 8 slip with the node's turbulent
-    viscosity (the lattice needs a turbulence model).
+    viscosity (the lattice needs a turbulence model), 9 / 10 / 11 wall models
+    (StressCompressible, StressBounceBackCompressible,
+    StressBounceBackPressureCompressible): ``neighbors`` is the sampling node,
+    ``wall_normal`` (3, n), ``sampling_offset``, ``z0``, optional initial
+    ``exchange_velocity`` / ``wall_velocity`` (3, n); ``real`` is the
+    lattice's arithmetic type (float64 for FP64 storage, else float32).  The
+    filtered velocities (state) and the monitors ``u_star`` and ``wall_force``
+    live on the device and are updated by every call.
 
     The outflow kinds write the populations the collision after next reads
     (the kernel runs after a parity flip and fills the other parity's slots):
@@ -365,9 +376,11 @@ class BoundaryCondition:
     PRE_COLLISION = {V.VFX_BC_PRESSURE_NEQ, V.VFX_BC_OUTFLOW_NON_REFLECTING,
                      V.VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION}
     OUTFLOW = {V.VFX_BC_OUTFLOW_NON_REFLECTING, V.VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION}
+    STRESS = {V.VFX_BC_STRESS_INTERPOLATED, V.VFX_BC_STRESS_BOUNCE_BACK, V.VFX_BC_STRESS_BOUNCE_BACK_PRESSURE}
 
     def __init__(self, kind, nodes, q27=None, vx=None, vy=None, vz=None, cpu_float_velocity=False, rho=None,
-                 neighbors=None, normal=0, device="cuda", pressure_correction_factor=0.0):
+                 neighbors=None, normal=0, device="cuda", pressure_correction_factor=0.0, wall_normal=None,
+                 sampling_offset=None, z0=None, exchange_velocity=None, wall_velocity=None, real=torch.float64):
         nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
         n = nodes.size
         self.kind = int(kind)
@@ -375,6 +388,27 @@ class BoundaryCondition:
         dev = torch.device(device)
         self.max_node = int(nodes.max()) if n else -1
         self.k = torch.from_numpy(nodes.view(np.int32)).to(dev)
+        self.wm = None
+        if self.kind in self.STRESS:
+            if neighbors is None or wall_normal is None or sampling_offset is None or z0 is None:
+                raise ValueError("a wall model needs neighbors (sampling nodes), wall_normal, sampling_offset, z0")
+            kn = np.ascontiguousarray(neighbors, dtype=np.uint32)
+            if kn.size != n:
+                raise ValueError("one sampling node per boundary node")
+            self.max_node = max(self.max_node, int(kn.max()) if n else -1)
+            self.kn = torch.from_numpy(kn.view(np.int32)).to(dev)
+
+            def rows(a, default):
+                a = np.zeros((3, n)) if a is None and default else np.asarray(a, np.float64).reshape(3, -1)
+                return torch.from_numpy(np.array(np.broadcast_to(a, (3, n)), dtype=np.float64)).to(dev, real).contiguous()
+
+            self.wm = {"normal": rows(wall_normal, False), "exchange": rows(exchange_velocity, True),
+                       "wall": rows(wall_velocity, True),
+                       "offset": torch.from_numpy(np.array(np.broadcast_to(sampling_offset, (n,)), np.int32)).to(dev),
+                       "z0": torch.from_numpy(np.array(np.broadcast_to(z0, (n,)), np.float64)).to(dev, real),
+                       "u_star": torch.zeros(n, dtype=real, device=dev),
+                       "force": torch.zeros((3, n), dtype=real, device=dev)}
+            self.real = real
         if self.kind == V.VFX_BC_PRESSURE_NEQ or self.kind in self.OUTFLOW:
             if self.kind == V.VFX_BC_PRESSURE_NEQ and (rho is None or neighbors is None or not 1 <= int(normal) <= 26):
                 raise ValueError("pressure boundary needs rho, neighbors and normal in 1..26")
@@ -386,7 +420,7 @@ class BoundaryCondition:
                 raise ValueError("one inward neighbour per boundary node")
             self.max_node = max(self.max_node, int(kn.max()) if n else -1)
             self.kn = torch.from_numpy(kn.view(np.int32)).to(dev)
-        else:
+        elif self.kind not in self.STRESS:
             self.kn = None
         q = np.ascontiguousarray(q27, dtype=np.float32)
         if q.shape != (27, n):
@@ -420,6 +454,15 @@ class BoundaryCondition:
         b.k_neighbor = _ptr(self.kn) if self.n else None
         b.normal = self.normal
         b.pressure_correction_factor = self.pressure_correction_factor
+        if self.wm is not None and self.n:
+            w = self.wm
+            b.wm_normal[:] = [w["normal"][j].data_ptr() for j in range(3)]
+            b.wm_exchange_velocity[:] = [w["exchange"][j].data_ptr() for j in range(3)]
+            b.wm_wall_velocity[:] = [w["wall"][j].data_ptr() for j in range(3)]
+            b.wm_sampling_offset = w["offset"].data_ptr()
+            b.wm_z0 = w["z0"].data_ptr()
+            b.wm_u_star = w["u_star"].data_ptr()
+            b.wm_force[:] = [w["force"][j].data_ptr() for j in range(3)]
         return b
 
 
@@ -439,8 +482,10 @@ class Stepper:
     def step(self, n=1, stream=None):
         post = [b for b in self.bcs if not b.pre_collision]
         pre = [b for b in self.bcs if b.pre_collision]
+        # wall models sample the velocities the collision writes
+        write_macro = any(getattr(b, "wm", None) is not None for b in self.bcs)
         for _ in range(n):
-            self.lattice.collide_stream(self.collision, self.even, stream=stream)
+            self.lattice.collide_stream(self.collision, self.even, write_macro=write_macro, stream=stream)
             for bc in post:
                 self.lattice.apply_bc(bc, self.even, stream=stream)
             self.even = not self.even
