@@ -43,8 +43,17 @@ typedef enum { VFX_FP64 = 0, VFX_FP32 = 1, VFX_FP16_SCALED = 2 } vfx_storage;
 typedef enum { VFX_K17 = 0, VFX_BGK = 1 } vfx_collision;
 
 /* CollisionTemplate (src/gpu/core/Calculation/Calculation.h:55-66):
- * WriteMacroVars additionally stores drho, vx, vy, vz of every collided node. */
-typedef enum { VFX_TEMPLATE_DEFAULT = 0, VFX_TEMPLATE_WRITE_MACRO = 1 } vfx_collision_template;
+ * WriteMacroVars additionally stores drho, vx, vy, vz of every collided node;
+ * ApplyBodyForce adds the node's body force (vfx_lattice.body_force) to the
+ * global force, force = (forces + body_force[k]) / 2 * force_factor, and resets
+ * body_force[k] to 0 (RunCollision.cuh:88-97); AllFeatures does both (the
+ * reference's SubDomainBorder runs as AllFeatures, K17CompressibleNavierStokes.cu:126-131). */
+typedef enum {
+    VFX_TEMPLATE_DEFAULT = 0,
+    VFX_TEMPLATE_WRITE_MACRO = 1,
+    VFX_TEMPLATE_APPLY_BODY_FORCE = 2,
+    VFX_TEMPLATE_ALL_FEATURES = 3
+} vfx_collision_template;
 
 typedef struct vfx_lattice {
     void* f;               /* 27 * n_nodes populations of the storage type */
@@ -120,6 +129,16 @@ typedef struct vfx_lattice {
      * AMD kernel then takes GEO_FLUID (19) from type_of_grid_node, which must
      * be set.  Direct addressing: neighbours implicit, non-solid = fluid. */
     const uint32_t* neighbor_inverse;
+
+    /* VFX_TEMPLATE_APPLY_BODY_FORCE / _ALL_FEATURES: per-node body force
+     * (LBMSimulationParameter::forceX_SP/Y/Z, n_nodes values, double for VFX_FP64
+     * else float; reset to 0 by the collision), the global force unscaled
+     * (Parameter::forcesDouble) and the level's force factor 1 / 2^level
+     * (getForceFactor, KernelUtilities.h:56-64; must be > 0).  The templates
+     * without body force use `force` above, which is (forces / 2) * force_factor. */
+    void* body_force[3];
+    double forces[3];
+    double force_factor;
 } vfx_lattice;
 
 /* vf::lbm::TurbulenceModel */

@@ -69,6 +69,9 @@ def lib():
         L.orc_run_turb.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _u8p, C.c_double, _dp, _dp,
                                    C.c_int, C.POINTER(OrcBC), C.c_int, _dp, C.c_int, C.c_double, _dp]
         L.orc_run_turb.restype = C.c_int
+        L.orc_run_body_force.argtypes = [_dp, C.c_int, C.c_int, C.c_int, _u8p, C.c_double, _dp, _dp, C.c_double, _dp,
+                                         C.c_int, C.POINTER(OrcBC), C.c_int]
+        L.orc_run_body_force.restype = C.c_int
         _i32p = C.POINTER(C.c_int32)
         L.orc_interp_bc.argtypes = [C.c_int, _dp, C.POINTER(C.c_float), C.c_double, C.c_double, C.c_double,
                                     C.c_double, _dp, _i32p]
@@ -375,6 +378,24 @@ def run_turb(F, dims, nsteps, omega, turb, sgs, nut=None, solid=None, bcs=(), li
     if rc != 0:
         raise RuntimeError(f"orc_run_turb failed ({rc}): a fluid->solid link has no boundary condition")
     return (F, nut, macro) if want_macro else (F, nut)
+
+
+def run_body_force(F, dims, nsteps, omega, forces, force_factor, body_force, solid=None, bcs=(), lim=LIMITER):
+    """`run` with the ApplyBodyForce template: returns (F, body_force after
+    the run -- zero where a node collided)."""
+    nx, ny, nz = dims
+    F = np.ascontiguousarray(F, dtype=np.float64).copy()
+    bf = np.ascontiguousarray(body_force, dtype=np.float64).reshape(3, -1).copy()
+    sol = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8).ravel()
+    bcl = list(bcs)
+    arr = (OrcBC * max(1, len(bcl)))(*[b.c() for b in bcl]) if bcl else None
+    lim = np.ascontiguousarray(lim, dtype=np.float64)
+    g = np.ascontiguousarray(forces, dtype=np.float64)
+    rc = lib().orc_run_body_force(_p(F), nx, ny, nz, sol.ctypes.data_as(_u8p) if sol is not None else None, float(omega),
+                                  _p(lim), _p(g), float(force_factor), _p(bf), len(bcl), arr, int(nsteps))
+    if rc != 0:
+        raise RuntimeError("orc_run_body_force failed")
+    return F, bf
 
 
 def mass_momentum(F, solid=None):

@@ -1177,7 +1177,17 @@ double orc_wall_term(int d, double ux, double uy, double uz, int cpu_float) { re
 // condition covers.
 static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int collision, double omega,
                     const double* lim3, const double* force3, int n_bc, const orc_bc* bcs, int nsteps, double* macro,
-                    int turb, double sgs, double* nut);
+                    int turb, double sgs, double* nut, const double* gforce = nullptr, double ff = 1.0,
+                    double* bf = nullptr);
+
+// orc_run with ApplyBodyForce (RunCollision.cuh:88-97): node i's force is
+// ((gforce + bf[a*N + i]) / 2) * ff, and the collision resets bf to 0
+int orc_run_body_force(double* F, int nx, int ny, int nz, const uint8_t* solid, double omega, const double* lim3,
+                       const double* gforce3, double ff, double* bf, int n_bc, const orc_bc* bcs, int nsteps)
+{
+    return run_impl(F, nx, ny, nz, solid, ORC_K17, omega, lim3, nullptr, n_bc, bcs, nsteps, nullptr, TURB_NONE, 0.0,
+                    nullptr, gforce3, ff, bf);
+}
 
 int orc_run(double* F, int nx, int ny, int nz, const uint8_t* solid, int collision, double omega,
             const double* lim3, const double* force3, int n_bc, const orc_bc* bcs, int nsteps, double* macro)
@@ -1238,7 +1248,7 @@ void orc_turbulent_viscosity_amd(int nx, int ny, int nz, const uint8_t* solid, c
 
 static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int collision, double omega,
                     const double* lim3, const double* force3, int n_bc, const orc_bc* bcs, int nsteps, double* macro,
-                    int turb, double sgs, double* nut)
+                    int turb, double sgs, double* nut, const double* gforce, double ff, double* bf)
 {
     const int64_t N = (int64_t)nx * ny * nz;
     bool stress = false;
@@ -1301,7 +1311,16 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
             double f[27];
             std::memcpy(f, F + 27 * i, sizeof(f));
             double mv[4];
-            if (collision == ORC_K17) k17_collide(f, omega, lim3, force3, mv, turb, sgs, nut ? nut + i : nullptr);
+            double fnode[3];
+            const double* fo = force3;
+            if (bf) {
+                for (int a = 0; a < 3; ++a) {
+                    fnode[a] = (gforce[a] + bf[(size_t)a * N + i]) * c1o2 * ff;
+                    bf[(size_t)a * N + i] = 0.0;
+                }
+                fo = fnode;
+            }
+            if (collision == ORC_K17) k17_collide(f, omega, lim3, fo, mv, turb, sgs, nut ? nut + i : nullptr);
             else bgk_collide(f, omega, mv);
             if (!vel.empty())
                 for (int a = 0; a < 3; ++a) vel[(size_t)a * N + i] = mv[1 + a];

@@ -266,3 +266,21 @@ def test_uncovered_wall_link_is_rejected():
     solid[2, 2, 2] = True
     with pytest.raises(RuntimeError):
         oracle.run(np.zeros((216, 27)), (6, 6, 6), 1, 1.5, solid=solid)
+
+
+# ------------------------------------------------------------ body force --
+def test_body_force_template():
+    """ApplyBodyForce (RunCollision.cuh:88-97): a node's force is
+    ((forces + bodyForce) / 2) * forceFactor and the collision resets the body
+    force; zero body force is the plain force path, a uniform body force c is
+    the plain path with forces + c, and after one step the body force is 0."""
+    dims = (6, 5, 4)
+    F0 = oracle.init_field(*cases.random_flow(dims, seed=3))
+    g, ff = np.array([2e-5, -1e-5, 3e-6]), 0.5
+    n = int(np.prod(dims))
+    F, bf = oracle.run_body_force(F0, dims, 3, 1.8, g, ff, np.zeros((3, n)))
+    assert np.array_equal(F, oracle.run(F0, dims, 3, 1.8, force=g * 0.5 * ff))
+    c = np.array([1e-5, 2e-5, -4e-6])
+    F, bf = oracle.run_body_force(F0, dims, 1, 1.8, g, ff, np.repeat(c[:, None], n, axis=1))
+    assert np.array_equal(F, oracle.run(F0, dims, 1, 1.8, force=(g + c) * 0.5 * ff))
+    assert not bf.any()

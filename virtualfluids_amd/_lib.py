@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libvfx.so")
 VFX_OK, VFX_EINVAL, VFX_EHIP, VFX_EUNSUPPORTED = 0, 1, 2, 3
 VFX_FP64, VFX_FP32, VFX_FP16_SCALED = 0, 1, 2
 VFX_K17, VFX_BGK = 0, 1
-VFX_TEMPLATE_DEFAULT, VFX_TEMPLATE_WRITE_MACRO = 0, 1
+VFX_TEMPLATE_DEFAULT, VFX_TEMPLATE_WRITE_MACRO, VFX_TEMPLATE_APPLY_BODY_FORCE, VFX_TEMPLATE_ALL_FEATURES = 0, 1, 2, 3
 VFX_TURB_NONE, VFX_TURB_SMAGORINSKY, VFX_TURB_AMD, VFX_TURB_QR = 0, 1, 2, 3
 VFX_BC_VELOCITY_BOUNCE_BACK, VFX_BC_NOSLIP_INTERPOLATED, VFX_BC_VELOCITY_INTERPOLATED = 0, 1, 2
 VFX_BC_SLIP, VFX_BC_PRESSURE_NEQ = 3, 4
@@ -39,6 +39,7 @@ class VfxLattice(C.Structure):
         ("type_of_grid_node", C.c_void_p),
         ("turbulence_model", C.c_int32), ("sgs_constant", C.c_double), ("turbulent_viscosity", C.c_void_p),
         ("neighbor_inverse", C.c_void_p),
+        ("body_force", C.c_void_p * 3), ("forces", C.c_double * 3), ("force_factor", C.c_double),
     ]
 
 

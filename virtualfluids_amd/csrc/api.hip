@@ -164,12 +164,24 @@ vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t
     if (collision != VFX_K17 && collision != VFX_BGK) return fail(VFX_EINVAL, "unknown collision %d", collision);
     if (lat->kernel_variant != 0 && lat->kernel_variant != 1)
         return fail(VFX_EINVAL, "unknown kernel_variant %d", lat->kernel_variant);
-    if (tmpl != VFX_TEMPLATE_DEFAULT && tmpl != VFX_TEMPLATE_WRITE_MACRO)
+    if (tmpl < VFX_TEMPLATE_DEFAULT || tmpl > VFX_TEMPLATE_ALL_FEATURES)
         return fail(VFX_EUNSUPPORTED, "collision template %d not supported", tmpl);
+    const bool body_force = tmpl == VFX_TEMPLATE_APPLY_BODY_FORCE || tmpl == VFX_TEMPLATE_ALL_FEATURES;
+    if (body_force) {
+        if (collision != VFX_K17) return fail(VFX_EUNSUPPORTED, "body force templates belong to the K17 kernel");
+        if (!lat->body_force[0] || !lat->body_force[1] || !lat->body_force[2])
+            return fail(VFX_EINVAL, "ApplyBodyForce needs body_force");
+        if (!(lat->force_factor > 0.0)) return fail(VFX_EINVAL, "ApplyBodyForce needs force_factor > 0");
+        for (int a = 0; a < 3; ++a) {
+            L.bf[a] = lat->body_force[a];
+            L.gforce[a] = lat->forces[a];
+        }
+        L.force_factor = lat->force_factor;
+    }
     // the eddy-viscosity models belong to the K17 collision (BGK has none)
     const int turb = collision == VFX_K17 ? lat->turbulence_model : VFX_TURB_NONE;
     // runCollision writes the macroscopic values for the AMD model (RunCollision.cuh:133-138)
-    if (tmpl == VFX_TEMPLATE_WRITE_MACRO || turb == VFX_TURB_AMD) {
+    if (tmpl == VFX_TEMPLATE_WRITE_MACRO || tmpl == VFX_TEMPLATE_ALL_FEATURES || turb == VFX_TURB_AMD) {
         if (!lat->rho || !lat->vx || !lat->vy || !lat->vz)
             return fail(VFX_EINVAL, turb == VFX_TURB_AMD ? "the AMD model needs rho/vx/vy/vz"
                                                          : "WriteMacroVars needs rho/vx/vy/vz");
@@ -178,17 +190,18 @@ vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     const bool even = is_even != 0;
+    const int coll = collision | (body_force ? vfx::kCollBodyForce : 0);
     switch (lat->storage) {
     case VFX_FP64:
-        e = vfx::launch_step<double>(L, lat->direct, collision, even, lat->kernel_variant, lat->omega,
+        e = vfx::launch_step<double>(L, lat->direct, coll, even, lat->kernel_variant, lat->omega,
                                      lat->quadric_limiter, lat->force, turb, lat->sgs_constant, st);
         break;
     case VFX_FP32:
-        e = vfx::launch_step<float>(L, lat->direct, collision, even, lat->kernel_variant, lat->omega,
+        e = vfx::launch_step<float>(L, lat->direct, coll, even, lat->kernel_variant, lat->omega,
                                     lat->quadric_limiter, lat->force, turb, lat->sgs_constant, st);
         break;
     default:
-        e = vfx::launch_step<_Float16>(L, lat->direct, collision, even, lat->kernel_variant, lat->omega,
+        e = vfx::launch_step<_Float16>(L, lat->direct, coll, even, lat->kernel_variant, lat->omega,
                                        lat->quadric_limiter, lat->force, turb, lat->sgs_constant, st);
         break;
     }

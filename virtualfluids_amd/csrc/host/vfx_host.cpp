@@ -18,10 +18,13 @@ void check(vfx_status s, const char* what)
 static int32_t vfx_template(CollisionTemplate t)
 {
     switch (t) {
-    case CollisionTemplate::Default:
-    case CollisionTemplate::SubDomainBorder: return VFX_TEMPLATE_DEFAULT;
+    case CollisionTemplate::Default: return VFX_TEMPLATE_DEFAULT;
     case CollisionTemplate::WriteMacroVars: return VFX_TEMPLATE_WRITE_MACRO;
-    default: throw std::runtime_error("collision template with body force is not supported by the MI355X stepper");
+    case CollisionTemplate::ApplyBodyForce: return VFX_TEMPLATE_APPLY_BODY_FORCE;
+    // K17CompressibleNavierStokes.cu:126-131: the subdomain border runs with all features
+    case CollisionTemplate::SubDomainBorder:
+    case CollisionTemplate::AllFeatures: return VFX_TEMPLATE_ALL_FEATURES;
+    default: throw std::runtime_error("unknown collision template");
     }
 }
 
@@ -97,8 +100,10 @@ void KernelImp::runOnIndices(const unsigned int* indices, unsigned int size_indi
     lat.fluid_nodes = indices;
     lat.n_fluid = size_indices;
     set_collision_parameters(lat, *para);
-    check(vfx_collide_stream(&lat, collision, p->isEvenTimestep ? 1 : 0, vfx_template(collisionTemplate), p->stream),
-          "Kernel::runOnIndices");
+    int32_t tmpl = vfx_template(collisionTemplate);
+    // a level without body-force arrays runs its subdomain border with the macroscopic write only
+    if (collisionTemplate == CollisionTemplate::SubDomainBorder && !lat.body_force[0]) tmpl = VFX_TEMPLATE_WRITE_MACRO;
+    check(vfx_collide_stream(&lat, collision, p->isEvenTimestep ? 1 : 0, tmpl, p->stream), "Kernel::runOnIndices");
 }
 
 std::vector<std::shared_ptr<Kernel>> KernelFactoryImp::makeKernels(std::shared_ptr<Parameter> para)

@@ -110,6 +110,9 @@ struct Lat {
     int32_t write_macro;
     void* nut;             // turbulent viscosity per node (T), eddy-viscosity models
     const uint32_t* nbinv; // sparse: the node in -x-y-z (AMD model)
+    void* bf[3];           // per-node body force (T), ApplyBodyForce
+    double gforce[3];      // global force, unscaled
+    double force_factor;
 };
 
 template <typename S> __device__ __forceinline__ S* plane(const Lat& L, int p) { return (S*)L.f + (uint64_t)p * L.N; }
@@ -257,8 +260,30 @@ __device__ __forceinline__ void scatter_pre(const Lat& L, const uint32_t (&kk)[8
 // reduced-operation K17 (FP32 only); OUTER2 (with FAST) leaves class-c
 // populations scaled by 2^c for a scaling store.  nut: the node's turbulent
 // viscosity in, the model's new value out.
+constexpr int kCollBodyForce = 256;
 constexpr int coll_op(int coll) { return coll & 15; }
-constexpr int coll_turb(int coll) { return coll >> 4; }
+constexpr int coll_turb(int coll) { return (coll >> 4) & 15; }
+constexpr bool coll_body_force(int coll) { return (coll & kCollBodyForce) != 0; }
+
+// runCollision with ApplyBodyForce (RunCollision.cuh:88-97): the node's force
+// ((forces + bodyForce[k]) / 2) * forceFactor, then bodyForce[k] = 0
+template <typename T, int COLL>
+__device__ __forceinline__ void node_force(const Lat& L, uint32_t k, const Rates<T>& R, Rates<T>& Rn)
+{
+    if constexpr (coll_body_force(COLL)) {
+        Rn = R;
+        T* bx = (T*)L.bf[0];
+        T* by = (T*)L.bf[1];
+        T* bz = (T*)L.bf[2];
+        const T ff = (T)L.force_factor;
+        Rn.fx = ((T)L.gforce[0] + bx[k]) * T(0.5) * ff;
+        Rn.fy = ((T)L.gforce[1] + by[k]) * T(0.5) * ff;
+        Rn.fz = ((T)L.gforce[2] + bz[k]) * T(0.5) * ff;
+        bx[k] = T(0);
+        by[k] = T(0);
+        bz[k] = T(0);
+    }
+}
 template <typename T, int COLL, bool FAST, bool OUTER2 = false>
 __device__ __forceinline__ void collide(T (&m)[27], const Rates<T>& R, T (&mv)[4], T& nut)
 {
@@ -325,10 +350,13 @@ __global__ void __launch_bounds__(256) k_step_direct(Lat L, Rates<typename Compu
     nbrs_direct_xyz(L, x, y, z, kk);
     if (is_solid(L, kk[0])) return;
     T m[27], mv[4];
-    constexpr bool O2 = FAST && COLL == VFX_K17 && sizeof(S) == 2;
+    constexpr bool O2 = FAST && coll_op(COLL) == VFX_K17 && coll_turb(COLL) == kTurbNone && sizeof(S) == 2;
     T nut = load_nut<T, COLL>(L, kk[0]);
+    Rates<T> Rn;
+    node_force<T, COLL>(L, kk[0], R, Rn);
+    const Rates<T>& Ru = coll_body_force(COLL) ? Rn : R;
     gather<S, T, EVEN>(L, kk, m);
-    collide<T, COLL, FAST, O2>(m, R, mv, nut);
+    collide<T, COLL, FAST, O2>(m, Ru, mv, nut);
     scatter_post<S, T, EVEN, O2>(L, kk, m);
     if (L.write_macro) write_macro<T>(L, kk[0], mv);
     store_nut<T, COLL>(L, kk[0], nut);
@@ -345,10 +373,13 @@ __global__ void __launch_bounds__(256) k_step_sparse(Lat L, Rates<typename Compu
     if (!sparse_node(L, i, k)) return;
     nbrs_sparse(L, k, kk);
     T m[27], mv[4];
-    constexpr bool O2 = FAST && COLL == VFX_K17 && sizeof(S) == 2;
+    constexpr bool O2 = FAST && coll_op(COLL) == VFX_K17 && coll_turb(COLL) == kTurbNone && sizeof(S) == 2;
     T nut = load_nut<T, COLL>(L, kk[0]);
+    Rates<T> Rn;
+    node_force<T, COLL>(L, kk[0], R, Rn);
+    const Rates<T>& Ru = coll_body_force(COLL) ? Rn : R;
     gather<S, T, EVEN>(L, kk, m);
-    collide<T, COLL, FAST, O2>(m, R, mv, nut);
+    collide<T, COLL, FAST, O2>(m, Ru, mv, nut);
     scatter_post<S, T, EVEN, O2>(L, kk, m);
     if (L.write_macro) write_macro<T>(L, kk[0], mv);
     store_nut<T, COLL>(L, kk[0], nut);
@@ -1559,13 +1590,28 @@ template <typename S> hipError_t launch_step(const Lat& L, int direct, int coll,
 {
     using T = typename Compute<S>::T;
     const Rates<T> R = make_rates<T>(omega, lim, force, sgs);
+    const bool bf = (coll & kCollBodyForce) != 0; // ApplyBodyForce (K17 only)
+    coll &= 15;
     const bool fast = sizeof(T) == 4 && variant == 0 && coll == VFX_K17;
+    constexpr int K = VFX_K17, BF = kCollBodyForce;
     if (coll == VFX_K17 && turb != kTurbNone) {
         switch (turb) {
-        case kTurbSmagorinsky: return launch_step_p<S, VFX_K17 | (kTurbSmagorinsky << 4)>(L, direct, even, R, st);
-        case kTurbAMD: return launch_step_p<S, VFX_K17 | (kTurbAMD << 4)>(L, direct, even, R, st);
-        default: return launch_step_p<S, VFX_K17 | (kTurbQR << 4)>(L, direct, even, R, st);
+        case kTurbSmagorinsky:
+            return bf ? launch_step_p<S, K | (kTurbSmagorinsky << 4) | BF>(L, direct, even, R, st)
+                      : launch_step_p<S, K | (kTurbSmagorinsky << 4)>(L, direct, even, R, st);
+        case kTurbAMD:
+            return bf ? launch_step_p<S, K | (kTurbAMD << 4) | BF>(L, direct, even, R, st)
+                      : launch_step_p<S, K | (kTurbAMD << 4)>(L, direct, even, R, st);
+        default:
+            return bf ? launch_step_p<S, K | (kTurbQR << 4) | BF>(L, direct, even, R, st)
+                      : launch_step_p<S, K | (kTurbQR << 4)>(L, direct, even, R, st);
         }
+    }
+    if (coll == VFX_K17 && bf) {
+        if (fast) return even ? launch_step_t<S, K | BF, true, true>(L, direct, R, st)
+                              : launch_step_t<S, K | BF, false, true>(L, direct, R, st);
+        return even ? launch_step_t<S, K | BF, true, false>(L, direct, R, st)
+                    : launch_step_t<S, K | BF, false, false>(L, direct, R, st);
     }
     if (coll == VFX_K17) {
         if (fast) return even ? launch_step_t<S, VFX_K17, true, true>(L, direct, R, st)

@@ -130,6 +130,11 @@ class Lattice:
             raise ValueError(f"turbulence must be one of {sorted(TURBULENCE)}")
         self.turbulence = turbulence
         self.sgs_constant = float(sgs_constant)
+        # ApplyBodyForce: per-node body force (3, n_nodes) in the arithmetic
+        # type, the global force unscaled and the level's force factor 1/2^level
+        self.body_force = None
+        self.forces = (0.0, 0.0, 0.0)
+        self.force_factor = 1.0
         self.nut = None
         self.nbinv = None
         if turbulence != "none":
@@ -175,6 +180,10 @@ class Lattice:
             s.rho, s.vx, s.vy, s.vz = (t.data_ptr() for t in self.macro)
         s.kernel_variant = 0 if self.variant == "default" else 1
         s.fp16_reference_velocity[:] = list(getattr(self, "reference_velocity", (0.0, 0.0, 0.0)))
+        if getattr(self, "body_force", None) is not None:
+            s.body_force[:] = [self.body_force[a].data_ptr() for a in range(3)]
+            s.forces[:] = list(self.forces)
+            s.force_factor = self.force_factor
         if getattr(self, "nut", None) is not None:
             s.turbulence_model = TURBULENCE[self.turbulence]
             s.sgs_constant = self.sgs_constant
@@ -198,13 +207,25 @@ class Lattice:
         V.check(V.lib().vfx_init_equilibrium(C.byref(s), int(bool(even)), *(a.data_ptr() for a in f64),
                                              _stream(stream)), "vfx_init_equilibrium")
 
-    def collide_stream(self, collision="k17", even=True, write_macro=False, stream=None):
+    def set_body_force(self, fx, fy, fz, forces=(0.0, 0.0, 0.0), force_factor=1.0):
+        """Per-node body force for the ApplyBodyForce collision (read once, then
+        reset to 0 by it), with the global force and the level's force factor."""
+        bf = torch.stack([torch.as_tensor(np.broadcast_to(np.asarray(a, np.float64), (self.n_nodes,)).copy())
+                          for a in (fx, fy, fz)]).to(self.device, self._macro_dtype()).contiguous()
+        self.body_force = bf
+        self.forces = tuple(float(x) for x in forces)
+        self.force_factor = float(force_factor)
+
+    def collide_stream(self, collision="k17", even=True, write_macro=False, stream=None, apply_body_force=False):
         if write_macro:
             self._alloc_macro()
+        if apply_body_force and self.body_force is None:
+            raise ValueError("apply_body_force needs set_body_force first")
+        tmpl = (V.VFX_TEMPLATE_ALL_FEATURES if write_macro else V.VFX_TEMPLATE_APPLY_BODY_FORCE) if apply_body_force \
+            else (V.VFX_TEMPLATE_WRITE_MACRO if write_macro else V.VFX_TEMPLATE_DEFAULT)
         s = self.struct()
-        V.check(V.lib().vfx_collide_stream(C.byref(s), COLLISION[collision], int(bool(even)),
-                                           V.VFX_TEMPLATE_WRITE_MACRO if write_macro else V.VFX_TEMPLATE_DEFAULT,
-                                           _stream(stream)), "vfx_collide_stream")
+        V.check(V.lib().vfx_collide_stream(C.byref(s), COLLISION[collision], int(bool(even)), tmpl, _stream(stream)),
+                "vfx_collide_stream")
 
     def macroscopic_quantities(self, even=True, stream=None):
         """calculateMacroscopicQuantitiesCompressible: (rho, vx, vy, vz, press)
@@ -478,6 +499,7 @@ class Stepper:
     collision: str = "k17"
     bcs: list = field(default_factory=list)
     even: bool = True
+    apply_body_force: bool = False  # CollisionTemplate ApplyBodyForce (Lattice.set_body_force)
 
     def step(self, n=1, stream=None):
         post = [b for b in self.bcs if not b.pre_collision]
@@ -485,7 +507,8 @@ class Stepper:
         # wall models sample the velocities the collision writes
         write_macro = any(getattr(b, "wm", None) is not None for b in self.bcs)
         for _ in range(n):
-            self.lattice.collide_stream(self.collision, self.even, write_macro=write_macro, stream=stream)
+            self.lattice.collide_stream(self.collision, self.even, write_macro=write_macro, stream=stream,
+                                        apply_body_force=self.apply_body_force)
             for bc in post:
                 self.lattice.apply_bc(bc, self.even, stream=stream)
             self.even = not self.even
