@@ -172,8 +172,24 @@ typedef enum {
        velocities of this step's collision (collide with VFX_TEMPLATE_WRITE_MACRO) */
     VFX_BC_STRESS_INTERPOLATED = 9,    /* StressCompressible_Device, Stress/StressCompressible.cu:46 */
     VFX_BC_STRESS_BOUNCE_BACK = 10,    /* StressBounceBackCompressible_Device, Stress/StressBounceBackCompressible.cu:46 */
-    VFX_BC_STRESS_BOUNCE_BACK_PRESSURE = 11 /* StressBounceBackPressureCompressible_Device,
+    VFX_BC_STRESS_BOUNCE_BACK_PRESSURE = 11, /* StressBounceBackPressureCompressible_Device,
                                                Stress/StressBounceBackPressureCompressible.cu:46 */
+    /* Precursor inflows (QforPrecursorBoundaryConditions, Calculation.h:120-150): the
+       plane data of the precursor nodes, interpolated to each boundary node with
+       four inverse-distance weights and blended in time,
+       last*(1 - time_ratio) + current*time_ratio.  Post-collision, the last
+       boundary condition of UpdateGrid27::postCollisionBC (UpdateGrid27.cpp:252). */
+    /* PrecursorNonReflectiveCompressible_Device (Precursor/PrecursorNonReflectiveCompressible.cu:46):
+       velocity inlet with pressure on the q links, velocity = (vx + blended
+       plane velocity) / velocity_ratio; planes hold vx, vy, vz of every
+       precursor node (3 * n_precursor_nodes).  Writes the parity's own slots, so
+       a value is read by the node's collision two steps later. */
+    VFX_BC_PRECURSOR_VELOCITY = 12,
+    /* PrecursorDistributions_Device (Precursor/PrecursorDistributions.cu:44): sets
+       the nine populations with a positive x component (dP00, dPP0, dPM0, dP0P,
+       dP0M, dPPP, dPMP, dPPM, dPMM); planes hold those nine of every precursor
+       node (9 * n_precursor_nodes).  No q links. */
+    VFX_BC_PRECURSOR_DISTRIBUTIONS = 13
 } vfx_bc_kind;
 
 /* QforBoundaryConditions (src/gpu/core/Calculation/Calculation.h:105-117) */
@@ -207,6 +223,18 @@ typedef struct vfx_bc {
     const void* wm_z0;
     void* wm_u_star;
     void* wm_force[3];
+    /* Precursor kinds: planeNeighbor0PP/0PM/0MP/0MM (precursor node of each
+       boundary node), weights0PP/... (arrays of the lattice's real type; a first
+       weight >= 1e6 takes the 0PP node alone), the plane arrays last and current
+       of n_precursor_nodes nodes (real type), the time ratio and
+       Parameter::getVelocityRatio() (velocity kind only). */
+    uint32_t n_precursor_nodes;
+    const uint32_t* plane_neighbor[4];
+    const void* plane_weight[4];
+    const void* precursor_last;
+    const void* precursor_current;
+    double time_ratio;
+    double velocity_ratio;
 } vfx_bc;
 
 /* ---------------------------------------------------------------- kernels */

@@ -32,7 +32,9 @@ class OrcBC(C.Structure):
                 ("cpu_float_velocity", C.c_int32), ("rho", _dp), ("nbr", C.POINTER(C.c_int64)),
                 ("normal", C.c_int32), ("correction", C.c_double),
                 ("wm_normal", _dp * 3), ("wm_el", _dp * 3), ("wm_wall", _dp * 3), ("wm_offset", C.POINTER(C.c_int32)),
-                ("wm_z0", _dp), ("wm_ustar", _dp), ("wm_force", _dp * 3)]
+                ("wm_z0", _dp), ("wm_ustar", _dp), ("wm_force", _dp * 3),
+                ("n_pre", C.c_int32), ("pn", C.POINTER(C.c_int64) * 4), ("pw", _dp * 4), ("plast", _dp),
+                ("pcur", _dp), ("tratio", C.c_double), ("vratio", C.c_double)]
 
 
 def build(ref: bool | None = None) -> None:
@@ -76,6 +78,8 @@ def lib():
         L.orc_interp_bc.argtypes = [C.c_int, _dp, C.POINTER(C.c_float), C.c_double, C.c_double, C.c_double,
                                     C.c_double, _dp, _i32p]
         L.orc_outflow_bc.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, _i32p]
+        _i64p = C.POINTER(C.c_int64)
+        L.orc_precursor_blend.argtypes = [C.c_int, C.c_int, C.c_int] + [_i64p] * 4 + [_dp] * 6 + [C.c_double, _dp]
         L.orc_stress_bc.argtypes = [C.c_int, _dp, C.POINTER(C.c_float), C.c_double, _dp, _dp, C.c_int32, C.c_double,
                                     _dp, _dp, _dp, _i32p, _dp, _dp]
         L.orc_macroscopic_quantities.argtypes = [_dp, _dp]
@@ -168,6 +172,22 @@ def macroscopic_quantities(f27, impl="oracle"):
     out = np.zeros(5)
     fn = lib().orc_macroscopic_quantities if impl == "oracle" else ref().ref_macroscopic_quantities
     fn(_p(f), _p(out))
+    return out
+
+
+def precursor_blend(neighbors, weights, last, current, time_ratio):
+    """Plane values at the boundary nodes, last*(1 - tr) + current*tr of the
+    inverse-distance weighted four precursor nodes (first weight >= 1e6: the
+    0PP node alone); planes (components, n_pre) -> (components, n)."""
+    nb = np.ascontiguousarray(neighbors, dtype=np.int64).reshape(4, -1)
+    w = np.ascontiguousarray(weights, dtype=np.float64).reshape(4, -1)
+    la = np.ascontiguousarray(last, dtype=np.float64)
+    cu = np.ascontiguousarray(current, dtype=np.float64)
+    comps, n_pre = la.shape
+    n = nb.shape[1]
+    out = np.zeros((comps, n))
+    lib().orc_precursor_blend(n, n_pre, comps, *(nb[j].ctypes.data_as(C.POINTER(C.c_int64)) for j in range(4)),
+                              *(_p(w[j]) for j in range(4)), _p(la), _p(cu), float(time_ratio), _p(out))
     return out
 
 
@@ -287,11 +307,14 @@ class BCGroup:
     9/10/11 wall models (interpolated, bounce-back, bounce-back with pressure:
     nbr = sampling node, wall_normal (3, n), sampling_offset, z0, optional
     initial exchange / wall velocities (3, n); the filtered velocities and the
-    monitors u_star, force are updated in place)."""
+    monitors u_star, force are updated in place), 12 precursor velocity inlet
+    (q links, ux/uy/uz added to the plane velocity), 13 precursor
+    distributions (no q): precursor = dict(neighbors (4, n), weights (4, n),
+    last, current (components x n_pre), time_ratio, velocity_ratio)."""
 
     def __init__(self, node, q=None, ux=None, uy=None, uz=None, cpu_float_velocity=False, kind=0, rho=None,
                  nbr=None, normal=0, correction=0.0, wall_normal=None, sampling_offset=None, z0=None,
-                 exchange_velocity=None, wall_velocity=None):
+                 exchange_velocity=None, wall_velocity=None, precursor=None):
         self.node = np.ascontiguousarray(node, dtype=np.int64)
         n = self.node.size
         self.q = np.ascontiguousarray(np.full((27, n), -1.0) if q is None else q, dtype=np.float32)
@@ -304,7 +327,16 @@ class BCGroup:
         self.nbr = np.ascontiguousarray(self.node if nbr is None else nbr, dtype=np.int64)
         self.normal = int(normal)
         self.correction = float(correction)
-        self.stress = kind >= 9
+        self.stress = 9 <= kind <= 11
+        self.pre = None
+        if kind in (12, 13):
+            pc = precursor
+            self.pre = {"nb": np.ascontiguousarray(pc["neighbors"], dtype=np.int64).reshape(4, n),
+                        "w": np.ascontiguousarray(pc["weights"], dtype=np.float64).reshape(4, n),
+                        "last": np.ascontiguousarray(pc["last"], dtype=np.float64),
+                        "cur": np.ascontiguousarray(pc["current"], dtype=np.float64),
+                        "tr": float(pc.get("time_ratio", 0.0)), "vr": float(pc.get("velocity_ratio", 1.0))}
+            self.pre["n_pre"] = self.pre["last"].shape[-1]
         if self.stress:
             self.wall_normal = np.ascontiguousarray(np.broadcast_to(np.asarray(wall_normal, np.float64).reshape(3, -1),
                                                                     (3, n)), dtype=np.float64).copy()
@@ -323,7 +355,14 @@ class BCGroup:
         return OrcBC(self.kind, self.node.size, self.node.ctypes.data_as(C.POINTER(C.c_int64)),
                      self.q.ctypes.data_as(C.POINTER(C.c_float)), _p(self.ux), _p(self.uy), _p(self.uz),
                      int(self.cpu_float_velocity), _p(self.rho), self.nbr.ctypes.data_as(C.POINTER(C.c_int64)),
-                     self.normal, self.correction, *self._wm())
+                     self.normal, self.correction, *self._wm(), *self._pre())
+
+    def _pre(self):
+        if self.pre is None:
+            return 0, (C.POINTER(C.c_int64) * 4)(), (_dp * 4)(), None, None, 0.0, 1.0
+        p = self.pre
+        return (p["n_pre"], (C.POINTER(C.c_int64) * 4)(*(p["nb"][j].ctypes.data_as(C.POINTER(C.c_int64)) for j in range(4))),
+                (_dp * 4)(*(_p(p["w"][j]) for j in range(4))), _p(p["last"]), _p(p["cur"]), p["tr"], p["vr"])
 
     def _wm(self):
         z3 = (_dp * 3)(None, None, None)

@@ -39,6 +39,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <array>
 #include <vector>
 #include <algorithm>
 #ifdef _OPENMP
@@ -57,7 +58,9 @@ enum {
     ORC_BC_SLIP_TURBULENT_VISCOSITY = 8,
     ORC_BC_STRESS_INTERPOLATED = 9,
     ORC_BC_STRESS_BOUNCE_BACK = 10,
-    ORC_BC_STRESS_BOUNCE_BACK_PRESSURE = 11
+    ORC_BC_STRESS_BOUNCE_BACK_PRESSURE = 11,
+    ORC_BC_PRECURSOR_VELOCITY = 12,
+    ORC_BC_PRECURSOR_DISTRIBUTIONS = 13
 };
 
 namespace {
@@ -580,7 +583,13 @@ void interp_bc(int kind, const double* p, const float* q, int n, int i, double o
         const double w = weight(d);
         const double vlb = proj(d, vx1, vx2, vx3);
         const double feq = w * (drho + c9o2 * vlb * vlb * (1.0 + drho) - cu_sq);
-        const double f = p[d], fi = p[T.inv[d]];
+        double f = p[d], fi = p[T.inv[d]];
+        if (kind == ORC_BC_PRECURSOR_VELOCITY) {
+            // PrecursorNonReflectiveCompressible.cu:383, :392: f_0PP stands in
+            // for f_0MP on the d0PM and d0MP links
+            if (d == d0PM) fi = p[d0PP];
+            if (d == d0MP) f = p[d0PP];
+        }
         const double a = (1.0 - qd) / (1.0 + qd) * (f - fi + (f + fi - 2.0 * feq * omega) / (1.0 - omega)) * c1o2;
         double v;
         if (kind == ORC_BC_NOSLIP_INTERPOLATED) {
@@ -599,7 +608,7 @@ void interp_bc(int kind, const double* p, const float* q, int n, int i, double o
             }
             v = a + (qd * (f + fi) - 6.0 * w * vbc) / (1.0 + qd);
             // getInterpolatedDistributionForVeloWithPressureBC (KernelUtilities.h:207-213)
-            if (kind == ORC_BC_VELOCITY_PRESSURE_INTERPOLATED) v = v - w * drho;
+            if (kind == ORC_BC_VELOCITY_PRESSURE_INTERPOLATED || kind == ORC_BC_PRECURSOR_VELOCITY) v = v - w * drho;
         }
         out[T.inv[d]] = v;
         written[T.inv[d]] = true;
@@ -933,7 +942,65 @@ typedef struct {
     const double* wm_z0;
     double* wm_ustar;
     double* wm_force[3];
+    // precursor kinds: stencil 0PP/0PM/0MP/0MM per boundary node, plane arrays
+    // (components x n_pre), time ratio, velocity ratio
+    int32_t n_pre;
+    const int64_t* pn[4];
+    const double* pw[4];
+    const double* plast;
+    const double* pcur;
+    double tratio;
+    double vratio;
 } orc_bc;
+
+static bool stress_kind(int kind)
+{
+    return kind >= ORC_BC_STRESS_INTERPOLATED && kind <= ORC_BC_STRESS_BOUNCE_BACK_PRESSURE;
+}
+
+// Plane value of component c at boundary node i from the plane array a
+// (components x n_pre): the inverse-distance weighted four precursor nodes,
+// or the 0PP node alone when its weight is >= 1e6
+// (PrecursorNonReflectiveCompressible.cu:82-110, PrecursorDistributions.cu:95-152).
+static double plane_value(int n_pre, const int64_t* const pn[4], const double* const pw[4], int i, const double* a,
+                          int c)
+{
+    const double* ac = a + (int64_t)c * n_pre;
+    const double w0 = pw[0][i];
+    if (!(w0 < 1e6)) return ac[pn[0][i]];
+    const double w1 = pw[1][i], w2 = pw[2][i], w3 = pw[3][i];
+    const double inv = 1.0 / (w0 + w1 + w2 + w3);
+    return (ac[pn[0][i]] * w0 + ac[pn[1][i]] * w1 + ac[pn[2][i]] * w2 + ac[pn[3][i]] * w3) * inv;
+}
+
+// Blended plane values out[c*n + i] = last*(1 - tr) + current*tr for every
+// boundary node and component (PrecursorDistributions.cu:163-171).
+void orc_precursor_blend(int n, int n_pre, int comps, const int64_t* pn0, const int64_t* pn1, const int64_t* pn2,
+                         const int64_t* pn3, const double* pw0, const double* pw1, const double* pw2,
+                         const double* pw3, const double* last, const double* cur, double tr, double* out)
+{
+    const int64_t* pn[4] = {pn0, pn1, pn2, pn3};
+    const double* pw[4] = {pw0, pw1, pw2, pw3};
+    for (int c = 0; c < comps; ++c)
+        for (int i = 0; i < n; ++i)
+            out[(int64_t)c * n + i] =
+                plane_value(n_pre, pn, pw, i, last, c) * (1.0 - tr) + plane_value(n_pre, pn, pw, i, cur, c) * tr;
+}
+
+// The inlet velocity of the precursor velocity kind at boundary node i:
+// (velocityX + (1 - tr) last + tr current) / velocityRatio
+// (PrecursorNonReflectiveCompressible.cu:112-114).
+static void precursor_velocity(const orc_bc& bc, int i, double* v3)
+{
+    const double u[3] = {bc.ux ? bc.ux[i] : 0.0, bc.uy ? bc.uy[i] : 0.0, bc.uz ? bc.uz[i] : 0.0};
+    for (int c = 0; c < 3; ++c)
+        v3[c] = (u[c] + (1.0 - bc.tratio) * plane_value(bc.n_pre, bc.pn, bc.pw, i, bc.plast, c) +
+                 bc.tratio * plane_value(bc.n_pre, bc.pn, bc.pw, i, bc.pcur, c)) /
+                bc.vratio;
+}
+
+// The nine populations the precursor distributions kind sets, in plane order
+static const int PRECURSOR_DIRS[9] = {dP00, dPP0, dPM0, dP0P, dP0M, dPPP, dPMP, dPPM, dPMM};
 
 // Wall-model boundaries (StressCompressible.cu, StressBounceBackCompressible.cu,
 // StressBounceBackPressureCompressible.cu with iMEM.cuh:43-141), restated:
@@ -950,7 +1017,7 @@ static const int STRESS_ORDER[26] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 
 void stress_bc(const orc_bc& bc, int i, const double* p, double omega, const double* vel, int64_t N, double* out,
                bool* written)
 {
-    const int mode = bc.kind - ORC_BC_STRESS_INTERPOLATED;
+    const int mode = bc.kind - ORC_BC_STRESS_INTERPOLATED;  // 0, 1, 2
     const double drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] +
                         p[d0MP] + p[d0PP] + p[d0MM] + p[dP0M] + p[dM0P] + p[dP0P] + p[dM0M] + p[dPM0] + p[dMP0] +
                         p[dPP0] + p[dMM0] + p[d00P] + p[d00M] + p[d0P0] + p[d0M0] + p[dP00] + p[dM00] + p[d000];
@@ -1252,7 +1319,7 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
 {
     const int64_t N = (int64_t)nx * ny * nz;
     bool stress = false;
-    for (int b = 0; b < n_bc; ++b) stress = stress || bcs[b].kind >= ORC_BC_STRESS_INTERPOLATED;
+    for (int b = 0; b < n_bc; ++b) stress = stress || stress_kind(bcs[b].kind);
     std::vector<double> vel(turb == TURB_AMD || stress ? (size_t)N * 3 : 0);
     std::vector<double> post((size_t)N * 27), G((size_t)N * 27);
     static const double zero[3] = {0, 0, 0};
@@ -1272,6 +1339,11 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
                             covered[(size_t)bcs[b].node[i] * 27 + d] = 1;
                         continue;
                     }
+                    if (bcs[b].kind == ORC_BC_PRECURSOR_DISTRIBUTIONS) {
+                        // the populations entering with a positive x component are set
+                        if (CX[d] < 0) covered[(size_t)bcs[b].node[i] * 27 + d] = 1;
+                        continue;
+                    }
                     const float q = bcs[b].q[(int64_t)d * bcs[b].n + i];
                     if (q >= 0.0f && q <= 1.0f) covered[(size_t)bcs[b].node[i] * 27 + d] = 1;
                 }
@@ -1284,6 +1356,13 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
         return (int64_t)x + (int64_t)nx * ((int64_t)y + (int64_t)ny * z);
     };
 
+    // precursor velocity kind: the value computed at step s is written over
+    // the outgoing population it read (the parity's own slot) and enters the
+    // node two steps later; pv[b][parity] holds it in between.  The step after
+    // the first reads the element as it was initially, the population of the
+    // node across the link (kept, since that node does not collide).
+    std::vector<std::array<std::vector<double>, 2>> pv(n_bc);
+    std::vector<std::array<std::vector<uint8_t>, 2>> pv_wr(n_bc);
     for (int s = 0; s < nsteps; ++s) {
         // non-reflecting outflow: the populations entering the boundary nodes
         // in the next state come from this pre-collision state (the kernel
@@ -1364,9 +1443,41 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
                     }
                     continue;
                 }
+                if (bc.kind == ORC_BC_PRECURSOR_DISTRIBUTIONS) {
+                    for (int c = 0; c < 9; ++c)
+                        G[(size_t)k * 27 + PRECURSOR_DIRS[c]] =
+                            plane_value(bc.n_pre, bc.pn, bc.pw, i, bc.plast, c) * (1.0 - bc.tratio) +
+                            plane_value(bc.n_pre, bc.pn, bc.pw, i, bc.pcur, c) * bc.tratio;
+                    continue;
+                }
                 double out[27];
                 bool wr[27] = {false};
-                if (bc.kind >= ORC_BC_STRESS_INTERPOLATED) {
+                if (bc.kind == ORC_BC_PRECURSOR_VELOCITY) {
+                    auto& now = pv[b][s & 1];
+                    auto& now_wr = pv_wr[b][s & 1];
+                    const auto& prev = pv[b][(s + 1) & 1];
+                    const auto& prev_wr = pv_wr[b][(s + 1) & 1];
+                    if (now.empty()) {
+                        now.assign((size_t)bc.n * 27, 0.0);
+                        now_wr.assign((size_t)bc.n * 27, 0);
+                    }
+                    for (int d = 1; d < 27; ++d) {
+                        const float q = bc.q[(int64_t)d * bc.n + i];
+                        if (!(q >= 0.0f && q <= 1.0f)) continue;
+                        const int e = T.inv[d];
+                        if (s == 0) G[(size_t)k * 27 + e] = F[27 * neighbour(k, d) + d];
+                        else if (prev_wr[(size_t)i * 27 + e]) G[(size_t)k * 27 + e] = prev[(size_t)i * 27 + e];
+                    }
+                    double v3[3];
+                    precursor_velocity(bc, i, v3);
+                    interp_bc(bc.kind, &post[(size_t)k * 27], bc.q, bc.n, i, omega, v3[0], v3[1], v3[2], out, wr);
+                    for (int d = 0; d < 27; ++d) {
+                        now[(size_t)i * 27 + d] = out[d];
+                        now_wr[(size_t)i * 27 + d] = wr[d];
+                    }
+                    continue;
+                }
+                if (stress_kind(bc.kind)) {
                     const double om = nut ? omega / (1.0 + 3.0 * omega * std::max(0.0, nut[k])) : omega;
                     stress_bc(bc, i, &post[(size_t)k * 27], om, vel.data(), N, out, wr);
                     for (int d = 0; d < 27; ++d)

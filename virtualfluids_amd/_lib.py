@@ -22,6 +22,7 @@ VFX_BC_VELOCITY_PRESSURE_INTERPOLATED, VFX_BC_OUTFLOW_NON_REFLECTING = 5, 6
 VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION = 7
 VFX_BC_SLIP_TURBULENT_VISCOSITY = 8
 VFX_BC_STRESS_INTERPOLATED, VFX_BC_STRESS_BOUNCE_BACK, VFX_BC_STRESS_BOUNCE_BACK_PRESSURE = 9, 10, 11
+VFX_BC_PRECURSOR_VELOCITY, VFX_BC_PRECURSOR_DISTRIBUTIONS = 12, 13
 
 _u32p = C.POINTER(C.c_uint32)
 _dp = C.POINTER(C.c_double)
@@ -52,6 +53,9 @@ class VfxBC(C.Structure):
         ("wm_normal", C.c_void_p * 3), ("wm_exchange_velocity", C.c_void_p * 3), ("wm_wall_velocity", C.c_void_p * 3),
         ("wm_sampling_offset", C.c_void_p), ("wm_z0", C.c_void_p), ("wm_u_star", C.c_void_p),
         ("wm_force", C.c_void_p * 3),
+        ("n_precursor_nodes", C.c_uint32), ("plane_neighbor", C.c_void_p * 4), ("plane_weight", C.c_void_p * 4),
+        ("precursor_last", C.c_void_p), ("precursor_current", C.c_void_p), ("time_ratio", C.c_double),
+        ("velocity_ratio", C.c_double),
     ]
 
 

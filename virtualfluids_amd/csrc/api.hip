@@ -242,7 +242,7 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
     vfx::Lat L;
     if (vfx_status s = make_lat(lat, L)) return s;
     if (!bc) return fail(VFX_EINVAL, "bc is NULL");
-    if (bc->kind < VFX_BC_VELOCITY_BOUNCE_BACK || bc->kind > VFX_BC_STRESS_BOUNCE_BACK_PRESSURE)
+    if (bc->kind < VFX_BC_VELOCITY_BOUNCE_BACK || bc->kind > VFX_BC_PRECURSOR_DISTRIBUTIONS)
         return fail(VFX_EUNSUPPORTED, "bc kind %d not supported", bc->kind);
     if (bc->kind == VFX_BC_SLIP_TURBULENT_VISCOSITY && !L.nut)
         return fail(VFX_EINVAL, "slip with turbulent viscosity needs a turbulence model and turbulent_viscosity");
@@ -251,6 +251,15 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
     if (bc->kind == VFX_BC_PRESSURE_NEQ) {
         if (bc->n && (!bc->k || !bc->k_neighbor || !bc->rho)) return fail(VFX_EINVAL, "pressure bc needs k, k_neighbor, rho");
         if (bc->normal < 1 || bc->normal > 26) return fail(VFX_EINVAL, "pressure bc normal %d invalid", bc->normal);
+    } else if (bc->kind == VFX_BC_PRECURSOR_VELOCITY || bc->kind == VFX_BC_PRECURSOR_DISTRIBUTIONS) {
+        if (bc->n && (!bc->k || (bc->kind == VFX_BC_PRECURSOR_VELOCITY && !bc->q27)))
+            return fail(VFX_EINVAL, "precursor bc needs k%s", bc->kind == VFX_BC_PRECURSOR_VELOCITY ? " and q27" : "");
+        if (bc->n && (!bc->plane_neighbor[0] || !bc->plane_neighbor[1] || !bc->plane_neighbor[2] ||
+                      !bc->plane_neighbor[3] || !bc->plane_weight[0] || !bc->plane_weight[1] || !bc->plane_weight[2] ||
+                      !bc->plane_weight[3] || !bc->precursor_last || !bc->precursor_current || !bc->n_precursor_nodes))
+            return fail(VFX_EINVAL, "precursor bc needs plane neighbours, weights, last and current planes");
+        if (bc->kind == VFX_BC_PRECURSOR_VELOCITY && !(bc->velocity_ratio != 0.0))
+            return fail(VFX_EINVAL, "precursor velocity bc needs a non-zero velocity_ratio");
     } else if (bc->kind >= VFX_BC_STRESS_INTERPOLATED) {
         if (bc->n && (!bc->k || !bc->q27 || !bc->k_neighbor)) return fail(VFX_EINVAL, "stress bc needs k, q27, k_neighbor");
         if (bc->n && (!bc->wm_normal[0] || !bc->wm_normal[1] || !bc->wm_normal[2] || !bc->wm_exchange_velocity[0] ||
@@ -272,7 +281,11 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
                  {bc->wm_exchange_velocity[0], bc->wm_exchange_velocity[1], bc->wm_exchange_velocity[2]},
                  {bc->wm_wall_velocity[0], bc->wm_wall_velocity[1], bc->wm_wall_velocity[2]},
                  bc->wm_sampling_offset, bc->wm_z0, bc->wm_u_star,
-                 {bc->wm_force[0], bc->wm_force[1], bc->wm_force[2]}};
+                 {bc->wm_force[0], bc->wm_force[1], bc->wm_force[2]},
+                 bc->n_precursor_nodes,
+                 {bc->plane_neighbor[0], bc->plane_neighbor[1], bc->plane_neighbor[2], bc->plane_neighbor[3]},
+                 {bc->plane_weight[0], bc->plane_weight[1], bc->plane_weight[2], bc->plane_weight[3]},
+                 bc->precursor_last, bc->precursor_current, bc->time_ratio, bc->velocity_ratio};
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     switch (lat->storage) {

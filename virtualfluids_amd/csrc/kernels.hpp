@@ -790,7 +790,59 @@ struct BCDev {
     const void* wm_z0;            // roughness length
     void* wm_ustar;               // monitors (NULL: none)
     void* wm_force[3];
+    // precursor kinds: plane stencil per boundary node and the two plane
+    // arrays (lattice's arithmetic type), blended last*(1 - tr) + current*tr
+    uint32_t n_pre;
+    const uint32_t* pn[4];        // precursor nodes 0PP, 0PM, 0MP, 0MM
+    const void* pw[4];            // their inverse-distance weights
+    const void* plast;
+    const void* pcur;
+    double tratio;
+    double vratio;                // velocity ratio (velocity kind)
 };
+
+// The plane value at boundary node i: the four precursor nodes around it
+// weighted by inverse distance, or the 0PP node alone when its weight is
+// >= 1e6 (PrecursorNonReflectiveCompressible.cu:82-110,
+// PrecursorDistributions.cu:95-152).  The weighted sum runs in the
+// reference's order and is scaled by 1 / (sum of weights).
+template <typename T> struct PlaneStencil {
+    uint32_t n[4];
+    T w[4];
+    T inv;
+    bool one;
+};
+
+template <typename T> __device__ __forceinline__ PlaneStencil<T> plane_stencil(const BCDev& B, uint32_t i)
+{
+    PlaneStencil<T> s;
+    s.n[0] = B.pn[0][i];
+    s.w[0] = ((const T*)B.pw[0])[i];
+    s.one = !(s.w[0] < T(1e6));
+    s.inv = T(0);
+    if (!s.one) {
+        for (int c = 1; c < 4; ++c) {
+            s.n[c] = B.pn[c][i];
+            s.w[c] = ((const T*)B.pw[c])[i];
+        }
+        s.inv = T(1) / (s.w[0] + s.w[1] + s.w[2] + s.w[3]);
+    }
+    return s;
+}
+
+template <typename T> __device__ __forceinline__ T plane_value(const PlaneStencil<T>& s, const T* a)
+{
+    if (s.one) return a[s.n[0]];
+    return (a[s.n[0]] * s.w[0] + a[s.n[1]] * s.w[1] + a[s.n[2]] * s.w[2] + a[s.n[3]] * s.w[3]) * s.inv;
+}
+
+// last*(1 - tr) + current*tr of plane component c (PrecursorDistributions.cu:163-171)
+template <typename T> __device__ __forceinline__ T plane_blend(const BCDev& B, const PlaneStencil<T>& s, int c)
+{
+    const T tr = (T)B.tratio;
+    const size_t off = (size_t)c * B.n_pre;
+    return plane_value(s, (const T*)B.plast + off) * (T(1) - tr) + plane_value(s, (const T*)B.pcur + off) * tr;
+}
 
 // c_d . (ux, uy, uz) spelled as VirtualFluids writes it: first non-zero
 // component with its sign, then the others added or subtracted in x, y, z order
@@ -926,10 +978,22 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
 #pragma unroll
     for (int d = 1; d < 27; ++d) qs[d] = (T)B.q[(uint64_t)d * B.n + i];
     T VX = 0, VY = 0, VZ = 0;
-    if (KIND == VFX_BC_VELOCITY_INTERPOLATED || KIND == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED) {
+    if (KIND == VFX_BC_VELOCITY_INTERPOLATED || KIND == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED ||
+        KIND == VFX_BC_PRECURSOR_VELOCITY) {
         VX = B.vx ? (T)B.vx[i] : T(0);
         VY = B.vy ? (T)B.vy[i] : T(0);
         VZ = B.vz ? (T)B.vz[i] : T(0);
+    }
+    if constexpr (KIND == VFX_BC_PRECURSOR_VELOCITY) {
+        // PrecursorNonReflectiveCompressible.cu:112-114: (velocityX + (1 - tr) last + tr current) / velocityRatio
+        const PlaneStencil<T> s = plane_stencil<T>(B, i);
+        const T tr = (T)B.tratio, vr = (T)B.vratio;
+        const T* last = (const T*)B.plast;
+        const T* cur = (const T*)B.pcur;
+        const size_t np = B.n_pre;
+        VX = (VX + (T(1) - tr) * plane_value(s, last) + tr * plane_value(s, cur)) / vr;
+        VY = (VY + (T(1) - tr) * plane_value(s, last + np) + tr * plane_value(s, cur + np)) / vr;
+        VZ = (VZ + (T(1) - tr) * plane_value(s, last + 2 * np) + tr * plane_value(s, cur + 2 * np)) / vr;
     }
     bool fx = false, fy = false, fz = false; // slip: axes normal to a face link
 #pragma unroll
@@ -939,7 +1003,13 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
         const T w = weight<T>(d);
         const T vlb = proj<T>(d, vx1, vx2, vx3);
         const T feq = w * (drho + T(4.5) * vlb * vlb * (T(1) + drho) - cu_sq);
-        const T f = p[d], fi = p[inv(d)];
+        T f = p[d], fi = p[inv(d)];
+        if constexpr (KIND == VFX_BC_PRECURSOR_VELOCITY) {
+            // kept from PrecursorNonReflectiveCompressible.cu:383 and :392: the
+            // d0PM link takes f_0PP for its opposite, the d0MP link f_0PP for its own
+            if (d == d0PM) fi = p[d0PP];
+            if (d == d0MP) f = p[d0PP];
+        }
         // FP64 keeps the reference's divisions (bit parity with the CPU
         // restatement); the FP32 arithmetic of the reduced forms uses one
         // hardware reciprocal per link and 1/(1 - omega) once per node
@@ -972,10 +1042,39 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
             else v = a + (q * (f + fi) - T(6) * w * vbc) / (T(1) + q);
             // getInterpolatedDistributionForVeloWithPressureBC (KernelUtilities.h:207-213):
             // the entering population carries the bulk pressure
-            if (KIND == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED) v = v - w * drho;
+            if (KIND == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED || KIND == VFX_BC_PRECURSOR_VELOCITY) v = v - w * drho;
         }
         const int e = inv(d);
-        Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)], L.ss[e]);
+        // the precursor inlet writes with the parity's own pointers
+        // (PrecursorNonReflectiveCompressible.cu:142): over the outgoing
+        // population it read, which the node reads back as e two steps later
+        constexpr bool WRITE_EVEN = KIND == VFX_BC_PRECURSOR_VELOCITY ? EVEN : !EVEN;
+        Storage<S, T>::st(plane<S>(L, slot(e, WRITE_EVEN)), kk[nbr(e)], v, L.st[weight_class(e)], L.ss[e]);
+    }
+}
+
+// Precursor distributions (PrecursorDistributions.cu:44-185): post-collision,
+// parity EVEN.  The nine populations with a positive x component entering
+// boundary node k are the blended plane values f0..f8, written for the next
+// step like the other post-collision boundary conditions.
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) k_bc_precursor_dist(Lat L, BCDev B)
+{
+    using T = typename Compute<S>::T;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    const uint32_t k = B.k[i];
+    if (k >= L.N) return;
+    uint32_t kk[8];
+    if (B.direct) nbrs_direct(L, k, kk);
+    else nbrs_sparse(L, k, kk);
+    const PlaneStencil<T> s = plane_stencil<T>(B, i);
+    constexpr int dirs[9] = {dP00, dPP0, dPM0, dP0P, dP0M, dPPP, dPMP, dPPM, dPMM};
+#pragma unroll
+    for (int c = 0; c < 9; ++c) {
+        const int x = dirs[c];
+        const T v = plane_blend(B, s, c);
+        Storage<S, T>::st(plane<S>(L, slot(x, !EVEN)), kk[nbr(x)], v, L.st[weight_class(x)], L.ss[x]);
     }
 }
 
@@ -1799,6 +1898,14 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     case VFX_BC_STRESS_BOUNCE_BACK_PRESSURE:
         if (even) hipLaunchKernelGGL((k_bc_stress<S, true, 2>), g, b, 0, st, L, B);
         else hipLaunchKernelGGL((k_bc_stress<S, false, 2>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_PRECURSOR_VELOCITY:
+        if (even) hipLaunchKernelGGL((k_bc_interp<S, true, VFX_BC_PRECURSOR_VELOCITY>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_interp<S, false, VFX_BC_PRECURSOR_VELOCITY>), g, b, 0, st, L, B);
+        break;
+    case VFX_BC_PRECURSOR_DISTRIBUTIONS:
+        if (even) hipLaunchKernelGGL((k_bc_precursor_dist<S, true>), g, b, 0, st, L, B);
+        else hipLaunchKernelGGL((k_bc_precursor_dist<S, false>), g, b, 0, st, L, B);
         break;
     case VFX_BC_OUTFLOW_NON_REFLECTING:
         if (even) hipLaunchKernelGGL((k_bc_outflow<S, true, false>), g, b, 0, st, L, B);

@@ -251,6 +251,8 @@ class Lattice:
             if bc.real != self._macro_dtype():
                 raise ValueError("wall-model arrays must have the lattice's arithmetic type")
             self._alloc_macro()
+        if bc.precursor is not None and bc.precursor.real != self._macro_dtype():
+            raise ValueError("precursor weights and planes must have the lattice's arithmetic type")
         s = self.struct()
         b = bc.struct()
         V.check(V.lib().vfx_bc_apply(C.byref(s), C.byref(b), int(bool(even)), _stream(stream)), "vfx_bc_apply")
@@ -398,13 +400,24 @@ class BoundaryCondition:
                      V.VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION}
     OUTFLOW = {V.VFX_BC_OUTFLOW_NON_REFLECTING, V.VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION}
     STRESS = {V.VFX_BC_STRESS_INTERPOLATED, V.VFX_BC_STRESS_BOUNCE_BACK, V.VFX_BC_STRESS_BOUNCE_BACK_PRESSURE}
+    PRECURSOR = {V.VFX_BC_PRECURSOR_VELOCITY, V.VFX_BC_PRECURSOR_DISTRIBUTIONS}
 
     def __init__(self, kind, nodes, q27=None, vx=None, vy=None, vz=None, cpu_float_velocity=False, rho=None,
                  neighbors=None, normal=0, device="cuda", pressure_correction_factor=0.0, wall_normal=None,
-                 sampling_offset=None, z0=None, exchange_velocity=None, wall_velocity=None, real=torch.float64):
+                 sampling_offset=None, z0=None, exchange_velocity=None, wall_velocity=None, real=torch.float64,
+                 precursor=None):
         nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
         n = nodes.size
         self.kind = int(kind)
+        self.precursor = precursor
+        if self.kind in self.PRECURSOR:
+            if precursor is None or precursor.n != n:
+                raise ValueError("a precursor boundary needs PrecursorPlanes with one stencil per boundary node")
+            want = 3 if self.kind == V.VFX_BC_PRECURSOR_VELOCITY else 9
+            if precursor.components != want:
+                raise ValueError(f"this precursor kind reads {want} plane components")
+            if self.kind == V.VFX_BC_PRECURSOR_DISTRIBUTIONS and q27 is None:
+                q27 = np.full((27, n), -1.0, np.float32)
         self.n = n
         dev = torch.device(device)
         self.max_node = int(nodes.max()) if n else -1
@@ -484,7 +497,80 @@ class BoundaryCondition:
             b.wm_z0 = w["z0"].data_ptr()
             b.wm_u_star = w["u_star"].data_ptr()
             b.wm_force[:] = [w["force"][j].data_ptr() for j in range(3)]
+        if self.precursor is not None and self.n:
+            pc = self.precursor
+            b.n_precursor_nodes = pc.n_precursor_nodes
+            b.plane_neighbor[:] = [pc.neighbors[j].data_ptr() for j in range(4)]
+            b.plane_weight[:] = [pc.weights[j].data_ptr() for j in range(4)]
+            b.precursor_last = pc.last.data_ptr()
+            b.precursor_current = pc.current.data_ptr()
+            b.time_ratio = pc.time_ratio
+            b.velocity_ratio = pc.velocity_ratio
         return b
+
+
+class PrecursorPlanes:
+    """Plane data of a precursor inflow (QforPrecursorBoundaryConditions,
+    src/gpu/core/Calculation/Calculation.h:120-150) and its time cycling.
+
+    neighbors / weights: (4, n) precursor node and inverse-distance weight of
+    the 0PP, 0PM, 0MP, 0MM stencil of every boundary node; components: 3
+    (velocities, VFX_BC_PRECURSOR_VELOCITY) or 9 (the positive-x populations,
+    VFX_BC_PRECURSOR_DISTRIBUTIONS).  Planes are (components, n_precursor_nodes)
+    arrays of the lattice's real type.
+
+    With a reader -- reader(load_time) returning the plane array at a physical
+    time -- the cycle follows VirtualFluids: GridGenerator loads t = 0 into
+    last, time_steps_between_reads into current and twice that into next
+    (GridGenerator.cpp:436-478, nPrecursorReads = 2); runPrecursorBCKernelPost
+    (BoundaryConditionKernelManager.cpp:346-382) rotates last <- current <-
+    next and reads the following plane once the level's time step reaches
+    current's, and passes time_ratio = (t - lastTime) / time_steps_between_reads.
+    Without a reader, set last, current and time_ratio directly."""
+
+    def __init__(self, neighbors, weights, n_precursor_nodes, components, reader=None, time_steps_between_reads=1,
+                 velocity_ratio=1.0, level=0, physical_time_ratio=1.0, real=torch.float64, device="cuda"):
+        dev = torch.device(device)
+        nb = np.ascontiguousarray(neighbors, dtype=np.uint32).reshape(4, -1)
+        if nb.size and int(nb.max()) >= int(n_precursor_nodes):
+            raise ValueError("plane neighbour beyond the precursor nodes")
+        self.n = nb.shape[1]
+        self.n_precursor_nodes = int(n_precursor_nodes)
+        self.components = int(components)
+        self.neighbors = torch.from_numpy(nb.view(np.int32)).to(dev)
+        self.weights = torch.from_numpy(np.asarray(weights, np.float64).reshape(4, -1).copy()).to(dev, real)
+        self.real, self.device = real, dev
+        self.velocity_ratio = float(velocity_ratio)
+        self.time_ratio = 0.0
+        self.reader = reader
+        self.steps_between_reads = int(time_steps_between_reads)
+        self.level = int(level)
+        self.physical_time_ratio = float(physical_time_ratio)
+        zeros = np.zeros((self.components, self.n_precursor_nodes))
+        self.last, self.current, self.next = (self.plane(zeros) for _ in range(3))
+        self.n_reads = 0
+        if reader is not None:
+            dt = self.steps_between_reads * 2.0 ** -self.level * self.physical_time_ratio
+            self.last, self.current, self.next = self.plane(reader(0.0)), self.plane(reader(dt)), self.plane(
+                reader(2 * dt))
+            self.n_reads = 2
+
+    def plane(self, a):
+        a = np.asarray(a, np.float64).reshape(self.components, self.n_precursor_nodes)
+        return torch.from_numpy(a.copy()).to(self.device, self.real).contiguous()
+
+    def advance(self, t_level):
+        """The cycling of runPrecursorBCKernelPost at level time step t_level."""
+        T = self.steps_between_reads
+        last_time, current_time = (self.n_reads - 2) * T, (self.n_reads - 1) * T
+        next_time = self.n_reads * T
+        if t_level >= current_time:
+            last_time, current_time, next_time = current_time, next_time, next_time + T
+            self.last, self.current, self.next = self.current, self.next, self.last
+            self.next = self.plane(self.reader(next_time * 2.0 ** -self.level * self.physical_time_ratio))
+            self.n_reads += 1
+        self.time_ratio = (t_level - last_time) / T
+        return self.time_ratio
 
 
 @dataclass
@@ -500,9 +586,12 @@ class Stepper:
     bcs: list = field(default_factory=list)
     even: bool = True
     apply_body_force: bool = False  # CollisionTemplate ApplyBodyForce (Lattice.set_body_force)
+    t: int = 1  # time step of the next step; precursor planes with a reader cycle on it
 
     def step(self, n=1, stream=None):
-        post = [b for b in self.bcs if not b.pre_collision]
+        # precursor inflows run last among the post-collision conditions (UpdateGrid27.cpp:252)
+        post = [b for b in self.bcs if not b.pre_collision and b.kind not in BoundaryCondition.PRECURSOR] + \
+               [b for b in self.bcs if b.kind in BoundaryCondition.PRECURSOR]
         pre = [b for b in self.bcs if b.pre_collision]
         # wall models sample the velocities the collision writes
         write_macro = any(getattr(b, "wm", None) is not None for b in self.bcs)
@@ -510,7 +599,10 @@ class Stepper:
             self.lattice.collide_stream(self.collision, self.even, write_macro=write_macro, stream=stream,
                                         apply_body_force=self.apply_body_force)
             for bc in post:
+                if bc.precursor is not None and bc.precursor.reader is not None:
+                    bc.precursor.advance(self.t)
                 self.lattice.apply_bc(bc, self.even, stream=stream)
+            self.t += 1
             self.even = not self.even
             if self.collision == "k17" and getattr(self.lattice, "turbulence", "none") == "amd":
                 self.lattice.turbulent_viscosity_amd(stream=stream)
