@@ -150,6 +150,37 @@ static boundaryCondition make_bc(int32_t kind)
     };
 }
 
+// Precursor/Precursor.cu:44-117 through vfx_bc_apply kinds 12 / 13
+precursorBoundaryCondition BoundaryConditionFactory::getPrecursorBoundaryConditionPost() const
+{
+    int32_t kind;
+    switch (precursor) {
+    case PrecursorBC::PrecursorNonReflectiveCompressible: kind = VFX_BC_PRECURSOR_VELOCITY; break;
+    case PrecursorBC::PrecursorDistributions: kind = VFX_BC_PRECURSOR_DISTRIBUTIONS; break;
+    default: return nullptr;
+    }
+    return [kind](LBMSimulationParameter* p, QforPrecursorBoundaryConditions* q, double tRatio, double velocityRatio) {
+        vfx_bc bc{};
+        bc.kind = kind;
+        bc.n = q->numberOfBCnodes;
+        bc.k = q->k;
+        bc.q27 = q->q27;
+        bc.vx = q->velocityX;
+        bc.vy = q->velocityY;
+        bc.vz = q->velocityZ;
+        bc.n_precursor_nodes = q->numberOfPrecursorNodes;
+        for (int c = 0; c < 4; ++c) {
+            bc.plane_neighbor[c] = q->planeNeighbor[c];
+            bc.plane_weight[c] = q->weights[c];
+        }
+        bc.precursor_last = q->last;
+        bc.precursor_current = q->current;
+        bc.time_ratio = tRatio;
+        bc.velocity_ratio = velocityRatio;
+        check(vfx_bc_apply(&p->lattice, &bc, p->isEvenTimestep ? 1 : 0, p->stream), "precursor boundary condition");
+    };
+}
+
 boundaryCondition BoundaryConditionFactory::getVelocityBoundaryConditionPost() const
 {
     switch (velocity) {
@@ -243,7 +274,8 @@ UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared
                            std::shared_ptr<TurbulenceModelFactory> tm)
     : tmFactory(std::move(tm)), para(std::move(p)), kernels(std::move(k)), velocityBC(f.getVelocityBoundaryConditionPost()),
       noSlipBC(f.getNoSlipBoundaryConditionPost()), slipBC(f.getSlipBoundaryConditionPost()),
-      pressureBC(f.getPressureBoundaryConditionPre()), stressBC(f.getStressBoundaryConditionPost())
+      pressureBC(f.getPressureBoundaryConditionPre()), stressBC(f.getStressBoundaryConditionPost()),
+      precursorBC(f.getPrecursorBoundaryConditionPost())
 {
     if ((int)kernels.size() != para->getMaxLevel() + 1) throw std::runtime_error("one kernel per level expected");
     // GridScalingKernelManager.cpp:45-60
@@ -265,16 +297,47 @@ UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared
         require(slipBC, d->slipBC, "slip");
         require(pressureBC, d->pressureBC, "pressure");
         require(stressBC, d->stressBC, "stress");
+        if (d->precursorBC.numberOfBCnodes && !precursorBC)
+            throw std::runtime_error("precursor boundary nodes exist but no boundary condition was selected");
+        if (d->precursorBC.numberOfBCnodes && l != 0)
+            throw std::runtime_error("precursor boundary conditions are supported on level 0");
     }
 }
 
-void UpdateGrid27::postCollisionBC(int level)
+// BoundaryConditionKernelManager.cpp:346-382 on level 0 (t_level = t): once
+// the step reaches the time of `current`, rotate last <- current <- next, read
+// the following plane into next, and pass (t - lastTime) / timeStepsBetweenReads
+void UpdateGrid27::runPrecursorBCKernelPost(int level, unsigned int t)
+{
+    LBMSimulationParameter* p = para->getParD(level);
+    QforPrecursorBoundaryConditions& q = p->precursorBC;
+    if (q.numberOfBCnodes == 0) return;
+    const uint32_t T = q.timeStepsBetweenReads;
+    uint32_t lastTime = (q.nPrecursorReads - 2) * T;
+    const uint32_t currentTime = (q.nPrecursorReads - 1) * T;
+    uint32_t nextTime = q.nPrecursorReads * T;
+    if (t >= currentTime) {
+        lastTime = currentTime;
+        nextTime += T;
+        void* tmp = q.last;
+        q.last = q.current;
+        q.current = q.next;
+        q.next = tmp;
+        if (q.getNextData) q.getNextData(q.next, q.numberOfPrecursorNodes, nextTime * std::exp2(-level) * para->getTimeRatio());
+        q.nPrecursorReads++;
+    }
+    const double tRatio = double(t - lastTime) / T;
+    precursorBC(p, &q, tRatio, para->getVelocityRatio());
+}
+
+void UpdateGrid27::postCollisionBC(int level, unsigned int timestep)
 {
     LBMSimulationParameter* p = para->getParD(level);
     if (p->velocityBC.numberOfBCnodes) velocityBC(p, &p->velocityBC);
     if (p->noSlipBC.numberOfBCnodes) noSlipBC(p, &p->noSlipBC);
     if (p->slipBC.numberOfBCnodes) slipBC(p, &p->slipBC);
     if (p->stressBC.numberOfBCnodes) stressBC(p, &p->stressBC); // UpdateGrid27.cpp:241-243
+    runPrecursorBCKernelPost(level, timestep);                     // :252
 }
 
 void UpdateGrid27::preCollisionBC(int level)
@@ -314,7 +377,7 @@ void UpdateGrid27::updateGrid(int level, unsigned int timestep)
     } else {
         kernels[level]->run();
     }
-    postCollisionBC(level);
+    postCollisionBC(level, timestep);
     p->isEvenTimestep = !p->isEvenTimestep; // UpdateGrid27.cpp:264-268
     if (tmFactory) tmFactory->runTurbulenceModelKernel(level); // :81
     preCollisionBC(level);

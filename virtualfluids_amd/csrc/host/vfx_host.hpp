@@ -57,6 +57,29 @@ struct QforBoundaryConditions {
     void* wallForce[3] = {nullptr, nullptr, nullptr};
 };
 
+// Calculation.h:120-150: a precursor inflow's stencil into the precursor
+// plane, the planes (arrays of the lattice's real type: 3 velocities or 9
+// populations per precursor node) and the state of the read cycle.
+// getNextData stands for TransientBCInputFileReader::getNextData: it fills the
+// device array `next` with the plane at the physical time loadTime.
+struct QforPrecursorBoundaryConditions {
+    uint32_t* k = nullptr;
+    float* q27 = nullptr;                 // PrecursorNonReflectiveCompressible only
+    double* velocityX = nullptr;          // added to the plane velocity (NULL = 0)
+    double* velocityY = nullptr;
+    double* velocityZ = nullptr;
+    uint32_t numberOfBCnodes = 0;
+    uint32_t numberOfPrecursorNodes = 0;
+    uint32_t* planeNeighbor[4] = {nullptr, nullptr, nullptr, nullptr}; // 0PP, 0PM, 0MP, 0MM
+    void* weights[4] = {nullptr, nullptr, nullptr, nullptr};
+    void* last = nullptr;
+    void* current = nullptr;
+    void* next = nullptr;
+    uint32_t timeStepsBetweenReads = 1;
+    uint32_t nPrecursorReads = 2;         // GridGenerator.cpp:477: last, current and next loaded
+    std::function<void(void* next, uint32_t numberOfPrecursorNodes, double loadTime)> getNextData;
+};
+
 // Calculation.h:72-89: interface cells between a level and the next finer one
 struct ICells {
     uint32_t* fineCellIndices = nullptr;
@@ -88,6 +111,7 @@ struct LBMSimulationParameter {
     std::map<CollisionTemplate, std::pair<uint32_t*, uint32_t>> taggedFluidNodeIndices;
     std::vector<CollisionTemplate> allocatedBulkFluidNodeTags{CollisionTemplate::Default};
     QforBoundaryConditions velocityBC, noSlipBC, slipBC, pressureBC, stressBC;
+    QforPrecursorBoundaryConditions precursorBC;
     double outflowPressureCorrectionFactor = 0.0; // Parameter.h:158
     HostGrid host;                                // for the output (FileWriter)
     vfx_stream stream = nullptr;
@@ -117,6 +141,11 @@ public:
         for (auto& l : levels) l.outflowPressureCorrectionFactor = f;
     }
     bool getUseTurbulentViscosity() const { return useTurbulentViscosity; }
+    // Parameter::getVelocityRatio / getTimeRatio (physical per lattice units)
+    void setVelocityRatio(double v) { velocityRatio = v; }
+    double getVelocityRatio() const { return velocityRatio; }
+    void setTimeRatio(double t) { timeRatio = t; }
+    double getTimeRatio() const { return timeRatio; }
 
 private:
     std::vector<LBMSimulationParameter> levels;
@@ -125,6 +154,8 @@ private:
     TurbulenceModel turbulenceModel = TurbulenceModel::None;
     double sgsConstant = 0.0;
     bool useTurbulentViscosity = false;
+    double velocityRatio = 1.0;
+    double timeRatio = 1.0;
 };
 
 int32_t vfx_turbulence_model_of(TurbulenceModel m);
@@ -215,6 +246,9 @@ public:
 
 // BoundaryConditionFactory.h:50-53
 using boundaryCondition = std::function<void(LBMSimulationParameter*, QforBoundaryConditions*)>;
+// BoundaryConditionFactory.h:53
+using precursorBoundaryCondition =
+    std::function<void(LBMSimulationParameter*, QforPrecursorBoundaryConditions*, double timeRatio, double velocityRatio)>;
 
 class BoundaryConditionFactory {
 public:
@@ -245,6 +279,9 @@ public:
     void setSlipBoundaryCondition(SlipBC t) { slip = t; }
     void setPressureBoundaryCondition(PressureBC t) { pressure = t; }
     void setStressBoundaryCondition(StressBC t) { stress = t; }
+    enum class PrecursorBC { PrecursorNonReflectiveCompressible, PrecursorDistributions, NotSpecified };
+    void setPrecursorBoundaryCondition(PrecursorBC t) { precursor = t; }
+    precursorBoundaryCondition getPrecursorBoundaryConditionPost() const;
 
     boundaryCondition getVelocityBoundaryConditionPost() const;
     boundaryCondition getNoSlipBoundaryConditionPost() const;
@@ -258,6 +295,7 @@ private:
     SlipBC slip = SlipBC::NotSpecified;
     PressureBC pressure = PressureBC::NotSpecified;
     StressBC stress = StressBC::NotSpecified;
+    PrecursorBC precursor = PrecursorBC::NotSpecified;
 };
 
 // GridScalingFactory.h:46-71
@@ -292,12 +330,14 @@ public:
 
 private:
     std::shared_ptr<TurbulenceModelFactory> tmFactory;
-    void postCollisionBC(int level);
+    void postCollisionBC(int level, unsigned int timestep);
+    void runPrecursorBCKernelPost(int level, unsigned int timestep);
     void preCollisionBC(int level);
     void refinement(int level);
     std::shared_ptr<Parameter> para;
     std::vector<std::shared_ptr<Kernel>> kernels;
     boundaryCondition velocityBC, noSlipBC, slipBC, pressureBC, stressBC;
+    precursorBoundaryCondition precursorBC;
     gridScaling scalingFineToCoarse, scalingCoarseToFine;
 };
 
