@@ -318,6 +318,21 @@ uint64_t vfx_reduce_workspace_bytes(const vfx_lattice* lat);
 vfx_status vfx_reduce_mass_momentum(const vfx_lattice* lat, int32_t is_even_timestep, double* workspace, double* out4,
                                     vfx_stream stream);
 
+/* The same reduction without a workspace and without any allocation: the
+ * per-block partials live in a 2 MiB device array of the library (one per
+ * GPU, held until the process ends), so the call suits callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * No counterpart in the reference, which allocates through thrust.  At most
+ * vfx_reduce_noalloc_max_items() work items (colliding-node candidates: n_nodes
+ * for direct addressing or a type_of_grid_node selection, else n_fluid);
+ * larger lattices get VFX_EINVAL.  Calls that overlap on one GPU (two streams or
+ * two host threads) share the array and race.  Bit-identical to
+ * vfx_reduce_mass_momentum. */
+uint64_t vfx_reduce_noalloc_max_items(void);
+vfx_status vfx_reduce_mass_momentum_noalloc(const vfx_lattice* lat, int32_t is_even_timestep, double* out4,
+                                            vfx_stream stream);
+
+
 /* KineticEnergyAnalyzer::run (src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:66-154) and
  * EnstrophyAnalyzer::run (EnstrophyAnalyzer.cu:64-336) on macroscopic fields from
  * vfx_macroscopic: out2 = {sum, number of fluid nodes with a fluid 7-point stencil};

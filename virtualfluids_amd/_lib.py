@@ -92,6 +92,7 @@ _SIGNATURES = {
     "vfx_macroscopic": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                         C.c_void_p],
     "vfx_reduce_mass_momentum": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p],
+    "vfx_reduce_mass_momentum_noalloc": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
     "vfx_kinetic_energy": [C.POINTER(VfxLattice)] + [C.c_void_p] * 7,
     "vfx_enstrophy": [C.POINTER(VfxLattice)] + [C.c_void_p] * 7,
     "vfx_accumulate_stats": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p],
@@ -134,6 +135,8 @@ def lib():
             fn.restype = C.c_int32
         L.vfx_reduce_workspace_bytes.argtypes = [C.POINTER(VfxLattice)]
         L.vfx_reduce_workspace_bytes.restype = C.c_uint64
+        L.vfx_reduce_noalloc_max_items.argtypes = []
+        L.vfx_reduce_noalloc_max_items.restype = C.c_uint64
         L.vfx_last_error.restype = C.c_char_p
         L.vfx_version.restype = C.c_int32
         _lib = L

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 namespace vfx {
@@ -427,6 +428,68 @@ vfx_status vfx_reduce_mass_momentum(const vfx_lattice* lat, int32_t is_even, dou
     default: e = vfx::launch_reduce_mass<_Float16>(L, lat->direct, is_even != 0, workspace, out4, st); break;
     }
     return hip_status(e, "mass/momentum reduction launch");
+}
+
+}
+
+// ------------------------------------------- reduction without allocation ----
+// vfx_reduce_mass_momentum_noalloc keeps the per-block partial sums in this
+// file-scope device array instead of a caller workspace: 65536 blocks x 4
+// doubles = 2 MiB per GPU, held until the process ends.  In host code the
+// array's name is not a device address, so each device's address is looked up
+// once (hipGetSymbolAddress, with that device current) and only that address
+// is handed to kernels.  Every call on one GPU shares the array: calls that
+// overlap on one GPU (two streams, two host threads) race.  Nothing in it has
+// to start at zero (each reduction block writes its own four partials before
+// the final pass reads them).
+namespace {
+constexpr uint32_t kNoAllocMaxBlocks = 65536;
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+} // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocMaxBlocks * 4];
+
+static vfx_status scratch_for_current_device(double** out)
+{
+    int dev = -1;
+    if (hipError_t e = hipGetDevice(&dev)) return hip_status(e, "hipGetDevice");
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return fail(VFX_EUNSUPPORTED, "device %d beyond the scratch table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) return hip_status(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(VFX_EHIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return VFX_OK;
+}
+
+extern "C" {
+
+uint64_t vfx_reduce_noalloc_max_items(void) { return (uint64_t)kNoAllocMaxBlocks * vfx::kRedChunk; }
+
+vfx_status vfx_reduce_mass_momentum_noalloc(const vfx_lattice* lat, int32_t is_even, double* out4, vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    if (!out4) return fail(VFX_EINVAL, "out4 is NULL");
+    const uint64_t n = vfx::node_items(L, lat->direct);
+    if (vfx::reduce_blocks(n ? n : 1) > kNoAllocMaxBlocks)
+        return fail(VFX_EINVAL, "%llu work items exceed the no-alloc reduction's maximum of %llu",
+                    (unsigned long long)n, (unsigned long long)vfx_reduce_noalloc_max_items());
+    double* ws = nullptr;
+    if (vfx_status s = scratch_for_current_device(&ws)) return s;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    switch (lat->storage) {
+    case VFX_FP64: e = vfx::launch_reduce_mass<double>(L, lat->direct, is_even != 0, ws, out4, st); break;
+    case VFX_FP32: e = vfx::launch_reduce_mass<float>(L, lat->direct, is_even != 0, ws, out4, st); break;
+    default: e = vfx::launch_reduce_mass<_Float16>(L, lat->direct, is_even != 0, ws, out4, st); break;
+    }
+    return hip_status(e, "mass/momentum reduction launch (no-alloc)");
 }
 
 static vfx_status energy(const vfx_lattice* lat, bool enstrophy, const double* drho, const double* vx, const double* vy,
