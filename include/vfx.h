@@ -192,17 +192,24 @@ typedef enum {
     VFX_BC_PRECURSOR_DISTRIBUTIONS = 13
 } vfx_bc_kind;
 
-/* QforBoundaryConditions (src/gpu/core/Calculation/Calculation.h:105-117) */
+/* QforBoundaryConditions (src/gpu/core/Calculation/Calculation.h:105-117).
+ * Every floating-point array is of the lattice's `real`: double for VFX_FP64
+ * storage (VirtualFluids built with VF_DOUBLE_ACCURACY), float for VFX_FP32 and
+ * VFX_FP16_SCALED (VirtualFluids' default `real`).  The reference's launchers
+ * hand their kernels q27[0], the base of one SoA block of 27 * numberOfBCnodes
+ * values (e.g. NoSlip.cu, Stress.cu:51); q27 here is that pointer. */
 typedef struct vfx_bc {
     int32_t kind;              /* vfx_bc_kind */
     uint32_t n;                /* numberOfBCnodes */
-    const uint32_t* k;         /* node index of each boundary node */
-    const float* q27;          /* 27*n subgrid distances, SoA; link d of node i is a wall link iff 0 <= q27[d*n+i] <= 1 */
-    const double* vx;          /* wall velocity per boundary node (NULL = 0) */
-    const double* vy;
-    const double* vz;
-    int32_t cpu_float_velocity; /* 1: wall term rounded as VirtualFluids' CPU BoundaryConditions does (float storage) */
-    const double* rho;         /* VFX_BC_PRESSURE_NEQ: prescribed density deviation per boundary node (RhoBC) */
+    const uint32_t* k;         /* node index of each boundary node (QforBoundaryConditions::k) */
+    const void* q27;           /* real[27*n] subgrid distances, SoA (q27[0]); link d of node i is a wall link
+                                  iff 0 <= q27[d*n+i] <= 1 */
+    const void* vx;            /* real[n] wall velocity per boundary node, Vx/Vy/Vz (NULL = 0) */
+    const void* vy;
+    const void* vz;
+    int32_t cpu_float_velocity; /* 1: wall term of VFX_BC_VELOCITY_BOUNCE_BACK rounded as VirtualFluids' CPU
+                                   BoundaryConditions does (float velocity and sum, BoundaryConditions.h:195-262) */
+    const void* rho;           /* real[n], VFX_BC_PRESSURE_NEQ: prescribed density deviation per boundary node (RhoBC) */
     const uint32_t* k_neighbor;/* VFX_BC_PRESSURE_NEQ / outflow: inward neighbour node of each boundary node */
     int32_t normal;            /* VFX_BC_PRESSURE_NEQ / outflow: outward direction of the boundary (e.g. 1 = dP00);
                                   the outflow kinds act on the six face directions only */
@@ -235,6 +242,10 @@ typedef struct vfx_bc {
     const void* precursor_current;
     double time_ratio;
     double velocity_ratio;
+    /* VFX_BC_PRECURSOR_VELOCITY: QforPrecursorBoundaryConditions velocityX/Y/Z,
+       the scalars the reference passes by value; added to vx/vy/vz[i] (0 where
+       those are NULL) before the blended plane velocity */
+    double precursor_velocity[3];
 } vfx_bc;
 
 /* ---------------------------------------------------------------- kernels */
@@ -332,7 +343,6 @@ uint64_t vfx_reduce_noalloc_max_items(void);
 vfx_status vfx_reduce_mass_momentum_noalloc(const vfx_lattice* lat, int32_t is_even_timestep, double* out4,
                                             vfx_stream stream);
 
-
 /* KineticEnergyAnalyzer::run (src/gpu/core/PostProcessor/KineticEnergyAnalyzer.cu:66-154) and
  * EnstrophyAnalyzer::run (EnstrophyAnalyzer.cu:64-336) on macroscopic fields from
  * vfx_macroscopic: out2 = {sum, number of fluid nodes with a fluid 7-point stencil};
@@ -362,12 +372,12 @@ vfx_status vfx_accumulate_stats(const vfx_lattice* lat, int32_t is_even_timestep
  * offset_x/y/z (may be NULL) shift the interpolation point for cells next to
  * walls, in source-cell units. */
 typedef struct {
-    const uint32_t* fine_cells;
-    const uint32_t* coarse_cells;
-    uint32_t n_cells;
-    const double* offset_x;
-    const double* offset_y;
-    const double* offset_z;
+    const uint32_t* fine_cells;   /* ICells::fineCellIndices */
+    const uint32_t* coarse_cells; /* ICells::coarseCellIndices */
+    uint32_t n_cells;             /* ICells::numberOfCells */
+    const void* offset_x;         /* ICellNeigh x/y/z: real[n_cells] (double for VFX_FP64, float otherwise) */
+    const void* offset_y;
+    const void* offset_z;
 } vfx_icells;
 
 /* scaleCoarseToFineCompressible (src/gpu/core/GridScaling/Scaling.cu:105-138,
@@ -421,8 +431,14 @@ vfx_status vfx_batch_init_rest(vfx_batch batch, vfx_stream stream);
  * tile, otherwise a persistent grid that strides over the tiles */
 vfx_status vfx_batch_set_launch(vfx_batch batch, uint32_t block, uint32_t grid);
 vfx_status vfx_batch_get_launch(vfx_batch batch, uint32_t* block, uint32_t* grid);
-/* times every launch shape on `steps` (even) steps from parity is_even_timestep
- * and keeps the fastest; the state advances, its parity does not change */
+/* tile order: workgroup b starts at tile xcd_block(b), which gives each of the 8
+ * XCDs runs of `run` consecutive tiles (tiles next to each other share cache
+ * lines, and every XCD has its own L2); 1 = plain order */
+vfx_status vfx_batch_set_xcd_run(vfx_batch batch, uint32_t run);
+vfx_status vfx_batch_get_xcd_run(vfx_batch batch, uint32_t* run);
+/* times every launch shape (block, grid, XCD run of 1 / 4 / 16 tiles) on
+ * `steps` (even) steps from parity is_even_timestep and keeps the fastest; the
+ * state advances, its parity does not change */
 vfx_status vfx_batch_autotune(vfx_batch batch, int32_t is_even_timestep, int32_t steps, vfx_stream stream,
                               uint32_t* block, uint32_t* grid, float* ms_per_step);
 /* one step of every domain in one launch / of one domain (the per-domain baseline) */

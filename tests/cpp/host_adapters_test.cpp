@@ -1,5 +1,6 @@
 // Drives the C++ host layer (KernelFactory / Kernel / BoundaryConditionFactory /
-// UpdateGrid27) the way VirtualFluids' Simulation does.
+// UpdateGrid27) the way VirtualFluids' Simulation does.  Built with
+// VF_DOUBLE_ACCURACY (real = double): the lattices here are FP64.
 //   host_adapters_test cpu : factory and error behaviour only (no device calls)
 //   host_adapters_test gpu : a periodic K17 lattice stepped through the host
 //                            layer equals the same lattice stepped through the
@@ -378,19 +379,19 @@ static void gpu_turbulence_tests()
 static void restart_tests(const std::string& prefix)
 {
     const std::vector<std::vector<float>> data = {{1, 2, 3}, {4, 5, 6}, {0.123456789f, -1e-7f, 12345678.f, -0.f}};
-    BinaryRestartObject b;
+    BinaryRestartObjectT<float> b;
     b.fs = data;
     b.serialize_internal(prefix + "_cpp");
-    ASCIIRestartObject a;
+    ASCIIRestartObjectT<float> a;
     a.fs = data;
     a.serialize_internal(prefix + "_cpp");
     BinaryRestartObjectT<double> d;
     d.fs = {{1, 2, 3}, {0.1, -2.5e-300}};
     d.serialize_internal(prefix + "_cpp64");
-    BinaryRestartObject rb;
+    BinaryRestartObjectT<float> rb;
     rb.deserialize_internal(prefix + "_py");
     EXPECT(rb.fs == data);
-    ASCIIRestartObject ra;
+    ASCIIRestartObjectT<float> ra;
     ra.deserialize_internal(prefix + "_py");
     EXPECT(ra.fs.size() == data.size());
     for (size_t i = 0; i < ra.fs.size() && i < data.size(); ++i) {

@@ -137,7 +137,7 @@ def test_batch_arguments_fail_without_device():
 
 
 def build_host_test():
-    cmd = ["g++", "-O1", "-std=c++17", TEST_SRC, "-o", TEST_BIN, "-L" + os.path.dirname(vfx._lib.LIB_PATH), "-lvfx",
+    cmd = ["g++", "-O1", "-std=c++17", "-DVF_DOUBLE_ACCURACY", TEST_SRC, "-o", TEST_BIN, "-L" + os.path.dirname(vfx._lib.LIB_PATH), "-lvfx",
            "-Wl,-rpath," + os.path.dirname(vfx._lib.LIB_PATH)]
     subprocess.run(cmd, check=True)
     return TEST_BIN

@@ -15,6 +15,7 @@ the reference's schedule: at step t, last = plane(mT), current =
 plane((m+1)T), ratio (t - mT)/T with m = floor(t/T).
 """
 import numpy as np
+import pytest
 import torch
 
 import oracle
@@ -149,7 +150,14 @@ def test_plane_cycling_closed_form():
         assert float(pc.last[0, 0]) == m * T * 0.5 and float(pc.current[0, 0]) == (m + 1) * T * 0.5
     assert loads == [k * T * 0.5 for k in range(len(loads))]
     b = BoundaryCondition(13, np.arange(2), precursor=pc, device="cpu")
-    assert b.q.shape == (27, 2) and not ((b.q >= 0) & (b.q <= 1)).any()
+    q = b.tables(torch.float32)["q"]
+    assert q.shape == (27, 2) and q.dtype == torch.float32 and not ((q >= 0) & (q <= 1)).any()
+    # the planes rotate through three arrays allocated once: a new read lands in
+    # the array that held `last`
+    ptrs = {pc.last.data_ptr(), pc.current.data_ptr(), pc.next.data_ptr()}
+    pc.advance(14)
+    pc.advance(15)
+    assert {pc.last.data_ptr(), pc.current.data_ptr(), pc.next.data_ptr()} == ptrs
     try:
         BoundaryCondition(12, np.arange(2), np.zeros((27, 2)), precursor=pc, device="cpu")
     except ValueError:
@@ -157,3 +165,35 @@ def test_plane_cycling_closed_form():
     else:
         raise AssertionError("a velocity inlet must refuse nine-component planes")
     assert torch.equal(pc.neighbors, torch.zeros((4, 2), dtype=torch.int32))
+
+
+def test_plane_series_is_the_stepwise_closed_form_cycle():
+    """The oracle's plane series (one run through the read cycle) equals
+    stepping one state at a time with last = plane(mT), current = plane((m+1)T)
+    and ratio (t - mT)/T for the distributions kind (no state carried between
+    steps).  For the velocity kind one run is needed: its two-step lag carries
+    each step's value into the state after next."""
+    dims = (10, 6, 6)
+    solid, groups, onodes, onbrs = cases.inflow_channel(*dims)
+    inlet, iq = groups["inlet"]
+    n = inlet.size
+    rng = np.random.default_rng(2)
+    nb = rng.integers(0, 12, (4, n))
+    w = rng.uniform(0.1, 2.0, (4, n))
+    base = oracle.init_field(np.zeros(1), np.full(1, 0.03), np.zeros(1), np.zeros(1))[0][PDIRS]
+    T, steps = 2, 7
+    series = [base[:, None] + 1e-4 * rng.normal(size=(9, 12)) for _ in range(steps // T + 2)]
+    F0 = oracle.init_field(*cases.random_flow(dims, seed=1, u_amp=0.01))
+    F0[solid] = 0.0
+    walls = [oracle.BCGroup(onodes, kind=6, nbr=onbrs, normal=1), oracle.BCGroup(*groups["wall"], kind=1)]
+    pre = dict(neighbors=nb, weights=w, series=series, period=T, t0=1)
+    one = oracle.run(F0, dims, steps, 1.9, solid=solid, bcs=walls + [oracle.BCGroup(inlet, kind=13, precursor=pre)])
+    F = F0
+    for t in range(1, steps + 1):
+        m = t // T
+        p = dict(neighbors=nb, weights=w, last=series[m], current=series[m + 1], time_ratio=(t - m * T) / T)
+        F = oracle.run(F, dims, 1, 1.9, solid=solid, bcs=walls + [oracle.BCGroup(inlet, kind=13, precursor=p)])
+    assert np.array_equal(one, F)
+    # a series too short for the steps is refused
+    with pytest.raises(RuntimeError):
+        oracle.run(F0, dims, steps + 4, 1.9, solid=solid, bcs=walls + [oracle.BCGroup(inlet, kind=13, precursor=pre)])

@@ -55,7 +55,7 @@ class VfxBC(C.Structure):
         ("wm_force", C.c_void_p * 3),
         ("n_precursor_nodes", C.c_uint32), ("plane_neighbor", C.c_void_p * 4), ("plane_weight", C.c_void_p * 4),
         ("precursor_last", C.c_void_p), ("precursor_current", C.c_void_p), ("time_ratio", C.c_double),
-        ("velocity_ratio", C.c_double),
+        ("velocity_ratio", C.c_double), ("precursor_velocity", C.c_double * 3),
     ]
 
 
@@ -106,6 +106,8 @@ _SIGNATURES = {
     "vfx_batch_init_rest": [C.c_void_p, C.c_void_p],
     "vfx_batch_set_launch": [C.c_void_p, C.c_uint32, C.c_uint32],
     "vfx_batch_get_launch": [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)],
+    "vfx_batch_set_xcd_run": [C.c_void_p, C.c_uint32],
+    "vfx_batch_get_xcd_run": [C.c_void_p, C.POINTER(C.c_uint32)],
     "vfx_batch_autotune": [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                            C.POINTER(C.c_float)],
     "vfx_batch_step": [C.c_void_p, C.c_int32, C.c_void_p],

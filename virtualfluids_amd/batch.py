@@ -130,6 +130,17 @@ class Ensemble:
                                            C.byref(ms)), "vfx_batch_autotune")
         return b.value, g.value, ms.value
 
+    @property
+    def xcd_run(self) -> int:
+        """Tiles per XCD run of the batched launch (1 = plain order)."""
+        r = C.c_uint32()
+        V.check(V.lib().vfx_batch_get_xcd_run(self.handle, C.byref(r)), "vfx_batch_get_xcd_run")
+        return r.value
+
+    @xcd_run.setter
+    def xcd_run(self, run: int):
+        V.check(V.lib().vfx_batch_set_xcd_run(self.handle, int(run)), "vfx_batch_set_xcd_run")
+
     def step(self, n=1, stream=None):
         for _ in range(n):
             V.check(V.lib().vfx_batch_step(self.handle, int(self.even), _stream(stream)), "vfx_batch_step")

@@ -33,8 +33,11 @@ UNITS = {
     "step_fp32.hip": [],
     "api.hip": ["-ffp-contract=off"],
 }
+# the C++ host layer once per VirtualFluids `real` (float, and double as with
+# VF_DOUBLE_ACCURACY); its inline namespaces keep the two apart
 HOST_UNITS = {
     "host/vfx_host.cpp": [],
+    "host/vfx_host.cpp#double": ["-DVF_DOUBLE_ACCURACY"],
 }
 GXX = shutil.which("g++") or "g++"
 
@@ -51,9 +54,9 @@ def _compile(src: str, flags: list[str]) -> str:
 
 
 def _compile_host(src: str, flags: list[str]) -> str:
-    out = os.path.join(OBJ, src.replace("/", "_") + ".o")
-    cmd = [GXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off"] + flags + ["-c", os.path.join(CSRC, src),
-                                                                                    "-o", out]
+    out = os.path.join(OBJ, src.replace("/", "_").replace("#", "_") + ".o")
+    path = os.path.join(CSRC, src.split("#")[0])
+    cmd = [GXX, "-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off"] + flags + ["-c", path, "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"g++ failed for {src}:\n{' '.join(cmd)}\n{r.stderr}")

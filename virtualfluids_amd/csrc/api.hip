@@ -286,7 +286,8 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
                  bc->n_precursor_nodes,
                  {bc->plane_neighbor[0], bc->plane_neighbor[1], bc->plane_neighbor[2], bc->plane_neighbor[3]},
                  {bc->plane_weight[0], bc->plane_weight[1], bc->plane_weight[2], bc->plane_weight[3]},
-                 bc->precursor_last, bc->precursor_current, bc->time_ratio, bc->velocity_ratio};
+                 bc->precursor_last, bc->precursor_current, bc->time_ratio, bc->velocity_ratio,
+                 {bc->precursor_velocity[0], bc->precursor_velocity[1], bc->precursor_velocity[2]}};
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     switch (lat->storage) {
@@ -610,7 +611,7 @@ struct vfx_batch_s {
     uint2* dev_tiles[3] = {nullptr, nullptr, nullptr}; // tile tables for blocks 64/128/256
     std::vector<uint32_t> first_tile[3];       // per domain, n+1 entries
     uint32_t ntiles[3] = {0, 0, 0};
-    unsigned block = 256, grid = 0;
+    unsigned block = 256, grid = 0, xcd_run = 1;
     float ld[4], st[4];
 };
 
@@ -671,6 +672,7 @@ vfx::BatchDev batch_view(const vfx_batch_s* b, int bi, uint32_t first, uint32_t 
     B.domains = b->dev_domains;
     B.tiles = b->dev_tiles[bi] + first;
     B.ntiles = count;
+    B.xcd_run = b->xcd_run;
     for (int c = 0; c < 4; ++c) {
         B.ld[c] = b->ld[c];
         B.st[c] = b->st[c];
@@ -788,6 +790,21 @@ vfx_status vfx_batch_set_launch(vfx_batch b, uint32_t block, uint32_t grid)
     return VFX_OK;
 }
 
+vfx_status vfx_batch_set_xcd_run(vfx_batch b, uint32_t run)
+{
+    if (!b) return fail(VFX_EINVAL, "batch is NULL");
+    if (run == 0 || run > 4096) return fail(VFX_EINVAL, "xcd run %u outside 1..4096", run);
+    b->xcd_run = run;
+    return VFX_OK;
+}
+
+vfx_status vfx_batch_get_xcd_run(vfx_batch b, uint32_t* run)
+{
+    if (!b || !run) return fail(VFX_EINVAL, "NULL argument");
+    *run = b->xcd_run;
+    return VFX_OK;
+}
+
 vfx_status vfx_batch_get_launch(vfx_batch b, uint32_t* block, uint32_t* grid)
 {
     if (!b || !block || !grid) return fail(VFX_EINVAL, "NULL argument");
@@ -867,16 +884,19 @@ vfx_status vfx_batch_autotune(vfx_batch b, int32_t is_even, int32_t steps, vfx_s
         if (e0) (void)hipEventDestroy(e0);
         return fail(VFX_EHIP, "event create");
     }
-    const unsigned keep_block = b->block, keep_grid = b->grid;
+    const unsigned keep_block = b->block, keep_grid = b->grid, keep_run = b->xcd_run;
     float best = 1e30f;
-    unsigned bb = keep_block, bg = keep_grid;
+    unsigned bb = keep_block, bg = keep_grid, br = keep_run;
     vfx_status r = VFX_OK;
     const unsigned blocks[3] = {64, 128, 256};
+    const unsigned runs[3] = {1, 4, 16};
     for (unsigned block : blocks) {
         const unsigned grids[4] = {0, (unsigned)cus * 8, (unsigned)cus * 16, (unsigned)cus * 32};
-        for (unsigned grid : grids) {
+        for (int cfg = 0; cfg < 12; ++cfg) {
+            const unsigned grid = grids[cfg / 3], run = runs[cfg % 3];
             b->block = block;
             b->grid = grid;
+            b->xcd_run = run;
             for (int s = 0; s < 2 && r == VFX_OK; ++s) r = vfx_batch_step(b, (s % 2 == 0) == even0, stream); // warm-up
             if (r == VFX_OK) r = hip_status(hipEventRecord(e0, st), "event record");
             for (int s = 0; s < steps && r == VFX_OK; ++s) r = vfx_batch_step(b, (s % 2 == 0) == even0, stream);
@@ -890,6 +910,7 @@ vfx_status vfx_batch_autotune(vfx_batch b, int32_t is_even, int32_t steps, vfx_s
                 best = ms;
                 bb = block;
                 bg = grid;
+                br = run;
             }
         }
         if (r != VFX_OK) break;
@@ -899,10 +920,12 @@ vfx_status vfx_batch_autotune(vfx_batch b, int32_t is_even, int32_t steps, vfx_s
     if (r != VFX_OK) {
         b->block = keep_block;
         b->grid = keep_grid;
+        b->xcd_run = keep_run;
         return r;
     }
     b->block = bb;
     b->grid = bg;
+    b->xcd_run = br;
     if (block_out) *block_out = bb;
     if (grid_out) *grid_out = bg;
     if (ms_out) *ms_out = best;

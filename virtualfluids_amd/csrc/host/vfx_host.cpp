@@ -8,7 +8,7 @@
 #include <sstream>
 #include <string>
 
-namespace vfx::host {
+VFX_HOST_BEGIN
 
 void check(vfx_status s, const char* what)
 {
@@ -120,38 +120,93 @@ std::shared_ptr<Kernel> KernelFactoryImp::makeKernel(std::shared_ptr<Parameter> 
     throw std::runtime_error("KernelFactory does not know the KernelType.");
 }
 
+// The lattice storage must carry the host layer's `real`: its boundary and
+// interface tables go to the C ABI as they are.
+static void check_real(const vfx_lattice& lat, const char* what)
+{
+    if ((lat.storage == VFX_FP64) != (sizeof(real) == 8))
+        throw std::runtime_error(std::string(what) +
+                                 (sizeof(real) == 8 ? ": the host layer built with VF_DOUBLE_ACCURACY needs VFX_FP64 storage"
+                                                    : ": the host layer built with float real needs VFX_FP32 or "
+                                                      "VFX_FP16_SCALED storage"));
+}
+
+// VelocityBounceBack / NoSlip* / Slip* launchers (e.g. Velocity.cu, NoSlip.cu):
+// QforBoundaryConditions k, q27[0], Vx/Vy/Vz as they are
 static boundaryCondition make_bc(int32_t kind)
 {
     return [kind](LBMSimulationParameter* p, QforBoundaryConditions* q) {
+        check_real(p->lattice, "boundary condition");
         vfx_bc bc{};
         bc.kind = kind;
         bc.n = q->numberOfBCnodes;
-        bc.k = q->k;
-        bc.q27 = q->q27;
+        bc.k = reinterpret_cast<const uint32_t*>(q->k);
+        bc.q27 = q->q27[0];
         bc.vx = q->Vx;
         bc.vy = q->Vy;
         bc.vz = q->Vz;
-        bc.rho = q->RhoBC;
-        bc.k_neighbor = q->kN;
-        bc.normal = q->direction;
-        bc.pressure_correction_factor = p->outflowPressureCorrectionFactor;
-        bc.wm_normal[0] = q->normalX;
-        bc.wm_normal[1] = q->normalY;
-        bc.wm_normal[2] = q->normalZ;
-        for (int a = 0; a < 3; ++a) {
-            bc.wm_exchange_velocity[a] = q->exchangeVelocity[a];
-            bc.wm_wall_velocity[a] = q->wallVelocity[a];
-            bc.wm_force[a] = q->wallForce[a];
-        }
-        bc.wm_sampling_offset = q->samplingOffset;
-        bc.wm_z0 = q->z0;
-        bc.wm_u_star = q->u_star;
         check(vfx_bc_apply(&p->lattice, &bc, p->isEvenTimestep ? 1 : 0, p->stream), "boundary condition");
     };
 }
 
-// Precursor/Precursor.cu:44-117 through vfx_bc_apply kinds 12 / 13
-precursorBoundaryCondition BoundaryConditionFactory::getPrecursorBoundaryConditionPost() const
+// Pressure.cu:63-80, Outflow.cu:44-90: QforDirectionalBoundaryCondition k, kN,
+// RhoBC and direction (the outflow kinds also the level's correction factor)
+static boundaryConditionDirectional make_directional(int32_t kind)
+{
+    return [kind](LBMSimulationParameter* p, QforDirectionalBoundaryCondition* q) {
+        check_real(p->lattice, "pressure boundary condition");
+        vfx_bc bc{};
+        bc.kind = kind;
+        bc.n = q->numberOfBCnodes;
+        bc.k = reinterpret_cast<const uint32_t*>(q->k);
+        bc.q27 = q->q27[0];
+        bc.rho = q->RhoBC;
+        bc.k_neighbor = reinterpret_cast<const uint32_t*>(q->kN);
+        bc.normal = (int32_t)q->direction;
+        bc.pressure_correction_factor = p->outflowPressureCorrectionFactor;
+        check(vfx_bc_apply(&p->lattice, &bc, p->isEvenTimestep ? 1 : 0, p->stream), "pressure boundary condition");
+    };
+}
+
+// Stress.cu:43-80: the boundary's k, kN (sampling node), q27[0], normals,
+// Vx/Vy/Vz and Vx1/Vy1/Vz1 (filtered velocities, updated) and the level's
+// WallModelParameters (monitors written when hasMonitor)
+static boundaryConditionWithParameter make_stress(int32_t kind)
+{
+    return [kind](Parameter* para, QforBoundaryConditions* q, const int level) {
+        LBMSimulationParameter* p = para->getParD(level);
+        check_real(p->lattice, "stress boundary condition");
+        vfx_bc bc{};
+        bc.kind = kind;
+        bc.n = q->numberOfBCnodes;
+        bc.k = reinterpret_cast<const uint32_t*>(q->k);
+        bc.q27 = q->q27[0];
+        bc.k_neighbor = reinterpret_cast<const uint32_t*>(q->kN);
+        bc.wm_normal[0] = q->normalX;
+        bc.wm_normal[1] = q->normalY;
+        bc.wm_normal[2] = q->normalZ;
+        bc.wm_exchange_velocity[0] = q->Vx;
+        bc.wm_exchange_velocity[1] = q->Vy;
+        bc.wm_exchange_velocity[2] = q->Vz;
+        bc.wm_wall_velocity[0] = q->Vx1;
+        bc.wm_wall_velocity[1] = q->Vy1;
+        bc.wm_wall_velocity[2] = q->Vz1;
+        const WallModelParameters& wm = p->wallModel;
+        bc.wm_sampling_offset = wm.samplingOffset;
+        bc.wm_z0 = wm.z0;
+        if (wm.hasMonitor) {
+            bc.wm_u_star = wm.u_star;
+            bc.wm_force[0] = wm.Fx;
+            bc.wm_force[1] = wm.Fy;
+            bc.wm_force[2] = wm.Fz;
+        }
+        check(vfx_bc_apply(&p->lattice, &bc, p->isEvenTimestep ? 1 : 0, p->stream), "stress boundary condition");
+    };
+}
+
+// Precursor.cu:44-117 through vfx_bc_apply kinds 12 / 13 (velocityX/Y/Z are
+// the reference's scalars, handed over as they are)
+precursorBoundaryConditionFunc BoundaryConditionFactory::getPrecursorBoundaryConditionPost() const
 {
     int32_t kind;
     switch (precursor) {
@@ -159,20 +214,25 @@ precursorBoundaryCondition BoundaryConditionFactory::getPrecursorBoundaryConditi
     case PrecursorBC::PrecursorDistributions: kind = VFX_BC_PRECURSOR_DISTRIBUTIONS; break;
     default: return nullptr;
     }
-    return [kind](LBMSimulationParameter* p, QforPrecursorBoundaryConditions* q, double tRatio, double velocityRatio) {
+    return [kind](LBMSimulationParameter* p, QforPrecursorBoundaryConditions* q, real tRatio, real velocityRatio) {
+        check_real(p->lattice, "precursor boundary condition");
         vfx_bc bc{};
         bc.kind = kind;
-        bc.n = q->numberOfBCnodes;
-        bc.k = q->k;
-        bc.q27 = q->q27;
-        bc.vx = q->velocityX;
-        bc.vy = q->velocityY;
-        bc.vz = q->velocityZ;
-        bc.n_precursor_nodes = q->numberOfPrecursorNodes;
-        for (int c = 0; c < 4; ++c) {
-            bc.plane_neighbor[c] = q->planeNeighbor[c];
-            bc.plane_weight[c] = q->weights[c];
-        }
+        bc.n = (uint32_t)q->numberOfBCnodes;
+        bc.k = reinterpret_cast<const uint32_t*>(q->k);
+        bc.q27 = q->q27[0];
+        bc.precursor_velocity[0] = q->velocityX;
+        bc.precursor_velocity[1] = q->velocityY;
+        bc.precursor_velocity[2] = q->velocityZ;
+        bc.n_precursor_nodes = (uint32_t)q->numberOfPrecursorNodes;
+        bc.plane_neighbor[0] = q->planeNeighbor0PP;
+        bc.plane_neighbor[1] = q->planeNeighbor0PM;
+        bc.plane_neighbor[2] = q->planeNeighbor0MP;
+        bc.plane_neighbor[3] = q->planeNeighbor0MM;
+        bc.plane_weight[0] = q->weights0PP;
+        bc.plane_weight[1] = q->weights0PM;
+        bc.plane_weight[2] = q->weights0MP;
+        bc.plane_weight[3] = q->weights0MM;
         bc.precursor_last = q->last;
         bc.precursor_current = q->current;
         bc.time_ratio = tRatio;
@@ -206,25 +266,29 @@ boundaryCondition BoundaryConditionFactory::getSlipBoundaryConditionPost() const
     default: return nullptr;
     }
 }
-boundaryCondition BoundaryConditionFactory::getStressBoundaryConditionPost() const
+boundaryConditionWithParameter BoundaryConditionFactory::getStressBoundaryConditionPost() const
 {
     switch (stress) {
-    case StressBC::StressCompressible: return make_bc(VFX_BC_STRESS_INTERPOLATED);
-    case StressBC::StressBounceBackCompressible: return make_bc(VFX_BC_STRESS_BOUNCE_BACK);
-    case StressBC::StressBounceBackPressureCompressible: return make_bc(VFX_BC_STRESS_BOUNCE_BACK_PRESSURE);
+    case StressBC::StressCompressible: return make_stress(VFX_BC_STRESS_INTERPOLATED);
+    case StressBC::StressBounceBackCompressible: return make_stress(VFX_BC_STRESS_BOUNCE_BACK);
+    case StressBC::StressBounceBackPressureCompressible: return make_stress(VFX_BC_STRESS_BOUNCE_BACK_PRESSURE);
     default: return nullptr;
     }
 }
 
-boundaryCondition BoundaryConditionFactory::getPressureBoundaryConditionPre() const
+std::variant<boundaryCondition, boundaryConditionDirectional> BoundaryConditionFactory::getPressureBoundaryConditionPre() const
 {
     switch (pressure) {
-    case PressureBC::PressureNonEquilibriumCompressible: return make_bc(VFX_BC_PRESSURE_NEQ);
-    case PressureBC::OutflowNonReflective: return make_bc(VFX_BC_OUTFLOW_NON_REFLECTING);
+    case PressureBC::PressureNonEquilibriumCompressible: return make_directional(VFX_BC_PRESSURE_NEQ);
+    case PressureBC::OutflowNonReflective: return make_directional(VFX_BC_OUTFLOW_NON_REFLECTING);
     case PressureBC::OutflowNonReflectivePressureCorrection:
-        return make_bc(VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION);
-    default: return nullptr;
+        return make_directional(VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION);
+    default: return boundaryCondition(nullptr);
     }
+}
+bool BoundaryConditionFactory::hasDirectionalPressureBoundaryCondition() const
+{
+    return std::holds_alternative<boundaryConditionDirectional>(getPressureBoundaryConditionPre());
 }
 
 // BoundaryConditionKernelManager.cpp:53-83: boundary nodes without a selected
@@ -246,6 +310,7 @@ static gridScaling make_scaling(bool coarse_to_fine)
         ic.fine_cells = cells->fineCellIndices;
         ic.coarse_cells = cells->coarseCellIndices;
         ic.n_cells = cells->numberOfCells;
+        check_real(c->lattice, "grid scaling");
         ic.offset_x = nb.x;
         ic.offset_y = nb.y;
         ic.offset_z = nb.z;
@@ -274,9 +339,10 @@ UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared
                            std::shared_ptr<TurbulenceModelFactory> tm)
     : tmFactory(std::move(tm)), para(std::move(p)), kernels(std::move(k)), velocityBC(f.getVelocityBoundaryConditionPost()),
       noSlipBC(f.getNoSlipBoundaryConditionPost()), slipBC(f.getSlipBoundaryConditionPost()),
-      pressureBC(f.getPressureBoundaryConditionPre()), stressBC(f.getStressBoundaryConditionPost()),
-      precursorBC(f.getPrecursorBoundaryConditionPost())
+      stressBC(f.getStressBoundaryConditionPost()), precursorBC(f.getPrecursorBoundaryConditionPost())
 {
+    if (f.hasDirectionalPressureBoundaryCondition())
+        directionalPressureBC = std::get<boundaryConditionDirectional>(f.getPressureBoundaryConditionPre());
     if ((int)kernels.size() != para->getMaxLevel() + 1) throw std::runtime_error("one kernel per level expected");
     // GridScalingKernelManager.cpp:45-60
     if (para->getMaxLevel() != 0) {
@@ -295,8 +361,11 @@ UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared
         require(velocityBC, d->velocityBC, "velocity");
         require(noSlipBC, d->noSlipBC, "no-slip");
         require(slipBC, d->slipBC, "slip");
-        require(pressureBC, d->pressureBC, "pressure");
-        require(stressBC, d->stressBC, "stress");
+        for (const auto& q : d->pressureBCDirectional)
+            if (q.numberOfBCnodes > 0 && !directionalPressureBC)
+                throw std::runtime_error("pressure boundary nodes exist but no boundary condition was selected");
+        if (d->stressBC.numberOfBCnodes > 0 && !stressBC)
+            throw std::runtime_error("stress boundary nodes exist but no boundary condition was selected");
         if (d->precursorBC.numberOfBCnodes && !precursorBC)
             throw std::runtime_error("precursor boundary nodes exist but no boundary condition was selected");
         if (d->precursorBC.numberOfBCnodes && l != 0)
@@ -306,7 +375,8 @@ UpdateGrid27::UpdateGrid27(std::shared_ptr<Parameter> p, std::vector<std::shared
 
 // BoundaryConditionKernelManager.cpp:346-382 on level 0 (t_level = t): once
 // the step reaches the time of `current`, rotate last <- current <- next, read
-// the following plane into next, and pass (t - lastTime) / timeStepsBetweenReads
+// the following plane (getNextData fills a host array, copied to next as
+// cudaCopyPrecursorData does) and pass (t - lastTime) / timeStepsBetweenReads
 void UpdateGrid27::runPrecursorBCKernelPost(int level, unsigned int t)
 {
     LBMSimulationParameter* p = para->getParD(level);
@@ -319,15 +389,22 @@ void UpdateGrid27::runPrecursorBCKernelPost(int level, unsigned int t)
     if (t >= currentTime) {
         lastTime = currentTime;
         nextTime += T;
-        void* tmp = q.last;
+        real* tmp = q.last;
         q.last = q.current;
         q.current = q.next;
         q.next = tmp;
-        if (q.getNextData) q.getNextData(q.next, q.numberOfPrecursorNodes, nextTime * std::exp2(-level) * para->getTimeRatio());
+        if (q.getNextData) {
+            const size_t n = (size_t)q.numberOfPrecursorNodes * q.numberOfQuantities;
+            std::vector<real> plane(n);
+            q.getNextData(plane.data(), q.numberOfPrecursorNodes, nextTime * std::exp2(-level) * para->getTimeRatio());
+            // the kernels of this level read the planes on p->stream: finish them before overwriting `next`
+            check(vfx_synchronize(p->stream), "precursor plane read");
+            check(vfx_copy_to_device(q.next, plane.data(), n * sizeof(real)), "precursor plane copy");
+        }
         q.nPrecursorReads++;
     }
-    const double tRatio = double(t - lastTime) / T;
-    precursorBC(p, &q, tRatio, para->getVelocityRatio());
+    const real tRatio = real(t - lastTime) / real(T);
+    precursorBC(p, &q, tRatio, (real)para->getVelocityRatio());
 }
 
 void UpdateGrid27::postCollisionBC(int level, unsigned int timestep)
@@ -336,14 +413,17 @@ void UpdateGrid27::postCollisionBC(int level, unsigned int timestep)
     if (p->velocityBC.numberOfBCnodes) velocityBC(p, &p->velocityBC);
     if (p->noSlipBC.numberOfBCnodes) noSlipBC(p, &p->noSlipBC);
     if (p->slipBC.numberOfBCnodes) slipBC(p, &p->slipBC);
-    if (p->stressBC.numberOfBCnodes) stressBC(p, &p->stressBC); // UpdateGrid27.cpp:241-243
-    runPrecursorBCKernelPost(level, timestep);                     // :252
+    if (p->stressBC.numberOfBCnodes) stressBC(para.get(), &p->stressBC, level); // UpdateGrid27.cpp:241-243
+    runPrecursorBCKernelPost(level, timestep);                                    // :252
 }
 
+// BoundaryConditionKernelManager::runPressureBCKernelPre (:321-328): every
+// directional pressure/outflow boundary in turn
 void UpdateGrid27::preCollisionBC(int level)
 {
     LBMSimulationParameter* p = para->getParD(level);
-    if (p->pressureBC.numberOfBCnodes) pressureBC(p, &p->pressureBC);
+    for (auto& q : p->pressureBCDirectional)
+        if (q.numberOfBCnodes) directionalPressureBC(p, &q);
 }
 
 void UpdateGrid27::refinement(int level)
@@ -506,10 +586,10 @@ template class ASCIIRestartObjectT<double>;
 template class BinaryRestartObjectT<float>;
 template class BinaryRestartObjectT<double>;
 
-} // namespace vfx::host
+VFX_HOST_END
 
 // ---------------------------------------------------------------- output ----
-namespace vfx::host {
+VFX_HOST_BEGIN
 
 uint32_t FilePartCalculator::calculateNumberOfNodesInPart(uint32_t numberOfNodes, uint32_t indexOfFilePart)
 {
@@ -815,4 +895,4 @@ void FileWriter::writeTimestep(const std::shared_ptr<Parameter>& para, unsigned 
     fileNamesForCollectionFile.clear();
 }
 
-} // namespace vfx::host
+VFX_HOST_END

@@ -20,9 +20,27 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <variant>
 #include <vector>
 
-namespace vfx::host {
+// VirtualFluids' `real` (src/basics/DataTypes.h): float, or double when built
+// with VF_DOUBLE_ACCURACY.  The host layer is compiled once for each (both are
+// in libvfx.so); the inline namespace keeps the two builds' symbols apart, so
+// a caller sees the structs of the `real` it was compiled with.  The lattice
+// storage must match: VFX_FP64 for double, VFX_FP32 or VFX_FP16_SCALED for float.
+#ifdef VF_DOUBLE_ACCURACY
+#define VFX_HOST_BEGIN namespace vfx::host { inline namespace real_double {
+#define VFX_HOST_REAL double
+#else
+#define VFX_HOST_BEGIN namespace vfx::host { inline namespace real_float {
+#define VFX_HOST_REAL float
+#endif
+#define VFX_HOST_END } }
+
+VFX_HOST_BEGIN
+
+using real = VFX_HOST_REAL;
+using uint = unsigned int;
 
 // src/gpu/core/Calculation/Calculation.h:55-66
 enum class CollisionTemplate { Default, WriteMacroVars, ApplyBodyForce, AllFeatures, SubDomainBorder };
@@ -32,64 +50,81 @@ enum class PreProcessorType { InitNavierStokesCompressible };
 // vf::lbm::TurbulenceModel (src/lbm/collision/TurbulentViscosity.h:50-61)
 enum class TurbulenceModel { Smagorinsky, AMD, QR, None };
 
-// Calculation.h:105-117 (plus the pressure boundary's inward neighbours)
+// Calculation.h:105-117, field for field.  q27[d] points at the SoA block of
+// direction d (q27[d] = q27[0] + d * numberOfBCnodes, as the grid provider sets
+// them); the kernels receive q27[0].  The wall models read Vx/Vy/Vz (filtered
+// velocity at the sampling node) and Vx1/Vy1/Vz1 (at the wall node), and the
+// wall normals normalX/Y/Z; kN is the sampling (stress) or inward (pressure)
+// neighbour.
 struct QforBoundaryConditions {
-    uint32_t* k = nullptr;
-    float* q27 = nullptr;       // 27 * numberOfBCnodes
-    double* Vx = nullptr;
-    double* Vy = nullptr;
-    double* Vz = nullptr;
-    double* RhoBC = nullptr;
-    uint32_t* kN = nullptr;
-    uint32_t numberOfBCnodes = 0;
-    int direction = 0;          // pressure boundary: outward direction (dP00 = 1 ...)
-    // wall models (stressBC): normalX/Y/Z, Vx/Vy/Vz = filtered exchange velocity,
-    // Vx1/Vy1/Vz1 = filtered wall velocity (lattice real type), and the level's
-    // WallModelParameters samplingOffset, z0 and monitors
-    void* normalX = nullptr;
-    void* normalY = nullptr;
-    void* normalZ = nullptr;
-    void* exchangeVelocity[3] = {nullptr, nullptr, nullptr};
-    void* wallVelocity[3] = {nullptr, nullptr, nullptr};
-    int32_t* samplingOffset = nullptr;
-    void* z0 = nullptr;
-    void* u_star = nullptr;
-    void* wallForce[3] = {nullptr, nullptr, nullptr};
+    int* k = nullptr;
+    int* kN = nullptr;
+    real* q27[27] = {};
+    real* q19[19] = {};
+    unsigned int numberOfBCnodes = 0;
+    int kArray = 0;
+    real *Vx = nullptr, *Vy = nullptr, *Vz = nullptr;
+    real *Vx1 = nullptr, *Vy1 = nullptr, *Vz1 = nullptr;
+    real *deltaVz = nullptr, *RhoBC = nullptr;
+    real *normalX = nullptr, *normalY = nullptr, *normalZ = nullptr;
 };
 
-// Calculation.h:120-150: a precursor inflow's stencil into the precursor
-// plane, the planes (arrays of the lattice's real type: 3 velocities or 9
-// populations per precursor node) and the state of the read cycle.
-// getNextData stands for TransientBCInputFileReader::getNextData: it fills the
-// device array `next` with the plane at the physical time loadTime.
+// Calculation.h:119-127: the pressure and outflow boundaries, one struct per
+// outward direction (LBMSimulationParameter::pressureBCDirectional)
+struct QforDirectionalBoundaryCondition {
+    int* k = nullptr;
+    int* kN = nullptr;
+    real* q27[27] = {};
+    unsigned int numberOfBCnodes = 0;
+    real* RhoBC = nullptr;
+    size_t direction = 0;
+};
+
+// Calculation.h:129-145.  getNextData stands for
+// TransientBCInputFileReader::getNextData (TransientBCSetter.h:174-215): it
+// fills the plane array `next` (numberOfPrecursorNodes * numberOfQuantities
+// values; host memory, as the reference's reader fills planeData before
+// cudaCopyPrecursorData) with the plane at the physical time loadTime; the
+// host layer copies it to the device array next.
 struct QforPrecursorBoundaryConditions {
-    uint32_t* k = nullptr;
-    float* q27 = nullptr;                 // PrecursorNonReflectiveCompressible only
-    double* velocityX = nullptr;          // added to the plane velocity (NULL = 0)
-    double* velocityY = nullptr;
-    double* velocityZ = nullptr;
-    uint32_t numberOfBCnodes = 0;
-    uint32_t numberOfPrecursorNodes = 0;
-    uint32_t* planeNeighbor[4] = {nullptr, nullptr, nullptr, nullptr}; // 0PP, 0PM, 0MP, 0MM
-    void* weights[4] = {nullptr, nullptr, nullptr, nullptr};
-    void* last = nullptr;
-    void* current = nullptr;
-    void* next = nullptr;
-    uint32_t timeStepsBetweenReads = 1;
-    uint32_t nPrecursorReads = 2;         // GridGenerator.cpp:477: last, current and next loaded
-    std::function<void(void* next, uint32_t numberOfPrecursorNodes, double loadTime)> getNextData;
+    int* k = nullptr;
+    int numberOfBCnodes = 0;
+    int sizeQ = 0;
+    int numberOfPrecursorNodes = 0;
+    uint streamIndex = 0;
+    uint nPrecursorReads = 0;
+    uint timeStepsBetweenReads = 1;
+    size_t numberOfQuantities = 0;
+    real* q27[27] = {};
+    uint *planeNeighbor0PP = nullptr, *planeNeighbor0PM = nullptr, *planeNeighbor0MP = nullptr,
+         *planeNeighbor0MM = nullptr;
+    real *weights0PP = nullptr, *weights0PM = nullptr, *weights0MP = nullptr, *weights0MM = nullptr;
+    real *last = nullptr, *current = nullptr, *next = nullptr;
+    real velocityX = 0, velocityY = 0, velocityZ = 0;
+    std::function<void(real* planeData, int numberOfPrecursorNodes, double loadTime)> getNextData;
+};
+
+// Calculation.h:162-171
+struct WallModelParameters {
+    real* z0 = nullptr;
+    int* samplingOffset = nullptr;
+    bool hasMonitor = false;
+    real* u_star = nullptr;
+    real* Fx = nullptr;
+    real* Fy = nullptr;
+    real* Fz = nullptr;
 };
 
 // Calculation.h:72-89: interface cells between a level and the next finer one
 struct ICells {
-    uint32_t* fineCellIndices = nullptr;
-    uint32_t* coarseCellIndices = nullptr;
-    uint32_t numberOfCells = 0;
+    uint* fineCellIndices = nullptr;
+    uint* coarseCellIndices = nullptr;
+    uint numberOfCells = 0;
 };
 struct ICellNeigh {
-    double* x = nullptr;
-    double* y = nullptr;
-    double* z = nullptr;
+    real* x = nullptr;
+    real* y = nullptr;
+    real* z = nullptr;
 };
 
 // Host copies of a level's grid the output needs (LBMSimulationParameter's
@@ -110,8 +145,11 @@ struct LBMSimulationParameter {
     // fluid node lists per collision template (Parameter::taggedFluidNodeIndices)
     std::map<CollisionTemplate, std::pair<uint32_t*, uint32_t>> taggedFluidNodeIndices;
     std::vector<CollisionTemplate> allocatedBulkFluidNodeTags{CollisionTemplate::Default};
-    QforBoundaryConditions velocityBC, noSlipBC, slipBC, pressureBC, stressBC;
+    // Parameter.h:112-138
+    QforBoundaryConditions velocityBC, noSlipBC, slipBC, stressBC;
     QforPrecursorBoundaryConditions precursorBC;
+    std::vector<QforDirectionalBoundaryCondition> pressureBCDirectional;
+    WallModelParameters wallModel;                // Parameter.h:152
     double outflowPressureCorrectionFactor = 0.0; // Parameter.h:158
     HostGrid host;                                // for the output (FileWriter)
     vfx_stream stream = nullptr;
@@ -246,9 +284,10 @@ public:
 
 // BoundaryConditionFactory.h:50-53
 using boundaryCondition = std::function<void(LBMSimulationParameter*, QforBoundaryConditions*)>;
-// BoundaryConditionFactory.h:53
-using precursorBoundaryCondition =
-    std::function<void(LBMSimulationParameter*, QforPrecursorBoundaryConditions*, double timeRatio, double velocityRatio)>;
+using boundaryConditionDirectional = std::function<void(LBMSimulationParameter*, QforDirectionalBoundaryCondition*)>;
+using boundaryConditionWithParameter = std::function<void(Parameter*, QforBoundaryConditions*, const int level)>;
+using precursorBoundaryConditionFunc =
+    std::function<void(LBMSimulationParameter*, QforPrecursorBoundaryConditions*, real timeRatio, real velocityRatio)>;
 
 class BoundaryConditionFactory {
 public:
@@ -281,13 +320,16 @@ public:
     void setStressBoundaryCondition(StressBC t) { stress = t; }
     enum class PrecursorBC { PrecursorNonReflectiveCompressible, PrecursorDistributions, NotSpecified };
     void setPrecursorBoundaryCondition(PrecursorBC t) { precursor = t; }
-    precursorBoundaryCondition getPrecursorBoundaryConditionPost() const;
+    precursorBoundaryConditionFunc getPrecursorBoundaryConditionPost() const;
 
     boundaryCondition getVelocityBoundaryConditionPost() const;
     boundaryCondition getNoSlipBoundaryConditionPost() const;
     boundaryCondition getSlipBoundaryConditionPost() const;
-    boundaryCondition getPressureBoundaryConditionPre() const;
-    boundaryCondition getStressBoundaryConditionPost() const;
+    // BoundaryConditionFactory.cpp:151-170: every pressure kind of this
+    // stepper is directional (QforDirectionalBoundaryCondition)
+    std::variant<boundaryCondition, boundaryConditionDirectional> getPressureBoundaryConditionPre() const;
+    bool hasDirectionalPressureBoundaryCondition() const;
+    boundaryConditionWithParameter getStressBoundaryConditionPost() const;
 
 private:
     VelocityBC velocity = VelocityBC::NotSpecified;
@@ -336,8 +378,10 @@ private:
     void refinement(int level);
     std::shared_ptr<Parameter> para;
     std::vector<std::shared_ptr<Kernel>> kernels;
-    boundaryCondition velocityBC, noSlipBC, slipBC, pressureBC, stressBC;
-    precursorBoundaryCondition precursorBC;
+    boundaryCondition velocityBC, noSlipBC, slipBC;
+    boundaryConditionDirectional directionalPressureBC;
+    boundaryConditionWithParameter stressBC;
+    precursorBoundaryConditionFunc precursorBC;
     gridScaling scalingFineToCoarse, scalingCoarseToFine;
 };
 
@@ -384,9 +428,9 @@ protected:
     std::string getFileExtension() const override { return ".bin"; }
 };
 
-using RestartObject = RestartObjectT<float>;
-using ASCIIRestartObject = ASCIIRestartObjectT<float>;
-using BinaryRestartObject = BinaryRestartObjectT<float>;
+using RestartObject = RestartObjectT<real>;
+using ASCIIRestartObject = ASCIIRestartObjectT<real>;
+using BinaryRestartObject = BinaryRestartObjectT<real>;
 
 // ---------------------------------------------------------------- output ----
 // FilePartCalculator (src/gpu/core/Output/FilePartCalculator.cpp:36-58)
@@ -451,4 +495,4 @@ private:
     std::map<uint32_t, std::vector<std::string>> fileNamesForCollectionFileTimeSeries;
 };
 
-} // namespace vfx::host
+VFX_HOST_END

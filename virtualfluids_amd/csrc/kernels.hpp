@@ -768,17 +768,20 @@ __global__ void __launch_bounds__(256) k_stats(Lat L, int direct, double* sums)
 }
 
 // -------------------------------------------------- boundary conditions ----
+// Every floating-point table of a boundary condition is of the lattice's
+// arithmetic type T (double for FP64 storage, float otherwise), the `real` of
+// VirtualFluids' QforBoundaryConditions; bc_at<T> reads element i.
 struct BCDev {
     uint32_t n;
     const uint32_t* k;
-    const float* q;
-    const double* vx;
-    const double* vy;
-    const double* vz;
+    const void* q;         // q27[0]: 27 * n subgrid distances, SoA
+    const void* vx;        // wall velocity per boundary node (NULL = 0)
+    const void* vy;
+    const void* vz;
     int32_t cpu_float;
     int32_t direct;
     double omega;          // interpolated kinds
-    const double* rho;     // pressure: density deviation per boundary node
+    const void* rho;       // pressure: density deviation per boundary node
     const uint32_t* kn;    // pressure / outflow: inward neighbour of each boundary node
     int32_t normal;        // pressure / outflow: outward direction of the boundary (d000..dMMM)
     double corr;           // outflow with pressure correction: density correction factor
@@ -799,7 +802,14 @@ struct BCDev {
     const void* pcur;
     double tratio;
     double vratio;                // velocity ratio (velocity kind)
+    double pu[3];                 // velocity kind: scalar velocity added to vx/vy/vz
 };
+
+template <typename T> __device__ __forceinline__ T bc_at(const void* a, uint64_t i) { return ((const T*)a)[i]; }
+template <typename T> __device__ __forceinline__ T bc_q(const BCDev& B, int d, uint32_t i)
+{
+    return bc_at<T>(B.q, (uint64_t)d * B.n + i);
+}
 
 // The plane value at boundary node i: the four precursor nodes around it
 // weighted by inverse distance, or the 0PP node alone when its weight is
@@ -811,6 +821,7 @@ template <typename T> struct PlaneStencil {
     T w[4];
     T inv;
     bool one;
+    bool ok; // every precursor node the stencil reads lies inside the planes
 };
 
 template <typename T> __device__ __forceinline__ PlaneStencil<T> plane_stencil(const BCDev& B, uint32_t i)
@@ -820,10 +831,12 @@ template <typename T> __device__ __forceinline__ PlaneStencil<T> plane_stencil(c
     s.w[0] = ((const T*)B.pw[0])[i];
     s.one = !(s.w[0] < T(1e6));
     s.inv = T(0);
+    s.ok = s.n[0] < B.n_pre;
     if (!s.one) {
         for (int c = 1; c < 4; ++c) {
             s.n[c] = B.pn[c][i];
             s.w[c] = ((const T*)B.pw[c])[i];
+            s.ok = s.ok && s.n[c] < B.n_pre;
         }
         s.inv = T(1) / (s.w[0] + s.w[1] + s.w[2] + s.w[3]);
     }
@@ -909,11 +922,12 @@ __global__ void __launch_bounds__(256) k_bc_velocity_bb(Lat L, BCDev B)
     if (k >= L.N) return; // malformed list entry: never touch memory outside the lattice
     if (B.direct) nbrs_direct(L, k, kk);
     else nbrs_sparse(L, k, kk);
-    const double ux = B.vx ? B.vx[i] : 0.0, uy = B.vy ? B.vy[i] : 0.0, uz = B.vz ? B.vz[i] : 0.0;
+    const double ux = B.vx ? (double)bc_at<T>(B.vx, i) : 0.0, uy = B.vy ? (double)bc_at<T>(B.vy, i) : 0.0,
+                 uz = B.vz ? (double)bc_at<T>(B.vz, i) : 0.0;
 #pragma unroll
     for (int d = 1; d < 27; ++d) {
-        const float q = B.q[(uint64_t)d * B.n + i];
-        if (!(q >= 0.f && q <= 1.f)) continue;
+        const T q = bc_q<T>(B, d, i);
+        if (!(q >= T(0) && q <= T(1))) continue;
         const int e = inv(d);
         // post-collision population of direction d sits where e was read from
         const T fpost = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)], L.sh[d]);
@@ -976,17 +990,21 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
     // the 26 links ran as 26 round trips
     T qs[27];
 #pragma unroll
-    for (int d = 1; d < 27; ++d) qs[d] = (T)B.q[(uint64_t)d * B.n + i];
+    for (int d = 1; d < 27; ++d) qs[d] = bc_q<T>(B, d, i);
     T VX = 0, VY = 0, VZ = 0;
     if (KIND == VFX_BC_VELOCITY_INTERPOLATED || KIND == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED ||
         KIND == VFX_BC_PRECURSOR_VELOCITY) {
-        VX = B.vx ? (T)B.vx[i] : T(0);
-        VY = B.vy ? (T)B.vy[i] : T(0);
-        VZ = B.vz ? (T)B.vz[i] : T(0);
+        VX = B.vx ? bc_at<T>(B.vx, i) : T(0);
+        VY = B.vy ? bc_at<T>(B.vy, i) : T(0);
+        VZ = B.vz ? bc_at<T>(B.vz, i) : T(0);
     }
     if constexpr (KIND == VFX_BC_PRECURSOR_VELOCITY) {
         // PrecursorNonReflectiveCompressible.cu:112-114: (velocityX + (1 - tr) last + tr current) / velocityRatio
         const PlaneStencil<T> s = plane_stencil<T>(B, i);
+        if (!s.ok) return; // malformed plane index: never read outside the planes
+        VX += (T)B.pu[0];
+        VY += (T)B.pu[1];
+        VZ += (T)B.pu[2];
         const T tr = (T)B.tratio, vr = (T)B.vratio;
         const T* last = (const T*)B.plast;
         const T* cur = (const T*)B.pcur;
@@ -1069,6 +1087,7 @@ __global__ void __launch_bounds__(256) k_bc_precursor_dist(Lat L, BCDev B)
     if (B.direct) nbrs_direct(L, k, kk);
     else nbrs_sparse(L, k, kk);
     const PlaneStencil<T> s = plane_stencil<T>(B, i);
+    if (!s.ok) return; // malformed plane index: never read outside the planes
     constexpr int dirs[9] = {dP00, dPP0, dPM0, dP0P, dP0M, dPPP, dPMP, dPPM, dPMM};
 #pragma unroll
     for (int c = 0; c < 9; ++c) {
@@ -1133,7 +1152,7 @@ __global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
     const T cusq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3);
 #pragma unroll
     for (int d = 0; d < 27; ++d) f[d] -= feq_shifted<T>(d, drho1, vx1, vx2, vx3, cusq);
-    drho1 = (T)B.rho[i];
+    drho1 = bc_at<T>(B.rho, i);
 #pragma unroll
     for (int d = 0; d < 27; ++d) f[d] += feq_shifted<T>(d, drho1, vx1, vx2, vx3, cusq);
     const int nx = cx(B.normal), ny = cy(B.normal), nz = cz(B.normal);
@@ -1259,7 +1278,7 @@ __global__ void __launch_bounds__(256) k_bc_stress(Lat L, BCDev B)
     }
     T qs[27];
 #pragma unroll
-    for (int d = 1; d < 27; ++d) qs[d] = (T)B.q[(uint64_t)d * B.n + i];
+    for (int d = 1; d < 27; ++d) qs[d] = bc_q<T>(B, d, i);
     T fin[27] = {};
     T wmx = T(0), wmy = T(0), wmz = T(0);
 #pragma unroll
@@ -1440,6 +1459,7 @@ struct BatchDev {
     const void* domains;        // DomainDev<T>[n]
     const uint2* tiles;         // (domain, first interior node) per tile
     uint32_t ntiles;
+    uint32_t xcd_run;           // tiles per XCD run (xcd_block); 1 = plain order
     float ld[4], st[4];
 };
 
@@ -1451,9 +1471,10 @@ __global__ void __launch_bounds__(256)
     k_step_batched(BatchDev B, const uint2* __restrict__ tiles, const DomainDev<typename Compute<S>::T>* __restrict__ doms)
 {
     using T = typename Compute<S>::T;
-    // tiles in plain order: the ensemble's domains are small and their tiles
-    // start off line boundaries anyway; runs per XCD measured slower here
-    for (uint32_t t = blockIdx.x; t < B.ntiles; t += gridDim.x) {
+    // workgroup b starts at tile xcd_block(b): runs of xcd_run consecutive
+    // tiles (which share cache lines at their ends) on one XCD, or plain order
+    const uint32_t t0 = B.xcd_run > 1 ? xcd_block(blockIdx.x, gridDim.x, B.xcd_run) : blockIdx.x;
+    for (uint32_t t = t0; t < B.ntiles; t += gridDim.x) {
         const uint2 tile = tiles[t];
         const DomainDev<T>& D = doms[tile.x];
         const uint32_t idx = tile.y + threadIdx.x;

@@ -41,9 +41,9 @@ namespace vfx {
 struct ICellsDev {
     const uint32_t* coarse; // coarse-to-fine: coarse corner-MMM node; fine-to-coarse: destination coarse node
     const uint32_t* fine;   // fine corner-MMM node of the cell
-    const double* ox;       // offsets of the cells (may be null: no offset)
-    const double* oy;
-    const double* oz;
+    const void* ox;         // offsets of the cells, arithmetic type T (may be null: no offset)
+    const void* oy;
+    const void* oz;
     uint32_t n;
     int32_t direct_c, direct_f;
 };
@@ -281,9 +281,9 @@ __global__ void __launch_bounds__(256) k_scale_cf(Lat C, Lat F, ICellsDev cells,
     __syncthreads();
     if (!active) return;
     Coeffs<T> k;
-    const T xo = cells.ox ? (T)cells.ox[cell] : T(0);
-    const T yo = cells.oy ? (T)cells.oy[cell] : T(0);
-    const T zo = cells.oz ? (T)cells.oz[cell] : T(0);
+    const T xo = cells.ox ? ((const T*)cells.ox)[cell] : T(0);
+    const T yo = cells.oy ? ((const T*)cells.oy)[cell] : T(0);
+    const T zo = cells.oz ? ((const T*)cells.oz)[cell] : T(0);
     cell_coefficients(src, xo, yo, zo, k);
     const T x = (lane & 1) ? T(0.25) : T(-0.25);
     const T y = (lane & 2) ? T(0.25) : T(-0.25);
@@ -321,9 +321,9 @@ __global__ void __launch_bounds__(256) k_scale_fc(Lat C, Lat F, ICellsDev cells,
     __syncthreads();
     if (!active) return;
     Coeffs<T> k;
-    const T xo = cells.ox ? (T)cells.ox[cell] : T(0);
-    const T yo = cells.oy ? (T)cells.oy[cell] : T(0);
-    const T zo = cells.oz ? (T)cells.oz[cell] : T(0);
+    const T xo = cells.ox ? ((const T*)cells.ox)[cell] : T(0);
+    const T yo = cells.oy ? ((const T*)cells.oy)[cell] : T(0);
+    const T zo = cells.oz ? ((const T*)cells.oz)[cell] : T(0);
     cell_coefficients(src, xo, yo, zo, k);
     const T drho = k.d[P000] - T(2) * k.laplace_rho * T(0.125);
     const T gx[3] = {k.a[P100], k.a[P010], k.a[P001]};

@@ -254,7 +254,7 @@ class Lattice:
         if bc.precursor is not None and bc.precursor.real != self._macro_dtype():
             raise ValueError("precursor weights and planes must have the lattice's arithmetic type")
         s = self.struct()
-        b = bc.struct()
+        b = bc.struct(self._macro_dtype())
         V.check(V.lib().vfx_bc_apply(C.byref(s), C.byref(b), int(bool(even)), _stream(stream)), "vfx_bc_apply")
 
     def read_populations(self, even=True, stream=None) -> torch.Tensor:
@@ -456,18 +456,21 @@ class BoundaryCondition:
             self.kn = torch.from_numpy(kn.view(np.int32)).to(dev)
         elif self.kind not in self.STRESS:
             self.kn = None
-        q = np.ascontiguousarray(q27, dtype=np.float32)
+        q = np.ascontiguousarray(q27, dtype=np.float64)
         if q.shape != (27, n):
             raise ValueError("q27 must be (27, n)")
-        self.q = torch.from_numpy(q).to(dev)
 
         def arr(v):
             if v is None:
                 return None
-            return torch.from_numpy(np.array(np.broadcast_to(v, (n,)), dtype=np.float64)).to(dev)
+            return np.array(np.broadcast_to(v, (n,)), dtype=np.float64)
 
-        self.vx, self.vy, self.vz = arr(vx), arr(vy), arr(vz)
-        self.rho = arr(rho)
+        # q27, Vx/Vy/Vz and RhoBC as given (float64); the device copies take
+        # the lattice's `real` (float64 for FP64 storage, float32 otherwise),
+        # made once per type and handed to the C ABI as they are
+        self._host = {"q": q, "vx": arr(vx), "vy": arr(vy), "vz": arr(vz), "rho": arr(rho)}
+        self._dev = {}
+        self.device = dev
         self.normal = int(normal)
         self.pressure_correction_factor = float(pressure_correction_factor)
         self.cpu_float_velocity = bool(cpu_float_velocity)
@@ -476,15 +479,26 @@ class BoundaryCondition:
     def pre_collision(self) -> bool:
         return self.kind in self.PRE_COLLISION
 
-    def struct(self) -> V.VfxBC:
+    def tables(self, real=torch.float64) -> dict:
+        """Device q27 (27, n), vx, vy, vz, rho (n) in the arithmetic type `real`."""
+        t = self._dev.get(real)
+        if t is None:
+            t = {k: None if v is None else torch.from_numpy(v).to(self.device, real).contiguous()
+                 for k, v in self._host.items()}
+            self._dev[real] = t
+        return t
+
+    def struct(self, real=torch.float64) -> V.VfxBC:
+        """The vfx_bc of this boundary for a lattice whose arithmetic type is `real`."""
+        t = self.tables(real)
         b = V.VfxBC()
         b.kind = self.kind
         b.n = self.n
         b.k = self.k.data_ptr() if self.n else None
-        b.q27 = self.q.data_ptr() if self.n else None
-        b.vx, b.vy, b.vz = _ptr(self.vx), _ptr(self.vy), _ptr(self.vz)
+        b.q27 = t["q"].data_ptr() if self.n else None
+        b.vx, b.vy, b.vz = _ptr(t["vx"]), _ptr(t["vy"]), _ptr(t["vz"])
         b.cpu_float_velocity = int(self.cpu_float_velocity)
-        b.rho = _ptr(self.rho)
+        b.rho = _ptr(t["rho"])
         b.k_neighbor = _ptr(self.kn) if self.n else None
         b.normal = self.normal
         b.pressure_correction_factor = self.pressure_correction_factor
@@ -559,15 +573,30 @@ class PrecursorPlanes:
         a = np.asarray(a, np.float64).reshape(self.components, self.n_precursor_nodes)
         return torch.from_numpy(a.copy()).to(self.device, self.real).contiguous()
 
-    def advance(self, t_level):
-        """The cycling of runPrecursorBCKernelPost at level time step t_level."""
+    def _load(self, dst, a, stream=None):
+        """Copy a plane read into the existing device array dst, on the stream
+        the boundary kernels run on (torch's current stream when None), so the
+        copy waits for the kernels that still read dst's previous contents."""
+        src = torch.from_numpy(np.ascontiguousarray(np.asarray(a, np.float64).reshape(
+            self.components, self.n_precursor_nodes))).to(self.real)
+        if stream is None:
+            dst.copy_(src)
+        else:
+            s = stream if isinstance(stream, torch.cuda.Stream) else torch.cuda.ExternalStream(stream)
+            with torch.cuda.stream(s):
+                dst.copy_(src)
+
+    def advance(self, t_level, stream=None):
+        """The cycling of runPrecursorBCKernelPost at level time step t_level.
+        The three plane arrays are allocated once and rotate; a new read goes
+        into the array that held `last`."""
         T = self.steps_between_reads
         last_time, current_time = (self.n_reads - 2) * T, (self.n_reads - 1) * T
         next_time = self.n_reads * T
         if t_level >= current_time:
             last_time, current_time, next_time = current_time, next_time, next_time + T
             self.last, self.current, self.next = self.current, self.next, self.last
-            self.next = self.plane(self.reader(next_time * 2.0 ** -self.level * self.physical_time_ratio))
+            self._load(self.next, self.reader(next_time * 2.0 ** -self.level * self.physical_time_ratio), stream)
             self.n_reads += 1
         self.time_ratio = (t_level - last_time) / T
         return self.time_ratio
@@ -600,7 +629,7 @@ class Stepper:
                                         apply_body_force=self.apply_body_force)
             for bc in post:
                 if bc.precursor is not None and bc.precursor.reader is not None:
-                    bc.precursor.advance(self.t)
+                    bc.precursor.advance(self.t, stream)
                 self.lattice.apply_bc(bc, self.even, stream=stream)
             self.t += 1
             self.even = not self.even

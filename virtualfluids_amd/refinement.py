@@ -186,17 +186,29 @@ class ICells:
                                      ).to(device)
         self.max_coarse = int(np.max(coarse)) if self.n else -1
         self.max_fine = int(np.max(fine)) if self.n else -1
-        self.offsets = None
-        if offsets is not None:
-            self.offsets = [torch.as_tensor(np.ascontiguousarray(o, dtype=np.float64)).to(device) for o in offsets]
+        # ICellNeigh x/y/z as given (float64); device copies in the lattices'
+        # `real` (float64 for FP64 storage, float32 otherwise), made once per type
+        self.device = device
+        self._offsets_host = None if offsets is None else [np.ascontiguousarray(o, dtype=np.float64) for o in offsets]
+        self._offsets = {}
 
-    def struct(self) -> V.VfxICells:
+    def offsets(self, real=torch.float64):
+        if self._offsets_host is None:
+            return None
+        t = self._offsets.get(real)
+        if t is None:
+            t = [torch.from_numpy(o).to(self.device, real).contiguous() for o in self._offsets_host]
+            self._offsets[real] = t
+        return t
+
+    def struct(self, real=torch.float64) -> V.VfxICells:
         s = V.VfxICells()
         s.fine_cells = self.fine.data_ptr()
         s.coarse_cells = self.coarse.data_ptr()
         s.n_cells = self.n
-        if self.offsets is not None:
-            s.offset_x, s.offset_y, s.offset_z = (o.data_ptr() for o in self.offsets)
+        o = self.offsets(real)
+        if o is not None:
+            s.offset_x, s.offset_y, s.offset_z = (a.data_ptr() for a in o)
         return s
 
 
@@ -207,14 +219,14 @@ def _check_cells(coarse: Lattice, fine: Lattice, cells: ICells):
 
 def scale_cf(coarse: Lattice, fine: Lattice, cells: ICells, coarse_even: bool, fine_even: bool = True, stream=None):
     _check_cells(coarse, fine, cells)
-    c, f, i = coarse.struct(), fine.struct(), cells.struct()
+    c, f, i = coarse.struct(), fine.struct(), cells.struct(coarse._macro_dtype())
     V.check(V.lib().vfx_scale_cf(C.byref(c), C.byref(f), C.byref(i), int(bool(coarse_even)), int(bool(fine_even)),
                                  _stream(stream)), "vfx_scale_cf")
 
 
 def scale_fc(coarse: Lattice, fine: Lattice, cells: ICells, coarse_even: bool, fine_even: bool = True, stream=None):
     _check_cells(coarse, fine, cells)
-    c, f, i = coarse.struct(), fine.struct(), cells.struct()
+    c, f, i = coarse.struct(), fine.struct(), cells.struct(coarse._macro_dtype())
     V.check(V.lib().vfx_scale_fc(C.byref(c), C.byref(f), C.byref(i), int(bool(coarse_even)), int(bool(fine_even)),
                                  _stream(stream)), "vfx_scale_fc")
 
