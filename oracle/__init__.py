@@ -28,13 +28,14 @@ _u8p = C.POINTER(C.c_uint8)
 
 class OrcBC(C.Structure):
     _fields_ = [("kind", C.c_int32), ("n", C.c_int32), ("node", C.POINTER(C.c_int64)),
-                ("q", C.POINTER(C.c_float)), ("ux", _dp), ("uy", _dp), ("uz", _dp),
+                ("q", _dp), ("ux", _dp), ("uy", _dp), ("uz", _dp),
                 ("cpu_float_velocity", C.c_int32), ("rho", _dp), ("nbr", C.POINTER(C.c_int64)),
                 ("normal", C.c_int32), ("correction", C.c_double),
                 ("wm_normal", _dp * 3), ("wm_el", _dp * 3), ("wm_wall", _dp * 3), ("wm_offset", C.POINTER(C.c_int32)),
                 ("wm_z0", _dp), ("wm_ustar", _dp), ("wm_force", _dp * 3),
                 ("n_pre", C.c_int32), ("pn", C.POINTER(C.c_int64) * 4), ("pw", _dp * 4), ("plast", _dp),
-                ("pcur", _dp), ("tratio", C.c_double), ("vratio", C.c_double)]
+                ("pcur", _dp), ("tratio", C.c_double), ("vratio", C.c_double),
+                ("pseries", _dp), ("n_series", C.c_int32), ("period", C.c_int32), ("t0", C.c_int32)]
 
 
 def build(ref: bool | None = None) -> None:
@@ -75,12 +76,13 @@ def lib():
                                          C.c_int, C.POINTER(OrcBC), C.c_int]
         L.orc_run_body_force.restype = C.c_int
         _i32p = C.POINTER(C.c_int32)
-        L.orc_interp_bc.argtypes = [C.c_int, _dp, C.POINTER(C.c_float), C.c_double, C.c_double, C.c_double,
+        L.orc_pressure_bc.argtypes = [_dp, C.c_double, C.c_int, _dp, C.POINTER(C.c_int32)]
+        L.orc_interp_bc.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_double,
                                     C.c_double, _dp, _i32p]
         L.orc_outflow_bc.argtypes = [C.c_int, _dp, _dp, C.c_int, C.c_double, _dp, _i32p]
         _i64p = C.POINTER(C.c_int64)
         L.orc_precursor_blend.argtypes = [C.c_int, C.c_int, C.c_int] + [_i64p] * 4 + [_dp] * 6 + [C.c_double, _dp]
-        L.orc_stress_bc.argtypes = [C.c_int, _dp, C.POINTER(C.c_float), C.c_double, _dp, _dp, C.c_int32, C.c_double,
+        L.orc_stress_bc.argtypes = [C.c_int, _dp, _dp, C.c_double, _dp, _dp, C.c_int32, C.c_double,
                                     _dp, _dp, _dp, _i32p, _dp, _dp]
         L.orc_macroscopic_quantities.argtypes = [_dp, _dp]
         L.orc_k17_collide_turb.argtypes = [_dp, C.c_double, _dp, _dp, _dp, C.c_int, C.c_double, _dp]
@@ -120,7 +122,9 @@ def ref():
         L.ref_run_periodic.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]
         L.ref_run_cavity.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                      C.c_double, C.c_int]
-        L.ref_time_blocks.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(C.c_int)]
+        L.ref_time_blocks.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.ref_run_blocks.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]
+        L.ref_run_periodic_b92.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int]
         L.ref_time_blocks.restype = C.c_double
         L.ref_interp_cell.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
                                       _dp, _dp]
@@ -196,11 +200,21 @@ def interp_bc(kind, p27, q27, omega, u=(0.0, 0.0, 0.0)):
     populations p27, link fractions q27 -> (values, written mask) indexed by the
     direction written (the opposite of the wall link)."""
     p = np.ascontiguousarray(p27, dtype=np.float64)
-    q = np.ascontiguousarray(q27, dtype=np.float32)
+    q = np.ascontiguousarray(q27, dtype=np.float64)
     out = np.zeros(27)
     wr = np.zeros(27, np.int32)
-    lib().orc_interp_bc(int(kind), _p(p), q.ctypes.data_as(C.POINTER(C.c_float)), float(omega),
+    lib().orc_interp_bc(int(kind), _p(p), _p(q), float(omega),
                         *(float(x) for x in u), _p(out), wr.ctypes.data_as(C.POINTER(C.c_int32)))
+    return out, wr.astype(bool)
+
+
+def pressure_bc(fn27, rho_bc, normal):
+    """One node of the non-equilibrium pressure boundary (kind 4) from its
+    inward neighbour's populations -> (values, written mask)."""
+    f = np.ascontiguousarray(fn27, dtype=np.float64)
+    out = np.zeros(27)
+    wr = np.zeros(27, np.int32)
+    lib().orc_pressure_bc(_p(f), float(rho_bc), int(normal), _p(out), wr.ctypes.data_as(C.POINTER(C.c_int32)))
     return out, wr.astype(bool)
 
 
@@ -210,7 +224,7 @@ def stress_bc(kind, p27, q27, omega, sample_velocity, normal, offset, z0, exchan
     the kernel reads at the sampling node (its z from the y array).  Returns
     (values, written, new exchange velocity, new wall velocity, u_star, force)."""
     p = np.ascontiguousarray(p27, dtype=np.float64)
-    q = np.ascontiguousarray(q27, dtype=np.float32)
+    q = np.ascontiguousarray(q27, dtype=np.float64)
     vs = np.ascontiguousarray(sample_velocity, dtype=np.float64)
     nrm = np.ascontiguousarray(normal, dtype=np.float64)
     el = np.array(exchange_velocity, dtype=np.float64)
@@ -219,7 +233,7 @@ def stress_bc(kind, p27, q27, omega, sample_velocity, normal, offset, z0, exchan
     wr = np.zeros(27, np.int32)
     us = np.zeros(1)
     fo = np.zeros(3)
-    lib().orc_stress_bc(int(kind), _p(p), q.ctypes.data_as(C.POINTER(C.c_float)), float(omega), _p(vs), _p(nrm),
+    lib().orc_stress_bc(int(kind), _p(p), _p(q), float(omega), _p(vs), _p(nrm),
                         int(offset), float(z0), _p(el), _p(wm), _p(out), wr.ctypes.data_as(C.POINTER(C.c_int32)),
                         _p(us), _p(fo))
     return out, wr.astype(bool), el, wm, float(us[0]), fo
@@ -310,14 +324,18 @@ class BCGroup:
     monitors u_star, force are updated in place), 12 precursor velocity inlet
     (q links, ux/uy/uz added to the plane velocity), 13 precursor
     distributions (no q): precursor = dict(neighbors (4, n), weights (4, n),
-    last, current (components x n_pre), time_ratio, velocity_ratio)."""
+    last, current (components x n_pre), time_ratio, velocity_ratio), or
+    instead of last/current/time_ratio a read cycle: series = planes at times
+    0, period, 2 period, ..., period, and t0 = the time of the first step
+    (each step at time t reads plane(t // period), the next one and
+    (t - m period) / period)."""
 
     def __init__(self, node, q=None, ux=None, uy=None, uz=None, cpu_float_velocity=False, kind=0, rho=None,
                  nbr=None, normal=0, correction=0.0, wall_normal=None, sampling_offset=None, z0=None,
                  exchange_velocity=None, wall_velocity=None, precursor=None):
         self.node = np.ascontiguousarray(node, dtype=np.int64)
         n = self.node.size
-        self.q = np.ascontiguousarray(np.full((27, n), -1.0) if q is None else q, dtype=np.float32)
+        self.q = np.ascontiguousarray(np.full((27, n), -1.0) if q is None else q, dtype=np.float64)
         assert self.q.shape == (27, n)
         z = np.zeros(n)
         self.ux = np.ascontiguousarray(z if ux is None else np.broadcast_to(ux, (n,)), dtype=np.float64)
@@ -331,11 +349,15 @@ class BCGroup:
         self.pre = None
         if kind in (12, 13):
             pc = precursor
+            series = pc.get("series")
+            if series is not None:
+                series = np.ascontiguousarray(np.stack(series), dtype=np.float64)
             self.pre = {"nb": np.ascontiguousarray(pc["neighbors"], dtype=np.int64).reshape(4, n),
                         "w": np.ascontiguousarray(pc["weights"], dtype=np.float64).reshape(4, n),
-                        "last": np.ascontiguousarray(pc["last"], dtype=np.float64),
-                        "cur": np.ascontiguousarray(pc["current"], dtype=np.float64),
-                        "tr": float(pc.get("time_ratio", 0.0)), "vr": float(pc.get("velocity_ratio", 1.0))}
+                        "last": np.ascontiguousarray(pc["last"] if series is None else series[0], dtype=np.float64),
+                        "cur": np.ascontiguousarray(pc["current"] if series is None else series[1], dtype=np.float64),
+                        "tr": float(pc.get("time_ratio", 0.0)), "vr": float(pc.get("velocity_ratio", 1.0)),
+                        "series": series, "period": int(pc.get("period", 1)), "t0": int(pc.get("t0", 1))}
             self.pre["n_pre"] = self.pre["last"].shape[-1]
         if self.stress:
             self.wall_normal = np.ascontiguousarray(np.broadcast_to(np.asarray(wall_normal, np.float64).reshape(3, -1),
@@ -353,16 +375,18 @@ class BCGroup:
 
     def c(self):
         return OrcBC(self.kind, self.node.size, self.node.ctypes.data_as(C.POINTER(C.c_int64)),
-                     self.q.ctypes.data_as(C.POINTER(C.c_float)), _p(self.ux), _p(self.uy), _p(self.uz),
+                     _p(self.q), _p(self.ux), _p(self.uy), _p(self.uz),
                      int(self.cpu_float_velocity), _p(self.rho), self.nbr.ctypes.data_as(C.POINTER(C.c_int64)),
                      self.normal, self.correction, *self._wm(), *self._pre())
 
     def _pre(self):
         if self.pre is None:
-            return 0, (C.POINTER(C.c_int64) * 4)(), (_dp * 4)(), None, None, 0.0, 1.0
+            return 0, (C.POINTER(C.c_int64) * 4)(), (_dp * 4)(), None, None, 0.0, 1.0, None, 0, 1, 0
         p = self.pre
+        ser = p["series"]
         return (p["n_pre"], (C.POINTER(C.c_int64) * 4)(*(p["nb"][j].ctypes.data_as(C.POINTER(C.c_int64)) for j in range(4))),
-                (_dp * 4)(*(_p(p["w"][j]) for j in range(4))), _p(p["last"]), _p(p["cur"]), p["tr"], p["vr"])
+                (_dp * 4)(*(_p(p["w"][j]) for j in range(4))), _p(p["last"]), _p(p["cur"]), p["tr"], p["vr"],
+                None if ser is None else _p(ser), 0 if ser is None else ser.shape[0], p["period"], p["t0"])
 
     def _wm(self):
         z3 = (_dp * 3)(None, None, None)
@@ -392,6 +416,8 @@ def run(F, dims, nsteps, omega, collision="k17", solid=None, bcs=(), lim=LIMITER
     rc = lib().orc_run(_p(F), nx, ny, nz, sol.ctypes.data_as(_u8p) if sol is not None else None,
                        0 if collision == "k17" else 1, float(omega), _p(lim), _p(fo), len(bcl), arr,
                        int(nsteps), _p(macro) if macro is not None else None)
+    if rc == -2:
+        raise RuntimeError("orc_run failed: a precursor plane series ends before the last step")
     if rc != 0:
         raise RuntimeError(f"orc_run failed ({rc}): a fluid->solid link has no boundary condition")
     return (F, macro) if want_macro else F
@@ -462,6 +488,15 @@ def ref_run_periodic(F, dims, nsteps, omega):
     return F
 
 
+def ref_run_periodic_b92(F, dims, nsteps, omega):
+    """The reference's incompressible BGK CPU kernel (B92IncompressibleNavierStokes)
+    on a periodic block."""
+    nx, ny, nz = dims
+    F = np.ascontiguousarray(F, dtype=np.float64).copy()
+    ref().ref_run_periodic_b92(_p(F), nx, ny, nz, float(omega), int(nsteps))
+    return F
+
+
 def ref_run_cavity(F, dims, nsteps, omega, lid=(0.05, 0.0, 0.0)):
     nx, ny, nz = dims
     F = np.ascontiguousarray(F, dtype=np.float64).copy()
@@ -470,10 +505,21 @@ def ref_run_cavity(F, dims, nsteps, omega, lid=(0.05, 0.0, 0.0)):
     return F
 
 
-def ref_time_blocks(nblocks, bs, omega, nsteps):
+def ref_time_blocks(nblocks, bs, omega, nsteps, exchange=True):
+    """Wall seconds of nsteps reference K17 steps on nblocks bs^3 blocks with
+    (or without) the block exchange; returns (seconds, OpenMP threads, block grid)."""
     t = C.c_int(0)
-    secs = ref().ref_time_blocks(int(nblocks), int(bs), float(omega), int(nsteps), C.byref(t))
-    return secs, t.value
+    g = (C.c_int * 3)()
+    secs = ref().ref_time_blocks(int(nblocks), int(bs), float(omega), int(nsteps), int(bool(exchange)), C.byref(t), g)
+    return secs, t.value, tuple(g)
+
+
+def ref_run_blocks(F, grid, bs, nsteps, omega):
+    """The CPU baseline's block grid (grid = blocks per axis, bs^3 nodes each,
+    periodic) on one node-major field of grid*bs nodes per axis."""
+    F = np.ascontiguousarray(F, dtype=np.float64).copy()
+    ref().ref_run_blocks(_p(F), int(grid[0]), int(grid[1]), int(grid[2]), int(bs), float(omega), int(nsteps))
+    return F
 
 
 # ---------------------------------------------------------------- refinement

@@ -560,7 +560,7 @@ double proj(int d, double ux, double uy, double uz)
 // VelocityInterpolatedCompressible.cu, SlipCompressible.cu; formulas
 // KernelUtilities.h:171-197).  out[d'] receives the population that re-enters
 // the node along d' = inv(d) for every wall link d; returns a mask of written d'.
-void interp_bc(int kind, const double* p, const float* q, int n, int i, double omega, double VX, double VY, double VZ,
+void interp_bc(int kind, const double* p, const double* q, int n, int i, double omega, double VX, double VY, double VZ,
                double* out, bool* written)
 {
     const double drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] +
@@ -924,7 +924,7 @@ typedef struct {
     int32_t kind;             // ORC_BC_*
     int32_t n;                // number of boundary nodes
     const int64_t* node;      // dense node index of each boundary node
-    const float* q;           // [27*n] structure of arrays; link d of node i is a wall link iff 0 <= q[d*n+i] <= 1
+    const double* q;           // [27*n] structure of arrays; link d of node i is a wall link iff 0 <= q[d*n+i] <= 1
     const double* ux;         // [n] wall velocity (NULL = 0)
     const double* uy;
     const double* uz;
@@ -951,6 +951,15 @@ typedef struct {
     const double* pcur;
     double tratio;
     double vratio;
+    // optional read cycle in closed form: n_series planes (components x n_pre
+    // each), plane m at time m * period; at the step of time t = t0 + s the
+    // boundary reads last = plane(t / period), current = the next one, with
+    // ratio (t - m period) / period (the cycle of runPrecursorBCKernelPost,
+    // BoundaryConditionKernelManager.cpp:346-382).  NULL: plast/pcur/tratio fixed.
+    const double* pseries;
+    int32_t n_series;
+    int32_t period;
+    int32_t t0;
 } orc_bc;
 
 static bool stress_kind(int kind)
@@ -1031,7 +1040,7 @@ void stress_bc(const orc_bc& bc, int i, const double* p, double omega, const dou
                         (-(p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + ((p[dP0P] - p[dM0M]) - (p[dP0M] - p[dM0P])) +
                         (p[d00P] - p[d00M])) / (1.0 + drho);
     const double cu_sq = c3o2 * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3) * (1.0 + drho);
-    const float* q = bc.q;
+    const double* q = bc.q;
     auto qd = [&](int d) { return (double)q[(int64_t)d * bc.n + i]; };
     double fin[27] = {0};
     double wm[3] = {0, 0, 0};
@@ -1132,11 +1141,19 @@ void orc_k17_collide_turb(double* f27, double omega, const double* lim3, const d
 
 // one boundary node: post-collision populations p27 (interpolated kinds) or
 // pre-collision f27 / neighbour fn27 (outflow kinds); out27 / written27 (0/1)
-void orc_interp_bc(int kind, const double* p27, const float* q27, double omega, double vx, double vy, double vz,
+void orc_interp_bc(int kind, const double* p27, const double* q27, double omega, double vx, double vy, double vz,
                    double* out27, int32_t* written27)
 {
     bool wr[27] = {false};
     interp_bc(kind, p27, q27, 1, 0, omega, vx, vy, vz, out27, wr);
+    for (int d = 0; d < 27; ++d) written27[d] = wr[d];
+}
+// non-equilibrium pressure boundary of one node from its inward neighbour's
+// populations fn27 (PressureNonEquilibriumCompressible.cu)
+void orc_pressure_bc(const double* fn27, double rho_bc, int normal, double* out27, int32_t* written27)
+{
+    bool wr[27] = {false};
+    pressure_bc(fn27, rho_bc, normal, out27, wr);
     for (int d = 0; d < 27; ++d) written27[d] = wr[d];
 }
 void orc_outflow_bc(int kind, const double* f27, const double* fn27, int normal, double factor, double* out27,
@@ -1172,7 +1189,7 @@ void orc_macroscopic_quantities(const double* f, double* out5)
 // sampling node's velocity vs3 (its z component as the kernels read it),
 // normal3, offset, z0; el3 / wall3 filtered velocities in and out; out27,
 // written27; ustar1 and force3 monitors (may be NULL)
-void orc_stress_bc(int kind, const double* p27, const float* q27, double omega, const double* vs3, const double* normal3,
+void orc_stress_bc(int kind, const double* p27, const double* q27, double omega, const double* vs3, const double* normal3,
                    int32_t offset, double z0, double* el3, double* wall3, double* out27, int32_t* written27,
                    double* ustar1, double* force3)
 {
@@ -1318,6 +1335,9 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
                     int turb, double sgs, double* nut, const double* gforce, double ff, double* bf)
 {
     const int64_t N = (int64_t)nx * ny * nz;
+    // a private copy of the tables: precursor plane series set plast/pcur/tratio per step
+    std::vector<orc_bc> bcv(bcs, bcs + n_bc);
+    bcs = bcv.data();
     bool stress = false;
     for (int b = 0; b < n_bc; ++b) stress = stress || stress_kind(bcs[b].kind);
     std::vector<double> vel(turb == TURB_AMD || stress ? (size_t)N * 3 : 0);
@@ -1344,8 +1364,8 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
                         if (CX[d] < 0) covered[(size_t)bcs[b].node[i] * 27 + d] = 1;
                         continue;
                     }
-                    const float q = bcs[b].q[(int64_t)d * bcs[b].n + i];
-                    if (q >= 0.0f && q <= 1.0f) covered[(size_t)bcs[b].node[i] * 27 + d] = 1;
+                    const double q = bcs[b].q[(int64_t)d * bcs[b].n + i];
+                    if (q >= 0.0 && q <= 1.0) covered[(size_t)bcs[b].node[i] * 27 + d] = 1;
                 }
     }
     auto neighbour = [&](int64_t i, int d) {
@@ -1364,6 +1384,15 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
     std::vector<std::array<std::vector<double>, 2>> pv(n_bc);
     std::vector<std::array<std::vector<uint8_t>, 2>> pv_wr(n_bc);
     for (int s = 0; s < nsteps; ++s) {
+        for (orc_bc& bc : bcv) {
+            if (!bc.pseries) continue;
+            const int t = bc.t0 + s, m = t / bc.period;
+            if (m + 1 >= bc.n_series) return -2; // the series ends before this step
+            const int64_t psz = (int64_t)(bc.kind == ORC_BC_PRECURSOR_VELOCITY ? 3 : 9) * bc.n_pre;
+            bc.plast = bc.pseries + m * psz;
+            bc.pcur = bc.pseries + (m + 1) * psz;
+            bc.tratio = (double)(t - m * bc.period) / bc.period;
+        }
         // non-reflecting outflow: the populations entering the boundary nodes
         // in the next state come from this pre-collision state (the kernel
         // runs after a parity flip and writes the other parity's slots, which
@@ -1436,7 +1465,7 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
                 const double ux = bc.ux ? bc.ux[i] : 0.0, uy = bc.uy ? bc.uy[i] : 0.0, uz = bc.uz ? bc.uz[i] : 0.0;
                 if (bc.kind == ORC_BC_VELOCITY_BOUNCE_BACK) {
                     for (int d = 1; d < 27; ++d) {
-                        const float q = bc.q[(int64_t)d * bc.n + i];
+                        const double q = bc.q[(int64_t)d * bc.n + i];
                         if (!(q >= 0.0f && q <= 1.0f)) continue;
                         const double fd = post[(size_t)k * 27 + d];
                         G[(size_t)k * 27 + T.inv[d]] = fd - wall_term(d, ux, uy, uz, bc.cpu_float_velocity != 0);
@@ -1462,7 +1491,7 @@ static int run_impl(double* F, int nx, int ny, int nz, const uint8_t* solid, int
                         now_wr.assign((size_t)bc.n * 27, 0);
                     }
                     for (int d = 1; d < 27; ++d) {
-                        const float q = bc.q[(int64_t)d * bc.n + i];
+                        const double q = bc.q[(int64_t)d * bc.n + i];
                         if (!(q >= 0.0f && q <= 1.0f)) continue;
                         const int e = T.inv[d];
                         if (s == 0) G[(size_t)k * 27 + e] = F[27 * neighbour(k, d) + d];

@@ -13,6 +13,7 @@
 // its own neighbour: every EsoSplit slot in the ghost layer receives the value
 // the periodic image node wrote.
 
+#include "B92IncompressibleNavierStokes.h"
 #include "K17CompressibleNavierStokes.h"
 #include "BCArray3D.h"
 #include "BCSet.h"
@@ -29,6 +30,8 @@
 #include <lbm/refinement/InterpolationFC.h>
 
 #include <chrono>
+#include <cmath>
+#include <cstdlib>
 #include <cstdint>
 #include <memory>
 #include <vector>
@@ -71,19 +74,23 @@ void periodic_fill(EsoSplit& eso, int nx, int ny, int nz)
 }
 
 struct Block {
-    std::shared_ptr<K17CompressibleNavierStokes> kernel;
+    std::shared_ptr<LBMKernel> kernel;
     std::shared_ptr<EsoSplit> eso;
     std::vector<std::shared_ptr<BCStrategy>> bcs;
 };
 
-Block make_block(int nx, int ny, int nz, double omega)
+// a block of the reference's K17CompressibleNavierStokes (default) or, with
+// incompressible_bgk, of its B92IncompressibleNavierStokes CPU kernel
+Block make_block(int nx, int ny, int nz, double omega, bool incompressible_bgk = false)
 {
-    auto proto = std::make_shared<K17CompressibleNavierStokes>();
+    std::shared_ptr<LBMKernel> proto;
+    if (incompressible_bgk) proto = std::make_shared<B92IncompressibleNavierStokes>();
+    else proto = std::make_shared<K17CompressibleNavierStokes>();
     proto->setNX({nx, ny, nz});
     proto->setCollisionFactor(omega);
     proto->setBCSet(std::make_shared<BCSet>());
     Block b;
-    b.kernel = std::dynamic_pointer_cast<K17CompressibleNavierStokes>(proto->clone());
+    b.kernel = proto->clone();
     b.eso = std::dynamic_pointer_cast<EsoSplit>(b.kernel->getDataSet()->getFdistributions());
     return b;
 }
@@ -204,6 +211,21 @@ int ref_run_periodic(double* F, int nx, int ny, int nz, double omega, int nsteps
     return 0;
 }
 
+// The same with the reference's incompressible BGK CPU kernel
+// (B92IncompressibleNavierStokes::calculate, B92IncompressibleNavierStokes.cpp:74-338).
+int ref_run_periodic_b92(double* F, int nx, int ny, int nz, double omega, int nsteps)
+{
+    Block b = make_block(nx, ny, nz, omega, true);
+    load(b, F, nx, ny, nz);
+    for (int s = 0; s < nsteps; ++s) {
+        b.kernel->calculate(s);
+        periodic_fill(*b.eso, nx, ny, nz);
+        b.kernel->swapDistributions();
+    }
+    store(b, F, nx, ny, nz);
+    return 0;
+}
+
 // Closed box of nx*ny*nz fluid nodes surrounded by the block's ghost layer as
 // walls.  Every wall link uses the reference VelocityBounceBack strategy
 // (post-collision); links into the z = nz+1 plane move with (lux, luy, luz),
@@ -251,36 +273,178 @@ int ref_run_cavity(double* F, int nx, int ny, int nz, double omega, double lux, 
     return 0;
 }
 
-// CPU baseline: nblocks independent periodic-less blocks of bs^3 nodes advanced
-// nsteps times with K17CompressibleNavierStokes::calculate + swap, OpenMP
-// guided over blocks as Simulation::calculateBlocks (Simulation.cpp:366-401).
-// Returns wall seconds of the timed loop (after one warm-up step).
-double ref_time_blocks(int nblocks, int bs, double omega, int nsteps, int* threads_used)
-{
+} // extern "C"
+
+namespace {
+
+// A periodic bx*by*bz grid of bs^3 blocks advanced the way Simulation::run
+// does for one level (Simulation.cpp:249-364): K17CompressibleNavierStokes::calculate
+// of every block (OpenMP guided, Simulation::calculateBlocks :366-401), the
+// block exchange, then swap.  The exchange gives every ghost slot of a block
+// the value its neighbouring block wrote there -- the copy the local
+// FullDirectConnectors make (OneDistributionFullDirectConnector.h:86-146),
+// OpenMP over blocks as connectorsSendLocal (:452-467); reads touch only
+// non-ghost slots, so blocks exchange in parallel without races.
+struct BlockGrid {
+    int B[3];
+    int n;
     std::vector<Block> blocks;
-    std::vector<double> F((size_t)bs * bs * bs * 27);
-    for (size_t i = 0; i < F.size(); ++i) F[i] = 1e-4 * (double)((i * 2654435761u) % 1000) / 1000.0;
-    for (int i = 0; i < nblocks; ++i) {
-        blocks.push_back(make_block(bs, bs, bs, omega));
-        load(blocks.back(), F.data(), bs, bs, bs);
+
+    BlockGrid(int bx, int by, int bz, int bs, double omega) : B{bx, by, bz}, n(bs)
+    {
+        for (int i = 0; i < bx * by * bz; ++i) blocks.push_back(make_block(bs, bs, bs, omega));
     }
-    auto step = [&](int s) {
-#pragma omp parallel for schedule(guided)
-        for (int i = 0; i < nblocks; ++i) {
-            blocks[i].kernel->calculate(s);
-            blocks[i].kernel->swapDistributions();
+    int block_of(const int (&c)[3]) const
+    {
+        int w[3];
+        for (int a = 0; a < 3; ++a) w[a] = ((c[a] % B[a]) + B[a]) % B[a];
+        return w[0] + B[0] * (w[1] + B[1] * w[2]);
+    }
+    void exchange_block(int ib)
+    {
+        const int bc[3] = {ib % B[0], (ib / B[0]) % B[1], ib / (B[0] * B[1])};
+        const int xs[4] = {0, 1, n + 1, n + 2};
+        // raw array pointers of the 27 neighbouring blocks (shift -1..1 per axis),
+        // fetched once: the shared_ptr getters would contend on reference counts
+        using A4 = decltype(blocks[0].eso->getLocalDistributions().get());
+        using A3 = decltype(blocks[0].eso->getZeroDistributions().get());
+        A4 nbr4[2][27];
+        A3 nbr3[27];
+        for (int c = 0; c < 27; ++c) {
+            const int nbc[3] = {bc[0] + c % 3 - 1, bc[1] + (c / 3) % 3 - 1, bc[2] + c / 9 - 1};
+            const Block& nb = blocks[block_of(nbc)];
+            nbr4[0][c] = nb.eso->getLocalDistributions().get();
+            nbr4[1][c] = nb.eso->getNonLocalDistributions().get();
+            nbr3[c] = nb.eso->getZeroDistributions().get();
         }
-    };
-    step(0);
+        for (int which = 0; which < 2; ++which) {
+            A4 to = nbr4[which][13];
+            const auto& wo = which == 0 ? WO_LOCAL : WO_NONLOCAL;
+            for (int e = 0; e < 13; ++e) {
+                // per axis: the slots p with p - wo in 1..n are the block's own;
+                // every other slot (within two layers of a face) is a ghost that
+                // takes the neighbouring block's value at wrap(p - wo) + wo
+                std::vector<int> shv(3 * (n + 3)), srcv(3 * (n + 3));
+                int* sh[3] = {shv.data(), shv.data() + (n + 3), shv.data() + 2 * (n + 3)};
+                int* src[3] = {srcv.data(), srcv.data() + (n + 3), srcv.data() + 2 * (n + 3)};
+                for (int a = 0; a < 3; ++a)
+                    for (int p = 0; p <= n + 2; ++p) {
+                        const int w = p - wo[e][a];
+                        sh[a][p] = w < 1 ? 0 : w > n ? 2 : 1;
+                        src[a][p] = wrap1(w, n) + wo[e][a];
+                    }
+                const int ox = wo[e][0];
+                for (int z = 0; z <= n + 2; ++z)
+                    for (int y = 0; y <= n + 2; ++y) {
+                        const int cyz = 9 * sh[2][z] + 3 * sh[1][y];
+                        const bool inner = sh[2][z] == 1 && sh[1][y] == 1;
+                        // inner rows: only x in 0..ox and n+ox+1..n+2 are ghosts
+                        for (int x = 0; x <= n + 2; x = (inner && x == ox) ? n + ox + 1 : x + 1) {
+                            const int c = cyz + sh[0][x];
+                            (*to)(e, x, y, z) = (*nbr4[which][c])(e, src[0][x], src[1][y], src[2][z]);
+                        }
+                    }
+            }
+        }
+        A3 zto = nbr3[13];
+        for (int z = 0; z <= n + 1; ++z)
+            for (int y = 0; y <= n + 1; ++y)
+                for (int x = 0; x <= n + 1; ++x) {
+                    if (x >= 1 && x <= n && y >= 1 && y <= n && z >= 1 && z <= n) continue;
+                    const int p[3] = {x, y, z};
+                    int c = 0, src[3];
+                    for (int a = 2; a >= 0; --a) {
+                        c = 3 * c + (p[a] < 1 ? 0 : p[a] > n ? 2 : 1);
+                        src[a] = wrap1(p[a], n);
+                    }
+                    (*zto)(x, y, z) = (*nbr3[c])(src[0], src[1], src[2]);
+                }
+    }
+    void step(int s, bool exchange = true)
+    {
+        const int nb = (int)blocks.size();
+#pragma omp parallel for schedule(guided)
+        for (int i = 0; i < nb; ++i) blocks[i].kernel->calculate(s);
+        if (exchange) {
+#pragma omp parallel for schedule(guided)
+            for (int i = 0; i < nb; ++i) exchange_block(i);
+        }
+#pragma omp parallel for schedule(guided)
+        for (int i = 0; i < nb; ++i) blocks[i].kernel->swapDistributions();
+    }
+    // node-major field of the whole (bx*bs, by*bs, bz*bs) domain, x fastest
+    void transfer(double* F, bool to_blocks)
+    {
+        const int NX = B[0] * n, NY = B[1] * n;
+        std::vector<double> tmp((size_t)n * n * n * 27);
+        for (int ib = 0; ib < (int)blocks.size(); ++ib) {
+            const int bc[3] = {ib % B[0], (ib / B[0]) % B[1], ib / (B[0] * B[1])};
+            if (!to_blocks) store(blocks[ib], tmp.data(), n, n, n);
+            for (int z = 0; z < n; ++z)
+                for (int y = 0; y < n; ++y)
+                    for (int x = 0; x < n; ++x) {
+                        const int64_t g = (bc[0] * n + x) + (int64_t)NX * ((bc[1] * n + y) + (int64_t)NY * (bc[2] * n + z));
+                        const int64_t l = x + (int64_t)n * (y + (int64_t)n * z);
+                        for (int d = 0; d < 27; ++d) {
+                            if (to_blocks) tmp[27 * l + d] = F[27 * g + d];
+                            else F[27 * g + d] = tmp[27 * l + d];
+                        }
+                    }
+            if (to_blocks) load(blocks[ib], tmp.data(), n, n, n);
+        }
+    }
+};
+
+} // namespace
+
+extern "C" {
+
+// CPU baseline: nblocks blocks of bs^3 nodes (split into an as-cubic-as-possible
+// periodic block grid) advanced nsteps times by BlockGrid::step (exchange = 0
+// leaves the block exchange out).  Returns wall seconds of the timed loop
+// (after one warm-up step) and the grid in dims3.
+double ref_time_blocks(int nblocks, int bs, double omega, int nsteps, int exchange, int* threads_used, int* dims3)
+{
+    int B[3] = {1, 1, 1};
+    int rest = nblocks;
+    for (int a = 0; a < 2; ++a) {
+        const double want = std::cbrt((double)rest);
+        int best = 1;
+        for (int f = 1; f <= rest; ++f)
+            if (rest % f == 0 && std::fabs(f - want) < std::fabs(best - want)) best = f;
+        B[a] = best;
+        rest /= best;
+    }
+    B[2] = rest;
+    BlockGrid g(B[0], B[1], B[2], bs, omega);
+    std::vector<double> F((size_t)nblocks * bs * bs * bs * 27);
+    for (size_t i = 0; i < F.size(); ++i) F[i] = 1e-4 * (double)((i * 2654435761u) % 1000) / 1000.0;
+    g.transfer(F.data(), true);
+    g.step(0, exchange != 0);
     auto t0 = std::chrono::steady_clock::now();
-    for (int s = 1; s <= nsteps; ++s) step(s);
+    for (int s = 1; s <= nsteps; ++s) g.step(s, exchange != 0);
     auto t1 = std::chrono::steady_clock::now();
 #ifdef _OPENMP
     if (threads_used) *threads_used = omp_get_max_threads();
 #else
     if (threads_used) *threads_used = 1;
 #endif
+    if (dims3)
+        for (int a = 0; a < 3; ++a) dims3[a] = B[a];
     return std::chrono::duration<double>(t1 - t0).count();
+}
+
+// The baseline's block grid on one node-major periodic field F of
+// (bx*bs, by*bs, bz*bs) nodes: nsteps of BlockGrid::step, F receives the
+// pre-collision state.  Equal to ref_run_periodic on the same field when the
+// exchange is the streaming a periodic domain sees.
+int ref_run_blocks(double* F, int bx, int by, int bz, int bs, double omega, int nsteps)
+{
+    BlockGrid g(bx, by, bz, bs, omega);
+    g.transfer(F, true);
+    for (int s = 0; s < nsteps; ++s) g.step(s);
+    g.transfer(F, false);
+    return 0;
 }
 
 } // extern "C"

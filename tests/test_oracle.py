@@ -127,6 +127,64 @@ def test_periodic_matches_reference(dims):
 
 
 @needs_ref
+@pytest.mark.parametrize("grid", [(2, 1, 1), (2, 3, 2)])
+def test_cpu_baseline_block_exchange_is_periodic_streaming(grid):
+    """The CPU baseline's multi-block run (reference calculate, ghost-slot
+    exchange between neighbouring blocks, swap) gives exactly the populations of
+    one periodic domain of the same size: a wrong source block, slot or offset
+    in the exchange changes them."""
+    bs = 5
+    dims = tuple(g * bs for g in grid)
+    F0 = oracle.init_field(*cases.random_flow(dims, seed=11))
+    got = oracle.ref_run_blocks(F0, grid, bs, 6, 1.8)
+    assert np.array_equal(got, oracle.ref_run_periodic(F0, dims, 6, 1.8))
+    assert np.array_equal(got, oracle.run(F0, dims, 6, 1.8))
+
+
+@needs_ref
+def test_k17_unstable_below_omega_1_7_in_reference_and_oracle():
+    """K17 as implemented goes non-finite at low omega in a periodic flow: the
+    compiled reference and the oracle blow up identically (same step, same
+    non-finite pattern), so tests that check physics use omega >= 1.8."""
+    dims = (8, 8, 8)
+    F0 = oracle.init_field(*cases.random_flow(dims, seed=3, u_amp=0.05))
+    for omega in (1.3248, 1.5):
+        a = oracle.ref_run_periodic(F0, dims, 400, omega)
+        b = oracle.run(F0, dims, 400, omega)
+        assert not np.isfinite(a).all()
+        assert np.array_equal(np.isfinite(a), np.isfinite(b))
+        fin = np.isfinite(a)
+        assert np.array_equal(a[fin], b[fin])
+    # at the C1 rate the same flow stays finite
+    assert np.isfinite(oracle.ref_run_periodic(F0, dims, 400, 1.8)).all()
+
+
+@needs_ref
+def test_bgk_agrees_with_reference_incompressible_bgk_to_third_order():
+    """The oracle's compressible BGK (the GPU kernel B92CompressibleNavierStokes
+    has no CPU twin) against the reference's own CPU B92IncompressibleNavierStokes
+    compiled from its sources.  Written in the momentum j = (1 + drho) u, the
+    compressible equilibrium w (drho + 3 c.j + 4.5 (c.j)^2 / (1 + drho) - 1.5 j^2 / (1 + drho))
+    differs from the incompressible one only by O(drho j^2), so with velocity
+    and density deviations of amplitude a the two runs part by O(a^3): halving
+    a divides the difference by 8.  A wrong weight, sign or rate leaves an O(a)
+    difference, a wrong velocity definition an O(a^2) one."""
+    dims = (8, 7, 6)
+    errs = []
+    amps = (2e-2, 1e-2, 5e-3, 2.5e-3)
+    for a in amps:
+        F0 = oracle.init_field(*cases.random_flow(dims, seed=5, u_amp=a, rho_amp=a))
+        comp = oracle.run(F0, dims, 2, 1.7, collision="bgk")
+        inc = oracle.ref_run_periodic_b92(F0, dims, 2, 1.7)
+        errs.append(float(np.abs(comp - inc).max()))
+        # both moved the state by O(a): the comparison is not trivially small
+        assert np.abs(comp - F0).max() > 0.3 * a
+    ratios = [errs[i] / errs[i + 1] for i in range(len(errs) - 1)]
+    assert all(7.0 < r < 9.0 for r in ratios), (errs, ratios)
+    assert errs[0] < amps[0] ** 3
+
+
+@needs_ref
 def test_cavity_matches_reference():
     n = 7
     F = oracle.ref_run_cavity(np.zeros((n ** 3, 27)), (n, n, n), 30, 1.62, lid=(0.04, 0.01, 0.0))

@@ -74,24 +74,85 @@ def test_slip_keeps_uniform_flow_uniform():
     assert res[1] > 0.1
 
 
-def test_pressure_boundary_keeps_matching_equilibrium():
-    """If the inward neighbour is at the equilibrium of the prescribed density
-    the non-equilibrium pressure boundary reproduces that equilibrium, and it
-    replaces exactly the directions with c . normal <= 0."""
+@pytest.mark.parametrize("normal", [1, 2, 3, 6])
+def test_pressure_boundary_replaces_the_density_of_an_equilibrium_neighbour(normal):
+    """The inward neighbour holds the equilibrium of a density drho1 that
+    differs from the prescribed rho_bc, with a velocity along all three axes:
+    the non-equilibrium pressure boundary writes exactly feq(rho_bc, u) on the
+    directions with c . normal <= 0 (the neighbour's non-equilibrium part is
+    zero) and nothing else.  Dropping the density replacement, or replacing
+    the wrong directions, fails by about w * |drho1 - rho_bc|."""
+    drho1, rho_bc, u = 5e-3, -1e-3, (0.01, 0.004, -0.003)
+    fn = oracle.equilibrium(drho1, *u)
+    out, wr = oracle.pressure_bc(fn, rho_bc, normal)
+    c = np.stack([cases.CX, cases.CY, cases.CZ])
+    expect = (c.T @ c[:, normal]) <= 0
+    assert np.array_equal(wr, expect)
+    np.testing.assert_allclose(out[wr], oracle.equilibrium(rho_bc, *u)[wr], rtol=0, atol=2e-17)
+    assert np.abs(oracle.equilibrium(rho_bc, *u) - fn)[wr].min() > 1e-5
+
+
+def test_pressure_boundary_in_a_channel_step():
+    """The same in a one-step channel run: the outlet node's replaced
+    populations come from its neighbour's state after the collision and
+    streaming (K17 moves an equilibrium by O(u^3)), the others stay as the
+    streaming left them."""
     nx, ny, nz = 6, 5, 5
     solid, groups, onodes, onbrs = cases.inflow_channel(nx, ny, nz)
     N = nx * ny * nz
-    rho_bc = 2e-3
-    F0 = oracle.init_field(np.full(N, rho_bc), np.full(N, 0.01), np.zeros(N), np.zeros(N))
+    drho1, rho_bc, u = 5e-3, -1e-3, (0.01, 0.004, -0.003)
+    F0 = oracle.init_field(np.full(N, drho1), np.full(N, u[0]), np.full(N, u[1]), np.full(N, u[2]))
     bcs = [oracle.BCGroup(onodes, kind=4, rho=rho_bc, nbr=onbrs, normal=1),
-           oracle.BCGroup(*groups["wall"], kind=1), oracle.BCGroup(*groups["inlet"], ux=0.01, kind=2)]
+           oracle.BCGroup(*groups["wall"], kind=1), oracle.BCGroup(*groups["inlet"], ux=u[0], kind=2)]
+    F_nobc = oracle.run(F0, (nx, ny, nz), 1, OMEGA, solid=solid, bcs=bcs[1:] + [
+        oracle.BCGroup(onodes, kind=4, rho=drho1, nbr=onbrs, normal=1)])
     F = oracle.run(F0, (nx, ny, nz), 1, OMEGA, solid=solid, bcs=bcs)
-    # a single step from a uniform equilibrium: interior stays at equilibrium,
-    # the outlet node's replaced populations equal the neighbour's equilibrium
     k = onodes[len(onodes) // 2]
-    feq = oracle.equilibrium(rho_bc, 0.01, 0.0, 0.0)
-    replaced = [d for d in range(27) if cases.CX[d] <= 0]
-    np.testing.assert_allclose(F[k][replaced], feq[replaced], rtol=0, atol=1e-15)
+    kn = onbrs[len(onodes) // 2]
+    out, wr = oracle.pressure_bc(F[kn], rho_bc, 1)
+    np.testing.assert_array_equal(F[k][wr], out[wr])
+    np.testing.assert_array_equal(F[k][~wr], F_nobc[k][~wr])
+    assert np.abs(F[k][wr] - F_nobc[k][wr]).min() > 1e-5
+
+
+@pytest.mark.parametrize("q", [0.3, 0.5, 0.8])
+def test_slip_walls_let_a_body_force_accelerate_a_plug_flow(q):
+    """Slip walls on both sides of a periodic channel exert no tangential
+    friction: a body force (acceleration 2 F per step for the K17 force F)
+    accelerates the fluid as a plug, u_x = 2 F t everywhere.  No-slip walls at
+    the same positions leave a strongly sheared profile instead."""
+    nx, ny, nz, steps, Fh = 2, 16, 2, 400, 1e-6
+    solid, groups = cases.plate_channel(nx, ny, nz, q, q)
+    N = nx * ny * nz
+    prof = {}
+    for kind in (3, 1):
+        bcs = [oracle.BCGroup(*groups["bottom"], kind=kind), oracle.BCGroup(*groups["top"], kind=kind)]
+        F = oracle.run(np.zeros((N, 27)), (nx, ny, nz), steps, OMEGA, solid=solid, bcs=bcs, force=(Fh, 0.0, 0.0))
+        prof[kind] = np.array([oracle.macroscopic(f)[1] for f in F]).reshape(nz, ny, nx)[1, 1:-1, 1]
+    ux = prof[3]
+    assert abs(ux.mean() / (2 * Fh * steps) - 1) < 0.02
+    assert (ux.max() - ux.min()) / ux.mean() < 0.03
+    assert (prof[1].max() - prof[1].min()) / prof[1].mean() > 0.5
+
+
+@pytest.mark.parametrize("q", [0.3, 0.5])
+def test_slip_walls_stop_the_normal_flow_and_keep_the_tangential(q):
+    """A uniform stream u = (0.02, 0.01, 0) into slip walls at y: the normal
+    component is stopped (mean u_y from 0.01 to below 1e-4 after 2000 steps)
+    while the tangential momentum stays (mean u_x within 1e-3 of 0.02).  A
+    slip kernel that kept the fluid's normal velocity as the wall velocity
+    lets the normal flow through the wall and goes unstable."""
+    nx, ny, nz = 2, 16, 2
+    solid, groups = cases.plate_channel(nx, ny, nz, q, q)
+    N = nx * ny * nz
+    F0 = oracle.init_field(np.zeros(N), np.full(N, 0.02), np.full(N, 0.01), np.zeros(N))
+    F0[solid] = 0.0
+    bcs = [oracle.BCGroup(*groups["bottom"], kind=3), oracle.BCGroup(*groups["top"], kind=3)]
+    F = oracle.run(F0, (nx, ny, nz), 2000, OMEGA, solid=solid, bcs=bcs)
+    m = np.array([oracle.macroscopic(f) for f in F]).reshape(nz, ny, nx, 4)[:, 1:-1, :]
+    assert np.isfinite(m).all()
+    assert abs(m[..., 2].mean()) < 1e-4
+    assert abs(m[..., 1].mean() / 0.02 - 1) < 1e-3
 
 
 def test_inflow_outflow_channel_reaches_prescribed_outlet_density():
