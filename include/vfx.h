@@ -259,6 +259,36 @@ vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t
 vfx_status vfx_k17_step(const vfx_lattice* lat, int32_t is_even_timestep, int32_t collision_template, vfx_stream stream);
 vfx_status vfx_bgk_step(const vfx_lattice* lat, int32_t is_even_timestep, vfx_stream stream);
 
+/* Boundary conditions fused into the collide-and-stream kernel of a
+ * structured lattice, so a boundary node's populations are not read again by
+ * a kernel of their own (on a face normal to x each of them sits in its own
+ * cache line).  node_mask: ceil(n_nodes/32) words, bit k set iff node k is an
+ * entry of bc; word_base: ceil(n_nodes/32) words, the number of entries on
+ * nodes below 32*w.  bc->k must be sorted ascending, one entry per node. */
+typedef struct vfx_fused_bc {
+    const vfx_bc* bc;
+    const uint32_t* node_mask;
+    const uint32_t* word_base;
+} vfx_fused_bc;
+
+/* vfx_k17_step (VFX_TEMPLATE_DEFAULT, default kernel variant, no turbulence
+ * model) of a structured lattice with fused boundary conditions:
+ *  - post[0..n_post-1] (n_post <= 2): interpolated kinds 1, 2, 3, 5, applied by
+ *    the thread that collided each boundary node -- the same populations as
+ *    vfx_bc_apply after the step;
+ *  - pre (or NULL): VFX_BC_PRESSURE_NEQ with normal dP00 (dM00) whose every
+ *    entry has k_neighbor = k - 1 (k + 1) in the same wavefront of 64 nodes,
+ *    i.e. x mod 64 != 0 (!= 63): the replacement vfx_bc_apply makes after the
+ *    previous step's parity flip is made inside this step, from the
+ *    neighbour's populations the neighbouring lane gathers (apply_pre = 1;
+ *    0 on the first step of a run, before which no pressure boundary ran).
+ *    Until the next step the boundary nodes then hold the un-replaced
+ *    populations: vfx_bc_apply(pre->bc) with the new parity gives the state
+ *    the separate launches leave, before it is read.
+ * Results equal vfx_k17_step + vfx_bc_apply bit for bit in every storage form. */
+vfx_status vfx_k17_step_fused(const vfx_lattice* lat, int32_t is_even_timestep, const vfx_fused_bc* post,
+                              int32_t n_post, const vfx_fused_bc* pre, int32_t apply_pre, vfx_stream stream);
+
 /* calcTurbulentViscosityAMD (src/gpu/core/PostProcessor/TurbulentViscosityKernels.cu:62-130):
  * the AMD eddy viscosity of every fluid node from vx/vy/vz into
  * turbulent_viscosity, with constant sgs_constant.  UpdateGrid27 runs it after

@@ -1,0 +1,183 @@
+"""Boundary conditions applied inside the collide-stream kernel
+(vfx_k17_step_fused through Stepper(fuse=True)) against the separate boundary
+kernels of the unfused step: the stored populations agree bit for bit in FP64,
+FP32 and FP16 storage, over runs split into several step() calls, with the
+entries the kernel cannot take left to residual kernels; FP64 also against the
+oracle."""
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import virtualfluids_amd as vfx
+from tests.test_geometry import oracle_groups, small_channel
+from virtualfluids_amd import geometry
+
+pytestmark = pytest.mark.gpu
+
+FORMS = ("fp64", "fp32", "fp16")
+
+
+def _lattice(dims, solid, omega, storage, u_in):
+    uref = (u_in, 0.0, 0.0) if storage == "fp16" else (0.0, 0.0, 0.0)
+    lat = vfx.Lattice(dims, storage=storage, omega=omega, solid=solid, reference_velocity=uref)
+    N = int(np.prod(dims))
+    z = np.zeros(N)
+    lat.init_equilibrium(z, np.full(N, 0.5 * u_in), z, z)
+    return lat
+
+
+def _pair(dims, solid, omega, tables, storage, u_in, chunks):
+    """The same run unfused (one call) and fused (calls of `chunks` steps)."""
+    a = _lattice(dims, solid, omega, storage, u_in)
+    b = _lattice(dims, solid, omega, storage, u_in)
+    sa = vfx.Stepper(a, "k17", geometry.boundary_conditions(tables))
+    sb = vfx.Stepper(b, "k17", geometry.boundary_conditions(tables), fuse=True)
+    sa.step(sum(chunks))
+    for c in chunks:
+        sb.step(c)
+    torch.cuda.synchronize()
+    assert sa.even == sb.even
+    return a, b, sb
+
+
+def _same(a, b):
+    fa, fb = a.f.view(torch.uint8), b.f.view(torch.uint8)
+    diff = int((fa != fb).sum())
+    print(f"differing bytes {diff}", flush=True)
+    return diff == 0
+
+
+@pytest.mark.parametrize("storage", FORMS)
+def test_channel_fused_equals_separate_kernels(cuda, storage):
+    print(f"[fused channel {storage}]", flush=True)
+    ch = small_channel()
+    a, b, st = _pair(ch.dims, ch.solid, ch.omega, ch.bcs, storage, ch.u_in, (7, 1, 17))
+    plan = st.fused_plan()
+    assert len(plan.post) == 2 and plan.pre is not None and not plan.residual_post and not plan.residual_pre
+    assert _same(a, b)
+
+
+def test_channel_fused_fp64_matches_oracle(cuda):
+    steps = 60
+    print(f"[fused channel fp64 vs oracle x {steps}]", flush=True)
+    ch = small_channel()
+    lat = vfx.Lattice(ch.dims, storage="fp64", omega=ch.omega, solid=ch.solid)
+    N = int(np.prod(ch.dims))
+    z = np.zeros(N)
+    lat.init_equilibrium(z, z, z, z)
+    st = vfx.Stepper(lat, "k17", geometry.boundary_conditions(ch.bcs), fuse=True)
+    st.step(steps)
+    got = st.populations().cpu().numpy()
+    ref = oracle.run(np.zeros((N, 27)), ch.dims, steps, ch.omega, solid=ch.solid, bcs=oracle_groups(ch))
+    fl = ~ch.solid
+    err = float(np.abs(got[fl] - ref[fl]).max() / np.abs(ref[fl]).max())
+    print(f"max rel err {err:.3e}")
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("storage", ("fp64", "fp16"))
+def test_outlet_on_a_wavefront_edge_runs_as_residual(cuda, storage):
+    """Outlet at x = 64: the inward neighbour x = 63 sits in the previous
+    wavefront, so the pressure entries stay in a kernel of their own."""
+    print(f"[fused channel outlet x=64 {storage}]", flush=True)
+    ch = geometry.spheres_in_channel(dims=(66, 20, 18), n_spheres=3, r_range=(2.5, 4.0), x_clear=(8, 10))
+    a, b, st = _pair(ch.dims, ch.solid, ch.omega, ch.bcs, storage, ch.u_in, (5, 9))
+    plan = st.fused_plan()
+    assert plan.pre is None and len(plan.residual_pre) == 1
+    assert _same(a, b)
+
+
+@pytest.mark.parametrize("storage", ("fp64", "fp32", "fp16"))
+def test_mixed_fused_and_residual_entries(cuda, storage):
+    """A fluid node behind one wall link and solid inward neighbours of some
+    outlet nodes: those entries run in residual kernels, the rest fused."""
+    print(f"[fused channel mixed split {storage}]", flush=True)
+    ch = small_channel()
+    nx, ny, nz = ch.dims
+    solid = ch.solid.copy()
+    ns = next(t for t in ch.bcs if t.kind == 1)
+    i = ns.nodes.size // 3
+    k = int(ns.nodes[i])
+    d = int(np.flatnonzero(ns.q27[:, i] >= 0)[0])
+    x, y, z = k % nx, (k // nx) % ny, k // (nx * ny)
+    far = (x + vfx.CX[d]) % nx + nx * ((y + vfx.CY[d]) % ny + ny * ((z + vfx.CZ[d]) % nz))
+    solid[far] = False
+    out = next(t for t in ch.bcs if t.kind == 4)
+    in_tables = set(np.concatenate([t.nodes for t in ch.bcs]).tolist())
+    inner = [j for j, kn in enumerate(out.neighbors) if int(kn) not in in_tables][:4]
+    assert len(inner) == 4
+    solid[out.neighbors[inner]] = True
+    a, b, st = _pair(ch.dims, solid, ch.omega, ch.bcs, storage, ch.u_in, (6, 6))
+    plan = st.fused_plan()
+    assert plan.pre is not None and len(plan.residual_pre) == 1 and plan.residual_pre[0].n == 4
+    assert len(plan.residual_post) >= 1
+    assert _same(a, b)
+
+
+def _mirrored_channel():
+    """Pressure inlet on x = 1 (outward normal dM00, neighbour k + 1, higher
+    density) and a velocity outflow on x = nx-2 (links into the x = nx-1
+    plane, q = 1/2), spheres and no-slip walls as in the channel."""
+    ch = small_channel()
+    nx, ny, nz = ch.dims
+    fluid = ~ch.solid
+    zz, yy = np.meshgrid(np.arange(1, nz - 1), np.arange(1, ny - 1), indexing="ij")
+    face = lambda x: (x + nx * (yy + ny * zz)).ravel().astype(np.int64)
+    inl = face(1)
+    inl = inl[fluid[inl]]
+    outl = face(nx - 2)
+    outl = outl[fluid[outl]]
+    q = np.full((27, outl.size), -1.0, np.float32)
+    for dd in range(1, 27):
+        if vfx.CX[dd] == 1:
+            q[dd] = 0.5
+    vel = np.zeros((3, outl.size))
+    vel[0] = ch.u_in
+    tables = [t for t in ch.bcs if t.kind == 1]
+    tables.append(geometry.BCTable(4, inl, rho=np.full(inl.size, 0.003), neighbors=inl + 1, normal=2))
+    tables.append(geometry.BCTable(2, outl, q, velocity=vel))
+    return ch, tables
+
+
+@pytest.mark.parametrize("storage", FORMS)
+def test_pressure_on_a_minus_x_face(cuda, storage):
+    print(f"[fused mirrored channel {storage}]", flush=True)
+    ch, tables = _mirrored_channel()
+    a, b, st = _pair(ch.dims, ch.solid, ch.omega, tables, storage, ch.u_in, (4, 11))
+    plan = st.fused_plan()
+    assert plan.pre is not None and plan.pre.bc.normal == 2 and not plan.residual_pre
+    assert _same(a, b)
+
+
+@pytest.mark.parametrize("storage", FORMS)
+def test_no_groups_is_the_plain_step(cuda, storage):
+    print(f"[fused step without groups {storage}]", flush=True)
+    ch = small_channel()
+    a, b, st = _pair(ch.dims, ch.solid, ch.omega, [], storage, ch.u_in, (3, 4))
+    assert st.fused_plan().post == [] and st.fused_plan().pre is None
+    assert _same(a, b)
+
+
+@pytest.mark.parametrize("storage", FORMS)
+@pytest.mark.parametrize("part", ("noslip", "velocity", "pressure"))
+def test_one_group(cuda, storage, part):
+    """Each boundary group of the channel fused alone."""
+    print(f"[fused {part} only {storage}]", flush=True)
+    ch = small_channel()
+    kind = {"noslip": 1, "velocity": 2, "pressure": 4}[part]
+    tables = [t for t in ch.bcs if t.kind == kind]
+    a, b, st = _pair(ch.dims, ch.solid, ch.omega, tables, storage, ch.u_in, (3, 4))
+    assert _same(a, b)
+
+
+def test_fused_rejects_unsupported_lattices(cuda):
+    ch = small_channel()
+    lat = vfx.Lattice(ch.dims, storage="fp32", omega=ch.omega, solid=ch.solid)
+    with pytest.raises(ValueError):
+        vfx.Stepper(lat, "bgk", [], fuse=True).step(1)
+    s = lat.struct()
+    bad = vfx._lib.VfxFusedBC()
+    with pytest.raises(vfx.VfxError):
+        vfx.check(vfx.lib().vfx_k17_step_fused(vfx._lib.C.byref(s), 1, vfx._lib.C.byref(bad), 1, None, 0, None),
+                  "vfx_k17_step_fused")

@@ -59,6 +59,10 @@ class VfxBC(C.Structure):
     ]
 
 
+class VfxFusedBC(C.Structure):
+    _fields_ = [("bc", C.POINTER(VfxBC)), ("node_mask", C.c_void_p), ("word_base", C.c_void_p)]
+
+
 class VfxDomain(C.Structure):
     _fields_ = [("f_offset", C.c_uint64), ("dims", C.c_uint32 * 3), ("omega", C.c_double),
                 ("lid_velocity", C.c_double * 3)]
@@ -79,6 +83,8 @@ _SIGNATURES = {
     "vfx_collide_stream": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_int32, C.c_void_p],
     "vfx_k17_step": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_void_p],
     "vfx_bgk_step": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p],
+    "vfx_k17_step_fused": [C.POINTER(VfxLattice), C.c_int32, C.POINTER(VfxFusedBC), C.c_int32,
+                           C.POINTER(VfxFusedBC), C.c_int32, C.c_void_p],
     "vfx_bc_apply": [C.POINTER(VfxLattice), C.POINTER(VfxBC), C.c_int32, C.c_void_p],
     "vfx_init_equilibrium": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                              C.c_void_p],

@@ -238,10 +238,9 @@ vfx_status vfx_bgk_step(const vfx_lattice* lat, int32_t is_even, vfx_stream stre
     return vfx_collide_stream(lat, VFX_BGK, is_even, VFX_TEMPLATE_DEFAULT, stream);
 }
 
-vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_even, vfx_stream stream)
+// argument checks of a boundary table and its device view
+static vfx_status make_bcdev(const vfx_lattice* lat, const vfx::Lat& L, const vfx_bc* bc, vfx::BCDev& B)
 {
-    vfx::Lat L;
-    if (vfx_status s = make_lat(lat, L)) return s;
     if (!bc) return fail(VFX_EINVAL, "bc is NULL");
     if (bc->kind < VFX_BC_VELOCITY_BOUNCE_BACK || bc->kind > VFX_BC_PRECURSOR_DISTRIBUTIONS)
         return fail(VFX_EUNSUPPORTED, "bc kind %d not supported", bc->kind);
@@ -276,7 +275,7 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
     } else if (bc->n && (!bc->k || !bc->q27)) {
         return fail(VFX_EINVAL, "bc.k / bc.q27 is NULL");
     }
-    vfx::BCDev B{bc->n, bc->k, bc->q27, bc->vx, bc->vy, bc->vz, bc->cpu_float_velocity, lat->direct,
+    B = vfx::BCDev{bc->n, bc->k, bc->q27, bc->vx, bc->vy, bc->vz, bc->cpu_float_velocity, lat->direct,
                  lat->omega, bc->rho, bc->k_neighbor, bc->normal, bc->pressure_correction_factor,
                  {bc->wm_normal[0], bc->wm_normal[1], bc->wm_normal[2]},
                  {bc->wm_exchange_velocity[0], bc->wm_exchange_velocity[1], bc->wm_exchange_velocity[2]},
@@ -288,6 +287,15 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
                  {bc->plane_weight[0], bc->plane_weight[1], bc->plane_weight[2], bc->plane_weight[3]},
                  bc->precursor_last, bc->precursor_current, bc->time_ratio, bc->velocity_ratio,
                  {bc->precursor_velocity[0], bc->precursor_velocity[1], bc->precursor_velocity[2]}};
+    return VFX_OK;
+}
+
+vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_even, vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    vfx::BCDev B;
+    if (vfx_status s = make_bcdev(lat, L, bc, B)) return s;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     switch (lat->storage) {
@@ -296,6 +304,53 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
     default: e = vfx::launch_bc<_Float16>(L, B, bc->kind, is_even != 0, st); break;
     }
     return hip_status(e, "bc launch");
+}
+
+vfx_status vfx_k17_step_fused(const vfx_lattice* lat, int32_t is_even, const vfx_fused_bc* post, int32_t n_post,
+                              const vfx_fused_bc* pre, int32_t apply_pre, vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    if (!lat->direct) return fail(VFX_EUNSUPPORTED, "fused boundary conditions need structured (direct) addressing");
+    if (lat->kernel_variant != 0) return fail(VFX_EUNSUPPORTED, "fused boundary conditions run the default K17 variant");
+    if (lat->turbulence_model != VFX_TURB_NONE)
+        return fail(VFX_EUNSUPPORTED, "fused boundary conditions run K17 without a turbulence model");
+    if (n_post < 0 || n_post > 2 || (n_post > 0 && !post)) return fail(VFX_EINVAL, "n_post must be 0, 1 or 2");
+    vfx::FusedGroup g[3];
+    std::memset(g, 0, sizeof(g));
+    for (int j = 0; j < n_post + 1; ++j) {
+        const vfx_fused_bc* f = j < n_post ? &post[j] : pre;
+        if (!f) continue;
+        if (!f->bc || !f->node_mask || !f->word_base) return fail(VFX_EINVAL, "fused bc needs bc, node_mask, word_base");
+        const int32_t kind = f->bc->kind;
+        if (j < n_post && kind != VFX_BC_NOSLIP_INTERPOLATED && kind != VFX_BC_VELOCITY_INTERPOLATED &&
+            kind != VFX_BC_SLIP && kind != VFX_BC_VELOCITY_PRESSURE_INTERPOLATED)
+            return fail(VFX_EUNSUPPORTED, "fused post-collision kind %d (kinds 1, 2, 3, 5 only)", kind);
+        if (j == n_post && (kind != VFX_BC_PRESSURE_NEQ || (f->bc->normal != 1 && f->bc->normal != 2)))
+            return fail(VFX_EUNSUPPORTED, "the fused pre-collision group is a pressure boundary with normal dP00 or dM00");
+        vfx::FusedGroup& G = g[j < n_post ? j : 2];
+        if (vfx_status s = make_bcdev(lat, L, f->bc, G.B)) return s;
+        if (f->bc->n == 0) continue; // an empty group selects nothing
+        G.mask = f->node_mask;
+        G.base = f->word_base;
+        G.kind = kind;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    const int ap = apply_pre != 0;
+    switch (lat->storage) {
+    case VFX_FP64:
+        e = vfx::launch_step_fused<double>(L, is_even != 0, lat->omega, lat->quadric_limiter, lat->force, g[0], g[1], g[2], ap, st);
+        break;
+    case VFX_FP32:
+        e = vfx::launch_step_fused<float>(L, is_even != 0, lat->omega, lat->quadric_limiter, lat->force, g[0], g[1], g[2], ap, st);
+        break;
+    default:
+        e = vfx::launch_step_fused<_Float16>(L, is_even != 0, lat->omega, lat->quadric_limiter, lat->force, g[0], g[1], g[2], ap,
+                                             st);
+        break;
+    }
+    return hip_status(e, "fused collide-stream launch");
 }
 
 vfx_status vfx_init_equilibrium(const vfx_lattice* lat, int32_t is_even, const double* drho, const double* ux,

@@ -46,13 +46,22 @@ template <> struct Storage<float, float> {
 // sh: offset of the stored deviation (the reference-velocity equilibrium, in
 // f - w units); st takes ss = -sh * s, so both are one mixed-precision FMA
 template <> struct Storage<_Float16, float> {
+    // the stored half of v and the value a load gives back (one expression
+    // each, shared by the memory accesses and the fused boundary conditions);
+    // explicit FMAs: a contractible multiply-add would be fused in one kernel
+    // and rounded twice in another, depending on the surrounding code
+    static __device__ __forceinline__ _Float16 enc2(float v, float s, float ss)
+    {
+        return (_Float16)__builtin_fmaf(v, s, ss);
+    }
+    static __device__ __forceinline__ float dec2(_Float16 h, float s, float sh) { return __builtin_fmaf((float)h, s, sh); }
     static __device__ __forceinline__ float ld(const _Float16* p, uint32_t i, float s, float sh = 0.f)
     {
-        return (float)elem(p, i) * s + sh;
+        return dec2(elem(p, i), s, sh);
     }
     static __device__ __forceinline__ void st(_Float16* p, uint32_t i, float v, float s, float ss = 0.f)
     {
-        elem(p, i) = (_Float16)(v * s + ss);
+        elem(p, i) = enc2(v, s, ss);
     }
     static __device__ __forceinline__ _Float16 enc(float v, float s) { return (_Float16)(v * s); }
     static __device__ __forceinline__ float dec(_Float16 h, float s) { return (float)h * s; }
@@ -943,23 +952,20 @@ __global__ void __launch_bounds__(256) k_bc_velocity_bb(Lat L, BCDev B)
 // boundary node, density and velocity (with the sum order those kernels use),
 // the symmetric equilibrium part of each wall link, and the Bouzidi-type
 // interpolation with the subgrid distance q.  KIND: 1 no-slip, 2 velocity, 3 slip.
-template <typename S, bool EVEN, int KIND>
-__global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
+// The links of boundary entry i at node k from its post-collision
+// populations p (indexed by direction): computes and stores the population
+// re-entering along every wall link.  KIND >= 0 fixes the kind at compile
+// time; KIND = -1 takes it from `kind` at run time (kinds 1, 2, 3, 5).
+template <typename S, typename T, bool EVEN, int KIND>
+__device__ __forceinline__ void interp_links(const Lat& L, const BCDev& B, uint32_t i, uint32_t k,
+                                             const uint32_t (&kk)[8], const T (&p)[27], int kind = KIND)
 {
-    using T = typename Compute<S>::T;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= B.n) return;
-    uint32_t kk[8];
-    const uint32_t k = B.k[i];
-    if (k >= L.N) return;
-    if (B.direct) nbrs_direct(L, k, kk);
-    else nbrs_sparse(L, k, kk);
-    T p[27]; // post-collision population of direction d
-#pragma unroll
-    for (int d = 0; d < 27; ++d) {
-        const int e = inv(d);
-        p[d] = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)], L.sh[d]);
-    }
+    // no contraction into FMAs here: which product the compiler would fuse
+    // depends on the code around the inlined body (a shared subexpression
+    // changes the choice), and the separate boundary kernel and the fused
+    // collide-stream kernel must give the same bits
+#pragma clang fp contract(off)
+    const int kd = KIND >= 0 ? KIND : kind;
     const T drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] + p[d0MP] +
                    p[d0PP] + p[d0MM] + p[dP0M] + p[dM0P] + p[dP0P] + p[dM0M] + p[dPM0] + p[dMP0] + p[dPP0] + p[dMM0] +
                    p[d00P] + p[d00M] + p[d0P0] + p[d0M0] + p[dP00] + p[dM00] + p[d000];
@@ -992,8 +998,8 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
 #pragma unroll
     for (int d = 1; d < 27; ++d) qs[d] = bc_q<T>(B, d, i);
     T VX = 0, VY = 0, VZ = 0;
-    if (KIND == VFX_BC_VELOCITY_INTERPOLATED || KIND == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED ||
-        KIND == VFX_BC_PRECURSOR_VELOCITY) {
+    if (kd == VFX_BC_VELOCITY_INTERPOLATED || kd == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED ||
+        kd == VFX_BC_PRECURSOR_VELOCITY) {
         VX = B.vx ? bc_at<T>(B.vx, i) : T(0);
         VY = B.vy ? bc_at<T>(B.vy, i) : T(0);
         VZ = B.vz ? bc_at<T>(B.vz, i) : T(0);
@@ -1040,12 +1046,12 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
             a = (T(1) - q) / (T(1) + q) * (f - fi + (f + fi - T(2) * feq * omega) / (T(1) - omega)) * T(0.5);
         }
         T v;
-        if (KIND == VFX_BC_NOSLIP_INTERPOLATED) {
+        if (kd == VFX_BC_NOSLIP_INTERPOLATED) {
             if constexpr (sizeof(T) == 4) v = a + q * (f + fi) * r1q;
             else v = a + (q * (f + fi)) / (T(1) + q);
         } else {
             T vbc;
-            if (KIND == VFX_BC_SLIP || KIND == VFX_BC_SLIP_TURBULENT_VISCOSITY) {
+            if (kd == VFX_BC_SLIP || kd == VFX_BC_SLIP_TURBULENT_VISCOSITY) {
                 if (weight_class(d) == 1) {
                     if (cx(d) != 0) fx = true;
                     if (cy(d) != 0) fy = true;
@@ -1060,7 +1066,7 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
             else v = a + (q * (f + fi) - T(6) * w * vbc) / (T(1) + q);
             // getInterpolatedDistributionForVeloWithPressureBC (KernelUtilities.h:207-213):
             // the entering population carries the bulk pressure
-            if (KIND == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED || KIND == VFX_BC_PRECURSOR_VELOCITY) v = v - w * drho;
+            if (kd == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED || kd == VFX_BC_PRECURSOR_VELOCITY) v = v - w * drho;
         }
         const int e = inv(d);
         // the precursor inlet writes with the parity's own pointers
@@ -1069,6 +1075,26 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
         constexpr bool WRITE_EVEN = KIND == VFX_BC_PRECURSOR_VELOCITY ? EVEN : !EVEN;
         Storage<S, T>::st(plane<S>(L, slot(e, WRITE_EVEN)), kk[nbr(e)], v, L.st[weight_class(e)], L.ss[e]);
     }
+}
+
+template <typename S, bool EVEN, int KIND>
+__global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
+{
+    using T = typename Compute<S>::T;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= B.n) return;
+    uint32_t kk[8];
+    const uint32_t k = B.k[i];
+    if (k >= L.N) return;
+    if (B.direct) nbrs_direct(L, k, kk);
+    else nbrs_sparse(L, k, kk);
+    T p[27]; // post-collision population of direction d
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        const int e = inv(d);
+        p[d] = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)], L.sh[d]);
+    }
+    interp_links<S, T, EVEN, KIND>(L, B, i, k, kk, p);
 }
 
 // Precursor distributions (PrecursorDistributions.cu:44-185): post-collision,
@@ -1104,9 +1130,44 @@ __global__ void __launch_bounds__(256) k_bc_precursor_dist(Lat L, BCDev B)
 // direction not pointing out of the domain (c_d . normal <= 0).
 template <typename T> __device__ __forceinline__ T feq_shifted(int d, T drho, T vx1, T vx2, T vx3, T cusq)
 {
+#pragma clang fp contract(off)
     if (d == d000) return T(8. / 27.) * (drho - (drho + T(1)) * cusq);
     const T cu = proj<T>(d, vx1, vx2, vx3);
     return weight<T>(d) * (drho + (drho + T(1)) * (T(3) * cu + T(4.5) * cu * cu - cusq));
+}
+
+// PressureNonEquilibriumCompressible.cu: the inward neighbour's populations f
+// (by direction) with their equilibrium replaced by the one of the prescribed
+// density rho_bc, in place.
+template <typename T> __device__ __forceinline__ void pressure_values(T (&f)[27], T rho_bc)
+{
+    // no FMA contraction: the same bits wherever the body is inlined
+#pragma clang fp contract(off)
+    // getCompressibleMacroscopicValues (MacroscopicQuantities.h:117-127)
+    const T corners = ((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]));
+    const T edges = (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P]))) +
+                    ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]));
+    const T faces = ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0])) + (f[d00M] + f[d00P]);
+    T drho1 = ((corners + edges) + f[d000]) + faces;
+    T vx1 = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dPMM] - f[dMPP]) + (f[dPPM] - f[dMMP]))) +
+            (((f[dP0M] - f[dM0P]) + (f[dP0P] - f[dM0M])) + ((f[dPM0] - f[dMP0]) + (f[dPP0] - f[dMM0]))) +
+            (f[dP00] - f[dM00]);
+    T vx2 = (((f[dPPP] - f[dMMM]) + (f[dMPM] - f[dPMP])) + ((f[dMPP] - f[dPMM]) + (f[dPPM] - f[dMMP]))) +
+            (((f[d0PM] - f[d0MP]) + (f[d0PP] - f[d0MM])) + ((f[dMP0] - f[dPM0]) + (f[dPP0] - f[dMM0]))) +
+            (f[d0P0] - f[d0M0]);
+    T vx3 = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dMPP] - f[dPMM]) + (f[dMMP] - f[dPPM]))) +
+            (((f[d0MP] - f[d0PM]) + (f[d0PP] - f[d0MM])) + ((f[dM0P] - f[dP0M]) + (f[dP0P] - f[dM0M]))) +
+            (f[d00P] - f[d00M]);
+    const T oor = T(1) / (drho1 + T(1));
+    vx1 *= oor;
+    vx2 *= oor;
+    vx3 *= oor;
+    const T cusq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3);
+#pragma unroll
+    for (int d = 0; d < 27; ++d) f[d] -= feq_shifted<T>(d, drho1, vx1, vx2, vx3, cusq);
+    drho1 = rho_bc;
+#pragma unroll
+    for (int d = 0; d < 27; ++d) f[d] += feq_shifted<T>(d, drho1, vx1, vx2, vx3, cusq);
 }
 
 template <typename S, bool EVEN>
@@ -1130,31 +1191,7 @@ __global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
     T f[27];
 #pragma unroll
     for (int s = 0; s < 27; ++s) f[dir_at(s)] = m[s];
-    // getCompressibleMacroscopicValues (MacroscopicQuantities.h:117-127)
-    const T corners = ((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]));
-    const T edges = (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P]))) +
-                    ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]));
-    const T faces = ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0])) + (f[d00M] + f[d00P]);
-    T drho1 = ((corners + edges) + f[d000]) + faces;
-    T vx1 = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dPMM] - f[dMPP]) + (f[dPPM] - f[dMMP]))) +
-            (((f[dP0M] - f[dM0P]) + (f[dP0P] - f[dM0M])) + ((f[dPM0] - f[dMP0]) + (f[dPP0] - f[dMM0]))) +
-            (f[dP00] - f[dM00]);
-    T vx2 = (((f[dPPP] - f[dMMM]) + (f[dMPM] - f[dPMP])) + ((f[dMPP] - f[dPMM]) + (f[dPPM] - f[dMMP]))) +
-            (((f[d0PM] - f[d0MP]) + (f[d0PP] - f[d0MM])) + ((f[dMP0] - f[dPM0]) + (f[dPP0] - f[dMM0]))) +
-            (f[d0P0] - f[d0M0]);
-    T vx3 = (((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dMPP] - f[dPMM]) + (f[dMMP] - f[dPPM]))) +
-            (((f[d0MP] - f[d0PM]) + (f[d0PP] - f[d0MM])) + ((f[dM0P] - f[dP0M]) + (f[dP0P] - f[dM0M]))) +
-            (f[d00P] - f[d00M]);
-    const T oor = T(1) / (drho1 + T(1));
-    vx1 *= oor;
-    vx2 *= oor;
-    vx3 *= oor;
-    const T cusq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3);
-#pragma unroll
-    for (int d = 0; d < 27; ++d) f[d] -= feq_shifted<T>(d, drho1, vx1, vx2, vx3, cusq);
-    drho1 = bc_at<T>(B.rho, i);
-#pragma unroll
-    for (int d = 0; d < 27; ++d) f[d] += feq_shifted<T>(d, drho1, vx1, vx2, vx3, cusq);
+    pressure_values<T>(f, bc_at<T>(B.rho, i));
     const int nx = cx(B.normal), ny = cy(B.normal), nz = cz(B.normal);
 #pragma unroll
     for (int d = 0; d < 27; ++d) {
@@ -1162,6 +1199,7 @@ __global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
         Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], f[d], L.st[weight_class(d)], L.ss[d]);
     }
 }
+
 
 // Wall-model (stress) boundaries (BoundaryConditions/Stress/*.cu with the
 // instantaneous wall model iMEM, iMEM.cuh:43-141): post-collision, parity EVEN.
@@ -1392,6 +1430,149 @@ __global__ void __launch_bounds__(256) k_bc_outflow(Lat L, BCDev B)
         T v = fn[d] * cs + (T(1) - cs) * f[d];
         if constexpr (CORR) v = v - weight<T>(d) * rc;
         Storage<S, T>::st(plane<S>(L, slot(d, !EVEN)), kk[nbr(d)], v, L.st[weight_class(d)], L.ss[d]);
+    }
+}
+
+// ---------------------------------------------- fused boundary conditions ----
+// Boundary conditions applied inside the collide-stream kernel of a structured
+// (direct) lattice, so that a boundary node's 27 populations are not read a
+// second time by a kernel of its own.  On a face normal to x every element a
+// boundary node touches sits in its own cache line (x is the fastest index),
+// so a separate boundary kernel moves ~64 times the bytes it uses.
+//  * post-collision interpolated kinds (1 no-slip, 2 velocity, 3 slip, 5
+//    velocity with pressure): the thread that collided node k computes the
+//    re-entering populations from its post-collision values and stores them,
+//    as k_bc_interp would right after the kernel (the element written belongs
+//    to the wall node, which no thread of the launch touches);
+//  * the non-equilibrium pressure boundary on a face normal to +-x
+//    (PressureNonEquilibriumCompressible, pre-collision): node k's incoming
+//    populations are replaced at the start of the next step from its inward
+//    neighbour k -+ 1, whose pre-collision populations the neighbouring lane
+//    of the same wavefront has just gathered (cross-lane shuffle), instead of
+//    by a kernel after the parity flip.  The values are the ones that kernel
+//    would store and the collision read back (the storage round trip is
+//    applied), so fused and separate runs agree bit for bit.
+// Node k belongs to a group iff bit k of `mask` is set; its entry is
+// base[k/32] + the number of set bits below it in its word (entries sorted by
+// node).
+struct FusedGroup {
+    const uint32_t* mask;
+    const uint32_t* base;
+    BCDev B;
+    int32_t kind;
+};
+
+__device__ __forceinline__ bool in_group(const FusedGroup& G, uint32_t k)
+{
+    return G.mask && ((G.mask[k >> 5] >> (k & 31u)) & 1u);
+}
+
+__device__ __forceinline__ bool fused_entry(const FusedGroup& G, uint32_t k, uint32_t& i)
+{
+    const uint32_t w = G.mask[k >> 5];
+    const uint32_t b = k & 31u;
+    if (!((w >> b) & 1u)) return false;
+    i = G.base[k >> 5] + (uint32_t)__builtin_popcount(w & ((1u << b) - 1u));
+    return i < G.B.n && G.B.k[i] == k; // malformed tables select nothing
+}
+
+// what a store of v as direction d followed by a load gives back (the value
+// the separate boundary kernels exchange through memory)
+template <typename S, typename T> __device__ __forceinline__ T round_trip(T v, float st, float ss, float ld, float sh)
+{
+    if constexpr (sizeof(S) == 2) return Storage<S, T>::dec2(Storage<S, T>::enc2(v, st, ss), ld, sh);
+    else return v;
+}
+
+template <typename S, typename T, bool EVEN>
+__device__ __forceinline__ void fused_post(const Lat& L, const FusedGroup& G, uint32_t k)
+{
+    uint32_t i;
+    if (!fused_entry(G, k, i)) return;
+    // The neighbour indices are recomputed from an index opaque to the
+    // compiler: otherwise it keeps them, or the 64-bit element addresses of
+    // the scatter, for the loads below and spills them on every node.
+    asm volatile("" : "+v"(k));
+    uint32_t kb[8];
+    nbrs_direct(L, k, kb);
+    // the node's post-collision populations, read back from the elements this
+    // thread has just stored them to (no other node writes those elements in
+    // the step): keeping the 27 collision outputs in registers for this rarely
+    // taken path would spill them on every node
+    T p[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        const int e = inv(d);
+        p[d] = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kb[nbr(e)], L.ld[weight_class(d)], L.sh[d]);
+    }
+    interp_links<S, T, EVEN, -1>(L, G.B, i, kb[0], kb, p, G.kind);
+}
+
+// K17 (default operation order of the storage form) with up to two fused
+// post-collision groups and one fused pressure group; apply_pre = 0 on the
+// first step of a run (no pressure boundary ran before it).
+// The boundary paths need more registers than the collision; the occupancy of
+// the plain kernel (7 waves per SIMD at <= 72 VGPRs for 2- and 4-byte storage)
+// is kept by capping the register budget, so that those rarely taken paths
+// spill instead.
+template <typename S> constexpr int fused_waves() { return sizeof(S) == 8 ? 3 : 7; }
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves<S>())))
+    k_step_fused(Lat L, Rates<typename Compute<S>::T> R, FusedGroup g0, FusedGroup g1, FusedGroup gp, int32_t apply_pre)
+{
+    using T = typename Compute<S>::T;
+    constexpr bool FAST = sizeof(S) != 8;
+    constexpr bool O2 = sizeof(S) == 2;
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, xcd_run<S>());
+    const uint32_t row = blk / L.chunks;
+    const uint32_t x = (blk - row * L.chunks) * blockDim.x + threadIdx.x;
+    const uint32_t y = row % L.ny, z = row / L.ny;
+    uint32_t kk[8];
+    nbrs_direct_xyz(L, x < L.nx ? x : L.nx - 1, y, z, kk);
+    const bool active = x < L.nx && !is_solid(L, kk[0]);
+    T m[27], mv[4], nut = T(0);
+#pragma unroll
+    for (int s = 0; s < 27; ++s) m[s] = T(0);
+    if (active) gather<S, T, EVEN>(L, kk, m);
+    if (gp.mask && apply_pre) {
+        uint32_t ip = 0;
+        const bool pl = active && fused_entry(gp, kk[0], ip);
+        if (__ballot(pl)) {
+            // inward neighbour k - 1 (outward normal +x) or k + 1 (-x): the lane next to this one
+            const int lane = (int)(threadIdx.x & 63u);
+            const int src = gp.B.normal == dP00 ? lane - 1 : lane + 1;
+            T f[27];
+#pragma unroll
+            for (int s = 0; s < 27; ++s) f[dir_at(s)] = __shfl(m[s], src & 63, 64);
+            const uint32_t kn = gp.B.normal == dP00 ? kk[0] - 1u : kk[0] + 1u;
+            const bool row = gp.B.normal == dP00 ? x >= 1u : x + 1u < L.nx; // neighbour in this row
+            if (pl && row && src >= 0 && src < 64 && gp.B.kn[ip] == kn) {
+                pressure_values<T>(f, bc_at<T>(gp.B.rho, ip));
+                const int ncx = cx(gp.B.normal);
+#pragma unroll
+                for (int s = 0; s < 27; ++s) {
+                    const int d = dir_at(s);
+                    if (cx(d) * ncx > 0) continue;
+                    const int c = weight_class(d);
+                    m[s] = round_trip<S, T>(f[d], L.st[c], L.ss[d], L.ld[c], L.sh[d]);
+                }
+            }
+        }
+    }
+    if (!active) return;
+    collide<T, VFX_K17, FAST, O2>(m, R, mv, nut);
+    scatter_post<S, T, EVEN, O2>(L, kk, m);
+    // the boundary code reads the stores above back from memory
+    asm volatile("" ::: "memory");
+    // only nodes of a group go on (whatever the boundary code needs is set up
+    // behind this branch, not on every node)
+    const uint32_t k = kk[0];
+    if (!(in_group(g0, k) || in_group(g1, k))) return;
+    // one inlined copy of the boundary code serves both groups
+#pragma unroll 1
+    for (int j = 0; j < 2; ++j) {
+        const FusedGroup& G = j == 0 ? g0 : g1;
+        if (G.mask) fused_post<S, T, EVEN>(L, G, k);
     }
 }
 
@@ -1743,6 +1924,23 @@ template <typename S> hipError_t launch_step(const Lat& L, int direct, int coll,
                 : launch_step_t<S, VFX_BGK, false, false>(L, direct, R, st);
 }
 
+// K17 collide-and-stream of a structured lattice with fused boundary groups
+// (k_step_fused); same grid as the direct k_step_direct launch
+template <typename S>
+hipError_t launch_step_fused(const Lat& L, bool even, double omega, const double* lim, const double* force,
+                             const FusedGroup& g0, const FusedGroup& g1, const FusedGroup& gp, int apply_pre,
+                             hipStream_t st)
+{
+    using T = typename Compute<S>::T;
+    const Rates<T> R = make_rates<T>(omega, lim, force, 0.0);
+    const unsigned grid = L.chunks * L.ny * L.nz;
+    const unsigned block = (unsigned)((L.nx + L.chunks - 1) / L.chunks);
+    const unsigned b = block <= 64 ? 64 : block <= 128 ? 128 : 256;
+    if (even) hipLaunchKernelGGL((k_step_fused<S, true>), dim3(grid), dim3(b), 0, st, L, R, g0, g1, gp, apply_pre);
+    else hipLaunchKernelGGL((k_step_fused<S, false>), dim3(grid), dim3(b), 0, st, L, R, g0, g1, gp, apply_pre);
+    return hipGetLastError();
+}
+
 template <typename S> hipError_t launch_turb_amd(const Lat& L, int direct, double sgs, hipStream_t st)
 {
     using T = typename Compute<S>::T;
@@ -1945,6 +2143,8 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
 #define VFX_INSTANTIATE_LAUNCHERS(S)                                                                                   \
     template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*, int,      \
                                        double, hipStream_t);                                                           \
+    template hipError_t launch_step_fused<S>(const Lat&, bool, double, const double*, const double*, const FusedGroup&, \
+                                             const FusedGroup&, const FusedGroup&, int, hipStream_t);                  \
     template hipError_t launch_turb_amd<S>(const Lat&, int, double, hipStream_t);                                      \
     template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,             \
                                        const double*, hipStream_t);                                                    \
@@ -1963,6 +2163,9 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
 #define VFX_EXTERN_LAUNCHERS(S)                                                                                        \
     extern template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*,   \
                                               int, double, hipStream_t);                                               \
+    extern template hipError_t launch_step_fused<S>(const Lat&, bool, double, const double*, const double*,            \
+                                                    const FusedGroup&, const FusedGroup&, const FusedGroup&, int,      \
+                                                    hipStream_t);                                                      \
     extern template hipError_t launch_turb_amd<S>(const Lat&, int, double, hipStream_t);                               \
     extern template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,      \
                                               const double*, hipStream_t);                                             \

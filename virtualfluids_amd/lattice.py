@@ -79,6 +79,7 @@ class Lattice:
         self.reference_velocity = tuple(float(x) for x in reference_velocity)
         self.direct = neighbors is None
         self.solid_mask = None
+        self.solid_host = None  # bool per node (structured lattice with a solid mask)
         self.nb = None
         self.fluid = None
         self.types = None
@@ -94,6 +95,7 @@ class Lattice:
                 if solid.size != self.n_nodes:
                     raise ValueError("solid mask size does not match dims")
                 self.solid_mask = torch.from_numpy(pack_mask(solid).view(np.int32)).to(self.device)
+                self.solid_host = solid.copy()
                 self.n_fluid = int((~solid).sum())
             else:
                 self.n_fluid = self.n_nodes
@@ -257,6 +259,23 @@ class Lattice:
         b = bc.struct(self._macro_dtype())
         V.check(V.lib().vfx_bc_apply(C.byref(s), C.byref(b), int(bool(even)), _stream(stream)), "vfx_bc_apply")
 
+    def k17_step_fused(self, even, post=(), pre=None, apply_pre=False, stream=None):
+        """K17 collide+stream with boundary groups applied inside the kernel
+        (vfx_k17_step_fused): up to two post-collision FusedGroups and one
+        pressure FusedGroup whose values the step takes from the previous
+        step's populations (apply_pre False on the first step of a run)."""
+        real = self._macro_dtype()
+        for g in list(post) + ([pre] if pre is not None else []):
+            if g.bc.max_node >= self.n_nodes or g.n_nodes != self.n_nodes:
+                raise ValueError("fused group does not belong to this lattice")
+        s = self.struct()
+        ps = [g.struct(real) for g in post]
+        arr = (V.VfxFusedBC * len(ps))(*ps) if ps else None
+        pr = pre.struct(real) if pre is not None else None
+        V.check(V.lib().vfx_k17_step_fused(C.byref(s), int(bool(even)), arr, len(ps),
+                                           C.byref(pr) if pr is not None else None, int(bool(apply_pre)),
+                                           _stream(stream)), "vfx_k17_step_fused")
+
     def read_populations(self, even=True, stream=None) -> torch.Tensor:
         out = torch.zeros(self.n_nodes, 27, dtype=torch.float64, device=self.device)
         s = self.struct()
@@ -408,6 +427,8 @@ class BoundaryCondition:
                  precursor=None):
         nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
         n = nodes.size
+        self.nodes = nodes
+        self.kn_host = None if neighbors is None else np.ascontiguousarray(neighbors, dtype=np.uint32).ravel()
         self.kind = int(kind)
         self.precursor = precursor
         if self.kind in self.PRECURSOR:
@@ -478,6 +499,19 @@ class BoundaryCondition:
     @property
     def pre_collision(self) -> bool:
         return self.kind in self.PRE_COLLISION
+
+    def subset(self, idx) -> "BoundaryCondition":
+        """The entries idx (in that order) as a boundary of their own (interpolated
+        and pressure kinds: no wall-model state, no precursor planes)."""
+        if self.wm is not None or self.precursor is not None:
+            raise ValueError("subset() covers boundaries without wall-model or precursor state")
+        idx = np.asarray(idx, dtype=np.int64)
+        h = self._host
+        pick = (lambda a: None if a is None else a[idx])
+        return BoundaryCondition(self.kind, self.nodes[idx], h["q"][:, idx], pick(h["vx"]), pick(h["vy"]),
+                                 pick(h["vz"]), cpu_float_velocity=self.cpu_float_velocity, rho=pick(h["rho"]),
+                                 neighbors=None if self.kn_host is None else self.kn_host[idx], normal=self.normal,
+                                 device=self.device, pressure_correction_factor=self.pressure_correction_factor)
 
     def tables(self, real=torch.float64) -> dict:
         """Device q27 (27, n), vx, vy, vz, rho (n) in the arithmetic type `real`."""
@@ -602,6 +636,153 @@ class PrecursorPlanes:
         return self.time_ratio
 
 
+FUSED_POST_KINDS = (V.VFX_BC_NOSLIP_INTERPOLATED, V.VFX_BC_VELOCITY_INTERPOLATED, V.VFX_BC_SLIP,
+                    V.VFX_BC_VELOCITY_PRESSURE_INTERPOLATED)
+
+
+class FusedGroup:
+    """A boundary whose entries the collide-stream kernel applies itself
+    (vfx_fused_bc): entries sorted by node, one per node; node_mask has bit k
+    set for every node k of the group, word_base[w] counts the entries of the
+    nodes below 32 w."""
+
+    def __init__(self, bc: BoundaryCondition, n_nodes: int):
+        nodes = bc.nodes.astype(np.int64)
+        if nodes.size and (np.any(np.diff(nodes) <= 0) or nodes[-1] >= n_nodes):
+            raise ValueError("a fused group needs distinct nodes in increasing order inside the lattice")
+        nw = (n_nodes + 31) // 32
+        words = nodes >> 5
+        mask = np.zeros(nw, np.uint32)
+        np.bitwise_or.at(mask, words, np.left_shift(np.uint32(1), (nodes & 31).astype(np.uint32)))
+        base = np.zeros(nw, np.uint32)
+        base[1:] = np.cumsum(np.bincount(words, minlength=nw))[:-1]
+        self.bc = bc
+        self.n_nodes = n_nodes
+        self.node_mask = torch.from_numpy(mask.view(np.int32)).to(bc.device)
+        self.word_base = torch.from_numpy(base.view(np.int32)).to(bc.device)
+        self._keep = None
+
+    def struct(self, real=torch.float64) -> V.VfxFusedBC:
+        self._keep = self.bc.struct(real)
+        f = V.VfxFusedBC()
+        f.bc = C.pointer(self._keep)
+        f.node_mask = self.node_mask.data_ptr()
+        f.word_base = self.word_base.data_ptr()
+        return f
+
+
+def _xyz(lattice, k):
+    nx, ny, _ = lattice.dims
+    k = np.asarray(k, dtype=np.int64)
+    return k % nx, (k // nx) % ny, k // (nx * ny)
+
+
+def _unique_nodes(nodes) -> np.ndarray:
+    """True for the entries whose node appears once in the table."""
+    _, inv, cnt = np.unique(nodes, return_inverse=True, return_counts=True)
+    return cnt[inv] == 1
+
+
+def fusable_post_entries(lattice, bc) -> np.ndarray:
+    """Entries of an interpolated boundary the collide-stream kernel may apply
+    itself: the node is fluid, appears once, and the far node of every wall
+    link (0 <= q <= 1, in the storage real's rounding too) is solid, so that no
+    other thread of the launch touches the elements the boundary writes or
+    the node's own post-collision elements it reads back."""
+    if lattice.solid_host is None or bc.n == 0:
+        return np.zeros(bc.n, dtype=bool)
+    nx, ny, nz = lattice.dims
+    x, y, z = _xyz(lattice, bc.nodes)
+    q64 = bc._host["q"]
+    q32 = q64.astype(np.float32)
+    ok = _unique_nodes(bc.nodes) & ~lattice.solid_host[bc.nodes]
+    for d in range(1, 27):
+        link = ((q64[d] >= 0) & (q64[d] <= 1)) | ((q32[d] >= 0) & (q32[d] <= 1))
+        if not link.any():
+            continue
+        far = (x + CX[d]) % nx + nx * ((y + CY[d]) % ny + ny * ((z + CZ[d]) % nz))
+        ok &= ~link | lattice.solid_host[far]
+    return ok
+
+
+def fusable_pressure_entries(lattice, bc) -> np.ndarray:
+    """Entries of a non-equilibrium pressure boundary on a face normal to +-x
+    the kernel may apply itself: fluid node appearing once, inward neighbour
+    k -+ 1 in the same row and the same 64-node wavefront, fluid and no node of
+    the boundary.  All False when any neighbour is a boundary node."""
+    n = bc.n
+    if bc.kind != V.VFX_BC_PRESSURE_NEQ or bc.normal not in (1, 2) or n == 0:
+        return np.zeros(n, dtype=bool)
+    k = bc.nodes.astype(np.int64)
+    kn = bc.kn_host.astype(np.int64)
+    if np.isin(kn, k).any():
+        return np.zeros(n, dtype=bool)
+    nx = lattice.dims[0]
+    x = k % nx
+    if bc.normal == 1:
+        ok = (kn == k - 1) & (x >= 1) & (x % 64 != 0)
+    else:
+        ok = (kn == k + 1) & (x + 1 < nx) & (x % 64 != 63)
+    ok &= _unique_nodes(k) & (kn < lattice.n_nodes) & (k < lattice.n_nodes)
+    if lattice.solid_host is not None:
+        ok &= ~lattice.solid_host[np.minimum(k, lattice.n_nodes - 1)]
+        ok &= ~lattice.solid_host[np.minimum(kn, lattice.n_nodes - 1)]
+    return ok
+
+
+@dataclass
+class FusedPlan:
+    """Which boundary entries a fused step applies inside the collide-stream
+    kernel (post: up to two groups, pre: one pressure group) and which run as
+    boundary kernels of their own (residual_post / residual_pre, in the order
+    of the unfused step)."""
+
+    post: list
+    pre: FusedGroup | None
+    residual_post: list
+    residual_pre: list
+
+    @staticmethod
+    def build(lattice, bcs) -> "FusedPlan":
+        def split(b, ok):
+            sel = np.flatnonzero(ok)
+            sel = sel[np.argsort(b.nodes[sel], kind="stable")]
+            rest = np.flatnonzero(~ok)
+            return b.subset(sel), (b.subset(rest) if rest.size else None)
+
+        post_all = [b for b in bcs if not b.pre_collision and b.kind not in BoundaryCondition.PRECURSOR]
+        prec = [b for b in bcs if b.kind in BoundaryCondition.PRECURSOR]
+        pre_all = [b for b in bcs if b.pre_collision]
+        # the two interpolated boundaries with the most fusable entries
+        cand = []
+        for j, b in enumerate(post_all):
+            if b.kind in FUSED_POST_KINDS and b.wm is None:
+                ok = fusable_post_entries(lattice, b)
+                if ok.any():
+                    cand.append((int(ok.sum()), j, ok))
+        chosen = {j: ok for _, j, ok in sorted(cand, key=lambda c: (-c[0], c[1]))[:2]}
+        post, residual_post = [], []
+        for j, b in enumerate(post_all):
+            if j in chosen:
+                fused, rest = split(b, chosen[j])
+                post.append(FusedGroup(fused, lattice.n_nodes))
+                if rest is not None:
+                    residual_post.append(rest)
+            else:
+                residual_post.append(b)
+        residual_post += prec
+        pre, residual_pre = None, list(pre_all)
+        # the pressure group is fused only as the sole pre-collision boundary
+        if len(pre_all) == 1:
+            b = pre_all[0]
+            ok = fusable_pressure_entries(lattice, b)
+            if ok.any():
+                fused, rest = split(b, ok)
+                pre = FusedGroup(fused, lattice.n_nodes)
+                residual_pre = [rest] if rest is not None else []
+        return FusedPlan(post, pre, residual_post, residual_pre)
+
+
 @dataclass
 class Stepper:
     """The per-level time step of UpdateGrid27::updateGrid without refinement
@@ -616,8 +797,49 @@ class Stepper:
     even: bool = True
     apply_body_force: bool = False  # CollisionTemplate ApplyBodyForce (Lattice.set_body_force)
     t: int = 1  # time step of the next step; precursor planes with a reader cycle on it
+    # Apply the interpolated and pressure boundaries inside the collide-stream
+    # kernel where the geometry allows (FusedPlan); same populations as the
+    # separate boundary kernels.
+    fuse: bool = False
+    _plan: FusedPlan | None = field(default=None, init=False, repr=False)
+    _plan_key: tuple | None = field(default=None, init=False, repr=False)
+
+    def fused_plan(self) -> FusedPlan:
+        lat = self.lattice
+        if not lat.direct or self.collision != "k17" or lat.variant != "default" or \
+                getattr(lat, "turbulence", "none") != "none" or self.apply_body_force:
+            raise ValueError("fused boundaries need a structured lattice stepped by the default K17 "
+                             "without turbulence model or body-force arrays")
+        if any(getattr(b, "wm", None) is not None for b in self.bcs):
+            raise ValueError("fused boundaries do not run with wall models")
+        key = (id(lat), tuple(id(b) for b in self.bcs))
+        if self._plan_key != key:
+            self._plan = FusedPlan.build(lat, self.bcs)
+            self._plan_key = key
+        return self._plan
+
+    def _step_fused(self, n, stream):
+        plan = self.fused_plan()
+        lat = self.lattice
+        for i in range(n):
+            # the pressure group of the previous step of this run is applied
+            # while the node's populations are gathered
+            lat.k17_step_fused(self.even, plan.post, plan.pre, apply_pre=i > 0, stream=stream)
+            for bc in plan.residual_post:
+                if bc.precursor is not None and bc.precursor.reader is not None:
+                    bc.precursor.advance(self.t, stream)
+                lat.apply_bc(bc, self.even, stream=stream)
+            self.t += 1
+            self.even = not self.even
+            for bc in plan.residual_pre:
+                lat.apply_bc(bc, self.even, stream=stream)
+        # leave the populations as the separate kernels would
+        if n > 0 and plan.pre is not None:
+            lat.apply_bc(plan.pre.bc, self.even, stream=stream)
 
     def step(self, n=1, stream=None):
+        if self.fuse:
+            return self._step_fused(n, stream)
         # precursor inflows run last among the post-collision conditions (UpdateGrid27.cpp:252)
         post = [b for b in self.bcs if not b.pre_collision and b.kind not in BoundaryCondition.PRECURSOR] + \
                [b for b in self.bcs if b.kind in BoundaryCondition.PRECURSOR]
