@@ -382,9 +382,11 @@ def channel_leg(torch, vfx, steps, sample_every=10, forms=("fp64", "fp32", "fp16
     ray-sphere q), K17 at Re = 200 on the mean sphere diameter, started from a
     uniform stream (FP16: populations stored as deviations from the inflow
     equilibrium); `steps` steps with FP64 running mean/Reynolds-stress sums
-    every `sample_every` steps.  The boundary conditions run inside the
-    collide-stream kernel (Stepper(fuse=True)); `separate_steps` FP16 steps
-    with the separate boundary kernels are timed beside it.  Reports
+    every `sample_every` steps.  The pressure outlet runs inside the
+    collide-stream kernel (Stepper(fuse=True, fuse_post=False)), the
+    interpolated groups as kernels of their own; `separate_steps` FP16 steps
+    with every group separate, and with every group fused, are timed beside
+    it.  Reports
     whole-step MLUPS (fluid nodes, all boundary kernels and statistics
     included), the time-averaged mean velocity error of FP32/FP16 storage vs
     FP64, and the FP64 mass/momentum monitor."""
@@ -407,7 +409,7 @@ def channel_leg(torch, vfx, steps, sample_every=10, forms=("fp64", "fp32", "fp16
         lat = vfx.Lattice(ch.dims, storage=storage, omega=ch.omega, solid=ch.solid, device="cuda",
                           reference_velocity=uref)
         lat.init_equilibrium(zero, u0, zero, zero)
-        st = vfx.Stepper(lat, "k17", bcs, fuse=True)
+        st = vfx.Stepper(lat, "k17", bcs, fuse=True, fuse_post=False)
         plan = st.fused_plan()
         stats = vfx.Statistics(lat)
         mm0 = lat.mass_momentum(st.even).cpu().numpy()
@@ -426,14 +428,16 @@ def channel_leg(torch, vfx, steps, sample_every=10, forms=("fp64", "fp32", "fp16
              "fused_entries": {str(g.bc.kind): g.bc.n for g in plan.post + ([plan.pre] if plan.pre else [])},
              "separate_kernels": len(plan.residual_post) + len(plan.residual_pre)}
         if storage == "fp16" and separate_steps > 0:
-            # the same channel with one kernel per boundary group
-            su = vfx.Stepper(lat, "k17", bcs)
-            su.step(20)
-            ms_u = _timed_loop(torch, stream, separate_steps // 10, lambda i: su.step(10))
-            r["separate_bc_kernels"] = {"steps": separate_steps // 10 * 10,
-                                        "ms_per_step": round(ms_u / (separate_steps // 10 * 10), 4),
-                                        "mlups": round(ch.n_fluid * (separate_steps // 10 * 10) / (ms_u * 1e-3) / 1e6, 1)}
-            del su
+            # the same channel with one kernel per boundary group, and with
+            # every group inside the collide-stream kernel
+            n_sep = separate_steps // 10 * 10
+            for name, kw in (("separate_bc_kernels", {}), ("all_groups_fused", {"fuse": True})):
+                su = vfx.Stepper(lat, "k17", bcs, **kw)
+                su.step(20)
+                ms_u = _timed_loop(torch, stream, n_sep // 10, lambda i: su.step(10))
+                r[name] = {"steps": n_sep, "ms_per_step": round(ms_u / n_sep, 4),
+                           "mlups": round(ch.n_fluid * n_sep / (ms_u * 1e-3) / 1e6, 1)}
+                del su
         if ref_mean is None:
             ref_mean = mean
         else:

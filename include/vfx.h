@@ -284,10 +284,18 @@ typedef struct vfx_fused_bc {
  *    0 on the first step of a run, before which no pressure boundary ran).
  *    Until the next step the boundary nodes then hold the un-replaced
  *    populations: vfx_bc_apply(pre->bc) with the new parity gives the state
- *    the separate launches leave, before it is read.
+ *    the separate launches leave, before it is read;
+ *  - any_mask (or NULL): ceil(n_nodes/32) words, bit k set iff node k is an
+ *    entry of some group above; loaded with the populations, it spares the
+ *    other nodes the look-up in each group's mask.
+ * A post group entry must be a fluid node whose every wall link (0 <= q <= 1)
+ * ends in a solid node: the kernel writes the re-entering populations into
+ * elements only that node would touch, and reads the boundary node's own
+ * post-collision elements back, while the launch is running.
  * Results equal vfx_k17_step + vfx_bc_apply bit for bit in every storage form. */
 vfx_status vfx_k17_step_fused(const vfx_lattice* lat, int32_t is_even_timestep, const vfx_fused_bc* post,
-                              int32_t n_post, const vfx_fused_bc* pre, int32_t apply_pre, vfx_stream stream);
+                              int32_t n_post, const vfx_fused_bc* pre, const uint32_t* any_mask, int32_t apply_pre,
+                              vfx_stream stream);
 
 /* calcTurbulentViscosityAMD (src/gpu/core/PostProcessor/TurbulentViscosityKernels.cu:62-130):
  * the AMD eddy viscosity of every fluid node from vx/vy/vz into

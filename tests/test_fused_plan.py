@@ -149,3 +149,20 @@ def test_pressure_fused_only_as_the_sole_pre_collision_boundary():
     extra = vfx.BoundaryCondition(6, p.nodes[:2], neighbors=p.kn_host[:2], normal=1, device="cpu")
     plan = vfx.FusedPlan.build(lat, bcs + [extra])
     assert plan.pre is None and plan.residual_pre == [p, extra]
+
+
+def test_pressure_only_plan():
+    ch = small_channel()
+    lat = _lat(ch)
+    bcs = _bcs(ch)
+    plan = vfx.FusedPlan.build(lat, bcs, post=False)
+    assert plan.post == [] and plan.pre is not None
+    assert plan.residual_post == [b for b in bcs if not b.pre_collision]
+    u = plan.any_mask.numpy().view(np.uint32)
+    assert np.array_equal(u, plan.pre.node_mask.numpy().view(np.uint32))
+    full = vfx.FusedPlan.build(lat, bcs)
+    u = full.any_mask.numpy().view(np.uint32)
+    want = np.zeros_like(u)
+    for g in full.post + [full.pre]:
+        want |= g.node_mask.numpy().view(np.uint32)
+    assert np.array_equal(u, want)

@@ -27,12 +27,12 @@ def _lattice(dims, solid, omega, storage, u_in):
     return lat
 
 
-def _pair(dims, solid, omega, tables, storage, u_in, chunks):
+def _pair(dims, solid, omega, tables, storage, u_in, chunks, fuse_post=True):
     """The same run unfused (one call) and fused (calls of `chunks` steps)."""
     a = _lattice(dims, solid, omega, storage, u_in)
     b = _lattice(dims, solid, omega, storage, u_in)
     sa = vfx.Stepper(a, "k17", geometry.boundary_conditions(tables))
-    sb = vfx.Stepper(b, "k17", geometry.boundary_conditions(tables), fuse=True)
+    sb = vfx.Stepper(b, "k17", geometry.boundary_conditions(tables), fuse=True, fuse_post=fuse_post)
     sa.step(sum(chunks))
     for c in chunks:
         sb.step(c)
@@ -55,6 +55,16 @@ def test_channel_fused_equals_separate_kernels(cuda, storage):
     a, b, st = _pair(ch.dims, ch.solid, ch.omega, ch.bcs, storage, ch.u_in, (7, 1, 17))
     plan = st.fused_plan()
     assert len(plan.post) == 2 and plan.pre is not None and not plan.residual_post and not plan.residual_pre
+    assert _same(a, b)
+
+
+@pytest.mark.parametrize("storage", FORMS)
+def test_channel_pressure_only_fused(cuda, storage):
+    print(f"[fused channel, pressure group only, {storage}]", flush=True)
+    ch = small_channel()
+    a, b, st = _pair(ch.dims, ch.solid, ch.omega, ch.bcs, storage, ch.u_in, (6, 9), fuse_post=False)
+    plan = st.fused_plan()
+    assert plan.post == [] and plan.pre is not None and len(plan.residual_post) == 2
     assert _same(a, b)
 
 
@@ -179,5 +189,20 @@ def test_fused_rejects_unsupported_lattices(cuda):
     s = lat.struct()
     bad = vfx._lib.VfxFusedBC()
     with pytest.raises(vfx.VfxError):
-        vfx.check(vfx.lib().vfx_k17_step_fused(vfx._lib.C.byref(s), 1, vfx._lib.C.byref(bad), 1, None, 0, None),
+        vfx.check(vfx.lib().vfx_k17_step_fused(vfx._lib.C.byref(s), 1, vfx._lib.C.byref(bad), 1, None, None, 0,
+                                                 None),
                   "vfx_k17_step_fused")
+
+
+def test_without_union_mask(cuda):
+    """any_mask NULL: every node looks its groups up itself; same result."""
+    print("[fused channel without union mask fp16]", flush=True)
+    ch = small_channel()
+    a, b, st = _pair(ch.dims, ch.solid, ch.omega, ch.bcs, "fp16", ch.u_in, (5, 5))
+    c = _lattice(ch.dims, ch.solid, ch.omega, "fp16", ch.u_in)
+    sc = vfx.Stepper(c, "k17", geometry.boundary_conditions(ch.bcs), fuse=True)
+    sc.fused_plan().any_mask = None
+    sc.step(5)
+    sc.step(5)
+    torch.cuda.synchronize()
+    assert _same(a, c) and _same(b, c)

@@ -84,7 +84,7 @@ _SIGNATURES = {
     "vfx_k17_step": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_void_p],
     "vfx_bgk_step": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p],
     "vfx_k17_step_fused": [C.POINTER(VfxLattice), C.c_int32, C.POINTER(VfxFusedBC), C.c_int32,
-                           C.POINTER(VfxFusedBC), C.c_int32, C.c_void_p],
+                           C.POINTER(VfxFusedBC), C.c_void_p, C.c_int32, C.c_void_p],
     "vfx_bc_apply": [C.POINTER(VfxLattice), C.POINTER(VfxBC), C.c_int32, C.c_void_p],
     "vfx_init_equilibrium": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                              C.c_void_p],

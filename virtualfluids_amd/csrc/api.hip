@@ -307,7 +307,7 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
 }
 
 vfx_status vfx_k17_step_fused(const vfx_lattice* lat, int32_t is_even, const vfx_fused_bc* post, int32_t n_post,
-                              const vfx_fused_bc* pre, int32_t apply_pre, vfx_stream stream)
+                              const vfx_fused_bc* pre, const uint32_t* any_mask, int32_t apply_pre, vfx_stream stream)
 {
     vfx::Lat L;
     if (vfx_status s = make_lat(lat, L)) return s;
@@ -338,15 +338,16 @@ vfx_status vfx_k17_step_fused(const vfx_lattice* lat, int32_t is_even, const vfx
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     const int ap = apply_pre != 0;
+    const double* lim = lat->quadric_limiter;
     switch (lat->storage) {
     case VFX_FP64:
-        e = vfx::launch_step_fused<double>(L, is_even != 0, lat->omega, lat->quadric_limiter, lat->force, g[0], g[1], g[2], ap, st);
+        e = vfx::launch_step_fused<double>(L, is_even != 0, lat->omega, lim, lat->force, g[0], g[1], g[2], any_mask, ap, st);
         break;
     case VFX_FP32:
-        e = vfx::launch_step_fused<float>(L, is_even != 0, lat->omega, lat->quadric_limiter, lat->force, g[0], g[1], g[2], ap, st);
+        e = vfx::launch_step_fused<float>(L, is_even != 0, lat->omega, lim, lat->force, g[0], g[1], g[2], any_mask, ap, st);
         break;
     default:
-        e = vfx::launch_step_fused<_Float16>(L, is_even != 0, lat->omega, lat->quadric_limiter, lat->force, g[0], g[1], g[2], ap,
+        e = vfx::launch_step_fused<_Float16>(L, is_even != 0, lat->omega, lim, lat->force, g[0], g[1], g[2], any_mask, ap,
                                              st);
         break;
     }

@@ -1518,7 +1518,8 @@ __device__ __forceinline__ void fused_post(const Lat& L, const FusedGroup& G, ui
 template <typename S> constexpr int fused_waves() { return sizeof(S) == 8 ? 3 : 7; }
 template <typename S, bool EVEN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves<S>())))
-    k_step_fused(Lat L, Rates<typename Compute<S>::T> R, FusedGroup g0, FusedGroup g1, FusedGroup gp, int32_t apply_pre)
+    k_step_fused(Lat L, Rates<typename Compute<S>::T> R, FusedGroup g0, FusedGroup g1, FusedGroup gp,
+                 const uint32_t* any, int32_t apply_pre)
 {
     using T = typename Compute<S>::T;
     constexpr bool FAST = sizeof(S) != 8;
@@ -1530,13 +1531,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_
     uint32_t kk[8];
     nbrs_direct_xyz(L, x < L.nx ? x : L.nx - 1, y, z, kk);
     const bool active = x < L.nx && !is_solid(L, kk[0]);
+    // union of the groups' node masks, loaded with the populations: nodes of
+    // no group (most of them) take no further round trip to memory
+    const bool member = any == nullptr || ((any[kk[0] >> 5] >> (kk[0] & 31u)) & 1u);
     T m[27], mv[4], nut = T(0);
 #pragma unroll
     for (int s = 0; s < 27; ++s) m[s] = T(0);
     if (active) gather<S, T, EVEN>(L, kk, m);
-    if (gp.mask && apply_pre) {
+    if (gp.mask && apply_pre && __ballot(active && member)) {
         uint32_t ip = 0;
-        const bool pl = active && fused_entry(gp, kk[0], ip);
+        const bool pl = active && member && fused_entry(gp, kk[0], ip);
         if (__ballot(pl)) {
             // inward neighbour k - 1 (outward normal +x) or k + 1 (-x): the lane next to this one
             const int lane = (int)(threadIdx.x & 63u);
@@ -1567,7 +1571,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_
     // only nodes of a group go on (whatever the boundary code needs is set up
     // behind this branch, not on every node)
     const uint32_t k = kk[0];
-    if (!(in_group(g0, k) || in_group(g1, k))) return;
+    if (!member || !(in_group(g0, k) || in_group(g1, k))) return;
     // one inlined copy of the boundary code serves both groups
 #pragma unroll 1
     for (int j = 0; j < 2; ++j) {
@@ -1928,16 +1932,16 @@ template <typename S> hipError_t launch_step(const Lat& L, int direct, int coll,
 // (k_step_fused); same grid as the direct k_step_direct launch
 template <typename S>
 hipError_t launch_step_fused(const Lat& L, bool even, double omega, const double* lim, const double* force,
-                             const FusedGroup& g0, const FusedGroup& g1, const FusedGroup& gp, int apply_pre,
-                             hipStream_t st)
+                             const FusedGroup& g0, const FusedGroup& g1, const FusedGroup& gp, const uint32_t* any,
+                             int apply_pre, hipStream_t st)
 {
     using T = typename Compute<S>::T;
     const Rates<T> R = make_rates<T>(omega, lim, force, 0.0);
     const unsigned grid = L.chunks * L.ny * L.nz;
     const unsigned block = (unsigned)((L.nx + L.chunks - 1) / L.chunks);
     const unsigned b = block <= 64 ? 64 : block <= 128 ? 128 : 256;
-    if (even) hipLaunchKernelGGL((k_step_fused<S, true>), dim3(grid), dim3(b), 0, st, L, R, g0, g1, gp, apply_pre);
-    else hipLaunchKernelGGL((k_step_fused<S, false>), dim3(grid), dim3(b), 0, st, L, R, g0, g1, gp, apply_pre);
+    if (even) hipLaunchKernelGGL((k_step_fused<S, true>), dim3(grid), dim3(b), 0, st, L, R, g0, g1, gp, any, apply_pre);
+    else hipLaunchKernelGGL((k_step_fused<S, false>), dim3(grid), dim3(b), 0, st, L, R, g0, g1, gp, any, apply_pre);
     return hipGetLastError();
 }
 
@@ -2144,7 +2148,7 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*, int,      \
                                        double, hipStream_t);                                                           \
     template hipError_t launch_step_fused<S>(const Lat&, bool, double, const double*, const double*, const FusedGroup&, \
-                                             const FusedGroup&, const FusedGroup&, int, hipStream_t);                  \
+                                             const FusedGroup&, const FusedGroup&, const uint32_t*, int, hipStream_t); \
     template hipError_t launch_turb_amd<S>(const Lat&, int, double, hipStream_t);                                      \
     template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,             \
                                        const double*, hipStream_t);                                                    \
@@ -2164,8 +2168,8 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     extern template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*,   \
                                               int, double, hipStream_t);                                               \
     extern template hipError_t launch_step_fused<S>(const Lat&, bool, double, const double*, const double*,            \
-                                                    const FusedGroup&, const FusedGroup&, const FusedGroup&, int,      \
-                                                    hipStream_t);                                                      \
+                                                    const FusedGroup&, const FusedGroup&, const FusedGroup&,           \
+                                                    const uint32_t*, int, hipStream_t);                                \
     extern template hipError_t launch_turb_amd<S>(const Lat&, int, double, hipStream_t);                               \
     extern template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,      \
                                               const double*, hipStream_t);                                             \

@@ -259,11 +259,12 @@ class Lattice:
         b = bc.struct(self._macro_dtype())
         V.check(V.lib().vfx_bc_apply(C.byref(s), C.byref(b), int(bool(even)), _stream(stream)), "vfx_bc_apply")
 
-    def k17_step_fused(self, even, post=(), pre=None, apply_pre=False, stream=None):
+    def k17_step_fused(self, even, post=(), pre=None, apply_pre=False, any_mask=None, stream=None):
         """K17 collide+stream with boundary groups applied inside the kernel
         (vfx_k17_step_fused): up to two post-collision FusedGroups and one
         pressure FusedGroup whose values the step takes from the previous
-        step's populations (apply_pre False on the first step of a run)."""
+        step's populations (apply_pre False on the first step of a run);
+        any_mask: the union of the groups' node masks (FusedPlan.any_mask)."""
         real = self._macro_dtype()
         for g in list(post) + ([pre] if pre is not None else []):
             if g.bc.max_node >= self.n_nodes or g.n_nodes != self.n_nodes:
@@ -273,8 +274,8 @@ class Lattice:
         arr = (V.VfxFusedBC * len(ps))(*ps) if ps else None
         pr = pre.struct(real) if pre is not None else None
         V.check(V.lib().vfx_k17_step_fused(C.byref(s), int(bool(even)), arr, len(ps),
-                                           C.byref(pr) if pr is not None else None, int(bool(apply_pre)),
-                                           _stream(stream)), "vfx_k17_step_fused")
+                                           C.byref(pr) if pr is not None else None, _ptr(any_mask),
+                                           int(bool(apply_pre)), _stream(stream)), "vfx_k17_step_fused")
 
     def read_populations(self, even=True, stream=None) -> torch.Tensor:
         out = torch.zeros(self.n_nodes, 27, dtype=torch.float64, device=self.device)
@@ -741,9 +742,12 @@ class FusedPlan:
     pre: FusedGroup | None
     residual_post: list
     residual_pre: list
+    any_mask: torch.Tensor | None = None  # union of the groups' node masks
 
     @staticmethod
-    def build(lattice, bcs) -> "FusedPlan":
+    def build(lattice, bcs, post=True) -> "FusedPlan":
+        """post=False: only the pressure group is fused (the interpolated
+        kinds run as separate kernels)."""
         def split(b, ok):
             sel = np.flatnonzero(ok)
             sel = sel[np.argsort(b.nodes[sel], kind="stable")]
@@ -756,7 +760,7 @@ class FusedPlan:
         # the two interpolated boundaries with the most fusable entries
         cand = []
         for j, b in enumerate(post_all):
-            if b.kind in FUSED_POST_KINDS and b.wm is None:
+            if post and b.kind in FUSED_POST_KINDS and b.wm is None:
                 ok = fusable_post_entries(lattice, b)
                 if ok.any():
                     cand.append((int(ok.sum()), j, ok))
@@ -780,7 +784,14 @@ class FusedPlan:
                 fused, rest = split(b, ok)
                 pre = FusedGroup(fused, lattice.n_nodes)
                 residual_pre = [rest] if rest is not None else []
-        return FusedPlan(post, pre, residual_post, residual_pre)
+        groups = post + ([pre] if pre is not None else [])
+        any_mask = None
+        if groups:
+            u = groups[0].node_mask.clone()
+            for g in groups[1:]:
+                u |= g.node_mask
+            any_mask = u
+        return FusedPlan(post, pre, residual_post, residual_pre, any_mask)
 
 
 @dataclass
@@ -801,6 +812,11 @@ class Stepper:
     # kernel where the geometry allows (FusedPlan); same populations as the
     # separate boundary kernels.
     fuse: bool = False
+    # with fuse: also the interpolated kinds (False: the pressure group only).
+    # Inside the kernel a boundary node costs its whole wavefront the boundary
+    # code; on a face normal to x that is one node in 64, where a separate
+    # kernel runs 64 boundary nodes per wavefront.
+    fuse_post: bool = True
     _plan: FusedPlan | None = field(default=None, init=False, repr=False)
     _plan_key: tuple | None = field(default=None, init=False, repr=False)
 
@@ -812,9 +828,9 @@ class Stepper:
                              "without turbulence model or body-force arrays")
         if any(getattr(b, "wm", None) is not None for b in self.bcs):
             raise ValueError("fused boundaries do not run with wall models")
-        key = (id(lat), tuple(id(b) for b in self.bcs))
+        key = (id(lat), tuple(id(b) for b in self.bcs), self.fuse_post)
         if self._plan_key != key:
-            self._plan = FusedPlan.build(lat, self.bcs)
+            self._plan = FusedPlan.build(lat, self.bcs, post=self.fuse_post)
             self._plan_key = key
         return self._plan
 
@@ -824,7 +840,8 @@ class Stepper:
         for i in range(n):
             # the pressure group of the previous step of this run is applied
             # while the node's populations are gathered
-            lat.k17_step_fused(self.even, plan.post, plan.pre, apply_pre=i > 0, stream=stream)
+            lat.k17_step_fused(self.even, plan.post, plan.pre, apply_pre=i > 0, any_mask=plan.any_mask,
+                               stream=stream)
             for bc in plan.residual_post:
                 if bc.precursor is not None and bc.precursor.reader is not None:
                     bc.precursor.advance(self.t, stream)
