@@ -350,6 +350,22 @@ vfx_status vfx_import_distributions(const vfx_lattice* lat, int32_t is_even_time
 vfx_status vfx_macroscopic_quantities(const vfx_lattice* lat, int32_t is_even_timestep, void* rho, void* vx, void* vy,
                                       void* vz, void* pressure, vfx_stream stream);
 
+/* calculateMeanCompressible (src/gpu/core/PostProcessor/MacroscopicQuantities.cu:390-536,
+ * :975-984), called by Simulation::run at every step in [TimeStartCalcMean,
+ * TimeEndCalcMean] (src/gpu/core/Calculation/Simulation.cpp:448-470): adds the
+ * density deviation and velocity of every fluid node to the running sums
+ * rho, vx, vy, vz (lattice real, n_nodes each), and to pressure the
+ * pressure expression evaluated with the new density and velocity sums, as the
+ * reference kernel does.  Nodes that are not fluid (direct: solid; sparse:
+ * type_of_grid_node != GEO_FLUID) get zero sums; a sparse lattice without
+ * type_of_grid_node updates the fluid_nodes only.  The mean is sum / tdiff
+ * (calcMean, src/gpu/core/PostProcessor/CalcMean.cpp:57-67), taken on the host. */
+vfx_status vfx_calculate_mean(const vfx_lattice* lat, int32_t is_even_timestep, void* vx, void* vy, void* vz,
+                              void* rho, void* pressure, vfx_stream stream);
+/* ResetMeanValuesSP27 (MacroscopicQuantities.cu:772-796): zero the five sums. */
+vfx_status vfx_reset_mean(const vfx_lattice* lat, void* vx, void* vy, void* vz, void* rho, void* pressure,
+                          vfx_stream stream);
+
 /* Macroscopic fields of the pre-collision state (calculateMacroscopicQuantitiesCompressible_device,
  * src/gpu/core/PostProcessor/MacroscopicQuantities.cu:194-237), doubles, n_nodes each. */
 vfx_status vfx_macroscopic(const vfx_lattice* lat, int32_t is_even_timestep, double* drho, double* vx,

@@ -62,6 +62,7 @@ def lib():
         L.orc_bgk_collide.argtypes = [_dp, C.c_double, _dp]
         L.orc_chimera.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.orc_macroscopic.argtypes = [_dp, _dp]
+        L.orc_mean_add.argtypes = [_dp, _dp]
         L.orc_equilibrium.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double]
         L.orc_init_field.argtypes = [_dp, C.c_int64, _dp, _dp, _dp, _dp]
         L.orc_wall_term.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int]
@@ -177,6 +178,15 @@ def macroscopic_quantities(f27, impl="oracle"):
     fn = lib().orc_macroscopic_quantities if impl == "oracle" else ref().ref_macroscopic_quantities
     fn(_p(f), _p(out))
     return out
+
+
+def mean_add(f27, acc5):
+    """calculateMeanCompressible on one fluid node: the running sums acc5 =
+    (rho, vx, vy, vz, pressure) after adding populations f27."""
+    f = np.ascontiguousarray(f27, dtype=np.float64)
+    a = np.array(acc5, dtype=np.float64)
+    lib().orc_mean_add(_p(f), _p(a))
+    return a
 
 
 def precursor_blend(neighbors, weights, last, current, time_ratio):

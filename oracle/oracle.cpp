@@ -1185,6 +1185,50 @@ void orc_macroscopic_quantities(const double* f, double* out5)
     out5[4] = press;
 }
 
+// calculateMeanCompressible_device per fluid node (MacroscopicQuantities.cu:390-536):
+// acc5 = {rho, vx, vy, vz, pressure} running sums in and out.  The density and
+// velocity of f are added to the sums in the kernel's summation order; the
+// pressure sum gets the pressure expression with the new density and velocity
+// sums in place of the node's own (the kernel reads rhoD/vxD back after
+// writing them).
+void orc_mean_add(const double* f, double* acc5)
+{
+    const double drho =
+        ((((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]))) +
+         (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P])) +
+          ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]))) +
+         ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0]) + (f[d00M] + f[d00P]))) +
+        f[d000];
+    const double rho = 1.0 + drho;
+    const double r = drho + acc5[0];
+    const double x = (((((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dPMM] - f[dMPP]) + (f[dPPM] - f[dMMP]))) +
+                       (((f[dP0M] - f[dM0P]) + (f[dP0P] - f[dM0M])) + ((f[dPM0] - f[dMP0]) + (f[dPP0] - f[dMM0]))) +
+                       (f[dP00] - f[dM00])) /
+                      rho) +
+                     acc5[1];
+    const double y = (((((f[dPPP] - f[dMMM]) + (f[dMPM] - f[dPMP])) + ((f[dMPP] - f[dPMM]) + (f[dPPM] - f[dMMP]))) +
+                       (((f[d0PM] - f[d0MP]) + (f[d0PP] - f[d0MM])) + ((f[dMP0] - f[dPM0]) + (f[dPP0] - f[dMM0]))) +
+                       (f[d0P0] - f[d0M0])) /
+                      rho) +
+                     acc5[2];
+    const double z = (((((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dMPP] - f[dPMM]) + (f[dMMP] - f[dPPM]))) +
+                       (((f[d0MP] - f[d0PM]) + (f[d0PP] - f[d0MM])) + ((f[dM0P] - f[dP0M]) + (f[dP0P] - f[dM0M]))) +
+                       (f[d00P] - f[d00M])) /
+                      rho) +
+                     acc5[3];
+    acc5[4] = (f[dP00] + f[dM00] + f[d0P0] + f[d0M0] + f[d00P] + f[d00M] +
+               2.0 * (f[dPP0] + f[dMM0] + f[dPM0] + f[dMP0] + f[dP0P] + f[dM0M] + f[dP0M] + f[dM0P] + f[d0PP] +
+                      f[d0MM] + f[d0PM] + f[d0MP]) +
+               3.0 * (f[dPPP] + f[dMMP] + f[dPMP] + f[dMPP] + f[dPPM] + f[dMMM] + f[dPMM] + f[dMPM]) - r -
+               (x * x + y * y + z * z) * (1.0 + r)) *
+                  c1o2 +
+              r + acc5[4];
+    acc5[0] = r;
+    acc5[1] = x;
+    acc5[2] = y;
+    acc5[3] = z;
+}
+
 // one node of a wall-model boundary: post-collision p27, q27, the rate, the
 // sampling node's velocity vs3 (its z component as the kernels read it),
 // normal3, offset, z0; el3 / wall3 filtered velocities in and out; out27,

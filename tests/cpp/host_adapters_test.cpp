@@ -483,6 +483,27 @@ static void gpu_output_tests()
     }
     EXPECT(std::ifstream(prefix + "_bin_ID_0_t_0.vtk.pvtu").good());
     EXPECT(std::ifstream(prefix + "_bin_ID_0.pvd").good());
+    // time means: two calculateMean calls on the same state, calcMean over 2
+    // steps gives the state's own density and velocity; the _bin_mean files
+    para->setCalcMean(true);
+    allocMean(para.get());
+    calculateMean(para.get(), 0);
+    calculateMean(para.get(), 0);
+    calcMean(para.get(), 2);
+    bool mean_ok = p->meanDensityOut.size() == N;
+    for (uint32_t k = 0; k < N && mean_ok; ++k)
+        mean_ok = std::abs(p->meanDensityOut[k] - want[0][k]) <= 1e-15 &&
+                  std::abs(p->meanVelocityInXdirectionOut[k] - want[1][k]) <= 1e-15;
+    EXPECT(mean_ok);
+    w.writeTimestep(para, 2, false);
+    EXPECT(std::ifstream(prefix + "_bin_mean_lev_0_ID_0_Part_1_t_2.vtk.bin.vtu").good());
+    EXPECT(std::ifstream(prefix + "_bin_mean_ID_0_t_2.vtk.pvtu").good());
+    resetMean(para.get());
+    calcMean(para.get(), 1);
+    bool zero = true;
+    for (uint32_t k = 0; k < N; ++k) zero = zero && p->meanPressureOut[k] == 0.0;
+    EXPECT(zero);
+    freeMean(para.get());
     std::filesystem::remove_all("host_adapters_output");
     for (void* d : q) vfx_device_free(d);
     for (void* d : fields) vfx_device_free(d);
@@ -577,9 +598,48 @@ static int vtu_mode(const std::string& input, const std::string& prefix)
     return 0;
 }
 
+// the _bin_mean files of one level from host means: grid as in vtu_mode, then
+// five double arrays (vx, vy, vz, rho, press means), int32 t, double density
+// and velocity ratio
+static int vtu_mean_mode(const std::string& input, const std::string& prefix)
+{
+    std::ifstream in(input, std::ios::binary);
+    if (!in) return 2;
+    auto rd = [&](void* p, size_t n) { in.read((char*)p, (std::streamsize)n); };
+    int32_t n = 0, t = 0;
+    rd(&n, 4);
+    HostGrid g;
+    for (auto* v : {&g.coordinateX, &g.coordinateY, &g.coordinateZ}) {
+        v->resize(n);
+        rd(v->data(), 8 * (size_t)n);
+    }
+    for (auto* v : {&g.typeOfGridNode, &g.neighborX, &g.neighborY, &g.neighborZ}) {
+        v->resize(n);
+        rd(v->data(), 4 * (size_t)n);
+    }
+    rd(&g.gridSpacing, 8);
+    LBMSimulationParameter lev;
+    for (auto* v : {&lev.meanVelocityInXdirectionOut, &lev.meanVelocityInYdirectionOut,
+                    &lev.meanVelocityInZdirectionOut, &lev.meanDensityOut, &lev.meanPressureOut}) {
+        v->resize(n);
+        rd(v->data(), 8 * (size_t)n);
+    }
+    rd(&t, 4);
+    FileWriter w(prefix, 0);
+    rd(&w.densityRatio, 8);
+    rd(&w.velocityRatio, 8);
+    if (!in) return 3;
+    std::vector<std::string> files = w.writeUnstructuredGridMeanLT(g, lev, 0, (unsigned)t);
+    for (auto& f : files) f = f.substr(f.find_last_of('/') + 1);
+    WbWriterVtkXmlBinary::writeParallelFile(prefix + "_bin_mean_ID_0_t_" + std::to_string(t) + ".vtk", files,
+                                            FileWriter::getMeanNodeDataNames(), {});
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc > 3 && std::strcmp(argv[1], "vtu") == 0) return vtu_mode(argv[2], argv[3]);
+    if (argc > 3 && std::strcmp(argv[1], "vtumean") == 0) return vtu_mean_mode(argv[2], argv[3]);
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     if (argc > 2 && std::strcmp(argv[1], "restart") == 0) {
         try {

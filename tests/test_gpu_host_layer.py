@@ -38,7 +38,11 @@ STORAGE = {"fp64": 0, "fp32": 1, "fp16": 2}
 
 def harness(real):
     exe = os.path.join(ROOT, "tests", "cpp", f"host_update_grid_{real}")
-    if not os.path.exists(exe) or os.path.getmtime(exe) < os.path.getmtime(SRC):
+    # the harness links the host classes' layout: rebuild after any change to
+    # its source, the host header or the library
+    deps = [SRC, os.path.join(ROOT, "virtualfluids_amd", "csrc", "host", "vfx_host.hpp"),
+            os.path.join(LIBDIR, "libvfx.so")]
+    if not os.path.exists(exe) or os.path.getmtime(exe) < max(os.path.getmtime(d) for d in deps):
         flags = ["-DVF_DOUBLE_ACCURACY"] if real == "f64" else []
         subprocess.run(["g++", "-O1", "-std=c++17"] + flags + [SRC, "-o", exe, "-L" + LIBDIR, "-lvfx",
                         "-Wl,-rpath," + LIBDIR], check=True)

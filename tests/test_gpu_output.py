@@ -100,3 +100,75 @@ def test_file_writer_output_step(cuda, tmp_path, storage):
     pvd = open(str(tmp_path / "cav_bin_ID_0.pvd")).read()
     assert 'timestep="9"' in pvd and 'timestep="10"' in pvd
     assert open(path).read().count("<Piece Source=") == 1
+
+
+def test_mean_values_match_oracle(cuda):
+    """MeanValues (calculateMeanCompressible at every step) on the cavity:
+    FP64 sums equal the oracle's mean_add over the populations of each step
+    bit for bit; solid nodes hold zero sums; calcMean is sum / (real)tdiff and
+    resetMean zeroes the sums."""
+    print("[mean values fp64]", flush=True)
+    lat, st, solid = _cavity_state(steps=3)
+    mv = vfx.MeanValues(lat)
+    fl = ~solid
+    acc = np.zeros((int(fl.sum()), 5))
+    for _ in range(4):
+        mv.accumulate(st.even)
+        F = st.populations().cpu().numpy()[fl]
+        acc = np.array([oracle.mean_add(f, a) for f, a in zip(F, acc)])
+        st.step(1)
+    torch.cuda.synchronize()
+    got = np.stack([t.cpu().numpy() for t in mv.sums], axis=1)  # vx, vy, vz, rho, press
+    want = acc[:, [1, 2, 3, 0, 4]]
+    print(f"  identical {np.array_equal(got[fl], want)}", flush=True)
+    assert np.array_equal(got[fl], want)
+    assert not got[~fl].any()
+    mean = mv.mean(4)
+    assert torch.equal(mean["rho"], mv.sums[3] / 4.0)
+    mv.reset()
+    torch.cuda.synchronize()
+    assert not any(bool(t.any()) for t in mv.sums)
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_mean_values_reduced_storage(cuda, storage):
+    """FP32 sums of an FP32 / FP16 lattice against the FP64 oracle sums of
+    the same populations (read back in double)."""
+    print(f"[mean values {storage}]", flush=True)
+    lat, st, solid = _cavity_state(storage, steps=3)
+    mv = vfx.MeanValues(lat)
+    assert mv.sums[0].dtype == torch.float32
+    fl = ~solid
+    acc = np.zeros((int(fl.sum()), 5))
+    for _ in range(3):
+        mv.accumulate(st.even)
+        F = st.populations().cpu().numpy()[fl]
+        acc = np.array([oracle.mean_add(f, a) for f, a in zip(F, acc)])
+        st.step(1)
+    got = np.stack([t.cpu().numpy().astype(np.float64) for t in mv.sums], axis=1)[fl]
+    want = acc[:, [1, 2, 3, 0, 4]]
+    err = np.abs(got - want).max() / np.abs(want).max()
+    print(f"  max rel err {err:.2e}", flush=True)
+    assert err < 1e-5
+
+
+def test_mean_output_step(cuda, tmp_path):
+    """An output step with means: the _bin_mean part holds the means of the
+    interval (sums / tdiff) scaled as writeUnstructuredGridMeanLT, next to the
+    instantaneous part; the mean .pvtu lists it."""
+    print("[mean output step]", flush=True)
+    lat, st, solid = _cavity_state(steps=2)
+    mv = vfx.MeanValues(lat)
+    for _ in range(5):
+        st.step(1)
+        mv.accumulate(st.even)
+    mean = {k: v.cpu().numpy() for k, v in mv.mean(5).items()}
+    w = out.FileWriter(str(tmp_path / "cav"))
+    w.write_timestep([out.structured_level(lat, st.even)], 7, means=[mean])
+    files = sorted(os.listdir(tmp_path))
+    assert "cav_bin_mean_lev_0_ID_0_Part_1_t_7.vtk.bin.vtu" in files and "cav_bin_mean_ID_0_t_7.vtk.pvtu" in files
+    r = out.read_vtu(str(tmp_path / "cav_bin_mean_lev_0_ID_0_Part_1_t_7.vtk.bin.vtu"))
+    fl = ~solid
+    assert np.array_equal(r["data"]["rhoMed"][fl], (mean["rho"] / 3.0)[fl])
+    assert np.array_equal(r["data"]["vx1Med"][fl], mean["vx"][fl])
+    assert "cav_bin_mean_lev_0_ID_0_Part_1_t_7.vtk.bin.vtu" in open(tmp_path / "cav_bin_mean_ID_0_t_7.vtk.pvtu").read()

@@ -152,7 +152,9 @@ def test_host_layer_factories_cpu():
 
 @pytest.mark.gpu
 def test_host_layer_steps_like_c_abi(cuda):
-    exe = TEST_BIN if os.path.exists(TEST_BIN) else build_host_test()
+    deps = [TEST_SRC, vfx._lib.LIB_PATH, os.path.join(ROOT, "virtualfluids_amd", "csrc", "host", "vfx_host.hpp")]
+    fresh = os.path.exists(TEST_BIN) and os.path.getmtime(TEST_BIN) >= max(os.path.getmtime(d) for d in deps)
+    exe = TEST_BIN if fresh else build_host_test()
     r = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr

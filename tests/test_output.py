@@ -9,7 +9,11 @@ C++ FileWriter of the host layer), without a GPU:
   periodic boundary -- (nx-1)(ny-1)(nz-1) in a periodic box, the interior
   octs inside a solid layer;
 * the Python and the C++ writers produce byte-identical .bin.vtu, .pvtu and
-  .pvd files from the same input;
+  .pvd files from the same input, identical to the files VirtualFluids' own
+  writer (WbWriterVtkXmlBinary, compiled from the reference sources) wrote
+  from that input (tests/golden/vtk, tests/golden/make_vtk_golden.py);
+* the _bin_mean files (FileWriter::writeUnstructuredGridMeanLT) scale the
+  means in the real type, and the two writers agree on them;
 * parts above 30 million nodes keep the reference's resize behaviour
   (checked with a lowered part limit).
 """
@@ -65,6 +69,15 @@ def _box(dims, solid_layer=False, seed=0):
     data = [rng.normal(size=k.size) for _ in range(5)] + [types.astype(np.float64)]
     coords = (0.5 + 0.25 * x, -1.0 + 0.25 * y, 2.0 + 0.25 * z)
     return coords, types, (nbx, nby, nbz), 0.25, data
+
+
+def vtk_golden_case():
+    """The input of the VTK golden files (tests/golden/make_vtk_golden.py)."""
+    coords, types, nbs, dx, data = _box((9, 6, 5), solid_layer=True, seed=3)
+    data.append(np.abs(data[0]) * 1e-3)  # nut
+    return {"parts": list(out.unstructured_grid_parts(coords, types, nbs, dx, data)),
+            "names": ["press", "rho", "vx1", "vx2", "vx3", "geo", "nut"],
+            "mean_names": ["pressMed", "rhoMed", "vx1Med", "vx2Med", "vx3Med", "geo"], "t": 400}
 
 
 @pytest.mark.parametrize("solid_layer", [False, True])
@@ -125,7 +138,108 @@ def test_python_and_cpp_writers_are_identical(tmp_path):
     assert a == b == sorted([files[0], "run_bin_ID_0_t_400.vtk.pvtu", "run_bin_ID_0.pvd"])
     for f in a:
         assert (tmp_path / "cpp" / f).read_bytes() == (tmp_path / "py" / f).read_bytes(), f
+        # this input is the golden case: the reference writer's bytes
+        assert (tmp_path / "cpp" / f).read_bytes() == open(os.path.join(GOLDEN_VTK, f), "rb").read(), f
     assert b'timestep="1e+06"' in (tmp_path / "py" / "run_bin_ID_0.pvd").read_bytes()
+
+
+GOLDEN_VTK = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "vtk")
+
+
+def _python_golden_files(dest):
+    case = vtk_golden_case()
+    for prefix, mean in (("run", False), ("mean", True)):
+        names = case["mean_names"] if mean else case["names"]
+        pieces = []
+        for i, (points, cells, data) in enumerate(case["parts"], start=1):
+            name = out.part_file_name(os.path.join(dest, prefix), 0, 0, i, case["t"], mean=mean)
+            pieces.append(os.path.basename(out.write_octs_with_node_data(name, points, cells, names,
+                                                                         data[:len(names)])))
+        out.write_parallel_file(out.collection_file_name(os.path.join(dest, prefix), 0, case["t"], mean=mean), pieces,
+                                names)
+        if not mean:
+            out.write_collection_for_time_series(out.pvd_file_name(os.path.join(dest, prefix), 0),
+                                                 {case["t"]: pieces, 1000000: pieces})
+
+
+def test_python_writer_matches_reference_writer(tmp_path):
+    """Byte identity with the files VirtualFluids' WbWriterVtkXmlBinary wrote
+    from the same points, voxels and data (writeOctsWithNodeData,
+    writeParallelFile, writeCollectionForTimeSeries)."""
+    _python_golden_files(str(tmp_path))
+    golden = sorted(os.listdir(GOLDEN_VTK))
+    assert len(golden) == 5 and sorted(os.listdir(tmp_path)) == golden
+    for f in golden:
+        assert (tmp_path / f).read_bytes() == open(os.path.join(GOLDEN_VTK, f), "rb").read(), f
+
+
+def test_reference_writer_still_writes_the_golden_files(tmp_path):
+    """Where the reference tree exists: its writer, run now, writes the
+    committed golden files."""
+    sys_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    import sys
+
+    sys.path.insert(0, sys_path)
+    try:
+        import make_vtk_golden as g
+    finally:
+        sys.path.remove(sys_path)
+    if not os.path.exists(g.EXE):
+        pytest.skip("oracle/_ref/ref_vtk not built (no reference tree here)")
+    files = g.reference_files(str(tmp_path))
+    assert files == sorted(os.listdir(GOLDEN_VTK))
+    for f in files:
+        assert (tmp_path / f).read_bytes() == open(os.path.join(GOLDEN_VTK, f), "rb").read(), f
+
+
+def test_mean_files_scale_in_the_real_type(tmp_path):
+    """_bin_mean parts: meanPressureOut / 3.0f * densityRatio * velocityRatio^2
+    and meanVelocityOut * velocityRatio evaluated in float for a float lattice
+    (the ratios rounded to float first), in double for a double one; the C++
+    host writer writes the same bytes for the double case."""
+    coords, types, nbs, dx, data = _box((6, 5, 4), solid_layer=True, seed=11)
+    rng = np.random.default_rng(12)
+    dr, vr = 1.2345678901, 0.0987654321
+    for real in (np.float32, np.float64):
+        mean = {k: rng.normal(size=types.size).astype(real) for k in ("vx", "vy", "vz", "rho", "press")}
+        lev = {"coords": coords, "types": types, "neighbors": nbs, "grid_spacing": dx, "rho": mean["rho"]}
+        w = out.FileWriter(str(tmp_path / real.__name__ / "m"), density_ratio=dr, velocity_ratio=vr)
+        f, = w.write_level_mean(lev, mean, 0, 7)
+        r = out.read_vtu(f)
+        assert list(r["data"]) == out.FileWriter.mean_node_data_names()
+        fl = types == out.GEO_FLUID
+        R = real
+        want = (mean["press"] / R(3.0)) * R(dr) * R(vr) * R(vr)
+        assert np.array_equal(r["data"]["pressMed"][fl], want[fl].astype(np.float64))
+        assert np.array_equal(r["data"]["vx2Med"][fl], (mean["vy"] * R(vr))[fl].astype(np.float64))
+        if real is np.float32:  # a double-rounded ratio would differ
+            assert not np.array_equal(r["data"]["pressMed"][fl], (mean["press"].astype(np.float64) / 3.0 * dr * vr * vr)[fl])
+    exe = build_host_test()
+    n, t = types.size, 7
+    inp = tmp_path / "mean.bin"
+    with open(inp, "wb") as fh:
+        fh.write(np.int32(n).tobytes())
+        for c in coords:
+            fh.write(np.asarray(c, np.float64).tobytes())
+        for a in (types, *nbs):
+            fh.write(np.asarray(a, np.uint32).tobytes())
+        fh.write(np.float64(dx).tobytes())
+        for k in ("vx", "vy", "vz", "rho", "press"):
+            fh.write(np.asarray(mean[k], np.float64).tobytes())
+        fh.write(np.int32(t).tobytes())
+        fh.write(np.float64(dr).tobytes())
+        fh.write(np.float64(vr).tobytes())
+    cpp = str(tmp_path / "cpp" / "m")
+    r = subprocess.run([exe, "vtumean", str(inp), cpp], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    py = tmp_path / "float64"
+    out.write_parallel_file(out.collection_file_name(str(py / "m"), 0, t, mean=True),
+                            [os.path.basename(out.part_file_name("m", 0, 0, 1, t, mean=True)) + ".bin.vtu"],
+                            out.FileWriter.mean_node_data_names())
+    a = sorted(os.listdir(tmp_path / "cpp"))
+    assert a == sorted(os.listdir(py)) == ["m_bin_mean_ID_0_t_7.vtk.pvtu", "m_bin_mean_lev_0_ID_0_Part_1_t_7.vtk.bin.vtu"]
+    for f in a:
+        assert (tmp_path / "cpp" / f).read_bytes() == (py / f).read_bytes(), f
 
 
 def test_parts_keep_the_reference_resize_behaviour(monkeypatch):

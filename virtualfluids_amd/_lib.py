@@ -95,6 +95,8 @@ _SIGNATURES = {
     "vfx_macroscopic_quantities": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                    C.c_void_p, C.c_void_p],
     "vfx_import_distributions": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_int32, C.c_void_p],
+    "vfx_calculate_mean": [C.POINTER(VfxLattice), C.c_int32] + [C.c_void_p] * 6,
+    "vfx_reset_mean": [C.POINTER(VfxLattice)] + [C.c_void_p] * 6,
     "vfx_macroscopic": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                         C.c_void_p],
     "vfx_reduce_mass_momentum": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p],

@@ -464,6 +464,37 @@ vfx_status vfx_macroscopic_quantities(const vfx_lattice* lat, int32_t is_even, v
     return hip_status(e, "macroscopic quantities launch");
 }
 
+vfx_status vfx_calculate_mean(const vfx_lattice* lat, int32_t is_even, void* vx, void* vy, void* vz, void* rho,
+                              void* pressure, vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    if (!rho || !vx || !vy || !vz || !pressure) return fail(VFX_EINVAL, "mean sums are NULL");
+    if (!lat->direct && !lat->type_of_grid_node && lat->n_fluid && !lat->fluid_nodes)
+        return fail(VFX_EINVAL, "fluid_nodes is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    const bool even = is_even != 0;
+    switch (lat->storage) {
+    case VFX_FP64: e = vfx::launch_mean_vf<double>(L, lat->direct, even, vx, vy, vz, rho, pressure, st); break;
+    case VFX_FP32: e = vfx::launch_mean_vf<float>(L, lat->direct, even, vx, vy, vz, rho, pressure, st); break;
+    default: e = vfx::launch_mean_vf<_Float16>(L, lat->direct, even, vx, vy, vz, rho, pressure, st); break;
+    }
+    return hip_status(e, "mean launch");
+}
+
+vfx_status vfx_reset_mean(const vfx_lattice* lat, void* vx, void* vy, void* vz, void* rho, void* pressure,
+                          vfx_stream stream)
+{
+    vfx::Lat L;
+    if (vfx_status s = make_lat(lat, L)) return s;
+    if (!rho || !vx || !vy || !vz || !pressure) return fail(VFX_EINVAL, "mean sums are NULL");
+    hipStream_t st = (hipStream_t)stream;
+    const hipError_t e = lat->storage == VFX_FP64 ? vfx::launch_zero5<double>(L.N, vx, vy, vz, rho, pressure, st)
+                                                  : vfx::launch_zero5<float>(L.N, vx, vy, vz, rho, pressure, st);
+    return hip_status(e, "mean reset launch");
+}
+
 uint64_t vfx_reduce_workspace_bytes(const vfx_lattice* lat)
 {
     if (!lat) return 0;

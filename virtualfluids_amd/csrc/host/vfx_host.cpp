@@ -828,20 +828,20 @@ void FileWriter::writeLevel(const std::shared_ptr<Parameter>& para, unsigned int
     const uint64_t n = p->lattice.n_nodes;
     if (g.typeOfGridNode.size() != n || g.coordinateX.size() != n || g.neighborX.size() != n)
         throw std::runtime_error("FileWriter: the level's host grid does not match its lattice");
-    const size_t real = p->lattice.storage == VFX_FP64 ? 8 : 4;
+    const size_t rbytes = p->lattice.storage == VFX_FP64 ? 8 : 4;
     void* dev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     std::vector<std::vector<double>> fields(5, std::vector<double>(n));
     try {
-        for (auto& d : dev) check(vfx_device_alloc(n * real, &d), "FileWriter alloc");
+        for (auto& d : dev) check(vfx_device_alloc(n * rbytes, &d), "FileWriter alloc");
         check(vfx_macroscopic_quantities(&p->lattice, p->isEvenTimestep ? 1 : 0, dev[0], dev[1], dev[2], dev[3], dev[4],
                                          p->stream),
               "calculateMacroscopicQuantities");
         check(vfx_synchronize(p->stream), "FileWriter sync");
-        std::vector<char> buf(n * real);
+        std::vector<char> buf(n * rbytes);
         for (int i = 0; i < 5; ++i) {
-            check(vfx_copy_to_host(buf.data(), dev[i], n * real), "FileWriter d2h");
+            check(vfx_copy_to_host(buf.data(), dev[i], n * rbytes), "FileWriter d2h");
             for (uint64_t k = 0; k < n; ++k)
-                fields[i][k] = real == 8 ? ((const double*)buf.data())[k] : (double)((const float*)buf.data())[k];
+                fields[i][k] = rbytes == 8 ? ((const double*)buf.data())[k] : (double)((const float*)buf.data())[k];
         }
     } catch (...) {
         for (void* d : dev) vfx_device_free(d);
@@ -853,12 +853,13 @@ void FileWriter::writeLevel(const std::shared_ptr<Parameter>& para, unsigned int
     for (int i : {4, 0}) {
         std::vector<double> v(n);
         for (uint64_t k = 0; k < n; ++k)
-            v[k] = fields[i][k] / (double)3.0 * densityRatio * velocityRatio * velocityRatio;
+            v[k] = fields[i][k] / (double)3.0 * (double)(real)densityRatio * (double)(real)velocityRatio *
+                   (double)(real)velocityRatio;
         data.push_back(std::move(v));
     }
     for (int i : {1, 2, 3}) {
         std::vector<double> v(n);
-        for (uint64_t k = 0; k < n; ++k) v[k] = fields[i][k] * velocityRatio;
+        for (uint64_t k = 0; k < n; ++k) v[k] = fields[i][k] * (double)(real)velocityRatio;
         data.push_back(std::move(v));
     }
     std::vector<double> geo(n);
@@ -866,15 +867,75 @@ void FileWriter::writeLevel(const std::shared_ptr<Parameter>& para, unsigned int
     data.push_back(std::move(geo));
     if (para->getUseTurbulentViscosity()) {
         if (!p->lattice.turbulent_viscosity) throw std::runtime_error("FileWriter: no turbulent viscosity array");
-        std::vector<char> buf(n * real);
-        check(vfx_copy_to_host(buf.data(), p->lattice.turbulent_viscosity, n * real), "FileWriter d2h");
+        std::vector<char> buf(n * rbytes);
+        check(vfx_copy_to_host(buf.data(), p->lattice.turbulent_viscosity, n * rbytes), "FileWriter d2h");
         std::vector<double> v(n);
         for (uint64_t k = 0; k < n; ++k)
-            v[k] = (real == 8 ? ((const double*)buf.data())[k] : (double)((const float*)buf.data())[k]) * viscosityRatio;
+            v[k] = (rbytes == 8 ? ((const double*)buf.data())[k] : (double)((const float*)buf.data())[k]) *
+                   (double)(real)viscosityRatio;
         data.push_back(std::move(v));
     }
     for (const std::string& f : writeUnstructuredGridLT(g, getNodeDataNames(*para), data, level, timestep))
         fileNamesForCollectionFile.push_back(f.substr(f.find_last_of('/') + 1));
+}
+
+std::vector<std::string> FileWriter::getMeanNodeDataNames()
+{
+    return {"pressMed", "rhoMed", "vx1Med", "vx2Med", "vx3Med", "geo"};
+}
+
+std::vector<std::string> FileWriter::writeUnstructuredGridMeanLT(const HostGrid& g, const LBMSimulationParameter& p,
+                                                                 int level, unsigned int timestep)
+{
+    const size_t n = g.typeOfGridNode.size();
+    if (p.meanPressureOut.size() != n || p.meanDensityOut.size() != n || p.meanVelocityInXdirectionOut.size() != n ||
+        p.meanVelocityInYdirectionOut.size() != n || p.meanVelocityInZdirectionOut.size() != n)
+        throw std::runtime_error("FileWriter: the level's means do not match its grid (calcMean first)");
+    // meanPressureOut / 3.0f * densityRatio * velocityRatio * velocityRatio, all real
+    const real dr = (real)densityRatio, vr = (real)velocityRatio;
+    std::vector<std::vector<double>> data(6, std::vector<double>(n));
+    for (size_t k = 0; k < n; ++k) {
+        data[0][k] = p.meanPressureOut[k] / 3.0f * dr * vr * vr;
+        data[1][k] = p.meanDensityOut[k] / 3.0f * dr * vr * vr;
+        data[2][k] = p.meanVelocityInXdirectionOut[k] * vr;
+        data[3][k] = p.meanVelocityInYdirectionOut[k] * vr;
+        data[4][k] = p.meanVelocityInZdirectionOut[k] * vr;
+        data[5][k] = (double)g.typeOfGridNode[k];
+    }
+    // the cells of writeUnstructuredGridMeanLT are those of writeUnstructuredGridLT
+    // (same oct, fluid, bound and periodic tests); only the file name differs
+    const std::vector<std::string> names = getMeanNodeDataNames();
+    const uint32_t numberOfNodes = (uint32_t)n;
+    const uint32_t parts = FilePartCalculator::calculateNumberOfParts(numberOfNodes);
+    std::vector<std::array<float, 3>> nodes;
+    std::vector<std::array<uint32_t, 8>> cells;
+    std::vector<std::vector<double>> part(names.size());
+    std::vector<std::string> outNames;
+    for (uint32_t pi = 0; pi < parts; pi++) {
+        const uint32_t size = FilePartCalculator::calculateNumberOfNodesInPart(numberOfNodes, pi);
+        const uint32_t start = FilePartCalculator::calculateStartingPostionOfPart(pi);
+        const uint32_t end = start + size;
+        cells.clear();
+        nodes.resize(size);
+        for (auto& d : part) d.resize(size);
+        for (uint32_t pos = start; pos < end; pos++) {
+            if (g.typeOfGridNode[pos] != 19) continue; // GEO_FLUID
+            const uint32_t rel = pos - start;
+            nodes[rel] = {(float)g.coordinateX[pos], (float)g.coordinateY[pos], (float)g.coordinateZ[pos]};
+            for (size_t i = 0; i < part.size(); ++i) part[i][rel] = data[i][pos];
+            std::array<uint32_t, 8> octs, relPos;
+            WriterUtilities::getIndicesOfAllNodesInOct(octs, pos, g);
+            if (WriterUtilities::isPeriodicCell(g, octs[0], octs[6])) continue;
+            if (WriterUtilities::areAllNodesInOctValidForWriting(octs, g, end)) {
+                WriterUtilities::calculateRelativeNodeIndexInPart(relPos, octs, start);
+                cells.push_back(relPos);
+            }
+        }
+        const std::string fname = prefix + "_bin_mean" +
+                                  WriterUtilities::makePartFileNameEnding((uint32_t)level, ID, (int)pi + 1, (int)timestep);
+        outNames.push_back(WbWriterVtkXmlBinary::writeOctsWithNodeData(fname, nodes, cells, names, part));
+    }
+    return outNames;
 }
 
 void FileWriter::writeInit(const std::shared_ptr<Parameter>& para, unsigned int timestep)
@@ -884,15 +945,102 @@ void FileWriter::writeInit(const std::shared_ptr<Parameter>& para, unsigned int 
 
 void FileWriter::writeTimestep(const std::shared_ptr<Parameter>& para, unsigned int timestep, bool lastTimestep)
 {
-    for (int level = 0; level <= para->getMaxLevel(); level++) writeLevel(para, timestep, level);
+    for (int level = 0; level <= para->getMaxLevel(); level++) {
+        writeLevel(para, timestep, level);
+        if (para->getCalcMean()) {
+            LBMSimulationParameter* p = para->getParD(level);
+            for (const std::string& f : writeUnstructuredGridMeanLT(p->host, *p, level, timestep))
+                fileNamesForCollectionFileMean.push_back(f.substr(f.find_last_of('/') + 1));
+        }
+    }
     fileNamesForCollectionFileTimeSeries[timestep] = fileNamesForCollectionFile;
     if (lastTimestep)
         WbWriterVtkXmlBinary::writeCollectionForTimeSeries(prefix + "_bin_ID_" + std::to_string(ID),
                                                            fileNamesForCollectionFileTimeSeries, false);
-    WbWriterVtkXmlBinary::writeParallelFile(prefix + "_bin_ID_" + std::to_string(ID) + "_t_" + std::to_string(timestep) +
-                                                ".vtk",
-                                            fileNamesForCollectionFile, getNodeDataNames(*para), {});
+    const std::string ending = "_ID_" + std::to_string(ID) + "_t_" + std::to_string(timestep) + ".vtk";
+    WbWriterVtkXmlBinary::writeParallelFile(prefix + "_bin" + ending, fileNamesForCollectionFile,
+                                            getNodeDataNames(*para), {});
     fileNamesForCollectionFile.clear();
+    if (para->getCalcMean()) {
+        WbWriterVtkXmlBinary::writeParallelFile(prefix + "_bin_mean" + ending, fileNamesForCollectionFileMean,
+                                                getMeanNodeDataNames(), {});
+        fileNamesForCollectionFileMean.clear();
+    }
+}
+
+// ------------------------------------------------------------- time means --
+static std::array<void**, 5> mean_sums(LBMSimulationParameter* p)
+{
+    return {&p->meanVelocityInXdirection, &p->meanVelocityInYdirection, &p->meanVelocityInZdirection, &p->meanDensity,
+            &p->meanPressure};
+}
+
+void allocMean(Parameter* para)
+{
+    for (int level = 0; level <= para->getMaxLevel(); level++) {
+        LBMSimulationParameter* p = para->getParD(level);
+        check_real(p->lattice, "mean values");
+        const auto s = mean_sums(p);
+        for (int i = 0; i < 5; ++i)
+            if (!*s[i]) check(vfx_device_alloc(p->lattice.n_nodes * sizeof(real), s[i]), "allocMean");
+        check(vfx_reset_mean(&p->lattice, p->meanVelocityInXdirection, p->meanVelocityInYdirection,
+                             p->meanVelocityInZdirection, p->meanDensity, p->meanPressure, p->stream),
+              "allocMean reset");
+        for (auto* v : {&p->meanVelocityInXdirectionOut, &p->meanVelocityInYdirectionOut, &p->meanVelocityInZdirectionOut,
+                        &p->meanDensityOut, &p->meanPressureOut})
+            v->assign(p->lattice.n_nodes, (real)0.0);
+    }
+}
+
+void calculateMean(Parameter* para, int level)
+{
+    LBMSimulationParameter* p = para->getParD(level);
+    if (!p->meanPressure) throw std::runtime_error("calculateMean: allocMean first");
+    check(vfx_calculate_mean(&p->lattice, p->isEvenTimestep ? 1 : 0, p->meanVelocityInXdirection,
+                             p->meanVelocityInYdirection, p->meanVelocityInZdirection, p->meanDensity, p->meanPressure,
+                             p->stream),
+          "calculateMeanCompressible");
+}
+
+void calcMean(Parameter* para, uint tdiff)
+{
+    for (int level = 0; level <= para->getMaxLevel(); level++) {
+        LBMSimulationParameter* p = para->getParD(level);
+        if (!p->meanPressure) throw std::runtime_error("calcMean: allocMean first");
+        check(vfx_synchronize(p->stream), "calcMean sync");
+        const uint64_t n = p->lattice.n_nodes;
+        std::vector<real> sum(n);
+        const auto s = mean_sums(p);
+        std::vector<real>* out[5] = {&p->meanVelocityInXdirectionOut, &p->meanVelocityInYdirectionOut,
+                                     &p->meanVelocityInZdirectionOut, &p->meanDensityOut, &p->meanPressureOut};
+        for (int i = 0; i < 5; ++i) {
+            check(vfx_copy_to_host(sum.data(), *s[i], n * sizeof(real)), "calcMean d2h");
+            out[i]->resize(n);
+            for (uint64_t k = 0; k < n; ++k) (*out[i])[k] = sum[k] / (real)tdiff;
+        }
+    }
+}
+
+void resetMean(Parameter* para)
+{
+    for (int level = 0; level <= para->getMaxLevel(); level++) {
+        LBMSimulationParameter* p = para->getParD(level);
+        if (!p->meanPressure) continue;
+        check(vfx_reset_mean(&p->lattice, p->meanVelocityInXdirection, p->meanVelocityInYdirection,
+                             p->meanVelocityInZdirection, p->meanDensity, p->meanPressure, p->stream),
+              "ResetMeanValuesSP27");
+    }
+}
+
+void freeMean(Parameter* para)
+{
+    for (int level = 0; level <= para->getMaxLevel(); level++) {
+        const auto s = mean_sums(para->getParD(level));
+        for (int i = 0; i < 5; ++i) {
+            vfx_device_free(*s[i]);
+            *s[i] = nullptr;
+        }
+    }
 }
 
 VFX_HOST_END

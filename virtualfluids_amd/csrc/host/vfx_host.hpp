@@ -153,6 +153,12 @@ struct LBMSimulationParameter {
     double outflowPressureCorrectionFactor = 0.0; // Parameter.h:158
     HostGrid host;                                // for the output (FileWriter)
     vfx_stream stream = nullptr;
+    // Parameter.h:324-329: time-average sums on the device (real, numberOfNodes
+    // each; allocMean) and the means of the last interval on the host (calcMean)
+    void *meanVelocityInXdirection = nullptr, *meanVelocityInYdirection = nullptr,
+         *meanVelocityInZdirection = nullptr, *meanDensity = nullptr, *meanPressure = nullptr;
+    std::vector<real> meanVelocityInXdirectionOut, meanVelocityInYdirectionOut, meanVelocityInZdirectionOut,
+        meanDensityOut, meanPressureOut;
 };
 
 class Parameter {
@@ -179,6 +185,13 @@ public:
         for (auto& l : levels) l.outflowPressureCorrectionFactor = f;
     }
     bool getUseTurbulentViscosity() const { return useTurbulentViscosity; }
+    // Parameter.cpp:141-144, 526-532: time averages between TimeStartCalcMean and TimeEndCalcMean
+    void setCalcMean(bool c) { calcMean = c; }
+    bool getCalcMean() const { return calcMean; }
+    void setTimeCalcMedStart(int t) { tCalcMedStart = t; }
+    int getTimeCalcMedStart() const { return tCalcMedStart; }
+    void setTimeCalcMedEnd(int t) { tCalcMedEnd = t; }
+    int getTimeCalcMedEnd() const { return tCalcMedEnd; }
     // Parameter::getVelocityRatio / getTimeRatio (physical per lattice units)
     void setVelocityRatio(double v) { velocityRatio = v; }
     double getVelocityRatio() const { return velocityRatio; }
@@ -194,7 +207,21 @@ private:
     bool useTurbulentViscosity = false;
     double velocityRatio = 1.0;
     double timeRatio = 1.0;
+    bool calcMean = false;
+    int tCalcMedStart = 0, tCalcMedEnd = 0;
 };
+
+// CalcMean.cpp:42-80 and the averaging block of Simulation::run
+// (Simulation.cpp:448-470, 689-697): allocMean gives every level zeroed sums,
+// calculateMean adds the current state of a level (vfx_calculate_mean, at
+// every step of the averaging window), calcMean divides the sums by the steps
+// since the previous output into the *Out arrays, resetMean zeroes the sums,
+// freeMean releases them.
+void allocMean(Parameter* para);
+void calculateMean(Parameter* para, int level);
+void calcMean(Parameter* para, uint tdiff);
+void resetMean(Parameter* para);
+void freeMean(Parameter* para);
 
 int32_t vfx_turbulence_model_of(TurbulenceModel m);
 
@@ -486,12 +513,17 @@ public:
     std::vector<std::string> writeUnstructuredGridLT(const HostGrid& grid, const std::vector<std::string>& names,
                                                      const std::vector<std::vector<double>>& nodeData, int level,
                                                      unsigned int timestep);
+    // writeUnstructuredGridMeanLT (FileWriter.cpp:332-431): the _bin_mean parts of
+    // a level from its *Out means, scaled in real arithmetic
+    std::vector<std::string> writeUnstructuredGridMeanLT(const HostGrid& grid, const LBMSimulationParameter& level,
+                                                         int levelIndex, unsigned int timestep);
+    static std::vector<std::string> getMeanNodeDataNames();
 
 private:
     void writeLevel(const std::shared_ptr<Parameter>& para, unsigned int timestep, int level);
     std::string prefix;
     int ID;
-    std::vector<std::string> fileNamesForCollectionFile;
+    std::vector<std::string> fileNamesForCollectionFile, fileNamesForCollectionFileMean;
     std::map<uint32_t, std::vector<std::string>> fileNamesForCollectionFileTimeSeries;
 };
 

@@ -608,6 +608,105 @@ __global__ void __launch_bounds__(256) k_macro_vf(Lat L, int direct, typename Co
     press[k] = p;
 }
 
+// Time-averaged fields (calculateMeanCompressible_device,
+// MacroscopicQuantities.cu:390-536): adds the node's density deviation and
+// velocity to running sums (sum order of that kernel), and to the pressure sum
+// the pressure expression evaluated with the new density and velocity sums in
+// place of the node's own values, as the reference kernel does.  Every node
+// of the all-node form zeroes its sums first, so only fluid nodes accumulate.
+template <typename T> __device__ __forceinline__ void vf_mean_add(const T (&f)[27], T& RHO, T& VX, T& VY, T& VZ, T& PRESS)
+{
+#pragma clang fp contract(off)
+    const T drho = ((((f[dPPP] + f[dMMM]) + (f[dMPM] + f[dPMP])) + ((f[dMPP] + f[dPMM]) + (f[dMMP] + f[dPPM]))) +
+                    (((f[d0MP] + f[d0PM]) + (f[d0MM] + f[d0PP])) + ((f[dM0P] + f[dP0M]) + (f[dM0M] + f[dP0P])) +
+                     ((f[dMP0] + f[dPM0]) + (f[dMM0] + f[dPP0]))) +
+                    ((f[dM00] + f[dP00]) + (f[d0M0] + f[d0P0]) + (f[d00M] + f[d00P]))) +
+                   f[d000];
+    const T rho = T(1) + drho;
+    const T r = drho + RHO;
+    const T x = (((((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dPMM] - f[dMPP]) + (f[dPPM] - f[dMMP]))) +
+                  (((f[dP0M] - f[dM0P]) + (f[dP0P] - f[dM0M])) + ((f[dPM0] - f[dMP0]) + (f[dPP0] - f[dMM0]))) +
+                  (f[dP00] - f[dM00])) /
+                 rho) +
+                VX;
+    const T y = (((((f[dPPP] - f[dMMM]) + (f[dMPM] - f[dPMP])) + ((f[dMPP] - f[dPMM]) + (f[dPPM] - f[dMMP]))) +
+                  (((f[d0PM] - f[d0MP]) + (f[d0PP] - f[d0MM])) + ((f[dMP0] - f[dPM0]) + (f[dPP0] - f[dMM0]))) +
+                  (f[d0P0] - f[d0M0])) /
+                 rho) +
+                VY;
+    const T z = (((((f[dPPP] - f[dMMM]) + (f[dPMP] - f[dMPM])) + ((f[dMPP] - f[dPMM]) + (f[dMMP] - f[dPPM]))) +
+                  (((f[d0MP] - f[d0PM]) + (f[d0PP] - f[d0MM])) + ((f[dM0P] - f[dP0M]) + (f[dP0P] - f[dM0M]))) +
+                  (f[d00P] - f[d00M])) /
+                 rho) +
+                VZ;
+    PRESS = (f[dP00] + f[dM00] + f[d0P0] + f[d0M0] + f[d00P] + f[d00M] +
+             T(2) * (f[dPP0] + f[dMM0] + f[dPM0] + f[dMP0] + f[dP0P] + f[dM0M] + f[dP0M] + f[dM0P] + f[d0PP] +
+                     f[d0MM] + f[d0PM] + f[d0MP]) +
+             T(3) * (f[dPPP] + f[dMMP] + f[dPMP] + f[dMPP] + f[dPPM] + f[dMMM] + f[dPMM] + f[dMPM]) - r -
+             (x * x + y * y + z * z) * (T(1) + r)) *
+                T(0.5) +
+            r + PRESS;
+    RHO = r;
+    VX = x;
+    VY = y;
+    VZ = z;
+}
+
+template <typename S, bool EVEN, int LIST>
+__global__ void __launch_bounds__(256) k_mean_vf(Lat L, int direct, typename Compute<S>::T* vx,
+                                                 typename Compute<S>::T* vy, typename Compute<S>::T* vz,
+                                                 typename Compute<S>::T* rho, typename Compute<S>::T* press)
+{
+    using T = typename Compute<S>::T;
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t k;
+    bool fluid;
+    if constexpr (LIST) {
+        if (i >= L.nfluid) return;
+        k = L.fluid[i];
+        if (k >= L.N) return;
+        fluid = true;
+    } else {
+        if (i >= L.N) return;
+        k = (uint32_t)i;
+        fluid = direct ? !is_solid(L, k) : (L.types && L.types[k] == kGeoFluid);
+    }
+    T P = press[k], R = rho[k], X = vx[k], Y = vy[k], Z = vz[k];
+    if (!LIST) {
+        press[k] = T(0);
+        rho[k] = T(0);
+        vx[k] = T(0);
+        vy[k] = T(0);
+        vz[k] = T(0);
+    }
+    if (!fluid) return;
+    uint32_t kk[8];
+    if (direct) nbrs_direct(L, k, kk);
+    else nbrs_sparse(L, k, kk);
+    T m[27];
+    gather<S, T, EVEN>(L, kk, m);
+    T f[27];
+#pragma unroll
+    for (int s = 0; s < 27; ++s) f[dir_at(s)] = m[s];
+    vf_mean_add<T>(f, R, X, Y, Z, P);
+    rho[k] = R;
+    vx[k] = X;
+    vy[k] = Y;
+    vz[k] = Z;
+    press[k] = P;
+}
+
+template <typename T> __global__ void __launch_bounds__(256) k_zero5(T* a, T* b, T* c, T* d, T* e, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    a[i] = T(0);
+    b[i] = T(0);
+    c[i] = T(0);
+    d[i] = T(0);
+    e[i] = T(0);
+}
+
 // ---------------------------------------------------------- reductions ----
 // Deterministic two-pass FP64 sums: every block sums a fixed chunk of
 // kRedBlock * kRedPerThread nodes (per-thread sequential, then a wave64
@@ -2032,6 +2131,30 @@ hipError_t launch_macro_vf(const Lat& L, int direct, bool even, void* rho, void*
 }
 
 template <typename S>
+hipError_t launch_mean_vf(const Lat& L, int direct, bool even, void* vx, void* vy, void* vz, void* rho, void* press,
+                          hipStream_t st)
+{
+    using T = typename Compute<S>::T;
+    T *x = (T*)vx, *y = (T*)vy, *z = (T*)vz, *r = (T*)rho, *p = (T*)press;
+    const dim3 b(kBlock);
+    if (direct || L.types) {
+        if (even) hipLaunchKernelGGL((k_mean_vf<S, true, 0>), dim3(grid_for(L.N, kBlock)), b, 0, st, L, direct, x, y, z, r, p);
+        else hipLaunchKernelGGL((k_mean_vf<S, false, 0>), dim3(grid_for(L.N, kBlock)), b, 0, st, L, direct, x, y, z, r, p);
+    } else if (L.nfluid) {
+        if (even) hipLaunchKernelGGL((k_mean_vf<S, true, 1>), dim3(grid_for(L.nfluid, kBlock)), b, 0, st, L, direct, x, y, z, r, p);
+        else hipLaunchKernelGGL((k_mean_vf<S, false, 1>), dim3(grid_for(L.nfluid, kBlock)), b, 0, st, L, direct, x, y, z, r, p);
+    }
+    return hipGetLastError();
+}
+
+template <typename T> hipError_t launch_zero5(uint64_t n, void* a, void* b, void* c, void* d, void* e, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_zero5<T>), dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, (T*)a, (T*)b, (T*)c, (T*)d, (T*)e, n);
+    return hipGetLastError();
+}
+
+template <typename S>
 hipError_t launch_batched(const BatchDev& B, bool even, int variant, unsigned block, unsigned grid, hipStream_t st)
 {
     using T = typename Compute<S>::T;
@@ -2156,6 +2279,7 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     template hipError_t launch_vf_array<S>(const Lat&, bool, bool, void*, const void*, hipStream_t);                  \
     template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t);       \
     template hipError_t launch_macro_vf<S>(const Lat&, int, bool, void*, void*, void*, void*, void*, hipStream_t);     \
+    template hipError_t launch_mean_vf<S>(const Lat&, int, bool, void*, void*, void*, void*, void*, hipStream_t);      \
     template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);                            \
     template hipError_t launch_reduce_mass<S>(const Lat&, int, bool, double*, double*, hipStream_t);              \
     template hipError_t launch_stats<S>(const Lat&, int, bool, double*, hipStream_t);                             \
@@ -2177,6 +2301,7 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     extern template hipError_t launch_vf_array<S>(const Lat&, bool, bool, void*, const void*, hipStream_t);           \
     extern template hipError_t launch_macro<S>(const Lat&, int, bool, double*, double*, double*, double*, hipStream_t); \
     extern template hipError_t launch_macro_vf<S>(const Lat&, int, bool, void*, void*, void*, void*, void*, hipStream_t); \
+    extern template hipError_t launch_mean_vf<S>(const Lat&, int, bool, void*, void*, void*, void*, void*, hipStream_t); \
     extern template hipError_t launch_bc<S>(const Lat&, const BCDev&, int, bool, hipStream_t);                     \
     extern template hipError_t launch_reduce_mass<S>(const Lat&, int, bool, double*, double*, hipStream_t);       \
     extern template hipError_t launch_stats<S>(const Lat&, int, bool, double*, hipStream_t);                      \

@@ -390,6 +390,37 @@ class Statistics:
         return torch.stack([self.sums[3 + c] / t - m[a] * m[b] for c, (a, b) in enumerate(pairs)])
 
 
+class MeanValues:
+    """The time averages VirtualFluids writes to its _bin_mean files
+    (Parameter::getCalcMean): running sums meanVelocityIn{X,Y,Z}direction,
+    meanDensity, meanPressure in the lattice's real type, advanced by
+    calculateMeanCompressible at every step of the averaging window
+    (Simulation.cpp:448-470), divided by the steps since the previous output
+    (calcMean, CalcMean.cpp:57-67) and reset (resetMean, :70-80)."""
+
+    NAMES = ("vx", "vy", "vz", "rho", "press")
+
+    def __init__(self, lattice: Lattice):
+        self.lattice = lattice
+        real = lattice._macro_dtype()
+        self.sums = [torch.zeros(lattice.n_nodes, dtype=real, device=lattice.device) for _ in self.NAMES]
+
+    def accumulate(self, even, stream=None):
+        s = self.lattice.struct()
+        V.check(V.lib().vfx_calculate_mean(C.byref(s), int(bool(even)), *(t.data_ptr() for t in self.sums),
+                                           _stream(stream)), "vfx_calculate_mean")
+
+    def mean(self, tdiff) -> dict:
+        """sum / (real)tdiff of every field (host division in the real type)."""
+        d = torch.tensor(float(tdiff), dtype=self.sums[0].dtype, device=self.sums[0].device)
+        return {name: t / d for name, t in zip(self.NAMES, self.sums)}
+
+    def reset(self, stream=None):
+        s = self.lattice.struct()
+        V.check(V.lib().vfx_reset_mean(C.byref(s), *(t.data_ptr() for t in self.sums), _stream(stream)),
+                "vfx_reset_mean")
+
+
 class BoundaryCondition:
     """A boundary-node list in the QforBoundaryConditions shape
     (src/gpu/core/Calculation/Calculation.h:105-117).

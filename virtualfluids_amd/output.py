@@ -15,7 +15,10 @@ What a VirtualFluids GPU run writes at an output step
 * ``<prefix>_bin_ID_<ID>_t_<t>.vtk.pvtu`` listing the parts
   (writeParallelFile, :141-177);
 * at the last step, ``<prefix>_bin_ID_<ID>.pvd`` with the collection files of
-  every written step (writeCollectionForTimeSeries, :88-99).
+  every written step (writeCollectionForTimeSeries, :88-99);
+* with time averages (Parameter::getCalcMean), ``<prefix>_bin_mean_...`` parts
+  of pressMed, rhoMed, vx1Med, vx2Med, vx3Med, geo (writeUnstructuredGridMeanLT,
+  FileWriter.cpp:332-431; the means scaled in the real type) and their .pvtu.
 
 Quirks kept so the files match: a node that is not GEO_FLUID keeps the point
 coordinates and data a previous part left at its slot (zero in the first
@@ -242,21 +245,60 @@ class FileWriter:
         self.velocity_ratio = float(velocity_ratio)
         self.viscosity_ratio = float(viscosity_ratio)
         self.collection = []
+        self.collection_mean = []
         self.time_series = {}
 
     def node_data_names(self, turbulent):
         names = ["press", "rho", "vx1", "vx2", "vx3", "geo"]
         return names + ["nut"] if turbulent else names
 
+    @staticmethod
+    def mean_node_data_names():
+        """FileWriter::getMeanNodeDataNames (FileWriter.cpp:197-210)."""
+        return ["pressMed", "rhoMed", "vx1Med", "vx2Med", "vx3Med", "geo"]
+
+    @staticmethod
+    def _real(lev):
+        """The level's real type (that of its fields): Parameter's ratios are
+        real, so they are rounded to it before any use."""
+        dt = np.asarray(lev["rho"]).dtype
+        return dt.type if dt in (np.float32, np.float64) else np.float64
+
     def _scaled(self, lev):
-        dr, vr = self.density_ratio, self.velocity_ratio
+        R = self._real(lev)
+        dr, vr = float(R(self.density_ratio)), float(R(self.velocity_ratio))
         # (double)p / 3.0 * densityRatio * velocityRatio * velocityRatio, left to right
         out = [np.asarray(lev[k], np.float64) / 3.0 * dr * vr * vr for k in ("press", "rho")]
-        out += [np.asarray(lev[k], np.float64) * self.velocity_ratio for k in ("vx", "vy", "vz")]
+        out += [np.asarray(lev[k], np.float64) * vr for k in ("vx", "vy", "vz")]
         out.append(np.asarray(lev["types"], np.float64))
         if lev.get("nut") is not None:
-            out.append(np.asarray(lev["nut"], np.float64) * lev.get("viscosity_ratio", self.viscosity_ratio))
+            out.append(np.asarray(lev["nut"], np.float64) * float(R(lev.get("viscosity_ratio", self.viscosity_ratio))))
         return out
+
+    def _scaled_mean(self, lev, mean):
+        """writeUnstructuredGridMeanLT (FileWriter.cpp:332-431): the means in the
+        real type, meanPressureOut / 3.0f * densityRatio * velocityRatio *
+        velocityRatio and meanVelocityOut * velocityRatio, then to double."""
+        R = self._real(lev)
+        dr, vr = R(self.density_ratio), R(self.velocity_ratio)
+        out = [((np.asarray(mean[k], R) / R(3.0)) * dr * vr * vr).astype(np.float64) for k in ("press", "rho")]
+        out += [(np.asarray(mean[k], R) * vr).astype(np.float64) for k in ("vx", "vy", "vz")]
+        out.append(np.asarray(lev["types"], np.float64))
+        return out
+
+    def write_level_mean(self, lev, mean, level, timestep):
+        """The _bin_mean parts of one level; mean: rho, vx, vy, vz, press
+        (MeanValues.mean)."""
+        names = self.mean_node_data_names()
+        files = []
+        parts = unstructured_grid_parts(lev["coords"], lev["types"], lev["neighbors"], lev["grid_spacing"],
+                                        self._scaled_mean(lev, mean))
+        for i, (points, cells, data) in enumerate(parts, start=1):
+            name = part_file_name(self.prefix, level, self.ID, i, timestep, mean=True)
+            files.append(write_octs_with_node_data(name, points, cells, names, data))
+        for f in files:
+            self.collection_mean.append(os.path.basename(f))
+        return files
 
     def write_level(self, lev, level, timestep):
         names = self.node_data_names(lev.get("nut") is not None)
@@ -270,17 +312,25 @@ class FileWriter:
             self.collection.append(os.path.basename(f))
         return files
 
-    def write_timestep(self, levels, timestep, last=False):
-        """Every level's parts, the step's .pvtu and, with last=True, the .pvd."""
+    def write_timestep(self, levels, timestep, last=False, means=None):
+        """Every level's parts, the step's .pvtu and, with last=True, the .pvd;
+        with means (one MeanValues.mean dict per level, Parameter::getCalcMean)
+        also the _bin_mean parts and their .pvtu (FileWriter.cpp:102-140)."""
         turbulent = any(lev.get("nut") is not None for lev in levels)
         for level, lev in enumerate(levels):
             self.write_level(lev, level, timestep)
+            if means is not None:
+                self.write_level_mean(lev, means[level], level, timestep)
         self.time_series[timestep] = list(self.collection)
         if last:
             write_collection_for_time_series(pvd_file_name(self.prefix, self.ID), self.time_series)
         path = write_parallel_file(collection_file_name(self.prefix, self.ID, timestep), self.collection,
                                    self.node_data_names(turbulent))
         self.collection = []
+        if means is not None:
+            write_parallel_file(collection_file_name(self.prefix, self.ID, timestep, mean=True), self.collection_mean,
+                                self.mean_node_data_names())
+            self.collection_mean = []
         return path
 
 
