@@ -487,10 +487,13 @@ vfx_status vfx_batch_set_launch(vfx_batch batch, uint32_t block, uint32_t grid);
 vfx_status vfx_batch_get_launch(vfx_batch batch, uint32_t* block, uint32_t* grid);
 /* tile order: workgroup b starts at tile xcd_block(b), which gives each of the 8
  * XCDs runs of `run` consecutive tiles (tiles next to each other share cache
- * lines, and every XCD has its own L2); 1 = plain order */
+ * lines, and every XCD has its own L2); 1 = plain order; 0 = slabs: every
+ * domain's tiles split into 8 contiguous slabs, slab x on XCD x (the tiles a
+ * node's y and z neighbours sit in then share its L2).  The per-domain step
+ * always runs its tiles in plain order. */
 vfx_status vfx_batch_set_xcd_run(vfx_batch batch, uint32_t run);
 vfx_status vfx_batch_get_xcd_run(vfx_batch batch, uint32_t* run);
-/* times every launch shape (block, grid, XCD run of 1 / 4 / 16 tiles) on
+/* times every launch shape (block, grid, XCD run of 1 / 4 / 16 tiles or slabs) on
  * `steps` (even) steps from parity is_even_timestep and keeps the fastest; the
  * state advances, its parity does not change */
 vfx_status vfx_batch_autotune(vfx_batch batch, int32_t is_even_timestep, int32_t steps, vfx_stream stream,

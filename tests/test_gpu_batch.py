@@ -94,13 +94,14 @@ def test_launch_shapes_identical(cuda, variant):
     ref = _ensemble("fp16", variant=variant)
     ref.step(6)
     cus = torch.cuda.get_device_properties(0).multi_processor_count
-    for block, grid in [(64, 0), (128, 0), (256, cus), (64, 3), (128, 2 * cus)]:
+    for block, grid, run in [(64, 0, 1), (128, 0, 4), (256, cus, 16), (64, 3, 0), (128, 2 * cus, 0), (256, 0, 0)]:
         e = _ensemble("fp16", variant=variant)
         e.set_launch(block, grid)
-        assert e.launch_shape() == (block, grid)
+        e.xcd_run = run  # 0: each domain's tiles in eight slabs, one per XCD
+        assert e.launch_shape() == (block, grid) and e.xcd_run == run
         e.step(6)
         torch.cuda.synchronize()
-        assert torch.equal(e.pool, ref.pool), (block, grid)
+        assert torch.equal(e.pool, ref.pool), (block, grid, run)
 
 
 def test_graph_replay_matches_eager(cuda):
@@ -167,10 +168,10 @@ def test_fp16_ensemble_velocity_error(cuda):
 
 
 def test_fp16_small_cavities_velocity_error(cuda):
-    """The same comparison on the small test cavities (200 steps), bar 1e-2.
-    The 5x5x10 cavity is the sensitive one: rounding only the stored
-    populations to FP16 (FP64 arithmetic otherwise, CPU emulation) already
-    gives 1.1e-3 there, against 2.1e-4 for the 10^3 cavity."""
+    """The same comparison on the small test cavities (200 steps), bar 1e-3
+    (2e-3 for the 5x5x10 cavity: rounding only the stored populations to FP16,
+    FP64 arithmetic otherwise, CPU emulation, already gives 1.1e-3 there,
+    against 2.1e-4 for the 10^3 cavity)."""
     steps = 200
     print(f"[fp16 small cavities x {steps}]", flush=True)
     ref = _ensemble("fp64")
@@ -181,7 +182,7 @@ def test_fp16_small_cavities_velocity_error(cuda):
     for i, d in enumerate(DOMS):
         l2 = _velocity_l2(ref, lo, i, d)
         print(f"domain {i} {d}: velocity L2 {l2:.3e}")
-        assert l2 <= 1e-2
+        assert l2 <= (2e-3 if d == (5, 5, 10) else 1e-3)
 
 
 def test_domain_view_diagnostics_skip_walls(cuda):

@@ -696,6 +696,7 @@ struct vfx_batch_s {
     std::vector<vfx_domain> domains;
     void* dev_domains = nullptr;               // DomainDev<T>[n]
     uint2* dev_tiles[3] = {nullptr, nullptr, nullptr}; // tile tables for blocks 64/128/256
+    uint2* dev_slabs[3] = {nullptr, nullptr, nullptr}; // the same tiles in XCD-slab order (xcd_run 0)
     std::vector<uint32_t> first_tile[3];       // per domain, n+1 entries
     uint32_t ntiles[3] = {0, 0, 0};
     unsigned block = 256, grid = 0, xcd_run = 1;
@@ -752,14 +753,17 @@ template <typename T> void fill_domains(const vfx_batch_s& b, const double lim[3
     }
 }
 
-vfx::BatchDev batch_view(const vfx_batch_s* b, int bi, uint32_t first, uint32_t count)
+// whole: every tile of the ensemble (XCD-slab order when xcd_run is 0);
+// otherwise the tiles [first, first + count) of the plain table
+vfx::BatchDev batch_view(const vfx_batch_s* b, int bi, uint32_t first, uint32_t count, bool whole = false)
 {
     vfx::BatchDev B;
     B.pool = b->pool;
     B.domains = b->dev_domains;
-    B.tiles = b->dev_tiles[bi] + first;
+    const bool slabs = whole && b->xcd_run == 0;
+    B.tiles = (slabs ? b->dev_slabs[bi] : b->dev_tiles[bi]) + first;
     B.ntiles = count;
-    B.xcd_run = b->xcd_run;
+    B.xcd_run = b->xcd_run == 0 ? 1 : b->xcd_run;
     for (int c = 0; c < 4; ++c) {
         B.ld[c] = b->ld[c];
         B.st[c] = b->st[c];
@@ -849,6 +853,33 @@ vfx_status vfx_batch_create(void* pool, uint64_t pool_elements, int32_t storage,
         b->ntiles[bi] = (uint32_t)tiles.size();
         e = hipMalloc((void**)&b->dev_tiles[bi], tiles.size() * sizeof(uint2));
         if (e == hipSuccess) e = hipMemcpy(b->dev_tiles[bi], tiles.data(), tiles.size() * sizeof(uint2), hipMemcpyHostToDevice);
+        // XCD-slab order: workgroup w runs on XCD w mod 8, so position 8j + x
+        // takes the j-th tile of XCD x's queue, and queue x holds the x-th of
+        // eight contiguous slabs of every domain -- a domain's tiles meet their
+        // z and y neighbours (other tiles of the same slab) in one L2.  The
+        // remainder tiles of uneven splits rotate over the XCDs.
+        std::vector<std::vector<uint2>> q(vfx::kXcds);
+        uint32_t rot = 0;
+        for (uint32_t i = 0; i < n_domains; ++i) {
+            const uint32_t t0 = b->first_tile[bi][i], T = b->first_tile[bi][i + 1] - t0;
+            const uint32_t base = T / vfx::kXcds, rem = T % vfx::kXcds;
+            uint32_t pos = t0;
+            for (uint32_t x = 0; x < vfx::kXcds; ++x) {
+                const uint32_t size = base + ((x + vfx::kXcds - rot) % vfx::kXcds < rem ? 1u : 0u);
+                for (uint32_t j = 0; j < size; ++j) q[x].push_back(tiles[pos + j]);
+                pos += size;
+            }
+            rot = (rot + rem) % vfx::kXcds;
+        }
+        std::vector<uint2> slab;
+        slab.reserve(tiles.size());
+        size_t longest = 0;
+        for (const auto& v : q) longest = std::max(longest, v.size());
+        for (size_t j = 0; j < longest; ++j)
+            for (uint32_t x = 0; x < vfx::kXcds; ++x)
+                if (j < q[x].size()) slab.push_back(q[x][j]);
+        if (e == hipSuccess) e = hipMalloc((void**)&b->dev_slabs[bi], slab.size() * sizeof(uint2));
+        if (e == hipSuccess) e = hipMemcpy(b->dev_slabs[bi], slab.data(), slab.size() * sizeof(uint2), hipMemcpyHostToDevice);
     }
     if (e != hipSuccess) {
         vfx_batch_destroy(b);
@@ -862,6 +893,8 @@ vfx_status vfx_batch_destroy(vfx_batch b)
 {
     if (!b) return VFX_OK;
     if (b->dev_domains) (void)hipFree(b->dev_domains);
+    for (auto* p : b->dev_slabs)
+        if (p) (void)hipFree(p);
     for (auto* p : b->dev_tiles)
         if (p) (void)hipFree(p);
     delete b;
@@ -880,7 +913,7 @@ vfx_status vfx_batch_set_launch(vfx_batch b, uint32_t block, uint32_t grid)
 vfx_status vfx_batch_set_xcd_run(vfx_batch b, uint32_t run)
 {
     if (!b) return fail(VFX_EINVAL, "batch is NULL");
-    if (run == 0 || run > 4096) return fail(VFX_EINVAL, "xcd run %u outside 1..4096", run);
+    if (run > 4096) return fail(VFX_EINVAL, "xcd run %u outside 0..4096", run);
     b->xcd_run = run;
     return VFX_OK;
 }
@@ -917,7 +950,7 @@ vfx_status vfx_batch_step(vfx_batch b, int32_t is_even, vfx_stream stream)
 {
     if (!b) return fail(VFX_EINVAL, "batch is NULL");
     const int bi = block_index(b->block);
-    return hip_status(launch_batch(b, batch_view(b, bi, 0, b->ntiles[bi]), is_even != 0, b->block, b->grid,
+    return hip_status(launch_batch(b, batch_view(b, bi, 0, b->ntiles[bi], true), is_even != 0, b->block, b->grid,
                                    (hipStream_t)stream),
                       "batch step");
 }
@@ -976,11 +1009,11 @@ vfx_status vfx_batch_autotune(vfx_batch b, int32_t is_even, int32_t steps, vfx_s
     unsigned bb = keep_block, bg = keep_grid, br = keep_run;
     vfx_status r = VFX_OK;
     const unsigned blocks[3] = {64, 128, 256};
-    const unsigned runs[3] = {1, 4, 16};
+    const unsigned runs[4] = {1, 4, 16, 0};
     for (unsigned block : blocks) {
         const unsigned grids[4] = {0, (unsigned)cus * 8, (unsigned)cus * 16, (unsigned)cus * 32};
-        for (int cfg = 0; cfg < 12; ++cfg) {
-            const unsigned grid = grids[cfg / 3], run = runs[cfg % 3];
+        for (int cfg = 0; cfg < 16; ++cfg) {
+            const unsigned grid = grids[cfg / 4], run = runs[cfg % 4];
             b->block = block;
             b->grid = grid;
             b->xcd_run = run;

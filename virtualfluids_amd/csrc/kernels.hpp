@@ -1808,25 +1808,23 @@ __global__ void __launch_bounds__(256)
         S sv[27];
 #pragma unroll
         for (int s = 0; s < 27; ++s) sv[s] = Storage<S, T>::enc(m[s], stc[weight_class(dir_at(s))]);
-        // streaming: population e goes to the slot direction d = inv(e) was
-        // read from.  Every lane stores, wall links included: a wall link's
-        // streaming slot is the one the wall node would read e from, and the
-        // wall layer never collides, so nothing reads it; it is not the
-        // bounce-back destination of any node either (that would be the wall
-        // node itself).  Only the bounce-back stores below need a lane mask.
-#pragma unroll
-        for (int s = 0; s < 27; ++s) {
-            const int d = inv(dir_at(s));
-            Buf<S>::st(rs, vo[nbr(d)], slot(d, EVEN) * pb, sv[s]);
-        }
-        // bounce-back lanes: the slot node k reads direction d from next step
+        // One store per population.  Streaming: population e goes to the slot
+        // direction d = inv(e) was read from.  On a wall link that slot is the
+        // one the wall node would read e from -- the wall layer never collides,
+        // nothing reads it -- so the lane writes the bounce-back instead: the
+        // slot node k reads direction d from next step, in the other parity's
+        // plane at the same element (plane offset moved into the lane's offset).
 #pragma unroll
         for (int s = 0; s < 27; ++s) {
             const int e = dir_at(s);
             const int d = inv(e);
             const bool wall = (cx(e) < 0 && xl) || (cx(e) > 0 && xh) || (cy(e) < 0 && yl) || (cy(e) > 0 && yh) ||
                               (cz(e) < 0 && zl) || (cz(e) > 0 && zh);
-            if (wall) Buf<S>::st(rs, vo[nbr(d)], slot(d, !EVEN) * pb, sv[s]);
+            // the lower of the two planes in the scalar offset, the lane adds a
+            // non-negative distance to its plane
+            const int se = slot(d, EVEN), so = slot(d, !EVEN), mn = se < so ? se : so;
+            const uint32_t off = vo[nbr(d)] + (uint32_t)((wall ? so : se) - mn) * pb;
+            Buf<S>::st(rs, off, (uint32_t)mn * pb, sv[s]);
         }
     }
 }
