@@ -371,6 +371,43 @@ __global__ void __launch_bounds__(256) k_step_direct(Lat L, Rates<typename Compu
     store_nut<T, COLL>(L, kk[0], nut);
 }
 
+// The eight nodes of sparse node k (as nbrs_sparse) with the second- and
+// third-level neighbour reads shared across the wavefront.  VirtualFluids
+// numbers a grid x-fastest, so the +x neighbour of a lane's node is mostly the
+// next lane's node: where it is, nby/nbz of the +x neighbour are the next
+// lane's first-level reads and nbz of the +x+y neighbour its +y neighbour's
+// +z read (cross-lane shuffles instead of two dependent rounds of loads);
+// the last lane reads the next node's arrays itself, and a lane whose +x
+// neighbour is not the next node falls back to the loads.  `node` is the
+// lane's node (0, the dummy node, for a lane without one), `next` the node of
+// work item i + 1.  Same indices as nbrs_sparse.
+__device__ __forceinline__ void nbrs_sparse_wave(const Lat& L, uint32_t node, uint32_t next, uint32_t (&kk)[8])
+{
+    const bool last = (threadIdx.x & 63u) == 63u;
+    kk[0] = node;
+    kk[1] = L.nbx[node];
+    kk[2] = L.nby[node];
+    kk[4] = L.nbz[node];
+    uint32_t kn = __shfl_down(node, 1, 64), yn = __shfl_down(kk[2], 1, 64), zn = __shfl_down(kk[4], 1, 64);
+    if (last) {
+        kn = next;
+        yn = L.nby[next];
+        zn = L.nbz[next];
+    }
+    const bool hit = kk[1] == kn;
+    kk[3] = yn;
+    kk[5] = zn;
+    if (!hit) {
+        kk[3] = L.nby[kk[1]];
+        kk[5] = L.nbz[kk[1]];
+    }
+    kk[6] = L.nbz[kk[2]];
+    // the +x+y node's +z neighbour: the next lane's +y node's +z read
+    const uint32_t z6n = __shfl_down(kk[6], 1, 64);
+    kk[7] = z6n;
+    if (!hit || last) kk[7] = L.nbz[kk[3]];
+}
+
 // VirtualFluids sparse addressing over the fluid-node list
 // (RunCollision.cuh:74-143 semantics).
 template <typename S, int COLL, bool EVEN, bool FAST>
@@ -379,8 +416,17 @@ __global__ void __launch_bounds__(256) k_step_sparse(Lat L, Rates<typename Compu
     using T = typename Compute<S>::T;
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x, xcd_run<S>()) * blockDim.x + threadIdx.x;
     uint32_t kk[8], k;
-    if (!sparse_node(L, i, k)) return;
-    nbrs_sparse(L, k, kk);
+    const bool active = sparse_node(L, i, k);
+    // the node of work item i + 1 (the last lane of a wavefront reads its arrays)
+    uint32_t next = 0;
+    if ((threadIdx.x & 63u) == 63u) {
+        if (L.types) next = i + 1 < L.N ? i + 1 : 0u;
+        else if (i + 1 < L.nfluid) next = L.fluid[i + 1] < L.N ? L.fluid[i + 1] : 0u;
+    }
+    // every lane takes part in the shuffles: lanes without a node use the dummy node
+    const uint32_t node = L.types ? (i < L.N ? i : 0u) : (active ? k : 0u);
+    nbrs_sparse_wave(L, node, next, kk);
+    if (!active) return;
     T m[27], mv[4];
     constexpr bool O2 = FAST && coll_op(COLL) == VFX_K17 && coll_turb(COLL) == kTurbNone && sizeof(S) == 2;
     T nut = load_nut<T, COLL>(L, kk[0]);
