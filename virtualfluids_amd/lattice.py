@@ -444,8 +444,9 @@ class BoundaryCondition:
     The outflow kinds write the populations the collision after next reads
     (the kernel runs after a parity flip and fills the other parity's slots):
     applied only after every step, as VirtualFluids does, the second step
-    reads whatever those elements held before.  Apply it once with the
-    starting parity before the first step to define them."""
+    reads whatever those elements held before.  Stepper applies them once with
+    the starting parity before its first step to define them
+    (Stepper.prime_outflow)."""
 
     PRE_COLLISION = {V.VFX_BC_PRESSURE_NEQ, V.VFX_BC_OUTFLOW_NON_REFLECTING,
                      V.VFX_BC_OUTFLOW_NON_REFLECTING_PRESSURE_CORRECTION}
@@ -848,6 +849,12 @@ class Stepper:
     # code; on a face normal to x that is one node in 64, where a separate
     # kernel runs 64 boundary nodes per wavefront.
     fuse_post: bool = True
+    # Before the first step, apply the non-reflecting outflow kinds once with
+    # the starting parity: they write the populations the step after next
+    # reads, so without it the second step would read whatever those elements
+    # held (False leaves that to the caller, as VirtualFluids does).
+    prime_outflow: bool = True
+    _primed: bool = field(default=False, init=False, repr=False)
     _plan: FusedPlan | None = field(default=None, init=False, repr=False)
     _plan_key: tuple | None = field(default=None, init=False, repr=False)
 
@@ -865,9 +872,17 @@ class Stepper:
             self._plan_key = key
         return self._plan
 
+    def _prime(self, stream):
+        if self.prime_outflow and not self._primed:
+            for bc in self.bcs:
+                if bc.kind in BoundaryCondition.OUTFLOW:
+                    self.lattice.apply_bc(bc, self.even, stream=stream)
+        self._primed = True
+
     def _step_fused(self, n, stream):
         plan = self.fused_plan()
         lat = self.lattice
+        self._prime(stream)
         for i in range(n):
             # the pressure group of the previous step of this run is applied
             # while the node's populations are gathered
@@ -888,6 +903,7 @@ class Stepper:
     def step(self, n=1, stream=None):
         if self.fuse:
             return self._step_fused(n, stream)
+        self._prime(stream)
         # precursor inflows run last among the post-collision conditions (UpdateGrid27.cpp:252)
         post = [b for b in self.bcs if not b.pre_collision and b.kind not in BoundaryCondition.PRECURSOR] + \
                [b for b in self.bcs if b.kind in BoundaryCondition.PRECURSOR]
