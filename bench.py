@@ -457,7 +457,11 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
     spanwise), K17 at Re = 10^4 on the cube height.  Every coarse step (two
     fine steps, one coarse step, both levels' boundary kernels, fine-to-coarse
     and coarse-to-fine) is recorded into a HIP graph of `sample_every` coarse
-    steps.  Time-averaged velocity over the second half on both levels; the
+    steps, in the merged schedule (the coarse step and the first fine step in
+    one launch, each phase's boundary groups of both levels in one launch,
+    fine-to-coarse and coarse-to-fine in one launch; the launch-per-kernel
+    schedule is timed beside it over 500 coarse steps).  Time-averaged
+    velocity over the second half on both levels; the
     FP32 and FP16 forms (FP16: deviations from the u_ref equilibrium) against
     FP64.  At Re = 10^4 the wake is chaotic: two forms that differ by rounding
     part ways once the rounding has grown, so the FP32 distance is the yardstick
@@ -484,11 +488,25 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
     means, early_means = {}, {}
     for storage in forms:
         uref = (case.u_ref, 0.0, 0.0) if storage == "fp16" else (0.0, 0.0, 0.0)
-        st = vfx.RefinedStepper(nest, case.omega, storage=storage, coarse_solid=case.coarse_solid,
-                                fine_solid=case.fine_solid, coarse_bcs=cbcs, fine_bcs=fbcs, reference_velocity=uref)
-        for lat, z in ((st.coarse, zc), (st.fine, zf)):
-            n = z.size
-            lat.init_equilibrium(np.zeros(n), profile(z), np.zeros(n), np.zeros(n))
+        def stepper(merged):
+            st = vfx.RefinedStepper(nest, case.omega, storage=storage, coarse_solid=case.coarse_solid,
+                                    fine_solid=case.fine_solid, coarse_bcs=cbcs, fine_bcs=fbcs,
+                                    reference_velocity=uref, merged=merged)
+            for lat, z in ((st.coarse, zc), (st.fine, zf)):
+                n = z.size
+                lat.init_equilibrium(np.zeros(n), profile(z), np.zeros(n), np.zeros(n))
+            return st
+
+        # the launch-per-kernel schedule, timed over a short window for comparison
+        st = stepper(False)
+        graph = st.capture(sample_every)
+        n_sep = 10
+        st.replay(graph, sample_every)
+        ms_sep = _timed_loop(torch, stream, n_sep, lambda i: st.replay(graph, sample_every))
+        launches_sep = st.launches_per_step
+        del st, graph
+        torch.cuda.empty_cache()
+        st = stepper(True)
         graph = st.capture(sample_every)
         sc, sf = vfx.Statistics(st.coarse), vfx.Statistics(st.fine)
         sce, sfe = vfx.Statistics(st.coarse), vfx.Statistics(st.fine)
@@ -511,6 +529,9 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
         means[storage] = (sc.mean()[:, act], sf.mean()[:, ffl])
         early_means[storage] = (sce.mean()[:, act], sfe.mean()[:, ffl])
         r = {"mlups": round(lups / (ms * 1e-3) / 1e6, 1), "ms_per_coarse_step": round(ms / (chunks * sample_every), 4),
+             "launches_per_coarse_step": st.launches_per_step,
+             "separate_launches": {"ms_per_coarse_step": round(ms_sep / (n_sep * sample_every), 4),
+                                   "launches_per_coarse_step": launches_sep},
              "finite": bool(torch.isfinite(means[storage][1]).all())}
         out[storage] = r
         print(f"# cube {storage}: {r['mlups']:.0f} MLUPS (refined, fine counted twice)", file=sys.stderr, flush=True)

@@ -297,6 +297,23 @@ vfx_status vfx_k17_step_fused(const vfx_lattice* lat, int32_t is_even_timestep, 
                               int32_t n_post, const vfx_fused_bc* pre, const uint32_t* any_mask, int32_t apply_pre,
                               vfx_stream stream);
 
+/* The collide-and-stream step of one or two lattices (the levels of a nest,
+ * each at its own parity and rate) in one launch, each as vfx_k17_step_fused
+ * with its own fused groups (n_post = 0 and pre = NULL: the plain K17 step,
+ * VFX_TEMPLATE_DEFAULT).  UpdateGrid27 runs the levels' kernels one after
+ * another (src/gpu/core/Calculation/UpdateGrid27.cpp:54-95); the levels'
+ * arrays are independent, so one launch gives the same values. */
+typedef struct vfx_level_step {
+    const vfx_lattice* lattice;
+    int32_t is_even_timestep;
+    int32_t n_post;
+    const vfx_fused_bc* post;
+    const vfx_fused_bc* pre;
+    const uint32_t* any_mask;
+    int32_t apply_pre;
+} vfx_level_step;
+vfx_status vfx_k17_step_levels(const vfx_level_step* levels, int32_t n_levels, vfx_stream stream);
+
 /* calcTurbulentViscosityAMD (src/gpu/core/PostProcessor/TurbulentViscosityKernels.cu:62-130):
  * the AMD eddy viscosity of every fluid node from vx/vy/vz into
  * turbulent_viscosity, with constant sgs_constant.  UpdateGrid27 runs it after
@@ -310,6 +327,21 @@ vfx_status vfx_turbulent_viscosity_amd(const vfx_lattice* lat, vfx_stream stream
  * (src/gpu/core/Calculation/UpdateGrid27.cpp:227-262); VFX_BC_PRESSURE_NEQ is
  * pre-collision and runs after the flip with the new parity (:288-307). */
 vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_even_timestep, vfx_stream stream);
+
+/* Boundary groups applied by one launch, as UpdateGrid27 applies a phase's
+ * boundary kernels one after another (src/gpu/core/Calculation/UpdateGrid27.cpp:227-307):
+ * the groups of a set (1..32, kinds 0, 1, 2, 3, 4, 5; lattices of one storage
+ * type, e.g. the coarse and the fine level of a nest) run side by side, each
+ * at its own parity (bit j of even_mask for group j), with exactly the values
+ * vfx_bc_apply gives.  The groups must not write elements another group of
+ * the set reads (the post-collision groups of one step write wall nodes'
+ * elements only; keep pre-collision pressure groups in a set of their own).
+ * The set copies the lattices' and tables' pointers at creation: recreate it
+ * when an array moves. */
+typedef struct vfx_bc_set_s* vfx_bc_set;
+vfx_status vfx_bc_set_create(const vfx_lattice* const* lattices, const vfx_bc* const* bcs, int32_t n, vfx_bc_set* out);
+vfx_status vfx_bc_set_apply(vfx_bc_set set, uint32_t even_mask, vfx_stream stream);
+vfx_status vfx_bc_set_destroy(vfx_bc_set set);
 
 /* InitNavierStokesCompressible_Device (src/gpu/core/PreProcessor/PreProcessorStrategy/
  * InitNavierStokesCompressible/InitNavierStokesCompressible_Device.cu:44-213):
@@ -452,6 +484,14 @@ vfx_status vfx_scale_cf(const vfx_lattice* coarse, const vfx_lattice* fine, cons
  * per-node omega as in vfx_scale_cf (scaleFineToCoarseCompressible.cu:64). */
 vfx_status vfx_scale_fc(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells,
                         int32_t coarse_is_even, int32_t fine_is_even, vfx_stream stream);
+
+/* vfx_scale_fc then vfx_scale_cf of one nest (RefinementStrategy.cpp:169-178)
+ * in one launch.  Needs the nest layout's separation: no coarse node the
+ * fine-to-coarse cells write is a corner of a coarse-to-fine cell, and no fine
+ * node the coarse-to-fine cells write is a corner of a fine-to-coarse cell;
+ * then the values equal the two launches in order. */
+vfx_status vfx_scale_fc_cf(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* fc,
+                           const vfx_icells* cf, int32_t coarse_is_even, int32_t fine_is_even, vfx_stream stream);
 
 /* An ensemble of independent box domains (for example many lid-driven
  * cavities) advanced by one launch from a per-domain work list.  Every

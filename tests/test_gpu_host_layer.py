@@ -44,8 +44,10 @@ def harness(real):
             os.path.join(LIBDIR, "libvfx.so")]
     if not os.path.exists(exe) or os.path.getmtime(exe) < max(os.path.getmtime(d) for d in deps):
         flags = ["-DVF_DOUBLE_ACCURACY"] if real == "f64" else []
-        subprocess.run(["g++", "-O1", "-std=c++17"] + flags + [SRC, "-o", exe, "-L" + LIBDIR, "-lvfx",
+        tmp = f"{exe}.{os.getpid()}"
+        subprocess.run(["g++", "-O1", "-std=c++17"] + flags + [SRC, "-o", tmp, "-L" + LIBDIR, "-lvfx",
                         "-Wl,-rpath," + LIBDIR], check=True)
+        os.replace(tmp, exe)
     return exe
 
 

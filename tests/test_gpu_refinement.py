@@ -212,3 +212,89 @@ def test_wall_mounted_cube_fp64_matches_oracle(cuda):
     ec, ef = rel(got_c[act], ref_c[act]), rel(got_f[ffl], ref_f[ffl])
     print(f"coarse {ec:.3e} fine {ef:.3e}")
     assert ec <= REL and ef <= REL
+
+
+def _cube_stepper(storage, merged):
+    """The small wall-mounted cube nest from a perturbed uniform flow."""
+    from tests.test_oracle_refinement import small_cube_case
+    from virtualfluids_amd import geometry
+
+    case = small_cube_case()
+    nest = case.nest
+    Nc, Nf = int(np.prod(nest.coarse_dims)), int(np.prod(nest.fine_dims))
+    rng = np.random.default_rng(12)
+    Fc = np.tile(oracle.equilibrium(0.0, case.u_ref, 0.0, 0.0), (Nc, 1)) + rng.normal(0, 1e-4, (Nc, 27)) * cases.W
+    Ff = np.tile(oracle.equilibrium(0.0, case.u_ref, 0.0, 0.0), (Nf, 1)) + rng.normal(0, 1e-4, (Nf, 27)) * cases.W
+    uref = (case.u_ref, 0.0, 0.0) if storage == "fp16" else (0.0, 0.0, 0.0)
+    st = vfx.RefinedStepper(nest, case.omega, storage=storage, coarse_solid=case.coarse_solid,
+                            fine_solid=case.fine_solid, coarse_bcs=geometry.boundary_conditions(case.coarse_bcs),
+                            fine_bcs=geometry.boundary_conditions(case.fine_bcs), reference_velocity=uref,
+                            merged=merged)
+    st.coarse.write_populations(Fc, even=True)
+    st.fine.write_populations(Ff, even=True)
+    return case, st, Fc, Ff
+
+
+def _shear_stepper(storage, merged):
+    nest = shear_wave_nest()
+    Fc, Ff, _ = shear_wave_init(nest, 0.02)
+    st = vfx.RefinedStepper(nest, 1.9, storage=storage, merged=merged)
+    st.coarse.write_populations(Fc, even=True)
+    st.fine.write_populations(Ff, even=True)
+    return None, st, Fc, Ff
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32", "fp16"])
+@pytest.mark.parametrize("make", [_cube_stepper, _shear_stepper], ids=["cube", "shear"])
+def test_merged_schedule_equals_separate_launches(cuda, storage, make):
+    """The merged coarse step (coarse and first fine step in one launch, each
+    phase's boundary groups of both levels in one launch, fine-to-coarse and
+    coarse-to-fine in one launch) leaves the same bytes as the launch-per-
+    kernel schedule, on the wall-mounted cube nest (boundary sets) and the
+    shear-wave nest (no boundary groups); eager and graph-replayed."""
+    print(f"[merged refined schedule {make.__name__} {storage}]", flush=True)
+    _, a, _, _ = make(storage, False)
+    _, b, _, _ = make(storage, True)
+    assert b._one_interface
+    a.step(6)
+    b.step(3)
+    g = b.capture(3)
+    b.replay(g, 3)
+    torch.cuda.synchronize()
+    assert (a.even_c, a.even_f) == (b.even_c, b.even_f)
+    for la, lb in ((a.coarse, b.coarse), (a.fine, b.fine)):
+        diff = int((la.f.view(torch.uint8) != lb.f.view(torch.uint8)).sum())
+        print(f"  differing bytes {diff}", flush=True)
+        assert diff == 0
+
+
+def test_merged_schedule_fp64_matches_oracle(cuda):
+    from tests.test_oracle_refinement import refined_oracle_run_walls
+
+    steps = 15
+    print(f"[merged refined fp64 vs oracle x {steps}]", flush=True)
+    case, st, Fc, Ff = _cube_stepper("fp64", True)
+    st.step(steps)
+    torch.cuda.synchronize()
+    got_c = st.coarse.read_populations(st.even_c).cpu().numpy()
+    got_f = st.fine.read_populations(st.even_f).cpu().numpy()
+    ref_c, ref_f = refined_oracle_run_walls(case, Fc, Ff, steps, case.omega)
+    act = (case.nest.coarse_state == 0) & ~case.coarse_solid
+    ffl = ~case.fine_solid
+    ec, ef = rel(got_c[act], ref_c[act]), rel(got_f[ffl], ref_f[ffl])
+    print(f"coarse {ec:.3e} fine {ef:.3e}")
+    assert ec <= REL and ef <= REL
+
+
+@pytest.mark.parametrize("merged", [False, True])
+def test_capture_before_any_eager_step(cuda, merged):
+    """A graph recorded straight after construction (boundary and interface
+    tables first used inside the capture) replays like eager stepping."""
+    print(f"[refined capture first merged={merged}]", flush=True)
+    _, a, _, _ = _cube_stepper("fp32", merged)
+    _, b, _, _ = _cube_stepper("fp32", merged)
+    g = b.capture(2)
+    b.replay(g, 2)
+    a.step(2)
+    torch.cuda.synchronize()
+    assert torch.equal(a.coarse.f, b.coarse.f) and torch.equal(a.fine.f, b.fine.f)

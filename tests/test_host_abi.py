@@ -137,9 +137,13 @@ def test_batch_arguments_fail_without_device():
 
 
 def build_host_test():
-    cmd = ["g++", "-O1", "-std=c++17", "-DVF_DOUBLE_ACCURACY", TEST_SRC, "-o", TEST_BIN, "-L" + os.path.dirname(vfx._lib.LIB_PATH), "-lvfx",
+    # built under a name of its own, then renamed into place: tests running in
+    # parallel processes build and run it at the same time
+    tmp = f"{TEST_BIN}.{os.getpid()}"
+    cmd = ["g++", "-O1", "-std=c++17", "-DVF_DOUBLE_ACCURACY", TEST_SRC, "-o", tmp, "-L" + os.path.dirname(vfx._lib.LIB_PATH), "-lvfx",
            "-Wl,-rpath," + os.path.dirname(vfx._lib.LIB_PATH)]
     subprocess.run(cmd, check=True)
+    os.replace(tmp, TEST_BIN)
     return TEST_BIN
 
 

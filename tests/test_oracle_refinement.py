@@ -13,6 +13,7 @@ pinned without a GPU:
 * two-level physics: a shear wave crossing the nest decays at the analytic
   viscous rate and the fine level follows the same wave.
 """
+import copy
 import os
 
 import numpy as np
@@ -368,3 +369,18 @@ def test_wall_nest_oracle_runs():
     assert np.isfinite(mc).all() and np.abs(mc[act, 1:]).max() < 0.2
     mf = np.array([oracle.macroscopic(f) for f in Ff])
     assert np.isfinite(mf).all() and np.abs(mf[~case.fine_solid, 1:]).max() < 0.2
+
+
+def test_interface_cells_apart_on_the_nests():
+    """Fine-to-coarse writes coarse nodes two cells inside the nest, coarse-to-
+    fine reads the outermost layer: the cell sets never meet, so one launch may
+    do both (planting an overlap is detected)."""
+    from virtualfluids_amd.refinement import _cell_corners, interface_cells_apart
+
+    case = small_cube_case()
+    for nest in (case.nest, shear_wave_nest()):
+        assert interface_cells_apart(nest)
+    nest = case.nest
+    bad = copy.copy(nest)
+    bad.fc_coarse = np.concatenate([nest.fc_coarse, _cell_corners(nest.cf_coarse[:1], nest.coarse_dims)[:1]])
+    assert not interface_cells_apart(bad)

@@ -7,10 +7,10 @@ call the library; nothing here computes populations on the CPU.
 """
 from ._lib import VfxError, check, lib  # noqa: F401
 from .batch import Ensemble, Graph, capture, cavity_ensemble  # noqa: F401
-from .refinement import ICells, NestedBox, RefinedStepper, fine_omega, scale_cf, scale_fc  # noqa: F401
+from .refinement import ICells, NestedBox, RefinedStepper, fine_omega, scale_cf, scale_fc, scale_fc_cf  # noqa: F401
 from .lattice import (BYTES, CX, CY, CZ, INV, W, BoundaryCondition, Lattice, PrecursorPlanes, Statistics,  # noqa: F401
-                      Stepper, FusedGroup, FusedPlan, MeanValues, pack_mask)
+                      Stepper, FusedGroup, FusedPlan, MeanValues, BCSet, step_levels, pack_mask)
 
 __all__ = ["Ensemble", "Graph", "capture", "cavity_ensemble", "ICells", "NestedBox", "RefinedStepper", "fine_omega", "scale_cf",
-           "scale_fc", "Lattice", "BoundaryCondition", "PrecursorPlanes", "Stepper", "FusedGroup", "FusedPlan", "MeanValues", "Statistics", "VfxError", "lib", "check", "pack_mask",
+           "scale_fc", "scale_fc_cf", "Lattice", "BoundaryCondition", "PrecursorPlanes", "Stepper", "FusedGroup", "FusedPlan", "MeanValues", "BCSet", "step_levels", "Statistics", "VfxError", "lib", "check", "pack_mask",
            "CX", "CY", "CZ", "INV", "W", "BYTES"]

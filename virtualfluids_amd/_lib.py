@@ -63,6 +63,12 @@ class VfxFusedBC(C.Structure):
     _fields_ = [("bc", C.POINTER(VfxBC)), ("node_mask", C.c_void_p), ("word_base", C.c_void_p)]
 
 
+class VfxLevelStep(C.Structure):
+    _fields_ = [("lattice", C.POINTER(VfxLattice)), ("is_even_timestep", C.c_int32), ("n_post", C.c_int32),
+                ("post", C.POINTER(VfxFusedBC)), ("pre", C.POINTER(VfxFusedBC)), ("any_mask", C.c_void_p),
+                ("apply_pre", C.c_int32)]
+
+
 class VfxDomain(C.Structure):
     _fields_ = [("f_offset", C.c_uint64), ("dims", C.c_uint32 * 3), ("omega", C.c_double),
                 ("lid_velocity", C.c_double * 3)]
@@ -108,6 +114,13 @@ _SIGNATURES = {
                      C.c_void_p],
     "vfx_scale_fc": [C.POINTER(VfxLattice), C.POINTER(VfxLattice), C.POINTER(VfxICells), C.c_int32, C.c_int32,
                      C.c_void_p],
+    "vfx_k17_step_levels": [C.POINTER(VfxLevelStep), C.c_int32, C.c_void_p],
+    "vfx_scale_fc_cf": [C.POINTER(VfxLattice), C.POINTER(VfxLattice), C.POINTER(VfxICells), C.POINTER(VfxICells),
+                        C.c_int32, C.c_int32, C.c_void_p],
+    "vfx_bc_set_create": [C.POINTER(C.POINTER(VfxLattice)), C.POINTER(C.POINTER(VfxBC)), C.c_int32,
+                          C.POINTER(C.c_void_p)],
+    "vfx_bc_set_apply": [C.c_void_p, C.c_uint32, C.c_void_p],
+    "vfx_bc_set_destroy": [C.c_void_p],
     "vfx_batch_create": [C.c_void_p, C.c_uint64, C.c_int32, C.c_float, C.POINTER(VfxDomain), C.c_uint32, C.c_void_p, C.c_int32,
                          C.POINTER(C.c_void_p)],
     "vfx_batch_destroy": [C.c_void_p],

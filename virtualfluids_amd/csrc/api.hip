@@ -306,17 +306,91 @@ vfx_status vfx_bc_apply(const vfx_lattice* lat, const vfx_bc* bc, int32_t is_eve
     return hip_status(e, "bc launch");
 }
 
-vfx_status vfx_k17_step_fused(const vfx_lattice* lat, int32_t is_even, const vfx_fused_bc* post, int32_t n_post,
-                              const vfx_fused_bc* pre, const uint32_t* any_mask, int32_t apply_pre, vfx_stream stream)
+} // extern "C"
+
+struct vfx_bc_set_s {
+    int32_t storage = 0;
+    uint32_t n = 0, blocks = 0;
+    vfx::BCJob* dev = nullptr;
+};
+
+extern "C" {
+
+vfx_status vfx_bc_set_create(const vfx_lattice* const* lattices, const vfx_bc* const* bcs, int32_t n, vfx_bc_set* out)
 {
-    vfx::Lat L;
+    if (!out) return fail(VFX_EINVAL, "out is NULL");
+    *out = nullptr;
+    if (n < 1 || n > vfx::kMaxBCJobs || !lattices || !bcs)
+        return fail(VFX_EINVAL, "a boundary set holds 1..%d groups", vfx::kMaxBCJobs);
+    std::vector<vfx::BCJob> jobs(n);
+    uint32_t blocks = 0;
+    for (int j = 0; j < n; ++j) {
+        if (!lattices[j] || !bcs[j]) return fail(VFX_EINVAL, "group %d: NULL lattice or boundary", j);
+        if (lattices[j]->storage != lattices[0]->storage)
+            return fail(VFX_EINVAL, "group %d: every lattice of a set has one storage type", j);
+        const int32_t kind = bcs[j]->kind;
+        if (kind != VFX_BC_VELOCITY_BOUNCE_BACK && kind != VFX_BC_NOSLIP_INTERPOLATED &&
+            kind != VFX_BC_VELOCITY_INTERPOLATED && kind != VFX_BC_SLIP && kind != VFX_BC_PRESSURE_NEQ &&
+            kind != VFX_BC_VELOCITY_PRESSURE_INTERPOLATED)
+            return fail(VFX_EUNSUPPORTED, "group %d: kind %d in a set (kinds 0-5 only)", j, kind);
+        vfx::BCJob& J = jobs[j];
+        std::memset(&J, 0, sizeof(J));
+        if (vfx_status s = make_lat(lattices[j], J.L)) return s;
+        if (vfx_status s = make_bcdev(lattices[j], J.L, bcs[j], J.B)) return s;
+        J.kind = kind;
+        J.first_block = blocks;
+        blocks += (bcs[j]->n + vfx::kBlock - 1) / vfx::kBlock;
+    }
+    vfx_bc_set_s* set = new vfx_bc_set_s;
+    set->storage = lattices[0]->storage;
+    set->n = (uint32_t)n;
+    set->blocks = blocks;
+    hipError_t e = hipMalloc((void**)&set->dev, jobs.size() * sizeof(vfx::BCJob));
+    if (e == hipSuccess) e = hipMemcpy(set->dev, jobs.data(), jobs.size() * sizeof(vfx::BCJob), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (set->dev) (void)hipFree(set->dev);
+        delete set;
+        return hip_status(e, "boundary set tables");
+    }
+    *out = set;
+    return VFX_OK;
+}
+
+vfx_status vfx_bc_set_apply(vfx_bc_set set, uint32_t even_mask, vfx_stream stream)
+{
+    if (!set) return fail(VFX_EINVAL, "set is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    switch (set->storage) {
+    case VFX_FP64: e = vfx::launch_bc_set<double>(set->dev, set->n, set->blocks, even_mask, st); break;
+    case VFX_FP32: e = vfx::launch_bc_set<float>(set->dev, set->n, set->blocks, even_mask, st); break;
+    default: e = vfx::launch_bc_set<_Float16>(set->dev, set->n, set->blocks, even_mask, st); break;
+    }
+    return hip_status(e, "boundary set launch");
+}
+
+vfx_status vfx_bc_set_destroy(vfx_bc_set set)
+{
+    if (!set) return VFX_OK;
+    if (set->dev) (void)hipFree(set->dev);
+    delete set;
+    return VFX_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+// the lattice and fused groups of one fused step, validated
+vfx_status fused_inputs(const vfx_lattice* lat, const vfx_fused_bc* post, int32_t n_post, const vfx_fused_bc* pre,
+                        vfx::Lat& L, vfx::FusedGroup (&g)[3])
+{
     if (vfx_status s = make_lat(lat, L)) return s;
     if (!lat->direct) return fail(VFX_EUNSUPPORTED, "fused boundary conditions need structured (direct) addressing");
     if (lat->kernel_variant != 0) return fail(VFX_EUNSUPPORTED, "fused boundary conditions run the default K17 variant");
     if (lat->turbulence_model != VFX_TURB_NONE)
         return fail(VFX_EUNSUPPORTED, "fused boundary conditions run K17 without a turbulence model");
     if (n_post < 0 || n_post > 2 || (n_post > 0 && !post)) return fail(VFX_EINVAL, "n_post must be 0, 1 or 2");
-    vfx::FusedGroup g[3];
     std::memset(g, 0, sizeof(g));
     for (int j = 0; j < n_post + 1; ++j) {
         const vfx_fused_bc* f = j < n_post ? &post[j] : pre;
@@ -335,6 +409,45 @@ vfx_status vfx_k17_step_fused(const vfx_lattice* lat, int32_t is_even, const vfx
         G.base = f->word_base;
         G.kind = kind;
     }
+    return VFX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+vfx_status vfx_k17_step_levels(const vfx_level_step* levels, int32_t n_levels, vfx_stream stream)
+{
+    if (!levels || n_levels < 1 || n_levels > 2) return fail(VFX_EINVAL, "one or two levels per launch");
+    vfx::LevelInput in[2];
+    for (int j = 0; j < n_levels; ++j) {
+        const vfx_level_step& v = levels[j];
+        if (!v.lattice) return fail(VFX_EINVAL, "level %d: lattice is NULL", j);
+        if (v.lattice->storage != levels[0].lattice->storage) return fail(VFX_EINVAL, "levels differ in storage");
+        if (vfx_status s = fused_inputs(v.lattice, v.post, v.n_post, v.pre, in[j].L, in[j].g)) return s;
+        in[j].omega = v.lattice->omega;
+        in[j].lim = v.lattice->quadric_limiter;
+        in[j].force = v.lattice->force;
+        in[j].even = v.is_even_timestep != 0;
+        in[j].any = v.any_mask;
+        in[j].apply_pre = v.apply_pre != 0;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e;
+    switch (levels[0].lattice->storage) {
+    case VFX_FP64: e = vfx::launch_step_levels<double>(in, n_levels, st); break;
+    case VFX_FP32: e = vfx::launch_step_levels<float>(in, n_levels, st); break;
+    default: e = vfx::launch_step_levels<_Float16>(in, n_levels, st); break;
+    }
+    return hip_status(e, "level step launch");
+}
+
+vfx_status vfx_k17_step_fused(const vfx_lattice* lat, int32_t is_even, const vfx_fused_bc* post, int32_t n_post,
+                              const vfx_fused_bc* pre, const uint32_t* any_mask, int32_t apply_pre, vfx_stream stream)
+{
+    vfx::Lat L;
+    vfx::FusedGroup g[3];
+    if (vfx_status s = fused_inputs(lat, post, n_post, pre, L, g)) return s;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     const int ap = apply_pre != 0;
@@ -629,20 +742,23 @@ vfx_status vfx_accumulate_stats(const vfx_lattice* lat, int32_t is_even, double*
 // --------------------------------------------------------------- refinement --
 namespace {
 
-vfx_status scale(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells, int32_t even_c,
-                 int32_t even_f, vfx_stream stream, bool coarse_to_fine)
+vfx_status check_levels(const vfx_lattice* coarse, const vfx_lattice* fine, vfx::Lat& C, vfx::Lat& F)
 {
-    vfx::Lat C, F;
     if (vfx_status s = make_lat(coarse, C)) return s;
     if (vfx_status s = make_lat(fine, F)) return s;
     if (coarse->storage != fine->storage) return fail(VFX_EINVAL, "coarse and fine storage differ");
     // Parameter::getUseTurbulentViscosity is one switch for every level
     if ((coarse->turbulence_model != VFX_TURB_NONE) != (fine->turbulence_model != VFX_TURB_NONE))
         return fail(VFX_EINVAL, "coarse and fine levels differ in using a turbulence model");
+    return VFX_OK;
+}
+
+vfx_status make_cells(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells, vfx::ICellsDev& I)
+{
+    std::memset(&I, 0, sizeof(I));
     if (!cells) return fail(VFX_EINVAL, "cells is NULL");
     if (cells->n_cells == 0) return VFX_OK;
     if (!cells->fine_cells || !cells->coarse_cells) return fail(VFX_EINVAL, "cell index arrays are NULL");
-    vfx::ICellsDev I;
     I.coarse = cells->coarse_cells;
     I.fine = cells->fine_cells;
     I.ox = cells->offset_x;
@@ -651,6 +767,17 @@ vfx_status scale(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_i
     I.n = cells->n_cells;
     I.direct_c = coarse->direct;
     I.direct_f = fine->direct;
+    return VFX_OK;
+}
+
+vfx_status scale(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* cells, int32_t even_c,
+                 int32_t even_f, vfx_stream stream, bool coarse_to_fine)
+{
+    vfx::Lat C, F;
+    if (vfx_status s = check_levels(coarse, fine, C, F)) return s;
+    vfx::ICellsDev I;
+    if (vfx_status s = make_cells(coarse, fine, cells, I)) return s;
+    if (I.n == 0) return VFX_OK;
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;
     switch (coarse->storage) {
@@ -682,6 +809,25 @@ vfx_status vfx_scale_fc(const vfx_lattice* coarse, const vfx_lattice* fine, cons
                         int32_t coarse_is_even, int32_t fine_is_even, vfx_stream stream)
 {
     return scale(coarse, fine, cells, coarse_is_even, fine_is_even, stream, false);
+}
+
+vfx_status vfx_scale_fc_cf(const vfx_lattice* coarse, const vfx_lattice* fine, const vfx_icells* fc,
+                           const vfx_icells* cf, int32_t coarse_is_even, int32_t fine_is_even, vfx_stream stream)
+{
+    vfx::Lat C, F;
+    if (vfx_status s = check_levels(coarse, fine, C, F)) return s;
+    vfx::ICellsDev A, B;
+    if (vfx_status s = make_cells(coarse, fine, fc, A)) return s;
+    if (vfx_status s = make_cells(coarse, fine, cf, B)) return s;
+    hipStream_t st = (hipStream_t)stream;
+    const bool ec = coarse_is_even != 0, ef = fine_is_even != 0;
+    hipError_t e;
+    switch (coarse->storage) {
+    case VFX_FP64: e = vfx::launch_scale_both<double>(C, F, A, B, ec, ef, coarse->omega, fine->omega, st); break;
+    case VFX_FP32: e = vfx::launch_scale_both<float>(C, F, A, B, ec, ef, coarse->omega, fine->omega, st); break;
+    default: e = vfx::launch_scale_both<_Float16>(C, F, A, B, ec, ef, coarse->omega, fine->omega, st); break;
+    }
+    return hip_status(e, "scale fine to coarse and coarse to fine");
 }
 
 } // extern "C"

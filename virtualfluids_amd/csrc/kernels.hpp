@@ -11,6 +11,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <cstring>
 
 namespace vfx {
 
@@ -346,11 +347,13 @@ template <typename S> constexpr uint32_t xcd_run() { return sizeof(S) == 8 ? 256
 
 // Direct (structured) addressing: one block covers `blockDim.x` consecutive
 // nodes of one x row, so every row index below is wave-uniform (scalar ALU).
+// Block `block` of the lattice's `blocks`.
 template <typename S, int COLL, bool EVEN, bool FAST>
-__global__ void __launch_bounds__(256) k_step_direct(Lat L, Rates<typename Compute<S>::T> R)
+__device__ __forceinline__ void step_direct_body(const Lat& L, const Rates<typename Compute<S>::T>& R, uint32_t block,
+                                                 uint32_t blocks)
 {
     using T = typename Compute<S>::T;
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, xcd_run<S>());
+    const uint32_t blk = xcd_block(block, blocks, xcd_run<S>());
     const uint32_t row = blk / L.chunks;
     const uint32_t x = (blk - row * L.chunks) * blockDim.x + threadIdx.x;
     if (x >= L.nx) return;
@@ -369,6 +372,11 @@ __global__ void __launch_bounds__(256) k_step_direct(Lat L, Rates<typename Compu
     scatter_post<S, T, EVEN, O2>(L, kk, m);
     if (L.write_macro) write_macro<T>(L, kk[0], mv);
     store_nut<T, COLL>(L, kk[0], nut);
+}
+template <typename S, int COLL, bool EVEN, bool FAST>
+__global__ void __launch_bounds__(256) k_step_direct(Lat L, Rates<typename Compute<S>::T> R)
+{
+    step_direct_body<S, COLL, EVEN, FAST>(L, R, blockIdx.x, gridDim.x);
 }
 
 // The eight nodes of sparse node k (as nbrs_sparse) with the second- and
@@ -1066,10 +1074,9 @@ __device__ __forceinline__ T wall_term(int d, double ux, double uy, double uz, i
 // as the opposite direction, minus the wall term.  Runs between the collision
 // of parity EVEN and the parity flip.
 template <typename S, bool EVEN>
-__global__ void __launch_bounds__(256) k_bc_velocity_bb(Lat L, BCDev B)
+__device__ __forceinline__ void bc_velocity_bb_entry(const Lat& L, const BCDev& B, uint32_t i)
 {
     using T = typename Compute<S>::T;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n) return;
     uint32_t kk[8];
     const uint32_t k = B.k[i];
@@ -1089,6 +1096,12 @@ __global__ void __launch_bounds__(256) k_bc_velocity_bb(Lat L, BCDev B)
         // next step (parity !EVEN) reads direction e of node k from here
         Storage<S, T>::st(plane<S>(L, slot(e, !EVEN)), kk[nbr(e)], v, L.st[weight_class(e)], L.ss[e]);
     }
+}
+
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) k_bc_velocity_bb(Lat L, BCDev B)
+{
+    bc_velocity_bb_entry<S, EVEN>(L, B, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Interpolated bounce-back family (NoSlipInterpolatedCompressible.cu:44-378,
@@ -1223,10 +1236,9 @@ __device__ __forceinline__ void interp_links(const Lat& L, const BCDev& B, uint3
 }
 
 template <typename S, bool EVEN, int KIND>
-__global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
+__device__ __forceinline__ void bc_interp_entry(const Lat& L, const BCDev& B, uint32_t i, int kind = KIND)
 {
     using T = typename Compute<S>::T;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n) return;
     uint32_t kk[8];
     const uint32_t k = B.k[i];
@@ -1239,7 +1251,13 @@ __global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
         const int e = inv(d);
         p[d] = Storage<S, T>::ld(plane<S>(L, slot(e, EVEN)), kk[nbr(e)], L.ld[weight_class(d)], L.sh[d]);
     }
-    interp_links<S, T, EVEN, KIND>(L, B, i, k, kk, p);
+    interp_links<S, T, EVEN, KIND>(L, B, i, k, kk, p, kind);
+}
+
+template <typename S, bool EVEN, int KIND>
+__global__ void __launch_bounds__(256) k_bc_interp(Lat L, BCDev B)
+{
+    bc_interp_entry<S, EVEN, KIND>(L, B, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // Precursor distributions (PrecursorDistributions.cu:44-185): post-collision,
@@ -1316,10 +1334,9 @@ template <typename T> __device__ __forceinline__ void pressure_values(T (&f)[27]
 }
 
 template <typename S, bool EVEN>
-__global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
+__device__ __forceinline__ void bc_pressure_entry(const Lat& L, const BCDev& B, uint32_t i)
 {
     using T = typename Compute<S>::T;
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= B.n) return;
     const uint32_t k = B.k[i], kn = B.kn[i];
     if (k >= L.N || kn >= L.N) return;
@@ -1342,6 +1359,52 @@ __global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
     for (int d = 0; d < 27; ++d) {
         if (cx(d) * nx + cy(d) * ny + cz(d) * nz > 0) continue;
         Storage<S, T>::st(plane<S>(L, slot(d, EVEN)), kk[nbr(d)], f[d], L.st[weight_class(d)], L.ss[d]);
+    }
+}
+
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) k_bc_pressure(Lat L, BCDev B)
+{
+    bc_pressure_entry<S, EVEN>(L, B, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Several boundary groups, possibly of different lattices of one storage
+// type, in one launch: block b runs the job whose block range holds b (jobs
+// in increasing first_block), at that job's parity (bit j of even_mask).
+// Kinds 0 (velocity bounce-back), 1, 2, 3, 5 (interpolated) and 4 (pressure),
+// each entry exactly as its own kernel computes it.  The groups of one launch
+// must not touch each other's elements (the post-collision groups of a step
+// write wall nodes' elements only; a pressure group runs with the groups of
+// its own phase).
+constexpr int kMaxBCJobs = 32;
+struct BCJob {
+    Lat L;
+    BCDev B;
+    int32_t kind;
+    uint32_t first_block;
+};
+template <typename S>
+__global__ void __launch_bounds__(256) k_bc_set(const BCJob* __restrict__ jobs, uint32_t njobs, uint32_t even_mask)
+{
+    uint32_t j = 0;
+    for (uint32_t t = 1; t < njobs; ++t)
+        if (jobs[t].first_block <= blockIdx.x) j = t;
+    const BCJob& J = jobs[j];
+    const uint32_t i = (blockIdx.x - J.first_block) * blockDim.x + threadIdx.x;
+    const bool even = (even_mask >> j) & 1u;
+    switch (J.kind) {
+    case VFX_BC_VELOCITY_BOUNCE_BACK:
+        if (even) bc_velocity_bb_entry<S, true>(J.L, J.B, i);
+        else bc_velocity_bb_entry<S, false>(J.L, J.B, i);
+        break;
+    case VFX_BC_PRESSURE_NEQ:
+        if (even) bc_pressure_entry<S, true>(J.L, J.B, i);
+        else bc_pressure_entry<S, false>(J.L, J.B, i);
+        break;
+    default:
+        if (even) bc_interp_entry<S, true, -1>(J.L, J.B, i, J.kind);
+        else bc_interp_entry<S, false, -1>(J.L, J.B, i, J.kind);
+        break;
     }
 }
 
@@ -1661,15 +1724,17 @@ __device__ __forceinline__ void fused_post(const Lat& L, const FusedGroup& G, ui
 // is kept by capping the register budget, so that those rarely taken paths
 // spill instead.
 template <typename S> constexpr int fused_waves() { return sizeof(S) == 8 ? 3 : 7; }
+// block `block` of `blocks` (the lattice's own numbering; XCD of a block =
+// block mod 8, so a lattice's blocks start at a multiple of 8)
 template <typename S, bool EVEN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves<S>())))
-    k_step_fused(Lat L, Rates<typename Compute<S>::T> R, FusedGroup g0, FusedGroup g1, FusedGroup gp,
-                 const uint32_t* any, int32_t apply_pre)
+__device__ __forceinline__ void fused_step_body(const Lat& L, const Rates<typename Compute<S>::T>& R,
+                                                const FusedGroup& g0, const FusedGroup& g1, const FusedGroup& gp,
+                                                const uint32_t* any, int32_t apply_pre, uint32_t block, uint32_t blocks)
 {
     using T = typename Compute<S>::T;
     constexpr bool FAST = sizeof(S) != 8;
     constexpr bool O2 = sizeof(S) == 2;
-    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, xcd_run<S>());
+    const uint32_t blk = xcd_block(block, blocks, xcd_run<S>());
     const uint32_t row = blk / L.chunks;
     const uint32_t x = (blk - row * L.chunks) * blockDim.x + threadIdx.x;
     const uint32_t y = row % L.ny, z = row / L.ny;
@@ -1722,6 +1787,68 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_
     for (int j = 0; j < 2; ++j) {
         const FusedGroup& G = j == 0 ? g0 : g1;
         if (G.mask) fused_post<S, T, EVEN>(L, G, k);
+    }
+}
+
+template <typename S, bool EVEN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves<S>())))
+    k_step_fused(Lat L, Rates<typename Compute<S>::T> R, FusedGroup g0, FusedGroup g1, FusedGroup gp,
+                 const uint32_t* any, int32_t apply_pre)
+{
+    fused_step_body<S, EVEN>(L, R, g0, g1, gp, any, apply_pre, blockIdx.x, gridDim.x);
+}
+
+// Up to two lattices (the levels of a nest) stepped by one launch, each with
+// its own rates, parity and fused boundary groups: level j owns the blocks
+// [first_block, first_block + blocks), first_block a multiple of 8 (padding
+// blocks exit), blocks of 256 nodes along x.
+template <typename S> struct LevelStep {
+    Lat L;
+    Rates<typename Compute<S>::T> R;
+    FusedGroup g0, g1, gp;
+    const uint32_t* any;
+    int32_t apply_pre;
+    int32_t even;
+    uint32_t first_block, blocks;
+};
+template <typename S> struct LevelPair {
+    LevelStep<S> lv[2];
+    uint32_t n;
+};
+template <typename S>
+__device__ __forceinline__ void level_block(const LevelStep<S>& V)
+{
+    const uint32_t b = blockIdx.x - V.first_block;
+    if (b >= V.blocks) return;
+    if (V.even) fused_step_body<S, true>(V.L, V.R, V.g0, V.g1, V.gp, V.any, V.apply_pre, b, V.blocks);
+    else fused_step_body<S, false>(V.L, V.R, V.g0, V.g1, V.gp, V.any, V.apply_pre, b, V.blocks);
+}
+template <typename S>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves<S>())))
+    k_step_levels(LevelPair<S> P)
+{
+    // a constant index per branch: the kernel arguments are read in place
+    // (an index computed at run time would copy the structs to scratch)
+    if (P.n > 1 && blockIdx.x >= P.lv[1].first_block) level_block<S>(P.lv[1]);
+    else level_block<S>(P.lv[0]);
+}
+// The same launch for levels without fused boundary groups: the plain step's
+// body (k_step_direct) per level, each level's parity a template argument, so
+// no boundary code is compiled in.
+template <typename S> struct PlainLevels {
+    Lat L0, L1;
+    Rates<typename Compute<S>::T> R0, R1;
+    uint32_t n, blocks0, first1, blocks1;
+};
+template <typename S, bool E0, bool E1>
+__global__ void __launch_bounds__(256) k_step_levels_plain(PlainLevels<S> P)
+{
+    constexpr bool FAST = sizeof(S) != 8;
+    if (P.n > 1 && blockIdx.x >= P.first1) {
+        if (blockIdx.x - P.first1 < P.blocks1)
+            step_direct_body<S, VFX_K17, E1, FAST>(P.L1, P.R1, blockIdx.x - P.first1, P.blocks1);
+    } else if (blockIdx.x < P.blocks0) {
+        step_direct_body<S, VFX_K17, E0, FAST>(P.L0, P.R0, blockIdx.x, P.blocks0);
     }
 }
 
@@ -2088,6 +2215,83 @@ hipError_t launch_step_fused(const Lat& L, bool even, double omega, const double
     return hipGetLastError();
 }
 
+template <typename S>
+hipError_t launch_bc_set(const BCJob* jobs, uint32_t njobs, uint32_t blocks, uint32_t even_mask, hipStream_t st)
+{
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_bc_set<S>), dim3(blocks), dim3(kBlock), 0, st, jobs, njobs, even_mask);
+    return hipGetLastError();
+}
+
+// The levels of one launch: per level its lattice, rates inputs, parity, fused
+// groups (g[0], g[1] post, g[2] pressure), union mask and apply_pre.
+struct LevelInput {
+    Lat L;
+    double omega;
+    const double* lim;
+    const double* force;
+    bool even;
+    FusedGroup g[3];
+    const uint32_t* any;
+    int apply_pre;
+};
+template <typename S> hipError_t launch_step_levels(const LevelInput* in, int n, hipStream_t st)
+{
+    using T = typename Compute<S>::T;
+    static_assert(sizeof(LevelPair<S>) <= 4000, "kernel arguments");
+    LevelPair<S> P;
+    std::memset(&P, 0, sizeof(P));
+    P.n = (uint32_t)n;
+    uint32_t first = 0;
+    for (int j = 0; j < n; ++j) {
+        LevelStep<S>& V = P.lv[j];
+        V.L = in[j].L;
+        V.L.chunks = (V.L.nx + 255u) / 256u; // blocks of 256 nodes in this launch
+        V.R = make_rates<T>(in[j].omega, in[j].lim, in[j].force, 0.0);
+        V.g0 = in[j].g[0];
+        V.g1 = in[j].g[1];
+        V.gp = in[j].g[2];
+        V.any = in[j].any;
+        V.apply_pre = in[j].apply_pre;
+        V.even = in[j].even ? 1 : 0;
+        V.first_block = first;
+        V.blocks = V.L.chunks * V.L.ny * V.L.nz;
+        first += (V.blocks + kXcds - 1) / kXcds * kXcds;
+    }
+    if (first == 0) return hipSuccess;
+    bool plain = true;
+    for (int j = 0; j < n; ++j)
+        plain = plain && !in[j].g[0].mask && !in[j].g[1].mask && !(in[j].g[2].mask && in[j].apply_pre);
+    if (plain) {
+        static_assert(sizeof(PlainLevels<S>) <= 4000, "kernel arguments");
+        PlainLevels<S> Q;
+        std::memset(&Q, 0, sizeof(Q));
+        Q.n = (uint32_t)n;
+        Q.L0 = P.lv[0].L;
+        Q.R0 = P.lv[0].R;
+        Q.blocks0 = P.lv[0].blocks;
+        const bool e0 = P.lv[0].even != 0;
+        bool e1 = e0;
+        if (n > 1) {
+            Q.L1 = P.lv[1].L;
+            Q.R1 = P.lv[1].R;
+            Q.first1 = P.lv[1].first_block;
+            Q.blocks1 = P.lv[1].blocks;
+            e1 = P.lv[1].even != 0;
+        }
+        const dim3 g(first), b(256);
+        switch ((e0 ? 2 : 0) + (e1 ? 1 : 0)) {
+        case 3: hipLaunchKernelGGL((k_step_levels_plain<S, true, true>), g, b, 0, st, Q); break;
+        case 2: hipLaunchKernelGGL((k_step_levels_plain<S, true, false>), g, b, 0, st, Q); break;
+        case 1: hipLaunchKernelGGL((k_step_levels_plain<S, false, true>), g, b, 0, st, Q); break;
+        default: hipLaunchKernelGGL((k_step_levels_plain<S, false, false>), g, b, 0, st, Q); break;
+        }
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_step_levels<S>), dim3(first), dim3(256), 0, st, P);
+    return hipGetLastError();
+}
+
 template <typename S> hipError_t launch_turb_amd(const Lat& L, int direct, double sgs, hipStream_t st)
 {
     using T = typename Compute<S>::T;
@@ -2317,6 +2521,8 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     template hipError_t launch_step_fused<S>(const Lat&, bool, double, const double*, const double*, const FusedGroup&, \
                                              const FusedGroup&, const FusedGroup&, const uint32_t*, int, hipStream_t); \
     template hipError_t launch_turb_amd<S>(const Lat&, int, double, hipStream_t);                                      \
+    template hipError_t launch_bc_set<S>(const BCJob*, uint32_t, uint32_t, uint32_t, hipStream_t);                    \
+    template hipError_t launch_step_levels<S>(const LevelInput*, int, hipStream_t);                                    \
     template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,             \
                                        const double*, hipStream_t);                                                    \
     template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);                      \
@@ -2330,7 +2536,9 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     template hipError_t launch_batched<S>(const BatchDev&, bool, int, unsigned, unsigned, hipStream_t);           \
     template hipError_t launch_batch_zero<S>(void*, uint64_t, hipStream_t);                                        \
     template hipError_t launch_scale<S>(const Lat&, const Lat&, const ICellsDev&, bool, bool, bool, double, double,     \
-                                        hipStream_t);
+                                        hipStream_t);                                                                  \
+    template hipError_t launch_scale_both<S>(const Lat&, const Lat&, const ICellsDev&, const ICellsDev&, bool, bool,     \
+                                             double, double, hipStream_t);
 
 #define VFX_EXTERN_LAUNCHERS(S)                                                                                        \
     extern template hipError_t launch_step<S>(const Lat&, int, int, bool, int, double, const double*, const double*,   \
@@ -2339,6 +2547,8 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
                                                     const FusedGroup&, const FusedGroup&, const FusedGroup&,           \
                                                     const uint32_t*, int, hipStream_t);                                \
     extern template hipError_t launch_turb_amd<S>(const Lat&, int, double, hipStream_t);                               \
+    extern template hipError_t launch_bc_set<S>(const BCJob*, uint32_t, uint32_t, uint32_t, hipStream_t);             \
+    extern template hipError_t launch_step_levels<S>(const LevelInput*, int, hipStream_t);                             \
     extern template hipError_t launch_init<S>(const Lat&, int, bool, const double*, const double*, const double*,      \
                                               const double*, hipStream_t);                                             \
     extern template hipError_t launch_rw<S>(const Lat&, int, bool, double*, const double*, hipStream_t);               \
@@ -2352,6 +2562,8 @@ template <typename S> hipError_t launch_bc(const Lat& L, const BCDev& B, int kin
     extern template hipError_t launch_batched<S>(const BatchDev&, bool, int, unsigned, unsigned, hipStream_t);    \
     extern template hipError_t launch_batch_zero<S>(void*, uint64_t, hipStream_t);                                 \
     extern template hipError_t launch_scale<S>(const Lat&, const Lat&, const ICellsDev&, bool, bool, bool, double,     \
-                                               double, hipStream_t);
+                                               double, hipStream_t);                                                   \
+    extern template hipError_t launch_scale_both<S>(const Lat&, const Lat&, const ICellsDev&, const ICellsDev&, bool,   \
+                                                    bool, double, double, hipStream_t);
 
 } // namespace vfx

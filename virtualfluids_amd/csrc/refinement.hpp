@@ -268,13 +268,13 @@ __device__ __forceinline__ void load_cell(const Lat& L, int direct, uint32_t bas
 // pre-collision at the fine parity (scaleCoarseToFineCompressible.cu:305-529;
 // eps = 1/2, fine omega).
 template <typename S, bool EVEN_C, bool EVEN_F>
-__global__ void __launch_bounds__(256) k_scale_cf(Lat C, Lat F, ICellsDev cells, double omega_c, double omega_f)
+__device__ __forceinline__ void scale_cf_block(const Lat& C, const Lat& F, const ICellsDev& cells, double omega_c,
+                                               double omega_f, uint32_t block, typename Compute<S>::T* sh)
 {
     using T = typename Compute<S>::T;
-    __shared__ T sh[kCellsPerBlock * kCellLanes * kMoments];
     const int lane = threadIdx.x % kCellLanes;
     const int slot_in_block = threadIdx.x / kCellLanes;
-    const uint32_t cell = blockIdx.x * kCellsPerBlock + slot_in_block;
+    const uint32_t cell = block * kCellsPerBlock + slot_in_block;
     const bool active = cell < cells.n && cells.coarse[cell] < C.N && cells.fine[cell] < F.N;
     T* src = sh + slot_in_block * kCellLanes * kMoments;
     load_cell<S, EVEN_C>(C, cells.direct_c, active ? cells.coarse[cell] : 0u, lane, active, (T)omega_c, src);
@@ -304,17 +304,24 @@ __global__ void __launch_bounds__(256) k_scale_cf(Lat C, Lat F, ICellsDev cells,
     scatter_pre<S, T, EVEN_F>(F, kk, m);
 }
 
+template <typename S, bool EVEN_C, bool EVEN_F>
+__global__ void __launch_bounds__(256) k_scale_cf(Lat C, Lat F, ICellsDev cells, double omega_c, double omega_f)
+{
+    __shared__ typename Compute<S>::T sh[kCellsPerBlock * kCellLanes * kMoments];
+    scale_cf_block<S, EVEN_C, EVEN_F>(C, F, cells, omega_c, omega_f, blockIdx.x, sh);
+}
+
 // Fine to coarse: fine moments at the fine parity, the coarse node written
 // pre-collision at the coarse parity (scaleFineToCoarseCompressible.cu:76-124;
 // eps = 2, coarse omega, pressure d000 - laplace / 4 of InterpolationFC.h:138).
 template <typename S, bool EVEN_C, bool EVEN_F>
-__global__ void __launch_bounds__(256) k_scale_fc(Lat C, Lat F, ICellsDev cells, double omega_c, double omega_f)
+__device__ __forceinline__ void scale_fc_block(const Lat& C, const Lat& F, const ICellsDev& cells, double omega_c,
+                                               double omega_f, uint32_t block, typename Compute<S>::T* sh)
 {
     using T = typename Compute<S>::T;
-    __shared__ T sh[kCellsPerBlock * kCellLanes * kMoments];
     const int lane = threadIdx.x % kCellLanes;
     const int slot_in_block = threadIdx.x / kCellLanes;
-    const uint32_t cell = blockIdx.x * kCellsPerBlock + slot_in_block;
+    const uint32_t cell = block * kCellsPerBlock + slot_in_block;
     const bool active = cell < cells.n && cells.coarse[cell] < C.N && cells.fine[cell] < F.N;
     T* src = sh + slot_in_block * kCellLanes * kMoments;
     load_cell<S, EVEN_F>(F, cells.direct_f, active ? cells.fine[cell] : 0u, lane, active, (T)omega_f, src);
@@ -341,6 +348,45 @@ __global__ void __launch_bounds__(256) k_scale_fc(Lat C, Lat F, ICellsDev cells,
         const int d = dir_at(s);
         Storage<S, T>::st(plane<S>(C, slot(d, EVEN_C)), kk[nbr(d)], m[s], C.st[weight_class(d)], C.ss[d]);
     }
+}
+
+template <typename S, bool EVEN_C, bool EVEN_F>
+__global__ void __launch_bounds__(256) k_scale_fc(Lat C, Lat F, ICellsDev cells, double omega_c, double omega_f)
+{
+    __shared__ typename Compute<S>::T sh[kCellsPerBlock * kCellLanes * kMoments];
+    scale_fc_block<S, EVEN_C, EVEN_F>(C, F, cells, omega_c, omega_f, blockIdx.x, sh);
+}
+
+// Fine-to-coarse and coarse-to-fine of one nest in one launch: the first
+// nfc blocks restrict, the rest prolongate.  Valid when no coarse node the
+// restriction writes is a corner of a coarse-to-fine cell and no fine node the
+// prolongation writes is a corner of a fine-to-coarse cell (the nest layout
+// keeps them apart: restriction two coarse cells inside the nest, prolongation
+// on its outermost layer); the values are those of the two launches in order.
+template <typename S, bool EVEN_C, bool EVEN_F>
+__global__ void __launch_bounds__(256)
+    k_scale_both(Lat C, Lat F, ICellsDev fc, ICellsDev cf, double omega_c, double omega_f, uint32_t nfc)
+{
+    __shared__ typename Compute<S>::T sh[kCellsPerBlock * kCellLanes * kMoments];
+    if (blockIdx.x < nfc) scale_fc_block<S, EVEN_C, EVEN_F>(C, F, fc, omega_c, omega_f, blockIdx.x, sh);
+    else scale_cf_block<S, EVEN_C, EVEN_F>(C, F, cf, omega_c, omega_f, blockIdx.x - nfc, sh);
+}
+
+template <typename S>
+hipError_t launch_scale_both(const Lat& C, const Lat& F, const ICellsDev& fc, const ICellsDev& cf, bool even_c,
+                             bool even_f, double omega_c, double omega_f, hipStream_t st)
+{
+    const uint32_t nfc = (fc.n + kCellsPerBlock - 1) / kCellsPerBlock, ncf = (cf.n + kCellsPerBlock - 1) / kCellsPerBlock;
+    if (nfc + ncf == 0) return hipSuccess;
+    const dim3 g(nfc + ncf), b(kCellsPerBlock * kCellLanes);
+    const int v = (even_c ? 2 : 0) + (even_f ? 1 : 0);
+    switch (v) {
+    case 3: hipLaunchKernelGGL((k_scale_both<S, true, true>), g, b, 0, st, C, F, fc, cf, omega_c, omega_f, nfc); break;
+    case 2: hipLaunchKernelGGL((k_scale_both<S, true, false>), g, b, 0, st, C, F, fc, cf, omega_c, omega_f, nfc); break;
+    case 1: hipLaunchKernelGGL((k_scale_both<S, false, true>), g, b, 0, st, C, F, fc, cf, omega_c, omega_f, nfc); break;
+    default: hipLaunchKernelGGL((k_scale_both<S, false, false>), g, b, 0, st, C, F, fc, cf, omega_c, omega_f, nfc); break;
+    }
+    return hipGetLastError();
 }
 
 template <typename S>

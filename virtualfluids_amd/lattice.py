@@ -763,6 +763,65 @@ def fusable_pressure_entries(lattice, bc) -> np.ndarray:
     return ok
 
 
+class BCSet:
+    """Boundary groups of one or more lattices of one storage type applied by
+    one launch (vfx_bc_set): kinds 0-5, each group at its own parity.  The
+    groups must not write what another group of the set reads (the
+    post-collision groups of a step never do; keep pressure groups apart from
+    them)."""
+
+    KINDS = {V.VFX_BC_VELOCITY_BOUNCE_BACK, V.VFX_BC_NOSLIP_INTERPOLATED, V.VFX_BC_VELOCITY_INTERPOLATED,
+             V.VFX_BC_SLIP, V.VFX_BC_PRESSURE_NEQ, V.VFX_BC_VELOCITY_PRESSURE_INTERPOLATED}
+
+    def __init__(self, pairs):
+        self.pairs = list(pairs)
+        self.handle = None
+        if not self.pairs:
+            return
+        for lat, bc in self.pairs:
+            if bc.kind not in self.KINDS:
+                raise ValueError(f"kind {bc.kind} does not run in a boundary set")
+            if bc.max_node >= lat.n_nodes:
+                raise ValueError("boundary node index out of range")
+        self._lats = [lat.struct() for lat, _ in self.pairs]
+        self._bcs = [bc.struct(lat._macro_dtype()) for lat, bc in self.pairs]
+        n = len(self.pairs)
+        la = (C.POINTER(V.VfxLattice) * n)(*[C.pointer(x) for x in self._lats])
+        ba = (C.POINTER(V.VfxBC) * n)(*[C.pointer(x) for x in self._bcs])
+        h = C.c_void_p()
+        V.check(V.lib().vfx_bc_set_create(la, ba, n, C.byref(h)), "vfx_bc_set_create")
+        self.handle = h
+
+    def apply(self, parities, stream=None):
+        """parities: one bool per group (its is_even_timestep)."""
+        if self.handle is None:
+            return
+        mask = 0
+        for j, even in enumerate(parities):
+            mask |= int(bool(even)) << j
+        V.check(V.lib().vfx_bc_set_apply(self.handle, mask, _stream(stream)), "vfx_bc_set_apply")
+
+    def __del__(self):
+        if getattr(self, "handle", None) is not None:
+            V.lib().vfx_bc_set_destroy(self.handle)
+            self.handle = None
+
+
+def step_levels(levels, stream=None):
+    """The K17 collide-and-stream of one or two lattices in one launch
+    (vfx_k17_step_levels): levels = [(lattice, is_even)], each the plain
+    default-template step of that lattice at its parity."""
+    n = len(levels)
+    if not 1 <= n <= 2:
+        raise ValueError("one or two levels per launch")
+    structs = [lat.struct() for lat, _ in levels]
+    arr = (V.VfxLevelStep * n)()
+    for j, ((lat, even), s) in enumerate(zip(levels, structs)):
+        arr[j].lattice = C.pointer(s)
+        arr[j].is_even_timestep = int(bool(even))
+    V.check(V.lib().vfx_k17_step_levels(arr, n, _stream(stream)), "vfx_k17_step_levels")
+
+
 @dataclass
 class FusedPlan:
     """Which boundary entries a fused step applies inside the collide-stream

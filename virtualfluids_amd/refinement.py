@@ -33,7 +33,7 @@ import torch
 
 from . import _lib as V
 from .batch import Graph, capture
-from .lattice import Lattice, _stream
+from .lattice import BCSet, Lattice, _stream, step_levels
 
 INACTIVE = 2  # value in the coarse node-state array
 
@@ -231,6 +231,46 @@ def scale_fc(coarse: Lattice, fine: Lattice, cells: ICells, coarse_even: bool, f
                                  _stream(stream)), "vfx_scale_fc")
 
 
+def scale_fc_cf(coarse: Lattice, fine: Lattice, fc: ICells, cf: ICells, coarse_even: bool, fine_even: bool = True,
+                stream=None):
+    """scale_fc then scale_cf in one launch (vfx_scale_fc_cf); needs
+    interface_cells_apart(...) to hold."""
+    _check_cells(coarse, fine, fc)
+    _check_cells(coarse, fine, cf)
+    c, f = coarse.struct(), fine.struct()
+    real = coarse._macro_dtype()
+    a, b = fc.struct(real), cf.struct(real)
+    V.check(V.lib().vfx_scale_fc_cf(C.byref(c), C.byref(f), C.byref(a), C.byref(b), int(bool(coarse_even)),
+                                    int(bool(fine_even)), _stream(stream)), "vfx_scale_fc_cf")
+
+
+def _cell_corners(base, dims):
+    """The 8 corner nodes (x fastest, periodic wrap) of the cells at `base`
+    of a structured box, as the interface kernels take them."""
+    nx, ny, nz = dims
+    base = np.asarray(base, dtype=np.int64)
+    x, y, z = base % nx, (base // nx) % ny, base // (nx * ny)
+    out = []
+    for c in range(8):
+        xx = (x + (c & 1)) % nx
+        yy = (y + ((c >> 1) & 1)) % ny
+        zz = (z + ((c >> 2) & 1)) % nz
+        out.append(xx + nx * (yy + ny * zz))
+    return np.concatenate(out) if base.size else np.zeros(0, np.int64)
+
+
+def interface_cells_apart(nest) -> bool:
+    """True when one launch may do fine-to-coarse and coarse-to-fine: no coarse
+    node the fine-to-coarse cells write is a corner of a coarse-to-fine cell,
+    and no fine node the coarse-to-fine cells write is a corner of a
+    fine-to-coarse cell."""
+    fc_targets = np.asarray(nest.fc_coarse, dtype=np.int64)
+    cf_sources = _cell_corners(nest.cf_coarse, nest.coarse_dims)
+    fc_sources = _cell_corners(nest.fc_fine, nest.fine_dims)
+    cf_targets = _cell_corners(nest.cf_fine, nest.fine_dims)
+    return not (np.intersect1d(fc_targets, cf_sources).size or np.intersect1d(fc_sources, cf_targets).size)
+
+
 def fine_omega(omega_coarse: float) -> float:
     """Relaxation rate of the 2:1 finer level for the same physical viscosity
     (nu_fine = 2 nu_coarse in fine lattice units; Parameter.cpp:404)."""
@@ -250,7 +290,8 @@ class RefinedStepper:
 
     def __init__(self, nest: NestedBox, omega_coarse, storage="fp64", collision="k17", device="cuda",
                  fp16_scale=None, variant="default", coarse_solid=None, fine_solid=None, coarse_bcs=(),
-                 fine_bcs=(), reference_velocity=(0.0, 0.0, 0.0), turbulence="none", sgs_constant=0.0):
+                 fine_bcs=(), reference_velocity=(0.0, 0.0, 0.0), turbulence="none", sgs_constant=0.0,
+                 merged=False):
         self.nest = nest
         self.collision = collision
         self.omega_c = float(omega_coarse)
@@ -270,6 +311,74 @@ class RefinedStepper:
         self.fine_bcs = list(fine_bcs)
         self.even_c = True
         self.even_f = True
+        # device copies of the boundary and interface tables are made here, not
+        # on first use: a first use may be inside a graph capture
+        real = self.coarse._macro_dtype()
+        for bc in self.coarse_bcs + self.fine_bcs:
+            bc.tables(real)
+        self.fc.offsets(real)
+        self.cf.offsets(real)
+        self.merged = bool(merged)
+        if self.merged:
+            self._setup_merged()
+
+    def _setup_merged(self):
+        """Fewer launches per coarse step (same values): the coarse step and the
+        first fine step in one launch (step_levels), each phase's boundary
+        groups of both levels in one launch (BCSet), fine-to-coarse and
+        coarse-to-fine in one launch (scale_fc_cf) where the nest keeps them
+        apart.  Needs the default K17 without turbulence model."""
+        if self.collision != "k17" or self.coarse.variant != "default" or self.coarse.turbulence != "none":
+            raise ValueError("the merged schedule runs the default K17 without turbulence model")
+
+        def split(bcs, pre):
+            inset = [b for b in bcs if b.pre_collision == pre and b.kind in BCSet.KINDS]
+            rest = [b for b in bcs if b.pre_collision == pre and b.kind not in BCSet.KINDS]
+            return inset, rest
+
+        cpost, self._c_post_rest = split(self.coarse_bcs, False)
+        fpost, self._f_post_rest = split(self.fine_bcs, False)
+        cpre, self._c_pre_rest = split(self.coarse_bcs, True)
+        fpre, self._f_pre_rest = split(self.fine_bcs, True)
+        self._n_cpost, self._n_cpre = len(cpost), len(cpre)
+        self._set_post_both = BCSet([(self.coarse, b) for b in cpost] + [(self.fine, b) for b in fpost])
+        self._set_post_fine = BCSet([(self.fine, b) for b in fpost])
+        self._set_pre_both = BCSet([(self.coarse, b) for b in cpre] + [(self.fine, b) for b in fpre])
+        self._set_pre_fine = BCSet([(self.fine, b) for b in fpre])
+        self._n_fpost, self._n_fpre = len(fpost), len(fpre)
+        self._one_interface = interface_cells_apart(self.nest)
+
+    def _step_merged(self, stream):
+        c, f = self.coarse, self.fine
+        ec, ef = self.even_c, self.even_f
+        # coarse step and first fine step
+        step_levels([(c, ec), (f, ef)], stream)
+        self._set_post_both.apply([ec] * self._n_cpost + [ef] * self._n_fpost, stream)
+        for b in self._c_post_rest:
+            c.apply_bc(b, ec, stream=stream)
+        for b in self._f_post_rest:
+            f.apply_bc(b, ef, stream=stream)
+        ec, ef = not ec, not ef
+        self._set_pre_both.apply([ec] * self._n_cpre + [ef] * self._n_fpre, stream)
+        for b in self._c_pre_rest:
+            c.apply_bc(b, ec, stream=stream)
+        for b in self._f_pre_rest:
+            f.apply_bc(b, ef, stream=stream)
+        # second fine step
+        step_levels([(f, ef)], stream)
+        self._set_post_fine.apply([ef] * self._n_fpost, stream)
+        for b in self._f_post_rest:
+            f.apply_bc(b, ef, stream=stream)
+        ef = not ef
+        self._set_pre_fine.apply([ef] * self._n_fpre, stream)
+        for b in self._f_pre_rest:
+            f.apply_bc(b, ef, stream=stream)
+        self.even_c, self.even_f = ec, ef
+        if self._one_interface:
+            scale_fc_cf(c, f, self.fc, self.cf, ec, ef, stream)
+        else:
+            scale_fc(c, f, self.fc, ec, ef, stream)
+            scale_cf(c, f, self.cf, ec, ef, stream)
 
     @staticmethod
     def _level_step(lat, bcs, collision, even, stream):
@@ -285,6 +394,10 @@ class RefinedStepper:
         return not even
 
     def step(self, n=1, stream=None):
+        if self.merged:
+            for _ in range(n):
+                self._step_merged(stream)
+            return
         for _ in range(n):
             for _ in range(2):
                 self.even_f = self._level_step(self.fine, self.fine_bcs, self.collision, self.even_f, stream)
@@ -310,6 +423,16 @@ class RefinedStepper:
         graph.launch(stream)
         if n % 2:
             self.even_c = not self.even_c
+
+    @property
+    def launches_per_step(self) -> int:
+        """Kernel launches of one coarse step in the current schedule."""
+        if self.merged:
+            sets = (self._set_post_both, self._set_pre_both, self._set_post_fine, self._set_pre_fine)
+            rest = 2 * (len(self._f_post_rest) + len(self._f_pre_rest)) + len(self._c_post_rest) + len(self._c_pre_rest)
+            return 2 + sum(s.handle is not None for s in sets) + rest + (1 if self._one_interface else 2)
+        turb = int(self.collision == "k17" and self.coarse.turbulence == "amd")
+        return 2 * (1 + turb + len(self.fine_bcs)) + (1 + turb + len(self.coarse_bcs)) + 2
 
     @property
     def lups_per_step(self) -> int:
