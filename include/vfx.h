@@ -298,19 +298,14 @@ vfx_status vfx_k17_step_fused(const vfx_lattice* lat, int32_t is_even_timestep, 
                               vfx_stream stream);
 
 /* The collide-and-stream step of one or two lattices (the levels of a nest,
- * each at its own parity and rate) in one launch, each as vfx_k17_step_fused
- * with its own fused groups (n_post = 0 and pre = NULL: the plain K17 step,
- * VFX_TEMPLATE_DEFAULT).  UpdateGrid27 runs the levels' kernels one after
- * another (src/gpu/core/Calculation/UpdateGrid27.cpp:54-95); the levels'
- * arrays are independent, so one launch gives the same values. */
+ * each at its own parity and rate) in one launch: per level the plain K17 step
+ * of vfx_k17_step (VFX_TEMPLATE_DEFAULT, default variant, no turbulence model,
+ * no body-force arrays), structured (direct) addressing.  UpdateGrid27 runs the
+ * levels' kernels one after another (src/gpu/core/Calculation/UpdateGrid27.cpp:54-95);
+ * the levels' arrays are independent, so one launch gives the same values. */
 typedef struct vfx_level_step {
     const vfx_lattice* lattice;
     int32_t is_even_timestep;
-    int32_t n_post;
-    const vfx_fused_bc* post;
-    const vfx_fused_bc* pre;
-    const uint32_t* any_mask;
-    int32_t apply_pre;
 } vfx_level_step;
 vfx_status vfx_k17_step_levels(const vfx_level_step* levels, int32_t n_levels, vfx_stream stream);
 

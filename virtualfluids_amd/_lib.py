@@ -64,9 +64,7 @@ class VfxFusedBC(C.Structure):
 
 
 class VfxLevelStep(C.Structure):
-    _fields_ = [("lattice", C.POINTER(VfxLattice)), ("is_even_timestep", C.c_int32), ("n_post", C.c_int32),
-                ("post", C.POINTER(VfxFusedBC)), ("pre", C.POINTER(VfxFusedBC)), ("any_mask", C.c_void_p),
-                ("apply_pre", C.c_int32)]
+    _fields_ = [("lattice", C.POINTER(VfxLattice)), ("is_even_timestep", C.c_int32)]
 
 
 class VfxDomain(C.Structure):

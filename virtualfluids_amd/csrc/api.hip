@@ -422,15 +422,20 @@ vfx_status vfx_k17_step_levels(const vfx_level_step* levels, int32_t n_levels, v
     vfx::LevelInput in[2];
     for (int j = 0; j < n_levels; ++j) {
         const vfx_level_step& v = levels[j];
-        if (!v.lattice) return fail(VFX_EINVAL, "level %d: lattice is NULL", j);
-        if (v.lattice->storage != levels[0].lattice->storage) return fail(VFX_EINVAL, "levels differ in storage");
-        if (vfx_status s = fused_inputs(v.lattice, v.post, v.n_post, v.pre, in[j].L, in[j].g)) return s;
-        in[j].omega = v.lattice->omega;
-        in[j].lim = v.lattice->quadric_limiter;
-        in[j].force = v.lattice->force;
+        const vfx_lattice* lat = v.lattice;
+        if (!lat) return fail(VFX_EINVAL, "level %d: lattice is NULL", j);
+        if (lat->storage != levels[0].lattice->storage) return fail(VFX_EINVAL, "levels differ in storage");
+        if (vfx_status s = make_lat(lat, in[j].L)) return s;
+        if (!lat->direct) return fail(VFX_EUNSUPPORTED, "level %d: one launch for levels needs structured addressing", j);
+        if (lat->kernel_variant != 0 || lat->turbulence_model != VFX_TURB_NONE || lat->body_force[0] ||
+            lat->body_force[1] || lat->body_force[2])
+            return fail(VFX_EUNSUPPORTED,
+                        "level %d: one launch for levels runs the default K17 without turbulence model or body-force arrays",
+                        j);
+        in[j].omega = lat->omega;
+        in[j].lim = lat->quadric_limiter;
+        in[j].force = lat->force;
         in[j].even = v.is_even_timestep != 0;
-        in[j].any = v.any_mask;
-        in[j].apply_pre = v.apply_pre != 0;
     }
     hipStream_t st = (hipStream_t)stream;
     hipError_t e;

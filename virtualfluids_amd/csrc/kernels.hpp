@@ -1724,17 +1724,15 @@ __device__ __forceinline__ void fused_post(const Lat& L, const FusedGroup& G, ui
 // is kept by capping the register budget, so that those rarely taken paths
 // spill instead.
 template <typename S> constexpr int fused_waves() { return sizeof(S) == 8 ? 3 : 7; }
-// block `block` of `blocks` (the lattice's own numbering; XCD of a block =
-// block mod 8, so a lattice's blocks start at a multiple of 8)
 template <typename S, bool EVEN>
-__device__ __forceinline__ void fused_step_body(const Lat& L, const Rates<typename Compute<S>::T>& R,
-                                                const FusedGroup& g0, const FusedGroup& g1, const FusedGroup& gp,
-                                                const uint32_t* any, int32_t apply_pre, uint32_t block, uint32_t blocks)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves<S>())))
+    k_step_fused(Lat L, Rates<typename Compute<S>::T> R, FusedGroup g0, FusedGroup g1, FusedGroup gp,
+                 const uint32_t* any, int32_t apply_pre)
 {
     using T = typename Compute<S>::T;
     constexpr bool FAST = sizeof(S) != 8;
     constexpr bool O2 = sizeof(S) == 2;
-    const uint32_t blk = xcd_block(block, blocks, xcd_run<S>());
+    const uint32_t blk = xcd_block(blockIdx.x, gridDim.x, xcd_run<S>());
     const uint32_t row = blk / L.chunks;
     const uint32_t x = (blk - row * L.chunks) * blockDim.x + threadIdx.x;
     const uint32_t y = row % L.ny, z = row / L.ny;
@@ -1790,58 +1788,19 @@ __device__ __forceinline__ void fused_step_body(const Lat& L, const Rates<typena
     }
 }
 
-template <typename S, bool EVEN>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves<S>())))
-    k_step_fused(Lat L, Rates<typename Compute<S>::T> R, FusedGroup g0, FusedGroup g1, FusedGroup gp,
-                 const uint32_t* any, int32_t apply_pre)
-{
-    fused_step_body<S, EVEN>(L, R, g0, g1, gp, any, apply_pre, blockIdx.x, gridDim.x);
-}
-
-// Up to two lattices (the levels of a nest) stepped by one launch, each with
-// its own rates, parity and fused boundary groups: level j owns the blocks
-// [first_block, first_block + blocks), first_block a multiple of 8 (padding
-// blocks exit), blocks of 256 nodes along x.
-template <typename S> struct LevelStep {
-    Lat L;
-    Rates<typename Compute<S>::T> R;
-    FusedGroup g0, g1, gp;
-    const uint32_t* any;
-    int32_t apply_pre;
-    int32_t even;
-    uint32_t first_block, blocks;
-};
+// The coarse step and the first fine step of a nest (or any two structured
+// lattices of one storage type) in one launch: the plain step's body
+// (k_step_direct) per level, each level's parity a template argument.  Level
+// j owns the blocks [first_j, first_j + blocks_j), first_1 a multiple of 8 so
+// that the XCD-aware tile order holds per level (padding blocks exit), blocks
+// of 256 nodes along x.
 template <typename S> struct LevelPair {
-    LevelStep<S> lv[2];
-    uint32_t n;
-};
-template <typename S>
-__device__ __forceinline__ void level_block(const LevelStep<S>& V)
-{
-    const uint32_t b = blockIdx.x - V.first_block;
-    if (b >= V.blocks) return;
-    if (V.even) fused_step_body<S, true>(V.L, V.R, V.g0, V.g1, V.gp, V.any, V.apply_pre, b, V.blocks);
-    else fused_step_body<S, false>(V.L, V.R, V.g0, V.g1, V.gp, V.any, V.apply_pre, b, V.blocks);
-}
-template <typename S>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_waves<S>())))
-    k_step_levels(LevelPair<S> P)
-{
-    // a constant index per branch: the kernel arguments are read in place
-    // (an index computed at run time would copy the structs to scratch)
-    if (P.n > 1 && blockIdx.x >= P.lv[1].first_block) level_block<S>(P.lv[1]);
-    else level_block<S>(P.lv[0]);
-}
-// The same launch for levels without fused boundary groups: the plain step's
-// body (k_step_direct) per level, each level's parity a template argument, so
-// no boundary code is compiled in.
-template <typename S> struct PlainLevels {
     Lat L0, L1;
     Rates<typename Compute<S>::T> R0, R1;
     uint32_t n, blocks0, first1, blocks1;
 };
 template <typename S, bool E0, bool E1>
-__global__ void __launch_bounds__(256) k_step_levels_plain(PlainLevels<S> P)
+__global__ void __launch_bounds__(256) k_step_levels(LevelPair<S> P)
 {
     constexpr bool FAST = sizeof(S) != 8;
     if (P.n > 1 && blockIdx.x >= P.first1) {
@@ -2223,17 +2182,13 @@ hipError_t launch_bc_set(const BCJob* jobs, uint32_t njobs, uint32_t blocks, uin
     return hipGetLastError();
 }
 
-// The levels of one launch: per level its lattice, rates inputs, parity, fused
-// groups (g[0], g[1] post, g[2] pressure), union mask and apply_pre.
+// The levels of one launch: per level its lattice, rates inputs and parity.
 struct LevelInput {
     Lat L;
     double omega;
     const double* lim;
     const double* force;
     bool even;
-    FusedGroup g[3];
-    const uint32_t* any;
-    int apply_pre;
 };
 template <typename S> hipError_t launch_step_levels(const LevelInput* in, int n, hipStream_t st)
 {
@@ -2242,53 +2197,27 @@ template <typename S> hipError_t launch_step_levels(const LevelInput* in, int n,
     LevelPair<S> P;
     std::memset(&P, 0, sizeof(P));
     P.n = (uint32_t)n;
+    Lat* L[2] = {&P.L0, &P.L1};
+    Rates<T>* R[2] = {&P.R0, &P.R1};
+    uint32_t* blocks[2] = {&P.blocks0, &P.blocks1};
     uint32_t first = 0;
     for (int j = 0; j < n; ++j) {
-        LevelStep<S>& V = P.lv[j];
-        V.L = in[j].L;
-        V.L.chunks = (V.L.nx + 255u) / 256u; // blocks of 256 nodes in this launch
-        V.R = make_rates<T>(in[j].omega, in[j].lim, in[j].force, 0.0);
-        V.g0 = in[j].g[0];
-        V.g1 = in[j].g[1];
-        V.gp = in[j].g[2];
-        V.any = in[j].any;
-        V.apply_pre = in[j].apply_pre;
-        V.even = in[j].even ? 1 : 0;
-        V.first_block = first;
-        V.blocks = V.L.chunks * V.L.ny * V.L.nz;
-        first += (V.blocks + kXcds - 1) / kXcds * kXcds;
+        *L[j] = in[j].L;
+        L[j]->chunks = (L[j]->nx + 255u) / 256u; // blocks of 256 nodes in this launch
+        *R[j] = make_rates<T>(in[j].omega, in[j].lim, in[j].force, 0.0);
+        *blocks[j] = L[j]->chunks * L[j]->ny * L[j]->nz;
+        if (j == 1) P.first1 = first;
+        first += (*blocks[j] + kXcds - 1) / kXcds * kXcds;
     }
     if (first == 0) return hipSuccess;
-    bool plain = true;
-    for (int j = 0; j < n; ++j)
-        plain = plain && !in[j].g[0].mask && !in[j].g[1].mask && !(in[j].g[2].mask && in[j].apply_pre);
-    if (plain) {
-        static_assert(sizeof(PlainLevels<S>) <= 4000, "kernel arguments");
-        PlainLevels<S> Q;
-        std::memset(&Q, 0, sizeof(Q));
-        Q.n = (uint32_t)n;
-        Q.L0 = P.lv[0].L;
-        Q.R0 = P.lv[0].R;
-        Q.blocks0 = P.lv[0].blocks;
-        const bool e0 = P.lv[0].even != 0;
-        bool e1 = e0;
-        if (n > 1) {
-            Q.L1 = P.lv[1].L;
-            Q.R1 = P.lv[1].R;
-            Q.first1 = P.lv[1].first_block;
-            Q.blocks1 = P.lv[1].blocks;
-            e1 = P.lv[1].even != 0;
-        }
-        const dim3 g(first), b(256);
-        switch ((e0 ? 2 : 0) + (e1 ? 1 : 0)) {
-        case 3: hipLaunchKernelGGL((k_step_levels_plain<S, true, true>), g, b, 0, st, Q); break;
-        case 2: hipLaunchKernelGGL((k_step_levels_plain<S, true, false>), g, b, 0, st, Q); break;
-        case 1: hipLaunchKernelGGL((k_step_levels_plain<S, false, true>), g, b, 0, st, Q); break;
-        default: hipLaunchKernelGGL((k_step_levels_plain<S, false, false>), g, b, 0, st, Q); break;
-        }
-        return hipGetLastError();
+    const bool e0 = in[0].even, e1 = n > 1 ? in[1].even : e0;
+    const dim3 g(first), b(256);
+    switch ((e0 ? 2 : 0) + (e1 ? 1 : 0)) {
+    case 3: hipLaunchKernelGGL((k_step_levels<S, true, true>), g, b, 0, st, P); break;
+    case 2: hipLaunchKernelGGL((k_step_levels<S, true, false>), g, b, 0, st, P); break;
+    case 1: hipLaunchKernelGGL((k_step_levels<S, false, true>), g, b, 0, st, P); break;
+    default: hipLaunchKernelGGL((k_step_levels<S, false, false>), g, b, 0, st, P); break;
     }
-    hipLaunchKernelGGL((k_step_levels<S>), dim3(first), dim3(256), 0, st, P);
     return hipGetLastError();
 }
 
