@@ -206,3 +206,21 @@ def test_without_union_mask(cuda):
     sc.step(5)
     torch.cuda.synchronize()
     assert _same(a, c) and _same(b, c)
+
+
+@pytest.mark.parametrize("storage", FORMS)
+def test_slip_group(cuda, storage):
+    """The channel's walls and spheres as slip boundaries (kind 3) fused with
+    the inlet: slip marks its face axes in link order, so its nodes take the
+    per-node path also in wavefronts whose other groups go link-parallel."""
+    print(f"[fused slip channel {storage}]", flush=True)
+    ch = small_channel()
+    tables = []
+    for t in ch.bcs:
+        if t.kind == 1:
+            t = geometry.BCTable(3, t.nodes, t.q27)
+        tables.append(t)
+    a, b, st = _pair(ch.dims, ch.solid, ch.omega, tables, storage, ch.u_in, (5, 6))
+    plan = st.fused_plan()
+    assert any(g.bc.kind == 3 for g in plan.post)
+    assert _same(a, b)

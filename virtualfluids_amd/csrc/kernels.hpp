@@ -1110,6 +1110,114 @@ __global__ void __launch_bounds__(256) k_bc_velocity_bb(Lat L, BCDev B)
 // boundary node, density and velocity (with the sum order those kernels use),
 // the symmetric equilibrium part of each wall link, and the Bouzidi-type
 // interpolation with the subgrid distance q.  KIND: 1 no-slip, 2 velocity, 3 slip.
+//
+// The node part of the interpolated boundaries: density, velocity, rates and
+// the boundary velocity of the node's post-collision populations p.
+template <typename T> struct InterpNode {
+    T drho, vx1, vx2, vx3, cu_sq, omega, inv1mo, VX, VY, VZ;
+};
+template <typename T, int KIND>
+__device__ __forceinline__ InterpNode<T> interp_node(const Lat& L, const BCDev& B, uint32_t i, uint32_t k,
+                                                     const T (&p)[27], int kd)
+{
+    // no contraction into FMAs here: which product the compiler would fuse
+    // depends on the code around the inlined body (a shared subexpression
+    // changes the choice), and the separate boundary kernel and the fused
+    // collide-stream kernel must give the same bits
+#pragma clang fp contract(off)
+    InterpNode<T> n;
+    const T drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] + p[d0MP] +
+                   p[d0PP] + p[d0MM] + p[dP0M] + p[dM0P] + p[dP0P] + p[dM0M] + p[dPM0] + p[dMP0] + p[dPP0] + p[dMM0] +
+                   p[d00P] + p[d00M] + p[d0P0] + p[d0M0] + p[dP00] + p[dM00] + p[d000];
+    n.drho = drho;
+    // division by 1 + drho: exact in FP64 (reference order), the hardware
+    // reciprocal in FP32
+    const T ovr = sizeof(T) == 4 ? T(__builtin_amdgcn_rcpf(float(T(1) + drho))) : T(0);
+    auto per_rho = [&](T x) -> T {
+        if constexpr (sizeof(T) == 4) return x * ovr;
+        else return x / (T(1) + drho);
+    };
+    n.vx1 = per_rho((((p[dPMP] - p[dMPM]) - (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                     ((p[dP0M] - p[dM0P]) + (p[dP0P] - p[dM0M])) + ((p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
+                     (p[dP00] - p[dM00])));
+    n.vx2 = per_rho(((-(p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
+                     ((p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + (-(p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
+                     (p[d0P0] - p[d0M0])));
+    n.vx3 = per_rho((((p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) + (p[dMMP] - p[dPPM])) +
+                     (-(p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + ((p[dP0P] - p[dM0M]) - (p[dP0M] - p[dM0P])) +
+                     (p[d00P] - p[d00M])));
+    n.cu_sq = T(1.5) * (n.vx1 * n.vx1 + n.vx2 * n.vx2 + n.vx3 * n.vx3) * (T(1) + drho);
+    T omega = (T)B.omega;
+    // SlipTurbulentViscosityCompressible.cu:130-134: the node's rate with its turbulent viscosity
+    if constexpr (KIND == VFX_BC_SLIP_TURBULENT_VISCOSITY)
+        omega = omega / (T(1) + T(3) * omega * fmax(T(0), ((const T*)L.nut)[k]));
+    n.omega = omega;
+    n.inv1mo = sizeof(T) == 4 ? T(1) / (T(1) - omega) : T(0);
+    n.VX = n.VY = n.VZ = T(0);
+    return n;
+}
+// the wall velocity of the velocity kinds
+template <typename T> __device__ __forceinline__ void interp_wall_velocity(InterpNode<T>& n, const BCDev& B, uint32_t i, int kd)
+{
+    if (kd == VFX_BC_VELOCITY_INTERPOLATED || kd == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED ||
+        kd == VFX_BC_PRECURSOR_VELOCITY) {
+        n.VX = B.vx ? bc_at<T>(B.vx, i) : T(0);
+        n.VY = B.vy ? bc_at<T>(B.vy, i) : T(0);
+        n.VZ = B.vz ? bc_at<T>(B.vz, i) : T(0);
+    }
+}
+
+// c_d . u with the direction's components given at run time (the order and
+// signs of proj())
+template <typename T> __device__ __forceinline__ T proj_rt(int ex, int ey, int ez, T ux, T uy, T uz)
+{
+    T v;
+    if (ex != 0) {
+        v = ex > 0 ? ux : -ux;
+        if (ey != 0) v = ey > 0 ? v + uy : v - uy;
+        if (ez != 0) v = ez > 0 ? v + uz : v - uz;
+    } else if (ey != 0) {
+        v = ey > 0 ? uy : -uy;
+        if (ez != 0) v = ez > 0 ? v + uz : v - uz;
+    } else {
+        v = ez > 0 ? uz : -uz;
+    }
+    return v;
+}
+
+// The population re-entering along a wall link of fraction q (0 <= q <= 1):
+// f, fi the node's post-collision populations of the link and its opposite,
+// w the link weight, vlb = c . u of the node, vbc = c . u of the wall.
+template <typename T>
+__device__ __forceinline__ T interp_link_value(const InterpNode<T>& n, int kd, T w, T vlb, T vbc, T q, T f, T fi)
+{
+#pragma clang fp contract(off)
+    const T feq = w * (n.drho + T(4.5) * vlb * vlb * (T(1) + n.drho) - n.cu_sq);
+    // FP64 keeps the reference's divisions (bit parity with the CPU
+    // restatement); the FP32 arithmetic of the reduced forms uses one
+    // hardware reciprocal per link and 1/(1 - omega) once per node
+    T a, r1q;
+    if constexpr (sizeof(T) == 4) {
+        r1q = __builtin_amdgcn_rcpf(T(1) + q);
+        a = (T(1) - q) * r1q * (f - fi + (f + fi - T(2) * feq * n.omega) * n.inv1mo) * T(0.5);
+    } else {
+        r1q = T(0);
+        a = (T(1) - q) / (T(1) + q) * (f - fi + (f + fi - T(2) * feq * n.omega) / (T(1) - n.omega)) * T(0.5);
+    }
+    T v;
+    if (kd == VFX_BC_NOSLIP_INTERPOLATED) {
+        if constexpr (sizeof(T) == 4) v = a + q * (f + fi) * r1q;
+        else v = a + (q * (f + fi)) / (T(1) + q);
+    } else {
+        if constexpr (sizeof(T) == 4) v = a + (q * (f + fi) - T(6) * w * vbc) * r1q;
+        else v = a + (q * (f + fi) - T(6) * w * vbc) / (T(1) + q);
+        // getInterpolatedDistributionForVeloWithPressureBC (KernelUtilities.h:207-213):
+        // the entering population carries the bulk pressure
+        if (kd == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED || kd == VFX_BC_PRECURSOR_VELOCITY) v = v - w * n.drho;
+    }
+    return v;
+}
+
 // The links of boundary entry i at node k from its post-collision
 // populations p (indexed by direction): computes and stores the population
 // re-entering along every wall link.  KIND >= 0 fixes the kind at compile
@@ -1118,64 +1226,30 @@ template <typename S, typename T, bool EVEN, int KIND>
 __device__ __forceinline__ void interp_links(const Lat& L, const BCDev& B, uint32_t i, uint32_t k,
                                              const uint32_t (&kk)[8], const T (&p)[27], int kind = KIND)
 {
-    // no contraction into FMAs here: which product the compiler would fuse
-    // depends on the code around the inlined body (a shared subexpression
-    // changes the choice), and the separate boundary kernel and the fused
-    // collide-stream kernel must give the same bits
 #pragma clang fp contract(off)
     const int kd = KIND >= 0 ? KIND : kind;
-    const T drho = p[dPMP] + p[dMPP] + p[dPPP] + p[dMMP] + p[dPMM] + p[dMPM] + p[dPPM] + p[dMMM] + p[d0PM] + p[d0MP] +
-                   p[d0PP] + p[d0MM] + p[dP0M] + p[dM0P] + p[dP0P] + p[dM0M] + p[dPM0] + p[dMP0] + p[dPP0] + p[dMM0] +
-                   p[d00P] + p[d00M] + p[d0P0] + p[d0M0] + p[dP00] + p[dM00] + p[d000];
-    // division by 1 + drho: exact in FP64 (reference order), the hardware
-    // reciprocal in FP32
-    const T ovr = sizeof(T) == 4 ? T(__builtin_amdgcn_rcpf(float(T(1) + drho))) : T(0);
-    auto per_rho = [&](T x) -> T {
-        if constexpr (sizeof(T) == 4) return x * ovr;
-        else return x / (T(1) + drho);
-    };
-    const T vx1 = per_rho((((p[dPMP] - p[dMPM]) - (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
-                   ((p[dP0M] - p[dM0P]) + (p[dP0P] - p[dM0M])) + ((p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
-                   (p[dP00] - p[dM00])));
-    const T vx2 = per_rho(((-(p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) - (p[dMMP] - p[dPPM])) +
-                   ((p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + (-(p[dPM0] - p[dMP0]) + (p[dPP0] - p[dMM0])) +
-                   (p[d0P0] - p[d0M0])));
-    const T vx3 = per_rho((((p[dPMP] - p[dMPM]) + (p[dMPP] - p[dPMM])) + ((p[dPPP] - p[dMMM]) + (p[dMMP] - p[dPPM])) +
-                   (-(p[d0PM] - p[d0MP]) + (p[d0PP] - p[d0MM])) + ((p[dP0P] - p[dM0M]) - (p[dP0M] - p[dM0P])) +
-                   (p[d00P] - p[d00M])));
-    const T cu_sq = T(1.5) * (vx1 * vx1 + vx2 * vx2 + vx3 * vx3) * (T(1) + drho);
-    T omega = (T)B.omega;
-    // SlipTurbulentViscosityCompressible.cu:130-134: the node's rate with its turbulent viscosity
-    if constexpr (KIND == VFX_BC_SLIP_TURBULENT_VISCOSITY)
-        omega = omega / (T(1) + T(3) * omega * fmax(T(0), ((const T*)L.nut)[k]));
-    const T inv1mo = sizeof(T) == 4 ? T(1) / (T(1) - omega) : T(0);
+    InterpNode<T> n = interp_node<T, KIND>(L, B, i, k, p, kd);
     // all link fractions first: loaded inside the link loop, each q waited for
     // the previous link's store as well (one memory counter covers both) and
     // the 26 links ran as 26 round trips
     T qs[27];
 #pragma unroll
     for (int d = 1; d < 27; ++d) qs[d] = bc_q<T>(B, d, i);
-    T VX = 0, VY = 0, VZ = 0;
-    if (kd == VFX_BC_VELOCITY_INTERPOLATED || kd == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED ||
-        kd == VFX_BC_PRECURSOR_VELOCITY) {
-        VX = B.vx ? bc_at<T>(B.vx, i) : T(0);
-        VY = B.vy ? bc_at<T>(B.vy, i) : T(0);
-        VZ = B.vz ? bc_at<T>(B.vz, i) : T(0);
-    }
+    interp_wall_velocity<T>(n, B, i, kd);
     if constexpr (KIND == VFX_BC_PRECURSOR_VELOCITY) {
         // PrecursorNonReflectiveCompressible.cu:112-114: (velocityX + (1 - tr) last + tr current) / velocityRatio
         const PlaneStencil<T> s = plane_stencil<T>(B, i);
         if (!s.ok) return; // malformed plane index: never read outside the planes
-        VX += (T)B.pu[0];
-        VY += (T)B.pu[1];
-        VZ += (T)B.pu[2];
+        n.VX += (T)B.pu[0];
+        n.VY += (T)B.pu[1];
+        n.VZ += (T)B.pu[2];
         const T tr = (T)B.tratio, vr = (T)B.vratio;
         const T* last = (const T*)B.plast;
         const T* cur = (const T*)B.pcur;
         const size_t np = B.n_pre;
-        VX = (VX + (T(1) - tr) * plane_value(s, last) + tr * plane_value(s, cur)) / vr;
-        VY = (VY + (T(1) - tr) * plane_value(s, last + np) + tr * plane_value(s, cur + np)) / vr;
-        VZ = (VZ + (T(1) - tr) * plane_value(s, last + 2 * np) + tr * plane_value(s, cur + 2 * np)) / vr;
+        n.VX = (n.VX + (T(1) - tr) * plane_value(s, last) + tr * plane_value(s, cur)) / vr;
+        n.VY = (n.VY + (T(1) - tr) * plane_value(s, last + np) + tr * plane_value(s, cur + np)) / vr;
+        n.VZ = (n.VZ + (T(1) - tr) * plane_value(s, last + 2 * np) + tr * plane_value(s, cur + 2 * np)) / vr;
     }
     bool fx = false, fy = false, fz = false; // slip: axes normal to a face link
 #pragma unroll
@@ -1183,8 +1257,7 @@ __device__ __forceinline__ void interp_links(const Lat& L, const BCDev& B, uint3
         const T q = qs[d];
         if (!(q >= T(0) && q <= T(1))) continue;
         const T w = weight<T>(d);
-        const T vlb = proj<T>(d, vx1, vx2, vx3);
-        const T feq = w * (drho + T(4.5) * vlb * vlb * (T(1) + drho) - cu_sq);
+        const T vlb = proj<T>(d, n.vx1, n.vx2, n.vx3);
         T f = p[d], fi = p[inv(d)];
         if constexpr (KIND == VFX_BC_PRECURSOR_VELOCITY) {
             // kept from PrecursorNonReflectiveCompressible.cu:383 and :392: the
@@ -1192,40 +1265,19 @@ __device__ __forceinline__ void interp_links(const Lat& L, const BCDev& B, uint3
             if (d == d0PM) fi = p[d0PP];
             if (d == d0MP) f = p[d0PP];
         }
-        // FP64 keeps the reference's divisions (bit parity with the CPU
-        // restatement); the FP32 arithmetic of the reduced forms uses one
-        // hardware reciprocal per link and 1/(1 - omega) once per node
-        T a, r1q;
-        if constexpr (sizeof(T) == 4) {
-            r1q = __builtin_amdgcn_rcpf(T(1) + q);
-            a = (T(1) - q) * r1q * (f - fi + (f + fi - T(2) * feq * omega) * inv1mo) * T(0.5);
-        } else {
-            r1q = T(0);
-            a = (T(1) - q) / (T(1) + q) * (f - fi + (f + fi - T(2) * feq * omega) / (T(1) - omega)) * T(0.5);
-        }
-        T v;
-        if (kd == VFX_BC_NOSLIP_INTERPOLATED) {
-            if constexpr (sizeof(T) == 4) v = a + q * (f + fi) * r1q;
-            else v = a + (q * (f + fi)) / (T(1) + q);
-        } else {
-            T vbc;
-            if (kd == VFX_BC_SLIP || kd == VFX_BC_SLIP_TURBULENT_VISCOSITY) {
-                if (weight_class(d) == 1) {
-                    if (cx(d) != 0) fx = true;
-                    if (cy(d) != 0) fy = true;
-                    if (cz(d) != 0) fz = true;
-                }
-                const T sx = fx ? T(0) : vx1, sy = fy ? T(0) : vx2, sz = fz ? T(0) : vx3;
-                vbc = proj<T>(d, sx, sy, sz);
-            } else {
-                vbc = proj<T>(d, VX, VY, VZ);
+        T vbc = T(0);
+        if (kd == VFX_BC_SLIP || kd == VFX_BC_SLIP_TURBULENT_VISCOSITY) {
+            if (weight_class(d) == 1) {
+                if (cx(d) != 0) fx = true;
+                if (cy(d) != 0) fy = true;
+                if (cz(d) != 0) fz = true;
             }
-            if constexpr (sizeof(T) == 4) v = a + (q * (f + fi) - T(6) * w * vbc) * r1q;
-            else v = a + (q * (f + fi) - T(6) * w * vbc) / (T(1) + q);
-            // getInterpolatedDistributionForVeloWithPressureBC (KernelUtilities.h:207-213):
-            // the entering population carries the bulk pressure
-            if (kd == VFX_BC_VELOCITY_PRESSURE_INTERPOLATED || kd == VFX_BC_PRECURSOR_VELOCITY) v = v - w * drho;
+            const T sx = fx ? T(0) : n.vx1, sy = fy ? T(0) : n.vx2, sz = fz ? T(0) : n.vx3;
+            vbc = proj<T>(d, sx, sy, sz);
+        } else if (kd != VFX_BC_NOSLIP_INTERPOLATED) {
+            vbc = proj<T>(d, n.VX, n.VY, n.VZ);
         }
+        const T v = interp_link_value<T>(n, kd, w, vlb, vbc, q, f, fi);
         const int e = inv(d);
         // the precursor inlet writes with the parity's own pointers
         // (PrecursorNonReflectiveCompressible.cu:142): over the outgoing
@@ -1716,6 +1768,94 @@ __device__ __forceinline__ void fused_post(const Lat& L, const FusedGroup& G, ui
     interp_links<S, T, EVEN, -1>(L, G.B, i, kb[0], kb, p, G.kind);
 }
 
+// Link-parallel form of fused_post for a wavefront with few boundary nodes:
+// the wavefront takes its boundary nodes one after another, lane x (< 27)
+// loads the node's post-collision population of direction x, the lanes share
+// them, and lane d (1..26) computes and stores the re-entering population of
+// link d.  The node part and every link value are the expressions of
+// interp_links (kinds 1, 2, 5; slip marks its face axes in link order and
+// stays per node), so the results are the same bits.
+constexpr uint32_t kLinkParallelNodes = 8; // boundary nodes per wavefront up to which links go across lanes
+struct LaneDir {
+    int x;              // direction of this lane (0 for lanes >= 27)
+    int ix;             // the opposite direction
+    int ex, ey, ez;     // components of x
+    uint32_t rd_plane;  // plane of the node's post-collision population x
+    uint32_t wr_plane;  // plane of the re-entering population of link x
+    uint32_t nbsel;     // which of the node's 8 neighbour indices holds both
+    float ld, sh, st, ss;
+};
+template <bool EVEN> __device__ __forceinline__ LaneDir lane_dir(const Lat& L, int lane)
+{
+    LaneDir r;
+    const int x = lane < 27 ? lane : 0;
+    r.x = x;
+    int ex = 0, ey = 0, ez = 0, ix = 0;
+    float sh = 0.f, ss = 0.f;
+    // constant-index selects: the kernel argument arrays stay in place
+#pragma unroll
+    for (int d = 0; d < 27; ++d) {
+        if (x == d) {
+            ex = cx(d);
+            ey = cy(d);
+            ez = cz(d);
+            ix = inv(d);
+            sh = L.sh[d];
+            ss = L.ss[inv(d)];
+        }
+    }
+    r.ix = ix;
+    r.ex = ex;
+    r.ey = ey;
+    r.ez = ez;
+    // population x is read where direction inv(x) is read (slot(inv(x), EVEN));
+    // link x re-enters as inv(x), written at slot(inv(x), !EVEN); both at the
+    // neighbour that is one step along every axis where c_x > 0
+    r.rd_plane = (uint32_t)(EVEN ? ix : x);
+    r.wr_plane = (uint32_t)(EVEN ? x : ix);
+    r.nbsel = (uint32_t)((ex > 0) | ((ey > 0) << 1) | ((ez > 0) << 2));
+    const int c = (ex != 0) + (ey != 0) + (ez != 0);
+    r.ld = c == 0 ? L.ld[0] : c == 1 ? L.ld[1] : c == 2 ? L.ld[2] : L.ld[3];
+    r.st = c == 0 ? L.st[0] : c == 1 ? L.st[1] : c == 2 ? L.st[2] : L.st[3];
+    r.sh = sh;
+    r.ss = ss;
+    return r;
+}
+template <typename T> __device__ __forceinline__ T weight_rt(int ex, int ey, int ez)
+{
+    const int c = (ex != 0) + (ey != 0) + (ez != 0);
+    return c == 0 ? T(8. / 27.) : c == 1 ? T(2. / 27.) : c == 2 ? T(1. / 54.) : T(1. / 216.);
+}
+template <typename S, typename T, bool EVEN>
+__device__ __forceinline__ void fused_post_links(const Lat& L, const FusedGroup& G, uint32_t kb, int lane,
+                                                 const LaneDir& D)
+{
+    uint32_t i;
+    if (!fused_entry(G, kb, i)) return; // the same for every lane
+    uint32_t nb[8];
+    nbrs_direct(L, kb, nb);
+    uint32_t idx = nb[0];
+#pragma unroll
+    for (int j = 1; j < 8; ++j)
+        if (D.nbsel == (uint32_t)j) idx = nb[j];
+    const T px = Storage<S, T>::ld(plane<S>(L, (int)D.rd_plane), idx, D.ld, D.sh);
+    T p[27];
+#pragma unroll
+    for (int d = 0; d < 27; ++d) p[d] = __shfl(px, d, 64);
+    const int kd = G.kind;
+    InterpNode<T> n = interp_node<T, -1>(L, G.B, i, kb, p, kd);
+    T q = T(-1);
+    if (lane >= 1 && lane < 27) q = bc_q<T>(G.B, lane, i);
+    interp_wall_velocity<T>(n, G.B, i, kd);
+    const T fi = __shfl(px, D.ix, 64);
+    if (!(lane >= 1 && lane < 27) || !(q >= T(0) && q <= T(1))) return;
+    const T w = weight_rt<T>(D.ex, D.ey, D.ez);
+    const T vlb = proj_rt<T>(D.ex, D.ey, D.ez, n.vx1, n.vx2, n.vx3);
+    const T vbc = kd == VFX_BC_NOSLIP_INTERPOLATED ? T(0) : proj_rt<T>(D.ex, D.ey, D.ez, n.VX, n.VY, n.VZ);
+    const T v = interp_link_value<T>(n, kd, w, vlb, vbc, q, px, fi);
+    Storage<S, T>::st(plane<S>(L, (int)D.wr_plane), idx, v, D.st, D.ss);
+}
+
 // K17 (default operation order of the storage form) with up to two fused
 // post-collision groups and one fused pressure group; apply_pre = 0 on the
 // first step of a run (no pressure boundary ran before it).
@@ -1771,15 +1911,43 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(fused_
             }
         }
     }
-    if (!active) return;
-    collide<T, VFX_K17, FAST, O2>(m, R, mv, nut);
-    scatter_post<S, T, EVEN, O2>(L, kk, m);
+    // lanes without a node stay for the link-parallel boundary path below
+    if (active) {
+        collide<T, VFX_K17, FAST, O2>(m, R, mv, nut);
+        scatter_post<S, T, EVEN, O2>(L, kk, m);
+    }
     // the boundary code reads the stores above back from memory
     asm volatile("" ::: "memory");
-    // only nodes of a group go on (whatever the boundary code needs is set up
-    // behind this branch, not on every node)
+    // only wavefronts with nodes of a group go on (whatever the boundary code
+    // needs is set up behind this branch, not on every node)
     const uint32_t k = kk[0];
-    if (!member || !(in_group(g0, k) || in_group(g1, k))) return;
+    const bool bc = active && member && (in_group(g0, k) || in_group(g1, k));
+    const uint64_t any_bc = __ballot(bc);
+    if (!any_bc) return;
+    const int lane = (int)(threadIdx.x & 63u);
+    if (__popcll(any_bc) <= kLinkParallelNodes) {
+        // the stores of other lanes are read back
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        const LaneDir D = lane_dir<EVEN>(L, lane);
+#pragma unroll 1
+        for (int j = 0; j < 2; ++j) {
+            const FusedGroup& G = j == 0 ? g0 : g1;
+            if (!G.mask) continue;
+            const bool mine = bc && in_group(G, k);
+            uint64_t bal = __ballot(mine);
+            if (G.kind == VFX_BC_SLIP) {
+                if (mine) fused_post<S, T, EVEN>(L, G, k);
+                continue;
+            }
+            while (bal) {
+                const int b = __builtin_ctzll(bal);
+                bal &= bal - 1;
+                fused_post_links<S, T, EVEN>(L, G, (uint32_t)__shfl((int)k, b, 64), lane, D);
+            }
+        }
+        return;
+    }
+    if (!bc) return;
     // one inlined copy of the boundary code serves both groups
 #pragma unroll 1
     for (int j = 0; j < 2; ++j) {
