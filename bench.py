@@ -44,6 +44,8 @@ def parse():
                    help="storage[:plain]; plain = reference operation order")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="time the built-in launch shape instead of the autotuned one")
     p.add_argument("--ensemble", type=int, default=512,
                    help="cavities in the batched-ensemble leg (0 skips it)")
     p.add_argument("--ensemble-steps", type=int, default=100, help="steps per captured graph of the ensemble leg")
@@ -108,12 +110,18 @@ def settle(torch, st, seconds):
     return n
 
 
-def run_form(torch, vfx, form, fields, steps, warmup, settle_s):
+def run_form(torch, vfx, form, fields, steps, warmup, settle_s, autotune=True):
     storage, _, variant = form.partition(":")
     lat = vfx.Lattice(DIMS, storage=storage, omega=OMEGA, device="cuda", variant=variant or "default")
     lat.init_equilibrium(*fields, even=True)
     st = vfx.Stepper(lat, "k17")
     settled = settle(torch, st, settle_s)
+    launch = {"tile_nodes": 256, "xcd_run": 0, "autotuned": False}
+    if autotune:
+        # vfx_k17_autotune: tile width and XCD run of the structured step,
+        # timed on the settled state; the timed steps run the fastest
+        tile, run, ms_t = lat.autotune(st.even, steps=4)
+        launch = {"tile_nodes": tile, "xcd_run": run, "autotuned": True, "autotune_ms_per_step": round(ms_t, 4)}
     st.step(warmup)
     stream = torch.cuda.current_stream()
     torch.cuda.synchronize()
@@ -128,7 +136,7 @@ def run_form(torch, vfx, form, fields, steps, warmup, settle_s):
     ms = e0.elapsed_time(e1)
     macro = lat.macroscopic(st.even)
     del lat
-    return {"ms_per_step": ms / steps, "wall_s": wall, "macro": macro, "settle_steps": settled}
+    return {"ms_per_step": ms / steps, "wall_s": wall, "macro": macro, "settle_steps": settled, "launch": launch}
 
 
 def velocity_l2(a, b):
@@ -585,7 +593,7 @@ def main():
     forms = [f.strip() for f in args.forms.split(",") if f.strip()]
     res = {}
     for form in forms:
-        r = run_form(torch, vfx, form, fields, args.steps, args.warmup, args.settle_seconds)
+        r = run_form(torch, vfx, form, fields, args.steps, args.warmup, args.settle_seconds, not args.no_autotune)
         r["mlups"] = n_nodes / (r["ms_per_step"] * 1e-3) / 1e6
         r["b_alg"] = 54 * vfx.BYTES[form.split(":")[0]]
         r["gbs"] = r["b_alg"] * n_nodes / (r["ms_per_step"] * 1e-3) / 1e9
@@ -615,7 +623,7 @@ def main():
                      "bytes_per_lup": h["b_alg"]},
         "forms": {f: {"mlups": round(r["mlups"], 1), "ms_per_step": round(r["ms_per_step"], 4),
                       "achieved_gbs": round(r["gbs"], 1), "frac_hbm_peak": round(r["gbs"] / PEAK_HBM_GBS, 4),
-                      "bytes_per_lup": r["b_alg"]} for f, r in res.items()},
+                      "bytes_per_lup": r["b_alg"], "launch": r["launch"]} for f, r in res.items()},
     }
     if "fp64" in res:
         ref = res["fp64"]["macro"]

@@ -139,6 +139,16 @@ typedef struct vfx_lattice {
     void* body_force[3];
     double forces[3];
     double force_factor;
+
+    /* Launch shape of the structured (direct) collide-and-stream step: nodes
+     * per workgroup along x (64, 128 or 256; 0 = 256) and the XCD tile run
+     * (workgroups reach the 8 XCDs round robin; consecutive tiles are handed
+     * to each XCD in runs of this many, a power of two up to 4096, 1 = plain
+     * order; 0 = the built-in choice, 16 for 2- and 4-byte populations, 256
+     * for 8-byte ones).
+     * vfx_k17_autotune measures them.  Other kernels ignore both. */
+    uint32_t tile_nodes;
+    uint32_t xcd_run;
 } vfx_lattice;
 
 /* vf::lbm::TurbulenceModel */
@@ -258,6 +268,16 @@ vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t
                               int32_t collision_template, vfx_stream stream);
 vfx_status vfx_k17_step(const vfx_lattice* lat, int32_t is_even_timestep, int32_t collision_template, vfx_stream stream);
 vfx_status vfx_bgk_step(const vfx_lattice* lat, int32_t is_even_timestep, vfx_stream stream);
+
+/* (new) Autotuner of the structured K17 step: times vfx_k17_step with every
+ * tile width (64, 128, 256 nodes) and XCD run (1, 4, 8, 16, 32, 64, 256) on
+ * `steps` (made even, at least 2) steps after two warm-up steps each, from
+ * parity is_even_timestep, and returns the fastest shape (*tile_nodes,
+ * *xcd_run, its ms per step in *ms) for the caller to set in its
+ * vfx_lattice.  The populations advance (42 trials), the parity is the same
+ * afterwards.  Direct addressing only. */
+vfx_status vfx_k17_autotune(const vfx_lattice* lat, int32_t is_even_timestep, int32_t collision_template,
+                            int32_t steps, vfx_stream stream, uint32_t* tile_nodes, uint32_t* xcd_run, float* ms);
 
 /* Boundary conditions fused into the collide-and-stream kernel of a
  * structured lattice, so a boundary node's populations are not read again by

@@ -162,3 +162,27 @@ def test_host_layer_steps_like_c_abi(cuda):
     r = subprocess.run([exe, "gpu"], capture_output=True, text=True, timeout=120)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_launch_shape_arguments_fail_without_device():
+    """tile_nodes and xcd_run of the structured step are checked before any
+    launch; the autotuner takes structured lattices only."""
+    L = vfx.lib()
+    s = vfx._lib.VfxLattice()
+    s.f = 1
+    s.n_nodes = 8
+    s.omega = 1.5
+    s.direct = 1
+    s.dims[:] = [2, 2, 2]
+    s.tile_nodes = 100
+    assert L.vfx_k17_step(C.byref(s), 1, 0, None) == vfx._lib.VFX_EINVAL
+    assert b"tile_nodes" in L.vfx_last_error()
+    s.tile_nodes = 128
+    for run in (5000, 8192, 12):
+        s.xcd_run = run
+        assert L.vfx_k17_step(C.byref(s), 1, 0, None) == vfx._lib.VFX_EINVAL
+        assert b"xcd_run" in L.vfx_last_error()
+    t, r, ms = C.c_uint32(), C.c_uint32(), C.c_float()
+    s.direct = 0
+    assert L.vfx_k17_autotune(C.byref(s), 1, 0, 4, None, C.byref(t), C.byref(r), C.byref(ms)) == vfx._lib.VFX_EUNSUPPORTED
+    assert L.vfx_k17_autotune(None, 1, 0, 4, None, C.byref(t), C.byref(r), C.byref(ms)) == vfx._lib.VFX_EINVAL

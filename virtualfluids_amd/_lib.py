@@ -41,6 +41,7 @@ class VfxLattice(C.Structure):
         ("turbulence_model", C.c_int32), ("sgs_constant", C.c_double), ("turbulent_viscosity", C.c_void_p),
         ("neighbor_inverse", C.c_void_p),
         ("body_force", C.c_void_p * 3), ("forces", C.c_double * 3), ("force_factor", C.c_double),
+        ("tile_nodes", C.c_uint32), ("xcd_run", C.c_uint32),
     ]
 
 
@@ -87,6 +88,8 @@ _SIGNATURES = {
     "vfx_collide_stream": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_int32, C.c_void_p],
     "vfx_k17_step": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_void_p],
     "vfx_bgk_step": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p],
+    "vfx_k17_autotune": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.POINTER(C.c_uint32),
+                         C.POINTER(C.c_uint32), C.POINTER(C.c_float)],
     "vfx_k17_step_fused": [C.POINTER(VfxLattice), C.c_int32, C.POINTER(VfxFusedBC), C.c_int32,
                            C.POINTER(VfxFusedBC), C.c_void_p, C.c_int32, C.c_void_p],
     "vfx_bc_apply": [C.POINTER(VfxLattice), C.POINTER(VfxBC), C.c_int32, C.c_void_p],

@@ -70,11 +70,16 @@ vfx_status make_lat(const vfx_lattice* in, vfx::Lat& L)
         if (nx * ny * nz != in->n_nodes)
             return fail(VFX_EINVAL, "dims %llu*%llu*%llu != n_nodes %llu", (unsigned long long)nx,
                         (unsigned long long)ny, (unsigned long long)nz, (unsigned long long)in->n_nodes);
-        if (ny * nz >= (1ull << 31) / ((nx + 255) / 256)) return fail(VFX_EUNSUPPORTED, "grid too large");
+        const uint64_t tile = in->tile_nodes ? in->tile_nodes : 256;
+        if (tile != 64 && tile != 128 && tile != 256) return fail(VFX_EINVAL, "tile_nodes must be 0, 64, 128 or 256");
+        if (in->xcd_run > 4096 || (in->xcd_run & (in->xcd_run - 1)))
+            return fail(VFX_EINVAL, "xcd_run must be 0 or a power of two up to 4096");
+        if (ny * nz >= (1ull << 31) / ((nx + tile - 1) / tile)) return fail(VFX_EUNSUPPORTED, "grid too large");
         L.nx = (uint32_t)nx;
         L.ny = (uint32_t)ny;
         L.nz = (uint32_t)nz;
-        L.chunks = (uint32_t)((nx + 255) / 256);
+        L.chunks = (uint32_t)((nx + tile - 1) / tile);
+        L.xrun_lg = in->xcd_run ? 1u + (uint32_t)__builtin_ctz(in->xcd_run) : 0u;
         L.solid = in->solid_mask;
     } else {
         if (!in->neighbor_x || !in->neighbor_y || !in->neighbor_z)
@@ -236,6 +241,56 @@ vfx_status vfx_k17_step(const vfx_lattice* lat, int32_t is_even, int32_t tmpl, v
 vfx_status vfx_bgk_step(const vfx_lattice* lat, int32_t is_even, vfx_stream stream)
 {
     return vfx_collide_stream(lat, VFX_BGK, is_even, VFX_TEMPLATE_DEFAULT, stream);
+}
+
+vfx_status vfx_k17_autotune(const vfx_lattice* lat, int32_t is_even, int32_t tmpl, int32_t steps, vfx_stream stream,
+                            uint32_t* tile_out, uint32_t* run_out, float* ms_out)
+{
+    if (!lat || !tile_out || !run_out) return fail(VFX_EINVAL, "NULL argument");
+    if (!lat->direct) return fail(VFX_EUNSUPPORTED, "the autotuner tunes the structured (direct) step");
+    if (steps < 2) steps = 2;
+    steps += steps & 1;
+    const bool even0 = is_even != 0;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return fail(VFX_EHIP, "event create");
+    }
+    vfx_lattice trial = *lat;
+    float best = 1e30f;
+    uint32_t bt = 256, br = 0;
+    vfx_status r = VFX_OK;
+    const uint32_t tiles[3] = {64, 128, 256};
+    const uint32_t runs[7] = {1, 4, 8, 16, 32, 64, 256};
+    for (uint32_t tile : tiles) {
+        for (uint32_t run : runs) {
+            trial.tile_nodes = tile;
+            trial.xcd_run = run;
+            for (int s = 0; s < 2 && r == VFX_OK; ++s) r = vfx_k17_step(&trial, (s % 2 == 0) == even0, tmpl, stream);
+            if (r == VFX_OK) r = hip_status(hipEventRecord(e0, st), "event record");
+            for (int s = 0; s < steps && r == VFX_OK; ++s) r = vfx_k17_step(&trial, (s % 2 == 0) == even0, tmpl, stream);
+            if (r == VFX_OK) r = hip_status(hipEventRecord(e1, st), "event record");
+            if (r == VFX_OK) r = hip_status(hipEventSynchronize(e1), "event synchronize");
+            float ms = 0;
+            if (r == VFX_OK) r = hip_status(hipEventElapsedTime(&ms, e0, e1), "event elapsed");
+            if (r != VFX_OK) break;
+            ms /= (float)steps;
+            if (ms < best) {
+                best = ms;
+                bt = tile;
+                br = run;
+            }
+        }
+        if (r != VFX_OK) break;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (r != VFX_OK) return r;
+    *tile_out = bt;
+    *run_out = br;
+    if (ms_out) *ms_out = best;
+    return VFX_OK;
 }
 
 // argument checks of a boundary table and its device view

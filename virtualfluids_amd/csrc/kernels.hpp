@@ -123,6 +123,7 @@ struct Lat {
     void* bf[3];           // per-node body force (T), ApplyBodyForce
     double gforce[3];      // global force, unscaled
     double force_factor;
+    uint32_t xrun_lg;      // 1 + log2 of the XCD tile run of the direct step (0: xcd_run<S>())
 };
 
 template <typename S> __device__ __forceinline__ S* plane(const Lat& L, int p) { return (S*)L.f + (uint64_t)p * L.N; }
@@ -344,6 +345,12 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t G, uint32_t C
     return b >= full ? b : (b / span) * span + (b % kXcds) * C + (b / kXcds) % C;
 }
 template <typename S> constexpr uint32_t xcd_run() { return sizeof(S) == 8 ? 256u : 16u; }
+// the same for a run of 2^lg tiles (shifts only)
+__device__ __forceinline__ uint32_t xcd_block_pow2(uint32_t b, uint32_t G, uint32_t lg)
+{
+    const uint32_t sh = lg + 3u, full = (G >> sh) << sh;
+    return b >= full ? b : ((b >> sh) << sh) + ((b & (kXcds - 1u)) << lg) + ((b >> 3) & ((1u << lg) - 1u));
+}
 
 // Direct (structured) addressing: one block covers `blockDim.x` consecutive
 // nodes of one x row, so every row index below is wave-uniform (scalar ALU).
@@ -353,7 +360,7 @@ __device__ __forceinline__ void step_direct_body(const Lat& L, const Rates<typen
                                                  uint32_t blocks)
 {
     using T = typename Compute<S>::T;
-    const uint32_t blk = xcd_block(block, blocks, xcd_run<S>());
+    const uint32_t blk = L.xrun_lg ? xcd_block_pow2(block, blocks, L.xrun_lg - 1u) : xcd_block(block, blocks, xcd_run<S>());
     const uint32_t row = blk / L.chunks;
     const uint32_t x = (blk - row * L.chunks) * blockDim.x + threadIdx.x;
     if (x >= L.nx) return;

@@ -69,6 +69,9 @@ class Lattice:
         if variant not in ("default", "plain"):
             raise ValueError("variant must be 'default' or 'plain'")
         self.variant = variant
+        # launch shape of the structured step (0: built-in; see autotune())
+        self.tile_nodes = 0
+        self.xcd_run = 0
         self.device = torch.device(device)
         self.omega = float(omega)
         self.limiter = tuple(float(x) for x in limiter)
@@ -191,7 +194,21 @@ class Lattice:
             s.sgs_constant = self.sgs_constant
             s.turbulent_viscosity = self.nut.data_ptr()
             s.neighbor_inverse = _ptr(self.nbinv)
+        s.tile_nodes = self.tile_nodes
+        s.xcd_run = self.xcd_run
         return s
+
+    def autotune(self, even=True, steps=4, collision_template=0, stream=None):
+        """Times the structured K17 step over every tile width and XCD run
+        (vfx_k17_autotune) and keeps the fastest in this lattice; the
+        populations advance, the parity is unchanged.  Returns (tile_nodes,
+        xcd_run, ms per step)."""
+        s = self.struct()
+        t, r, ms = C.c_uint32(), C.c_uint32(), C.c_float()
+        V.check(V.lib().vfx_k17_autotune(C.byref(s), int(bool(even)), int(collision_template), int(steps),
+                                         _stream(stream), C.byref(t), C.byref(r), C.byref(ms)), "vfx_k17_autotune")
+        self.tile_nodes, self.xcd_run = t.value, r.value
+        return t.value, r.value, ms.value
 
     @property
     def bytes_per_population(self) -> int:
