@@ -465,10 +465,11 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
     spanwise), K17 at Re = 10^4 on the cube height.  Every coarse step (two
     fine steps, one coarse step, both levels' boundary kernels, fine-to-coarse
     and coarse-to-fine) is recorded into a HIP graph of `sample_every` coarse
-    steps, in the merged schedule (the coarse step and the first fine step in
-    one launch, each phase's boundary groups of both levels in one launch,
-    fine-to-coarse and coarse-to-fine in one launch; the launch-per-kernel
-    schedule is timed beside it over 500 coarse steps).  Time-averaged
+    steps.  Both schedules (launch per kernel, and merged: the coarse step and
+    the first fine step in one launch, each phase's boundary groups of both
+    levels in one launch, fine-to-coarse and coarse-to-fine in one launch) are
+    timed over 500 coarse steps with the levels' launch shapes autotuned; the
+    faster one runs the long window.  Time-averaged
     velocity over the second half on both levels; the
     FP32 and FP16 forms (FP16: deviations from the u_ref equilibrium) against
     FP64.  At Re = 10^4 the wake is chaotic: two forms that differ by rounding
@@ -505,16 +506,26 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
                 lat.init_equilibrium(np.zeros(n), profile(z), np.zeros(n), np.zeros(n))
             return st
 
-        # the launch-per-kernel schedule, timed over a short window for comparison
-        st = stepper(False)
-        graph = st.capture(sample_every)
+        # both schedules with their levels' launch shapes tuned, timed over a
+        # short window without statistics; the faster one runs the long window
+        def timed(merged):
+            st = stepper(merged)
+            tuned = st.autotune()
+            graph = st.capture(sample_every)
+            st.replay(graph, sample_every)
+            ms = _timed_loop(torch, stream, n_sep, lambda i: st.replay(graph, sample_every))
+            launches = st.launches_per_step
+            del st, graph
+            torch.cuda.empty_cache()
+            return ms, launches, tuned
+
         n_sep = 10
-        st.replay(graph, sample_every)
-        ms_sep = _timed_loop(torch, stream, n_sep, lambda i: st.replay(graph, sample_every))
-        launches_sep = st.launches_per_step
-        del st, graph
-        torch.cuda.empty_cache()
-        st = stepper(True)
+        ms_sep, launches_sep, _ = timed(False)
+        ms_merged, launches_merged, tuned = timed(True)
+        use_merged = ms_merged <= ms_sep
+        st = stepper(use_merged)
+        for lat, name in ((st.coarse, "coarse"), (st.fine, "fine")):
+            lat.tile_nodes, lat.xcd_run = tuned[name][0], tuned[name][1]
         graph = st.capture(sample_every)
         sc, sf = vfx.Statistics(st.coarse), vfx.Statistics(st.fine)
         sce, sfe = vfx.Statistics(st.coarse), vfx.Statistics(st.fine)
@@ -537,9 +548,14 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
         means[storage] = (sc.mean()[:, act], sf.mean()[:, ffl])
         early_means[storage] = (sce.mean()[:, act], sfe.mean()[:, ffl])
         r = {"mlups": round(lups / (ms * 1e-3) / 1e6, 1), "ms_per_coarse_step": round(ms / (chunks * sample_every), 4),
-             "launches_per_coarse_step": st.launches_per_step,
-             "separate_launches": {"ms_per_coarse_step": round(ms_sep / (n_sep * sample_every), 4),
-                                   "launches_per_coarse_step": launches_sep},
+             "schedule": "merged" if use_merged else "separate", "launches_per_coarse_step": st.launches_per_step,
+             "launch": {k: {"tile_nodes": v[0], "xcd_run": v[1]} for k, v in tuned.items()},
+             "merged_launches_without_statistics": {"ms_per_coarse_step": round(ms_merged / (n_sep * sample_every), 4),
+                                                    "launches_per_coarse_step": launches_merged,
+                                                    "coarse_steps": n_sep * sample_every},
+             "separate_launches_without_statistics": {"ms_per_coarse_step": round(ms_sep / (n_sep * sample_every), 4),
+                                                      "launches_per_coarse_step": launches_sep,
+                                                      "coarse_steps": n_sep * sample_every},
              "finite": bool(torch.isfinite(means[storage][1]).all())}
         out[storage] = r
         print(f"# cube {storage}: {r['mlups']:.0f} MLUPS (refined, fine counted twice)", file=sys.stderr, flush=True)

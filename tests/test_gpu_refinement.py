@@ -298,3 +298,34 @@ def test_capture_before_any_eager_step(cuda, merged):
     a.step(2)
     torch.cuda.synchronize()
     assert torch.equal(a.coarse.f, b.coarse.f) and torch.equal(a.fine.f, b.fine.f)
+
+
+@pytest.mark.parametrize("merged", [False, True])
+def test_autotuned_levels_give_the_same_bytes(cuda, merged):
+    """RefinedStepper.autotune picks each level's launch shape and restores the
+    populations: stepping on gives the bytes of the untuned stepper."""
+    print(f"[refined autotune merged={merged}]", flush=True)
+    _, a, _, _ = _cube_stepper("fp16", merged)
+    _, b, _, _ = _cube_stepper("fp16", merged)
+    tuned = b.autotune(steps=2)
+    print(f"  {tuned}", flush=True)
+    assert set(tuned) == {"coarse", "fine"}
+    a.step(3)
+    b.step(3)
+    torch.cuda.synchronize()
+    assert torch.equal(a.coarse.f, b.coarse.f) and torch.equal(a.fine.f, b.fine.f)
+
+
+@pytest.mark.parametrize("tiles", [(64, 64), (128, 128), (64, 256)])
+def test_two_level_launch_in_the_levels_tile_width(cuda, tiles):
+    """The two-level launch runs in the levels' tile width where they agree
+    (else 256 nodes): same bytes as the built-in shape."""
+    print(f"[two-level launch tiles {tiles}]", flush=True)
+    _, a, _, _ = _cube_stepper("fp32", True)
+    _, b, _, _ = _cube_stepper("fp32", True)
+    b.coarse.tile_nodes, b.fine.tile_nodes = tiles
+    b.coarse.xcd_run, b.fine.xcd_run = 2, 8
+    a.step(3)
+    b.step(3)
+    torch.cuda.synchronize()
+    assert torch.equal(a.coarse.f, b.coarse.f) and torch.equal(a.fine.f, b.fine.f)

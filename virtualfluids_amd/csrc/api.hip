@@ -79,6 +79,7 @@ vfx_status make_lat(const vfx_lattice* in, vfx::Lat& L)
         L.ny = (uint32_t)ny;
         L.nz = (uint32_t)nz;
         L.chunks = (uint32_t)((nx + tile - 1) / tile);
+        L.tile = (uint32_t)tile;
         L.xrun_lg = in->xcd_run ? 1u + (uint32_t)__builtin_ctz(in->xcd_run) : 0u;
         L.solid = in->solid_mask;
     } else {

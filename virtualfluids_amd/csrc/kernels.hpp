@@ -124,6 +124,7 @@ struct Lat {
     double gforce[3];      // global force, unscaled
     double force_factor;
     uint32_t xrun_lg;      // 1 + log2 of the XCD tile run of the direct step (0: xcd_run<S>())
+    uint32_t tile;         // nodes per workgroup along x of the direct step
 };
 
 template <typename S> __device__ __forceinline__ S* plane(const Lat& L, int p) { return (S*)L.f + (uint64_t)p * L.N; }
@@ -2368,6 +2369,13 @@ struct LevelInput {
 template <typename S> hipError_t launch_step_levels(const LevelInput* in, int n, hipStream_t st)
 {
     using T = typename Compute<S>::T;
+    constexpr bool FAST = sizeof(S) != 8;
+    if (n == 1) {
+        // one level: the plain step itself, in the lattice's own launch shape
+        const Rates<T> R = make_rates<T>(in[0].omega, in[0].lim, in[0].force, 0.0);
+        return in[0].even ? launch_step_t<S, VFX_K17, true, FAST>(in[0].L, 1, R, st)
+                          : launch_step_t<S, VFX_K17, false, FAST>(in[0].L, 1, R, st);
+    }
     static_assert(sizeof(LevelPair<S>) <= 4000, "kernel arguments");
     LevelPair<S> P;
     std::memset(&P, 0, sizeof(P));
@@ -2375,10 +2383,12 @@ template <typename S> hipError_t launch_step_levels(const LevelInput* in, int n,
     Lat* L[2] = {&P.L0, &P.L1};
     Rates<T>* R[2] = {&P.R0, &P.R1};
     uint32_t* blocks[2] = {&P.blocks0, &P.blocks1};
+    // one workgroup width for both levels: their own where they agree, else 256
+    const uint32_t tile = in[0].L.tile == in[1].L.tile && in[0].L.tile ? in[0].L.tile : 256u;
     uint32_t first = 0;
     for (int j = 0; j < n; ++j) {
         *L[j] = in[j].L;
-        L[j]->chunks = (L[j]->nx + 255u) / 256u; // blocks of 256 nodes in this launch
+        L[j]->chunks = (L[j]->nx + tile - 1u) / tile; // blocks of `tile` nodes in this launch
         *R[j] = make_rates<T>(in[j].omega, in[j].lim, in[j].force, 0.0);
         *blocks[j] = L[j]->chunks * L[j]->ny * L[j]->nz;
         if (j == 1) P.first1 = first;
@@ -2386,7 +2396,7 @@ template <typename S> hipError_t launch_step_levels(const LevelInput* in, int n,
     }
     if (first == 0) return hipSuccess;
     const bool e0 = in[0].even, e1 = n > 1 ? in[1].even : e0;
-    const dim3 g(first), b(256);
+    const dim3 g(first), b(tile);
     switch ((e0 ? 2 : 0) + (e1 ? 1 : 0)) {
     case 3: hipLaunchKernelGGL((k_step_levels<S, true, true>), g, b, 0, st, P); break;
     case 2: hipLaunchKernelGGL((k_step_levels<S, true, false>), g, b, 0, st, P); break;

@@ -424,6 +424,20 @@ class RefinedStepper:
         if n % 2:
             self.even_c = not self.even_c
 
+    def autotune(self, steps=4):
+        """Launch shape of each level's structured step (Lattice.autotune),
+        with the populations restored afterwards.  The two-level launch of the
+        merged schedule keeps 256-node tiles; one-level launches and the
+        separate schedule use the tuned shapes.  Returns {level: (tile_nodes,
+        xcd_run, ms)}."""
+        out = {}
+        for name, lat, even in (("coarse", self.coarse, self.even_c), ("fine", self.fine, self.even_f)):
+            saved = lat.f.clone()
+            out[name] = lat.autotune(even, steps=steps)
+            lat.f.copy_(saved)
+            del saved
+        return out
+
     @property
     def launches_per_step(self) -> int:
         """Kernel launches of one coarse step in the current schedule."""
