@@ -7,8 +7,13 @@
 //       times { int32 length, name bytes, float64 values[n] }
 //   ref_vtk pvtu <prefix> <names,comma,separated> <piece>...   writeParallelFile
 //   ref_vtk pvd <prefix> <step>=<file>[,<file>...] ...         writeCollectionForTimeSeries
-// Prints the path the writer returns.
+//   ref_vtk vti <input> <file>   WbWriterVtkXmlImageBinary::writeData (the
+//       precursor planes of PrecursorWriter::write); input: int32 k, k times
+//       { int32 length, name bytes }, int32 n, float64 values[k][n],
+//       int32 extent[6], float32 origin[3], float32 spacing[3], int32 precision
+// Prints the path the writer returns (vti: the file name).
 #include <basics/writer/WbWriterVtkXmlBinary.h>
+#include <basics/writer/WbWriterVtkXmlImageBinary.h>
 
 #include <cstdint>
 #include <cstdio>
@@ -72,12 +77,46 @@ static int octs(const char* input, const std::string& prefix)
     return 0;
 }
 
+static int vti(const char* input, const std::string& file)
+{
+    std::ifstream in(input, std::ios::binary);
+    if (!in) return 2;
+    const int32_t k = get<int32_t>(in);
+    std::vector<std::string> names;
+    for (int32_t j = 0; j < k; ++j) {
+        const int32_t len = get<int32_t>(in);
+        std::string name(len, '\0');
+        in.read(&name[0], len);
+        names.push_back(name);
+    }
+    const int32_t n = get<int32_t>(in);
+    std::vector<std::vector<double>> data(k, std::vector<double>(n));
+    for (auto& d : data)
+        for (auto& x : d) x = get<double>(in);
+    int32_t e[6];
+    for (auto& x : e) x = get<int32_t>(in);
+    float o[3], sp[3];
+    for (auto& x : o) x = get<float>(in);
+    for (auto& x : sp) x = get<float>(in);
+    const int32_t precision = get<int32_t>(in);
+    UbTupleInt6 extent = makeUbTuple(e[0], e[1], e[2], e[3], e[4], e[5]);
+    UbTupleFloat3 origin = makeUbTuple(o[0], o[1], o[2]);
+    UbTupleFloat3 spacing = makeUbTuple(sp[0], sp[1], sp[2]);
+    std::vector<std::string> cellNames;
+    std::vector<std::vector<double>> cellData;
+    WbWriterVtkXmlImageBinary::getInstance()->writeData(file, names, cellNames, data, cellData, extent, origin, spacing,
+                                                        extent, (unsigned)precision);
+    std::printf("%s\n", file.c_str());
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     if (argc < 3) return 2;
     const std::string mode = argv[1];
     try {
         if (mode == "octs" && argc == 4) return octs(argv[2], argv[3]);
+        if (mode == "vti" && argc == 4) return vti(argv[2], argv[3]);
         if (mode == "pvtu" && argc >= 4) {
             std::vector<std::string> names = split(argv[3], ','), cells, pieces;
             for (int i = 4; i < argc; ++i) pieces.push_back(argv[i]);

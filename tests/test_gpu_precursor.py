@@ -65,7 +65,7 @@ def _oracle_bcs(c, tr, planes=None):
             oracle.BCGroup(c["inlet"], c["iq"] if c["kind"] == 12 else None, c["ux"], kind=c["kind"], precursor=pre)]
 
 
-def _gpu_run(c, storage, steps, tr=0.35, index=None, layout=None, reader=None, T=3):
+def _gpu_run(c, storage, steps, tr=0.35, index=None, layout=None, reader=None, T=3, n_pre=N_PRE):
     real = torch.float64 if storage == "fp64" else torch.float32
     m = (lambda a: a) if index is None else (lambda a: index[a])
     comps = 3 if c["kind"] == 12 else 9
@@ -78,7 +78,7 @@ def _gpu_run(c, storage, steps, tr=0.35, index=None, layout=None, reader=None, T
         Fs = np.zeros((index.size + 1, 27))
         Fs[index] = c["F0"]
         lat.write_populations(Fs)
-    pc = vfx.PrecursorPlanes(c["nb"], c["w"], N_PRE, comps, reader=reader, time_steps_between_reads=T,
+    pc = vfx.PrecursorPlanes(c["nb"], c["w"], n_pre, comps, reader=reader, time_steps_between_reads=T,
                              velocity_ratio=1.25, real=real)
     if reader is None:
         pc.last, pc.current = (pc.plane(p) for p in c["planes"])
@@ -164,3 +164,45 @@ def test_precursor_reduced_precision_close_to_fp64(cuda, kind, storage, bar):
     print(f"[precursor {storage} kind {kind}] velocity L2 vs FP64 {l2:.2e}", flush=True)
     assert np.isfinite(out[storage]).all()
     assert l2 < bar
+
+
+@pytest.mark.parametrize("kind", [12, 13])
+def test_precursor_planes_from_files(cuda, kind, tmp_path):
+    """A precursor series in VirtualFluids' file format (PrecursorWriter's
+    .bin.vti planes, one point per inlet node, records every T steps) drives
+    the inflow through precursor_from_files: the run equals the same stencil
+    fed with the same planes from memory, bit for bit."""
+    from virtualfluids_amd import precursor_files as pf
+
+    print(f"[precursor from files kind {kind}]", flush=True)
+    c = _setup(kind, seed=11)
+    nx, ny, nz = DIMS
+    y = (c["inlet"] // nx) % ny
+    z = c["inlet"] // (nx * ny)
+    comps = 3 if kind == 12 else 9
+    T, steps, rec = 3, 10, 3
+    py, pz = ny - 2, nz - 2  # plane points at y = 1 .. ny-2, z = 1 .. nz-2
+    gy, gz = np.meshgrid(np.arange(1, ny - 1), np.arange(1, nz - 1), indexing="xy")
+    base = c["planes"][0][:, :1]
+
+    def values(t):
+        return base + 1e-4 * np.sin(0.3 * t + 0.5 * gy.ravel() + gz.ravel())[None, :]
+
+    names = ["vx", "vy", "vz"] if kind == 12 else ["f%d" % j for j in range(9)]
+    prefix = str(tmp_path / "inflow")
+    for part in range(3):
+        data = np.concatenate([values(T * (rec * part + r)) for r in range(rec)], axis=1)
+        pf.write_vti(pf.file_name(prefix, 0, 0, part), names, data, (0, py - 1, 0, pz - 1, rec * part,
+                     rec * part + rec - 1), (1.0, 1.0, 0.0), (1.0, 1.0, float(T)))
+    nb, w, n, q, reader = pf.precursor_from_files(prefix, 0, y.astype(np.float64), z.astype(np.float64),
+                                                  real=np.float64)
+    assert q == comps and n == c["inlet"].size and np.all(w[0] == 1e6)
+    c["nb"], c["w"] = nb, w
+    _, a = _gpu_run(c, "fp64", steps, reader=reader, T=T, n_pre=n)
+    order = (y - 1) + py * (z - 1)  # the plane point of each inlet node, in slot order
+
+    def memory(t):
+        return values(t)[:, order]
+
+    _, b = _gpu_run(c, "fp64", steps, reader=memory, T=T, n_pre=n)
+    assert np.array_equal(a.populations().cpu().numpy(), b.populations().cpu().numpy())

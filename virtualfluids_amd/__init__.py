@@ -10,7 +10,9 @@ from .batch import Ensemble, Graph, capture, cavity_ensemble  # noqa: F401
 from .refinement import ICells, NestedBox, RefinedStepper, fine_omega, scale_cf, scale_fc, scale_fc_cf  # noqa: F401
 from .lattice import (BYTES, CX, CY, CZ, INV, W, BoundaryCondition, Lattice, PrecursorPlanes, Statistics,  # noqa: F401
                       Stepper, FusedGroup, FusedPlan, MeanValues, BCSet, step_levels, pack_mask)
+from .precursor_files import VTKFile, VTKFileCollection, VTKReader, precursor_from_files  # noqa: F401
 
 __all__ = ["Ensemble", "Graph", "capture", "cavity_ensemble", "ICells", "NestedBox", "RefinedStepper", "fine_omega", "scale_cf",
            "scale_fc", "scale_fc_cf", "Lattice", "BoundaryCondition", "PrecursorPlanes", "Stepper", "FusedGroup", "FusedPlan", "MeanValues", "BCSet", "step_levels", "Statistics", "VfxError", "lib", "check", "pack_mask",
+           "VTKFile", "VTKFileCollection", "VTKReader", "precursor_from_files",
            "CX", "CY", "CZ", "INV", "W", "BYTES"]
