@@ -436,10 +436,12 @@ def channel_leg(torch, vfx, steps, sample_every=10, forms=("fp64", "fp32", "fp16
              "fused_entries": {str(g.bc.kind): g.bc.n for g in plan.post + ([plan.pre] if plan.pre else [])},
              "separate_kernels": len(plan.residual_post) + len(plan.residual_pre)}
         if storage == "fp16" and separate_steps > 0:
-            # the same channel with one kernel per boundary group, and with
-            # every group inside the collide-stream kernel
+            # the leg's own policy without the statistics, the same channel
+            # with one kernel per boundary group, and with every group inside
+            # the collide-stream kernel
             n_sep = separate_steps // 10 * 10
-            for name, kw in (("separate_bc_kernels", {}), ("all_groups_fused", {"fuse": True})):
+            for name, kw in (("pressure_fused_without_statistics", {"fuse": True, "fuse_post": False}),
+                             ("separate_bc_kernels", {}), ("all_groups_fused", {"fuse": True})):
                 su = vfx.Stepper(lat, "k17", bcs, **kw)
                 su.step(20)
                 ms_u = _timed_loop(torch, stream, n_sep // 10, lambda i: su.step(10))
@@ -678,6 +680,10 @@ def main():
             # the channel has the periodic box's dims: the box's step time over the channel's
             c = out["channel"]["fp16"]
             c["bulk_over_channel_step_time"] = round(res["fp16"]["ms_per_step"] / c["ms_per_step"], 4)
+            if "pressure_fused_without_statistics" in c:
+                # the step alone (the leg's own timing includes a statistics pass every 10 steps)
+                c["bulk_over_channel_step_time_without_statistics"] = round(
+                    res["fp16"]["ms_per_step"] / c["pressure_fused_without_statistics"]["ms_per_step"], 4)
     if args.cube_steps > 0:
         out["refined_cube"] = cube_leg(torch, vfx, args.cube_steps)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)

@@ -194,8 +194,10 @@ class Lattice:
             s.sgs_constant = self.sgs_constant
             s.turbulent_viscosity = self.nut.data_ptr()
             s.neighbor_inverse = _ptr(self.nbinv)
-        s.tile_nodes = self.tile_nodes
-        s.xcd_run = self.xcd_run
+        # lattices made without __init__ (domain views of an ensemble) have
+        # the built-in launch shape
+        s.tile_nodes = getattr(self, "tile_nodes", 0)
+        s.xcd_run = getattr(self, "xcd_run", 0)
         return s
 
     def autotune(self, even=True, steps=4, collision_template=0, stream=None):
