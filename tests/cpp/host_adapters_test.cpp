@@ -9,6 +9,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -636,8 +637,60 @@ static int vtu_mean_mode(const std::string& input, const std::string& prefix)
     return 0;
 }
 
+// precursor <prefix> <n> y_0 .. y_n-1 z_0 .. z_n-1 t_0 ..: VTKReader over the
+// files of <prefix> (level 0): the stencil lists, then the plane at each time
+static int precursor_mode(int argc, char** argv)
+{
+    const std::string prefix = argv[2];
+    const int n = std::atoi(argv[3]);
+    if (argc < 4 + 2 * n) return 2;
+    std::vector<real> y(n), z(n);
+    for (int i = 0; i < n; i++) {
+        y[i] = (real)std::strtod(argv[4 + i], nullptr);
+        z[i] = (real)std::strtod(argv[4 + n + i], nullptr);
+    }
+    auto reader = createReaderForCollection(createFileCollection(prefix, TransientBCFileType::VTK), 0);
+    reader->fillArrays(y, z);
+    const uint m = reader->getNPointsRead();
+    const size_t q = reader->getNumberOfQuantities();
+    std::printf("points %u quantities %zu\n", m, q);
+    std::vector<uint> nb[4];
+    std::vector<real> w[4];
+    for (int s = 0; s < 4; s++) {
+        nb[s].resize(n);
+        w[s].resize(n);
+    }
+    reader->getNeighbors(nb[0].data(), nb[1].data(), nb[2].data(), nb[3].data());
+    reader->getWeights(w[0].data(), w[1].data(), w[2].data(), w[3].data());
+    for (int s = 0; s < 4; s++) {
+        std::printf("neighbors");
+        for (int i = 0; i < n; i++) std::printf(" %u", nb[s][i]);
+        std::printf("\nweights");
+        for (int i = 0; i < n; i++) std::printf(" %.17g", (double)w[s][i]);
+        std::printf("\n");
+    }
+    QforPrecursorBoundaryConditions pbc;
+    setPrecursorFileReader(pbc, reader);
+    std::vector<real> plane(m * q);
+    for (int a = 4 + 2 * n; a < argc; a++) {
+        pbc.getNextData(plane.data(), pbc.numberOfPrecursorNodes, std::strtod(argv[a], nullptr));
+        std::printf("plane");
+        for (real v : plane) std::printf(" %.17g", (double)v);
+        std::printf("\n");
+    }
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
+    if (argc > 3 && std::strcmp(argv[1], "precursor") == 0) {
+        try {
+            return precursor_mode(argc, argv);
+        } catch (const std::exception& e) {
+            std::printf("exception: %s\n", e.what());
+            return 3;
+        }
+    }
     if (argc > 3 && std::strcmp(argv[1], "vtu") == 0) return vtu_mode(argv[2], argv[3]);
     if (argc > 3 && std::strcmp(argv[1], "vtumean") == 0) return vtu_mean_mode(argv[2], argv[3]);
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;

@@ -9,7 +9,7 @@
   inverse squared distances (closed form), each point in one plane slot;
 * the planes come from the record nearest in time, from the second file once
   the first file's time range is left, and reading past the last file fails;
-* a NaN among the first values of the first file is refused.
+* a NaN in the first record at a point a stencil reads is refused.
 
 VirtualFluids' reader itself (TransientBCSetter.cpp) needs spdlog, which the
 reference tree does not hold, so it is not compiled here; these checks and
@@ -116,13 +116,13 @@ def test_off_grid_stencil_is_the_four_surrounding_points(tmp_path):
     assert np.array_equal(data.reshape(3, -1), rec[:, read].astype(np.float32))
 
 
-def test_nan_in_the_first_values_is_refused(tmp_path):
+def test_nan_at_a_read_point_is_refused(tmp_path):
     case = precursor_case()
-    case["parts"][0]["data"][0, 1] = np.nan
+    case["parts"][0]["data"][0, 1] = np.nan  # first record, plane point 1
     prefix = _write(tmp_path, case)
     with pytest.raises(RuntimeError):
         pf.precursor_from_files(prefix, 0, case["y"][:4], case["z"][:4])
-    # a NaN beyond the entries the check reads is not seen by it (as in the reference)
+    # a NaN at a point no stencil reads is allowed
     case = precursor_case()
     case["parts"][0]["data"][0, 25] = np.nan
     prefix = _write(tmp_path, case)
@@ -145,3 +145,34 @@ def test_shared_stencil_points_get_one_slot(tmp_path):
     read = rd.read_idx[0][0]
     # corners: y 0-3 at z 0 and 1 (first row of cells), y 0-2 at z 2 (second row)
     assert len(read) == len(set(read)) == rd.n_points_read == 11
+
+
+def test_host_precursor_reader_equals_python(tmp_path):
+    """The C++ host layer's VTKReader (double `real`) on the same files gives
+    the Python reader's stencils, weights and planes, on and off the grid."""
+    import subprocess
+    from tests.test_host_abi import build_host_test
+
+    exe = build_host_test()
+    case = precursor_case()
+    prefix = _write(tmp_path)
+    rng = np.random.default_rng(8)
+    y = np.concatenate([case["y"][[2, 9, 17]], Y0 + H * rng.uniform(0.05, NY - 1.05, 4)])
+    z = np.concatenate([case["z"][[2, 9, 17]], Z0 + H * rng.uniform(0.05, NZ - 1.05, 4)])
+    times = [0.0, 0.2, 0.37, 0.51, 0.8]
+    args = [exe, "precursor", prefix, str(y.size)] + [repr(float(v)) for v in y] + [repr(float(v)) for v in z] + \
+        [repr(t) for t in times]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.split("\n")
+    rd = pf.VTKReader(pf.VTKFileCollection(prefix, np.float64), 0, np.float64)
+    rd.fill_arrays(y, z)
+    nb, w = rd.stencil()
+    assert lines[0] == f"points {rd.n_points_read} quantities 3"
+    for s in range(4):
+        assert [int(v) for v in lines[1 + 2 * s].split()[1:]] == nb[s].tolist(), s
+        assert np.array_equal(np.array([float(v) for v in lines[2 + 2 * s].split()[1:]]), w[s]), s
+    for j, t in enumerate(times):
+        data = np.zeros(3 * rd.n_points_read)
+        rd.get_next_data(data, rd.n_points_read, t)
+        assert np.array_equal(np.array([float(v) for v in lines[9 + j].split()[1:]]), data), t

@@ -1,6 +1,7 @@
 // vfx_host.cpp -- implementation of the C++ host layer (see vfx_host.hpp).
 #include "vfx_host.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <filesystem>
 #include <cstdio>
@@ -1041,6 +1042,310 @@ void freeMean(Parameter* para)
             *s[i] = nullptr;
         }
     }
+}
+
+// ---------------------------------------------------------------- precursor files --
+namespace {
+
+std::string readElement(const std::string& line)
+{
+    const size_t start = line.find('<') + 1;
+    const size_t end = line.find(' ', start);
+    return end == std::string::npos ? line.substr(start) : line.substr(start, end - start);
+}
+
+std::string readAttribute(const std::string& line, const std::string& name)
+{
+    const size_t start = line.find(name) + name.size() + 2; // skip '="'
+    return line.substr(start, line.find('"', start) - start);
+}
+
+template <typename T> std::vector<T> readStringToVector(const std::string& s)
+{
+    // every number read as float, then converted (TransientBCSetter.cpp:71-82)
+    std::vector<T> out;
+    std::stringstream input(s);
+    float num;
+    while (input >> num) out.push_back((T)num);
+    return out;
+}
+
+} // namespace
+
+void VTKFile::readHeader()
+{
+    std::ifstream file(fileName);
+    if (!file) throw std::runtime_error("VTKFile: cannot open " + fileName);
+    std::string line;
+    std::getline(file, line); // VTKFile or the xml declaration
+    if (line.size() > 1 && line[1] == '?') std::getline(file, line);
+    std::getline(file, line); // ImageData
+    const std::vector<float> origin = readStringToVector<float>(readAttribute(line, "Origin"));
+    const std::vector<float> spacing = readStringToVector<float>(readAttribute(line, "Spacing"));
+    std::getline(file, line); // Piece
+    const std::vector<int> extent = readStringToVector<int>(readAttribute(line, "Extent"));
+    std::getline(file, line); // PointData
+    std::getline(file, line);
+    while (readElement(line) == "DataArray") {
+        Quantity q;
+        q.name = readAttribute(line, "Name");
+        q.offset = std::stoi(readAttribute(line, "offset"));
+        quantities.push_back(q);
+        std::getline(file, line);
+    }
+    std::getline(file, line); // </Piece>
+    std::getline(file, line); // </ImageData>
+    std::getline(file, line); // AppendedData
+    const int base = int(file.tellg()) + 1 + 4; // the underscore and the first array's byte count
+    for (auto& q : quantities) q.offset += base;
+    if (origin.size() < 3 || spacing.size() < 3 || extent.size() < 6)
+        throw std::runtime_error("VTKFile: malformed header in " + fileName);
+    deltaX = spacing[0];
+    deltaY = spacing[1];
+    deltaZ = spacing[2];
+    nx = extent[1] - extent[0] + 1;
+    ny = extent[3] - extent[2] + 1;
+    nz = extent[5] - extent[4] + 1;
+    minX = origin[0] + deltaX * extent[0];
+    maxX = (nx - 1) * deltaX + minX;
+    minY = origin[1] + deltaY * extent[2];
+    maxY = (ny - 1) * deltaY + minY;
+    minZ = origin[2] + deltaZ * extent[4];
+    maxZ = (nz - 1) * deltaZ + minZ;
+}
+
+int VTKFile::getClosestIdxZ(real z) const
+{
+    const int i = (int)std::round((z - minZ) / deltaZ);
+    return i > nz ? nz : (i < 0 ? 0 : i);
+}
+
+bool VTKFile::markNANs(const std::vector<uint>& readIndices) const
+{
+    // the first record's values at the read indices (the reference reads into
+    // a reserved, empty vector, so its check finds nothing)
+    std::ifstream buf(fileName, std::ios::in | std::ios::binary);
+    std::vector<double> first(std::max(1, getNumberOfPointsInXYPlane()));
+    buf.seekg(quantities.at(0).offset);
+    buf.read(reinterpret_cast<char*>(first.data()), (std::streamsize)(first.size() * sizeof(double)));
+    for (uint i : readIndices)
+        if (i < first.size() && std::isnan(first[i])) return true;
+    return false;
+}
+
+void VTKFile::loadFile()
+{
+    std::ifstream buf(fileName, std::ios::in | std::ios::binary);
+    for (auto& q : quantities) {
+        q.values.resize(getNumberOfPoints());
+        buf.seekg(q.offset);
+        buf.read(reinterpret_cast<char*>(q.values.data()), (std::streamsize)(getNumberOfPoints() * sizeof(double)));
+    }
+    if (!buf) throw std::runtime_error("VTKFile: short file " + fileName);
+    loaded = true;
+}
+
+void VTKFile::unloadFile()
+{
+    for (auto& q : quantities) std::vector<double>().swap(q.values);
+    loaded = false;
+}
+
+void VTKFile::getData(real* data, uint numberOfNodes, const std::vector<uint>& readIndices,
+                      const std::vector<uint>& writeIndices, uint offsetRead, uint offsetWrite)
+{
+    if (!loaded) loadFile();
+    for (size_t j = 0; j < quantities.size(); j++) {
+        real* quant = &data[j * numberOfNodes];
+        for (size_t i = 0; i < writeIndices.size(); i++)
+            quant[offsetWrite + writeIndices[i]] = (real)quantities[j].values.at(readIndices[i] + offsetRead);
+    }
+}
+
+std::string VTKFileCollection::makeFileName(int level, int id, int part) const
+{
+    return prefix + "_lev_" + std::to_string(level) + "_ID_" + std::to_string(id) + "_File_" + std::to_string(part) +
+           ".bin." + suffix;
+}
+
+void VTKFileCollection::findFiles()
+{
+    while (true) {
+        std::vector<std::vector<VTKFile>> level;
+        while (true) {
+            std::vector<VTKFile> parts;
+            while (true) {
+                const std::string name = makeFileName((int)files.size(), (int)level.size(), (int)parts.size());
+                if (!std::ifstream(name).good()) break;
+                parts.emplace_back(name);
+            }
+            if (parts.empty()) break;
+            level.push_back(std::move(parts));
+        }
+        if (level.empty()) break;
+        files.push_back(std::move(level));
+    }
+    if (files.empty()) throw std::runtime_error("VTKFileCollection found no files!");
+}
+
+void TransientBCInputFileReader::getNeighbors(uint* n0PP, uint* n0PM, uint* n0MP, uint* n0MM)
+{
+    std::copy(planeNeighbor0PP.begin(), planeNeighbor0PP.end(), &n0PP[writingOffset]);
+    std::copy(planeNeighbor0PM.begin(), planeNeighbor0PM.end(), &n0PM[writingOffset]);
+    std::copy(planeNeighbor0MP.begin(), planeNeighbor0MP.end(), &n0MP[writingOffset]);
+    std::copy(planeNeighbor0MM.begin(), planeNeighbor0MM.end(), &n0MM[writingOffset]);
+}
+
+void TransientBCInputFileReader::getWeights(real* w0PP, real* w0PM, real* w0MP, real* w0MM)
+{
+    std::copy(weights0PP.begin(), weights0PP.end(), &w0PP[writingOffset]);
+    std::copy(weights0PM.begin(), weights0PM.end(), &w0PM[writingOffset]);
+    std::copy(weights0MP.begin(), weights0MP.end(), &w0MP[writingOffset]);
+    std::copy(weights0MM.begin(), weights0MM.end(), &w0MM[writingOffset]);
+}
+
+VTKReader::VTKReader(std::shared_ptr<VTKFileCollection> collection, uint level)
+    : fileCollection(std::move(collection)), readLevel(level)
+{
+    nQuantities = fileCollection->getNumberOfQuantities();
+}
+
+void VTKReader::initializeIndexVectors()
+{
+    const size_t nl = fileCollection->files.size();
+    readIndices.assign(nl, {});
+    writeIndices.assign(nl, {});
+    nFile.assign(nl, {});
+    for (size_t l = 0; l < nl; l++) {
+        readIndices[l].resize(fileCollection->files[l].size());
+        writeIndices[l].resize(fileCollection->files[l].size());
+        nFile[l].assign(fileCollection->files[l].size(), 0);
+    }
+}
+
+uint VTKReader::getWriteIndex(int level, int id, int linearIndex)
+{
+    std::vector<uint>& r = readIndices[level][id];
+    const auto it = std::find(r.begin(), r.end(), (uint)linearIndex);
+    if (it != r.end()) return writeIndices[level][id][it - r.begin()];
+    writeIndices[level][id].push_back(nPointsRead);
+    r.push_back((uint)linearIndex);
+    return nPointsRead++;
+}
+
+void VTKReader::fillArrays(std::vector<real>& coordsY, std::vector<real>& coordsZ)
+{
+    nPoints = (uint)coordsY.size();
+    initializeIndexVectors();
+    const real maxDiff = (real)1e-4, eps = (real)1e-7;
+    const uint level = readLevel;
+    if (level >= fileCollection->files.size()) throw std::runtime_error("VTKReader: no files on the read level");
+    auto weight = [&](const VTKFile& f, int idx, real y, real z) {
+        const real dy = f.getX(idx) - y;
+        const real dz = f.getY(idx) - z;
+        return (real)1 / (dy * dy + dz * dz + eps);
+    };
+    for (uint i = 0; i < nPoints; i++) {
+        const real y = coordsY[i], z = coordsZ[i];
+        bool f0PP = false, f0PM = false, f0MP = false, f0MM = false, all = false;
+        for (int id = 0; id < (int)fileCollection->files[level].size(); id++) {
+            VTKFile& f = fileCollection->files[level][id][0];
+            if (!f.inBoundingBox(y, z, (real)0)) continue;
+            const int idx = f.findNeighborMMM(y, z, (real)0);
+            if (idx != -1) {
+                if (std::abs(y - f.getX(idx)) < maxDiff && std::abs(z - f.getY(idx)) < maxDiff) {
+                    weights0PP.push_back((real)1e6f);
+                    weights0PM.push_back(0);
+                    weights0MP.push_back(0);
+                    weights0MM.push_back(0);
+                    const uint w = getWriteIndex(level, id, idx);
+                    planeNeighbor0PP.push_back(w);
+                    planeNeighbor0PM.push_back(w);
+                    planeNeighbor0MP.push_back(w);
+                    planeNeighbor0MM.push_back(w);
+                    f0PP = f0PM = f0MP = f0MM = true;
+                }
+                if (!f0MM) {
+                    f0MM = true;
+                    weights0MM.push_back(weight(f, idx, y, z));
+                    planeNeighbor0MM.push_back(getWriteIndex(level, id, idx));
+                }
+            }
+            if (!f0PP) {
+                const int n = f.findNeighborPPM(y, z, (real)0);
+                if (n != -1) {
+                    f0PP = true;
+                    weights0PP.push_back(weight(f, n, y, z));
+                    planeNeighbor0PP.push_back(getWriteIndex(level, id, n));
+                }
+            }
+            if (!f0PM) {
+                const int n = f.findNeighborPMM(y, z, (real)0);
+                if (n != -1) {
+                    f0PM = true;
+                    weights0PM.push_back(weight(f, n, y, z));
+                    planeNeighbor0PM.push_back(getWriteIndex(level, id, n));
+                }
+            }
+            if (!f0MP) {
+                const int n = f.findNeighborMPM(y, z, (real)0);
+                if (n != -1) {
+                    f0MP = true;
+                    weights0MP.push_back(weight(f, n, y, z));
+                    planeNeighbor0MP.push_back(getWriteIndex(level, id, n));
+                }
+            }
+            all = f0PP && f0PM && f0MP && f0MM;
+            if (all) break;
+        }
+        if (!all)
+            throw std::runtime_error("VTKReader::fillArrays(): Did not find neighbors in the FileCollection for all points");
+    }
+    for (size_t l = 0; l < fileCollection->files.size(); l++)
+        for (size_t id = 0; id < fileCollection->files[l].size(); id++)
+            if (fileCollection->files[l][id][0].markNANs(readIndices[l][id]))
+                throw std::runtime_error("Found a NAN in the precursor where a velocity is needed");
+}
+
+void VTKReader::getNextData(real* data, uint numberOfNodes, real time)
+{
+    const uint level = readLevel;
+    for (size_t id = 0; id < fileCollection->files[level].size(); id++) {
+        auto& parts = fileCollection->files[level][id];
+        size_t k = nFile[level][id];
+        if (!parts[k].inZBounds(time)) {
+            k++;
+            if (k == parts.size()) throw std::runtime_error("Not enough Precursor Files to read");
+            parts[k - 1].unloadFile();
+        }
+        VTKFile& f = parts[k];
+        const int off = f.getClosestIdxZ(time) * f.getNumberOfPointsInXYPlane();
+        f.getData(data, numberOfNodes, readIndices[level][id], writeIndices[level][id], (uint)off, writingOffset);
+        nFile[level][id] = k;
+    }
+}
+
+std::shared_ptr<FileCollection> createFileCollection(std::string prefix, TransientBCFileType type)
+{
+    if (type == TransientBCFileType::VTK) return std::make_shared<VTKFileCollection>(std::move(prefix));
+    throw std::runtime_error("createFileCollection: Unknown file type");
+}
+
+std::shared_ptr<TransientBCInputFileReader> createReaderForCollection(std::shared_ptr<FileCollection> collection,
+                                                                      uint readLevel)
+{
+    if (collection->getFileType() == TransientBCFileType::VTK)
+        return std::make_shared<VTKReader>(std::static_pointer_cast<VTKFileCollection>(collection), readLevel);
+    throw std::runtime_error("createReaderForCollection: no reader for this file type");
+}
+
+void setPrecursorFileReader(QforPrecursorBoundaryConditions& q, std::shared_ptr<TransientBCInputFileReader> reader)
+{
+    q.numberOfPrecursorNodes = (int)reader->getNPointsRead();
+    q.numberOfQuantities = reader->getNumberOfQuantities();
+    q.getNextData = [reader](real* data, int numberOfNodes, double time) {
+        reader->getNextData(data, (uint)numberOfNodes, (real)time);
+    };
 }
 
 VFX_HOST_END

@@ -527,4 +527,134 @@ private:
     std::map<uint32_t, std::vector<std::string>> fileNamesForCollectionFileTimeSeries;
 };
 
+// Precursor input files (src/gpu/GridGenerator/TransientBCSetter/TransientBCSetter.h:53-233):
+// the planes PrecursorWriter writes (VTK image data, x and y = the inflow
+// plane's y and z, z = time; <prefix>_lev_<l>_ID_<id>_File_<part>.bin.vti),
+// their discovery and the reader that builds each boundary node's four-point
+// stencil (fillArrays) and fills the plane array at a physical time
+// (getNextData).  Same behaviour as the reference, with two changes off the
+// precursor grid: the 0PM neighbour goes to the 0PM list (the reference
+// appends it to 0PP, TransientBCSetter.cpp:386) and a file point is looked up
+// among the read indices (getWriteIndex, :430-441, searches the write indices).
+enum class TransientBCFileType { VTK };
+
+struct Quantity {
+    std::string name;
+    int offset = 0;
+    std::vector<double> values;
+};
+
+class VTKFile {
+public:
+    explicit VTKFile(std::string fileName) : fileName(std::move(fileName)) { readHeader(); }
+    void getData(real* data, uint numberOfNodes, const std::vector<uint>& readIndices, const std::vector<uint>& writeIndices,
+                 uint offsetRead, uint offsetWrite);
+    bool markNANs(const std::vector<uint>& readIndices) const;
+    bool inBoundingBox(real x, real y, real z) const { return inXBounds(x) && inYBounds(y) && inZBounds(z); }
+    bool inXBounds(real x) const { return x <= maxX && x >= minX; }
+    bool inYBounds(real y) const { return y <= maxY && y >= minY; }
+    bool inZBounds(real z) const { return z <= maxZ && z >= minZ; }
+    int findNeighborMMM(real x, real y, real z) const { return neighbor(getIdxM00(x), getIdx0M0(y), getIdx00M(z)); }
+    int findNeighborMPM(real x, real y, real z) const { return neighbor(getIdxM00(x), getIdx0M0(y) + 1, getIdx00M(z)); }
+    int findNeighborPMM(real x, real y, real z) const { return neighbor(getIdxM00(x) + 1, getIdx0M0(y), getIdx00M(z)); }
+    int findNeighborPPM(real x, real y, real z) const { return neighbor(getIdxM00(x) + 1, getIdx0M0(y) + 1, getIdx00M(z)); }
+    int getIdxX(int i) const { return i % nx; }
+    int getIdxY(int i) const { return (i / nx) % ny; }
+    int getIdxZ(int i) const { return i / (nx * ny); }
+    real getX(int i) const { return getIdxX(i) * deltaX + minX; }
+    real getY(int i) const { return getIdxY(i) * deltaY + minY; }
+    real getZ(int i) const { return getIdxZ(i) * deltaZ + minZ; }
+    int getIdxM00(real x) const { return (int)((x - minX) / deltaX); }
+    int getIdx0M0(real y) const { return (int)((y - minY) / deltaY); }
+    int getIdx00M(real z) const { return (int)((z - minZ) / deltaZ); }
+    int getClosestIdxZ(real z) const;
+    int getLinearIndex(int ix, int iy, int iz) const { return ix + nx * (iy + ny * iz); }
+    int getNumberOfPointsInXYPlane() const { return nx * ny; }
+    int getNumberOfPoints() const { return nx * ny * nz; }
+    size_t getNumberOfQuantities() const { return quantities.size(); }
+    void loadFile();
+    void unloadFile();
+    bool isLoaded() const { return loaded; }
+
+private:
+    int neighbor(int ix, int iy, int iz) const
+    {
+        const int i = getLinearIndex(ix, iy, iz);
+        return i >= 0 && i < nx * ny * nz ? i : -1;
+    }
+    void readHeader();
+    std::string fileName;
+    real minX = 0, maxX = 0, minY = 0, maxY = 0, minZ = 0, maxZ = 0;
+    real deltaX = 1, deltaY = 1, deltaZ = 1;
+    int nx = 0, ny = 0, nz = 0;
+    std::vector<Quantity> quantities;
+    bool loaded = false;
+};
+
+class FileCollection {
+public:
+    explicit FileCollection(std::string prefix) : prefix(std::move(prefix)) {}
+    virtual ~FileCollection() = default;
+    virtual size_t getNumberOfQuantities() = 0;
+    virtual TransientBCFileType getFileType() = 0;
+
+protected:
+    std::string prefix;
+};
+
+class VTKFileCollection : public FileCollection {
+public:
+    explicit VTKFileCollection(std::string prefix) : FileCollection(std::move(prefix)) { findFiles(); }
+    TransientBCFileType getFileType() override { return TransientBCFileType::VTK; }
+    size_t getNumberOfQuantities() override { return files[0][0][0].getNumberOfQuantities(); }
+    static inline const std::string suffix = "vti";
+    std::vector<std::vector<std::vector<VTKFile>>> files;
+
+private:
+    void findFiles();
+    std::string makeFileName(int level, int id, int part) const;
+};
+
+class TransientBCInputFileReader {
+public:
+    virtual ~TransientBCInputFileReader() = default;
+    virtual void getNextData(real* data, uint numberOfNodes, real time) = 0;
+    virtual void fillArrays(std::vector<real>& coordsY, std::vector<real>& coordsZ) = 0;
+    uint getNPoints() const { return nPoints; }
+    uint getNPointsRead() const { return nPointsRead; }
+    size_t getNumberOfQuantities() const { return nQuantities; }
+    void setWritingOffset(uint offset) { writingOffset = offset; }
+    void getNeighbors(uint* neighbor0PP, uint* neighbor0PM, uint* neighbor0MP, uint* neighbor0MM);
+    void getWeights(real* weights0PP, real* weights0PM, real* weights0MP, real* weights0MM);
+
+    std::vector<uint> planeNeighbor0PP, planeNeighbor0PM, planeNeighbor0MP, planeNeighbor0MM;
+    std::vector<real> weights0PP, weights0PM, weights0MP, weights0MM;
+
+protected:
+    uint nPoints = 0, nPointsRead = 0, writingOffset = 0;
+    size_t nQuantities = 0;
+};
+
+class VTKReader : public TransientBCInputFileReader {
+public:
+    VTKReader(std::shared_ptr<VTKFileCollection> fileCollection, uint readLevel);
+    void getNextData(real* data, uint numberOfNodes, real time) override;
+    void fillArrays(std::vector<real>& coordsY, std::vector<real>& coordsZ) override;
+
+private:
+    uint getWriteIndex(int level, int id, int linearIndex);
+    void initializeIndexVectors();
+    std::vector<std::vector<std::vector<uint>>> readIndices, writeIndices;
+    std::vector<std::vector<size_t>> nFile;
+    std::shared_ptr<VTKFileCollection> fileCollection;
+    uint readLevel;
+};
+
+std::shared_ptr<FileCollection> createFileCollection(std::string prefix, TransientBCFileType type);
+std::shared_ptr<TransientBCInputFileReader> createReaderForCollection(std::shared_ptr<FileCollection> fileCollection,
+                                                                      uint readLevel);
+// the reader as the precursor boundary's plane source (GridGenerator hands
+// the reader's getNextData the array `next`, GridGenerator.cpp:436-478)
+void setPrecursorFileReader(QforPrecursorBoundaryConditions& q, std::shared_ptr<TransientBCInputFileReader> reader);
+
 VFX_HOST_END

@@ -20,16 +20,17 @@ per call).  Arithmetic runs in VirtualFluids' `real` (float32 by default,
 float64 as with VF_DOUBLE_ACCURACY); the header's origin and spacing are read
 as float like readStringToVector<float>.
 
-Kept from the reference: the NaN check reads the first entries of the first
-quantity (VTKFile::markNANs, TransientBCSetter.cpp:163-175).  Two changes,
-both where the reference's lists go wrong off the precursor's grid (for a
-plane whose points coincide with the boundary nodes, the usual case, the
-results are the same): the reference appends the 0PM neighbour to the 0PP
-list (:386), which leaves 0PM short and makes the neighbour copy run past its
-array, here it goes to the 0PM list; and VTKReader::getWriteIndex (:430-441)
-looks the file index up among the write indices, so a point that two stencils
-share can resolve to another point's slot, here it is looked up among the
-read indices.
+Three changes.  Two are where the reference's lists go wrong off the
+precursor's grid (for a plane whose points coincide with the boundary nodes,
+the usual case, the results are the same): the reference appends the 0PM
+neighbour to the 0PP list (TransientBCSetter.cpp:386), which leaves 0PM short
+and makes the neighbour copy run past its array, here it goes to the 0PM
+list; and VTKReader::getWriteIndex (:430-441) looks the file index up among
+the write indices, so a point that two stencils share can resolve to another
+point's slot, here it is looked up among the read indices.  The third: the
+NaN check (VTKFile::markNANs, :163-175) reads into a vector it only reserved,
+so it never reports; here the first record's values at the read indices are
+checked.
 """
 from __future__ import annotations
 
@@ -197,9 +198,12 @@ class VTKFile:
     def loaded(self) -> bool:
         return self.values is not None
 
-    def has_nan_in_first(self, count: int) -> bool:
-        first = np.fromfile(self.filename, dtype="<f8", count=count, offset=self.quantities[0][1])
-        return bool(np.isnan(first).any())
+    def has_nan(self, read_idx) -> bool:
+        """A NaN in the first record of the first quantity at the read indices."""
+        first = np.fromfile(self.filename, dtype="<f8", count=max(1, self.nx * self.ny), offset=self.quantities[0][1])
+        idx = np.asarray(read_idx, dtype=np.int64)
+        idx = idx[idx < first.size]
+        return bool(np.isnan(first[idx]).any())
 
     def get_data(self, data, n_nodes, read_idx, write_idx, off_read, off_write):
         if not self.loaded:
@@ -322,7 +326,7 @@ class VTKReader:
                 raise RuntimeError(f"VTKReader.fill_arrays: no precursor neighbours for the point y={y}, z={z}")
         for lev, parts_of in enumerate(self.collection.files):
             for fid, parts in enumerate(parts_of):
-                if parts[0].has_nan_in_first(len(self.read_idx[lev][fid])):
+                if parts[0].has_nan(self.read_idx[lev][fid]):
                     raise RuntimeError("Found a NAN in the precursor where a velocity is needed")
 
     def stencil(self):
