@@ -467,11 +467,12 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
     spanwise), K17 at Re = 10^4 on the cube height.  Every coarse step (two
     fine steps, one coarse step, both levels' boundary kernels, fine-to-coarse
     and coarse-to-fine) is recorded into a HIP graph of `sample_every` coarse
-    steps.  Both schedules (launch per kernel, and merged: the coarse step and
+    steps.  Three schedules (launch per kernel; merged: the coarse step and
     the first fine step in one launch, each phase's boundary groups of both
-    levels in one launch, fine-to-coarse and coarse-to-fine in one launch) are
-    timed over 500 coarse steps with the levels' launch shapes autotuned; the
-    faster one runs the long window.  Time-averaged
+    levels in one launch, fine-to-coarse and coarse-to-fine in one launch;
+    launch per kernel with the coarse outlet inside the coarse collide-stream
+    kernel) are timed over 500 coarse steps with the levels' launch shapes
+    autotuned; the fastest runs the long window.  Time-averaged
     velocity over the second half on both levels; the
     FP32 and FP16 forms (FP16: deviations from the u_ref equilibrium) against
     FP64.  At Re = 10^4 the wake is chaotic: two forms that differ by rounding
@@ -499,10 +500,11 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
     means, early_means = {}, {}
     for storage in forms:
         uref = (case.u_ref, 0.0, 0.0) if storage == "fp16" else (0.0, 0.0, 0.0)
-        def stepper(merged):
+        def stepper(schedule):
             st = vfx.RefinedStepper(nest, case.omega, storage=storage, coarse_solid=case.coarse_solid,
                                     fine_solid=case.fine_solid, coarse_bcs=cbcs, fine_bcs=fbcs,
-                                    reference_velocity=uref, merged=merged)
+                                    reference_velocity=uref, merged=schedule == "merged",
+                                    fuse_outlet=schedule == "fused_outlet")
             for lat, z in ((st.coarse, zc), (st.fine, zf)):
                 n = z.size
                 lat.init_equilibrium(np.zeros(n), profile(z), np.zeros(n), np.zeros(n))
@@ -510,8 +512,8 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
 
         # both schedules with their levels' launch shapes tuned, timed over a
         # short window without statistics; the faster one runs the long window
-        def timed(merged):
-            st = stepper(merged)
+        def timed(schedule):
+            st = stepper(schedule)
             tuned = st.autotune()
             graph = st.capture(sample_every)
             st.replay(graph, sample_every)
@@ -522,10 +524,10 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
             return ms, launches, tuned
 
         n_sep = 10
-        ms_sep, launches_sep, _ = timed(False)
-        ms_merged, launches_merged, tuned = timed(True)
-        use_merged = ms_merged <= ms_sep
-        st = stepper(use_merged)
+        sched = {name: timed(name) for name in ("separate", "merged", "fused_outlet")}
+        best = min(sched, key=lambda k: sched[k][0])
+        tuned = sched[best][2]
+        st = stepper(best)
         for lat, name in ((st.coarse, "coarse"), (st.fine, "fine")):
             lat.tile_nodes, lat.xcd_run = tuned[name][0], tuned[name][1]
         graph = st.capture(sample_every)
@@ -550,14 +552,11 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
         means[storage] = (sc.mean()[:, act], sf.mean()[:, ffl])
         early_means[storage] = (sce.mean()[:, act], sfe.mean()[:, ffl])
         r = {"mlups": round(lups / (ms * 1e-3) / 1e6, 1), "ms_per_coarse_step": round(ms / (chunks * sample_every), 4),
-             "schedule": "merged" if use_merged else "separate", "launches_per_coarse_step": st.launches_per_step,
+             "schedule": best, "launches_per_coarse_step": st.launches_per_step,
              "launch": {k: {"tile_nodes": v[0], "xcd_run": v[1]} for k, v in tuned.items()},
-             "merged_launches_without_statistics": {"ms_per_coarse_step": round(ms_merged / (n_sep * sample_every), 4),
-                                                    "launches_per_coarse_step": launches_merged,
-                                                    "coarse_steps": n_sep * sample_every},
-             "separate_launches_without_statistics": {"ms_per_coarse_step": round(ms_sep / (n_sep * sample_every), 4),
-                                                      "launches_per_coarse_step": launches_sep,
-                                                      "coarse_steps": n_sep * sample_every},
+             "schedules_without_statistics": {
+                 k: {"ms_per_coarse_step": round(v[0] / (n_sep * sample_every), 4), "launches_per_coarse_step": v[1],
+                     "coarse_steps": n_sep * sample_every} for k, v in sched.items()},
              "finite": bool(torch.isfinite(means[storage][1]).all())}
         out[storage] = r
         print(f"# cube {storage}: {r['mlups']:.0f} MLUPS (refined, fine counted twice)", file=sys.stderr, flush=True)

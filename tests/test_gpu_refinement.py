@@ -329,3 +329,44 @@ def test_two_level_launch_in_the_levels_tile_width(cuda, tiles):
     b.step(3)
     torch.cuda.synchronize()
     assert torch.equal(a.coarse.f, b.coarse.f) and torch.equal(a.fine.f, b.fine.f)
+
+
+def _cube_stepper_kw(storage, **kw):
+    from tests.test_oracle_refinement import small_cube_case
+    from virtualfluids_amd import geometry
+
+    case = small_cube_case()
+    nest = case.nest
+    Nc, Nf = int(np.prod(nest.coarse_dims)), int(np.prod(nest.fine_dims))
+    rng = np.random.default_rng(12)
+    Fc = np.tile(oracle.equilibrium(0.0, case.u_ref, 0.0, 0.0), (Nc, 1)) + rng.normal(0, 1e-4, (Nc, 27)) * cases.W
+    Ff = np.tile(oracle.equilibrium(0.0, case.u_ref, 0.0, 0.0), (Nf, 1)) + rng.normal(0, 1e-4, (Nf, 27)) * cases.W
+    uref = (case.u_ref, 0.0, 0.0) if storage == "fp16" else (0.0, 0.0, 0.0)
+    st = vfx.RefinedStepper(nest, case.omega, storage=storage, coarse_solid=case.coarse_solid,
+                            fine_solid=case.fine_solid, coarse_bcs=geometry.boundary_conditions(case.coarse_bcs),
+                            fine_bcs=geometry.boundary_conditions(case.fine_bcs), reference_velocity=uref, **kw)
+    st.coarse.write_populations(Fc, even=True)
+    st.fine.write_populations(Ff, even=True)
+    return st
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp32", "fp16"])
+def test_fused_outlet_equals_separate_launches(cuda, storage):
+    """The coarse outlet inside the coarse collide-stream kernel (taken at the
+    next coarse gather, applied once more at the end of each step call) leaves
+    the bytes of the separate pressure kernel; split calls, eager and
+    graph-replayed."""
+    print(f"[refined fused outlet {storage}]", flush=True)
+    a = _cube_stepper_kw(storage)
+    b = _cube_stepper_kw(storage, fuse_outlet=True)
+    assert b._plan_c.pre is not None and b._plan_c.pre.bc.n > 0
+    a.step(7)
+    b.step(2)
+    b.step(1)
+    g = b.capture(2)
+    b.replay(g, 2)
+    b.step(2)
+    torch.cuda.synchronize()
+    assert (a.even_c, a.even_f) == (b.even_c, b.even_f)
+    assert torch.equal(a.coarse.f.view(torch.uint8), b.coarse.f.view(torch.uint8))
+    assert torch.equal(a.fine.f.view(torch.uint8), b.fine.f.view(torch.uint8))

@@ -33,7 +33,7 @@ import torch
 
 from . import _lib as V
 from .batch import Graph, capture
-from .lattice import BCSet, Lattice, _stream, step_levels
+from .lattice import BCSet, FusedPlan, Lattice, _stream, step_levels
 
 INACTIVE = 2  # value in the coarse node-state array
 
@@ -291,7 +291,7 @@ class RefinedStepper:
     def __init__(self, nest: NestedBox, omega_coarse, storage="fp64", collision="k17", device="cuda",
                  fp16_scale=None, variant="default", coarse_solid=None, fine_solid=None, coarse_bcs=(),
                  fine_bcs=(), reference_velocity=(0.0, 0.0, 0.0), turbulence="none", sgs_constant=0.0,
-                 merged=False):
+                 merged=False, fuse_outlet=False):
         self.nest = nest
         self.collision = collision
         self.omega_c = float(omega_coarse)
@@ -321,6 +321,19 @@ class RefinedStepper:
         self.merged = bool(merged)
         if self.merged:
             self._setup_merged()
+        self.fuse_outlet = bool(fuse_outlet)
+        self._plan_c = None
+        if self.fuse_outlet:
+            if self.merged:
+                raise ValueError("fuse_outlet runs with the launch-per-kernel schedule")
+            if collision != "k17" or variant != "default" or turbulence != "none":
+                raise ValueError("fuse_outlet runs the default K17 without turbulence model")
+            # the coarse level's pressure group inside its collide-stream kernel,
+            # taken at the next step's gather (the interface kernels between two
+            # coarse steps touch no node of the outlet or its neighbours)
+            self._plan_c = FusedPlan.build(self.coarse, self.coarse_bcs, post=False)
+            if self._plan_c.pre is None:
+                raise ValueError("fuse_outlet: the coarse level has no pressure group the kernel can take")
 
     def _setup_merged(self):
         """Fewer launches per coarse step (same values): the coarse step and the
@@ -393,17 +406,32 @@ class RefinedStepper:
                 lat.apply_bc(bc, not even, stream=stream)
         return not even
 
+    def _coarse_step_fused(self, even, first, stream):
+        plan, lat = self._plan_c, self.coarse
+        lat.k17_step_fused(even, [], plan.pre, apply_pre=not first, any_mask=plan.any_mask, stream=stream)
+        for bc in plan.residual_post:
+            lat.apply_bc(bc, even, stream=stream)
+        for bc in plan.residual_pre:
+            lat.apply_bc(bc, not even, stream=stream)
+        return not even
+
     def step(self, n=1, stream=None):
         if self.merged:
             for _ in range(n):
                 self._step_merged(stream)
             return
-        for _ in range(n):
+        for i in range(n):
             for _ in range(2):
                 self.even_f = self._level_step(self.fine, self.fine_bcs, self.collision, self.even_f, stream)
-            self.even_c = self._level_step(self.coarse, self.coarse_bcs, self.collision, self.even_c, stream)
+            if self._plan_c is not None:
+                self.even_c = self._coarse_step_fused(self.even_c, i == 0, stream)
+            else:
+                self.even_c = self._level_step(self.coarse, self.coarse_bcs, self.collision, self.even_c, stream)
             scale_fc(self.coarse, self.fine, self.fc, self.even_c, self.even_f, stream)
             scale_cf(self.coarse, self.fine, self.cf, self.even_c, self.even_f, stream)
+        if n > 0 and self._plan_c is not None:
+            # the outlet once more, so the stored state is the separate kernels'
+            self.coarse.apply_bc(self._plan_c.pre.bc, self.even_c, stream=stream)
 
     def capture(self, n=1) -> Graph:
         """n coarse steps from the current parities as one HIP graph."""
@@ -441,6 +469,9 @@ class RefinedStepper:
     @property
     def launches_per_step(self) -> int:
         """Kernel launches of one coarse step in the current schedule."""
+        if self._plan_c is not None:
+            p = self._plan_c
+            return 2 * (1 + len(self.fine_bcs)) + 1 + len(p.residual_post) + len(p.residual_pre) + 2
         if self.merged:
             sets = (self._set_post_both, self._set_pre_both, self._set_post_fine, self._set_pre_fine)
             rest = 2 * (len(self._f_post_rest) + len(self._f_pre_rest)) + len(self._c_post_rest) + len(self._c_pre_rest)
