@@ -296,7 +296,9 @@ def sparse_leg(torch, vfx, steps=50, warmup=10, forms=("fp16", "fp32", "fp64")):
     generator numbers a box.  This is the layout the reference's
     runCollision/K17 kernel reads (RunCollision.cuh:74-143); per node it adds the
     fluid index and three neighbour indices (16 B) to the populations, and four
-    second-level neighbour reads that mostly hit in cache."""
+    second-level neighbour reads that mostly hit in cache, except in the
+    wavefronts the regular-wavefront table marks (64 consecutive nodes with one
+    y and one z stride), which form their indices by arithmetic."""
     nx, ny, nz = DIMS
     N = nx * ny * nz
     k = np.arange(N, dtype=np.int64)
@@ -318,18 +320,25 @@ def sparse_leg(torch, vfx, steps=50, warmup=10, forms=("fp16", "fp32", "fp64")):
     out = {"layout": "VirtualFluids sparse (neighbour arrays + fluid-node list), x-fastest numbering",
            "nodes": N, "steps": steps}
     for form in forms:
-        lat = vfx.Lattice(storage=form, omega=OMEGA, device="cuda", neighbors=tuple(nb), fluid_nodes=fluid)
-        lat.init_equilibrium(*fields, even=True)
-        st = vfx.Stepper(lat, "k17")
-        st.step(warmup)
-        ms = _timed_loop(torch, stream, steps, lambda i: st.step(1)) / steps
         s = {"fp16": 2, "fp32": 4, "fp64": 8}[form]
-        out[form] = {"mlups": round(N / ms / 1e3, 1), "ms_per_step": round(ms, 4),
-                     "population_gbs": round(54 * s * N / ms / 1e6, 1),
-                     "gbs_with_index_streams": round((54 * s + 16) * N / ms / 1e6, 1)}
+        # with the regular-wavefront table (the default) and, for FP16, without
+        for groups in ((True, False) if form == "fp16" else (True,)):
+            lat = vfx.Lattice(storage=form, omega=OMEGA, device="cuda", neighbors=tuple(nb), fluid_nodes=fluid,
+                              regular_groups=groups)
+            lat.init_equilibrium(*fields, even=True)
+            st = vfx.Stepper(lat, "k17")
+            st.step(warmup)
+            ms = _timed_loop(torch, stream, steps, lambda i: st.step(1)) / steps
+            r = {"mlups": round(N / ms / 1e3, 1), "ms_per_step": round(ms, 4),
+                 "population_gbs": round(54 * s * N / ms / 1e6, 1),
+                 "regular_wavefront_fraction": round(lat.regular_fraction, 4)}
+            if groups:
+                out[form] = r
+            else:
+                out[form]["without_regular_table"] = r
+            del st, lat
+            torch.cuda.empty_cache()
         print(f"# sparse layout {form}: {out[form]['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
-        del st, lat
-        torch.cuda.empty_cache()
     return out
 
 

@@ -149,6 +149,11 @@ typedef struct vfx_lattice {
      * vfx_k17_autotune measures them.  Other kernels ignore both. */
     uint32_t tile_nodes;
     uint32_t xcd_run;
+
+    /* Sparse addressing: the table vfx_sparse_regular_groups fills for these
+     * index arrays (NULL: every wavefront reads its indices).  Recompute it
+     * when the index arrays change. */
+    const void* regular_groups;
 } vfx_lattice;
 
 /* vf::lbm::TurbulenceModel */
@@ -268,6 +273,16 @@ vfx_status vfx_collide_stream(const vfx_lattice* lat, int32_t collision, int32_t
                               int32_t collision_template, vfx_stream stream);
 vfx_status vfx_k17_step(const vfx_lattice* lat, int32_t is_even_timestep, int32_t collision_template, vfx_stream stream);
 vfx_status vfx_bgk_step(const vfx_lattice* lat, int32_t is_even_timestep, vfx_stream stream);
+
+/* (new) Regular wavefronts of a sparse lattice: where the 64 work items of a
+ * wavefront are 64 consecutive nodes (x-fastest numbering) whose eight twist
+ * nodes follow one y and one z stride -- checked against neighbor_x/y/z for
+ * every colliding node -- the step forms the indices by arithmetic instead of
+ * reading them (same indices, same results).  `table` is a device buffer of
+ * vfx_sparse_regular_groups_bytes(lat) bytes (16 per wavefront of the
+ * step's grid); set lat->regular_groups to it.  Sparse lattices only. */
+uint64_t vfx_sparse_regular_groups_bytes(const vfx_lattice* lat);
+vfx_status vfx_sparse_regular_groups(const vfx_lattice* lat, void* table, uint64_t table_bytes, vfx_stream stream);
 
 /* (new) Autotuner of the structured K17 step: times vfx_k17_step with every
  * tile width (64, 128, 256 nodes) and XCD run (1, 4, 8, 16, 32, 64, 256) on

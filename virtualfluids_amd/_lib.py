@@ -41,7 +41,7 @@ class VfxLattice(C.Structure):
         ("turbulence_model", C.c_int32), ("sgs_constant", C.c_double), ("turbulent_viscosity", C.c_void_p),
         ("neighbor_inverse", C.c_void_p),
         ("body_force", C.c_void_p * 3), ("forces", C.c_double * 3), ("force_factor", C.c_double),
-        ("tile_nodes", C.c_uint32), ("xcd_run", C.c_uint32),
+        ("tile_nodes", C.c_uint32), ("xcd_run", C.c_uint32), ("regular_groups", C.c_void_p),
     ]
 
 
@@ -88,6 +88,7 @@ _SIGNATURES = {
     "vfx_collide_stream": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_int32, C.c_void_p],
     "vfx_k17_step": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_void_p],
     "vfx_bgk_step": [C.POINTER(VfxLattice), C.c_int32, C.c_void_p],
+    "vfx_sparse_regular_groups": [C.POINTER(VfxLattice), C.c_void_p, C.c_uint64, C.c_void_p],
     "vfx_k17_autotune": [C.POINTER(VfxLattice), C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.POINTER(C.c_uint32),
                          C.POINTER(C.c_uint32), C.POINTER(C.c_float)],
     "vfx_k17_step_fused": [C.POINTER(VfxLattice), C.c_int32, C.POINTER(VfxFusedBC), C.c_int32,
@@ -161,6 +162,8 @@ def lib():
         L.vfx_reduce_workspace_bytes.restype = C.c_uint64
         L.vfx_reduce_noalloc_max_items.argtypes = []
         L.vfx_reduce_noalloc_max_items.restype = C.c_uint64
+        L.vfx_sparse_regular_groups_bytes.argtypes = [C.POINTER(VfxLattice)]
+        L.vfx_sparse_regular_groups_bytes.restype = C.c_uint64
         L.vfx_last_error.restype = C.c_char_p
         L.vfx_version.restype = C.c_int32
         _lib = L

@@ -92,7 +92,10 @@ vfx_status make_lat(const vfx_lattice* in, vfx::Lat& L)
         L.fluid = in->fluid_nodes;
         L.nfluid = in->n_fluid;
         L.types = in->type_of_grid_node;
+        L.rgroups = (const uint4*)in->regular_groups;
     }
+    if (in->direct && in->regular_groups)
+        return fail(VFX_EINVAL, "regular_groups belongs to the sparse layout");
     if (in->direct && in->type_of_grid_node)
         return fail(VFX_EINVAL, "type_of_grid_node selects nodes of the sparse layout; direct addressing uses solid_mask");
     if (in->storage == VFX_FP16_SCALED) {
@@ -242,6 +245,35 @@ vfx_status vfx_k17_step(const vfx_lattice* lat, int32_t is_even, int32_t tmpl, v
 vfx_status vfx_bgk_step(const vfx_lattice* lat, int32_t is_even, vfx_stream stream)
 {
     return vfx_collide_stream(lat, VFX_BGK, is_even, VFX_TEMPLATE_DEFAULT, stream);
+}
+
+uint64_t vfx_sparse_regular_groups_bytes(const vfx_lattice* lat)
+{
+    if (!lat || lat->direct) return 0;
+    vfx::Lat L;
+    std::memset(&L, 0, sizeof(L));
+    L.N = lat->n_nodes;
+    L.nfluid = lat->n_fluid;
+    L.types = lat->type_of_grid_node;
+    return vfx::sparse_groups(L) * 16u;
+}
+
+vfx_status vfx_sparse_regular_groups(const vfx_lattice* lat, void* table, uint64_t table_bytes, vfx_stream stream)
+{
+    if (!lat || !table) return fail(VFX_EINVAL, "NULL argument");
+    if (lat->direct) return fail(VFX_EINVAL, "regular groups belong to the sparse layout");
+    vfx_lattice plain = *lat;
+    plain.regular_groups = nullptr;
+    vfx::Lat L;
+    if (vfx_status s = make_lat(&plain, L)) return s;
+    const uint64_t groups = vfx::sparse_groups(L);
+    if (table_bytes < groups * 16u)
+        return fail(VFX_EINVAL, "table holds %llu bytes, %llu needed", (unsigned long long)table_bytes,
+                    (unsigned long long)(groups * 16u));
+    if (groups == 0) return VFX_OK;
+    if (groups >= (1ull << 32)) return fail(VFX_EUNSUPPORTED, "too many wavefronts");
+    return hip_status(vfx::launch_sparse_groups(L, (uint4*)table, (uint32_t)groups, (hipStream_t)stream),
+                      "sparse groups launch");
 }
 
 vfx_status vfx_k17_autotune(const vfx_lattice* lat, int32_t is_even, int32_t tmpl, int32_t steps, vfx_stream stream,

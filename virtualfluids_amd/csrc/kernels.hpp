@@ -125,6 +125,7 @@ struct Lat {
     double force_factor;
     uint32_t xrun_lg;      // 1 + log2 of the XCD tile run of the direct step (0: xcd_run<S>())
     uint32_t tile;         // nodes per workgroup along x of the direct step
+    const uint4* rgroups;  // sparse: per 64 work items {base node, y stride, z stride, regular}, or NULL
 };
 
 template <typename S> __device__ __forceinline__ S* plane(const Lat& L, int p) { return (S*)L.f + (uint64_t)p * L.N; }
@@ -424,6 +425,55 @@ __device__ __forceinline__ void nbrs_sparse_wave(const Lat& L, uint32_t node, ui
     if (!hit || last) kk[7] = L.nbz[kk[3]];
 }
 
+// Regular wavefronts of the sparse layout.  Where a grid is numbered x
+// fastest (as VirtualFluids' generator numbers it), the 64 work items of a
+// wavefront are mostly 64 consecutive nodes whose eight twist nodes are
+// node + dx (j & 1) + sy ((j >> 1) & 1) + sz (j >> 2) with dx = 1 and one y
+// and one z stride; at the end of a periodic row the last lane's +x neighbour
+// is the row's first node, dx = 1 - sy (flag bit 1).  k_sparse_groups marks
+// those wavefronts (every colliding lane checked against its index arrays,
+// every formed index in bounds); the step forms their indices arithmetically
+// and reads no index array.  Same indices, so the same populations, as the
+// loads.
+__device__ __forceinline__ void regular_nbrs(uint32_t k, uint32_t dx, uint32_t sy, uint32_t sz, uint32_t (&kk)[8])
+{
+#pragma unroll
+    for (int j = 0; j < 8; ++j) kk[j] = k + dx * (j & 1) + sy * ((j >> 1) & 1) + sz * (j >> 2);
+}
+template <int = 0> __global__ void __launch_bounds__(256) k_sparse_groups(Lat L, uint4* table, uint32_t ngroups)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t k = 0;
+    const bool active = sparse_node(L, i, k);
+    const bool has = L.types ? i < L.N : active;
+    const uint32_t node = L.types ? (i < L.N ? (uint32_t)i : 0u) : (active ? k : 0u);
+    const uint32_t k0 = __shfl(node, 0, 64);
+    uint32_t sy = 0, sz = 0;
+    if (lane == 0 && has) {
+        sy = L.nby[node] - node;
+        sz = L.nbz[node] - node;
+    }
+    sy = __shfl(sy, 0, 64);
+    sz = __shfl(sz, 0, 64);
+    bool ok = has && node == k0 + lane, okw = ok; // okw: the last lane wrapping in x
+    if (ok && active) {
+        uint32_t kk[8], r[8], rw[8];
+        nbrs_sparse(L, node, kk);
+        regular_nbrs(node, 1u, sy, sz, r);
+        regular_nbrs(node, lane == 63u ? 1u - sy : 1u, sy, sz, rw);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            ok = ok && kk[j] == r[j];
+            okw = okw && kk[j] == rw[j];
+        }
+    }
+    const bool inside = (uint64_t)k0 + 64u + (uint64_t)sy + (uint64_t)sz < L.N && sy < L.N && sz < L.N;
+    const bool all = __ballot(ok) == ~0ull, allw = __ballot(okw) == ~0ull;
+    const uint64_t w = i >> 6;
+    if (lane == 0 && w < ngroups) table[w] = make_uint4(k0, sy, sz, !inside ? 0u : all ? 1u : allw ? 3u : 0u);
+}
+
 // VirtualFluids sparse addressing over the fluid-node list
 // (RunCollision.cuh:74-143 semantics).
 template <typename S, int COLL, bool EVEN, bool FAST>
@@ -432,16 +482,31 @@ __global__ void __launch_bounds__(256) k_step_sparse(Lat L, Rates<typename Compu
     using T = typename Compute<S>::T;
     const uint32_t i = xcd_block(blockIdx.x, gridDim.x, xcd_run<S>()) * blockDim.x + threadIdx.x;
     uint32_t kk[8], k;
-    const bool active = sparse_node(L, i, k);
-    // the node of work item i + 1 (the last lane of a wavefront reads its arrays)
-    uint32_t next = 0;
-    if ((threadIdx.x & 63u) == 63u) {
-        if (L.types) next = i + 1 < L.N ? i + 1 : 0u;
-        else if (i + 1 < L.nfluid) next = L.fluid[i + 1] < L.N ? L.fluid[i + 1] : 0u;
+    bool active;
+    uint4 g = make_uint4(0u, 0u, 0u, 0u);
+    if (L.rgroups) g = L.rgroups[__builtin_amdgcn_readfirstlane(i >> 6)];
+    if (g.w & 1u) {
+        // a regular wavefront: its nodes and their twist nodes by arithmetic
+        const uint32_t lane = threadIdx.x & 63u;
+        k = g.x + lane;
+        active = true;
+        if (L.types) {
+            const uint32_t t = L.types[k];
+            active = t != kGeoSolid && t != kGeoVoid;
+        }
+        regular_nbrs(k, (g.w & 2u) && lane == 63u ? 1u - g.y : 1u, g.y, g.z, kk);
+    } else {
+        active = sparse_node(L, i, k);
+        // the node of work item i + 1 (the last lane of a wavefront reads its arrays)
+        uint32_t next = 0;
+        if ((threadIdx.x & 63u) == 63u) {
+            if (L.types) next = i + 1 < L.N ? i + 1 : 0u;
+            else if (i + 1 < L.nfluid) next = L.fluid[i + 1] < L.N ? L.fluid[i + 1] : 0u;
+        }
+        // every lane takes part in the shuffles: lanes without a node use the dummy node
+        const uint32_t node = L.types ? (i < L.N ? i : 0u) : (active ? k : 0u);
+        nbrs_sparse_wave(L, node, next, kk);
     }
-    // every lane takes part in the shuffles: lanes without a node use the dummy node
-    const uint32_t node = L.types ? (i < L.N ? i : 0u) : (active ? k : 0u);
-    nbrs_sparse_wave(L, node, next, kk);
     if (!active) return;
     T m[27], mv[4];
     constexpr bool O2 = FAST && coll_op(COLL) == VFX_K17 && coll_turb(COLL) == kTurbNone && sizeof(S) == 2;
@@ -2403,6 +2468,15 @@ template <typename S> hipError_t launch_step_levels(const LevelInput* in, int n,
     case 1: hipLaunchKernelGGL((k_step_levels<S, false, true>), g, b, 0, st, P); break;
     default: hipLaunchKernelGGL((k_step_levels<S, false, false>), g, b, 0, st, P); break;
     }
+    return hipGetLastError();
+}
+
+// one entry per wavefront of the step's grid (4 per 256 work items)
+inline uint64_t sparse_groups(const Lat& L) { return (node_items(L, 0) + kBlock - 1) / kBlock * (kBlock / 64); }
+inline hipError_t launch_sparse_groups(const Lat& L, uint4* table, uint32_t ngroups, hipStream_t st)
+{
+    const uint64_t items = (uint64_t)ngroups * 64u;
+    hipLaunchKernelGGL((k_sparse_groups<0>), dim3(grid_for(items, kBlock)), dim3(kBlock), 0, st, L, table, ngroups);
     return hipGetLastError();
 }
 

@@ -62,7 +62,8 @@ class Lattice:
     def __init__(self, dims=None, *, storage="fp64", omega=1.0, limiter=(0.01, 0.01, 0.01),
                  force=(0.0, 0.0, 0.0), solid=None, fp16_scale=None, neighbors=None, fluid_nodes=None,
                  n_nodes=None, device="cuda", variant="default", reference_velocity=(0.0, 0.0, 0.0),
-                 node_types=None, turbulence="none", sgs_constant=0.0, neighbor_inverse=None):
+                 node_types=None, turbulence="none", sgs_constant=0.0, neighbor_inverse=None,
+                 regular_groups=True):
         if storage not in STORAGE:
             raise ValueError(f"storage must be one of {sorted(STORAGE)}")
         self.storage = storage
@@ -153,6 +154,25 @@ class Lattice:
                 if inv.size != self.n_nodes or (inv.size and inv.max() >= self.n_nodes):
                     raise ValueError("neighbor_inverse needs n_nodes entries in range")
                 self.nbinv = torch.from_numpy(inv.view(np.int32)).to(self.device)
+        # sparse layout: wavefronts of consecutive nodes with one y and one z
+        # stride form their indices by arithmetic (vfx_sparse_regular_groups)
+        self.rgroups = None
+        if not self.direct and regular_groups:
+            st = self.struct()
+            nbytes = int(V.lib().vfx_sparse_regular_groups_bytes(C.byref(st)))
+            if nbytes:
+                t = torch.zeros(nbytes // 4, dtype=torch.int32, device=self.device)
+                V.check(V.lib().vfx_sparse_regular_groups(C.byref(st), t.data_ptr(), nbytes, _stream(None)),
+                        "vfx_sparse_regular_groups")
+                self.rgroups = t
+
+    @property
+    def regular_fraction(self) -> float:
+        """Share of the sparse step's wavefronts that form their indices by arithmetic."""
+        if self.rgroups is None:
+            return 0.0
+        flags = self.rgroups.view(-1, 4)[:, 3].cpu().numpy()
+        return float((flags & 1).mean()) if flags.size else 0.0
 
     def _macro_dtype(self):
         return torch.float64 if self.storage == "fp64" else torch.float32
@@ -198,6 +218,7 @@ class Lattice:
         # the built-in launch shape
         s.tile_nodes = getattr(self, "tile_nodes", 0)
         s.xcd_run = getattr(self, "xcd_run", 0)
+        s.regular_groups = _ptr(getattr(self, "rgroups", None))
         return s
 
     def autotune(self, even=True, steps=4, collision_template=0, stream=None):
