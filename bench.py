@@ -55,6 +55,8 @@ def parse():
     p.add_argument("--tgv-steps", type=int, default=2000)
     p.add_argument("--channel-steps", type=int, default=5000,
                    help="steps of the flow-past-spheres channel leg (0 skips it)")
+    p.add_argument("--channel-separate-steps", type=int, default=500,
+                   help="FP16 channel steps timed per boundary-kernel setting beside the leg's own")
     p.add_argument("--cube-steps", type=int, default=10000,
                    help="coarse steps of the refined wall-mounted-cube leg (0 skips it)")
     p.add_argument("--copy-probe", action="store_true",
@@ -304,35 +306,50 @@ def sparse_leg(torch, vfx, steps=50, warmup=10, forms=("fp16", "fp32", "fp64")):
     k = np.arange(N, dtype=np.int64)
     x, y, z = k % nx, (k // nx) % ny, k // (nx * ny)
 
-    def kof(xx, yy, zz):
-        return (xx % nx) + nx * ((yy % ny) + ny * (zz % nz)) + 1
+    def arrays(first):
+        """Neighbour arrays and fluid list with the box's nodes numbered from
+        `first` (nodes 0 .. first-1 are the dummy node and unused padding)."""
+        def kof(xx, yy, zz):
+            return (xx % nx) + nx * ((yy % ny) + ny * (zz % nz)) + first
 
-    nb = [np.zeros(N + 1, np.uint32) for _ in range(3)]
-    nb[0][1:] = kof(x + 1, y, z)
-    nb[1][1:] = kof(x, y + 1, z)
-    nb[2][1:] = kof(x, y, z + 1)
-    fluid = (k + 1).astype(np.uint32)
+        nb = [np.zeros(N + first, np.uint32) for _ in range(3)]
+        nb[0][first:] = kof(x + 1, y, z)
+        nb[1][first:] = kof(x, y + 1, z)
+        nb[2][first:] = kof(x, y, z + 1)
+        return tuple(nb), (k + first).astype(np.uint32)
+
+    layouts = {1: arrays(1), 64: arrays(64)}
     del k, x, y, z
-    fields = initial_flow(torch, DIMS, "cuda")
-    zero = torch.zeros(1, dtype=torch.float64, device="cuda")
-    fields = [torch.cat([zero, f]) for f in fields]
+    base = initial_flow(torch, DIMS, "cuda")
+
+    def fields(first):
+        zero = torch.zeros(first, dtype=torch.float64, device="cuda")
+        return [torch.cat([zero, f]) for f in base]
+
     stream = torch.cuda.current_stream()
     out = {"layout": "VirtualFluids sparse (neighbour arrays + fluid-node list), x-fastest numbering",
            "nodes": N, "steps": steps}
     for form in forms:
         s = {"fp16": 2, "fp32": 4, "fp64": 8}[form]
-        # with the regular-wavefront table (the default) and, for FP16, without
-        for groups in ((True, False) if form == "fp16" else (True,)):
-            lat = vfx.Lattice(storage=form, omega=OMEGA, device="cuda", neighbors=tuple(nb), fluid_nodes=fluid,
+        # the generator's numbering (dummy node 0, nodes from 1, plane stride
+        # N + 1) with the regular-wavefront table (the default); for FP16 also
+        # without the table, and with the nodes numbered from 64 (stride N + 64),
+        # so every row and plane starts on a 128-byte line
+        runs = (((True, 1), (False, 1), (True, 64)) if form == "fp16" else ((True, 1),))
+        for groups, first in runs:
+            nb, fluid = layouts[first]
+            lat = vfx.Lattice(storage=form, omega=OMEGA, device="cuda", neighbors=nb, fluid_nodes=fluid,
                               regular_groups=groups)
-            lat.init_equilibrium(*fields, even=True)
+            lat.init_equilibrium(*fields(first), even=True)
             st = vfx.Stepper(lat, "k17")
             st.step(warmup)
             ms = _timed_loop(torch, stream, steps, lambda i: st.step(1)) / steps
             r = {"mlups": round(N / ms / 1e3, 1), "ms_per_step": round(ms, 4),
                  "population_gbs": round(54 * s * N / ms / 1e6, 1),
                  "regular_wavefront_fraction": round(lat.regular_fraction, 4)}
-            if groups:
+            if first != 1:
+                out[form]["rows_on_line_boundaries"] = dict(r, first_node=first, plane_stride=N + first)
+            elif groups:
                 out[form] = r
             else:
                 out[form]["without_regular_table"] = r
@@ -683,7 +700,7 @@ def main():
         out["taylor_green"] = tgv_leg(torch, vfx, args.tgv, args.tgv_steps)
         torch.cuda.empty_cache()
     if args.channel_steps > 0:
-        out["channel"] = channel_leg(torch, vfx, args.channel_steps)
+        out["channel"] = channel_leg(torch, vfx, args.channel_steps, separate_steps=args.channel_separate_steps)
         if "fp16" in res and "fp16" in out["channel"]:
             # the channel has the periodic box's dims: the box's step time over the channel's
             c = out["channel"]["fp16"]
