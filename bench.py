@@ -291,7 +291,7 @@ def _timed_loop(torch, stream, chunks, body):
     return e0.elapsed_time(e1)
 
 
-def sparse_leg(torch, vfx, steps=50, warmup=10, forms=("fp16", "fp32", "fp64")):
+def sparse_leg(torch, vfx, steps=50, warmup=10, forms=("fp16", "fp32", "fp64"), autotune=True):
     """The same 512 x 256 x 256 periodic flow in VirtualFluids' own sparse
     layout: neighborX/Y/Z arrays (+x/+y/+z neighbour per node, index 0 = dummy
     node) and the fluid-node list, numbered x-fastest as VirtualFluids' grid
@@ -300,7 +300,8 @@ def sparse_leg(torch, vfx, steps=50, warmup=10, forms=("fp16", "fp32", "fp64")):
     fluid index and three neighbour indices (16 B) to the populations, and four
     second-level neighbour reads that mostly hit in cache, except in the
     wavefronts the regular-wavefront table marks (64 consecutive nodes with one
-    y and one z stride), which form their indices by arithmetic."""
+    y and one z stride), which form their indices by arithmetic.  Each run
+    tunes the step's XCD run first (vfx_k17_autotune)."""
     nx, ny, nz = DIMS
     N = nx * ny * nz
     k = np.arange(N, dtype=np.int64)
@@ -343,10 +344,15 @@ def sparse_leg(torch, vfx, steps=50, warmup=10, forms=("fp16", "fp32", "fp64")):
             lat.init_equilibrium(*fields(first), even=True)
             st = vfx.Stepper(lat, "k17")
             st.step(warmup)
+            launch = {"xcd_run": lat.xcd_run, "autotuned": autotune}
+            if autotune:
+                _, run, ms_t = lat.autotune(st.even, steps=4)
+                launch.update(xcd_run=run, autotune_ms_per_step=round(ms_t, 4))
+                st.step(2)
             ms = _timed_loop(torch, stream, steps, lambda i: st.step(1)) / steps
             r = {"mlups": round(N / ms / 1e3, 1), "ms_per_step": round(ms, 4),
                  "population_gbs": round(54 * s * N / ms / 1e6, 1),
-                 "regular_wavefront_fraction": round(lat.regular_fraction, 4)}
+                 "regular_wavefront_fraction": round(lat.regular_fraction, 4), "launch": launch}
             if first != 1:
                 out[form]["rows_on_line_boundaries"] = dict(r, first_node=first, plane_stride=N + first)
             elif groups:
@@ -695,7 +701,7 @@ def main():
               f"{e['per_domain_graph_mlups']:.0f}, per-domain eager {e['per_domain_eager_mlups']:.0f}",
               file=sys.stderr, flush=True)
     if args.sparse_steps > 0:
-        out["sparse_layout"] = sparse_leg(torch, vfx, args.sparse_steps)
+        out["sparse_layout"] = sparse_leg(torch, vfx, args.sparse_steps, autotune=not args.no_autotune)
     if args.tgv > 0:
         out["taylor_green"] = tgv_leg(torch, vfx, args.tgv, args.tgv_steps)
         torch.cuda.empty_cache()

@@ -140,8 +140,9 @@ typedef struct vfx_lattice {
     double forces[3];
     double force_factor;
 
-    /* Launch shape of the structured (direct) collide-and-stream step: nodes
-     * per workgroup along x (64, 128 or 256; 0 = 256) and the XCD tile run
+    /* Launch shape of the collide-and-stream step: nodes per workgroup along
+     * x (structured step: 64, 128 or 256; 0 = 256; the sparse step's
+     * workgroups are 256 work items: 0 or 256) and the XCD tile run
      * (workgroups reach the 8 XCDs round robin; consecutive tiles are handed
      * to each XCD in runs of this many, a power of two up to 4096, 1 = plain
      * order; 0 = the built-in choice, 16 for 2- and 4-byte populations, 256
@@ -284,13 +285,13 @@ vfx_status vfx_bgk_step(const vfx_lattice* lat, int32_t is_even_timestep, vfx_st
 uint64_t vfx_sparse_regular_groups_bytes(const vfx_lattice* lat);
 vfx_status vfx_sparse_regular_groups(const vfx_lattice* lat, void* table, uint64_t table_bytes, vfx_stream stream);
 
-/* (new) Autotuner of the structured K17 step: times vfx_k17_step with every
- * tile width (64, 128, 256 nodes) and XCD run (1, 4, 8, 16, 32, 64, 256) on
- * `steps` (made even, at least 2) steps after two warm-up steps each, from
- * parity is_even_timestep, and returns the fastest shape (*tile_nodes,
- * *xcd_run, its ms per step in *ms) for the caller to set in its
- * vfx_lattice.  The populations advance (42 trials), the parity is the same
- * afterwards.  Direct addressing only. */
+/* (new) Autotuner of the K17 step: times vfx_k17_step with every tile width
+ * (64, 128, 256 nodes; sparse lattices: 256 only) and XCD run (1, 4, 8, 16,
+ * 32, 64, 256) on `steps` (made even, at least 2) steps after two warm-up
+ * steps each, from parity is_even_timestep, and returns the fastest shape
+ * (*tile_nodes, *xcd_run, its ms per step in *ms) for the caller to set in
+ * its vfx_lattice.  The populations advance (42 trials structured, 14
+ * sparse), the parity is the same afterwards. */
 vfx_status vfx_k17_autotune(const vfx_lattice* lat, int32_t is_even_timestep, int32_t collision_template,
                             int32_t steps, vfx_stream stream, uint32_t* tile_nodes, uint32_t* xcd_run, float* ms);
 

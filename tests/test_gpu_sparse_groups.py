@@ -82,3 +82,48 @@ def test_regular_wavefronts_give_the_same_bytes(cuda, storage, name):
     else:
         assert 0.3 < frac < 0.95
     assert torch.equal(a.f.view(torch.uint8), b.f.view(torch.uint8))
+
+
+def _run_shape(storage, layout, run, steps=6):
+    nb, fluid, sidx, types = layout
+    N = nb[0].size - 1
+    F0 = oracle.init_field(*cases.random_flow(DIMS, seed=4))
+    Fs = np.zeros((N + 1, 27))
+    Fs[sidx] = F0
+    lat = vfx.Lattice(storage=storage, omega=1.8, neighbors=tuple(nb), fluid_nodes=fluid)
+    lat.xcd_run = run
+    lat.write_populations(Fs)
+    for i in range(steps):
+        lat.collide_stream("k17", i % 2 == 0)
+    torch.cuda.synchronize()
+    return lat
+
+
+@pytest.mark.parametrize("storage", ["fp64", "fp16"])
+def test_sparse_xcd_runs_give_the_same_bytes(cuda, storage):
+    """The XCD run of the sparse step only decides which workgroup takes which
+    256 work items: every run leaves the built-in order's bytes."""
+    print(f"[sparse xcd runs {storage}]", flush=True)
+    layout = _layouts()["list"]
+    ref = _run_shape(storage, layout, 0)
+    for run in (1, 2, 16, 256):
+        lat = _run_shape(storage, layout, run)
+        assert torch.equal(lat.f.view(torch.uint8), ref.f.view(torch.uint8)), run
+
+
+def test_sparse_autotune_keeps_parity_and_picks_a_run(cuda):
+    print("[sparse autotune fp16]", flush=True)
+    layout = _layouts()["list"]
+    a, b = _run_shape("fp16", layout, 0, steps=0), _run_shape("fp16", layout, 0, steps=0)
+    tile, run, ms = b.autotune(even=True, steps=2)
+    print(f"  tile {tile} run {run} {ms:.4f} ms", flush=True)
+    assert tile == 256 and run in (1, 4, 8, 16, 32, 64, 256) and ms > 0
+    assert (b.tile_nodes, b.xcd_run) == (tile, run)
+    # 7 trials of 2 warm-up + 2 timed steps: a continues from the same parity
+    for i in range(28):
+        a.collide_stream("k17", i % 2 == 0)
+    for i in range(4):
+        a.collide_stream("k17", i % 2 == 0)
+        b.collide_stream("k17", i % 2 == 0)
+    torch.cuda.synchronize()
+    assert torch.equal(a.f.view(torch.uint8), b.f.view(torch.uint8))

@@ -165,8 +165,8 @@ def test_host_layer_steps_like_c_abi(cuda):
 
 
 def test_launch_shape_arguments_fail_without_device():
-    """tile_nodes and xcd_run of the structured step are checked before any
-    launch; the autotuner takes structured lattices only."""
+    """tile_nodes and xcd_run of the structured and the sparse step are
+    checked before any launch, by the step and by the autotuner."""
     L = vfx.lib()
     s = vfx._lib.VfxLattice()
     s.f = 1
@@ -183,6 +183,20 @@ def test_launch_shape_arguments_fail_without_device():
         assert L.vfx_k17_step(C.byref(s), 1, 0, None) == vfx._lib.VFX_EINVAL
         assert b"xcd_run" in L.vfx_last_error()
     t, r, ms = C.c_uint32(), C.c_uint32(), C.c_float()
+    assert L.vfx_k17_autotune(C.byref(s), 1, 0, 4, None, C.byref(t), C.byref(r), C.byref(ms)) == vfx._lib.VFX_EINVAL
+    assert b"xcd_run" in L.vfx_last_error()
+    # the sparse step: 256-item workgroups, any power-of-two XCD run
+    nb = (C.c_uint32 * 8)()
     s.direct = 0
-    assert L.vfx_k17_autotune(C.byref(s), 1, 0, 4, None, C.byref(t), C.byref(r), C.byref(ms)) == vfx._lib.VFX_EUNSUPPORTED
+    s.neighbor_x = s.neighbor_y = s.neighbor_z = C.cast(nb, C.c_void_p).value
+    s.xcd_run = 16
+    for tile in (64, 128):
+        s.tile_nodes = tile
+        assert L.vfx_k17_step(C.byref(s), 1, 0, None) == vfx._lib.VFX_EINVAL
+        assert b"tile_nodes of the sparse step" in L.vfx_last_error()
+        assert L.vfx_k17_autotune(C.byref(s), 1, 0, 4, None, C.byref(t), C.byref(r), C.byref(ms)) == vfx._lib.VFX_EINVAL
+    s.tile_nodes = 256
+    s.xcd_run = 12
+    assert L.vfx_k17_step(C.byref(s), 1, 0, None) == vfx._lib.VFX_EINVAL
+    assert b"xcd_run" in L.vfx_last_error()
     assert L.vfx_k17_autotune(None, 1, 0, 4, None, C.byref(t), C.byref(r), C.byref(ms)) == vfx._lib.VFX_EINVAL

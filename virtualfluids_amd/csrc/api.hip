@@ -64,6 +64,9 @@ vfx_status make_lat(const vfx_lattice* in, vfx::Lat& L)
     std::memset(&L, 0, sizeof(L));
     L.f = in->f;
     L.N = in->n_nodes;
+    if (in->xcd_run > 4096 || (in->xcd_run & (in->xcd_run - 1)))
+        return fail(VFX_EINVAL, "xcd_run must be 0 or a power of two up to 4096");
+    L.xrun_lg = in->xcd_run ? 1u + (uint32_t)__builtin_ctz(in->xcd_run) : 0u;
     if (in->direct) {
         const uint64_t nx = in->dims[0], ny = in->dims[1], nz = in->dims[2];
         if (nx == 0 || ny == 0 || nz == 0) return fail(VFX_EINVAL, "direct addressing needs dims > 0");
@@ -72,19 +75,18 @@ vfx_status make_lat(const vfx_lattice* in, vfx::Lat& L)
                         (unsigned long long)ny, (unsigned long long)nz, (unsigned long long)in->n_nodes);
         const uint64_t tile = in->tile_nodes ? in->tile_nodes : 256;
         if (tile != 64 && tile != 128 && tile != 256) return fail(VFX_EINVAL, "tile_nodes must be 0, 64, 128 or 256");
-        if (in->xcd_run > 4096 || (in->xcd_run & (in->xcd_run - 1)))
-            return fail(VFX_EINVAL, "xcd_run must be 0 or a power of two up to 4096");
         if (ny * nz >= (1ull << 31) / ((nx + tile - 1) / tile)) return fail(VFX_EUNSUPPORTED, "grid too large");
         L.nx = (uint32_t)nx;
         L.ny = (uint32_t)ny;
         L.nz = (uint32_t)nz;
         L.chunks = (uint32_t)((nx + tile - 1) / tile);
         L.tile = (uint32_t)tile;
-        L.xrun_lg = in->xcd_run ? 1u + (uint32_t)__builtin_ctz(in->xcd_run) : 0u;
         L.solid = in->solid_mask;
     } else {
         if (!in->neighbor_x || !in->neighbor_y || !in->neighbor_z)
             return fail(VFX_EINVAL, "sparse addressing needs neighbor_x/y/z");
+        if (in->tile_nodes != 0 && in->tile_nodes != 256)
+            return fail(VFX_EINVAL, "tile_nodes of the sparse step must be 0 or 256");
         if (in->n_fluid && !in->fluid_nodes && !in->type_of_grid_node) return fail(VFX_EINVAL, "fluid_nodes is NULL");
         L.nbx = in->neighbor_x;
         L.nby = in->neighbor_y;
@@ -280,7 +282,10 @@ vfx_status vfx_k17_autotune(const vfx_lattice* lat, int32_t is_even, int32_t tmp
                             uint32_t* tile_out, uint32_t* run_out, float* ms_out)
 {
     if (!lat || !tile_out || !run_out) return fail(VFX_EINVAL, "NULL argument");
-    if (!lat->direct) return fail(VFX_EUNSUPPORTED, "the autotuner tunes the structured (direct) step");
+    {
+        vfx::Lat L;
+        if (vfx_status s = make_lat(lat, L)) return s;
+    }
     if (steps < 2) steps = 2;
     steps += steps & 1;
     const bool even0 = is_even != 0;
@@ -294,9 +299,12 @@ vfx_status vfx_k17_autotune(const vfx_lattice* lat, int32_t is_even, int32_t tmp
     float best = 1e30f;
     uint32_t bt = 256, br = 0;
     vfx_status r = VFX_OK;
-    const uint32_t tiles[3] = {64, 128, 256};
+    // the sparse step's workgroups are 256 work items; only its XCD run varies
+    const uint32_t tiles[3] = {256, 128, 64};
+    const int ntiles = lat->direct ? 3 : 1;
     const uint32_t runs[7] = {1, 4, 8, 16, 32, 64, 256};
-    for (uint32_t tile : tiles) {
+    for (int ti = 0; ti < ntiles; ++ti) {
+        const uint32_t tile = tiles[ti];
         for (uint32_t run : runs) {
             trial.tile_nodes = tile;
             trial.xcd_run = run;

@@ -480,7 +480,9 @@ template <typename S, int COLL, bool EVEN, bool FAST>
 __global__ void __launch_bounds__(256) k_step_sparse(Lat L, Rates<typename Compute<S>::T> R)
 {
     using T = typename Compute<S>::T;
-    const uint32_t i = xcd_block(blockIdx.x, gridDim.x, xcd_run<S>()) * blockDim.x + threadIdx.x;
+    const uint32_t blk = L.xrun_lg ? xcd_block_pow2(blockIdx.x, gridDim.x, L.xrun_lg - 1u)
+                                   : xcd_block(blockIdx.x, gridDim.x, xcd_run<S>());
+    const uint32_t i = blk * blockDim.x + threadIdx.x;
     uint32_t kk[8], k;
     bool active;
     uint4 g = make_uint4(0u, 0u, 0u, 0u);

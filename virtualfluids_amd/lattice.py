@@ -222,10 +222,10 @@ class Lattice:
         return s
 
     def autotune(self, even=True, steps=4, collision_template=0, stream=None):
-        """Times the structured K17 step over every tile width and XCD run
-        (vfx_k17_autotune) and keeps the fastest in this lattice; the
-        populations advance, the parity is unchanged.  Returns (tile_nodes,
-        xcd_run, ms per step)."""
+        """Times the K17 step over every tile width (the sparse step: 256
+        work items per workgroup) and XCD run (vfx_k17_autotune) and keeps the
+        fastest in this lattice; the populations advance, the parity is
+        unchanged.  Returns (tile_nodes, xcd_run, ms per step)."""
         s = self.struct()
         t, r, ms = C.c_uint32(), C.c_uint32(), C.c_float()
         V.check(V.lib().vfx_k17_autotune(C.byref(s), int(bool(even)), int(collision_template), int(steps),
