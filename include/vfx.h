@@ -152,8 +152,11 @@ typedef struct vfx_lattice {
     uint32_t xcd_run;
 
     /* Sparse addressing: the table vfx_sparse_regular_groups fills for these
-     * index arrays (NULL: every wavefront reads its indices).  Recompute it
-     * when the index arrays change. */
+     * index arrays and this fluid-node list (NULL: every wavefront reads its
+     * indices).  It is read unchecked for every wavefront of the step, so it
+     * must be the one made for these arrays and list: recompute it when they
+     * change, and pass NULL with any other list (the host layer's
+     * runOnIndices does). */
     const void* regular_groups;
 } vfx_lattice;
 

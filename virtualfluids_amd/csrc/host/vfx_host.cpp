@@ -98,6 +98,8 @@ void KernelImp::runOnIndices(const unsigned int* indices, unsigned int size_indi
     LBMSimulationParameter* p = para->getParD(level);
     vfx_lattice lat = p->lattice;
     if (lat.direct) throw std::runtime_error("runOnIndices needs the sparse (neighbour array) layout");
+    // a regular-wavefront table describes the level's own fluid-node list only
+    if (indices != lat.fluid_nodes || size_indices != lat.n_fluid) lat.regular_groups = nullptr;
     lat.fluid_nodes = indices;
     lat.n_fluid = size_indices;
     set_collision_parameters(lat, *para);
