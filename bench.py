@@ -59,6 +59,8 @@ def parse():
                    help="FP16 channel steps timed per boundary-kernel setting beside the leg's own")
     p.add_argument("--cube-steps", type=int, default=10000,
                    help="coarse steps of the refined wall-mounted-cube leg (0 skips it)")
+    p.add_argument("--cube-schedules", default="separate,merged,fused_outlet",
+                   help="refined-cube schedules timed before the long window (the fastest runs it)")
     p.add_argument("--copy-probe", action="store_true",
                    help="only run a plain device copy of known size (reference for HBM counters and bandwidth)")
     return p.parse_args()
@@ -492,7 +494,8 @@ def channel_leg(torch, vfx, steps, sample_every=10, forms=("fp64", "fp32", "fp16
     return out
 
 
-def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "fp16")):
+def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "fp16"),
+             schedules=("separate", "merged", "fused_outlet")):
     """Wall-mounted cube in a boundary layer on a 256 x 128 x 128 coarse grid
     with a 2:1 nest of 192 x 128 x 96 fine nodes around the cube (power-law
     inflow with 2% noise, pressure outlet, no-slip floor, slip top, periodic
@@ -503,8 +506,8 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
     the first fine step in one launch, each phase's boundary groups of both
     levels in one launch, fine-to-coarse and coarse-to-fine in one launch;
     launch per kernel with the coarse outlet inside the coarse collide-stream
-    kernel) are timed over 500 coarse steps with the levels' launch shapes
-    autotuned; the fastest runs the long window.  Time-averaged
+    kernel; `schedules` picks which) are timed over 500 coarse steps with
+    the levels' launch shapes autotuned; the fastest runs the long window.  Time-averaged
     velocity over the second half on both levels; the
     FP32 and FP16 forms (FP16: deviations from the u_ref equilibrium) against
     FP64.  At Re = 10^4 the wake is chaotic: two forms that differ by rounding
@@ -556,7 +559,7 @@ def cube_leg(torch, vfx, coarse_steps, sample_every=50, forms=("fp64", "fp32", "
             return ms, launches, tuned
 
         n_sep = 10
-        sched = {name: timed(name) for name in ("separate", "merged", "fused_outlet")}
+        sched = {name: timed(name) for name in schedules}
         best = min(sched, key=lambda k: sched[k][0])
         tuned = sched[best][2]
         st = stepper(best)
@@ -716,7 +719,10 @@ def main():
                 c["bulk_over_channel_step_time_without_statistics"] = round(
                     res["fp16"]["ms_per_step"] / c["pressure_fused_without_statistics"]["ms_per_step"], 4)
     if args.cube_steps > 0:
-        out["refined_cube"] = cube_leg(torch, vfx, args.cube_steps)
+        sch = tuple(x for x in args.cube_schedules.split(",") if x)
+        if not sch or set(sch) - {"separate", "merged", "fused_outlet"}:
+            sys.exit(f"--cube-schedules: pick from separate, merged, fused_outlet (got {args.cube_schedules!r})")
+        out["refined_cube"] = cube_leg(torch, vfx, args.cube_steps, schedules=sch)
     out["cpu_baseline"] = None if args.no_cpu else cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
 
