@@ -2336,6 +2336,23 @@ constexpr int kBlock = 256;
 
 inline unsigned grid_for(uint64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Occupancy cap of the structured step: LDS requested at launch and not
+// used, so that at most 160 KiB / cap workgroups share a CU.  The FP32
+// reduced-operation step (58 VGPRs, 8 waves per SIMD by registers) runs
+// fastest with at most 6 workgroups per CU (MI355X, 512x256x256, tuned launch
+// shapes: 1.33 ms per step against 1.40 uncapped; the tuner then picks
+// 128-node tiles, ~3 waves per SIMD).  The FP16 step is fastest at the 7
+// waves its registers allow (6 or 5 waves per SIMD measured slower), FP64 and
+// the reference-order kernels are limited by their registers; other
+// collision templates keep the uncapped launch.
+template <typename S, int COLL, bool FAST> constexpr unsigned step_lds_cap()
+{
+    return sizeof(S) == 4 && FAST && coll_op(COLL) == VFX_K17 && coll_turb(COLL) == kTurbNone &&
+                   !coll_body_force(COLL)
+               ? 24576u
+               : 0u;
+}
+
 template <typename S, int COLL, bool EVEN, bool FAST>
 hipError_t launch_step_t(const Lat& L, int direct, const Rates<typename Compute<S>::T>& R, hipStream_t st)
 {
@@ -2343,7 +2360,8 @@ hipError_t launch_step_t(const Lat& L, int direct, const Rates<typename Compute<
         const unsigned grid = L.chunks * L.ny * L.nz;
         const unsigned block = (unsigned)((L.nx + L.chunks - 1) / L.chunks);
         const unsigned b = block <= 64 ? 64 : block <= 128 ? 128 : 256;
-        hipLaunchKernelGGL((k_step_direct<S, COLL, EVEN, FAST>), dim3(grid), dim3(b), 0, st, L, R);
+        hipLaunchKernelGGL((k_step_direct<S, COLL, EVEN, FAST>), dim3(grid), dim3(b), (step_lds_cap<S, COLL, FAST>()), st, L,
+                           R);
     } else {
         const uint64_t n = node_items(L, 0);
         if (n == 0) return hipSuccess;
