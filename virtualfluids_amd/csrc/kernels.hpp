@@ -2341,7 +2341,8 @@ inline unsigned grid_for(uint64_t n, int block) { return (unsigned)((n + block -
 // reduced-operation step (58 VGPRs, 8 waves per SIMD by registers) runs
 // fastest with at most 6 workgroups per CU (MI355X, 512x256x256, tuned launch
 // shapes: 1.33 ms per step against 1.40 uncapped; the tuner then picks
-// 128-node tiles, ~3 waves per SIMD).  The FP16 step is fastest at the 7
+// 128-node tiles, ~3 waves per SIMD); the sparse FP32 step with the same
+// cap 1.497 ms against 1.530.  The FP16 step is fastest at the 7
 // waves its registers allow (6 or 5 waves per SIMD measured slower), FP64 and
 // the reference-order kernels are limited by their registers; other
 // collision templates keep the uncapped launch.
@@ -2365,7 +2366,8 @@ hipError_t launch_step_t(const Lat& L, int direct, const Rates<typename Compute<
     } else {
         const uint64_t n = node_items(L, 0);
         if (n == 0) return hipSuccess;
-        hipLaunchKernelGGL((k_step_sparse<S, COLL, EVEN, FAST>), dim3(grid_for(n, kBlock)), dim3(kBlock), 0, st, L, R);
+        hipLaunchKernelGGL((k_step_sparse<S, COLL, EVEN, FAST>), dim3(grid_for(n, kBlock)), dim3(kBlock),
+                           (step_lds_cap<S, COLL, FAST>()), st, L, R);
     }
     return hipGetLastError();
 }
